@@ -1,0 +1,308 @@
+"""cvmatch_amd -- TM_CCOEFF_NORMED template matching on MI355X (gfx950).
+
+A Python mirror of the reference Go API (cvmatch.go, matcher.go, fleet.go)
+over the C ABI of ``libcvmatch_hip.so`` (declared in include/cvmatch_hip.h):
+
+    match(parent, sub)      -> (minV, minX, minY, maxV, maxX, maxY)   Match cvmatch.go:75
+    match_map(parent, sub)  -> (res[h, w] float32, w, h)              MatchMap cvmatch.go:100
+    match_gray(parent, sub) -> six-tuple                              MatchGray cvmatch.go:128
+    Matcher(sub) / GrayMatcher(sub) .find(parent)                    matcher.go:45-130
+    Fleet(*matchers).find_all(parent) -> [Result]                    fleet.go:85-332
+
+Images are numpy uint8 arrays: (H, W, 4) interleaved RGBA or (H, W) gray.
+Row-strided views (``parent[y0:y1, x0:x1]``) are passed zero-copy with their
+row stride, like the reference's sub-images.  Errors raise ``CvmatchError``
+carrying the reference's panic text.  Every call runs on the GPU; there is
+no CPU path: if the HIP library is not built, the import of any entry point
+fails loudly.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from collections import namedtuple
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libcvmatch_hip.so")
+_lib = None
+
+Result = namedtuple("Result", "MinV MinX MinY MaxV MaxX MaxY")
+
+CVM_OK, CVM_E_BAD_ARGS, CVM_E_STATS_CAP, CVM_E_EMPTY_FLEET, CVM_E_MIXED_FLEET = 0, 1, 2, 3, 4
+CVM_E_HIP, CVM_E_UNSUPPORTED, CVM_E_NIL_IMAGE, CVM_E_EMPTY_IMAGE = 5, 6, 7, 8
+
+# Worker-count knob of the reference (cvmatch.go:42-63).  Kept for API
+# compatibility; it has no effect on the GPU path and never changes bits.
+Threads = 0
+
+
+class CvmatchError(RuntimeError):
+    def __init__(self, code: int, detail: str):
+        super().__init__(detail)
+        self.code = code
+
+
+class cvm_result(C.Structure):
+    _fields_ = [
+        ("min_v", C.c_float),
+        ("min_x", C.c_int32),
+        ("min_y", C.c_int32),
+        ("max_v", C.c_float),
+        ("max_x", C.c_int32),
+        ("max_y", C.c_int32),
+    ]
+
+    def tuple(self):
+        return (self.min_v, self.min_x, self.min_y, self.max_v, self.max_x, self.max_y)
+
+
+# exported symbols and their ctypes signatures
+_P, _I32, _I64, _SZ = C.c_void_p, C.c_int32, C.c_int64, C.c_size_t
+_RES = C.POINTER(cvm_result)
+SIGNATURES = {
+    "cvm_last_error_detail": ([], C.c_char_p),
+    "cvm_version": ([], C.c_char_p),
+    "cvm_match_u8": ([_P, _I64, _I32, _I32, _P, _I64, _I32, _I32, _I32, _I32, _P, _RES], C.c_int),
+    "cvm_match": ([_P, _I64, _I32, _I32, _P, _I64, _I32, _I32, _RES], C.c_int),
+    "cvm_match_map": ([_P, _I64, _I32, _I32, _P, _I64, _I32, _I32, _P, _RES], C.c_int),
+    "cvm_match_gray": ([_P, _I64, _I32, _I32, _I32, _P, _I64, _I32, _I32, _I32, _RES], C.c_int),
+    "cvm_alpha_const": ([_P, _I64, _I32, _I32, C.POINTER(_I32)], C.c_int),
+    "cvm_rgba_to_gray": ([_P, _I64, _I32, _I32, _P], C.c_int),
+    "cvm_stats_cap": ([_I32], _I64),
+    "cvm_plan": ([_I32, _I32, _I32, _I32, _P], _I64),
+    "cvm_matcher_new": ([_P, _I64, _I32, _I32, _I32, C.POINTER(_P)], C.c_int),
+    "cvm_matcher_find": ([_P, _P, _I64, _I32, _I32, _RES], C.c_int),
+    "cvm_matcher_free": ([_P], None),
+    "cvm_fleet_new": ([_P, _I32, C.POINTER(_P)], C.c_int),
+    "cvm_fleet_find_all": ([_P, _P, _I64, _I32, _I32, _P], C.c_int),
+    "cvm_fleet_free": ([_P], None),
+    "cvm_match_u8_device": ([_P, _I64, _I32, _I32, _P, _I64, _I32, _I32, _I32, _I32, _P, _RES, _P], C.c_int),
+    "cvm_device_alloc": ([_SZ, C.POINTER(_P)], C.c_int),
+    "cvm_device_free": ([_P], C.c_int),
+    "cvm_copy_h2d": ([_P, _P, _SZ], C.c_int),
+    "cvm_copy_d2h": ([_P, _P, _SZ], C.c_int),
+    "cvm_synchronize": ([], C.c_int),
+    "cvm_profile_enable": ([_I32], C.c_int),
+    "cvm_profile_reset": ([], C.c_int),
+    "cvm_profile_read": ([_P, _P, _P, _I32], C.c_int),
+    "cvm_selftest_fp": ([_P, _P, _P, _I32, _P, _P, _P], C.c_int),
+    "cvm_twiddles": ([_I32, _P, _P, C.POINTER(_I32)], C.c_int),
+}
+
+
+def lib():
+    """Load libcvmatch_hip.so (built by cvmatch_amd/build.py).  Raises if absent."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError(f"{LIB_PATH} is not built; run `python cvmatch_amd/build.py` (hipcc, gfx950)")
+        L = C.CDLL(LIB_PATH)
+        for name, (args, res) in SIGNATURES.items():
+            fn = getattr(L, name)
+            fn.argtypes = args
+            fn.restype = res
+        _lib = L
+    return _lib
+
+
+def _check(rc: int):
+    if rc != CVM_OK:
+        raise CvmatchError(rc, lib().cvm_last_error_detail().decode())
+
+
+def _ptr(a: np.ndarray):
+    return C.c_void_p(a.ctypes.data)
+
+
+def view(img: np.ndarray):
+    """(pointer-owner array, stride, w, h, step) of an RGBA (H,W,4) or gray (H,W)
+    uint8 image.  Row-strided views are kept zero-copy; anything else is made
+    contiguous."""
+    if img is None:
+        raise CvmatchError(CVM_E_NIL_IMAGE, "cvmatch: nil image")
+    img = np.asarray(img)
+    if img.dtype != np.uint8 or img.ndim not in (2, 3) or (img.ndim == 3 and img.shape[2] != 4):
+        raise CvmatchError(CVM_E_BAD_ARGS, "cvmatch: images are (H, W, 4) RGBA or (H, W) gray uint8")
+    h, w = img.shape[:2]
+    if w <= 0 or h <= 0:
+        raise CvmatchError(CVM_E_EMPTY_IMAGE, "cvmatch: empty image")
+    step = 4 if img.ndim == 3 else 1
+    inner_ok = img.strides[1] == step and (img.ndim == 2 or img.strides[2] == 1)
+    if not inner_ok or img.strides[0] < w * step:
+        img = np.ascontiguousarray(img)
+    return img, int(img.strides[0]), w, h, step
+
+
+# ---------------------------------------------------------------- one-shot
+
+
+def match_u8(img, istride, iw, ih, tpl, tstride, tw, th, cn, step, want_map=False):
+    """matchU8 impl.go:1126 on flat uint8 buffers; returns (map or None, six-tuple)."""
+    img = np.ascontiguousarray(img, dtype=np.uint8)
+    tpl = np.ascontiguousarray(tpl, dtype=np.uint8)
+    rw, rh = iw - tw + 1, ih - th + 1
+    res = np.empty((rh, rw), dtype=np.float32) if want_map and rw > 0 and rh > 0 else None
+    out = cvm_result()
+    _check(lib().cvm_match_u8(_ptr(img), istride, iw, ih, _ptr(tpl), tstride, tw, th, cn, step,
+                              _ptr(res) if res is not None else None, C.byref(out)))
+    return res, out.tuple()
+
+
+def match(parent, sub):
+    """Match cvmatch.go:75: RGBA TM_CCOEFF_NORMED + minMaxLoc six-tuple."""
+    p, ps, pw, ph, pst = view(parent)
+    s, ss, sw, sh, sst = view(sub)
+    if pst != 4 or sst != 4:
+        raise CvmatchError(CVM_E_BAD_ARGS, "cvmatch: Match takes RGBA (H, W, 4) images")
+    out = cvm_result()
+    _check(lib().cvm_match(_ptr(p), ps, pw, ph, _ptr(s), ss, sw, sh, C.byref(out)))
+    return out.tuple()
+
+
+def match_map(parent, sub):
+    """MatchMap cvmatch.go:100: (res[h, w] float32, w, h)."""
+    p, ps, pw, ph, pst = view(parent)
+    s, ss, sw, sh, sst = view(sub)
+    if pst != 4 or sst != 4:
+        raise CvmatchError(CVM_E_BAD_ARGS, "cvmatch: MatchMap takes RGBA (H, W, 4) images")
+    if sw > pw or sh > ph:
+        raise CvmatchError(CVM_E_BAD_ARGS,
+                           f"cvmatch: bad match arguments ({sw}x{sh} in {pw}x{ph}, cn=4 step=4)")
+    w, h = pw - sw + 1, ph - sh + 1
+    res = np.empty((h, w), dtype=np.float32)
+    out = cvm_result()
+    _check(lib().cvm_match_map(_ptr(p), ps, pw, ph, _ptr(s), ss, sw, sh, _ptr(res), C.byref(out)))
+    return res, w, h
+
+
+def match_gray(parent, sub):
+    """MatchGray cvmatch.go:128: gray inputs as is, RGBA inputs via BT.601."""
+    p, ps, pw, ph, pst = view(parent)
+    s, ss, sw, sh, sst = view(sub)
+    out = cvm_result()
+    _check(lib().cvm_match_gray(_ptr(p), ps, pw, ph, pst, _ptr(s), ss, sw, sh, sst, C.byref(out)))
+    return out.tuple()
+
+
+def alpha_const(pix) -> bool:
+    """alphaConst cvmatch.go:146."""
+    p, ps, w, h, st = view(pix)
+    if st != 4:
+        raise CvmatchError(CVM_E_BAD_ARGS, "cvmatch: alpha scan needs RGBA")
+    out = C.c_int32()
+    _check(lib().cvm_alpha_const(_ptr(p), ps, w, h, C.byref(out)))
+    return bool(out.value)
+
+
+def rgba_to_gray(pix) -> np.ndarray:
+    """rgbToGray cvmatch.go:222."""
+    p, ps, w, h, st = view(pix)
+    if st != 4:
+        raise CvmatchError(CVM_E_BAD_ARGS, "cvmatch: gray conversion needs RGBA")
+    out = np.empty((h, w), dtype=np.uint8)
+    _check(lib().cvm_rgba_to_gray(_ptr(p), ps, w, h, _ptr(out)))
+    return out
+
+
+def stats_cap(cn: int) -> int:
+    return int(lib().cvm_stats_cap(cn))
+
+
+def plan(tw, th, rw, rh):
+    out = np.zeros(4, dtype=np.int32)
+    cost = lib().cvm_plan(tw, th, rw, rh, _ptr(out))
+    return tuple(int(v) for v in out), int(cost)
+
+
+# ----------------------------------------------------------------- Matcher
+
+
+class Matcher:
+    """NewMatcher matcher.go:45 (RGBA) / NewGrayMatcher matcher.go:57 (gray=True).
+
+    ``find(parent)`` is bit-identical to ``match`` / ``match_gray``."""
+
+    def __init__(self, sub, gray: bool = False):
+        s, ss, sw, sh, sst = view(sub)
+        if gray and sst == 4:
+            s = rgba_to_gray(s)
+            ss, sst = sw, 1
+        if not gray and sst != 4:
+            raise CvmatchError(CVM_E_BAD_ARGS, "cvmatch: NewMatcher takes an RGBA template")
+        self.w, self.h, self.step = sw, sh, sst
+        h = C.c_void_p()
+        _check(lib().cvm_matcher_new(_ptr(s), ss, sw, sh, sst, C.byref(h)))
+        self._h = h
+
+    def find(self, parent):
+        p, ps, pw, ph, pst = view(parent)
+        if self.step == 1 and pst == 4:
+            p = rgba_to_gray(p)
+            ps, pst = pw, 1
+        if pst != self.step:
+            raise CvmatchError(CVM_E_BAD_ARGS, "cvmatch: parent format does not match the matcher")
+        out = cvm_result()
+        _check(lib().cvm_matcher_find(self._h, _ptr(p), ps, pw, ph, C.byref(out)))
+        return out.tuple()
+
+    Find = find
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h is not None and h.value and _lib is not None:
+            _lib.cvm_matcher_free(h)
+            self._h = None
+
+
+def NewMatcher(sub) -> Matcher:
+    return Matcher(sub, gray=False)
+
+
+def NewGrayMatcher(sub) -> Matcher:
+    return Matcher(sub, gray=True)
+
+
+# ------------------------------------------------------------------- Fleet
+
+
+class Fleet:
+    """NewFleet fleet.go:85: many prepared matchers against one frame."""
+
+    def __init__(self, *ms: Matcher):
+        if len(ms) == 1 and isinstance(ms[0], (list, tuple)):
+            ms = tuple(ms[0])
+        self._ms = list(ms)
+        arr = (C.c_void_p * max(len(ms), 1))(*[m._h.value for m in ms])
+        h = C.c_void_p()
+        _check(lib().cvm_fleet_new(C.cast(arr, C.c_void_p), len(ms), C.byref(h)))
+        self._h = h
+        self.step = ms[0].step
+
+    def find_all(self, parent):
+        p, ps, pw, ph, pst = view(parent)
+        if self.step == 1 and pst == 4:
+            p = rgba_to_gray(p)
+            ps, pst = pw, 1
+        if pst != self.step:
+            raise CvmatchError(CVM_E_BAD_ARGS, "cvmatch: parent format does not match the fleet")
+        n = len(self._ms)
+        out = (cvm_result * n)()
+        _check(lib().cvm_fleet_find_all(self._h, _ptr(p), ps, pw, ph, C.cast(out, C.c_void_p)))
+        return [Result(*out[i].tuple()) for i in range(n)]
+
+    FindAll = find_all
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h is not None and h.value and _lib is not None:
+            _lib.cvm_fleet_free(h)
+            self._h = None
+
+
+def NewFleet(*ms) -> Fleet:
+    return Fleet(*ms)
+
+
+Match, MatchMap, MatchGray = match, match_map, match_gray

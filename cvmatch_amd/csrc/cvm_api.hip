@@ -1,0 +1,1312 @@
+// Host orchestration behind the C ABI of include/cvmatch_hip.h.
+//
+// One engine per process owns the HIP stream, the per-size twiddle tables
+// (generated on the device), grow-only scratch buffers and the profiling
+// events.  Every pipeline runs stream-ordered with no host round trip inside
+// a call: the channel count (Match's alpha rule, cvmatch.go:82-85) and the
+// flat-template short cut (impl.go:1137) are decided on the device and read
+// by the kernels from a CallState.  The host only plans the tile geometry,
+// which is pure integer arithmetic (newGoPlan impl.go:246-295).
+#include <stdarg.h>
+#include <stdio.h>
+#include <string.h>
+
+#include <algorithm>
+#include <atomic>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../include/cvmatch_hip.h"
+#include "cvm_internal.h"
+
+namespace cvm {
+
+// ---------------------------------------------------------------- errors --
+
+static thread_local char g_err[512];
+
+static int fail(int code, const char* fmt, ...) {
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(g_err, sizeof(g_err), fmt, ap);
+  va_end(ap);
+  return code;
+}
+
+#define HIPCHK(x)                                                                                      \
+  do {                                                                                                 \
+    hipError_t e_ = (x);                                                                               \
+    if (e_ != hipSuccess)                                                                              \
+      return fail(CVM_E_HIP, "cvmatch: HIP error '%s' at %s:%d", hipGetErrorString(e_), __FILE__, __LINE__); \
+  } while (0)
+
+#define CHK(x)          \
+  do {                  \
+    int rc_ = (x);      \
+    if (rc_) return rc_; \
+  } while (0)
+
+// ------------------------------------------------------------------ plan --
+
+struct Plan {
+  int dftW, dftH, hw, blockW, blockH;
+  long long cost;
+};
+
+static int next_pow2(int v) {
+  int n = 1;
+  while (n < v) n <<= 1;
+  return n;
+}
+static int bit_len(unsigned long long n) {
+  int l = 0;
+  while (n) {
+    l++;
+    n >>= 1;
+  }
+  return l;
+}
+static int ilog2(int n) { return bit_len((unsigned)n) - 1; }
+
+// newGoPlan impl.go:246-295: integer cost-model argmin over power-of-two
+// transform sizes (areaCap 2^22), ties to the smaller area.
+static Plan make_plan(int tw, int th, int rw, int rh) {
+  const long long areaCap = 1LL << 22;
+  int minW = std::max(2, next_pow2(tw)), minH = std::max(2, next_pow2(th));
+  int maxW = std::max(minW, next_pow2(rw + tw - 1)), maxH = std::max(minH, next_pow2(rh + th - 1));
+  int bestW = minW, bestH = minH;
+  long long bestCost = -1, bestArea = 0;
+  for (int dftW = minW; dftW <= maxW; dftW <<= 1) {
+    for (int dftH = minH; dftH <= maxH; dftH <<= 1) {
+      long long area = (long long)dftW * dftH;
+      if (area > areaCap) continue;
+      int blockW = std::min(dftW - tw + 1, rw), blockH = std::min(dftH - th + 1, rh);
+      long long ntx = (rw + blockW - 1) / blockW;
+      long long hw = dftW / 2 + 1;
+      long long rowFFT = 5LL * dftW * ilog2(dftW);
+      long long colPass = 5LL * dftH * ilog2(dftH) * hw;
+      long long specN = (long long)dftH * hw;
+      long long cost = 0;
+      for (int y0 = 0; y0 < rh; y0 += blockH) {
+        int bh = std::min(blockH, rh - y0);
+        int loadH = std::min(bh + th - 1, dftH);
+        cost += ntx * ((long long)((loadH + 1) / 2 + (bh + 1) / 2) * rowFFT + 2 * colPass + 16 * specN);
+      }
+      if (bestCost < 0 || cost < bestCost || (cost == bestCost && area < bestArea)) {
+        bestCost = cost;
+        bestArea = area;
+        bestW = dftW;
+        bestH = dftH;
+      }
+    }
+  }
+  Plan p;
+  p.dftW = bestW;
+  p.dftH = bestH;
+  p.hw = bestW / 2 + 1;
+  p.blockW = std::min(bestW - tw + 1, rw);
+  p.blockH = std::min(bestH - th + 1, rh);
+  p.cost = bestCost;
+  return p;
+}
+
+// shrinkDH impl.go:573-581
+static int shrink_dh(const Plan& p, int rh, int th) {
+  int lastY0 = ((rh - 1) / p.blockH) * p.blockH;
+  if (lastY0 > 0) {
+    int dh2 = std::max(2, next_pow2(rh - lastY0 + th - 1));
+    if (dh2 < p.dftH) return dh2;
+  }
+  return 0;
+}
+
+static int64_t stats_cap(int cn) {
+  static const int64_t caps[5] = {0, 11909805, 8421504, 6876129, 5954902};
+  return (cn >= 1 && cn <= 4) ? caps[cn] : 0;
+}
+
+// Kernel limits of this build: row transforms keep two dftW rows in LDS and
+// column slabs dftH x CG complex, both <= 64 KiB; the normalisation strip
+// holds 256 + tw - 1 <= 2048 columns.
+constexpr int kMaxFftW = 4096;
+constexpr int kMaxFftH = 8192;
+constexpr int kMaxNormTw = 2048 - 255;
+
+static int round_up(int v, int m) { return (v + m - 1) / m * m; }
+
+// --------------------------------------------------------------- engine --
+
+struct DevBuf {
+  void* p = nullptr;
+  size_t cap = 0;
+  int ensure(size_t n) {
+    if (n <= cap) return 0;
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    cap = 0;
+    size_t want = std::max(n, (size_t)256);
+    hipError_t e = hipMalloc(&p, want);
+    if (e != hipSuccess) {
+      p = nullptr;
+      return fail(CVM_E_HIP, "cvmatch: device allocation of %zu bytes failed: %s", want, hipGetErrorString(e));
+    }
+    cap = want;
+    return 0;
+  }
+  template <class T>
+  T* as() const {
+    return (T*)p;
+  }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    cap = 0;
+  }
+};
+
+struct Res6 {
+  float min_v;
+  int32_t min_x, min_y;
+  float max_v;
+  int32_t max_x, max_y;
+};
+
+struct ProfStage {
+  std::string name;
+  hipEvent_t a, b;
+};
+
+struct Engine {
+  std::recursive_mutex mu;
+  bool ready = false;
+  hipStream_t stream = nullptr;
+  std::map<int, FftTables> tables;
+  DevBuf img, tpl, spec, tspec, tspec2, corr, parts, gray0, gray1, mapbuf;
+  CallState* st = nullptr;
+  Res6* res_d = nullptr;
+  Res6* res_h = nullptr;
+  bool prof = false;
+  std::vector<hipEvent_t> evpool;
+  std::vector<ProfStage> pending;
+  std::map<std::string, std::pair<double, long long>> prof_acc;
+
+  int init() {
+    if (ready) return 0;
+    int ndev = 0;
+    HIPCHK(hipGetDeviceCount(&ndev));
+    if (ndev < 1) return fail(CVM_E_HIP, "cvmatch: no HIP device available");
+    HIPCHK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+    HIPCHK(hipMalloc(&st, sizeof(CallState)));
+    HIPCHK(hipMalloc(&res_d, sizeof(Res6) * 2));
+    HIPCHK(hipHostMalloc(&res_h, sizeof(Res6) * 2, hipHostMallocDefault));
+    ready = true;
+    return 0;
+  }
+
+  int tab(int n, const FftTables** out) {
+    auto it = tables.find(n);
+    if (it == tables.end()) {
+      FftTables t;
+      t.n = n;
+      int len = tri_len(n);
+      HIPCHK(hipMalloc(&t.tri, sizeof(float2) * (size_t)std::max(len, 1)));
+      HIPCHK(hipMalloc(&t.brev, sizeof(int32_t) * (size_t)n));
+      launch_twiddles(t.tri, t.brev, n, stream);
+      HIPCHK(hipGetLastError());
+      // tables are shared by every stream a caller may pass in
+      HIPCHK(hipStreamSynchronize(stream));
+      it = tables.emplace(n, t).first;
+    }
+    *out = &it->second;
+    return 0;
+  }
+
+  hipEvent_t ev() {
+    if (!evpool.empty()) {
+      hipEvent_t e = evpool.back();
+      evpool.pop_back();
+      return e;
+    }
+    hipEvent_t e;
+    if (hipEventCreate(&e) != hipSuccess) return nullptr;
+    return e;
+  }
+
+  int stage_begin(const char* name, hipStream_t s) {
+    if (!prof) return -1;
+    ProfStage p;
+    p.name = name;
+    p.a = ev();
+    p.b = ev();
+    if (!p.a || !p.b) return -1;
+    (void)hipEventRecord(p.a, s);
+    pending.push_back(p);
+    return (int)pending.size() - 1;
+  }
+  void stage_end(int i, hipStream_t s) {
+    if (i < 0) return;
+    (void)hipEventRecord(pending[i].b, s);
+  }
+  // after the stream has been synchronised
+  void stage_collect() {
+    for (auto& p : pending) {
+      float ms = 0.f;
+      if (hipEventElapsedTime(&ms, p.a, p.b) == hipSuccess) {
+        auto& acc = prof_acc[p.name];
+        acc.first += ms;
+        acc.second += 1;
+      }
+      evpool.push_back(p.a);
+      evpool.push_back(p.b);
+    }
+    pending.clear();
+  }
+};
+
+static Engine& engine() {
+  static Engine* e = new Engine();
+  return *e;
+}
+
+struct Stage {
+  Engine& E;
+  int i;
+  hipStream_t s;
+  Stage(Engine& e, const char* name, hipStream_t st) : E(e), s(st) { i = E.stage_begin(name, st); }
+  ~Stage() { E.stage_end(i, s); }
+};
+
+// ------------------------------------------------------------ pipeline --
+
+// Template side of one correlation on a plan: its spectrum (all cnMax
+// channels) and the shrunk-band spectrum, device resident.
+struct TplSpec {
+  Plan p;
+  int dh2 = 0;
+  int hwPad = 0;
+  int cnMax = 0;
+  float2* spec = nullptr;
+  float2* spec2 = nullptr;
+};
+
+static int plan_tables(Engine& E, const Plan& p, int dh2, PlanTables& pt) {
+  const FftTables *tw, *th, *th2 = nullptr;
+  CHK(E.tab(p.dftW, &tw));
+  CHK(E.tab(p.dftH, &th));
+  if (dh2) CHK(E.tab(dh2, &th2));
+  pt.triW = tw->tri;
+  pt.brevW = tw->brev;
+  pt.triH = th->tri;
+  pt.brevH = th->brev;
+  pt.triH2 = th2 ? th2->tri : th->tri;
+  pt.brevH2 = th2 ? th2->brev : th->brev;
+  return 0;
+}
+
+static int check_limits(const Plan& p, int tw, int rw, int rh) {
+  if (p.dftW > kMaxFftW || p.dftH > kMaxFftH)
+    return fail(CVM_E_UNSUPPORTED, "cvmatch: transform %dx%d exceeds this build's LDS limits (%d x %d)", p.dftW,
+                p.dftH, kMaxFftW, kMaxFftH);
+  if (tw > kMaxNormTw)
+    return fail(CVM_E_UNSUPPORTED, "cvmatch: template width %d exceeds this build's normalisation strip (%d)", tw,
+                kMaxNormTw);
+  if ((long long)rw * rh >= (1LL << 31))
+    return fail(CVM_E_UNSUPPORTED, "cvmatch: result map of %dx%d exceeds 2^31 elements", rw, rh);
+  return 0;
+}
+
+// Template spectrum on plan p (buildTSpecSet impl.go:520-561): forward
+// transform of the template at (0,0), scaled by 1/(dftW*dftH), plus the
+// shrunk band (shrinkInto impl.go:591-610).  dst buffers sized by caller.
+static int build_tspec(Engine& E, hipStream_t s, const CallState* st, const uint8_t* d_tpl, int64_t tstride, int tw,
+                       int th, int step, int cnMax, const Plan& p, int dh2, const PlanTables& pt, float2* spec,
+                       float2* spec2) {
+  Stage S(E, "tspec", s);
+  int hwPad = round_up(p.hw, 16);
+  int64_t tchan = (int64_t)p.dftH * hwPad, tchan2 = (int64_t)dh2 * hwPad;
+  TileGrid gt{};
+  gt.ntx = gt.nty = 1;
+  gt.blockW = tw;
+  gt.blockH = th;
+  gt.extW = tw;
+  gt.extH = th;
+  gt.padW = gt.padH = 0;
+  gt.lastY0 = 0;
+  gt.dftW = p.dftW;
+  gt.log2W = ilog2(p.dftW);
+  gt.hw = p.hw;
+  gt.hwPad = hwPad;
+  gt.dftH = p.dftH;
+  gt.log2H = ilog2(p.dftH);
+  gt.dftH2 = 0;
+  gt.log2H2 = 0;
+  launch_row_fwd(d_tpl, tstride, step, cnMax, st, gt, pt, spec, 0, tchan, 1, s);
+  float scale = 1.0f / ((float)p.dftW * (float)p.dftH);
+  launch_cols(COL_TSPEC, spec, spec, 0, 0, tchan, cnMax, st, gt, pt, nullptr, nullptr, 0, 0, scale, 1, s);
+  if (dh2) {
+    int shift = ilog2(p.dftH) - ilog2(dh2);
+    launch_shrink(spec, spec2, cnMax, dh2, shift, hwPad, tchan, tchan2, s);
+  }
+  HIPCHK(hipGetLastError());
+  return 0;
+}
+
+// Correlation + normalisation + extrema for one parent against a prepared
+// template spectrum (crossCorrGo + normalizeParallelGo, impl.go:1150-1154).
+// CallState must be finalised on the stream before this runs.
+static int corr_and_normalize(Engine& E, hipStream_t s, const CallState* st, const uint8_t* d_img, int64_t istride,
+                              int iw, int ih, int tw, int th, int step, int cnMax, const TplSpec& ts,
+                              const PlanTables& pt, float* d_map, Res6* d_res) {
+  const Plan& p = ts.p;
+  int rw = iw - tw + 1, rh = ih - th + 1;
+  int hwPad = ts.hwPad;
+  int ntx = (rw + p.blockW - 1) / p.blockW, nty = (rh + p.blockH - 1) / p.blockH, ntiles = ntx * nty;
+  TileGrid g{};
+  g.ntx = ntx;
+  g.nty = nty;
+  g.blockW = p.blockW;
+  g.blockH = p.blockH;
+  g.extW = rw;
+  g.extH = rh;
+  g.padW = tw - 1;
+  g.padH = th - 1;
+  g.lastY0 = ((rh - 1) / p.blockH) * p.blockH;
+  g.dftW = p.dftW;
+  g.log2W = ilog2(p.dftW);
+  g.hw = p.hw;
+  g.hwPad = hwPad;
+  g.dftH = p.dftH;
+  g.log2H = ilog2(p.dftH);
+  g.dftH2 = ts.dh2;
+  g.log2H2 = ts.dh2 ? ilog2(ts.dh2) : 0;
+  int64_t chan = (int64_t)p.dftH * hwPad, tileStride = chan * cnMax;
+  int64_t tchan = chan, tchan2 = (int64_t)ts.dh2 * hwPad;
+  CHK(E.spec.ensure(sizeof(float2) * (size_t)tileStride * ntiles));
+  float2* spec = E.spec.as<float2>();
+  {
+    Stage S(E, "row_fwd", s);
+    launch_row_fwd(d_img, istride, step, cnMax, st, g, pt, spec, tileStride, chan, ntiles, s);
+  }
+  {
+    Stage S(E, "cols", s);
+    launch_cols(COL_CORR, spec, spec, tileStride, tileStride, chan, cnMax, st, g, pt, ts.spec, ts.spec2, tchan,
+                tchan2, 1.0f, ntiles, s);
+  }
+  float* corr = d_map;
+  if (!corr) {
+    CHK(E.corr.ensure(sizeof(float) * (size_t)rw * rh));
+    corr = E.corr.as<float>();
+  }
+  {
+    Stage S(E, "row_inv", s);
+    launch_row_inv(spec, tileStride, chan, st, cnMax, g, pt, corr, rw, ntiles, s);
+  }
+  int nparts = norm_partials_count(rw, rh);
+  CHK(E.parts.ensure(sizeof(Extrema) * (size_t)nparts));
+  {
+    Stage S(E, "normalize", s);
+    launch_normalize(d_img, istride, iw, step, tw, th, rw, rh, st, corr, d_map, E.parts.as<Extrema>(), s);
+  }
+  {
+    Stage S(E, "reduce", s);
+    launch_reduce(E.parts.as<Extrema>(), nparts, st, rw, d_res, nullptr, rh, s);
+  }
+  HIPCHK(hipGetLastError());
+  return 0;
+}
+
+static int check_args(int64_t istride, int iw, int ih, int64_t tstride, int tw, int th, int cn, int step) {
+  if (cn < 1 || cn > 4 || step < cn || tw < 1 || th < 1 || tw > iw || th > ih || istride < (int64_t)iw * step ||
+      tstride < (int64_t)tw * step)
+    return fail(CVM_E_BAD_ARGS, "cvmatch: bad match arguments (%dx%d in %dx%d, cn=%d step=%d)", tw, th, iw, ih, cn,
+                step);
+  if ((int64_t)tw * th > stats_cap(cn))
+    return fail(CVM_E_STATS_CAP, "cvmatch: template area %dx%d exceeds the exact-statistics bound for cn=%d", tw, th,
+                cn);
+  return 0;
+}
+
+// matchU8 on device buffers.  cn_mode 0 = Match's RGBA alpha rule.
+static int run_match(Engine& E, hipStream_t s, const uint8_t* d_img, int64_t istride, int iw, int ih,
+                     const uint8_t* d_tpl, int64_t tstride, int tw, int th, int cn_mode, int step, float* d_map) {
+  int rw = iw - tw + 1, rh = ih - th + 1;
+  Plan p = make_plan(tw, th, rw, rh);
+  CHK(check_limits(p, tw, rw, rh));
+  int dh2 = shrink_dh(p, rh, th);
+  PlanTables pt;
+  CHK(plan_tables(E, p, dh2, pt));
+  int cnMax = cn_mode ? cn_mode : 4;
+  {
+    Stage S(E, "stats", s);
+    launch_state_init(E.st, s);
+    launch_tpl_stats(d_tpl, tstride, tw, th, cnMax, step, E.st, s);
+    if (cn_mode == 0) {
+      launch_alpha_scan(d_img, istride, iw, ih, E.st, 0, s);
+      launch_alpha_scan(d_tpl, tstride, tw, th, E.st, 1, s);
+    }
+    launch_state_finalize(E.st, tw, th, cn_mode, s);
+  }
+  TplSpec ts;
+  ts.p = p;
+  ts.dh2 = dh2;
+  ts.hwPad = round_up(p.hw, 16);
+  ts.cnMax = cnMax;
+  size_t tchan = (size_t)p.dftH * ts.hwPad, tchan2 = (size_t)dh2 * ts.hwPad;
+  CHK(E.tspec.ensure(sizeof(float2) * tchan * cnMax));
+  ts.spec = E.tspec.as<float2>();
+  if (dh2) {
+    CHK(E.tspec2.ensure(sizeof(float2) * tchan2 * cnMax));
+    ts.spec2 = E.tspec2.as<float2>();
+  }
+  CHK(build_tspec(E, s, E.st, d_tpl, tstride, tw, th, step, cnMax, p, dh2, pt, ts.spec, ts.spec2));
+  return corr_and_normalize(E, s, E.st, d_img, istride, iw, ih, tw, th, step, cnMax, ts, pt, d_map, E.res_d);
+}
+
+static int finish_result(Engine& E, hipStream_t s, cvm_result* out) {
+  HIPCHK(hipMemcpyAsync(E.res_h, E.res_d, sizeof(Res6), hipMemcpyDeviceToHost, s));
+  HIPCHK(hipStreamSynchronize(s));
+  E.stage_collect();
+  memcpy(out, E.res_h, sizeof(cvm_result));
+  return 0;
+}
+
+static int64_t img_bytes(int64_t stride, int w, int h, int step) { return (int64_t)(h - 1) * stride + (int64_t)w * step; }
+
+// Host-buffer front end shared by the one-shot entry points.
+static int match_host(const uint8_t* img, int64_t istride, int iw, int ih, const uint8_t* tpl, int64_t tstride,
+                      int tw, int th, int cn_mode, int step, float* map, cvm_result* out) {
+  Engine& E = engine();
+  std::lock_guard<std::recursive_mutex> lk(E.mu);
+  CHK(E.init());
+  hipStream_t s = E.stream;
+  int rw = iw - tw + 1, rh = ih - th + 1;
+  size_t ib = (size_t)img_bytes(istride, iw, ih, step), tb = (size_t)img_bytes(tstride, tw, th, step);
+  CHK(E.img.ensure(ib));
+  CHK(E.tpl.ensure(tb));
+  {
+    Stage S(E, "h2d", s);
+    HIPCHK(hipMemcpyAsync(E.img.p, img, ib, hipMemcpyHostToDevice, s));
+    HIPCHK(hipMemcpyAsync(E.tpl.p, tpl, tb, hipMemcpyHostToDevice, s));
+  }
+  float* d_map = nullptr;
+  if (map) {
+    CHK(E.mapbuf.ensure(sizeof(float) * (size_t)rw * rh));
+    d_map = E.mapbuf.as<float>();
+  }
+  CHK(run_match(E, s, E.img.as<uint8_t>(), istride, iw, ih, E.tpl.as<uint8_t>(), tstride, tw, th, cn_mode, step,
+                d_map));
+  if (map) {
+    Stage S(E, "d2h", s);
+    HIPCHK(hipMemcpyAsync(map, d_map, sizeof(float) * (size_t)rw * rh, hipMemcpyDeviceToHost, s));
+  }
+  return finish_result(E, s, out);
+}
+
+
+// --------------------------------------------------------- state helpers --
+
+__global__ void k_state_set(CallState* st, TemplStats ts, int tpl_alpha_bad) {
+  if (threadIdx.x != 0) return;
+  for (int k = 0; k < 4; k++) {
+    st->tsum[k] = ts.tsum[k];
+    st->tsq[k] = ts.tsq[k];
+  }
+  st->varSum = 0;
+  st->cn = 0;
+  st->flat = 0;
+  st->alpha_bad[0] = 0;
+  st->alpha_bad[1] = (unsigned)tpl_alpha_bad;
+  st->templNorm = st->numScale = st->varScale = 0.f;
+}
+
+void launch_state_set(CallState* st, const TemplStats& ts, int tpl_alpha_bad, hipStream_t s) {
+  hipLaunchKernelGGL(k_state_set, dim3(1), dim3(64), 0, s, st, ts, tpl_alpha_bad);
+}
+
+// ------------------------------------------------------------- Matcher --
+
+// Per-parent-geometry template state (matcherGeom matcher.go:36-40): the
+// plan and the template spectrum of every channel the template has, so the
+// device-side channel choice (3 or 4) never needs a rebuild.
+struct MatcherGeom {
+  int rw = 0, rh = 0;
+  TplSpec ts;
+  DevBuf spec, spec2;
+  ~MatcherGeom() {
+    spec.release();
+    spec2.release();
+  }
+};
+
+}  // namespace cvm
+
+struct cvm_matcher {
+  int w = 0, h = 0, step = 0;
+  int aConst = 0;
+  cvm::TemplStats stats{};
+  long long varSum = 0;
+  uint8_t* d_pix = nullptr;  // compact copy, stride w*step
+  std::mutex mu;
+  std::shared_ptr<cvm::MatcherGeom> geom;
+  std::atomic<int> refs{1};
+};
+
+struct cvm_fleet;
+
+namespace cvm {
+
+static void matcher_release(cvm_matcher* m) {
+  if (!m) return;
+  if (m->refs.fetch_sub(1) == 1) {
+    if (m->d_pix) (void)hipFree(m->d_pix);
+    delete m;
+  }
+}
+
+// newMatcher matcher.go:66-76 (+ alphaConst of the template, matcher.go:48):
+// private compact device copy and exact statistics over cn = step channels.
+static int matcher_create(const uint8_t* tpl, int64_t stride, int w, int h, int step, cvm_matcher** out) {
+  if (!tpl) return fail(CVM_E_NIL_IMAGE, "cvmatch: nil image");
+  if (w <= 0 || h <= 0) return fail(CVM_E_EMPTY_IMAGE, "cvmatch: empty image");
+  if (step != 1 && step != 4) return fail(CVM_E_BAD_ARGS, "cvmatch: matcher step must be 1 (gray) or 4 (RGBA)");
+  if (stride < (int64_t)w * step) return fail(CVM_E_BAD_ARGS, "cvmatch: stride below width*step");
+  if ((int64_t)w * h > stats_cap(step))
+    return fail(CVM_E_STATS_CAP, "cvmatch: template area %dx%d exceeds the exact-statistics bound", w, h);
+  Engine& E = engine();
+  std::lock_guard<std::recursive_mutex> lk(E.mu);
+  CHK(E.init());
+  hipStream_t s = E.stream;
+  std::unique_ptr<cvm_matcher> m(new cvm_matcher());
+  m->w = w;
+  m->h = h;
+  m->step = step;
+  HIPCHK(hipMalloc(&m->d_pix, (size_t)w * h * step));
+  HIPCHK(hipMemcpy2DAsync(m->d_pix, (size_t)w * step, tpl, (size_t)stride, (size_t)w * step, h, hipMemcpyHostToDevice,
+                          s));
+  launch_state_init(E.st, s);
+  launch_tpl_stats(m->d_pix, (int64_t)w * step, w, h, step, step, E.st, s);
+  if (step == 4) launch_alpha_scan(m->d_pix, (int64_t)w * 4, w, h, E.st, 1, s);
+  CallState hs;
+  HIPCHK(hipMemcpyAsync(&hs, E.st, sizeof(CallState), hipMemcpyDeviceToHost, s));
+  HIPCHK(hipStreamSynchronize(s));
+  long long area = (long long)w * h, var = 0;
+  for (int k = 0; k < 4; k++) {
+    m->stats.tsum[k] = hs.tsum[k];
+    m->stats.tsq[k] = hs.tsq[k];
+  }
+  for (int k = 0; k < step; k++) var += area * (long long)hs.tsq[k] - (long long)hs.tsum[k] * (long long)hs.tsum[k];
+  m->varSum = var;
+  m->aConst = step == 4 ? (hs.alpha_bad[1] == 0) : 0;
+  *out = m.release();
+  return 0;
+}
+
+// Sets the call state from a matcher's cached statistics and decides cn on
+// the device (Find matcher.go:103-112: 3 only when both alpha planes are
+// constant; the parent is scanned only when the template's alpha is).
+static void matcher_state(Engine& E, hipStream_t s, CallState* st, const cvm_matcher* m, const uint8_t* d_img,
+                          int64_t istride, int iw, int ih) {
+  launch_state_set(st, m->stats, m->step == 4 && !m->aConst, s);
+  if (m->step == 4 && m->aConst) launch_alpha_scan(d_img, istride, iw, ih, st, 0, s);
+  launch_state_finalize(st, m->w, m->h, m->step == 4 ? 0 : 1, s);
+}
+
+// Template spectrum of m on plan p built for all `step` channels.
+static int matcher_spectrum(Engine& E, hipStream_t s, CallState* st, const cvm_matcher* m, const Plan& p, int dh2,
+                            const PlanTables& pt, TplSpec& ts, DevBuf& spec, DevBuf& spec2) {
+  ts.p = p;
+  ts.dh2 = dh2;
+  ts.hwPad = round_up(p.hw, 16);
+  ts.cnMax = m->step;
+  size_t tchan = (size_t)p.dftH * ts.hwPad, tchan2 = (size_t)dh2 * ts.hwPad;
+  CHK(spec.ensure(sizeof(float2) * tchan * m->step));
+  ts.spec = spec.as<float2>();
+  ts.spec2 = nullptr;
+  if (dh2) {
+    CHK(spec2.ensure(sizeof(float2) * tchan2 * m->step));
+    ts.spec2 = spec2.as<float2>();
+  }
+  // build every channel: a state whose cn is the template's channel count
+  launch_state_set(st, m->stats, 0, s);
+  launch_state_finalize(st, m->w, m->h, m->step, s);
+  return build_tspec(E, s, st, m->d_pix, (int64_t)m->w * m->step, m->w, m->h, m->step, m->step, p, dh2, pt, ts.spec,
+                     ts.spec2);
+}
+
+static int matcher_find_dev(Engine& E, hipStream_t s, cvm_matcher* m, const uint8_t* d_img, int64_t istride, int iw,
+                            int ih, Res6* d_res) {
+  int rw = iw - m->w + 1, rh = ih - m->h + 1;
+  std::shared_ptr<MatcherGeom> g;
+  {
+    std::lock_guard<std::mutex> lk(m->mu);
+    g = m->geom;
+  }
+  Plan p;
+  int dh2;
+  PlanTables pt;
+  if (!g || g->rw != rw || g->rh != rh) {
+    p = make_plan(m->w, m->h, rw, rh);
+    CHK(check_limits(p, m->w, rw, rh));
+    dh2 = shrink_dh(p, rh, m->h);
+    CHK(plan_tables(E, p, dh2, pt));
+    auto ng = std::make_shared<MatcherGeom>();
+    ng->rw = rw;
+    ng->rh = rh;
+    CHK(matcher_spectrum(E, s, E.st, m, p, dh2, pt, ng->ts, ng->spec, ng->spec2));
+    std::lock_guard<std::mutex> lk(m->mu);
+    m->geom = ng;
+    g = ng;
+  } else {
+    CHK(plan_tables(E, g->ts.p, g->ts.dh2, pt));
+  }
+  matcher_state(E, s, E.st, m, d_img, istride, iw, ih);
+  return corr_and_normalize(E, s, E.st, d_img, istride, iw, ih, m->w, m->h, m->step, m->step, g->ts, pt, nullptr,
+                            d_res);
+}
+
+// --------------------------------------------------------------- Fleet --
+
+struct FleetBucket {
+  int tw = 0, th = 0;
+  std::vector<int> members;  // global member indices, ascending
+};
+
+// bucketize fleet.go:111-171: pow2 size classes of the non-flat members,
+// largest area first (then wider, then taller); a class joins the first
+// bucket that dominates it in both axes; bucket dims are its members' maxima.
+static std::vector<FleetBucket> bucketize(const std::vector<cvm_matcher*>& ms) {
+  std::vector<std::pair<int, int>> keys;
+  for (auto* m : ms) {
+    if (m->varSum == 0) continue;
+    std::pair<int, int> k(next_pow2(m->w), next_pow2(m->h));
+    if (std::find(keys.begin(), keys.end(), k) == keys.end()) keys.push_back(k);
+  }
+  std::sort(keys.begin(), keys.end(), [](const std::pair<int, int>& a, const std::pair<int, int>& b) {
+    long long aa = (long long)a.first * a.second, ab = (long long)b.first * b.second;
+    if (aa != ab) return aa > ab;
+    if (a.first != b.first) return a.first > b.first;
+    return a.second > b.second;
+  });
+  std::vector<std::pair<std::pair<int, int>, std::vector<int>>> slots;
+  for (auto& k : keys) {
+    std::vector<int> mem;
+    for (size_t i = 0; i < ms.size(); i++)
+      if (ms[i]->varSum != 0 && next_pow2(ms[i]->w) == k.first && next_pow2(ms[i]->h) == k.second)
+        mem.push_back((int)i);
+    bool placed = false;
+    for (auto& s : slots)
+      if (s.first.first >= k.first && s.first.second >= k.second) {
+        s.second.insert(s.second.end(), mem.begin(), mem.end());
+        placed = true;
+        break;
+      }
+    if (!placed) slots.push_back({k, mem});
+  }
+  std::vector<FleetBucket> out;
+  for (auto& s : slots) {
+    FleetBucket b;
+    b.members = s.second;
+    std::sort(b.members.begin(), b.members.end());
+    for (int mi : b.members) {
+      b.tw = std::max(b.tw, ms[mi]->w);
+      b.th = std::max(b.th, ms[mi]->h);
+    }
+    out.push_back(b);
+  }
+  return out;
+}
+
+// Per-bucket geometry for one parent size (buildGeom fleet.go:337-390).
+struct BucketGeom {
+  Plan p;
+  int dh2 = 0, hwPad = 0, cnMax = 0;
+  std::vector<int> cns;            // per bucket position
+  std::vector<TplSpec> ts;         // per bucket position
+  std::vector<DevBuf> specs, specs2;
+  ~BucketGeom() {
+    for (auto& b : specs) b.release();
+    for (auto& b : specs2) b.release();
+  }
+};
+
+struct FleetGeom {
+  int pw = 0, ph = 0, pConst = 0;
+  std::vector<std::unique_ptr<BucketGeom>> bg;
+  CallState* states = nullptr;  // per member
+  CallState* frame_state = nullptr;
+  ~FleetGeom() {
+    if (states) (void)hipFree(states);
+    if (frame_state) (void)hipFree(frame_state);
+  }
+};
+
+}  // namespace cvm
+
+struct cvm_fleet {
+  std::vector<cvm_matcher*> ms;
+  int step = 0;
+  std::vector<cvm::FleetBucket> buckets;
+  std::mutex mu;
+  std::shared_ptr<cvm::FleetGeom> geom;
+  cvm::DevBuf frame, mspec, corr, res;
+};
+
+namespace cvm {
+
+__global__ void k_state_frame(CallState* st, int cn) {
+  if (threadIdx.x != 0) return;
+  for (int k = 0; k < 4; k++) {
+    st->tsum[k] = 0;
+    st->tsq[k] = 0;
+  }
+  st->varSum = 1;
+  st->cn = cn;
+  st->flat = 0;
+  st->alpha_bad[0] = st->alpha_bad[1] = 0;
+  st->templNorm = st->numScale = st->varScale = 0.f;
+}
+
+static int fleet_build_geom(Engine& E, hipStream_t s, cvm_fleet* f, int pw, int ph, int pConst,
+                            std::shared_ptr<FleetGeom>& out) {
+  auto g = std::make_shared<FleetGeom>();
+  g->pw = pw;
+  g->ph = ph;
+  g->pConst = pConst;
+  size_t n = f->ms.size();
+  HIPCHK(hipMalloc(&g->states, sizeof(CallState) * std::max(n, (size_t)1)));
+  HIPCHK(hipMalloc(&g->frame_state, sizeof(CallState) * std::max(f->buckets.size(), (size_t)1)));
+  for (size_t bi = 0; bi < f->buckets.size(); bi++) {
+    const FleetBucket& b = f->buckets[bi];
+    auto bg = std::make_unique<BucketGeom>();
+    int rwF = pw - b.tw + 1, rhF = ph - b.th + 1;
+    bg->p = make_plan(b.tw, b.th, rwF, rhF);
+    CHK(check_limits(bg->p, b.tw, rwF, rhF));
+    bg->dh2 = shrink_dh(bg->p, rhF, b.th);
+    bg->hwPad = round_up(bg->p.hw, 16);
+    PlanTables pt;
+    CHK(plan_tables(E, bg->p, bg->dh2, pt));
+    size_t nm = b.members.size();
+    bg->cns.resize(nm);
+    bg->ts.resize(nm);
+    bg->specs.resize(nm);
+    bg->specs2.resize(nm);
+    for (size_t j = 0; j < nm; j++) {
+      cvm_matcher* m = f->ms[b.members[j]];
+      int cn = m->step == 4 ? ((m->aConst && pConst) ? 3 : 4) : 1;
+      bg->cns[j] = cn;
+      bg->cnMax = std::max(bg->cnMax, cn);
+      TplSpec& ts = bg->ts[j];
+      ts.p = bg->p;
+      ts.dh2 = bg->dh2;
+      ts.hwPad = bg->hwPad;
+      ts.cnMax = cn;
+      size_t tchan = (size_t)bg->p.dftH * bg->hwPad, tchan2 = (size_t)bg->dh2 * bg->hwPad;
+      CHK(bg->specs[j].ensure(sizeof(float2) * tchan * cn));
+      ts.spec = bg->specs[j].as<float2>();
+      ts.spec2 = nullptr;
+      if (bg->dh2) {
+        CHK(bg->specs2[j].ensure(sizeof(float2) * tchan2 * cn));
+        ts.spec2 = bg->specs2[j].as<float2>();
+      }
+      // member spectrum on the bucket plan, shrunk with the bucket th
+      // (buildTSpecSet(..., rhF, b.th, ...) fleet.go:373-374)
+      CallState* mst = g->states + b.members[j];
+      launch_state_set(mst, m->stats, 0, s);
+      launch_state_finalize(mst, m->w, m->h, cn, s);
+      CHK(build_tspec(E, s, mst, m->d_pix, (int64_t)m->w * m->step, m->w, m->h, m->step, cn, bg->p, bg->dh2, pt,
+                      ts.spec, ts.spec2));
+    }
+    hipLaunchKernelGGL(k_state_frame, dim3(1), dim3(64), 0, s, g->frame_state + bi, bg->cnMax);
+    g->bg.push_back(std::move(bg));
+  }
+  HIPCHK(hipGetLastError());
+  out = g;
+  return 0;
+}
+
+// FindAll fleet.go:176-332 on a device-resident parent.
+static int fleet_find_dev(Engine& E, hipStream_t s, cvm_fleet* f, const uint8_t* d_img, int64_t istride, int pw,
+                          int ph, int pConst, Res6* d_res, std::vector<int>& flat_members) {
+  std::shared_ptr<FleetGeom> g;
+  {
+    std::lock_guard<std::mutex> lk(f->mu);
+    g = f->geom;
+  }
+  if (!g || g->pw != pw || g->ph != ph || g->pConst != pConst) {
+    std::shared_ptr<FleetGeom> ng;
+    CHK(fleet_build_geom(E, s, f, pw, ph, pConst, ng));
+    std::lock_guard<std::mutex> lk(f->mu);
+    f->geom = ng;
+    g = ng;
+  }
+  for (size_t bi = 0; bi < f->buckets.size(); bi++) {
+    const FleetBucket& b = f->buckets[bi];
+    BucketGeom& bg = *g->bg[bi];
+    const Plan& p = bg.p;
+    PlanTables pt;
+    CHK(plan_tables(E, p, bg.dh2, pt));
+    int rwF = pw - b.tw + 1, rhF = ph - b.th + 1;
+    int ntx = (rwF + p.blockW - 1) / p.blockW, nty = (rhF + p.blockH - 1) / p.blockH, ntiles = ntx * nty;
+    TileGrid gF{};
+    gF.ntx = ntx;
+    gF.nty = nty;
+    gF.blockW = p.blockW;
+    gF.blockH = p.blockH;
+    gF.extW = rwF;
+    gF.extH = rhF;
+    gF.padW = b.tw - 1;
+    gF.padH = b.th - 1;
+    gF.lastY0 = ((rhF - 1) / p.blockH) * p.blockH;
+    gF.dftW = p.dftW;
+    gF.log2W = ilog2(p.dftW);
+    gF.hw = p.hw;
+    gF.hwPad = bg.hwPad;
+    gF.dftH = p.dftH;
+    gF.log2H = ilog2(p.dftH);
+    gF.dftH2 = bg.dh2;
+    gF.log2H2 = bg.dh2 ? ilog2(bg.dh2) : 0;
+    int64_t chan = (int64_t)p.dftH * bg.hwPad;
+    int64_t ftile = chan * bg.cnMax;
+    CHK(f->frame.ensure(sizeof(float2) * (size_t)ftile * ntiles));
+    float2* frame = f->frame.as<float2>();
+    CallState* fst = g->frame_state + bi;
+    {
+      // Phase 1 fleet.go:240-263: the parent's spectrum per tile, once
+      Stage S(E, "fleet_fwd", s);
+      launch_row_fwd(d_img, istride, f->step, bg.cnMax, fst, gF, pt, frame, ftile, chan, ntiles, s);
+      launch_cols(COL_FWD, frame, frame, ftile, ftile, chan, bg.cnMax, fst, gF, pt, nullptr, nullptr, 0, 0, 1.0f,
+                  ntiles, s);
+    }
+    // Phase 2 fleet.go:265-328: each member's tail
+    CHK(f->mspec.ensure(sizeof(float2) * (size_t)ftile * ntiles));
+    float2* mspec = f->mspec.as<float2>();
+    for (size_t j = 0; j < b.members.size(); j++) {
+      int mi = b.members[j];
+      cvm_matcher* m = f->ms[mi];
+      int cn = bg.cns[j];
+      int rw = pw - m->w + 1, rh = ph - m->h + 1;
+      CallState* mst = g->states + mi;
+      TileGrid gM = gF;
+      gM.extW = rw;
+      gM.extH = rh;
+      const TplSpec& ts = bg.ts[j];
+      int64_t tchan2 = (int64_t)bg.dh2 * bg.hwPad;
+      CHK(f->corr.ensure(sizeof(float) * (size_t)rw * rh));
+      float* corr = f->corr.as<float>();
+      {
+        Stage S(E, "fleet_cols", s);
+        launch_cols(COL_MULINV, frame, mspec, ftile, ftile, chan, cn, mst, gM, pt, ts.spec, ts.spec2, chan, tchan2,
+                    1.0f, ntiles, s);
+      }
+      {
+        Stage S(E, "fleet_row_inv", s);
+        launch_row_inv(mspec, ftile, chan, mst, cn, gM, pt, corr, rw, ntiles, s);
+      }
+      int nparts = norm_partials_count(rw, rh);
+      CHK(E.parts.ensure(sizeof(Extrema) * (size_t)nparts));
+      {
+        Stage S(E, "fleet_normalize", s);
+        launch_normalize(d_img, istride, pw, f->step, m->w, m->h, rw, rh, mst, corr, nullptr, E.parts.as<Extrema>(),
+                         s);
+        launch_reduce(E.parts.as<Extrema>(), nparts, mst, rw, d_res + mi, nullptr, rh, s);
+      }
+    }
+  }
+  HIPCHK(hipGetLastError());
+  return 0;
+}
+
+}  // namespace cvm
+
+// ====================================================================== ABI
+
+using namespace cvm;
+
+extern "C" {
+
+const char* cvm_last_error_detail(void) { return g_err; }
+
+const char* cvm_version(void) { return "cvmatch-hip 0.1 (gfx950)"; }
+
+int64_t cvm_stats_cap(int32_t cn) { return stats_cap(cn); }
+
+int64_t cvm_plan(int32_t tw, int32_t th, int32_t rw, int32_t rh, int32_t* out) {
+  Plan p = make_plan(tw, th, rw, rh);
+  out[0] = p.dftW;
+  out[1] = p.dftH;
+  out[2] = p.blockW;
+  out[3] = p.blockH;
+  return p.cost;
+}
+
+int cvm_match_u8(const uint8_t* img, int64_t istride, int32_t iw, int32_t ih, const uint8_t* tpl, int64_t tstride,
+                 int32_t tw, int32_t th, int32_t cn, int32_t step, float* map, cvm_result* out) {
+  if (!img || !tpl) return fail(CVM_E_NIL_IMAGE, "cvmatch: nil image");
+  CHK(check_args(istride, iw, ih, tstride, tw, th, cn, step));
+  return match_host(img, istride, iw, ih, tpl, tstride, tw, th, cn, step, map, out);
+}
+
+static int rgba_args(const uint8_t* img, int32_t iw, int32_t ih, const uint8_t* tpl, int32_t tw, int32_t th) {
+  if (!img || !tpl) return fail(CVM_E_NIL_IMAGE, "cvmatch: nil image");
+  if (iw <= 0 || ih <= 0 || tw <= 0 || th <= 0) return fail(CVM_E_EMPTY_IMAGE, "cvmatch: empty image");
+  return 0;
+}
+
+int cvm_match(const uint8_t* img, int64_t istride, int32_t iw, int32_t ih, const uint8_t* tpl, int64_t tstride,
+              int32_t tw, int32_t th, cvm_result* out) {
+  CHK(rgba_args(img, iw, ih, tpl, tw, th));
+  // the cn=4 bound is the one that applies before the alpha rule is known
+  // on the device; cn=3 and cn=4 share the argument checks otherwise
+  CHK(check_args(istride, iw, ih, tstride, tw, th, 4, 4));
+  return match_host(img, istride, iw, ih, tpl, tstride, tw, th, 0, 4, nullptr, out);
+}
+
+int cvm_match_map(const uint8_t* img, int64_t istride, int32_t iw, int32_t ih, const uint8_t* tpl, int64_t tstride,
+                  int32_t tw, int32_t th, float* map, cvm_result* out) {
+  CHK(rgba_args(img, iw, ih, tpl, tw, th));
+  CHK(check_args(istride, iw, ih, tstride, tw, th, 4, 4));
+  if (!map) return fail(CVM_E_BAD_ARGS, "cvmatch: MatchMap needs an output map");
+  return match_host(img, istride, iw, ih, tpl, tstride, tw, th, 0, 4, map, out);
+}
+
+int cvm_match_gray(const uint8_t* img, int64_t istride, int32_t iw, int32_t ih, int32_t istep, const uint8_t* tpl,
+                   int64_t tstride, int32_t tw, int32_t th, int32_t tstep, cvm_result* out) {
+  CHK(rgba_args(img, iw, ih, tpl, tw, th));
+  if ((istep != 1 && istep != 4) || (tstep != 1 && tstep != 4))
+    return fail(CVM_E_BAD_ARGS, "cvmatch: gray input step must be 1 or 4");
+  if (istride < (int64_t)iw * istep || tstride < (int64_t)tw * tstep)
+    return fail(CVM_E_BAD_ARGS, "cvmatch: bad match arguments (%dx%d in %dx%d, cn=%d step=%d)", tw, th, iw, ih, 1, 1);
+  CHK(check_args(iw, iw, ih, tw, tw, th, 1, 1));
+  Engine& E = engine();
+  std::lock_guard<std::recursive_mutex> lk(E.mu);
+  CHK(E.init());
+  hipStream_t s = E.stream;
+  size_t ib = (size_t)img_bytes(istride, iw, ih, istep), tb = (size_t)img_bytes(tstride, tw, th, tstep);
+  CHK(E.img.ensure(ib));
+  CHK(E.tpl.ensure(tb));
+  {
+    Stage S(E, "h2d", s);
+    HIPCHK(hipMemcpyAsync(E.img.p, img, ib, hipMemcpyHostToDevice, s));
+    HIPCHK(hipMemcpyAsync(E.tpl.p, tpl, tb, hipMemcpyHostToDevice, s));
+  }
+  const uint8_t* gi = E.img.as<uint8_t>();
+  const uint8_t* gt = E.tpl.as<uint8_t>();
+  int64_t gis = istride, gts = tstride;
+  if (istep == 4 || tstep == 4) {
+    Stage S(E, "gray", s);
+    if (istep == 4) {
+      CHK(E.gray0.ensure((size_t)iw * ih));
+      launch_rgba_to_gray(E.img.as<uint8_t>(), istride, iw, ih, E.gray0.as<uint8_t>(), s);
+      gi = E.gray0.as<uint8_t>();
+      gis = iw;
+    }
+    if (tstep == 4) {
+      CHK(E.gray1.ensure((size_t)tw * th));
+      launch_rgba_to_gray(E.tpl.as<uint8_t>(), tstride, tw, th, E.gray1.as<uint8_t>(), s);
+      gt = E.gray1.as<uint8_t>();
+      gts = tw;
+    }
+  }
+  CHK(run_match(E, s, gi, gis, iw, ih, gt, gts, tw, th, 1, 1, nullptr));
+  return finish_result(E, s, out);
+}
+
+int cvm_match_u8_device(const uint8_t* d_img, int64_t istride, int32_t iw, int32_t ih, const uint8_t* d_tpl,
+                        int64_t tstride, int32_t tw, int32_t th, int32_t cn, int32_t step, float* d_map,
+                        cvm_result* out, void* stream) {
+  if (!d_img || !d_tpl) return fail(CVM_E_NIL_IMAGE, "cvmatch: nil image");
+  if (cn == 0) {
+    if (step != 4) return fail(CVM_E_BAD_ARGS, "cvmatch: the alpha rule (cn=0) needs step 4");
+    CHK(check_args(istride, iw, ih, tstride, tw, th, 4, 4));
+  } else {
+    CHK(check_args(istride, iw, ih, tstride, tw, th, cn, step));
+  }
+  Engine& E = engine();
+  std::lock_guard<std::recursive_mutex> lk(E.mu);
+  CHK(E.init());
+  hipStream_t s = stream ? (hipStream_t)stream : E.stream;
+  CHK(run_match(E, s, d_img, istride, iw, ih, d_tpl, tstride, tw, th, cn, step, d_map));
+  return finish_result(E, s, out);
+}
+
+int cvm_alpha_const(const uint8_t* pix, int64_t stride, int32_t w, int32_t h, int32_t* out) {
+  if (!pix) return fail(CVM_E_NIL_IMAGE, "cvmatch: nil image");
+  if (w <= 0 || h <= 0) return fail(CVM_E_EMPTY_IMAGE, "cvmatch: empty image");
+  if (stride < (int64_t)w * 4) return fail(CVM_E_BAD_ARGS, "cvmatch: stride %lld below 4*width", (long long)stride);
+  Engine& E = engine();
+  std::lock_guard<std::recursive_mutex> lk(E.mu);
+  CHK(E.init());
+  hipStream_t s = E.stream;
+  size_t ib = (size_t)img_bytes(stride, w, h, 4);
+  CHK(E.img.ensure(ib));
+  HIPCHK(hipMemcpyAsync(E.img.p, pix, ib, hipMemcpyHostToDevice, s));
+  launch_state_init(E.st, s);
+  launch_alpha_scan(E.img.as<uint8_t>(), stride, w, h, E.st, 0, s);
+  CallState hs;
+  HIPCHK(hipMemcpyAsync(&hs, E.st, sizeof(CallState), hipMemcpyDeviceToHost, s));
+  HIPCHK(hipStreamSynchronize(s));
+  *out = hs.alpha_bad[0] == 0;
+  return 0;
+}
+
+int cvm_rgba_to_gray(const uint8_t* pix, int64_t stride, int32_t w, int32_t h, uint8_t* out) {
+  if (!pix || !out) return fail(CVM_E_NIL_IMAGE, "cvmatch: nil image");
+  if (w <= 0 || h <= 0) return fail(CVM_E_EMPTY_IMAGE, "cvmatch: empty image");
+  if (stride < (int64_t)w * 4) return fail(CVM_E_BAD_ARGS, "cvmatch: stride %lld below 4*width", (long long)stride);
+  Engine& E = engine();
+  std::lock_guard<std::recursive_mutex> lk(E.mu);
+  CHK(E.init());
+  hipStream_t s = E.stream;
+  size_t ib = (size_t)img_bytes(stride, w, h, 4);
+  CHK(E.img.ensure(ib));
+  CHK(E.gray0.ensure((size_t)w * h));
+  HIPCHK(hipMemcpyAsync(E.img.p, pix, ib, hipMemcpyHostToDevice, s));
+  launch_rgba_to_gray(E.img.as<uint8_t>(), stride, w, h, E.gray0.as<uint8_t>(), s);
+  HIPCHK(hipMemcpyAsync(out, E.gray0.p, (size_t)w * h, hipMemcpyDeviceToHost, s));
+  HIPCHK(hipStreamSynchronize(s));
+  return 0;
+}
+
+
+// ------------------------------------------------------ Matcher / Fleet --
+
+int cvm_matcher_new(const uint8_t* tpl, int64_t stride, int32_t w, int32_t h, int32_t step, cvm_matcher** out) {
+  if (!out) return fail(CVM_E_BAD_ARGS, "cvmatch: null output handle");
+  return matcher_create(tpl, stride, w, h, step, out);
+}
+
+int cvm_matcher_find(cvm_matcher* m, const uint8_t* img, int64_t stride, int32_t w, int32_t h, cvm_result* out) {
+  if (!m) return fail(CVM_E_BAD_ARGS, "cvmatch: null matcher");
+  if (!img) return fail(CVM_E_NIL_IMAGE, "cvmatch: nil image");
+  if (w <= 0 || h <= 0) return fail(CVM_E_EMPTY_IMAGE, "cvmatch: empty image");
+  if (m->w > w || m->h > h) return fail(CVM_E_BAD_ARGS, "cvmatch: bad match arguments (%dx%d in %dx%d)", m->w, m->h, w, h);
+  if (stride < (int64_t)w * m->step) return fail(CVM_E_BAD_ARGS, "cvmatch: stride below width*step");
+  if (m->varSum == 0) {  // exactly flat template matcher.go:100-102
+    out->min_v = out->max_v = 1.0f;
+    out->min_x = out->min_y = out->max_x = out->max_y = 0;
+    return 0;
+  }
+  Engine& E = engine();
+  std::lock_guard<std::recursive_mutex> lk(E.mu);
+  CHK(E.init());
+  hipStream_t s = E.stream;
+  size_t ib = (size_t)img_bytes(stride, w, h, m->step);
+  CHK(E.img.ensure(ib));
+  {
+    Stage S(E, "h2d", s);
+    HIPCHK(hipMemcpyAsync(E.img.p, img, ib, hipMemcpyHostToDevice, s));
+  }
+  CHK(matcher_find_dev(E, s, m, E.img.as<uint8_t>(), stride, w, h, E.res_d));
+  return finish_result(E, s, out);
+}
+
+void cvm_matcher_free(cvm_matcher* m) { matcher_release(m); }
+
+int cvm_fleet_new(cvm_matcher* const* ms, int32_t n, cvm_fleet** out) {
+  if (!out) return fail(CVM_E_BAD_ARGS, "cvmatch: null output handle");
+  if (n <= 0 || !ms) return fail(CVM_E_EMPTY_FLEET, "cvmatch: empty fleet");
+  for (int i = 0; i < n; i++)
+    if (!ms[i]) return fail(CVM_E_BAD_ARGS, "cvmatch: null matcher in fleet");
+  for (int i = 0; i < n; i++)
+    if (ms[i]->step != ms[0]->step) return fail(CVM_E_MIXED_FLEET, "cvmatch: fleet mixes color and gray matchers");
+  cvm_fleet* f = new cvm_fleet();
+  f->step = ms[0]->step;
+  for (int i = 0; i < n; i++) {
+    ms[i]->refs.fetch_add(1);
+    f->ms.push_back(ms[i]);
+  }
+  f->buckets = bucketize(f->ms);
+  *out = f;
+  return 0;
+}
+
+int cvm_fleet_find_all(cvm_fleet* f, const uint8_t* img, int64_t stride, int32_t w, int32_t h, cvm_result* out) {
+  if (!f) return fail(CVM_E_BAD_ARGS, "cvmatch: null fleet");
+  if (!img) return fail(CVM_E_NIL_IMAGE, "cvmatch: nil image");
+  if (w <= 0 || h <= 0) return fail(CVM_E_EMPTY_IMAGE, "cvmatch: empty image");
+  if (stride < (int64_t)w * f->step) return fail(CVM_E_BAD_ARGS, "cvmatch: stride below width*step");
+  size_t n = f->ms.size();
+  std::vector<int> flat;
+  for (size_t i = 0; i < n; i++) {
+    cvm_matcher* m = f->ms[i];
+    if (m->w > w || m->h > h)
+      return fail(CVM_E_BAD_ARGS, "cvmatch: bad match arguments (%dx%d in %dx%d)", m->w, m->h, w, h);
+    if (m->varSum == 0) {
+      out[i].min_v = out[i].max_v = 1.0f;
+      out[i].min_x = out[i].min_y = out[i].max_x = out[i].max_y = 0;
+      flat.push_back((int)i);
+    }
+  }
+  if (f->buckets.empty()) return 0;
+  Engine& E = engine();
+  std::lock_guard<std::recursive_mutex> lk(E.mu);
+  CHK(E.init());
+  hipStream_t s = E.stream;
+  size_t ib = (size_t)img_bytes(stride, w, h, f->step);
+  CHK(E.img.ensure(ib));
+  {
+    Stage S(E, "h2d", s);
+    HIPCHK(hipMemcpyAsync(E.img.p, img, ib, hipMemcpyHostToDevice, s));
+  }
+  int pConst = 0;
+  if (f->step == 4) {
+    // the parent's alpha constancy keys the geometry cache (fleet.go:202-212)
+    launch_state_init(E.st, s);
+    launch_alpha_scan(E.img.as<uint8_t>(), stride, w, h, E.st, 0, s);
+    CallState hs;
+    HIPCHK(hipMemcpyAsync(&hs, E.st, sizeof(CallState), hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    pConst = hs.alpha_bad[0] == 0;
+  }
+  CHK(f->res.ensure(sizeof(Res6) * n));
+  CHK(fleet_find_dev(E, s, f, E.img.as<uint8_t>(), stride, w, h, pConst, f->res.as<Res6>(), flat));
+  std::vector<Res6> hr(n);
+  HIPCHK(hipMemcpyAsync(hr.data(), f->res.p, sizeof(Res6) * n, hipMemcpyDeviceToHost, s));
+  HIPCHK(hipStreamSynchronize(s));
+  E.stage_collect();
+  for (size_t i = 0; i < n; i++)
+    if (f->ms[i]->varSum != 0) memcpy(&out[i], &hr[i], sizeof(cvm_result));
+  return 0;
+}
+
+void cvm_fleet_free(cvm_fleet* f) {
+  if (!f) return;
+  {
+    Engine& E = engine();
+    std::lock_guard<std::recursive_mutex> lk(E.mu);
+    if (E.ready) (void)hipStreamSynchronize(E.stream);
+    f->geom.reset();
+    f->frame.release();
+    f->mspec.release();
+    f->corr.release();
+    f->res.release();
+  }
+  for (auto* m : f->ms) matcher_release(m);
+  delete f;
+}
+
+// ------------------------------------------------------------- helpers --
+
+int cvm_device_alloc(size_t bytes, void** out) {
+  Engine& E = engine();
+  {
+    std::lock_guard<std::recursive_mutex> lk(E.mu);
+    CHK(E.init());
+  }
+  HIPCHK(hipMalloc(out, bytes));
+  return 0;
+}
+
+int cvm_device_free(void* p) {
+  HIPCHK(hipFree(p));
+  return 0;
+}
+
+int cvm_copy_h2d(void* dst, const void* src, size_t bytes) {
+  HIPCHK(hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice));
+  return 0;
+}
+
+int cvm_copy_d2h(void* dst, const void* src, size_t bytes) {
+  HIPCHK(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost));
+  return 0;
+}
+
+int cvm_synchronize(void) {
+  HIPCHK(hipDeviceSynchronize());
+  return 0;
+}
+
+int cvm_profile_enable(int32_t on) {
+  Engine& E = engine();
+  std::lock_guard<std::recursive_mutex> lk(E.mu);
+  E.prof = on != 0;
+  return 0;
+}
+
+int cvm_profile_reset(void) {
+  Engine& E = engine();
+  std::lock_guard<std::recursive_mutex> lk(E.mu);
+  E.prof_acc.clear();
+  return 0;
+}
+
+int cvm_profile_read(char* names, double* total_ms, int64_t* count, int32_t max) {
+  Engine& E = engine();
+  std::lock_guard<std::recursive_mutex> lk(E.mu);
+  int i = 0;
+  for (auto& kv : E.prof_acc) {
+    if (i >= max) break;
+    snprintf(names + 32 * i, 32, "%s", kv.first.c_str());
+    total_ms[i] = kv.second.first;
+    count[i] = kv.second.second;
+    i++;
+  }
+  return i;
+}
+
+}  // extern "C"
+
+// ------------------------------------------------------------ self tests --
+
+namespace cvm {
+__global__ void k_selftest_fp(const float* a, const float* b, const long long* ints, int n, float* osq, float* odiv,
+                              float* ocvt) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  osq[i] = (float)__dsqrt_rn((double)a[i]);
+  odiv[i] = (float)__ddiv_rn((double)a[i], (double)b[i]);
+  ocvt[i] = (float)(double)ints[i];
+}
+}  // namespace cvm
+
+extern "C" int cvm_selftest_fp(const float* a, const float* b, const int64_t* ints, int32_t n, float* out_sqrt,
+                               float* out_div, float* out_cvt) {
+  Engine& E = engine();
+  std::lock_guard<std::recursive_mutex> lk(E.mu);
+  CHK(E.init());
+  hipStream_t s = E.stream;
+  DevBuf da, db, di, o1, o2, o3;
+  size_t fb = sizeof(float) * (size_t)n, ibytes = sizeof(int64_t) * (size_t)n;
+  CHK(da.ensure(fb));
+  CHK(db.ensure(fb));
+  CHK(di.ensure(ibytes));
+  CHK(o1.ensure(fb));
+  CHK(o2.ensure(fb));
+  CHK(o3.ensure(fb));
+  HIPCHK(hipMemcpyAsync(da.p, a, fb, hipMemcpyHostToDevice, s));
+  HIPCHK(hipMemcpyAsync(db.p, b, fb, hipMemcpyHostToDevice, s));
+  HIPCHK(hipMemcpyAsync(di.p, ints, ibytes, hipMemcpyHostToDevice, s));
+  hipLaunchKernelGGL(k_selftest_fp, dim3((n + 255) / 256), dim3(256), 0, s, da.as<float>(), db.as<float>(),
+                     di.as<long long>(), n, o1.as<float>(), o2.as<float>(), o3.as<float>());
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipMemcpyAsync(out_sqrt, o1.p, fb, hipMemcpyDeviceToHost, s));
+  HIPCHK(hipMemcpyAsync(out_div, o2.p, fb, hipMemcpyDeviceToHost, s));
+  HIPCHK(hipMemcpyAsync(out_cvt, o3.p, fb, hipMemcpyDeviceToHost, s));
+  HIPCHK(hipStreamSynchronize(s));
+  da.release();
+  db.release();
+  di.release();
+  o1.release();
+  o2.release();
+  o3.release();
+  return 0;
+}
+
+extern "C" int cvm_twiddles(int32_t n, float* tri, int32_t* brev, int32_t* tri_len_out) {
+  if (n < 1 || (n & (n - 1)) || n > (1 << 22)) return fail(CVM_E_BAD_ARGS, "cvmatch: twiddle size %d", n);
+  Engine& E = engine();
+  std::lock_guard<std::recursive_mutex> lk(E.mu);
+  CHK(E.init());
+  const FftTables* t;
+  CHK(E.tab(n, &t));
+  int len = tri_len(n);
+  HIPCHK(hipMemcpyAsync(tri, t->tri, sizeof(float2) * (size_t)len, hipMemcpyDeviceToHost, E.stream));
+  HIPCHK(hipMemcpyAsync(brev, t->brev, sizeof(int32_t) * (size_t)n, hipMemcpyDeviceToHost, E.stream));
+  HIPCHK(hipStreamSynchronize(E.stream));
+  *tri_len_out = len;
+  return 0;
+}

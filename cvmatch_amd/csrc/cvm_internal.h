@@ -1,0 +1,138 @@
+// Internal declarations shared by the HIP translation units of libcvmatch_hip.
+//
+// Numerical contract (the reference's scalar op sequence, radix4.go:17-31,
+// impl.go:376-464, 816-849): every float op is one IEEE-754 rounding, never a
+// fused multiply-add.  All sources are compiled with -ffp-contract=off and
+// without fast-math, so `a * b - c * d` stays two roundings of products plus
+// one of the difference, exactly as the reference writes it.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace cvm {
+
+// ------------------------------------------------------------ geometry --
+
+// Tiling of one correlation (crossCorrGo impl.go:616-693, FindAll
+// fleet.go:214-328).  Tile t covers result columns [x0, x0+bw) and rows
+// [y0, y0+bh) with x0 = (t % ntx) * blockW, y0 = (t / ntx) * blockH; interior
+// tiles have bw = blockW / bh = blockH, the last tile on each axis runs to
+// the extent (extW - x0, extH - y0).  Forward loads are bw+padW columns by
+// bh+padH rows.  Tiles whose y0 == lastY0 use the shrunk height dftH2 when
+// dftH2 != 0 (shrinkDH impl.go:573-581).
+struct TileGrid {
+  int ntx, nty;
+  int blockW, blockH;
+  int extW, extH;
+  int padW, padH;
+  int lastY0;
+  int dftW, log2W, hw, hwPad;   // row transform and spectrum row stride (complex)
+  int dftH, log2H;              // full column height
+  int dftH2, log2H2;            // shrunk last-band height (0: none)
+};
+
+__host__ __device__ inline void tile_rect(const TileGrid& g, int t, int& x0, int& y0, int& bw, int& bh) {
+  int tx = t % g.ntx, ty = t / g.ntx;
+  x0 = tx * g.blockW;
+  y0 = ty * g.blockH;
+  bw = (tx == g.ntx - 1) ? g.extW - x0 : g.blockW;
+  bh = (ty == g.nty - 1) ? g.extH - y0 : g.blockH;
+}
+
+__host__ __device__ inline bool tile_shrunk(const TileGrid& g, int y0) { return g.dftH2 != 0 && y0 == g.lastY0; }
+
+// Device-resident per-call state.  Kernels read the channel count and the
+// flat flag from here, so no host round trip sits inside a call.
+struct CallState {
+  unsigned long long tsum[4];   // template channel sums (exact)
+  unsigned long long tsq[4];    // template channel square sums (exact)
+  long long varSum;             // sum_k area*tsq_k - tsum_k^2
+  int cn;                       // effective channel count used by every kernel
+  int flat;                     // varSum == 0
+  unsigned int alpha_bad[2];    // parent / template alpha not constant
+  float templNorm, numScale, varScale;
+  int pad_;
+};
+
+// Exact template statistics of up to four channels (templStats impl.go:1063-1079).
+struct TemplStats {
+  unsigned long long tsum[4];
+  unsigned long long tsq[4];
+};
+
+// Min/max partial of one normalisation workgroup: (value, raster index).
+struct Extrema {
+  float min_v;
+  int min_i;
+  float max_v;
+  int max_i;
+};
+
+// Tables for one FFT size: stage twiddle triplets and the bit-reversal map.
+struct FftTables {
+  int n = 0;
+  float2* tri = nullptr;   // device
+  int32_t* brev = nullptr; // device
+};
+
+// Tables for both axes of a plan.
+struct PlanTables {
+  const float2* triW;
+  const int32_t* brevW;
+  const float2* triH;
+  const int32_t* brevH;
+  const float2* triH2;
+  const int32_t* brevH2;
+};
+
+// ---------------------------------------------------------- launchers --
+// Implemented in cvm_fft.hip / cvm_norm.hip; all stream-ordered, no sync.
+
+int tri_len(int n);
+void launch_twiddles(float2* tri, int32_t* brev, int n, hipStream_t s);
+
+// Spectrum column modes.
+enum ColMode { COL_TSPEC = 0, COL_CORR = 1, COL_FWD = 2, COL_MULINV = 3 };
+
+// Forward packed row FFTs of every loaded row pair of every tile/channel.
+// spec layout: [tile][channel][dftH][hwPad] complex (tile stride given).
+void launch_row_fwd(const uint8_t* img, int64_t istride, int step, int cnMax, const CallState* st,
+                    const TileGrid& g, const PlanTables& pt, float2* spec, int64_t tileStride, int64_t chanStride,
+                    int ntiles, hipStream_t s);
+
+// Column pass: see ColMode.  tspec/tspec2 are [channel][rows][hwPad].
+void launch_cols(ColMode mode, const float2* src, float2* dst, int64_t srcTileStride, int64_t dstTileStride,
+                 int64_t chanStride, int cnMax, const CallState* st, const TileGrid& g, const PlanTables& pt,
+                 const float2* tspec, const float2* tspec2, int64_t tchanStride, int64_t tchanStride2, float scale,
+                 int ntiles, hipStream_t s);
+
+// Shrunk-band template spectrum: rows r<<shift of the full one times 2^shift.
+void launch_shrink(const float2* tspec, float2* tspec2, int cnMax, int rows2, int shift, int hwPad,
+                   int64_t chanStride, int64_t chanStride2, hipStream_t s);
+
+// Inverse packed row FFTs + channel accumulation + emit into corr[rh][rw].
+void launch_row_inv(const float2* spec, int64_t tileStride, int64_t chanStride, const CallState* st, int cnMax,
+                    const TileGrid& g, const PlanTables& pt, float* corr, int rw, int ntiles, hipStream_t s);
+
+// Call-state set-up: zero, template statistics, alpha scans, finalisation.
+void launch_state_init(CallState* st, hipStream_t s);
+void launch_state_set(CallState* st, const TemplStats& ts, int tpl_alpha_bad, hipStream_t s);
+void launch_tpl_stats(const uint8_t* tpl, int64_t tstride, int tw, int th, int cnMax, int step, CallState* st,
+                      hipStream_t s);
+void launch_alpha_scan(const uint8_t* pix, int64_t stride, int w, int h, CallState* st, int which, hipStream_t s);
+// cn_mode: 1..4 fixed; 0 = RGBA auto (3 iff both alpha planes constant)
+void launch_state_finalize(CallState* st, int tw, int th, int cn_mode, hipStream_t s);
+
+// Exact-statistics normalisation + tail + per-workgroup extrema.
+// map may alias corr (in place) or be null (extrema only).
+int norm_partials_count(int rw, int rh);
+void launch_normalize(const uint8_t* img, int64_t istride, int iw, int step, int tw, int th, int rw, int rh,
+                      const CallState* st, const float* corr, float* map, Extrema* partials, hipStream_t s);
+// Final first-occurrence reduction into res6 (device) = {min_v, min_x, min_y, max_v, max_x, max_y}.
+void launch_reduce(const Extrema* partials, int nparts, const CallState* st, int rw, void* res6, float* map_if_flat,
+                   int rh, hipStream_t s);
+
+void launch_rgba_to_gray(const uint8_t* pix, int64_t stride, int w, int h, uint8_t* out, hipStream_t s);
+
+}  // namespace cvm

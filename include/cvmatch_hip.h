@@ -1,0 +1,161 @@
+/*
+ * cvmatch_hip.h -- C ABI of libcvmatch_hip.so, the MI355X (gfx950) engine for
+ * OpenCV-compatible TM_CCOEFF_NORMED template matching + minMaxLoc.
+ *
+ * The entry points are what the reference's Go API binds to through a thin
+ * cgo shim.  Each declaration names the reference function it replaces
+ * (paths relative to the hkloudou/cvmatch source tree).  Conventions:
+ *   - images are row-major uint8 with a byte stride >= width*step; step is 4
+ *     for interleaved RGBA and 1 for gray; channel k sits at byte offset k;
+ *   - result maps are row-major float32 of rw x rh, rw = iw-tw+1, rh = ih-th+1,
+ *     element (x, y) at map[y*rw + x];
+ *   - every call is synchronous: on return the result (and map) are written;
+ *   - functions return CVM_OK or a CVM_E_* code; cvm_last_error_detail()
+ *     returns the calling thread's last message, worded as the reference's
+ *     panic strings so a shim can re-panic with them verbatim;
+ *   - all functions are safe to call from several host threads.
+ * Output bits are identical to the reference's scalar op sequence on the same
+ * inputs (same tile plan, twiddles, packing, rounding order; no FMA).
+ */
+#ifndef CVMATCH_HIP_H
+#define CVMATCH_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* The six-tuple of Match/MatchGray/Find and Result (fleet.go:13-18). */
+typedef struct {
+  float min_v;
+  int32_t min_x, min_y;
+  float max_v;
+  int32_t max_x, max_y;
+} cvm_result;
+
+enum {
+  CVM_OK = 0,
+  CVM_E_BAD_ARGS = 1,    /* "cvmatch: bad match arguments (...)"  impl.go:1129, matcher.go:98 */
+  CVM_E_STATS_CAP = 2,   /* "cvmatch: template area ... exact-statistics bound" impl.go:1132 */
+  CVM_E_EMPTY_FLEET = 3, /* "cvmatch: empty fleet" fleet.go:87 */
+  CVM_E_MIXED_FLEET = 4, /* "cvmatch: fleet mixes color and gray matchers" fleet.go:95 */
+  CVM_E_HIP = 5,         /* HIP runtime failure (no reference counterpart) */
+  CVM_E_UNSUPPORTED = 6, /* geometry beyond this build's kernel limits (no reference counterpart) */
+  CVM_E_NIL_IMAGE = 7,   /* "cvmatch: nil image" cvmatch.go:172 */
+  CVM_E_EMPTY_IMAGE = 8  /* "cvmatch: empty image" cvmatch.go:176 */
+};
+
+typedef struct cvm_matcher cvm_matcher;
+typedef struct cvm_fleet cvm_fleet;
+
+/* Thread-local detail of the last failing call on this thread. */
+const char *cvm_last_error_detail(void);
+
+/* Library version string. */
+const char *cvm_version(void);
+
+/* ---------------------------------------------------------------- core -- */
+
+/* matchU8 impl.go:1126-1155: the whole pipeline on host buffers.
+ * cn in [1,4], step >= cn.  map: NULL (extrema only) or rw*rh floats. */
+int cvm_match_u8(const uint8_t *img, int64_t istride, int32_t iw, int32_t ih, const uint8_t *tpl, int64_t tstride,
+                 int32_t tw, int32_t th, int32_t cn, int32_t step, float *map, cvm_result *out);
+
+/* Match cvmatch.go:75-94 on interleaved RGBA host buffers: cn = 3 when both
+ * alpha planes are constant (cvmatch.go:82-85), else 4. */
+int cvm_match(const uint8_t *img, int64_t istride, int32_t iw, int32_t ih, const uint8_t *tpl, int64_t tstride,
+              int32_t tw, int32_t th, cvm_result *out);
+
+/* MatchMap cvmatch.go:100-117: as cvm_match plus the full response map
+ * (caller-owned, rw*rh floats). */
+int cvm_match_map(const uint8_t *img, int64_t istride, int32_t iw, int32_t ih, const uint8_t *tpl, int64_t tstride,
+                  int32_t tw, int32_t th, float *map, cvm_result *out);
+
+/* MatchGray cvmatch.go:128-139.  istep/tstep: 1 = already gray (the
+ * *image.Gray / YCbCr-Y views of toGray cvmatch.go:201-217), 4 = RGBA/NRGBA
+ * bytes converted with rgbToGray's BT.601 weights (cvmatch.go:222-240) on
+ * the device. */
+int cvm_match_gray(const uint8_t *img, int64_t istride, int32_t iw, int32_t ih, int32_t istep, const uint8_t *tpl,
+                   int64_t tstride, int32_t tw, int32_t th, int32_t tstep, cvm_result *out);
+
+/* alphaConst cvmatch.go:146-168: *out = 1 iff every alpha byte equals the first. */
+int cvm_alpha_const(const uint8_t *pix, int64_t stride, int32_t w, int32_t h, int32_t *out);
+
+/* rgbToGray cvmatch.go:222-240: out is w*h bytes (stride w). */
+int cvm_rgba_to_gray(const uint8_t *pix, int64_t stride, int32_t w, int32_t h, uint8_t *out);
+
+/* statsCap impl.go:1118-1124: largest template area with exact int64 statistics. */
+int64_t cvm_stats_cap(int32_t cn);
+
+/* newGoPlan impl.go:246-295: the tile geometry the engine uses
+ * (out = {dftW, dftH, blockW, blockH}); returns the model cost. */
+int64_t cvm_plan(int32_t tw, int32_t th, int32_t rw, int32_t rh, int32_t *out);
+
+/* ------------------------------------------------------------- Matcher -- */
+
+/* NewMatcher matcher.go:45-53 (step 4) / NewGrayMatcher matcher.go:57-64
+ * (step 1, tpl already gray).  The template bytes are copied; the caller's
+ * buffer is not retained. */
+int cvm_matcher_new(const uint8_t *tpl, int64_t stride, int32_t w, int32_t h, int32_t step, cvm_matcher **out);
+
+/* (*Matcher).Find matcher.go:83-130: bit-identical to cvm_match (step 4) or
+ * cvm_match_gray on gray inputs (step 1).  img has the matcher's step. */
+int cvm_matcher_find(cvm_matcher *m, const uint8_t *img, int64_t stride, int32_t w, int32_t h, cvm_result *out);
+
+/* Releases a matcher (the reference relies on the Go GC; the shim calls this
+ * from a finalizer).  A fleet keeps its own reference to its members. */
+void cvm_matcher_free(cvm_matcher *m);
+
+/* --------------------------------------------------------------- Fleet -- */
+
+/* NewFleet fleet.go:85-100: members bucketed by shape dominance
+ * (bucketize fleet.go:111-171).  The member list is copied. */
+int cvm_fleet_new(cvm_matcher *const *ms, int32_t n, cvm_fleet **out);
+
+/* (*Fleet).FindAll fleet.go:176-332: out receives n results in
+ * construction order. */
+int cvm_fleet_find_all(cvm_fleet *f, const uint8_t *img, int64_t stride, int32_t w, int32_t h, cvm_result *out);
+
+void cvm_fleet_free(cvm_fleet *f);
+
+/* ------------------------------------------------ device-resident calls -- */
+/* No reference counterpart: the same pipeline on buffers already in device
+ * memory (the benchmark's "inputs resident in HBM" case and callers that
+ * keep frames on the GPU).  stream: a hipStream_t or NULL for the library's
+ * stream.  cn = 0 selects Match's RGBA alpha rule (step must be 4). */
+int cvm_match_u8_device(const uint8_t *d_img, int64_t istride, int32_t iw, int32_t ih, const uint8_t *d_tpl,
+                        int64_t tstride, int32_t tw, int32_t th, int32_t cn, int32_t step, float *d_map,
+                        cvm_result *out, void *stream);
+
+/* Device memory helpers for callers without their own allocator. */
+int cvm_device_alloc(size_t bytes, void **out);
+int cvm_device_free(void *p);
+int cvm_copy_h2d(void *dst, const void *src, size_t bytes);
+int cvm_copy_d2h(void *dst, const void *src, size_t bytes);
+int cvm_synchronize(void);
+
+/* ------------------------------------------------------------ profiling -- */
+/* No reference counterpart.  When enabled, every pipeline stage is bracketed
+ * by HIP events on the stream it runs on and its time accumulated per stage
+ * name.  cvm_profile_read copies up to max entries; returns the entry count. */
+int cvm_profile_enable(int32_t on);
+int cvm_profile_reset(void);
+int cvm_profile_read(char *names /* max*32 bytes */, double *total_ms, int64_t *count, int32_t max);
+
+/* Self-test of the device arithmetic the bit contract relies on (binary64
+ * sqrt/division rounding, int64->binary64->binary32 conversion).  Fills
+ * out[i] for the n inputs; used by the tests only. */
+int cvm_selftest_fp(const float *a, const float *b, const int64_t *ints, int32_t n, float *out_sqrt, float *out_div,
+                    float *out_cvt);
+
+/* Twiddle tables generated on the device (FP64), for parity tests:
+ * tri receives the stage triplets (complex64, tri_len entries), brev n ints. */
+int cvm_twiddles(int32_t n, float *tri, int32_t *brev, int32_t *tri_len);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* CVMATCH_HIP_H */

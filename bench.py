@@ -98,13 +98,41 @@ def cpu_baseline(parent, sub, budget_s=3.0):
     }
 
 
+def copy_calibration(mib):
+    import cvmatch_amd as cvm
+
+    L = cvm.lib()
+    n = mib << 20
+    a, b = C.c_void_p(), C.c_void_p()
+    assert L.cvm_device_alloc(n, C.byref(a)) == 0 and L.cvm_device_alloc(n, C.byref(b)) == 0
+    L.cvm_profile_enable(1)
+    for _ in range(5):
+        assert L.cvm_device_copy(b, a, n) == 0
+    names = C.create_string_buffer(32 * 8)
+    tot = (C.c_double * 8)()
+    cnt = (C.c_int64 * 8)()
+    k = L.cvm_profile_read(names, tot, cnt, 8)
+    for i in range(k):
+        nm = names.raw[32 * i:32 * (i + 1)].split(b"\0")[0].decode()
+        if nm == "copy":
+            ms = tot[i] / cnt[i]
+            print(json.dumps({"copy_bytes": n, "avg_ms": ms, "GBps_read_plus_write": 2 * n / ms / 1e6}), flush=True)
+    L.cvm_device_free(a)
+    L.cvm_device_free(b)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=30)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="only run the plain copy kernel over MIB MiB (calibrates FETCH_SIZE/WRITE_SIZE)")
     args = ap.parse_args()
+    if args.copy_calibration:
+        copy_calibration(args.copy_calibration)
+        return
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (the path shards per frame or per "
               f"fleet member; see DESIGN.md)", file=sys.stderr)

@@ -84,6 +84,7 @@ SIGNATURES = {
     "cvm_copy_h2d": ([_P, _P, _SZ], C.c_int),
     "cvm_copy_d2h": ([_P, _P, _SZ], C.c_int),
     "cvm_synchronize": ([], C.c_int),
+    "cvm_device_copy": ([_P, _P, _SZ], C.c_int),
     "cvm_profile_enable": ([_I32], C.c_int),
     "cvm_profile_reset": ([], C.c_int),
     "cvm_profile_read": ([_P, _P, _P, _I32], C.c_int),

@@ -184,7 +184,7 @@ struct Engine {
   bool ready = false;
   hipStream_t stream = nullptr;
   std::map<int, FftTables> tables;
-  DevBuf img, tpl, spec, tspec, tspec2, corr, parts, gray0, gray1, mapbuf;
+  DevBuf img, tpl, spec, tspec, tspec2, corr, parts, gray0, gray1, mapbuf, nscratch;
   CallState* st = nullptr;
   Res6* res_d = nullptr;
   Res6* res_h = nullptr;
@@ -404,11 +404,13 @@ static int corr_and_normalize(Engine& E, hipStream_t s, const CallState* st, con
     Stage S(E, "row_inv", s);
     launch_row_inv(spec, tileStride, chan, st, cnMax, g, pt, corr, rw, ntiles, s);
   }
-  int nparts = norm_partials_count(rw, rh);
+  int nparts = norm_partials_count(rw, rh, tw, th, cnMax);
   CHK(E.parts.ensure(sizeof(Extrema) * (size_t)nparts));
+  CHK(E.nscratch.ensure(norm_scratch_bytes(iw, ih, tw, th, cnMax)));
   {
     Stage S(E, "normalize", s);
-    launch_normalize(d_img, istride, iw, step, tw, th, rw, rh, st, corr, d_map, E.parts.as<Extrema>(), s);
+    launch_normalize(d_img, istride, iw, ih, step, tw, th, rw, rh, cnMax, st, corr, d_map, E.parts.as<Extrema>(),
+                     E.nscratch.p, s);
   }
   {
     Stage S(E, "reduce", s);
@@ -905,12 +907,13 @@ static int fleet_find_dev(Engine& E, hipStream_t s, cvm_fleet* f, const uint8_t*
         Stage S(E, "fleet_row_inv", s);
         launch_row_inv(mspec, ftile, chan, mst, cn, gM, pt, corr, rw, ntiles, s);
       }
-      int nparts = norm_partials_count(rw, rh);
+      int nparts = norm_partials_count(rw, rh, m->w, m->h, cn);
       CHK(E.parts.ensure(sizeof(Extrema) * (size_t)nparts));
+      CHK(E.nscratch.ensure(norm_scratch_bytes(pw, ph, m->w, m->h, cn)));
       {
         Stage S(E, "fleet_normalize", s);
-        launch_normalize(d_img, istride, pw, f->step, m->w, m->h, rw, rh, mst, corr, nullptr, E.parts.as<Extrema>(),
-                         s);
+        launch_normalize(d_img, istride, pw, ph, f->step, m->w, m->h, rw, rh, cn, mst, corr, nullptr,
+                         E.parts.as<Extrema>(), E.nscratch.p, s);
         launch_reduce(E.parts.as<Extrema>(), nparts, mst, rw, d_res + mi, nullptr, rh, s);
       }
     }
@@ -1212,6 +1215,41 @@ int cvm_copy_h2d(void* dst, const void* src, size_t bytes) {
 
 int cvm_copy_d2h(void* dst, const void* src, size_t bytes) {
   HIPCHK(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost));
+  return 0;
+}
+
+}  // extern "C"
+
+namespace cvm {
+__global__ void k_copy16(const uint4* __restrict__ src, uint4* __restrict__ dst, size_t n16) {
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n16; i += (size_t)gridDim.x * blockDim.x)
+    dst[i] = src[i];
+}
+__global__ void k_copy1(const uint8_t* __restrict__ src, uint8_t* __restrict__ dst, size_t n) {
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+    dst[i] = src[i];
+}
+}  // namespace cvm
+
+extern "C" {
+
+int cvm_device_copy(void* dst, const void* src, size_t bytes) {
+  Engine& E = engine();
+  std::lock_guard<std::recursive_mutex> lk(E.mu);
+  CHK(E.init());
+  hipStream_t s = E.stream;
+  bool aligned = (((uintptr_t)dst | (uintptr_t)src) & 15) == 0;
+  size_t n16 = aligned ? bytes / 16 : 0;
+  if (n16) {
+    Stage S(E, "copy", s);
+    hipLaunchKernelGGL(k_copy16, dim3(2048), dim3(256), 0, s, (const uint4*)src, (uint4*)dst, n16);
+  }
+  if (bytes > n16 * 16)
+    hipLaunchKernelGGL(k_copy1, dim3(64), dim3(256), 0, s, (const uint8_t*)src + n16 * 16, (uint8_t*)dst + n16 * 16,
+                       bytes - n16 * 16);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipStreamSynchronize(s));
+  E.stage_collect();
   return 0;
 }
 

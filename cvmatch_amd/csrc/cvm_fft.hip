@@ -4,14 +4,21 @@
 // op sequence (radix4.go:75-149, radix4cols.go:33-125, impl.go:301-464), so
 // the bits do not depend on the launch shape.  What differs from the
 // reference is only the schedule and the storage:
-//   * row transforms run one packed row pair per workgroup, in LDS, with the
-//     bit-reversal permutation folded into the LDS store (radix4.go:93-96);
+//   * transforms run in LDS as register passes: a thread loads the 8, 16 or
+//     32 elements one or two radix-4 stages (and the radix-2 head) combine,
+//     runs those butterflies in registers and stores them back, so an
+//     n-point transform makes ceil(stages/2) LDS round trips;
+//   * the bit-reversal permutation (radix4.go:93-96) is folded into the LDS
+//     stores that feed a transform;
 //   * the forward column transform, the conjugate product with the template
 //     spectrum and the inverse column transform are ONE kernel over an LDS
-//     slab of CG columns, so the spectrum makes one HBM round trip instead of
-//     three; zero pad rows are never read (they are synthesised in LDS);
-//   * the inverse row kernel accumulates the channels in LDS in channel
-//     order and stores each result row once.
+//     slab of CG columns, so the spectrum makes one HBM round trip; zero pad
+//     rows are synthesised in LDS, never read;
+//   * the inverse row kernel accumulates the channels in registers in
+//     channel order and stores each result row once;
+//   * workgroup numbers are remapped so that workgroups which share input
+//     (the channels of one row pair, the tiles of one template-spectrum slab)
+//     run on the same XCD and share its L2.
 #include "cvm_internal.h"
 
 namespace cvm {
@@ -55,69 +62,134 @@ __device__ __forceinline__ void bfly4(float2& A, float2& Cq, float2& B, float2& 
 
 __device__ __forceinline__ float2 conj_if(float2 w, bool inverse) { return inverse ? make_float2(w.x, -w.y) : w; }
 
-// Full cascade over an LDS vector a[n] that already holds bit-reversed input
-// (fftR4Scalar radix4.go:106-149).  Caller synchronises before the call.
-__device__ void cascade_row(float2* a, int n, int log2n, const float2* __restrict__ tri, bool inverse) {
-  const int tid = threadIdx.x, nt = blockDim.x;
-  int h = 1;
-  if (log2n & 1) {
-    for (int i = tid; i < (n >> 1); i += nt) {
-      float2 u = a[2 * i], v = a[2 * i + 1];
-      a[2 * i] = make_float2(u.x + v.x, u.y + v.y);
-      a[2 * i + 1] = make_float2(u.x - v.x, u.y - v.y);
+// Row vectors in LDS get one pad element per 32, so threads that read 8-32
+// consecutive elements each do not pile onto one bank.
+__device__ __forceinline__ int padi(int i) { return i + (i >> 5); }
+__host__ __device__ inline int padded_len(int n) { return n + (n >> 5); }
+
+// makeBitrev impl.go:148-165 as one bit-reverse instruction (log2n >= 1).
+__device__ __forceinline__ int bitrev_idx(int i, int log2n) { return (int)(__brev((unsigned)i) >> (32 - log2n)); }
+
+// Workgroups are dispatched to the 8 XCDs round robin.  Renumber them so
+// that consecutive logical ids run on the same XCD.
+__device__ __forceinline__ int xcd_remap(int p, int g) {
+  int per = g >> 3, rem = g & 7;
+  int x = p & 7, slot = p >> 3;
+  return x * per + (x < rem ? x : rem) + slot;
+}
+
+// ---------------------------------------------------------- LDS transform --
+//
+// nvec independent n-point vectors live in LDS: element i of vector v at
+// a[v * vstride + (PAD ? padi(i) : i) * es].  fft_lds runs the whole radix-4
+// cascade (fftR4Scalar radix4.go:106-149) on all of them, input already in
+// bit-reversed order.  A pass covers index bits [lo, lo+NB): the radix-2
+// head (bit 0, HEAD = 1) and NST radix-4 stages whose digit sits at bits
+// [lo+HEAD+2s, lo+HEAD+2s+2); stage s has quarter size h = 2^(lo+HEAD+2s)
+// and its twiddle index j = index mod h.
+template <int HEAD, int NST, bool PAD, bool VFAST>
+__device__ __forceinline__ void fft_pass(float2* a, int nvec, int lvec, int vstride, int es, int log2n, int lo,
+                                         const float2* __restrict__ tri0, const float2* __restrict__ tri1,
+                                         bool inverse) {
+  constexpr int NB = HEAD + 2 * NST;
+  constexpr int E = 1 << NB;
+  const int lgrp = log2n - NB;  // log2 of the groups per vector
+  const int total = 1 << (lgrp + lvec);
+  const int h0 = 1 << (lo + HEAD);
+  const int h1 = h0 << 2;
+  for (int gi = threadIdx.x; gi < total; gi += blockDim.x) {
+    int v, gg;
+    if (VFAST) {
+      v = gi & ((1 << lvec) - 1);
+      gg = gi >> lvec;
+    } else {
+      gg = gi & ((1 << lgrp) - 1);
+      v = gi >> lgrp;
     }
-    __syncthreads();
-    h = 2;
-  }
-  for (; 4 * h <= n; h *= 4) {
-    for (int b = tid; b < (n >> 2); b += nt) {
-      int j = b & (h - 1);
-      int base = (b - j) * 4 + j;
-      float2 w1 = conj_if(tri[j], inverse), w2 = conj_if(tri[h + j], inverse), w3 = conj_if(tri[2 * h + j], inverse);
-      float2 A = a[base], Cq = a[base + h], B = a[base + 2 * h], D = a[base + 3 * h];
-      bfly4(A, Cq, B, D, w1, w2, w3, inverse);
-      a[base] = A;
-      a[base + h] = Cq;
-      a[base + 2 * h] = B;
-      a[base + 3 * h] = D;
+    if (v >= nvec) continue;  // lvec rounds nvec up to a power of two
+    const int low = gg & ((1 << lo) - 1);
+    const int base = low | ((gg >> lo) << (lo + NB));
+    float2* av = a + v * vstride;
+    float2 r[E];
+#pragma unroll
+    for (int e = 0; e < E; e++) {
+      int idx = base | (e << lo);
+      r[e] = av[(PAD ? padi(idx) : idx) * es];
     }
-    __syncthreads();
-    tri += 3 * h;
+    if (HEAD) {
+#pragma unroll
+      for (int e = 0; e < E; e += 2) {
+        float2 u = r[e], w = r[e + 1];
+        r[e] = make_float2(u.x + w.x, u.y + w.y);
+        r[e + 1] = make_float2(u.x - w.x, u.y - w.y);
+      }
+    }
+    if (NST >= 1) {
+      constexpr int d = HEAD;
+#pragma unroll
+      for (int e = 0; e < E; e++) {
+        if ((e >> d) & 3) continue;
+        int j = low | ((e & ((1 << d) - 1)) << lo);
+        float2 w1 = conj_if(tri0[j], inverse), w2 = conj_if(tri0[h0 + j], inverse),
+               w3 = conj_if(tri0[2 * h0 + j], inverse);
+        bfly4(r[e], r[e | (1 << d)], r[e | (2 << d)], r[e | (3 << d)], w1, w2, w3, inverse);
+      }
+    }
+    if (NST >= 2) {
+      constexpr int d = HEAD + 2;
+#pragma unroll
+      for (int e = 0; e < E; e++) {
+        if ((e >> d) & 3) continue;
+        int j = low | ((e & ((1 << d) - 1)) << lo);
+        float2 w1 = conj_if(tri1[j], inverse), w2 = conj_if(tri1[h1 + j], inverse),
+               w3 = conj_if(tri1[2 * h1 + j], inverse);
+        bfly4(r[e], r[e | (1 << d)], r[e | (2 << d)], r[e | (3 << d)], w1, w2, w3, inverse);
+      }
+    }
+#pragma unroll
+    for (int e = 0; e < E; e++) {
+      int idx = base | (e << lo);
+      av[(PAD ? padi(idx) : idx) * es] = r[e];
+    }
   }
 }
 
-// The same cascade on every column of an LDS slab a[n][CG] (colsR4Go
-// radix4cols.go:260-274 with logical row L at LDS row L).
-template <int CG>
-__device__ void cascade_cols(float2* a, int n, int log2n, const float2* __restrict__ tri, bool inverse) {
-  const int tid = threadIdx.x, nt = blockDim.x;
-  int h = 1;
+// The full cascade: the caller has stored the input and synchronised; on
+// return every vector holds its transform and the block is synchronised.
+// Passes hold at most 16 elements per thread: [head + 1 stage] when log2 n
+// is odd, then two radix-4 stages per pass.
+template <bool PAD, bool VFAST>
+__device__ void fft_lds(float2* a, int nvec, int vstride, int es, int log2n, const float2* __restrict__ tri,
+                        bool inverse) {
+  int lvec = 0;
+  while ((1 << lvec) < nvec) lvec++;
+  const int S = log2n >> 1;  // radix-4 stages
+  int s = 0, lo = 0;
+  const float2* t = tri;
   if (log2n & 1) {
-    for (int idx = tid; idx < (n >> 1) * CG; idx += nt) {
-      int i = idx / CG, c = idx % CG;
-      float2 u = a[(2 * i) * CG + c], v = a[(2 * i + 1) * CG + c];
-      a[(2 * i) * CG + c] = make_float2(u.x + v.x, u.y + v.y);
-      a[(2 * i + 1) * CG + c] = make_float2(u.x - v.x, u.y - v.y);
+    if (S >= 1) {
+      fft_pass<1, 1, PAD, VFAST>(a, nvec, lvec, vstride, es, log2n, 0, t, t, inverse);
+      t += 3 * 2;
+      s = 1;
+      lo = 3;
+    } else {
+      fft_pass<1, 0, PAD, VFAST>(a, nvec, lvec, vstride, es, log2n, 0, t, t, inverse);
+      lo = 1;
     }
     __syncthreads();
-    h = 2;
   }
-  for (; 4 * h <= n; h *= 4) {
-    for (int idx = tid; idx < (n >> 2) * CG; idx += nt) {
-      int b = idx / CG, c = idx % CG;
-      int j = b & (h - 1);
-      int base = (b - j) * 4 + j;
-      float2 w1 = conj_if(tri[j], inverse), w2 = conj_if(tri[h + j], inverse), w3 = conj_if(tri[2 * h + j], inverse);
-      float2 A = a[base * CG + c], Cq = a[(base + h) * CG + c];
-      float2 B = a[(base + 2 * h) * CG + c], D = a[(base + 3 * h) * CG + c];
-      bfly4(A, Cq, B, D, w1, w2, w3, inverse);
-      a[base * CG + c] = A;
-      a[(base + h) * CG + c] = Cq;
-      a[(base + 2 * h) * CG + c] = B;
-      a[(base + 3 * h) * CG + c] = D;
-    }
+  while (s < S) {
+    int nst = (S - s) < 2 ? (S - s) : 2;
+    int h0 = 1 << lo;
+    const float2* t1 = t + 3 * h0;
+    if (nst == 2)
+      fft_pass<0, 2, PAD, VFAST>(a, nvec, lvec, vstride, es, log2n, lo, t, t1, inverse);
+    else
+      fft_pass<0, 1, PAD, VFAST>(a, nvec, lvec, vstride, es, log2n, lo, t, t1, inverse);
     __syncthreads();
-    tri += 3 * h;
+    for (int q = 0; q < nst; q++) t += 3 * (h0 << (2 * q));
+    s += nst;
+    lo += 2 * nst;
   }
 }
 
@@ -220,50 +292,93 @@ void launch_twiddles(float2* tri, int32_t* brev, int n, hipStream_t s) {
 
 // forwardRowPair impl.go:301-344: rows r, r+1 of channel k of tile t packed
 // as one complex row (zero padded to dftW), transformed, untangled into the
+// ------------------------------------------------------ forward row FFTs --
+
+// Channels per workgroup for row transforms of length n: as many as fit a
+// 144 KiB LDS budget (gfx950 gives a workgroup up to 160 KiB), so one
+// workgroup transforms every channel of its row pair.
+static int row_cpg(int n, int cnMax) {
+  int cpg = (int)(147456 / ((size_t)padded_len(n) * sizeof(float2)));
+  if (cpg > cnMax) cpg = cnMax;
+  return cpg < 1 ? 1 : cpg;
+}
+
+// forwardRowPair impl.go:301-344: rows r, r+1 of channel k of tile t packed
+// as one complex row (zero padded to dftW), transformed, untangled into the
 // two half spectra, stored at spectrum rows r and r+1 (natural order; the
-// column kernel applies the bit reversal when it loads).
+// column kernel applies the bit reversal when it loads).  One workgroup
+// handles up to cpg channels of one row pair.
 __global__ __launch_bounds__(256) void k_row_fwd(const uint8_t* __restrict__ img, int64_t istride, int step,
                                                  const CallState* __restrict__ st, TileGrid g,
                                                  const float2* __restrict__ triW, const int32_t* __restrict__ brevW,
-                                                 float2* __restrict__ spec, int64_t tileStride, int64_t chanStride) {
+                                                 float2* __restrict__ spec, int64_t tileStride, int64_t chanStride,
+                                                 int cpg, int ngroups, int npairs) {
   extern __shared__ float2 z[];
-  const int k = blockIdx.z, t = blockIdx.y, r = 2 * blockIdx.x;
-  if (st->flat || k >= st->cn) return;
+  const int L = xcd_remap(blockIdx.x, gridDim.x);
+  const int cgi = L % ngroups, rest = L / ngroups;
+  const int rp = rest % npairs, t = rest / npairs;
+  if (st->flat) return;
+  const int k0 = cgi * cpg;
+  const int nv = min(cpg, st->cn - k0);
+  if (nv <= 0) return;
   int x0, y0, bw, bh;
   tile_rect(g, t, x0, y0, bw, bh);
   const int loadW = bw + g.padW, loadH = bh + g.padH;
   const int dftH = tile_shrunk(g, y0) ? g.dftH2 : g.dftH;
+  const int r = 2 * rp;
   if (r >= loadH || r >= dftH) return;
-  const int n = g.dftW, tid = threadIdx.x, nt = blockDim.x;
-  const uint8_t* ra = img + (int64_t)(y0 + r) * istride + (int64_t)x0 * step + k;
+  const int n = g.dftW, np = padded_len(n), tid = threadIdx.x, nt = blockDim.x;
+  const uint8_t* ra = img + (int64_t)(y0 + r) * istride + (int64_t)x0 * step + k0;
   const bool two = r + 1 < loadH;
-  for (int x = tid; x < n; x += nt) {
-    float2 v = make_float2(0.f, 0.f);
+  // all of this thread's pixels are loaded before any is stored
+  constexpr int MAXI = 16;  // n / threads: 4096 / 256, 1024 / 64
+  uint32_t wa[MAXI], wb[MAXI];
+#pragma unroll
+  for (int i = 0; i < MAXI; i++) {
+    int x = tid + i * nt;
+    wa[i] = wb[i] = 0;
     if (x < loadW) {
-      v.x = (float)ra[(int64_t)x * step];
-      if (two) v.y = (float)ra[istride + (int64_t)x * step];
+      const uint8_t* pa = ra + (int64_t)x * step;
+      for (int v = 0; v < nv; v++) {
+        wa[i] |= (uint32_t)pa[v] << (8 * v);
+        if (two) wb[i] |= (uint32_t)pa[istride + v] << (8 * v);
+      }
     }
-    z[brevW[x]] = v;
+  }
+#pragma unroll
+  for (int i = 0; i < MAXI; i++) {
+    int x = tid + i * nt;
+    if (x < n) {
+      const int pos = padi(bitrev_idx(x, g.log2W));
+      for (int v = 0; v < nv; v++)
+        z[v * np + pos] = make_float2((float)((wa[i] >> (8 * v)) & 255u), (float)((wb[i] >> (8 * v)) & 255u));
+    }
   }
   __syncthreads();
-  cascade_row(z, n, g.log2W, triW, false);
-  float2* sa = spec + (int64_t)t * tileStride + (int64_t)k * chanStride + (int64_t)r * g.hwPad;
-  float2* sb = sa + g.hwPad;
+  fft_lds<true, false>(z, nv, np, 1, g.log2W, triW, false);
   const int mask = n - 1;
-  for (int q = tid; q < g.hw; q += nt) {
-    float2 zk = z[q], zn = z[(n - q) & mask];
-    sa[q] = make_float2(0.5f * (zk.x + zn.x), 0.5f * (zk.y - zn.y));
-    sb[q] = make_float2(0.5f * (zk.y + zn.y), 0.5f * (zn.x - zk.x));
+  for (int v = 0; v < nv; v++) {
+    float2* sa = spec + (int64_t)t * tileStride + (int64_t)(k0 + v) * chanStride + (int64_t)r * g.hwPad;
+    float2* sb = sa + g.hwPad;
+    const float2* zv = z + v * np;
+    for (int q = tid; q < g.hw; q += nt) {
+      float2 zk = zv[padi(q)], zn = zv[padi((n - q) & mask)];
+      sa[q] = make_float2(0.5f * (zk.x + zn.x), 0.5f * (zk.y - zn.y));
+      sb[q] = make_float2(0.5f * (zk.y + zn.y), 0.5f * (zn.x - zk.x));
+    }
   }
 }
 
 void launch_row_fwd(const uint8_t* img, int64_t istride, int step, int cnMax, const CallState* st,
                     const TileGrid& g, const PlanTables& pt, float2* spec, int64_t tileStride, int64_t chanStride,
                     int ntiles, hipStream_t s) {
-  int nt = g.dftW >= 1024 ? 256 : (g.dftW >= 256 ? 64 : 64);
-  size_t lds = sizeof(float2) * (size_t)g.dftW;
-  hipLaunchKernelGGL(k_row_fwd, dim3(g.dftH / 2, ntiles, cnMax), dim3(nt), lds, s, img, istride, step, st, g, pt.triW,
-                     pt.brevW, spec, tileStride, chanStride);
+  int nt = g.dftW >= 1024 ? 256 : 64;
+  int cpg = row_cpg(g.dftW, cnMax);
+  int ngroups = (cnMax + cpg - 1) / cpg;
+  int npairs = g.dftH / 2;
+  size_t lds = sizeof(float2) * (size_t)padded_len(g.dftW) * cpg;
+  hipLaunchKernelGGL(k_row_fwd, dim3(ngroups * npairs * ntiles), dim3(nt), lds, s, img, istride, step, st, g, pt.triW,
+                     pt.brevW, spec, tileStride, chanStride, cpg, ngroups, npairs);
 }
 
 // ----------------------------------------------------------- column pass --
@@ -278,7 +393,9 @@ void launch_row_fwd(const uint8_t* img, int64_t istride, int step, int cnMax, co
 // slots, blockForwardGo impl.go:350-374); after the forward cascade LDS row
 // k holds frequency k.  The product is written to LDS row brev[k] so the
 // inverse cascade again sees bit-reversed input (the rmap = brev relabelling
-// of colsR4Go), and leaves spatial row y at LDS row y.
+// of colsR4Go), and leaves spatial row y at LDS row y.  Logical workgroup
+// order is tile-fastest, so the tiles that read one template-spectrum slab
+// run together on one XCD.
 template <int CG>
 __global__ __launch_bounds__(256) void k_cols(int mode, const float2* __restrict__ src, float2* __restrict__ dst,
                                               int64_t srcTileStride, int64_t dstTileStride, int64_t chanStride,
@@ -286,9 +403,13 @@ __global__ __launch_bounds__(256) void k_cols(int mode, const float2* __restrict
                                               const float2* __restrict__ triH, const int32_t* __restrict__ brevH,
                                               const float2* __restrict__ triH2, const int32_t* __restrict__ brevH2,
                                               const float2* __restrict__ tspec, const float2* __restrict__ tspec2,
-                                              int64_t tchanStride, int64_t tchanStride2, float scale) {
+                                              int64_t tchanStride, int64_t tchanStride2, float scale, int ntiles,
+                                              int nslabs) {
   extern __shared__ float2 a[];
-  const int k = blockIdx.z, t = blockIdx.y, c0 = blockIdx.x * CG;
+  const int L = xcd_remap(blockIdx.x, gridDim.x);
+  const int t = L % ntiles, rest = L / ntiles;
+  const int slab = rest % nslabs, k = rest / nslabs;
+  const int c0 = slab * CG;
   if (st->flat || k >= st->cn || c0 >= g.hw) return;
   int x0, y0, bw, bh;
   tile_rect(g, t, x0, y0, bw, bh);
@@ -310,15 +431,28 @@ __global__ __launch_bounds__(256) void k_cols(int mode, const float2* __restrict
     if (loaded > n) loaded = n;
   }
   const int ncol = (g.hw - c0) < CG ? (g.hw - c0) : CG;
-  for (int idx = tid; idx < n * CG; idx += nt) {
-    int r = idx / CG, c = idx % CG;
-    float2 v = make_float2(0.f, 0.f);
-    if (r < loaded && c < ncol) v = sp[(int64_t)r * hwPad + c];
-    int pos = (mode == COL_MULINV) ? r : brev[r];
-    a[pos * CG + c] = v;
+  const int tot = n * CG;
+  for (int b0 = tid; b0 < tot; b0 += nt * 8) {
+    float2 v[8];
+#pragma unroll
+    for (int u = 0; u < 8; u++) {
+      int idx = b0 + u * nt;
+      int r = idx / CG, c = idx % CG;
+      v[u] = make_float2(0.f, 0.f);
+      if (idx < tot && r < loaded && c < ncol) v[u] = sp[(int64_t)r * hwPad + c];
+    }
+#pragma unroll
+    for (int u = 0; u < 8; u++) {
+      int idx = b0 + u * nt;
+      if (idx < tot) {
+        int r = idx / CG, c = idx % CG;
+        int pos = (mode == COL_MULINV) ? r : bitrev_idx(r, log2n);
+        a[pos * CG + c] = v[u];
+      }
+    }
   }
   __syncthreads();
-  if (mode != COL_MULINV) cascade_cols<CG>(a, n, log2n, tri, false);
+  if (mode != COL_MULINV) fft_lds<false, true>(a, CG, 1, CG, log2n, tri, false);
 
   if (mode == COL_TSPEC || mode == COL_FWD) {
     for (int idx = tid; idx < n * CG; idx += nt) {
@@ -334,23 +468,35 @@ __global__ __launch_bounds__(256) void k_cols(int mode, const float2* __restrict
 
   // conjugate product (mulConjGo impl.go:376-388) fused with the bit
   // reversal: rows k and brev[k] are handled by the same thread.
-  for (int idx = tid; idx < n * CG; idx += nt) {
-    int r = idx / CG, c = idx % CG;
-    int br = brev[r];
-    if (r > br) continue;
-    float2 x = a[r * CG + c], y = a[br * CG + c];
-    float2 tx = make_float2(0.f, 0.f), ty = make_float2(0.f, 0.f);
-    if (c < ncol) {
-      tx = ts[(int64_t)r * hwPad + c0 + c];
-      ty = ts[(int64_t)br * hwPad + c0 + c];
+  for (int b0 = tid; b0 < tot; b0 += nt * 4) {
+    float2 tx[4], ty[4];
+#pragma unroll
+    for (int u = 0; u < 4; u++) {
+      int idx = b0 + u * nt;
+      int r = idx / CG, c = idx % CG;
+      int br = bitrev_idx(r, log2n);
+      tx[u] = ty[u] = make_float2(0.f, 0.f);
+      if (idx < tot && r <= br && c < ncol) {
+        tx[u] = ts[(int64_t)r * hwPad + c0 + c];
+        ty[u] = ts[(int64_t)br * hwPad + c0 + c];
+      }
     }
-    float p0 = x.x * tx.x, p1 = x.y * tx.y, p2 = x.y * tx.x, p3 = x.x * tx.y;
-    float q0 = y.x * ty.x, q1 = y.y * ty.y, q2 = y.y * ty.x, q3 = y.x * ty.y;
-    a[br * CG + c] = make_float2(p0 + p1, p2 - p3);
-    a[r * CG + c] = make_float2(q0 + q1, q2 - q3);
+#pragma unroll
+    for (int u = 0; u < 4; u++) {
+      int idx = b0 + u * nt;
+      int r = idx / CG, c = idx % CG;
+      int br = bitrev_idx(r, log2n);
+      if (idx < tot && r <= br) {
+        float2 x = a[r * CG + c], y = a[br * CG + c];
+        float p0 = x.x * tx[u].x, p1 = x.y * tx[u].y, p2 = x.y * tx[u].x, p3 = x.x * tx[u].y;
+        float q0 = y.x * ty[u].x, q1 = y.y * ty[u].y, q2 = y.y * ty[u].x, q3 = y.x * ty[u].y;
+        a[br * CG + c] = make_float2(p0 + p1, p2 - p3);
+        a[r * CG + c] = make_float2(q0 + q1, q2 - q3);
+      }
+    }
   }
   __syncthreads();
-  cascade_cols<CG>(a, n, log2n, tri, true);
+  fft_lds<false, true>(a, CG, 1, CG, log2n, tri, true);
 
   int need = (bh + 1) & ~1;
   if (need > n) need = n;
@@ -373,12 +519,13 @@ void launch_cols(ColMode mode, const float2* src, float2* dst, int64_t srcTileSt
                  int ntiles, hipStream_t s) {
   int cg = col_group(g.dftH);
   size_t lds = sizeof(float2) * (size_t)g.dftH * cg;
-  dim3 grid((g.hw + cg - 1) / cg, ntiles, cnMax);
+  int nslabs = (g.hw + cg - 1) / cg;
+  dim3 grid(nslabs * ntiles * cnMax);
   dim3 block(256);
 #define CVM_COLS(CGV)                                                                                          \
   hipLaunchKernelGGL(k_cols<CGV>, grid, block, lds, s, (int)mode, src, dst, srcTileStride, dstTileStride,      \
                      chanStride, st, g, pt.triH, pt.brevH, pt.triH2, pt.brevH2, tspec, tspec2, tchanStride,    \
-                     tchanStride2, scale)
+                     tchanStride2, scale, ntiles, nslabs)
   switch (cg) {
     case 16: CVM_COLS(16); break;
     case 8: CVM_COLS(8); break;
@@ -411,77 +558,103 @@ void launch_shrink(const float2* tspec, float2* tspec2, int cnMax, int rows2, in
 
 // ------------------------------------------------------ inverse row FFTs --
 
+constexpr int ROWINV_MAXI = 16;  // dftW / threads: 4096/256 and 1024/64
+
 // inverseRowPair impl.go:394-444 for every channel of tile t, rows r, r+1:
 // combine the two half spectra into one complex row, inverse transform,
 // accumulate the real/imaginary parts into result rows y0+r / y0+r+1 with
-// channel 0 stored and channels 1.. added in order.
+// channel 0 stored and channels 1.. added in order (in registers; each
+// thread owns the same columns for every channel).
 __global__ __launch_bounds__(256) void k_row_inv(const float2* __restrict__ spec, int64_t tileStride,
                                                  int64_t chanStride, const CallState* __restrict__ st, TileGrid g,
                                                  const float2* __restrict__ triW, const int32_t* __restrict__ brevW,
-                                                 float* __restrict__ corr, int rw) {
-  extern __shared__ float2 sm[];
-  const int n = g.dftW;
-  float2* z = sm;
-  float2* acc = sm + n;
-  const int t = blockIdx.y, r = 2 * blockIdx.x;
+                                                 float* __restrict__ corr, int rw, int cpg, int npairs) {
+  extern __shared__ float2 z[];
+  const int n = g.dftW, np = padded_len(n);
+  const int L = blockIdx.x;
+  const int rp = L % npairs, t = L / npairs;
+  const int r = 2 * rp;
   if (st->flat) return;
   int x0, y0, bw, bh;
   tile_rect(g, t, x0, y0, bw, bh);
   if (r >= bh) return;
   const int cn = st->cn, tid = threadIdx.x, nt = blockDim.x, hw = g.hw;
-  for (int k = 0; k < cn; k++) {
-    const float2* sa = spec + (int64_t)t * tileStride + (int64_t)k * chanStride + (int64_t)r * g.hwPad;
-    const float2* sb = sa + g.hwPad;
-    for (int q = tid; q < n; q += nt) {
-      float2 v;
-      if (q < hw) {
-        float2 A = sa[q], B = sb[q];
-        v = make_float2(A.x - B.y, A.y + B.x);
-      } else {
-        int m = n - q;
-        float2 A = sa[m], B = sb[m];
-        v = make_float2(A.x + B.y, B.x - A.y);
+  float2 acc[ROWINV_MAXI];
+#pragma unroll
+  for (int i = 0; i < ROWINV_MAXI; i++) acc[i] = make_float2(0.f, 0.f);
+  for (int k0 = 0; k0 < cn; k0 += cpg) {
+    const int nv = min(cpg, cn - k0);
+    for (int v = 0; v < nv; v++) {
+      const float2* sa = spec + (int64_t)t * tileStride + (int64_t)(k0 + v) * chanStride + (int64_t)r * g.hwPad;
+      const float2* sb = sa + g.hwPad;
+      float2* zv = z + v * np;
+      for (int q0 = tid; q0 < n; q0 += nt * 8) {
+        float2 A[8], B[8];
+#pragma unroll
+        for (int u = 0; u < 8; u++) {
+          int q = q0 + u * nt;
+          int m = q < hw ? q : n - q;
+          A[u] = B[u] = make_float2(0.f, 0.f);
+          if (q < n) {
+            A[u] = sa[m];
+            B[u] = sb[m];
+          }
+        }
+#pragma unroll
+        for (int u = 0; u < 8; u++) {
+          int q = q0 + u * nt;
+          if (q < n) {
+            float2 w = q < hw ? make_float2(A[u].x - B[u].y, A[u].y + B[u].x)
+                              : make_float2(A[u].x + B[u].y, B[u].x - A[u].y);
+            zv[padi(bitrev_idx(q, g.log2W))] = w;
+          }
+        }
       }
-      z[brevW[q]] = v;
     }
     __syncthreads();
-    cascade_row(z, n, g.log2W, triW, true);
-    for (int x = tid; x < bw; x += nt) {
-      float2 v = z[x];
-      if (k == 0) {
-        acc[x] = v;
-      } else {
-        float2 o = acc[x];
-        acc[x] = make_float2(o.x + v.x, o.y + v.y);
+    fft_lds<true, false>(z, nv, np, 1, g.log2W, triW, true);
+    for (int v = 0; v < nv; v++) {
+      const float2* zv = z + v * np;
+#pragma unroll
+      for (int i = 0; i < ROWINV_MAXI; i++) {
+        int x = tid + i * nt;
+        if (x < bw) {
+          float2 w = zv[padi(x)];
+          if (k0 + v == 0)
+            acc[i] = w;
+          else
+            acc[i] = make_float2(acc[i].x + w.x, acc[i].y + w.y);
+        }
       }
     }
     __syncthreads();
   }
   float* o1 = corr + (int64_t)(y0 + r) * rw + x0;
-  for (int x = tid; x < bw; x += nt) o1[x] = acc[x].x;
-  if (r + 1 < bh) {
-    float* o2 = o1 + rw;
-    for (int x = tid; x < bw; x += nt) o2[x] = acc[x].y;
+  float* o2 = o1 + rw;
+  const bool second = r + 1 < bh;
+#pragma unroll
+  for (int i = 0; i < ROWINV_MAXI; i++) {
+    int x = tid + i * nt;
+    if (x < bw) {
+      o1[x] = acc[i].x;
+      if (second) o2[x] = acc[i].y;
+    }
   }
 }
 
 void launch_row_inv(const float2* spec, int64_t tileStride, int64_t chanStride, const CallState* st, int cnMax,
                     const TileGrid& g, const PlanTables& pt, float* corr, int rw, int ntiles, hipStream_t s) {
-  (void)cnMax;
   int nt = g.dftW >= 1024 ? 256 : 64;
-  size_t lds = 2 * sizeof(float2) * (size_t)g.dftW;
-  int maxPairs = (g.blockH > g.extH ? g.blockH : g.extH) / 2 + 1;
-  // last-band tiles may be taller than blockH (fleet members); cover the
+  int cpg = row_cpg(g.dftW, cnMax);
+  size_t lds = sizeof(float2) * (size_t)padded_len(g.dftW) * cpg;
+  // last-band tiles may be taller than blockH (fleet members): cover the
   // largest bh any tile can have
   int bhMax = g.blockH;
-  {
-    int y0last = (g.nty - 1) * g.blockH;
-    int bhl = g.extH - y0last;
-    if (bhl > bhMax) bhMax = bhl;
-  }
-  maxPairs = (bhMax + 1) / 2;
-  hipLaunchKernelGGL(k_row_inv, dim3(maxPairs, ntiles), dim3(nt), lds, s, spec, tileStride, chanStride, st, g, pt.triW,
-                     pt.brevW, corr, rw);
+  int bhl = g.extH - (g.nty - 1) * g.blockH;
+  if (bhl > bhMax) bhMax = bhl;
+  int npairs = (bhMax + 1) / 2;
+  hipLaunchKernelGGL(k_row_inv, dim3(npairs * ntiles), dim3(nt), lds, s, spec, tileStride, chanStride, st, g, pt.triW,
+                     pt.brevW, corr, rw, cpg, npairs);
 }
 
 }  // namespace cvm

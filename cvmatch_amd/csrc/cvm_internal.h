@@ -126,9 +126,13 @@ void launch_state_finalize(CallState* st, int tw, int th, int cn_mode, hipStream
 
 // Exact-statistics normalisation + tail + per-workgroup extrema.
 // map may alias corr (in place) or be null (extrema only).
-int norm_partials_count(int rw, int rh);
-void launch_normalize(const uint8_t* img, int64_t istride, int iw, int step, int tw, int th, int rw, int rh,
-                      const CallState* st, const float* corr, float* map, Extrema* partials, hipStream_t s);
+// cnMax: the largest channel count the call can take (the device decides).
+int norm_partials_count(int rw, int rh, int tw, int th, int cnMax);
+// bytes of scratch launch_normalize needs (chunk partial column sums)
+size_t norm_scratch_bytes(int iw, int ih, int tw, int th, int cnMax);
+void launch_normalize(const uint8_t* img, int64_t istride, int iw, int ih, int step, int tw, int th, int rw, int rh,
+                      int cnMax, const CallState* st, const float* corr, float* map, Extrema* partials,
+                      void* scratch, hipStream_t s);
 // Final first-occurrence reduction into res6 (device) = {min_v, min_x, min_y, max_v, max_x, max_y}.
 void launch_reduce(const Extrema* partials, int nparts, const CallState* st, int rw, void* res6, float* map_if_flat,
                    int rh, hipStream_t s);

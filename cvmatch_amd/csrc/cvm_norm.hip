@@ -14,6 +14,8 @@
 #include <float.h>
 #include <limits.h>
 
+#include <type_traits>
+
 #include "cvm_internal.h"
 
 namespace cvm {
@@ -42,20 +44,22 @@ __device__ __forceinline__ unsigned long long wave_sum_u64(unsigned long long v)
 }
 
 // templStats impl.go:1063-1079 (sums and square sums per channel, exact).
+// grid.y walks template rows, threads walk columns; no divisions.
 __global__ __launch_bounds__(256) void k_tpl_stats(const uint8_t* __restrict__ tpl, int64_t tstride, int tw, int th,
                                                    int cnMax, int step, CallState* st) {
   __shared__ unsigned long long red[4][8];
   unsigned long long s[4] = {0, 0, 0, 0}, q[4] = {0, 0, 0, 0};
-  const int64_t npx = (int64_t)tw * th;
-  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < npx; i += (int64_t)gridDim.x * blockDim.x) {
-    int y = (int)(i / tw), x = (int)(i - (int64_t)y * tw);
-    const uint8_t* p = tpl + (int64_t)y * tstride + (int64_t)x * step;
-    for (int k = 0; k < 4; k++)
-      if (k < cnMax) {
-        unsigned long long v = p[k];
-        s[k] += v;
-        q[k] += v * v;
-      }
+  for (int y = blockIdx.y; y < th; y += gridDim.y) {
+    const uint8_t* row = tpl + (int64_t)y * tstride;
+    for (int x = blockIdx.x * blockDim.x + threadIdx.x; x < tw; x += gridDim.x * blockDim.x) {
+      const uint8_t* p = row + (int64_t)x * step;
+      for (int k = 0; k < 4; k++)
+        if (k < cnMax) {
+          unsigned long long v = p[k];
+          s[k] += v;
+          q[k] += v * v;
+        }
+    }
   }
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   for (int k = 0; k < 4; k++) {
@@ -80,32 +84,53 @@ __global__ __launch_bounds__(256) void k_tpl_stats(const uint8_t* __restrict__ t
 
 void launch_tpl_stats(const uint8_t* tpl, int64_t tstride, int tw, int th, int cnMax, int step, CallState* st,
                       hipStream_t s) {
-  int64_t npx = (int64_t)tw * th;
-  int blocks = (int)((npx + 256 * 16 - 1) / (256 * 16));
-  if (blocks < 1) blocks = 1;
-  if (blocks > 1024) blocks = 1024;
-  hipLaunchKernelGGL(k_tpl_stats, dim3(blocks), dim3(256), 0, s, tpl, tstride, tw, th, cnMax, step, st);
+  int gx = (tw + 255) / 256;
+  if (gx > 16) gx = 16;
+  int gy = th < 256 ? th : 256;
+  hipLaunchKernelGGL(k_tpl_stats, dim3(gx, gy), dim3(256), 0, s, tpl, tstride, tw, th, cnMax, step, st);
 }
 
 // alphaConst cvmatch.go:146-168: every alpha byte equals the first one.
-__global__ __launch_bounds__(256) void k_alpha_scan(const uint8_t* __restrict__ pix, int64_t stride, int w, int h,
-                                                    CallState* st, int which) {
-  const uint8_t a0 = pix[3];
+// Like the reference's scan it stops at the first mismatch: workgroups poll
+// the flag between rows and leave once any workgroup has published one (the
+// answer is then "not constant" whatever the rest of the image holds).
+// grid.y walks rows, grid.x x threads cover a row in 16-byte groups of four
+// pixels when rows are 16-byte aligned, else in single pixels.
+__global__ __launch_bounds__(256) void k_alpha_scan(const uint8_t* __restrict__ pix, int64_t stride, int w, int y_begin,
+                                                    int h, CallState* st, int which, int vec) {
+  const uint32_t a0 = pix[3];
+  volatile unsigned int* flag = &st->alpha_bad[which];
   int bad = 0;
-  const int64_t npx = (int64_t)w * h;
-  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < npx; i += (int64_t)gridDim.x * blockDim.x) {
-    int y = (int)(i / w), x = (int)(i - (int64_t)y * w);
-    if (pix[(int64_t)y * stride + (int64_t)x * 4 + 3] != a0) bad = 1;
+  const int w4 = vec ? (w >> 2) : 0;
+  const uint32_t want = a0 * 0x01010101u;
+  for (int y = y_begin + blockIdx.y; y < h; y += gridDim.y) {
+    if (*flag) break;
+    const uint8_t* row = pix + (int64_t)y * stride;
+    for (int q = blockIdx.x * blockDim.x + threadIdx.x; q < w4; q += gridDim.x * blockDim.x) {
+      uint4 v = *(const uint4*)(row + (int64_t)q * 16);
+      uint32_t al = (v.x >> 24) | ((v.y >> 24) << 8) | ((v.z >> 24) << 16) | ((v.w >> 24) << 24);
+      if (al != want) bad = 1;
+    }
+    for (int x = 4 * w4 + blockIdx.x * blockDim.x + threadIdx.x; x < w; x += gridDim.x * blockDim.x)
+      if (row[(int64_t)x * 4 + 3] != a0) bad = 1;
+    if (bad) break;
   }
   if (__syncthreads_or(bad) && threadIdx.x == 0) atomicOr(&st->alpha_bad[which], 1u);
 }
 
 void launch_alpha_scan(const uint8_t* pix, int64_t stride, int w, int h, CallState* st, int which, hipStream_t s) {
-  int64_t npx = (int64_t)w * h;
-  int blocks = (int)((npx + 256 * 8 - 1) / (256 * 8));
-  if (blocks < 1) blocks = 1;
-  if (blocks > 2048) blocks = 2048;
-  hipLaunchKernelGGL(k_alpha_scan, dim3(blocks), dim3(256), 0, s, pix, stride, w, h, st, which);
+  int vec = ((((uintptr_t)pix) | (uintptr_t)stride) & 15) == 0;
+  int per_row = vec ? (w / 4 + 1) : w;
+  int gx = (per_row + 255) / 256;
+  if (gx > 16) gx = 16;
+  // the first rows alone first: a varying alpha plane shows up there and
+  // the full scan's workgroups then leave at their first flag check
+  int h0 = h < 8 ? h : 8;
+  hipLaunchKernelGGL(k_alpha_scan, dim3(gx, h0), dim3(256), 0, s, pix, stride, w, 0, h0, st, which, vec);
+  if (h > h0) {
+    int gy = (h - h0) < 512 ? (h - h0) : 512;
+    hipLaunchKernelGGL(k_alpha_scan, dim3(gx, gy), dim3(256), 0, s, pix, stride, w, h0, h, st, which, vec);
+  }
 }
 
 // cn choice (Match cvmatch.go:82-85), varSum, flat test (impl.go:1137),
@@ -135,21 +160,20 @@ void launch_state_finalize(CallState* st, int tw, int th, int cn_mode, hipStream
 
 // rgbToGray cvmatch.go:222-240: (4899 R + 9617 G + 1868 B + 8192) >> 14.
 __global__ void k_rgba_to_gray(const uint8_t* __restrict__ pix, int64_t stride, int w, int h, uint8_t* __restrict__ out) {
-  const int64_t npx = (int64_t)w * h;
-  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < npx; i += (int64_t)gridDim.x * blockDim.x) {
-    int y = (int)(i / w), x = (int)(i - (int64_t)y * w);
-    const uint8_t* p = pix + (int64_t)y * stride + (int64_t)x * 4;
-    uint32_t r = p[0], g = p[1], b = p[2];
-    out[i] = (uint8_t)((4899u * r + 9617u * g + 1868u * b + 8192u) >> 14);
+  for (int y = blockIdx.y; y < h; y += gridDim.y) {
+    const uint8_t* row = pix + (int64_t)y * stride;
+    for (int x = blockIdx.x * blockDim.x + threadIdx.x; x < w; x += gridDim.x * blockDim.x) {
+      uint32_t px = row[(int64_t)x * 4], g = row[(int64_t)x * 4 + 1], b = row[(int64_t)x * 4 + 2];
+      out[(int64_t)y * w + x] = (uint8_t)((4899u * px + 9617u * g + 1868u * b + 8192u) >> 14);
+    }
   }
 }
 
 void launch_rgba_to_gray(const uint8_t* pix, int64_t stride, int w, int h, uint8_t* out, hipStream_t s) {
-  int64_t npx = (int64_t)w * h;
-  int blocks = (int)((npx + 255) / 256);
-  if (blocks > 8192) blocks = 8192;
-  if (blocks < 1) blocks = 1;
-  hipLaunchKernelGGL(k_rgba_to_gray, dim3(blocks), dim3(256), 0, s, pix, stride, w, h, out);
+  int gx = (w + 255) / 256;
+  if (gx > 16) gx = 16;
+  int gy = h < 1024 ? h : 1024;
+  hipLaunchKernelGGL(k_rgba_to_gray, dim3(gx, gy), dim3(256), 0, s, pix, stride, w, h, out);
 }
 
 // --------------------------------------------------------- normalisation --
@@ -189,9 +213,52 @@ static int norm_band_rows(int rh) {
   return 64;
 }
 
-int norm_partials_count(int rw, int rh) {
+// Wave form (k_norm_wave below): one 64-lane workgroup per 64*Q result
+// columns x NB_ROWS rows.  It is used when every exact statistic of the
+// window fits the 53-bit significand (cnMax * 65025 * area^2 < 2^53, any
+// template up to 431x431 at four channels): the cross term, the sum of
+// squared window sums and area * s2 are then evaluated in binary64 with no
+// rounding at all, exactly the integers the int64 form computes.  Larger
+// templates take the 256-thread int64 form (k_normalize).
+constexpr int NB_ROWS = 16;
+
+struct NormPlan {
+  int wave;  // 0: int64 form
+  int q;     // result columns per lane (the strip is 64*q columns)
+  int sq32;  // window square sums fit uint32
+};
+
+// Each lane of the wave form owns 8 consecutive image columns of the strip
+// (64*q + tw - 1 <= 512 columns), loaded with 16-byte vector loads.
+constexpr int NORM_CPT = 8;
+
+static NormPlan norm_plan(int tw, int th, int cnMax) {
+  NormPlan p{0, 0, 0};
+  double area = (double)tw * th;
+  if ((double)cnMax * 65025.0 * area * area >= 9007199254740992.0) return p;
+  if (th > 257) return p;  // packed 16-bit channel column sums: 255 * th < 2^16
+  for (int q : {4, 2}) {
+    if (64 * q + tw - 1 <= 64 * NORM_CPT) {
+      p.wave = 1;
+      p.q = q;
+      p.sq32 = (double)cnMax * 65025.0 * area < 4294967296.0;
+      return p;
+    }
+  }
+  return p;
+}
+
+int norm_partials_count(int rw, int rh, int tw, int th, int cnMax) {
+  NormPlan np = norm_plan(tw, th, cnMax);
+  if (np.wave) return ((rw + 64 * np.q - 1) / (64 * np.q)) * ((rh + NB_ROWS - 1) / NB_ROWS);
   int br = norm_band_rows(rh);
   return ((rw + NORM_NX - 1) / NORM_NX) * ((rh + br - 1) / br);
+}
+
+size_t norm_scratch_bytes(int iw, int ih, int tw, int th, int cnMax) {
+  if (!norm_plan(tw, th, cnMax).wave) return 0;
+  size_t nch = (size_t)(ih + NB_ROWS - 1) / NB_ROWS;
+  return nch * (size_t)iw * 5 * sizeof(uint32_t);
 }
 
 // One workgroup: result columns [xs, xs+256) x rows [ys, ye).  Each thread
@@ -394,8 +461,335 @@ __global__ __launch_bounds__(256) void k_normalize(const uint8_t* __restrict__ i
   }
 }
 
-void launch_normalize(const uint8_t* img, int64_t istride, int iw, int step, int tw, int th, int rw, int rh,
-                      const CallState* st, const float* corr, float* map, Extrema* partials, hipStream_t s) {
+// ------------------------------------------------- normalisation, wave form --
+//
+// One wave (a 64-thread workgroup) owns result columns [xs, xs + 64*Q) and
+// rows [ys, ys + NB_ROWS).  Lane l keeps the exact column sums of CPT
+// consecutive image columns of the strip over the current th window rows.
+// The sums at the band start come from per-chunk partial column sums of
+// NB_ROWS rows (k_colsum_chunks reads every pixel once) plus th % NB_ROWS
+// rows read directly; the window then slides down one row per step, the
+// next slide's pixels being loaded before the current row's scan and tail.
+// Horizontal window sums are differences of a wave-level prefix scan in LDS.
+
+// Channel-byte mask of a packed pixel word: bytes of channels >= cn cleared.
+__device__ __forceinline__ uint32_t chan_mask(int cn) { return cn >= 4 ? 0xFFFFFFFFu : ((1u << (8 * cn)) - 1u); }
+
+__global__ __launch_bounds__(256) void k_colsum_chunks(const uint8_t* __restrict__ img, int64_t istride, int iw,
+                                                       int ih, int step, int words, const CallState* __restrict__ st,
+                                                       uint32_t* __restrict__ ps, uint32_t* __restrict__ pq) {
+  const int x = blockIdx.x * blockDim.x + threadIdx.x, c = blockIdx.y;
+  if (x >= iw || st->flat) return;
+  const int cn = st->cn;
+  const uint32_t m = chan_mask(cn);
+  const int y0 = c * NB_ROWS, y1 = min(ih, y0 + NB_ROWS);
+  uint32_t s02 = 0, s13 = 0, q = 0;
+  for (int y = y0; y < y1; y++) {
+    const uint8_t* p = img + (int64_t)y * istride + (int64_t)x * step;
+    uint32_t w;
+    if (words) {
+      w = *(const uint32_t*)p & m;
+    } else {
+      w = 0;
+      for (int k = 0; k < 4; k++)
+        if (k < cn) w |= (uint32_t)p[k] << (8 * k);
+    }
+    s02 += w & 0x00FF00FFu;
+    s13 += (w >> 8) & 0x00FF00FFu;
+    q = __builtin_amdgcn_udot4(w, w, q, false);
+  }
+  const uint32_t sv[4] = {s02 & 0xFFFFu, s13 & 0xFFFFu, s02 >> 16, s13 >> 16};
+  for (int k = 0; k < 4; k++)
+    if (k < cn) ps[((int64_t)c * 4 + k) * iw + x] = sv[k];
+  pq[(int64_t)c * iw + x] = q;
+}
+
+// Eight consecutive pixels x0..x0+7 of one row as packed channel words
+// (channel k in bits 8k..).  vmode 1: RGBA rows 16-byte aligned, two 16-byte
+// loads; vmode 2: gray rows 8-byte aligned, one 8-byte load; otherwise, or
+// when fewer than 8 columns are valid, byte loads of the valid columns.
+__device__ __forceinline__ void load_px8(const uint8_t* row, int x0, int step, int cn, int vmode, int nvalid,
+                                         uint32_t out[NORM_CPT]) {
+  if (nvalid >= NORM_CPT && vmode == 1) {
+    const uint4* p = (const uint4*)(row + (int64_t)x0 * 4);
+    uint4 a = p[0], b = p[1];
+    out[0] = a.x; out[1] = a.y; out[2] = a.z; out[3] = a.w;
+    out[4] = b.x; out[5] = b.y; out[6] = b.z; out[7] = b.w;
+    return;
+  }
+  if (nvalid >= NORM_CPT && vmode == 2) {
+    uint2 v = *(const uint2*)(row + x0);
+    for (int j = 0; j < 4; j++) {
+      out[j] = (v.x >> (8 * j)) & 255u;
+      out[4 + j] = (v.y >> (8 * j)) & 255u;
+    }
+    return;
+  }
+  for (int j = 0; j < NORM_CPT; j++) {
+    uint32_t w = 0;
+    if (j < nvalid) {
+      const uint8_t* p = row + (int64_t)(x0 + j) * step;
+      for (int k = 0; k < 4; k++)
+        if (k < cn) w |= (uint32_t)p[k] << (8 * k);
+    }
+    out[j] = w;
+  }
+}
+
+// Eight consecutive uint32 partials (16-byte loads when aligned).
+__device__ __forceinline__ void load_u32x8(const uint32_t* p, bool vec, int nvalid, uint32_t out[NORM_CPT]) {
+  if (vec && nvalid >= NORM_CPT) {
+    uint4 a = ((const uint4*)p)[0], b = ((const uint4*)p)[1];
+    out[0] = a.x; out[1] = a.y; out[2] = a.z; out[3] = a.w;
+    out[4] = b.x; out[5] = b.y; out[6] = b.z; out[7] = b.w;
+    return;
+  }
+  for (int j = 0; j < NORM_CPT; j++) out[j] = j < nvalid ? p[j] : 0u;
+}
+
+template <int Q, bool SQ32>
+__global__ __launch_bounds__(64) void k_norm_wave(const uint8_t* __restrict__ img, int64_t istride, int iw, int step,
+                                                  int vmode, int tw, int th, int rw, int rh,
+                                                  const CallState* __restrict__ st, const uint32_t* __restrict__ ps,
+                                                  const uint32_t* __restrict__ pq, const float* corr, float* map,
+                                                  Extrema* __restrict__ parts) {
+  typedef typename std::conditional<SQ32, uint32_t, unsigned long long>::type sq_t;
+  constexpr int CPT = NORM_CPT;
+  constexpr int NW = 64 * CPT;
+  constexpr int SX = 64 * Q;
+  __shared__ uint32_t P[4][NW + 1];
+  __shared__ sq_t P2[NW + 1];
+  const int lane = threadIdx.x;
+  const int xs = blockIdx.x * SX, ys = blockIdx.y * NB_ROWS;
+  const int ye = min(rh, ys + NB_ROWS);
+  const int cn = st->cn;
+  const bool flat = st->flat != 0;
+  const int W = min(SX + tw - 1, iw - xs);
+  const double aread = (double)tw * (double)th;
+  const float numScale = st->numScale, varScale = st->varScale, templNorm = st->templNorm;
+  double tsd[4];
+  for (int k = 0; k < 4; k++) tsd[k] = (double)st->tsum[k];
+  const int cbase = lane * CPT;
+  const int x0 = xs + cbase;
+  const int nvalid = min(CPT, W - cbase);  // may be <= 0
+  const bool pvec = (iw & 3) == 0;
+
+  // column sums over the th window rows, channels packed in 16-bit halves:
+  // cs02 = ch0 | ch2 << 16, cs13 = ch1 | ch3 << 16 (each <= 255 * th < 2^16)
+  const uint32_t cm = chan_mask(cn);
+  uint32_t cs02[CPT], cs13[CPT];
+  sq_t c2[CPT];
+  for (int j = 0; j < CPT; j++) {
+    cs02[j] = cs13[j] = 0;
+    c2[j] = 0;
+  }
+  if (!flat && nvalid > 0) {
+    const int nfull = th / NB_ROWS;
+    for (int i = 0; i < nfull; i++) {
+      int ch = blockIdx.y + i;
+      uint32_t t[CPT];
+      for (int k = 0; k < 4; k++)
+        if (k < cn) {
+          load_u32x8(ps + ((int64_t)ch * 4 + k) * iw + x0, pvec, nvalid, t);
+          for (int j = 0; j < CPT; j++) {
+            uint32_t sh = (k >> 1) * 16;
+            if (k & 1)
+              cs13[j] += t[j] << sh;
+            else
+              cs02[j] += t[j] << sh;
+          }
+        }
+      load_u32x8(pq + (int64_t)ch * iw + x0, pvec, nvalid, t);
+      for (int j = 0; j < CPT; j++) c2[j] += t[j];
+    }
+    for (int y = ys + nfull * NB_ROWS; y < ys + th; y++) {
+      uint32_t w[CPT];
+      load_px8(img + (int64_t)y * istride, x0, step, cn, vmode, nvalid, w);
+      for (int j = 0; j < CPT; j++) {
+        uint32_t v = w[j] & cm;
+        cs02[j] += v & 0x00FF00FFu;
+        cs13[j] += (v >> 8) & 0x00FF00FFu;
+        c2[j] += (sq_t)__builtin_amdgcn_udot4(v, v, 0u, false);
+      }
+    }
+  }
+
+  float bmin = FLT_MAX, bmax = -FLT_MAX;
+  int bmin_i = INT_MAX, bmax_i = INT_MAX;
+
+  // Row y's slide pixels (rows y+th and y) and corr values are loaded one
+  // row ahead, so their latency overlaps the previous row's scan and tail.
+  uint32_t na[CPT], nb[CPT];
+  float ncq[Q];
+  auto fetch = [&](int yy) {
+    for (int j = 0; j < CPT; j++) na[j] = nb[j] = 0;
+    if (!flat && yy + 1 < ye && nvalid > 0) {
+      load_px8(img + (int64_t)(yy + th) * istride, x0, step, cn, vmode, nvalid, na);
+      load_px8(img + (int64_t)yy * istride, x0, step, cn, vmode, nvalid, nb);
+    }
+    for (int q = 0; q < Q; q++) {
+      int x = xs + lane + 64 * q;
+      ncq[q] = (!flat && x < rw && yy < ye) ? corr[(int64_t)yy * rw + x] : 0.f;
+    }
+  };
+  fetch(ys);
+  for (int y = ys; y < ye; y++) {
+    const bool slide = !flat && (y + 1 < ye);
+    uint32_t va[CPT], vb[CPT];
+    float cq[Q];
+    for (int j = 0; j < CPT; j++) {
+      va[j] = na[j];
+      vb[j] = nb[j];
+    }
+    for (int q = 0; q < Q; q++) cq[q] = ncq[q];
+    if (y + 1 < ye) fetch(y + 1);
+    float vq[Q];
+    for (int q = 0; q < Q; q++) vq[q] = 1.0f;
+    if (!flat) {
+      uint32_t run[4] = {0, 0, 0, 0};
+      sq_t run2 = 0;
+      uint32_t loc[CPT][4];
+      sq_t loc2[CPT];
+      for (int j = 0; j < CPT; j++) {
+        const uint32_t cv[4] = {cs02[j] & 0xFFFFu, cs13[j] & 0xFFFFu, cs02[j] >> 16, cs13[j] >> 16};
+        for (int k = 0; k < 4; k++) {
+          run[k] += cv[k];
+          loc[j][k] = run[k];
+        }
+        run2 += c2[j];
+        loc2[j] = run2;
+      }
+      uint32_t inc[4] = {run[0], run[1], run[2], run[3]};
+      sq_t inc2 = run2;
+      for (int o = 1; o < 64; o <<= 1) {
+        uint32_t up[4];
+        for (int k = 0; k < 4; k++)
+          if (k < cn) up[k] = __shfl_up(inc[k], o, 64);
+        sq_t up2 = __shfl_up(inc2, o, 64);
+        if (lane >= o) {
+          for (int k = 0; k < 4; k++)
+            if (k < cn) inc[k] += up[k];
+          inc2 += up2;
+        }
+      }
+      for (int j = 0; j < CPT; j++) {
+        for (int k = 0; k < 4; k++)
+          if (k < cn) P[k][cbase + j + 1] = inc[k] - run[k] + loc[j][k];
+        P2[cbase + j + 1] = inc2 - run2 + loc2[j];
+      }
+      if (lane == 0) {
+        for (int k = 0; k < 4; k++) P[k][0] = 0;
+        P2[0] = 0;
+      }
+      __syncthreads();
+      for (int q = 0; q < Q; q++) {
+        int xl = lane + 64 * q, x = xs + xl;
+        if (x < rw) {
+          // exact integers carried in binary64 (all below 2^53)
+          double sd[4];
+          for (int k = 0; k < 4; k++) sd[k] = (k < cn) ? (double)(uint32_t)(P[k][xl + tw] - P[k][xl]) : 0.0;
+          double s2 = (double)(sq_t)(P2[xl + tw] - P2[xl]);
+          double lane0, sq;
+          if (cn == 1) {
+            lane0 = sd[0];
+            sq = sd[0] * sd[0];
+          } else {
+            lane0 = sd[0] * tsd[0];
+            sq = sd[0] * sd[0];
+            for (int k = 1; k < 4; k++)
+              if (k < cn) {
+                lane0 = lane0 + sd[k] * tsd[k];
+                sq = sq + sd[k] * sd[k];
+              }
+          }
+          double idiff = aread * s2 - sq;
+          vq[q] = norm_one(cq[q], (float)lane0, (float)idiff, (float)s2, numScale, varScale, templNorm);
+        }
+      }
+      __syncthreads();
+      if (slide) {
+        // the leaving row is inside the current sums, so each 16-bit half
+        // stays in [0, 2^16) through the subtraction: no borrow crosses halves
+        for (int j = 0; j < CPT; j++) {
+          uint32_t a = va[j] & cm, b = vb[j] & cm;
+          cs02[j] = cs02[j] - (b & 0x00FF00FFu) + (a & 0x00FF00FFu);
+          cs13[j] = cs13[j] - ((b >> 8) & 0x00FF00FFu) + ((a >> 8) & 0x00FF00FFu);
+          c2[j] += (sq_t)__builtin_amdgcn_udot4(a, a, 0u, false) - (sq_t)__builtin_amdgcn_udot4(b, b, 0u, false);
+        }
+      }
+    }
+    for (int q = 0; q < Q; q++) {
+      int x = xs + lane + 64 * q;
+      if (x < rw) {
+        int idx = y * rw + x;
+        if (map) map[(int64_t)y * rw + x] = vq[q];
+        if (better_min(vq[q], idx, bmin, bmin_i)) {
+          bmin = vq[q];
+          bmin_i = idx;
+        }
+        if (better_max(vq[q], idx, bmax, bmax_i)) {
+          bmax = vq[q];
+          bmax_i = idx;
+        }
+      }
+    }
+  }
+  for (int o = 32; o > 0; o >>= 1) {
+    float ov = __shfl_xor(bmin, o, 64);
+    int oi = __shfl_xor(bmin_i, o, 64);
+    if (better_min(ov, oi, bmin, bmin_i)) {
+      bmin = ov;
+      bmin_i = oi;
+    }
+    ov = __shfl_xor(bmax, o, 64);
+    oi = __shfl_xor(bmax_i, o, 64);
+    if (better_max(ov, oi, bmax, bmax_i)) {
+      bmax = ov;
+      bmax_i = oi;
+    }
+  }
+  if (lane == 0) {
+    Extrema e;
+    e.min_v = bmin;
+    e.min_i = bmin_i;
+    e.max_v = bmax;
+    e.max_i = bmax_i;
+    parts[blockIdx.y * gridDim.x + blockIdx.x] = e;
+  }
+}
+
+void launch_normalize(const uint8_t* img, int64_t istride, int iw, int ih, int step, int tw, int th, int rw, int rh,
+                      int cnMax, const CallState* st, const float* corr, float* map, Extrema* partials,
+                      void* scratch, hipStream_t s) {
+  NormPlan np = norm_plan(tw, th, cnMax);
+  if (np.wave && scratch) {
+    int nch = (ih + NB_ROWS - 1) / NB_ROWS;
+    uint32_t* ps = (uint32_t*)scratch;
+    uint32_t* pq = ps + (size_t)nch * iw * 4;
+    int words = (step == 4 && ((((uintptr_t)img) | (uintptr_t)istride) & 3) == 0) ? 1 : 0;
+    hipLaunchKernelGGL(k_colsum_chunks, dim3((iw + 255) / 256, nch), dim3(256), 0, s, img, istride, iw, ih, step,
+                       words, st, ps, pq);
+    int vmode = 0;
+    if (step == 4 && ((((uintptr_t)img) | (uintptr_t)istride) & 15) == 0) vmode = 1;
+    if (step == 1 && ((((uintptr_t)img) | (uintptr_t)istride) & 7) == 0) vmode = 2;
+    dim3 grid((rw + 64 * np.q - 1) / (64 * np.q), (rh + NB_ROWS - 1) / NB_ROWS);
+#define CVM_NW(Q, S)                                                                                         \
+  hipLaunchKernelGGL((k_norm_wave<Q, S>), grid, dim3(64), 0, s, img, istride, iw, step, vmode, tw, th, rw, rh, \
+                     st, ps, pq, corr, map, partials)
+    if (np.q == 4) {
+      if (np.sq32)
+        CVM_NW(4, true);
+      else
+        CVM_NW(4, false);
+    } else {
+      if (np.sq32)
+        CVM_NW(2, true);
+      else
+        CVM_NW(2, false);
+    }
+#undef CVM_NW
+    return;
+  }
   int br = norm_band_rows(rh);
   dim3 grid((rw + NORM_NX - 1) / NORM_NX, (rh + br - 1) / br);
   int W = NORM_NX + tw - 1;

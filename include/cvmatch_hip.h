@@ -135,6 +135,9 @@ int cvm_device_free(void *p);
 int cvm_copy_h2d(void *dst, const void *src, size_t bytes);
 int cvm_copy_d2h(void *dst, const void *src, size_t bytes);
 int cvm_synchronize(void);
+/* Device-to-device copy by a plain streaming kernel; also the known-size
+ * kernel that calibrates the HBM traffic counters. */
+int cvm_device_copy(void *dst, const void *src, size_t bytes);
 
 /* ------------------------------------------------------------ profiling -- */
 /* No reference counterpart.  When enabled, every pipeline stage is bracketed

@@ -309,3 +309,18 @@ def test_repeat_determinism(cvm):
     for _ in range(3):
         b, _, _ = cvm.match_map(parent, sub)
         assert np.array_equal(bits(a), bits(b))
+
+
+@pytest.mark.parametrize("w,h", [(37, 21), (64, 40), (1001, 13)])
+def test_alpha_scan_positions(cvm, orc, w, h):
+    base = np.zeros((h, w, 4), np.uint8)
+    base[..., 3] = 77
+    assert cvm.alpha_const(base) and orc.alpha_const(base)
+    for (y, x) in [(0, 0), (0, w - 1), (7, 3), (8, 0), (h - 1, w - 1), (h // 2, w // 2)]:
+        img = base.copy()
+        img[y, x, 3] = 78
+        assert cvm.alpha_const(img) is False and orc.alpha_const(img) is False, (y, x)
+        # unaligned row stride: a view with one extra pixel per row
+        padded = np.zeros((h, w + 1, 4), np.uint8)
+        padded[:, :w] = img
+        assert cvm.alpha_const(padded[:, :w]) is False

@@ -80,41 +80,54 @@ __device__ __forceinline__ int xcd_remap(int p, int g) {
 
 // ---------------------------------------------------------- LDS transform --
 //
-// nvec independent n-point vectors live in LDS: element i of vector v at
-// a[v * vstride + (PAD ? padi(i) : i) * es].  fft_lds runs the whole radix-4
-// cascade (fftR4Scalar radix4.go:106-149) on all of them, input already in
-// bit-reversed order.  A pass covers index bits [lo, lo+NB): the radix-2
-// head (bit 0, HEAD = 1) and NST radix-4 stages whose digit sits at bits
-// [lo+HEAD+2s, lo+HEAD+2s+2); stage s has quarter size h = 2^(lo+HEAD+2s)
-// and its twiddle index j = index mod h.
-template <int HEAD, int NST, bool PAD, bool VFAST>
-__device__ __forceinline__ void fft_pass(float2* a, int nvec, int lvec, int vstride, int es, int log2n, int lo,
-                                         const float2* __restrict__ tri0, const float2* __restrict__ tri1,
-                                         bool inverse) {
+// NVEC independent 2^LOG2N-point vectors live in LDS: element i of vector v
+// at a[v * vstride + (PAD ? padi(i) : i) * ES].  fft_lds runs the whole
+// radix-4 cascade (fftR4Scalar radix4.go:106-149) on them, input already in
+// bit-reversed order.  All sizes are compile-time constants, so the pass
+// schedule, the group indices and the twiddle offsets fold away.
+//
+// A pass covers index bits [LO, LO+NB): the radix-2 head (bit 0, HEAD = 1)
+// and NST radix-4 stages whose digit sits at bits [LO+HEAD+2s, +2); stage s
+// has quarter size h = 2^(LO+HEAD+2s), twiddle index j = index mod h, and its
+// w1/w2/w3 blocks start at tri + (h - base) (base = 2 for odd log2 n, else 1:
+// the stages before it hold 3*(base + 4 base + ...) = h - base entries).
+
+__host__ __device__ constexpr int ceil_log2c(int n) { return n <= 1 ? 0 : 1 + ceil_log2c((n + 1) >> 1); }
+
+template <int LOG2N, int LO, int HEAD, int NST, int NVEC, int NT, int ES, bool PAD, bool VFAST>
+__device__ __forceinline__ void fft_pass(float2* a, int nvec, int vstride, const float2* __restrict__ tri0,
+                                         const float2* __restrict__ tri1, bool inverse) {
   constexpr int NB = HEAD + 2 * NST;
   constexpr int E = 1 << NB;
-  const int lgrp = log2n - NB;  // log2 of the groups per vector
-  const int total = 1 << (lgrp + lvec);
-  const int h0 = 1 << (lo + HEAD);
-  const int h1 = h0 << 2;
-  for (int gi = threadIdx.x; gi < total; gi += blockDim.x) {
+  constexpr int LGRP = LOG2N - NB;
+  constexpr int LVEC = ceil_log2c(NVEC);
+  constexpr int TOTAL = 1 << (LGRP + LVEC);
+  constexpr int ITERS = (TOTAL + NT - 1) / NT;
+  constexpr int h0 = 1 << (LO + HEAD);
+  constexpr int h1 = h0 << 2;
+#pragma unroll
+  for (int it = 0; it < ITERS; it++) {
+    const int gi = threadIdx.x + it * NT;
+    if ((TOTAL % NT) != 0 && gi >= TOTAL) break;
     int v, gg;
     if (VFAST) {
-      v = gi & ((1 << lvec) - 1);
-      gg = gi >> lvec;
+      v = gi & ((1 << LVEC) - 1);
+      gg = gi >> LVEC;
     } else {
-      gg = gi & ((1 << lgrp) - 1);
-      v = gi >> lgrp;
+      gg = gi & ((1 << LGRP) - 1);
+      v = gi >> LGRP;
     }
-    if (v >= nvec) continue;  // lvec rounds nvec up to a power of two
-    const int low = gg & ((1 << lo) - 1);
-    const int base = low | ((gg >> lo) << (lo + NB));
+    if (NVEC != (1 << LVEC) || !VFAST) {
+      if (v >= nvec) continue;
+    }
+    const int low = gg & ((1 << LO) - 1);
+    const int base = low | ((gg >> LO) << (LO + NB));
     float2* av = a + v * vstride;
     float2 r[E];
 #pragma unroll
     for (int e = 0; e < E; e++) {
-      int idx = base | (e << lo);
-      r[e] = av[(PAD ? padi(idx) : idx) * es];
+      int idx = base | (e << LO);
+      r[e] = av[(PAD ? padi(idx) : idx) * ES];
     }
     if (HEAD) {
 #pragma unroll
@@ -129,7 +142,7 @@ __device__ __forceinline__ void fft_pass(float2* a, int nvec, int lvec, int vstr
 #pragma unroll
       for (int e = 0; e < E; e++) {
         if ((e >> d) & 3) continue;
-        int j = low | ((e & ((1 << d) - 1)) << lo);
+        int j = low | ((e & ((1 << d) - 1)) << LO);
         float2 w1 = conj_if(tri0[j], inverse), w2 = conj_if(tri0[h0 + j], inverse),
                w3 = conj_if(tri0[2 * h0 + j], inverse);
         bfly4(r[e], r[e | (1 << d)], r[e | (2 << d)], r[e | (3 << d)], w1, w2, w3, inverse);
@@ -140,7 +153,7 @@ __device__ __forceinline__ void fft_pass(float2* a, int nvec, int lvec, int vstr
 #pragma unroll
       for (int e = 0; e < E; e++) {
         if ((e >> d) & 3) continue;
-        int j = low | ((e & ((1 << d) - 1)) << lo);
+        int j = low | ((e & ((1 << d) - 1)) << LO);
         float2 w1 = conj_if(tri1[j], inverse), w2 = conj_if(tri1[h1 + j], inverse),
                w3 = conj_if(tri1[2 * h1 + j], inverse);
         bfly4(r[e], r[e | (1 << d)], r[e | (2 << d)], r[e | (3 << d)], w1, w2, w3, inverse);
@@ -148,48 +161,49 @@ __device__ __forceinline__ void fft_pass(float2* a, int nvec, int lvec, int vstr
     }
 #pragma unroll
     for (int e = 0; e < E; e++) {
-      int idx = base | (e << lo);
-      av[(PAD ? padi(idx) : idx) * es] = r[e];
+      int idx = base | (e << LO);
+      av[(PAD ? padi(idx) : idx) * ES] = r[e];
     }
+  }
+}
+
+// Radix-4 passes from bit LO on, two stages per pass (one for the last odd
+// stage out), each followed by a barrier.
+template <int LOG2N, int LO, int NVEC, int NT, int ES, bool PAD, bool VFAST>
+__device__ __forceinline__ void fft_rest(float2* a, int nvec, int vstride, const float2* __restrict__ tri,
+                                         bool inverse) {
+  constexpr int base = (LOG2N & 1) ? 2 : 1;
+  constexpr int SREM = (LOG2N - LO) / 2;
+  if constexpr (SREM >= 2) {
+    constexpr int h0 = 1 << LO;
+    fft_pass<LOG2N, LO, 0, 2, NVEC, NT, ES, PAD, VFAST>(a, nvec, vstride, tri + (h0 - base), tri + (4 * h0 - base),
+                                                        inverse);
+    __syncthreads();
+    fft_rest<LOG2N, LO + 4, NVEC, NT, ES, PAD, VFAST>(a, nvec, vstride, tri, inverse);
+  } else if constexpr (SREM == 1) {
+    constexpr int h0 = 1 << LO;
+    fft_pass<LOG2N, LO, 0, 1, NVEC, NT, ES, PAD, VFAST>(a, nvec, vstride, tri + (h0 - base), tri, inverse);
+    __syncthreads();
   }
 }
 
 // The full cascade: the caller has stored the input and synchronised; on
 // return every vector holds its transform and the block is synchronised.
-// Passes hold at most 16 elements per thread: [head + 1 stage] when log2 n
-// is odd, then two radix-4 stages per pass.
-template <bool PAD, bool VFAST>
-__device__ void fft_lds(float2* a, int nvec, int vstride, int es, int log2n, const float2* __restrict__ tri,
-                        bool inverse) {
-  int lvec = 0;
-  while ((1 << lvec) < nvec) lvec++;
-  const int S = log2n >> 1;  // radix-4 stages
-  int s = 0, lo = 0;
-  const float2* t = tri;
-  if (log2n & 1) {
-    if (S >= 1) {
-      fft_pass<1, 1, PAD, VFAST>(a, nvec, lvec, vstride, es, log2n, 0, t, t, inverse);
-      t += 3 * 2;
-      s = 1;
-      lo = 3;
+template <int LOG2N, int NVEC, int NT, int ES, bool PAD, bool VFAST>
+__device__ __forceinline__ void fft_lds(float2* a, int nvec, int vstride, const float2* __restrict__ tri,
+                                        bool inverse) {
+  if constexpr (LOG2N & 1) {
+    if constexpr (LOG2N >= 3) {
+      // head + the h = 2 stage (twiddles at offset 0)
+      fft_pass<LOG2N, 0, 1, 1, NVEC, NT, ES, PAD, VFAST>(a, nvec, vstride, tri, tri, inverse);
+      __syncthreads();
+      fft_rest<LOG2N, 3, NVEC, NT, ES, PAD, VFAST>(a, nvec, vstride, tri, inverse);
     } else {
-      fft_pass<1, 0, PAD, VFAST>(a, nvec, lvec, vstride, es, log2n, 0, t, t, inverse);
-      lo = 1;
+      fft_pass<LOG2N, 0, 1, 0, NVEC, NT, ES, PAD, VFAST>(a, nvec, vstride, tri, tri, inverse);
+      __syncthreads();
     }
-    __syncthreads();
-  }
-  while (s < S) {
-    int nst = (S - s) < 2 ? (S - s) : 2;
-    int h0 = 1 << lo;
-    const float2* t1 = t + 3 * h0;
-    if (nst == 2)
-      fft_pass<0, 2, PAD, VFAST>(a, nvec, lvec, vstride, es, log2n, lo, t, t1, inverse);
-    else
-      fft_pass<0, 1, PAD, VFAST>(a, nvec, lvec, vstride, es, log2n, lo, t, t1, inverse);
-    __syncthreads();
-    for (int q = 0; q < nst; q++) t += 3 * (h0 << (2 * q));
-    s += nst;
-    lo += 2 * nst;
+  } else {
+    fft_rest<LOG2N, 0, NVEC, NT, ES, PAD, VFAST>(a, nvec, vstride, tri, inverse);
   }
 }
 
@@ -290,13 +304,15 @@ void launch_twiddles(float2* tri, int32_t* brev, int n, hipStream_t s) {
 
 // ------------------------------------------------------ forward row FFTs --
 
-// forwardRowPair impl.go:301-344: rows r, r+1 of channel k of tile t packed
-// as one complex row (zero padded to dftW), transformed, untangled into the
-// ------------------------------------------------------ forward row FFTs --
+// Threads for a row transform of 2^LOG2N points.
+template <int LOG2N>
+struct RowNT {
+  static constexpr int value = LOG2N >= 10 ? 256 : 64;
+};
 
 // Channels per workgroup for row transforms of length n: as many as fit a
 // 144 KiB LDS budget (gfx950 gives a workgroup up to 160 KiB), so one
-// workgroup transforms every channel of its row pair.
+// workgroup usually transforms every channel of its row pair.
 static int row_cpg(int n, int cnMax) {
   int cpg = (int)(147456 / ((size_t)padded_len(n) * sizeof(float2)));
   if (cpg > cnMax) cpg = cnMax;
@@ -307,12 +323,13 @@ static int row_cpg(int n, int cnMax) {
 // as one complex row (zero padded to dftW), transformed, untangled into the
 // two half spectra, stored at spectrum rows r and r+1 (natural order; the
 // column kernel applies the bit reversal when it loads).  One workgroup
-// handles up to cpg channels of one row pair.
-__global__ __launch_bounds__(256) void k_row_fwd(const uint8_t* __restrict__ img, int64_t istride, int step,
-                                                 const CallState* __restrict__ st, TileGrid g,
-                                                 const float2* __restrict__ triW, const int32_t* __restrict__ brevW,
-                                                 float2* __restrict__ spec, int64_t tileStride, int64_t chanStride,
-                                                 int cpg, int ngroups, int npairs) {
+// handles up to NVEC channels of one row pair.
+template <int LOG2N, int NVEC>
+__global__ __launch_bounds__(RowNT<LOG2N>::value) void k_row_fwd(
+    const uint8_t* __restrict__ img, int64_t istride, int step, const CallState* __restrict__ st, TileGrid g,
+    const float2* __restrict__ triW, float2* __restrict__ spec, int64_t tileStride, int64_t chanStride, int cpg,
+    int ngroups, int npairs) {
+  constexpr int n = 1 << LOG2N, NT = RowNT<LOG2N>::value, np = n + (n >> 5);
   extern __shared__ float2 z[];
   const int L = xcd_remap(blockIdx.x, gridDim.x);
   const int cgi = L % ngroups, rest = L / ngroups;
@@ -327,15 +344,15 @@ __global__ __launch_bounds__(256) void k_row_fwd(const uint8_t* __restrict__ img
   const int dftH = tile_shrunk(g, y0) ? g.dftH2 : g.dftH;
   const int r = 2 * rp;
   if (r >= loadH || r >= dftH) return;
-  const int n = g.dftW, np = padded_len(n), tid = threadIdx.x, nt = blockDim.x;
+  const int tid = threadIdx.x;
   const uint8_t* ra = img + (int64_t)(y0 + r) * istride + (int64_t)x0 * step + k0;
   const bool two = r + 1 < loadH;
   // all of this thread's pixels are loaded before any is stored
-  constexpr int MAXI = 16;  // n / threads: 4096 / 256, 1024 / 64
+  constexpr int MAXI = (n + NT - 1) / NT;
   uint32_t wa[MAXI], wb[MAXI];
 #pragma unroll
   for (int i = 0; i < MAXI; i++) {
-    int x = tid + i * nt;
+    int x = tid + i * NT;
     wa[i] = wb[i] = 0;
     if (x < loadW) {
       const uint8_t* pa = ra + (int64_t)x * step;
@@ -347,21 +364,26 @@ __global__ __launch_bounds__(256) void k_row_fwd(const uint8_t* __restrict__ img
   }
 #pragma unroll
   for (int i = 0; i < MAXI; i++) {
-    int x = tid + i * nt;
+    int x = tid + i * NT;
     if (x < n) {
-      const int pos = padi(bitrev_idx(x, g.log2W));
-      for (int v = 0; v < nv; v++)
-        z[v * np + pos] = make_float2((float)((wa[i] >> (8 * v)) & 255u), (float)((wb[i] >> (8 * v)) & 255u));
+      const int pos = padi(bitrev_idx(x, LOG2N));
+#pragma unroll
+      for (int v = 0; v < NVEC; v++)
+        if (v < nv)
+          z[v * np + pos] = make_float2((float)((wa[i] >> (8 * v)) & 255u), (float)((wb[i] >> (8 * v)) & 255u));
     }
   }
   __syncthreads();
-  fft_lds<true, false>(z, nv, np, 1, g.log2W, triW, false);
-  const int mask = n - 1;
-  for (int v = 0; v < nv; v++) {
+  fft_lds<LOG2N, NVEC, NT, 1, true, false>(z, nv, np, triW, false);
+  constexpr int mask = n - 1;
+  const int hw = g.hw;
+#pragma unroll
+  for (int v = 0; v < NVEC; v++) {
+    if (v >= nv) break;
     float2* sa = spec + (int64_t)t * tileStride + (int64_t)(k0 + v) * chanStride + (int64_t)r * g.hwPad;
     float2* sb = sa + g.hwPad;
     const float2* zv = z + v * np;
-    for (int q = tid; q < g.hw; q += nt) {
+    for (int q = tid; q < hw; q += NT) {
       float2 zk = zv[padi(q)], zn = zv[padi((n - q) & mask)];
       sa[q] = make_float2(0.5f * (zk.x + zn.x), 0.5f * (zk.y - zn.y));
       sb[q] = make_float2(0.5f * (zk.y + zn.y), 0.5f * (zn.x - zk.x));
@@ -369,19 +391,50 @@ __global__ __launch_bounds__(256) void k_row_fwd(const uint8_t* __restrict__ img
   }
 }
 
+template <int LOG2N>
+static void row_fwd_n(const uint8_t* img, int64_t istride, int step, int cnMax, const CallState* st,
+                      const TileGrid& g, const PlanTables& pt, float2* spec, int64_t tileStride, int64_t chanStride,
+                      int ntiles, hipStream_t s) {
+  constexpr int n = 1 << LOG2N;
+  int cpg = row_cpg(n, cnMax);
+  int ngroups = (cnMax + cpg - 1) / cpg;
+  int npairs = g.dftH / 2;
+  size_t lds = sizeof(float2) * (size_t)padded_len(n) * cpg;
+  dim3 grid(ngroups * npairs * ntiles), block(RowNT<LOG2N>::value);
+  if (cpg <= 1)
+    hipLaunchKernelGGL((k_row_fwd<LOG2N, 1>), grid, block, lds, s, img, istride, step, st, g, pt.triW, spec,
+                       tileStride, chanStride, cpg, ngroups, npairs);
+  else if (cpg == 2)
+    hipLaunchKernelGGL((k_row_fwd<LOG2N, 2>), grid, block, lds, s, img, istride, step, st, g, pt.triW, spec,
+                       tileStride, chanStride, cpg, ngroups, npairs);
+  else
+    hipLaunchKernelGGL((k_row_fwd<LOG2N, 4>), grid, block, lds, s, img, istride, step, st, g, pt.triW, spec,
+                       tileStride, chanStride, cpg, ngroups, npairs);
+}
+
 void launch_row_fwd(const uint8_t* img, int64_t istride, int step, int cnMax, const CallState* st,
                     const TileGrid& g, const PlanTables& pt, float2* spec, int64_t tileStride, int64_t chanStride,
                     int ntiles, hipStream_t s) {
-  int nt = g.dftW >= 1024 ? 256 : 64;
-  int cpg = row_cpg(g.dftW, cnMax);
-  int ngroups = (cnMax + cpg - 1) / cpg;
-  int npairs = g.dftH / 2;
-  size_t lds = sizeof(float2) * (size_t)padded_len(g.dftW) * cpg;
-  hipLaunchKernelGGL(k_row_fwd, dim3(ngroups * npairs * ntiles), dim3(nt), lds, s, img, istride, step, st, g, pt.triW,
-                     pt.brevW, spec, tileStride, chanStride, cpg, ngroups, npairs);
+#define CVM_RF(L) \
+  case L:         \
+    row_fwd_n<L>(img, istride, step, cnMax, st, g, pt, spec, tileStride, chanStride, ntiles, s); \
+    break
+  switch (g.log2W) {
+    CVM_RF(1); CVM_RF(2); CVM_RF(3); CVM_RF(4); CVM_RF(5); CVM_RF(6);
+    CVM_RF(7); CVM_RF(8); CVM_RF(9); CVM_RF(10); CVM_RF(11); CVM_RF(12);
+    default: break;
+  }
+#undef CVM_RF
 }
 
 // ----------------------------------------------------------- column pass --
+
+// Columns per workgroup for a column transform of 2^LOG2N points: the LDS
+// slab n x CG complex stays at 64 KiB.
+template <int LOG2N>
+struct ColCG {
+  static constexpr int value = LOG2N <= 9 ? 16 : (1 << (13 - LOG2N));
+};
 
 // One workgroup owns CG spectrum columns of one tile/channel for the whole
 // column height.  Modes:
@@ -395,31 +448,25 @@ void launch_row_fwd(const uint8_t* img, int64_t istride, int step, int cnMax, co
 // inverse cascade again sees bit-reversed input (the rmap = brev relabelling
 // of colsR4Go), and leaves spatial row y at LDS row y.  Logical workgroup
 // order is tile-fastest, so the tiles that read one template-spectrum slab
-// run together on one XCD.
-template <int CG>
+// run together on one XCD.  One launch covers tiles [t0, t0 + ntiles), all of
+// one column height 2^LOG2N.
+template <int LOG2N>
 __global__ __launch_bounds__(256) void k_cols(int mode, const float2* __restrict__ src, float2* __restrict__ dst,
                                               int64_t srcTileStride, int64_t dstTileStride, int64_t chanStride,
                                               const CallState* __restrict__ st, TileGrid g,
-                                              const float2* __restrict__ triH, const int32_t* __restrict__ brevH,
-                                              const float2* __restrict__ triH2, const int32_t* __restrict__ brevH2,
-                                              const float2* __restrict__ tspec, const float2* __restrict__ tspec2,
-                                              int64_t tchanStride, int64_t tchanStride2, float scale, int ntiles,
-                                              int nslabs) {
+                                              const float2* __restrict__ tri, const float2* __restrict__ tspec,
+                                              int64_t tchanStride, float scale, int t0, int ntiles, int nslabs) {
+  constexpr int n = 1 << LOG2N, CG = ColCG<LOG2N>::value, NT = 256, TOT = n * CG;
   extern __shared__ float2 a[];
   const int L = xcd_remap(blockIdx.x, gridDim.x);
-  const int t = L % ntiles, rest = L / ntiles;
+  const int t = t0 + L % ntiles, rest = L / ntiles;
   const int slab = rest % nslabs, k = rest / nslabs;
   const int c0 = slab * CG;
   if (st->flat || k >= st->cn || c0 >= g.hw) return;
   int x0, y0, bw, bh;
   tile_rect(g, t, x0, y0, bw, bh);
-  const bool shr = tile_shrunk(g, y0);
-  const int n = shr ? g.dftH2 : g.dftH;
-  const int log2n = shr ? g.log2H2 : g.log2H;
-  const float2* tri = shr ? triH2 : triH;
-  const int32_t* brev = shr ? brevH2 : brevH;
-  const float2* ts = shr ? tspec2 + (int64_t)k * tchanStride2 : tspec + (int64_t)k * tchanStride;
-  const int tid = threadIdx.x, nt = blockDim.x;
+  const float2* ts = tspec + (int64_t)k * tchanStride;
+  const int tid = threadIdx.x;
   const int hwPad = g.hwPad;
   const float2* sp = src + (int64_t)t * srcTileStride + (int64_t)k * chanStride + c0;
   float2* dp = dst + (int64_t)t * dstTileStride + (int64_t)k * chanStride + c0;
@@ -431,31 +478,33 @@ __global__ __launch_bounds__(256) void k_cols(int mode, const float2* __restrict
     if (loaded > n) loaded = n;
   }
   const int ncol = (g.hw - c0) < CG ? (g.hw - c0) : CG;
-  const int tot = n * CG;
-  for (int b0 = tid; b0 < tot; b0 += nt * 8) {
-    float2 v[8];
+  constexpr int LB = 8;  // loads in flight per thread
 #pragma unroll
-    for (int u = 0; u < 8; u++) {
-      int idx = b0 + u * nt;
+  for (int b0 = 0; b0 < TOT; b0 += NT * LB) {
+    float2 v[LB];
+#pragma unroll
+    for (int u = 0; u < LB; u++) {
+      int idx = b0 + u * NT + tid;
       int r = idx / CG, c = idx % CG;
       v[u] = make_float2(0.f, 0.f);
-      if (idx < tot && r < loaded && c < ncol) v[u] = sp[(int64_t)r * hwPad + c];
+      if (idx < TOT && r < loaded && c < ncol) v[u] = sp[(int64_t)r * hwPad + c];
     }
 #pragma unroll
-    for (int u = 0; u < 8; u++) {
-      int idx = b0 + u * nt;
-      if (idx < tot) {
+    for (int u = 0; u < LB; u++) {
+      int idx = b0 + u * NT + tid;
+      if (idx < TOT) {
         int r = idx / CG, c = idx % CG;
-        int pos = (mode == COL_MULINV) ? r : bitrev_idx(r, log2n);
+        int pos = (mode == COL_MULINV) ? r : bitrev_idx(r, LOG2N);
         a[pos * CG + c] = v[u];
       }
     }
   }
   __syncthreads();
-  if (mode != COL_MULINV) fft_lds<false, true>(a, CG, 1, CG, log2n, tri, false);
+  if (mode != COL_MULINV) fft_lds<LOG2N, CG, NT, CG, false, true>(a, CG, 1, tri, false);
 
   if (mode == COL_TSPEC || mode == COL_FWD) {
-    for (int idx = tid; idx < n * CG; idx += nt) {
+#pragma unroll 4
+    for (int idx = tid; idx < TOT; idx += NT) {
       int r = idx / CG, c = idx % CG;
       if (c < ncol) {
         float2 v = a[r * CG + c];
@@ -468,25 +517,27 @@ __global__ __launch_bounds__(256) void k_cols(int mode, const float2* __restrict
 
   // conjugate product (mulConjGo impl.go:376-388) fused with the bit
   // reversal: rows k and brev[k] are handled by the same thread.
-  for (int b0 = tid; b0 < tot; b0 += nt * 4) {
-    float2 tx[4], ty[4];
+  constexpr int MB = 4;
 #pragma unroll
-    for (int u = 0; u < 4; u++) {
-      int idx = b0 + u * nt;
+  for (int b0 = 0; b0 < TOT; b0 += NT * MB) {
+    float2 tx[MB], ty[MB];
+#pragma unroll
+    for (int u = 0; u < MB; u++) {
+      int idx = b0 + u * NT + tid;
       int r = idx / CG, c = idx % CG;
-      int br = bitrev_idx(r, log2n);
+      int br = bitrev_idx(r, LOG2N);
       tx[u] = ty[u] = make_float2(0.f, 0.f);
-      if (idx < tot && r <= br && c < ncol) {
+      if (idx < TOT && r <= br && c < ncol) {
         tx[u] = ts[(int64_t)r * hwPad + c0 + c];
         ty[u] = ts[(int64_t)br * hwPad + c0 + c];
       }
     }
 #pragma unroll
-    for (int u = 0; u < 4; u++) {
-      int idx = b0 + u * nt;
+    for (int u = 0; u < MB; u++) {
+      int idx = b0 + u * NT + tid;
       int r = idx / CG, c = idx % CG;
-      int br = bitrev_idx(r, log2n);
-      if (idx < tot && r <= br) {
+      int br = bitrev_idx(r, LOG2N);
+      if (idx < TOT && r <= br) {
         float2 x = a[r * CG + c], y = a[br * CG + c];
         float p0 = x.x * tx[u].x, p1 = x.y * tx[u].y, p2 = x.y * tx[u].x, p3 = x.x * tx[u].y;
         float q0 = y.x * ty[u].x, q1 = y.y * ty[u].y, q2 = y.y * ty[u].x, q3 = y.x * ty[u].y;
@@ -496,44 +547,60 @@ __global__ __launch_bounds__(256) void k_cols(int mode, const float2* __restrict
     }
   }
   __syncthreads();
-  fft_lds<false, true>(a, CG, 1, CG, log2n, tri, true);
+  fft_lds<LOG2N, CG, NT, CG, false, true>(a, CG, 1, tri, true);
 
   int need = (bh + 1) & ~1;
   if (need > n) need = n;
-  for (int idx = tid; idx < need * CG; idx += nt) {
+#pragma unroll 4
+  for (int idx = tid; idx < TOT; idx += NT) {
     int r = idx / CG, c = idx % CG;
-    if (c < ncol) dp[(int64_t)r * hwPad + c] = a[r * CG + c];
+    if (r < need && c < ncol) dp[(int64_t)r * hwPad + c] = a[r * CG + c];
   }
 }
 
-static int col_group(int dftH) {
-  // LDS slab of dftH x CG complex kept at or below 64 KiB
-  int cg = 16;
-  while (cg > 1 && (size_t)dftH * cg * sizeof(float2) > 65536) cg >>= 1;
-  return cg;
+template <int LOG2N>
+static void cols_n(ColMode mode, const float2* src, float2* dst, int64_t srcTileStride, int64_t dstTileStride,
+                   int64_t chanStride, int cnMax, const CallState* st, const TileGrid& g, const float2* tri,
+                   const float2* tspec, int64_t tchanStride, float scale, int t0, int ntiles, hipStream_t s) {
+  constexpr int CG = ColCG<LOG2N>::value;
+  size_t lds = sizeof(float2) * (size_t)(1 << LOG2N) * CG;
+  int nslabs = (g.hw + CG - 1) / CG;
+  hipLaunchKernelGGL((k_cols<LOG2N>), dim3(nslabs * ntiles * cnMax), dim3(256), lds, s, (int)mode, src, dst,
+                     srcTileStride, dstTileStride, chanStride, st, g, tri, tspec, tchanStride, scale, t0, ntiles,
+                     nslabs);
+}
+
+static void cols_dispatch(int log2n, ColMode mode, const float2* src, float2* dst, int64_t srcTileStride,
+                          int64_t dstTileStride, int64_t chanStride, int cnMax, const CallState* st,
+                          const TileGrid& g, const float2* tri, const float2* tspec, int64_t tchanStride, float scale,
+                          int t0, int ntiles, hipStream_t s) {
+#define CVM_CN(L)                                                                                         \
+  case L:                                                                                                 \
+    cols_n<L>(mode, src, dst, srcTileStride, dstTileStride, chanStride, cnMax, st, g, tri, tspec, tchanStride, \
+              scale, t0, ntiles, s);                                                                      \
+    break
+  switch (log2n) {
+    CVM_CN(1); CVM_CN(2); CVM_CN(3); CVM_CN(4); CVM_CN(5); CVM_CN(6); CVM_CN(7);
+    CVM_CN(8); CVM_CN(9); CVM_CN(10); CVM_CN(11); CVM_CN(12); CVM_CN(13);
+    default: break;
+  }
+#undef CVM_CN
 }
 
 void launch_cols(ColMode mode, const float2* src, float2* dst, int64_t srcTileStride, int64_t dstTileStride,
                  int64_t chanStride, int cnMax, const CallState* st, const TileGrid& g, const PlanTables& pt,
                  const float2* tspec, const float2* tspec2, int64_t tchanStride, int64_t tchanStride2, float scale,
                  int ntiles, hipStream_t s) {
-  int cg = col_group(g.dftH);
-  size_t lds = sizeof(float2) * (size_t)g.dftH * cg;
-  int nslabs = (g.hw + cg - 1) / cg;
-  dim3 grid(nslabs * ntiles * cnMax);
-  dim3 block(256);
-#define CVM_COLS(CGV)                                                                                          \
-  hipLaunchKernelGGL(k_cols<CGV>, grid, block, lds, s, (int)mode, src, dst, srcTileStride, dstTileStride,      \
-                     chanStride, st, g, pt.triH, pt.brevH, pt.triH2, pt.brevH2, tspec, tspec2, tchanStride,    \
-                     tchanStride2, scale, ntiles, nslabs)
-  switch (cg) {
-    case 16: CVM_COLS(16); break;
-    case 8: CVM_COLS(8); break;
-    case 4: CVM_COLS(4); break;
-    case 2: CVM_COLS(2); break;
-    default: CVM_COLS(1); break;
-  }
-#undef CVM_COLS
+  // tiles of the shrunk last band (the last ntx tiles) run at height dftH2
+  int nshr = g.dftH2 ? g.ntx : 0;
+  if (nshr > ntiles) nshr = 0;
+  int nfull = ntiles - nshr;
+  if (nfull > 0)
+    cols_dispatch(g.log2H, mode, src, dst, srcTileStride, dstTileStride, chanStride, cnMax, st, g, pt.triH, tspec,
+                  tchanStride, scale, 0, nfull, s);
+  if (nshr > 0)
+    cols_dispatch(g.log2H2, mode, src, dst, srcTileStride, dstTileStride, chanStride, cnMax, st, g, pt.triH2, tspec2,
+                  tchanStride2, scale, nfull, nshr, s);
 }
 
 // shrinkInto impl.go:591-610: the shrunk band's template spectrum is the
@@ -558,19 +625,18 @@ void launch_shrink(const float2* tspec, float2* tspec2, int cnMax, int rows2, in
 
 // ------------------------------------------------------ inverse row FFTs --
 
-constexpr int ROWINV_MAXI = 16;  // dftW / threads: 4096/256 and 1024/64
-
 // inverseRowPair impl.go:394-444 for every channel of tile t, rows r, r+1:
 // combine the two half spectra into one complex row, inverse transform,
 // accumulate the real/imaginary parts into result rows y0+r / y0+r+1 with
 // channel 0 stored and channels 1.. added in order (in registers; each
 // thread owns the same columns for every channel).
-__global__ __launch_bounds__(256) void k_row_inv(const float2* __restrict__ spec, int64_t tileStride,
-                                                 int64_t chanStride, const CallState* __restrict__ st, TileGrid g,
-                                                 const float2* __restrict__ triW, const int32_t* __restrict__ brevW,
-                                                 float* __restrict__ corr, int rw, int cpg, int npairs) {
+template <int LOG2N, int NVEC>
+__global__ __launch_bounds__(RowNT<LOG2N>::value) void k_row_inv(
+    const float2* __restrict__ spec, int64_t tileStride, int64_t chanStride, const CallState* __restrict__ st,
+    TileGrid g, const float2* __restrict__ triW, float* __restrict__ corr, int rw, int cpg, int npairs) {
+  constexpr int n = 1 << LOG2N, NT = RowNT<LOG2N>::value, np = n + (n >> 5);
+  constexpr int MAXI = (n + NT - 1) / NT;
   extern __shared__ float2 z[];
-  const int n = g.dftW, np = padded_len(n);
   const int L = blockIdx.x;
   const int rp = L % npairs, t = L / npairs;
   const int r = 2 * rp;
@@ -578,46 +644,48 @@ __global__ __launch_bounds__(256) void k_row_inv(const float2* __restrict__ spec
   int x0, y0, bw, bh;
   tile_rect(g, t, x0, y0, bw, bh);
   if (r >= bh) return;
-  const int cn = st->cn, tid = threadIdx.x, nt = blockDim.x, hw = g.hw;
-  float2 acc[ROWINV_MAXI];
+  const int cn = st->cn, tid = threadIdx.x, hw = g.hw;
+  float2 acc[MAXI];
 #pragma unroll
-  for (int i = 0; i < ROWINV_MAXI; i++) acc[i] = make_float2(0.f, 0.f);
+  for (int i = 0; i < MAXI; i++) acc[i] = make_float2(0.f, 0.f);
   for (int k0 = 0; k0 < cn; k0 += cpg) {
     const int nv = min(cpg, cn - k0);
-    for (int v = 0; v < nv; v++) {
+#pragma unroll
+    for (int v = 0; v < NVEC; v++) {
+      if (v >= nv) break;
       const float2* sa = spec + (int64_t)t * tileStride + (int64_t)(k0 + v) * chanStride + (int64_t)r * g.hwPad;
       const float2* sb = sa + g.hwPad;
       float2* zv = z + v * np;
-      for (int q0 = tid; q0 < n; q0 += nt * 8) {
-        float2 A[8], B[8];
+      float2 A[MAXI], B[MAXI];
 #pragma unroll
-        for (int u = 0; u < 8; u++) {
-          int q = q0 + u * nt;
-          int m = q < hw ? q : n - q;
-          A[u] = B[u] = make_float2(0.f, 0.f);
-          if (q < n) {
-            A[u] = sa[m];
-            B[u] = sb[m];
-          }
+      for (int i = 0; i < MAXI; i++) {
+        int q = tid + i * NT;
+        int m = q < hw ? q : n - q;
+        A[i] = B[i] = make_float2(0.f, 0.f);
+        if (q < n) {
+          A[i] = sa[m];
+          B[i] = sb[m];
         }
+      }
 #pragma unroll
-        for (int u = 0; u < 8; u++) {
-          int q = q0 + u * nt;
-          if (q < n) {
-            float2 w = q < hw ? make_float2(A[u].x - B[u].y, A[u].y + B[u].x)
-                              : make_float2(A[u].x + B[u].y, B[u].x - A[u].y);
-            zv[padi(bitrev_idx(q, g.log2W))] = w;
-          }
+      for (int i = 0; i < MAXI; i++) {
+        int q = tid + i * NT;
+        if (q < n) {
+          float2 w = q < hw ? make_float2(A[i].x - B[i].y, A[i].y + B[i].x)
+                            : make_float2(A[i].x + B[i].y, B[i].x - A[i].y);
+          zv[padi(bitrev_idx(q, LOG2N))] = w;
         }
       }
     }
     __syncthreads();
-    fft_lds<true, false>(z, nv, np, 1, g.log2W, triW, true);
-    for (int v = 0; v < nv; v++) {
+    fft_lds<LOG2N, NVEC, NT, 1, true, false>(z, nv, np, triW, true);
+#pragma unroll
+    for (int v = 0; v < NVEC; v++) {
+      if (v >= nv) break;
       const float2* zv = z + v * np;
 #pragma unroll
-      for (int i = 0; i < ROWINV_MAXI; i++) {
-        int x = tid + i * nt;
+      for (int i = 0; i < MAXI; i++) {
+        int x = tid + i * NT;
         if (x < bw) {
           float2 w = zv[padi(x)];
           if (k0 + v == 0)
@@ -633,8 +701,8 @@ __global__ __launch_bounds__(256) void k_row_inv(const float2* __restrict__ spec
   float* o2 = o1 + rw;
   const bool second = r + 1 < bh;
 #pragma unroll
-  for (int i = 0; i < ROWINV_MAXI; i++) {
-    int x = tid + i * nt;
+  for (int i = 0; i < MAXI; i++) {
+    int x = tid + i * NT;
     if (x < bw) {
       o1[x] = acc[i].x;
       if (second) o2[x] = acc[i].y;
@@ -642,19 +710,42 @@ __global__ __launch_bounds__(256) void k_row_inv(const float2* __restrict__ spec
   }
 }
 
-void launch_row_inv(const float2* spec, int64_t tileStride, int64_t chanStride, const CallState* st, int cnMax,
-                    const TileGrid& g, const PlanTables& pt, float* corr, int rw, int ntiles, hipStream_t s) {
-  int nt = g.dftW >= 1024 ? 256 : 64;
-  int cpg = row_cpg(g.dftW, cnMax);
-  size_t lds = sizeof(float2) * (size_t)padded_len(g.dftW) * cpg;
+template <int LOG2N>
+static void row_inv_n(const float2* spec, int64_t tileStride, int64_t chanStride, const CallState* st, int cnMax,
+                      const TileGrid& g, const PlanTables& pt, float* corr, int rw, int ntiles, hipStream_t s) {
+  constexpr int n = 1 << LOG2N;
+  int cpg = row_cpg(n, cnMax);
+  size_t lds = sizeof(float2) * (size_t)padded_len(n) * cpg;
   // last-band tiles may be taller than blockH (fleet members): cover the
   // largest bh any tile can have
   int bhMax = g.blockH;
   int bhl = g.extH - (g.nty - 1) * g.blockH;
   if (bhl > bhMax) bhMax = bhl;
   int npairs = (bhMax + 1) / 2;
-  hipLaunchKernelGGL(k_row_inv, dim3(npairs * ntiles), dim3(nt), lds, s, spec, tileStride, chanStride, st, g, pt.triW,
-                     pt.brevW, corr, rw, cpg, npairs);
+  dim3 grid(npairs * ntiles), block(RowNT<LOG2N>::value);
+  if (cpg <= 1)
+    hipLaunchKernelGGL((k_row_inv<LOG2N, 1>), grid, block, lds, s, spec, tileStride, chanStride, st, g, pt.triW, corr,
+                       rw, cpg, npairs);
+  else if (cpg == 2)
+    hipLaunchKernelGGL((k_row_inv<LOG2N, 2>), grid, block, lds, s, spec, tileStride, chanStride, st, g, pt.triW, corr,
+                       rw, cpg, npairs);
+  else
+    hipLaunchKernelGGL((k_row_inv<LOG2N, 4>), grid, block, lds, s, spec, tileStride, chanStride, st, g, pt.triW, corr,
+                       rw, cpg, npairs);
+}
+
+void launch_row_inv(const float2* spec, int64_t tileStride, int64_t chanStride, const CallState* st, int cnMax,
+                    const TileGrid& g, const PlanTables& pt, float* corr, int rw, int ntiles, hipStream_t s) {
+#define CVM_RI(L) \
+  case L:         \
+    row_inv_n<L>(spec, tileStride, chanStride, st, cnMax, g, pt, corr, rw, ntiles, s); \
+    break
+  switch (g.log2W) {
+    CVM_RI(1); CVM_RI(2); CVM_RI(3); CVM_RI(4); CVM_RI(5); CVM_RI(6);
+    CVM_RI(7); CVM_RI(8); CVM_RI(9); CVM_RI(10); CVM_RI(11); CVM_RI(12);
+    default: break;
+  }
+#undef CVM_RI
 }
 
 }  // namespace cvm

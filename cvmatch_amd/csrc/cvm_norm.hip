@@ -504,6 +504,32 @@ __global__ __launch_bounds__(256) void k_colsum_chunks(const uint8_t* __restrict
   pq[(int64_t)c * iw + x] = q;
 }
 
+// Inclusive add-scan of a uint32 over the 64 lanes of a wave with DPP moves
+// (row_shr 1/2/4/8 inside each 16-lane row, then row_bcast 15/31 across
+// rows): VALU-latency steps instead of LDS permutes.
+template <int CTRL>
+__device__ __forceinline__ uint32_t dpp_mov(uint32_t v) {
+  return (uint32_t)__builtin_amdgcn_mov_dpp((int)v, CTRL, 0xf, 0xf, false);
+}
+
+__device__ __forceinline__ uint32_t wave_incl_scan_u32(uint32_t x, int lane) {
+  const int rl = lane & 15;
+  uint32_t t;
+  t = dpp_mov<0x111>(x);
+  if (rl >= 1) x += t;
+  t = dpp_mov<0x112>(x);
+  if (rl >= 2) x += t;
+  t = dpp_mov<0x114>(x);
+  if (rl >= 4) x += t;
+  t = dpp_mov<0x118>(x);
+  if (rl >= 8) x += t;
+  t = dpp_mov<0x142>(x);
+  if ((lane & 31) >= 16) x += t;
+  t = dpp_mov<0x143>(x);
+  if (lane >= 32) x += t;
+  return x;
+}
+
 // Eight consecutive pixels x0..x0+7 of one row as packed channel words
 // (channel k in bits 8k..).  vmode 1: RGBA rows 16-byte aligned, two 16-byte
 // loads; vmode 2: gray rows 8-byte aligned, one 8-byte load; otherwise, or
@@ -557,8 +583,11 @@ __global__ __launch_bounds__(64) void k_norm_wave(const uint8_t* __restrict__ im
   constexpr int CPT = NORM_CPT;
   constexpr int NW = 64 * CPT;
   constexpr int SX = 64 * Q;
-  __shared__ uint32_t P[4][NW + 1];
-  __shared__ sq_t P2[NW + 1];
+  // prefix index c lives at c + c/32: lane l writes columns 8l..8l+7, and
+  // the pad word spreads the 64 lanes of one store over all banks
+  constexpr int NWP = NW + 1 + (NW + 1) / 32 + 1;
+  __shared__ uint32_t P[4][NWP];
+  __shared__ sq_t P2[NWP];
   const int lane = threadIdx.x;
   const int xs = blockIdx.x * SX, ys = blockIdx.y * NB_ROWS;
   const int ye = min(rh, ys + NB_ROWS);
@@ -661,21 +690,22 @@ __global__ __launch_bounds__(64) void k_norm_wave(const uint8_t* __restrict__ im
       }
       uint32_t inc[4] = {run[0], run[1], run[2], run[3]};
       sq_t inc2 = run2;
-      for (int o = 1; o < 64; o <<= 1) {
-        uint32_t up[4];
-        for (int k = 0; k < 4; k++)
-          if (k < cn) up[k] = __shfl_up(inc[k], o, 64);
-        sq_t up2 = __shfl_up(inc2, o, 64);
-        if (lane >= o) {
-          for (int k = 0; k < 4; k++)
-            if (k < cn) inc[k] += up[k];
-          inc2 += up2;
+      for (int k = 0; k < 4; k++)
+        if (k < cn) inc[k] = wave_incl_scan_u32(inc[k], lane);
+      if (SQ32) {
+        inc2 = (sq_t)wave_incl_scan_u32((uint32_t)inc2, lane);
+      } else {
+        for (int o = 1; o < 64; o <<= 1) {
+          sq_t up2 = __shfl_up(inc2, o, 64);
+          if (lane >= o) inc2 += up2;
         }
       }
       for (int j = 0; j < CPT; j++) {
+        const int pc = cbase + j + 1;
+        const int pi = pc + (pc >> 5);
         for (int k = 0; k < 4; k++)
-          if (k < cn) P[k][cbase + j + 1] = inc[k] - run[k] + loc[j][k];
-        P2[cbase + j + 1] = inc2 - run2 + loc2[j];
+          if (k < cn) P[k][pi] = inc[k] - run[k] + loc[j][k];
+        P2[pi] = inc2 - run2 + loc2[j];
       }
       if (lane == 0) {
         for (int k = 0; k < 4; k++) P[k][0] = 0;
@@ -686,9 +716,10 @@ __global__ __launch_bounds__(64) void k_norm_wave(const uint8_t* __restrict__ im
         int xl = lane + 64 * q, x = xs + xl;
         if (x < rw) {
           // exact integers carried in binary64 (all below 2^53)
+          const int ia = xl + (xl >> 5), ib = (xl + tw) + ((xl + tw) >> 5);
           double sd[4];
-          for (int k = 0; k < 4; k++) sd[k] = (k < cn) ? (double)(uint32_t)(P[k][xl + tw] - P[k][xl]) : 0.0;
-          double s2 = (double)(sq_t)(P2[xl + tw] - P2[xl]);
+          for (int k = 0; k < 4; k++) sd[k] = (k < cn) ? (double)(uint32_t)(P[k][ib] - P[k][ia]) : 0.0;
+          double s2 = (double)(sq_t)(P2[ib] - P2[ia]);
           double lane0, sq;
           if (cn == 1) {
             lane0 = sd[0];

@@ -258,7 +258,7 @@ int norm_partials_count(int rw, int rh, int tw, int th, int cnMax) {
 size_t norm_scratch_bytes(int iw, int ih, int tw, int th, int cnMax) {
   if (!norm_plan(tw, th, cnMax).wave) return 0;
   size_t nch = (size_t)(ih + NB_ROWS - 1) / NB_ROWS;
-  return nch * (size_t)iw * 5 * sizeof(uint32_t);
+  return nch * (size_t)iw * 3 * sizeof(uint32_t);
 }
 
 // One workgroup: result columns [xs, xs+256) x rows [ys, ye).  Each thread
@@ -468,16 +468,18 @@ __global__ __launch_bounds__(256) void k_normalize(const uint8_t* __restrict__ i
 // consecutive image columns of the strip over the current th window rows.
 // The sums at the band start come from per-chunk partial column sums of
 // NB_ROWS rows (k_colsum_chunks reads every pixel once) plus th % NB_ROWS
-// rows read directly; the window then slides down one row per step, the
-// next slide's pixels being loaded before the current row's scan and tail.
+// rows read directly; the window then slides down one row per step.
 // Horizontal window sums are differences of a wave-level prefix scan in LDS.
 
 // Channel-byte mask of a packed pixel word: bytes of channels >= cn cleared.
 __device__ __forceinline__ uint32_t chan_mask(int cn) { return cn >= 4 ? 0xFFFFFFFFu : ((1u << (8 * cn)) - 1u); }
 
+// Partial column sums of chunk c (rows [c*NB_ROWS, c*NB_ROWS + NB_ROWS)),
+// three words per image column: ch0 | ch2 << 16, ch1 | ch3 << 16 (each at
+// most NB_ROWS * 255) and the sum of squares over the first cn channels.
 __global__ __launch_bounds__(256) void k_colsum_chunks(const uint8_t* __restrict__ img, int64_t istride, int iw,
                                                        int ih, int step, int words, const CallState* __restrict__ st,
-                                                       uint32_t* __restrict__ ps, uint32_t* __restrict__ pq) {
+                                                       uint32_t* __restrict__ ps) {
   const int x = blockIdx.x * blockDim.x + threadIdx.x, c = blockIdx.y;
   if (x >= iw || st->flat) return;
   const int cn = st->cn;
@@ -498,10 +500,10 @@ __global__ __launch_bounds__(256) void k_colsum_chunks(const uint8_t* __restrict
     s13 += (w >> 8) & 0x00FF00FFu;
     q = __builtin_amdgcn_udot4(w, w, q, false);
   }
-  const uint32_t sv[4] = {s02 & 0xFFFFu, s13 & 0xFFFFu, s02 >> 16, s13 >> 16};
-  for (int k = 0; k < 4; k++)
-    if (k < cn) ps[((int64_t)c * 4 + k) * iw + x] = sv[k];
-  pq[(int64_t)c * iw + x] = q;
+  uint32_t* o = ps + (int64_t)c * 3 * iw + x;
+  o[0] = s02;
+  o[iw] = s13;
+  o[2 * iw] = q;
 }
 
 // Inclusive add-scan of a uint32 over the 64 lanes of a wave with DPP moves
@@ -531,54 +533,67 @@ __device__ __forceinline__ uint32_t wave_incl_scan_u32(uint32_t x, int lane) {
 }
 
 // Eight consecutive pixels x0..x0+7 of one row as packed channel words
-// (channel k in bits 8k..).  vmode 1: RGBA rows 16-byte aligned, two 16-byte
-// loads; vmode 2: gray rows 8-byte aligned, one 8-byte load; otherwise, or
-// when fewer than 8 columns are valid, byte loads of the valid columns.
-__device__ __forceinline__ void load_px8(const uint8_t* row, int x0, int step, int cn, int vmode, int nvalid,
+// (channel k in bits 8k..).  VMODE 1: RGBA rows 16-byte aligned, two 16-byte
+// loads; VMODE 2: gray rows 8-byte aligned, one 8-byte load (both with all
+// eight columns inside the row); VMODE 0: byte loads of the nvalid valid
+// columns, zeros past them.
+template <int VMODE>
+__device__ __forceinline__ void load_px8(const uint8_t* row, int x0, int step, int cn, int nvalid,
                                          uint32_t out[NORM_CPT]) {
-  if (nvalid >= NORM_CPT && vmode == 1) {
+  if constexpr (VMODE == 1) {
     const uint4* p = (const uint4*)(row + (int64_t)x0 * 4);
     uint4 a = p[0], b = p[1];
     out[0] = a.x; out[1] = a.y; out[2] = a.z; out[3] = a.w;
     out[4] = b.x; out[5] = b.y; out[6] = b.z; out[7] = b.w;
-    return;
-  }
-  if (nvalid >= NORM_CPT && vmode == 2) {
+  } else if constexpr (VMODE == 2) {
     uint2 v = *(const uint2*)(row + x0);
+#pragma unroll
     for (int j = 0; j < 4; j++) {
       out[j] = (v.x >> (8 * j)) & 255u;
       out[4 + j] = (v.y >> (8 * j)) & 255u;
     }
-    return;
-  }
-  for (int j = 0; j < NORM_CPT; j++) {
-    uint32_t w = 0;
-    if (j < nvalid) {
-      const uint8_t* p = row + (int64_t)(x0 + j) * step;
-      for (int k = 0; k < 4; k++)
-        if (k < cn) w |= (uint32_t)p[k] << (8 * k);
+  } else {
+    for (int j = 0; j < NORM_CPT; j++) {
+      uint32_t w = 0;
+      if (j < nvalid) {
+        const uint8_t* p = row + (int64_t)(x0 + j) * step;
+        for (int k = 0; k < 4; k++)
+          if (k < cn) w |= (uint32_t)p[k] << (8 * k);
+      }
+      out[j] = w;
     }
-    out[j] = w;
   }
 }
 
-// Eight consecutive uint32 partials (16-byte loads when aligned).
-__device__ __forceinline__ void load_u32x8(const uint32_t* p, bool vec, int nvalid, uint32_t out[NORM_CPT]) {
-  if (vec && nvalid >= NORM_CPT) {
+// Eight consecutive uint32 partials: two 16-byte loads when VEC (16-byte
+// aligned, all eight inside the row), else the nvalid valid ones.
+template <bool VEC>
+__device__ __forceinline__ void load_u32x8(const uint32_t* p, int nvalid, uint32_t out[NORM_CPT]) {
+  if constexpr (VEC) {
     uint4 a = ((const uint4*)p)[0], b = ((const uint4*)p)[1];
     out[0] = a.x; out[1] = a.y; out[2] = a.z; out[3] = a.w;
     out[4] = b.x; out[5] = b.y; out[6] = b.z; out[7] = b.w;
-    return;
+  } else {
+    for (int j = 0; j < NORM_CPT; j++) out[j] = j < nvalid ? p[j] : 0u;
   }
-  for (int j = 0; j < NORM_CPT; j++) out[j] = j < nvalid ? p[j] : 0u;
 }
 
-template <int Q, bool SQ32>
-__global__ __launch_bounds__(64) void k_norm_wave(const uint8_t* __restrict__ img, int64_t istride, int iw, int step,
-                                                  int vmode, int tw, int th, int rw, int rh,
+// Slide inputs of one result row: the entering and leaving pixel rows and
+// the correlation values of the row.
+template <int Q>
+struct NormRow {
+  uint32_t a[NORM_CPT], b[NORM_CPT];
+  float c[Q];
+};
+
+// VMODE 1 / 2 (RGBA / gray rows with iw % 8 == 0, aligned for vector loads):
+// every lane loads eight whole columns, lanes past the image clamped to its
+// last eight, with no branch around any load; VMODE 0: byte loads.
+template <int Q, bool SQ32, int VMODE>
+__global__ __launch_bounds__(64) void k_norm_wave(const uint8_t* __restrict__ img, int64_t istride, int iw, int ih,
+                                                  int step, int tw, int th, int rw, int rh,
                                                   const CallState* __restrict__ st, const uint32_t* __restrict__ ps,
-                                                  const uint32_t* __restrict__ pq, const float* corr, float* map,
-                                                  Extrema* __restrict__ parts) {
+                                                  const float* corr, float* map, Extrema* __restrict__ parts) {
   typedef typename std::conditional<SQ32, uint32_t, unsigned long long>::type sq_t;
   constexpr int CPT = NORM_CPT;
   constexpr int NW = 64 * CPT;
@@ -591,96 +606,112 @@ __global__ __launch_bounds__(64) void k_norm_wave(const uint8_t* __restrict__ im
   const int lane = threadIdx.x;
   const int xs = blockIdx.x * SX, ys = blockIdx.y * NB_ROWS;
   const int ye = min(rh, ys + NB_ROWS);
-  const int cn = st->cn;
-  const bool flat = st->flat != 0;
-  const int W = min(SX + tw - 1, iw - xs);
-  const double aread = (double)tw * (double)th;
-  const float numScale = st->numScale, varScale = st->varScale, templNorm = st->templNorm;
-  double tsd[4];
-  for (int k = 0; k < 4; k++) tsd[k] = (double)st->tsum[k];
-  const int cbase = lane * CPT;
-  const int x0 = xs + cbase;
-  const int nvalid = min(CPT, W - cbase);  // may be <= 0
-  const bool pvec = (iw & 3) == 0;
-
-  // column sums over the th window rows, channels packed in 16-bit halves:
-  // cs02 = ch0 | ch2 << 16, cs13 = ch1 | ch3 << 16 (each <= 255 * th < 2^16)
-  const uint32_t cm = chan_mask(cn);
-  uint32_t cs02[CPT], cs13[CPT];
-  sq_t c2[CPT];
-  for (int j = 0; j < CPT; j++) {
-    cs02[j] = cs13[j] = 0;
-    c2[j] = 0;
-  }
-  if (!flat && nvalid > 0) {
-    const int nfull = th / NB_ROWS;
-    for (int i = 0; i < nfull; i++) {
-      int ch = blockIdx.y + i;
-      uint32_t t[CPT];
-      for (int k = 0; k < 4; k++)
-        if (k < cn) {
-          load_u32x8(ps + ((int64_t)ch * 4 + k) * iw + x0, pvec, nvalid, t);
-          for (int j = 0; j < CPT; j++) {
-            uint32_t sh = (k >> 1) * 16;
-            if (k & 1)
-              cs13[j] += t[j] << sh;
-            else
-              cs02[j] += t[j] << sh;
-          }
-        }
-      load_u32x8(pq + (int64_t)ch * iw + x0, pvec, nvalid, t);
-      for (int j = 0; j < CPT; j++) c2[j] += t[j];
-    }
-    for (int y = ys + nfull * NB_ROWS; y < ys + th; y++) {
-      uint32_t w[CPT];
-      load_px8(img + (int64_t)y * istride, x0, step, cn, vmode, nvalid, w);
-      for (int j = 0; j < CPT; j++) {
-        uint32_t v = w[j] & cm;
-        cs02[j] += v & 0x00FF00FFu;
-        cs13[j] += (v >> 8) & 0x00FF00FFu;
-        c2[j] += (sq_t)__builtin_amdgcn_udot4(v, v, 0u, false);
-      }
-    }
-  }
-
   float bmin = FLT_MAX, bmax = -FLT_MAX;
   int bmin_i = INT_MAX, bmax_i = INT_MAX;
 
-  // Row y's slide pixels (rows y+th and y) and corr values are loaded one
-  // row ahead, so their latency overlaps the previous row's scan and tail.
-  uint32_t na[CPT], nb[CPT];
-  float ncq[Q];
-  auto fetch = [&](int yy) {
-    for (int j = 0; j < CPT; j++) na[j] = nb[j] = 0;
-    if (!flat && yy + 1 < ye && nvalid > 0) {
-      load_px8(img + (int64_t)(yy + th) * istride, x0, step, cn, vmode, nvalid, na);
-      load_px8(img + (int64_t)yy * istride, x0, step, cn, vmode, nvalid, nb);
-    }
-    for (int q = 0; q < Q; q++) {
-      int x = xs + lane + 64 * q;
-      ncq[q] = (!flat && x < rw && yy < ye) ? corr[(int64_t)yy * rw + x] : 0.f;
-    }
-  };
-  fetch(ys);
-  for (int y = ys; y < ye; y++) {
-    const bool slide = !flat && (y + 1 < ye);
-    uint32_t va[CPT], vb[CPT];
-    float cq[Q];
+  if (st->flat) {
+    // templNorm below the flat threshold: the map is 1 everywhere
+    for (int y = ys; y < ye; y++)
+      for (int q = 0; q < Q; q++) {
+        int x = xs + lane + 64 * q;
+        if (x < rw) {
+          int idx = y * rw + x;
+          if (map) map[(int64_t)y * rw + x] = 1.0f;
+          if (better_min(1.0f, idx, bmin, bmin_i)) {
+            bmin = 1.0f;
+            bmin_i = idx;
+          }
+          if (better_max(1.0f, idx, bmax, bmax_i)) {
+            bmax = 1.0f;
+            bmax_i = idx;
+          }
+        }
+      }
+  } else {
+    const int cn = st->cn;
+    const bool one = cn == 1;
+    const double aread = (double)tw * (double)th;
+    const float numScale = st->numScale, varScale = st->varScale, templNorm = st->templNorm;
+    double tsd[4];
+    for (int k = 0; k < 4; k++) tsd[k] = k < cn ? (double)st->tsum[k] : 0.0;
+    const int cbase = lane * CPT;
+    // every in-image column of the lane is summed; columns past the strip
+    // (or, clamped, repeats of the image's last eight) only reach prefix
+    // entries that no result of the strip reads
+    const int x0 = VMODE ? min(xs + cbase, iw - CPT) : xs + cbase;
+    const int nload = VMODE ? CPT : min(CPT, iw - x0);  // may be <= 0
+    const uint32_t cm = chan_mask(cn);
+
+    // column sums over the th window rows, channels packed in 16-bit halves:
+    // cs02 = ch0 | ch2 << 16, cs13 = ch1 | ch3 << 16 (each <= 255 * th < 2^16)
+    uint32_t cs02[CPT], cs13[CPT];
+    sq_t c2[CPT];
+#pragma unroll
     for (int j = 0; j < CPT; j++) {
-      va[j] = na[j];
-      vb[j] = nb[j];
+      cs02[j] = cs13[j] = 0;
+      c2[j] = 0;
     }
-    for (int q = 0; q < Q; q++) cq[q] = ncq[q];
-    if (y + 1 < ye) fetch(y + 1);
-    float vq[Q];
-    for (int q = 0; q < Q; q++) vq[q] = 1.0f;
-    if (!flat) {
+    if (nload > 0) {
+      const int nfull = th / NB_ROWS;
+      const uint32_t* pc = ps + (int64_t)blockIdx.y * 3 * iw + x0;
+#pragma unroll 2
+      for (int i = 0; i < nfull; i++, pc += (int64_t)3 * iw) {
+        uint32_t t0[CPT], t1[CPT], t2[CPT];
+        load_u32x8<VMODE != 0>(pc, nload, t0);
+        load_u32x8<VMODE != 0>(pc + iw, nload, t1);
+        load_u32x8<VMODE != 0>(pc + 2 * iw, nload, t2);
+#pragma unroll
+        for (int j = 0; j < CPT; j++) {
+          cs02[j] += t0[j];
+          cs13[j] += t1[j];
+          c2[j] += t2[j];
+        }
+      }
+      for (int y = ys + nfull * NB_ROWS; y < ys + th; y++) {
+        uint32_t w[CPT];
+        load_px8<VMODE>(img + (int64_t)y * istride, x0, step, cn, nload, w);
+#pragma unroll
+        for (int j = 0; j < CPT; j++) {
+          uint32_t v = w[j] & cm;
+          cs02[j] += v & 0x00FF00FFu;
+          cs13[j] += (v >> 8) & 0x00FF00FFu;
+          c2[j] += (sq_t)__builtin_amdgcn_udot4(v, v, 0u, false);
+        }
+      }
+    }
+
+    // Row yy's inputs; rows past the band are clamped into the image (loaded
+    // and never used) so the loads need no branch.
+    auto fetch = [&](NormRow<Q>& r, int yy) {
+      const int yl = min(yy, ye - 1);
+      const int ya = min(yl + th, ih - 1);
+      if (VMODE) {
+        load_px8<VMODE>(img + (int64_t)ya * istride, x0, step, cn, nload, r.a);
+        load_px8<VMODE>(img + (int64_t)yl * istride, x0, step, cn, nload, r.b);
+      } else {
+#pragma unroll
+        for (int j = 0; j < CPT; j++) r.a[j] = r.b[j] = 0;
+        if (nload > 0) {
+          load_px8<0>(img + (int64_t)ya * istride, x0, step, cn, nload, r.a);
+          load_px8<0>(img + (int64_t)yl * istride, x0, step, cn, nload, r.b);
+        }
+      }
+#pragma unroll
+      for (int q = 0; q < Q; q++) {
+        int x = min(xs + lane + 64 * q, rw - 1);
+        r.c[q] = corr[(int64_t)yl * rw + x];
+      }
+    };
+
+    auto row = [&](const NormRow<Q>& r, int y) {
       uint32_t run[4] = {0, 0, 0, 0};
       sq_t run2 = 0;
       uint32_t loc[CPT][4];
       sq_t loc2[CPT];
+#pragma unroll
       for (int j = 0; j < CPT; j++) {
         const uint32_t cv[4] = {cs02[j] & 0xFFFFu, cs13[j] & 0xFFFFu, cs02[j] >> 16, cs13[j] >> 16};
+#pragma unroll
         for (int k = 0; k < 4; k++) {
           run[k] += cv[k];
           loc[j][k] = run[k];
@@ -688,81 +719,88 @@ __global__ __launch_bounds__(64) void k_norm_wave(const uint8_t* __restrict__ im
         run2 += c2[j];
         loc2[j] = run2;
       }
-      uint32_t inc[4] = {run[0], run[1], run[2], run[3]};
-      sq_t inc2 = run2;
-      for (int k = 0; k < 4; k++)
-        if (k < cn) inc[k] = wave_incl_scan_u32(inc[k], lane);
+      uint32_t off[4];
+#pragma unroll
+      for (int k = 0; k < 4; k++) off[k] = wave_incl_scan_u32(run[k], lane) - run[k];
+      sq_t off2;
       if (SQ32) {
-        inc2 = (sq_t)wave_incl_scan_u32((uint32_t)inc2, lane);
+        off2 = (sq_t)wave_incl_scan_u32((uint32_t)run2, lane) - run2;
       } else {
+        sq_t inc2 = run2;
         for (int o = 1; o < 64; o <<= 1) {
           sq_t up2 = __shfl_up(inc2, o, 64);
           if (lane >= o) inc2 += up2;
         }
+        off2 = inc2 - run2;
       }
+#pragma unroll
       for (int j = 0; j < CPT; j++) {
         const int pc = cbase + j + 1;
         const int pi = pc + (pc >> 5);
-        for (int k = 0; k < 4; k++)
-          if (k < cn) P[k][pi] = inc[k] - run[k] + loc[j][k];
-        P2[pi] = inc2 - run2 + loc2[j];
+#pragma unroll
+        for (int k = 0; k < 4; k++) P[k][pi] = off[k] + loc[j][k];
+        P2[pi] = off2 + loc2[j];
       }
       if (lane == 0) {
+#pragma unroll
         for (int k = 0; k < 4; k++) P[k][0] = 0;
         P2[0] = 0;
       }
       __syncthreads();
+      float vq[Q];
+#pragma unroll
       for (int q = 0; q < Q; q++) {
-        int xl = lane + 64 * q, x = xs + xl;
-        if (x < rw) {
-          // exact integers carried in binary64 (all below 2^53)
-          const int ia = xl + (xl >> 5), ib = (xl + tw) + ((xl + tw) >> 5);
-          double sd[4];
-          for (int k = 0; k < 4; k++) sd[k] = (k < cn) ? (double)(uint32_t)(P[k][ib] - P[k][ia]) : 0.0;
-          double s2 = (double)(sq_t)(P2[ib] - P2[ia]);
-          double lane0, sq;
-          if (cn == 1) {
-            lane0 = sd[0];
-            sq = sd[0] * sd[0];
-          } else {
-            lane0 = sd[0] * tsd[0];
-            sq = sd[0] * sd[0];
-            for (int k = 1; k < 4; k++)
-              if (k < cn) {
-                lane0 = lane0 + sd[k] * tsd[k];
-                sq = sq + sd[k] * sd[k];
-              }
-          }
-          double idiff = aread * s2 - sq;
-          vq[q] = norm_one(cq[q], (float)lane0, (float)idiff, (float)s2, numScale, varScale, templNorm);
-        }
+        const int xl = lane + 64 * q, xb = xl + tw;
+        const int ia = xl + (xl >> 5), ib = xb + (xb >> 5);
+        // exact integers carried in binary64 (all below 2^53)
+        double sd[4];
+#pragma unroll
+        for (int k = 0; k < 4; k++) sd[k] = (double)(uint32_t)(P[k][ib] - P[k][ia]);
+        const double s2 = (double)(sq_t)(P2[ib] - P2[ia]);
+        const double sq = sd[0] * sd[0] + sd[1] * sd[1] + sd[2] * sd[2] + sd[3] * sd[3];
+        double lane0 = sd[0] * tsd[0] + sd[1] * tsd[1] + sd[2] * tsd[2] + sd[3] * tsd[3];
+        if (one) lane0 = sd[0];
+        const double idiff = aread * s2 - sq;
+        vq[q] = norm_one(r.c[q], (float)lane0, (float)idiff, (float)s2, numScale, varScale, templNorm);
       }
       __syncthreads();
-      if (slide) {
-        // the leaving row is inside the current sums, so each 16-bit half
-        // stays in [0, 2^16) through the subtraction: no borrow crosses halves
-        for (int j = 0; j < CPT; j++) {
-          uint32_t a = va[j] & cm, b = vb[j] & cm;
-          cs02[j] = cs02[j] - (b & 0x00FF00FFu) + (a & 0x00FF00FFu);
-          cs13[j] = cs13[j] - ((b >> 8) & 0x00FF00FFu) + ((a >> 8) & 0x00FF00FFu);
-          c2[j] += (sq_t)__builtin_amdgcn_udot4(a, a, 0u, false) - (sq_t)__builtin_amdgcn_udot4(b, b, 0u, false);
+      // slide to row y+1: the leaving row is inside the current sums, so
+      // each 16-bit half stays in [0, 2^16) and no borrow crosses halves
+#pragma unroll
+      for (int j = 0; j < CPT; j++) {
+        const uint32_t a = r.a[j] & cm, b = r.b[j] & cm;
+        cs02[j] = cs02[j] - (b & 0x00FF00FFu) + (a & 0x00FF00FFu);
+        cs13[j] = cs13[j] - ((b >> 8) & 0x00FF00FFu) + ((a >> 8) & 0x00FF00FFu);
+        c2[j] += (sq_t)__builtin_amdgcn_udot4(a, a, 0u, false) - (sq_t)__builtin_amdgcn_udot4(b, b, 0u, false);
+      }
+#pragma unroll
+      for (int q = 0; q < Q; q++) {
+        const int x = xs + lane + 64 * q;
+        if (x < rw) {
+          const int idx = y * rw + x;
+          if (map) map[(int64_t)y * rw + x] = vq[q];
+          if (better_min(vq[q], idx, bmin, bmin_i)) {
+            bmin = vq[q];
+            bmin_i = idx;
+          }
+          if (better_max(vq[q], idx, bmax, bmax_i)) {
+            bmax = vq[q];
+            bmax_i = idx;
+          }
         }
       }
-    }
-    for (int q = 0; q < Q; q++) {
-      int x = xs + lane + 64 * q;
-      if (x < rw) {
-        int idx = y * rw + x;
-        if (map) map[(int64_t)y * rw + x] = vq[q];
-        if (better_min(vq[q], idx, bmin, bmin_i)) {
-          bmin = vq[q];
-          bmin_i = idx;
-        }
-        if (better_max(vq[q], idx, bmax, bmax_i)) {
-          bmax = vq[q];
-          bmax_i = idx;
-        }
-      }
+    };
+
+    // two register sets in turn: row y+1's loads are in flight while row y
+    // is evaluated, and no loaded register is copied before use
+    NormRow<Q> ra, rb;
+    fetch(ra, ys);
+    for (int y = ys; y < ye; y += 2) {
+      fetch(rb, y + 1);
+      row(ra, y);
+      if (y + 1 >= ye) break;
+      fetch(ra, y + 2);
+      row(rb, y + 1);
     }
   }
   for (int o = 32; o > 0; o >>= 1) {
@@ -796,28 +834,39 @@ void launch_normalize(const uint8_t* img, int64_t istride, int iw, int ih, int s
   if (np.wave && scratch) {
     int nch = (ih + NB_ROWS - 1) / NB_ROWS;
     uint32_t* ps = (uint32_t*)scratch;
-    uint32_t* pq = ps + (size_t)nch * iw * 4;
     int words = (step == 4 && ((((uintptr_t)img) | (uintptr_t)istride) & 3) == 0) ? 1 : 0;
     hipLaunchKernelGGL(k_colsum_chunks, dim3((iw + 255) / 256, nch), dim3(256), 0, s, img, istride, iw, ih, step,
-                       words, st, ps, pq);
+                       words, st, ps);
+    const bool a16 = ((((uintptr_t)img) | (uintptr_t)istride) & 15) == 0;
+    const bool a8 = ((((uintptr_t)img) | (uintptr_t)istride) & 7) == 0;
     int vmode = 0;
-    if (step == 4 && ((((uintptr_t)img) | (uintptr_t)istride) & 15) == 0) vmode = 1;
-    if (step == 1 && ((((uintptr_t)img) | (uintptr_t)istride) & 7) == 0) vmode = 2;
+    if (iw % NORM_CPT == 0 && step == 4 && a16) vmode = 1;
+    if (iw % NORM_CPT == 0 && step == 1 && a8) vmode = 2;
     dim3 grid((rw + 64 * np.q - 1) / (64 * np.q), (rh + NB_ROWS - 1) / NB_ROWS);
-#define CVM_NW(Q, S)                                                                                         \
-  hipLaunchKernelGGL((k_norm_wave<Q, S>), grid, dim3(64), 0, s, img, istride, iw, step, vmode, tw, th, rw, rh, \
-                     st, ps, pq, corr, map, partials)
+#define CVM_NW(Q, S, V)                                                                                          \
+  hipLaunchKernelGGL((k_norm_wave<Q, S, V>), grid, dim3(64), 0, s, img, istride, iw, ih, step, tw, th, rw, rh, st, \
+                     ps, corr, map, partials)
+#define CVM_NWV(Q, S)   \
+  if (vmode == 1)       \
+    CVM_NW(Q, S, 1);    \
+  else if (vmode == 2)  \
+    CVM_NW(Q, S, 2);    \
+  else                  \
+    CVM_NW(Q, S, 0)
     if (np.q == 4) {
-      if (np.sq32)
-        CVM_NW(4, true);
-      else
-        CVM_NW(4, false);
+      if (np.sq32) {
+        CVM_NWV(4, true);
+      } else {
+        CVM_NWV(4, false);
+      }
     } else {
-      if (np.sq32)
-        CVM_NW(2, true);
-      else
-        CVM_NW(2, false);
+      if (np.sq32) {
+        CVM_NWV(2, true);
+      } else {
+        CVM_NWV(2, false);
+      }
     }
+#undef CVM_NWV
 #undef CVM_NW
     return;
   }

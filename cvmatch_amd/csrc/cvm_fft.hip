@@ -94,17 +94,65 @@ __device__ __forceinline__ int xcd_remap(int p, int g) {
 
 __host__ __device__ constexpr int ceil_log2c(int n) { return n <= 1 ? 0 : 1 + ceil_log2c((n + 1) >> 1); }
 
+// The butterflies of one pass on the E = 2^(HEAD + 2*NST) elements of one
+// group held in registers: r[e] is index base | (e << LO), low = base mod
+// 2^LO.  The first radix-4 stage uses twiddles at tri + (h0 - base), the
+// second at tri + (4*h0 - base).
+template <int LOG2N, int LO, int HEAD, int NST>
+__device__ __forceinline__ void bfly_regs(float2* r, int low, const float2* __restrict__ tri, bool inverse) {
+  constexpr int E = 1 << (HEAD + 2 * NST);
+  constexpr int tbase = (LOG2N & 1) ? 2 : 1;
+  constexpr int h0 = 1 << (LO + HEAD);
+  constexpr int h1 = h0 << 2;
+  if (HEAD) {
+#pragma unroll
+    for (int e = 0; e < E; e += 2) {
+      float2 u = r[e], w = r[e + 1];
+      r[e] = make_float2(u.x + w.x, u.y + w.y);
+      r[e + 1] = make_float2(u.x - w.x, u.y - w.y);
+    }
+  }
+  if (NST >= 1) {
+    constexpr int d = HEAD;
+    const float2* t0 = tri + (h0 - tbase);
+#pragma unroll
+    for (int e = 0; e < E; e++) {
+      if ((e >> d) & 3) continue;
+      int j = low | ((e & ((1 << d) - 1)) << LO);
+      float2 w1 = conj_if(t0[j], inverse), w2 = conj_if(t0[h0 + j], inverse), w3 = conj_if(t0[2 * h0 + j], inverse);
+      bfly4(r[e], r[e | (1 << d)], r[e | (2 << d)], r[e | (3 << d)], w1, w2, w3, inverse);
+    }
+  }
+  if (NST >= 2) {
+    constexpr int d = HEAD + 2;
+    const float2* t1 = tri + (h1 - tbase);
+#pragma unroll
+    for (int e = 0; e < E; e++) {
+      if ((e >> d) & 3) continue;
+      int j = low | ((e & ((1 << d) - 1)) << LO);
+      float2 w1 = conj_if(t1[j], inverse), w2 = conj_if(t1[h1 + j], inverse), w3 = conj_if(t1[2 * h1 + j], inverse);
+      bfly4(r[e], r[e | (1 << d)], r[e | (2 << d)], r[e | (3 << d)], w1, w2, w3, inverse);
+    }
+  }
+}
+
+// Group gg of a pass over bits [LO, LO+NB): its lowest index and low bits.
+template <int LO, int NB>
+__device__ __forceinline__ int grp_base(int gg) {
+  return (gg & ((1 << LO) - 1)) | ((gg >> LO) << (LO + NB));
+}
+
+// One pass in place in LDS over every group of every vector (threads take
+// group-vector pairs, vector index fastest when VFAST).
 template <int LOG2N, int LO, int HEAD, int NST, int NVEC, int NT, int ES, bool PAD, bool VFAST>
-__device__ __forceinline__ void fft_pass(float2* a, int nvec, int vstride, const float2* __restrict__ tri0,
-                                         const float2* __restrict__ tri1, bool inverse) {
+__device__ __forceinline__ void fft_pass(float2* a, int nvec, int vstride, const float2* __restrict__ tri,
+                                         bool inverse) {
   constexpr int NB = HEAD + 2 * NST;
   constexpr int E = 1 << NB;
   constexpr int LGRP = LOG2N - NB;
   constexpr int LVEC = ceil_log2c(NVEC);
   constexpr int TOTAL = 1 << (LGRP + LVEC);
   constexpr int ITERS = (TOTAL + NT - 1) / NT;
-  constexpr int h0 = 1 << (LO + HEAD);
-  constexpr int h1 = h0 << 2;
 #pragma unroll
   for (int it = 0; it < ITERS; it++) {
     const int gi = threadIdx.x + it * NT;
@@ -120,8 +168,7 @@ __device__ __forceinline__ void fft_pass(float2* a, int nvec, int vstride, const
     if (NVEC != (1 << LVEC) || !VFAST) {
       if (v >= nvec) continue;
     }
-    const int low = gg & ((1 << LO) - 1);
-    const int base = low | ((gg >> LO) << (LO + NB));
+    const int base = grp_base<LO, NB>(gg);
     float2* av = a + v * vstride;
     float2 r[E];
 #pragma unroll
@@ -129,36 +176,7 @@ __device__ __forceinline__ void fft_pass(float2* a, int nvec, int vstride, const
       int idx = base | (e << LO);
       r[e] = av[(PAD ? padi(idx) : idx) * ES];
     }
-    if (HEAD) {
-#pragma unroll
-      for (int e = 0; e < E; e += 2) {
-        float2 u = r[e], w = r[e + 1];
-        r[e] = make_float2(u.x + w.x, u.y + w.y);
-        r[e + 1] = make_float2(u.x - w.x, u.y - w.y);
-      }
-    }
-    if (NST >= 1) {
-      constexpr int d = HEAD;
-#pragma unroll
-      for (int e = 0; e < E; e++) {
-        if ((e >> d) & 3) continue;
-        int j = low | ((e & ((1 << d) - 1)) << LO);
-        float2 w1 = conj_if(tri0[j], inverse), w2 = conj_if(tri0[h0 + j], inverse),
-               w3 = conj_if(tri0[2 * h0 + j], inverse);
-        bfly4(r[e], r[e | (1 << d)], r[e | (2 << d)], r[e | (3 << d)], w1, w2, w3, inverse);
-      }
-    }
-    if (NST >= 2) {
-      constexpr int d = HEAD + 2;
-#pragma unroll
-      for (int e = 0; e < E; e++) {
-        if ((e >> d) & 3) continue;
-        int j = low | ((e & ((1 << d) - 1)) << LO);
-        float2 w1 = conj_if(tri1[j], inverse), w2 = conj_if(tri1[h1 + j], inverse),
-               w3 = conj_if(tri1[2 * h1 + j], inverse);
-        bfly4(r[e], r[e | (1 << d)], r[e | (2 << d)], r[e | (3 << d)], w1, w2, w3, inverse);
-      }
-    }
+    bfly_regs<LOG2N, LO, HEAD, NST>(r, gg & ((1 << LO) - 1), tri, inverse);
 #pragma unroll
     for (int e = 0; e < E; e++) {
       int idx = base | (e << LO);
@@ -172,18 +190,12 @@ __device__ __forceinline__ void fft_pass(float2* a, int nvec, int vstride, const
 template <int LOG2N, int LO, int NVEC, int NT, int ES, bool PAD, bool VFAST>
 __device__ __forceinline__ void fft_rest(float2* a, int nvec, int vstride, const float2* __restrict__ tri,
                                          bool inverse) {
-  constexpr int base = (LOG2N & 1) ? 2 : 1;
   constexpr int SREM = (LOG2N - LO) / 2;
-  if constexpr (SREM >= 2) {
-    constexpr int h0 = 1 << LO;
-    fft_pass<LOG2N, LO, 0, 2, NVEC, NT, ES, PAD, VFAST>(a, nvec, vstride, tri + (h0 - base), tri + (4 * h0 - base),
-                                                        inverse);
+  if constexpr (SREM >= 1) {
+    constexpr int NST = SREM >= 2 ? 2 : 1;
+    fft_pass<LOG2N, LO, 0, NST, NVEC, NT, ES, PAD, VFAST>(a, nvec, vstride, tri, inverse);
     __syncthreads();
-    fft_rest<LOG2N, LO + 4, NVEC, NT, ES, PAD, VFAST>(a, nvec, vstride, tri, inverse);
-  } else if constexpr (SREM == 1) {
-    constexpr int h0 = 1 << LO;
-    fft_pass<LOG2N, LO, 0, 1, NVEC, NT, ES, PAD, VFAST>(a, nvec, vstride, tri + (h0 - base), tri, inverse);
-    __syncthreads();
+    fft_rest<LOG2N, LO + 2 * NST, NVEC, NT, ES, PAD, VFAST>(a, nvec, vstride, tri, inverse);
   }
 }
 
@@ -193,15 +205,11 @@ template <int LOG2N, int NVEC, int NT, int ES, bool PAD, bool VFAST>
 __device__ __forceinline__ void fft_lds(float2* a, int nvec, int vstride, const float2* __restrict__ tri,
                                         bool inverse) {
   if constexpr (LOG2N & 1) {
-    if constexpr (LOG2N >= 3) {
-      // head + the h = 2 stage (twiddles at offset 0)
-      fft_pass<LOG2N, 0, 1, 1, NVEC, NT, ES, PAD, VFAST>(a, nvec, vstride, tri, tri, inverse);
-      __syncthreads();
-      fft_rest<LOG2N, 3, NVEC, NT, ES, PAD, VFAST>(a, nvec, vstride, tri, inverse);
-    } else {
-      fft_pass<LOG2N, 0, 1, 0, NVEC, NT, ES, PAD, VFAST>(a, nvec, vstride, tri, tri, inverse);
-      __syncthreads();
-    }
+    // head + the h = 2 stage (a lone head for n = 2)
+    constexpr int NST = LOG2N >= 3 ? 1 : 0;
+    fft_pass<LOG2N, 0, 1, NST, NVEC, NT, ES, PAD, VFAST>(a, nvec, vstride, tri, inverse);
+    __syncthreads();
+    fft_rest<LOG2N, 1 + 2 * NST, NVEC, NT, ES, PAD, VFAST>(a, nvec, vstride, tri, inverse);
   } else {
     fft_rest<LOG2N, 0, NVEC, NT, ES, PAD, VFAST>(a, nvec, vstride, tri, inverse);
   }
@@ -328,7 +336,7 @@ template <int LOG2N, int NVEC>
 __global__ __launch_bounds__(RowNT<LOG2N>::value) void k_row_fwd(
     const uint8_t* __restrict__ img, int64_t istride, int step, const CallState* __restrict__ st, TileGrid g,
     const float2* __restrict__ triW, float2* __restrict__ spec, int64_t tileStride, int64_t chanStride, int cpg,
-    int ngroups, int npairs) {
+    int ngroups, int npairs, int words) {
   constexpr int n = 1 << LOG2N, NT = RowNT<LOG2N>::value, np = n + (n >> 5);
   extern __shared__ float2 z[];
   const int L = xcd_remap(blockIdx.x, gridDim.x);
@@ -350,15 +358,34 @@ __global__ __launch_bounds__(RowNT<LOG2N>::value) void k_row_fwd(
   // all of this thread's pixels are loaded before any is stored
   constexpr int MAXI = (n + NT - 1) / NT;
   uint32_t wa[MAXI], wb[MAXI];
+  if (NVEC == 4 && words && k0 == 0) {
+    // four-byte pixels, one dword load each (bytes of channels >= nv are
+    // never used)
+    const int64_t rb = two ? istride : 0;
 #pragma unroll
-  for (int i = 0; i < MAXI; i++) {
-    int x = tid + i * NT;
-    wa[i] = wb[i] = 0;
-    if (x < loadW) {
-      const uint8_t* pa = ra + (int64_t)x * step;
-      for (int v = 0; v < nv; v++) {
-        wa[i] |= (uint32_t)pa[v] << (8 * v);
-        if (two) wb[i] |= (uint32_t)pa[istride + v] << (8 * v);
+    for (int i = 0; i < MAXI; i++) {
+      int x = tid + i * NT;
+      wa[i] = wb[i] = 0;
+      if (x < loadW) {
+        const uint8_t* pa = ra + (int64_t)x * 4;
+        wa[i] = *(const uint32_t*)pa;
+        wb[i] = *(const uint32_t*)(pa + rb);
+      }
+    }
+    if (!two)
+#pragma unroll
+      for (int i = 0; i < MAXI; i++) wb[i] = 0;
+  } else {
+#pragma unroll
+    for (int i = 0; i < MAXI; i++) {
+      int x = tid + i * NT;
+      wa[i] = wb[i] = 0;
+      if (x < loadW) {
+        const uint8_t* pa = ra + (int64_t)x * step;
+        for (int v = 0; v < nv; v++) {
+          wa[i] |= (uint32_t)pa[v] << (8 * v);
+          if (two) wb[i] |= (uint32_t)pa[istride + v] << (8 * v);
+        }
       }
     }
   }
@@ -401,15 +428,16 @@ static void row_fwd_n(const uint8_t* img, int64_t istride, int step, int cnMax, 
   int npairs = g.dftH / 2;
   size_t lds = sizeof(float2) * (size_t)padded_len(n) * cpg;
   dim3 grid(ngroups * npairs * ntiles), block(RowNT<LOG2N>::value);
+  int words = (step == 4 && ((((uintptr_t)img) | (uintptr_t)istride) & 3) == 0) ? 1 : 0;
   if (cpg <= 1)
     hipLaunchKernelGGL((k_row_fwd<LOG2N, 1>), grid, block, lds, s, img, istride, step, st, g, pt.triW, spec,
-                       tileStride, chanStride, cpg, ngroups, npairs);
+                       tileStride, chanStride, cpg, ngroups, npairs, words);
   else if (cpg == 2)
     hipLaunchKernelGGL((k_row_fwd<LOG2N, 2>), grid, block, lds, s, img, istride, step, st, g, pt.triW, spec,
-                       tileStride, chanStride, cpg, ngroups, npairs);
+                       tileStride, chanStride, cpg, ngroups, npairs, words);
   else
     hipLaunchKernelGGL((k_row_fwd<LOG2N, 4>), grid, block, lds, s, img, istride, step, st, g, pt.triW, spec,
-                       tileStride, chanStride, cpg, ngroups, npairs);
+                       tileStride, chanStride, cpg, ngroups, npairs, words);
 }
 
 void launch_row_fwd(const uint8_t* img, int64_t istride, int step, int cnMax, const CallState* st,
@@ -436,27 +464,66 @@ struct ColCG {
   static constexpr int value = LOG2N <= 9 ? 16 : (1 << (13 - LOG2N));
 };
 
-// One workgroup owns CG spectrum columns of one tile/channel for the whole
-// column height.  Modes:
-//   COL_TSPEC : load rows (pad rows = 0), forward, scale, store all rows
-//   COL_FWD   : load rows (pad rows = 0), forward, store all rows
-//   COL_CORR  : load rows, forward, *conj(tspec), inverse, store needed rows
-//   COL_MULINV: load all rows (already forward), *conj(tspec), inverse, store
-// The forward input of row r goes to LDS row brev[r] (rows at bit-reversed
-// slots, blockForwardGo impl.go:350-374); after the forward cascade LDS row
-// k holds frequency k.  The product is written to LDS row brev[k] so the
-// inverse cascade again sees bit-reversed input (the rmap = brev relabelling
-// of colsR4Go), and leaves spatial row y at LDS row y.  Logical workgroup
-// order is tile-fastest, so the tiles that read one template-spectrum slab
-// run together on one XCD.  One launch covers tiles [t0, t0 + ntiles), all of
-// one column height 2^LOG2N.
+// Pass schedule of a column transform.  The first pass takes the radix-2
+// head (odd sizes) and up to two radix-4 stages (32 or 16 points per group);
+// every later pass two stages, the last one one or two.
 template <int LOG2N>
-__global__ __launch_bounds__(256) void k_cols(int mode, const float2* __restrict__ src, float2* __restrict__ dst,
+struct ColFirst {
+  static constexpr int HEAD = LOG2N & 1;
+  static constexpr int NST = (LOG2N - HEAD) / 2 >= 2 ? 2 : (LOG2N - HEAD) / 2;
+  static constexpr int NB = HEAD + 2 * NST;
+};
+
+// Lowest index bit of the last pass when the passes after the first start
+// at bit lo (remaining bit counts are even).
+__host__ __device__ constexpr int col_last_lo(int log2n, int lo) {
+  return log2n - lo <= 4 ? lo : col_last_lo(log2n, lo + 4);
+}
+
+// The two-stage passes from bit LO up to (not including) the last pass, in
+// place in the LDS slab, each followed by a barrier.
+template <int LOG2N, int LO, int LOL, int CG>
+__device__ __forceinline__ void col_mid(float2* a, const float2* __restrict__ tri, bool inverse) {
+  if constexpr (LO < LOL) {
+    fft_pass<LOG2N, LO, 0, 2, CG, 256, CG, false, true>(a, CG, 1, tri, inverse);
+    __syncthreads();
+    col_mid<LOG2N, LO + 4, LOL, CG>(a, tri, inverse);
+  }
+}
+
+// One workgroup owns CG spectrum columns of one tile/channel for the whole
+// column height; threads take (group, column) pairs with the column fastest,
+// so each row's CG columns are one contiguous access.  Modes:
+//   COL_TSPEC : rows (pad rows = 0) -> forward -> scaled, all rows stored
+//   COL_FWD   : rows (pad rows = 0) -> forward, all rows stored
+//   COL_CORR  : rows -> forward -> * conj(tspec) -> inverse, needed rows
+//   COL_MULINV: forward rows -> * conj(tspec) -> inverse, needed rows
+// The forward input in bit-reversed row order (blockForwardGo
+// impl.go:350-374) is read straight from the spectrum by the first pass:
+// transform index i is source row brev(i).  The product for frequency k
+// (mulConjGo impl.go:376-388) goes to slab row brev(k), the bit-reversed
+// input of the inverse (the rmap = brev relabelling of colsR4Go), whose last
+// pass writes spatial rows straight to the spectrum.  The template spectrum
+// values of the last forward pass are loaded before the first pass starts.
+// Logical workgroup order is tile-fastest, so the tiles that read one
+// template-spectrum slab run together on one XCD.  One launch covers tiles
+// [t0, t0 + ntiles), all of column height 2^LOG2N.
+template <int LOG2N, int MODE>
+__global__ __launch_bounds__(256) void k_cols(const float2* __restrict__ src, float2* __restrict__ dst,
                                               int64_t srcTileStride, int64_t dstTileStride, int64_t chanStride,
                                               const CallState* __restrict__ st, TileGrid g,
                                               const float2* __restrict__ tri, const float2* __restrict__ tspec,
                                               int64_t tchanStride, float scale, int t0, int ntiles, int nslabs) {
-  constexpr int n = 1 << LOG2N, CG = ColCG<LOG2N>::value, NT = 256, TOT = n * CG;
+  constexpr int n = 1 << LOG2N, CG = ColCG<LOG2N>::value, LCG = ceil_log2c(CG), NT = 256;
+  using F = ColFirst<LOG2N>;
+  constexpr int NB1 = F::NB, E1 = 1 << NB1;
+  constexpr int G1 = CG << (LOG2N - NB1), IT1 = (G1 + NT - 1) / NT;
+  constexpr bool SINGLE = NB1 == LOG2N;
+  constexpr int LOL = SINGLE ? 0 : col_last_lo(LOG2N, NB1);
+  constexpr int HEADL = SINGLE ? F::HEAD : 0;
+  constexpr int NSTL = SINGLE ? F::NST : (LOG2N - LOL) / 2;
+  constexpr int NBL = HEADL + 2 * NSTL, EL = 1 << NBL;
+  constexpr int GL = CG << (LOG2N - NBL), ITL = (GL + NT - 1) / NT;
   extern __shared__ float2 a[];
   const int L = xcd_remap(blockIdx.x, gridDim.x);
   const int t = t0 + L % ntiles, rest = L / ntiles;
@@ -465,96 +532,160 @@ __global__ __launch_bounds__(256) void k_cols(int mode, const float2* __restrict
   if (st->flat || k >= st->cn || c0 >= g.hw) return;
   int x0, y0, bw, bh;
   tile_rect(g, t, x0, y0, bw, bh);
-  const float2* ts = tspec + (int64_t)k * tchanStride;
+  const float2* ts = tspec + (int64_t)k * tchanStride + c0;
   const int tid = threadIdx.x;
   const int hwPad = g.hwPad;
   const float2* sp = src + (int64_t)t * srcTileStride + (int64_t)k * chanStride + c0;
   float2* dp = dst + (int64_t)t * dstTileStride + (int64_t)k * chanStride + c0;
-
-  int loaded = n;
-  if (mode != COL_MULINV) {
-    int loadH = bh + g.padH;
-    loaded = (loadH + 1) & ~1;
-    if (loaded > n) loaded = n;
-  }
   const int ncol = (g.hw - c0) < CG ? (g.hw - c0) : CG;
-  constexpr int LB = 8;  // loads in flight per thread
-#pragma unroll
-  for (int b0 = 0; b0 < TOT; b0 += NT * LB) {
-    float2 v[LB];
-#pragma unroll
-    for (int u = 0; u < LB; u++) {
-      int idx = b0 + u * NT + tid;
-      int r = idx / CG, c = idx % CG;
-      v[u] = make_float2(0.f, 0.f);
-      if (idx < TOT && r < loaded && c < ncol) v[u] = sp[(int64_t)r * hwPad + c];
-    }
-#pragma unroll
-    for (int u = 0; u < LB; u++) {
-      int idx = b0 + u * NT + tid;
-      if (idx < TOT) {
-        int r = idx / CG, c = idx % CG;
-        int pos = (mode == COL_MULINV) ? r : bitrev_idx(r, LOG2N);
-        a[pos * CG + c] = v[u];
-      }
-    }
-  }
-  __syncthreads();
-  if (mode != COL_MULINV) fft_lds<LOG2N, CG, NT, CG, false, true>(a, CG, 1, tri, false);
-
-  if (mode == COL_TSPEC || mode == COL_FWD) {
-#pragma unroll 4
-    for (int idx = tid; idx < TOT; idx += NT) {
-      int r = idx / CG, c = idx % CG;
-      if (c < ncol) {
-        float2 v = a[r * CG + c];
-        if (mode == COL_TSPEC) v = make_float2(v.x * scale, v.y * scale);
-        dp[(int64_t)r * hwPad + c] = v;
-      }
-    }
-    return;
-  }
-
-  // conjugate product (mulConjGo impl.go:376-388) fused with the bit
-  // reversal: rows k and brev[k] are handled by the same thread.
-  constexpr int MB = 4;
-#pragma unroll
-  for (int b0 = 0; b0 < TOT; b0 += NT * MB) {
-    float2 tx[MB], ty[MB];
-#pragma unroll
-    for (int u = 0; u < MB; u++) {
-      int idx = b0 + u * NT + tid;
-      int r = idx / CG, c = idx % CG;
-      int br = bitrev_idx(r, LOG2N);
-      tx[u] = ty[u] = make_float2(0.f, 0.f);
-      if (idx < TOT && r <= br && c < ncol) {
-        tx[u] = ts[(int64_t)r * hwPad + c0 + c];
-        ty[u] = ts[(int64_t)br * hwPad + c0 + c];
-      }
-    }
-#pragma unroll
-    for (int u = 0; u < MB; u++) {
-      int idx = b0 + u * NT + tid;
-      int r = idx / CG, c = idx % CG;
-      int br = bitrev_idx(r, LOG2N);
-      if (idx < TOT && r <= br) {
-        float2 x = a[r * CG + c], y = a[br * CG + c];
-        float p0 = x.x * tx[u].x, p1 = x.y * tx[u].y, p2 = x.y * tx[u].x, p3 = x.x * tx[u].y;
-        float q0 = y.x * ty[u].x, q1 = y.y * ty[u].y, q2 = y.y * ty[u].x, q3 = y.x * ty[u].y;
-        a[br * CG + c] = make_float2(p0 + p1, p2 - p3);
-        a[r * CG + c] = make_float2(q0 + q1, q2 - q3);
-      }
-    }
-  }
-  __syncthreads();
-  fft_lds<LOG2N, CG, NT, CG, false, true>(a, CG, 1, tri, true);
-
   int need = (bh + 1) & ~1;
   if (need > n) need = n;
-#pragma unroll 4
-  for (int idx = tid; idx < TOT; idx += NT) {
-    int r = idx / CG, c = idx % CG;
-    if (r < need && c < ncol) dp[(int64_t)r * hwPad + c] = a[r * CG + c];
+
+  // template spectrum for the elements this thread owns in the last
+  // forward pass
+  float2 tsv[MODE == COL_CORR ? ITL : 1][MODE == COL_CORR ? EL : 1];
+  if constexpr (MODE == COL_CORR) {
+#pragma unroll
+    for (int it = 0; it < ITL; it++) {
+      const int gi = tid + it * NT, v = gi & (CG - 1), gg = gi >> LCG;
+      const int base = grp_base<LOL, NBL>(gg);
+#pragma unroll
+      for (int e = 0; e < EL; e++) {
+        const int idx = base | (e << LOL);
+        tsv[it][e] = make_float2(0.f, 0.f);
+        if ((GL % NT == 0 || gi < GL) && v < ncol) tsv[it][e] = ts[(int64_t)idx * hwPad + v];
+      }
+    }
+  }
+
+  float2 rl[ITL][EL];
+  if constexpr (MODE != COL_MULINV) {
+    int loaded = (bh + g.padH + 1) & ~1;
+    if (loaded > n) loaded = n;
+    // first forward pass straight from the spectrum rows
+    float2 r1[IT1][E1];
+#pragma unroll
+    for (int it = 0; it < IT1; it++) {
+      const int gi = tid + it * NT, v = gi & (CG - 1), gg = gi >> LCG;
+      const bool ok = (G1 % NT == 0 || gi < G1) && v < ncol;
+#pragma unroll
+      for (int e = 0; e < E1; e++) {
+        const int row = bitrev_idx((gg << NB1) | e, LOG2N);
+        r1[it][e] = make_float2(0.f, 0.f);
+        if (ok && row < loaded) r1[it][e] = sp[(int64_t)row * hwPad + v];
+      }
+    }
+#pragma unroll
+    for (int it = 0; it < IT1; it++) bfly_regs<LOG2N, 0, F::HEAD, F::NST>(r1[it], 0, tri, false);
+    if constexpr (SINGLE) {
+#pragma unroll
+      for (int it = 0; it < ITL; it++)
+#pragma unroll
+        for (int e = 0; e < EL; e++) rl[it][e] = r1[it][e];
+    } else {
+#pragma unroll
+      for (int it = 0; it < IT1; it++) {
+        const int gi = tid + it * NT, v = gi & (CG - 1), gg = gi >> LCG;
+        if (G1 % NT == 0 || gi < G1)
+#pragma unroll
+          for (int e = 0; e < E1; e++) a[((gg << NB1) | e) * CG + v] = r1[it][e];
+      }
+      __syncthreads();
+      col_mid<LOG2N, NB1, LOL, CG>(a, tri, false);
+#pragma unroll
+      for (int it = 0; it < ITL; it++) {
+        const int gi = tid + it * NT, v = gi & (CG - 1), gg = gi >> LCG;
+        const int base = grp_base<LOL, NBL>(gg);
+        if (GL % NT == 0 || gi < GL) {
+#pragma unroll
+          for (int e = 0; e < EL; e++) rl[it][e] = a[(base | (e << LOL)) * CG + v];
+          bfly_regs<LOG2N, LOL, 0, NSTL>(rl[it], gg & ((1 << LOL) - 1), tri, false);
+        }
+      }
+    }
+    if constexpr (MODE == COL_TSPEC || MODE == COL_FWD) {
+#pragma unroll
+      for (int it = 0; it < ITL; it++) {
+        const int gi = tid + it * NT, v = gi & (CG - 1), gg = gi >> LCG;
+        const int base = grp_base<LOL, NBL>(gg);
+        if ((GL % NT == 0 || gi < GL) && v < ncol)
+#pragma unroll
+          for (int e = 0; e < EL; e++) {
+            float2 val = rl[it][e];
+            if (MODE == COL_TSPEC) val = make_float2(val.x * scale, val.y * scale);
+            dp[(int64_t)(base | (e << LOL)) * hwPad + v] = val;
+          }
+      }
+      return;
+    }
+    // conjugate product at frequency idx, to slab row brev(idx) once every
+    // thread has read its last-pass inputs
+#pragma unroll
+    for (int it = 0; it < ITL; it++)
+#pragma unroll
+      for (int e = 0; e < EL; e++) {
+        const float2 x = rl[it][e], w = tsv[it][e];
+        const float p0 = x.x * w.x, p1 = x.y * w.y, p2 = x.y * w.x, p3 = x.x * w.y;
+        rl[it][e] = make_float2(p0 + p1, p2 - p3);
+      }
+    __syncthreads();
+#pragma unroll
+    for (int it = 0; it < ITL; it++) {
+      const int gi = tid + it * NT, v = gi & (CG - 1), gg = gi >> LCG;
+      const int base = grp_base<LOL, NBL>(gg);
+      if (GL % NT == 0 || gi < GL)
+#pragma unroll
+        for (int e = 0; e < EL; e++) a[bitrev_idx(base | (e << LOL), LOG2N) * CG + v] = rl[it][e];
+    }
+  } else {
+    // forward rows of a fleet frame: product straight from global memory
+    constexpr int TOT = n * CG, PER = (TOT + NT - 1) / NT, LB = PER < 16 ? PER : 16;
+#pragma unroll
+    for (int b0 = 0; b0 < PER; b0 += LB) {
+      float2 x[LB], w[LB];
+#pragma unroll
+      for (int u = 0; u < LB; u++) {
+        const int idx = (b0 + u) * NT + tid, r = idx >> LCG, c = idx & (CG - 1);
+        x[u] = w[u] = make_float2(0.f, 0.f);
+        if ((TOT % NT == 0 || idx < TOT) && c < ncol) {
+          x[u] = sp[(int64_t)r * hwPad + c];
+          w[u] = ts[(int64_t)r * hwPad + c];
+        }
+      }
+#pragma unroll
+      for (int u = 0; u < LB; u++) {
+        const int idx = (b0 + u) * NT + tid, r = idx >> LCG, c = idx & (CG - 1);
+        if (TOT % NT == 0 || idx < TOT) {
+          const float p0 = x[u].x * w[u].x, p1 = x[u].y * w[u].y, p2 = x[u].y * w[u].x, p3 = x[u].x * w[u].y;
+          a[bitrev_idx(r, LOG2N) * CG + c] = make_float2(p0 + p1, p2 - p3);
+        }
+      }
+    }
+  }
+  __syncthreads();
+
+  // inverse: the passes in LDS, the last one to the spectrum rows
+  if constexpr (!SINGLE) {
+    fft_pass<LOG2N, 0, F::HEAD, F::NST, CG, NT, CG, false, true>(a, CG, 1, tri, true);
+    __syncthreads();
+    col_mid<LOG2N, NB1, LOL, CG>(a, tri, true);
+  }
+#pragma unroll
+  for (int it = 0; it < ITL; it++) {
+    const int gi = tid + it * NT, v = gi & (CG - 1), gg = gi >> LCG;
+    const int base = grp_base<LOL, NBL>(gg);
+    if (GL % NT == 0 || gi < GL) {
+      float2 r[EL];
+#pragma unroll
+      for (int e = 0; e < EL; e++) r[e] = a[(base | (e << LOL)) * CG + v];
+      bfly_regs<LOG2N, LOL, HEADL, NSTL>(r, gg & ((1 << LOL) - 1), tri, true);
+      if (v < ncol)
+#pragma unroll
+        for (int e = 0; e < EL; e++) {
+          const int y = base | (e << LOL);
+          if (y < need) dp[(int64_t)y * hwPad + v] = r[e];
+        }
+    }
   }
 }
 
@@ -565,9 +696,17 @@ static void cols_n(ColMode mode, const float2* src, float2* dst, int64_t srcTile
   constexpr int CG = ColCG<LOG2N>::value;
   size_t lds = sizeof(float2) * (size_t)(1 << LOG2N) * CG;
   int nslabs = (g.hw + CG - 1) / CG;
-  hipLaunchKernelGGL((k_cols<LOG2N>), dim3(nslabs * ntiles * cnMax), dim3(256), lds, s, (int)mode, src, dst,
-                     srcTileStride, dstTileStride, chanStride, st, g, tri, tspec, tchanStride, scale, t0, ntiles,
-                     nslabs);
+  dim3 grid(nslabs * ntiles * cnMax), block(256);
+#define CVM_CM(M)                                                                                                \
+  hipLaunchKernelGGL((k_cols<LOG2N, M>), grid, block, lds, s, src, dst, srcTileStride, dstTileStride, chanStride, \
+                     st, g, tri, tspec, tchanStride, scale, t0, ntiles, nslabs)
+  switch (mode) {
+    case COL_TSPEC: CVM_CM(COL_TSPEC); break;
+    case COL_FWD: CVM_CM(COL_FWD); break;
+    case COL_CORR: CVM_CM(COL_CORR); break;
+    case COL_MULINV: CVM_CM(COL_MULINV); break;
+  }
+#undef CVM_CM
 }
 
 static void cols_dispatch(int log2n, ColMode mode, const float2* src, float2* dst, int64_t srcTileStride,

@@ -183,6 +183,10 @@ struct Engine {
   std::recursive_mutex mu;
   bool ready = false;
   hipStream_t stream = nullptr;
+  // side stream of a MatchMap call, forked from and joined back into the
+  // caller's stream by the two events
+  hipStream_t side = nullptr;
+  hipEvent_t fork = nullptr, join = nullptr;
   std::map<int, FftTables> tables;
   DevBuf img, tpl, spec, tspec, tspec2, corr, parts, gray0, gray1, mapbuf, nscratch;
   CallState* st = nullptr;
@@ -199,6 +203,9 @@ struct Engine {
     HIPCHK(hipGetDeviceCount(&ndev));
     if (ndev < 1) return fail(CVM_E_HIP, "cvmatch: no HIP device available");
     HIPCHK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+    HIPCHK(hipStreamCreateWithFlags(&side, hipStreamNonBlocking));
+    HIPCHK(hipEventCreateWithFlags(&fork, hipEventDisableTiming));
+    HIPCHK(hipEventCreateWithFlags(&join, hipEventDisableTiming));
     HIPCHK(hipMalloc(&st, sizeof(CallState)));
     HIPCHK(hipMalloc(&res_d, sizeof(Res6) * 2));
     HIPCHK(hipHostMalloc(&res_h, sizeof(Res6) * 2, hipHostMallocDefault));
@@ -357,9 +364,14 @@ static int build_tspec(Engine& E, hipStream_t s, const CallState* st, const uint
 // Correlation + normalisation + extrema for one parent against a prepared
 // template spectrum (crossCorrGo + normalizeParallelGo, impl.go:1150-1154).
 // CallState must be finalised on the stream before this runs.
+// With join set, the call state, the template spectrum and the
+// normalisation partials are still being built on another stream that
+// records join when done: the parent's row transforms (which need none of
+// them) run first, then s waits for join.  Otherwise all of them are ready
+// on s, except the partials, built here.
 static int corr_and_normalize(Engine& E, hipStream_t s, const CallState* st, const uint8_t* d_img, int64_t istride,
                               int iw, int ih, int tw, int th, int step, int cnMax, const TplSpec& ts,
-                              const PlanTables& pt, float* d_map, Res6* d_res) {
+                              const PlanTables& pt, float* d_map, Res6* d_res, hipEvent_t join) {
   const Plan& p = ts.p;
   int rw = iw - tw + 1, rh = ih - th + 1;
   int hwPad = ts.hwPad;
@@ -388,8 +400,9 @@ static int corr_and_normalize(Engine& E, hipStream_t s, const CallState* st, con
   float2* spec = E.spec.as<float2>();
   {
     Stage S(E, "row_fwd", s);
-    launch_row_fwd(d_img, istride, step, cnMax, st, g, pt, spec, tileStride, chan, ntiles, s);
+    launch_row_fwd(d_img, istride, step, cnMax, join ? nullptr : st, g, pt, spec, tileStride, chan, ntiles, s);
   }
+  if (join) HIPCHK(hipStreamWaitEvent(s, join, 0));
   {
     Stage S(E, "cols", s);
     launch_cols(COL_CORR, spec, spec, tileStride, tileStride, chan, cnMax, st, g, pt, ts.spec, ts.spec2, tchan,
@@ -406,7 +419,11 @@ static int corr_and_normalize(Engine& E, hipStream_t s, const CallState* st, con
   }
   int nparts = norm_partials_count(rw, rh, tw, th, cnMax);
   CHK(E.parts.ensure(sizeof(Extrema) * (size_t)nparts));
-  CHK(E.nscratch.ensure(norm_scratch_bytes(iw, ih, tw, th, cnMax)));
+  if (!join) {
+    CHK(E.nscratch.ensure(norm_scratch_bytes(iw, ih, tw, th, cnMax)));
+    Stage S(E, "norm_prep", s);
+    launch_norm_prep(d_img, istride, iw, ih, step, tw, th, cnMax, st, E.nscratch.p, s);
+  }
   {
     Stage S(E, "normalize", s);
     launch_normalize(d_img, istride, iw, ih, step, tw, th, rw, rh, cnMax, st, corr, d_map, E.parts.as<Extrema>(),
@@ -441,16 +458,8 @@ static int run_match(Engine& E, hipStream_t s, const uint8_t* d_img, int64_t ist
   PlanTables pt;
   CHK(plan_tables(E, p, dh2, pt));
   int cnMax = cn_mode ? cn_mode : 4;
-  {
-    Stage S(E, "stats", s);
-    launch_state_init(E.st, s);
-    launch_tpl_stats(d_tpl, tstride, tw, th, cnMax, step, E.st, s);
-    if (cn_mode == 0) {
-      launch_alpha_scan(d_img, istride, iw, ih, E.st, 0, s);
-      launch_alpha_scan(d_tpl, tstride, tw, th, E.st, 1, s);
-    }
-    launch_state_finalize(E.st, tw, th, cn_mode, s);
-  }
+  // side-stream buffers first: no allocation lands between fork and join
+  CHK(E.nscratch.ensure(norm_scratch_bytes(iw, ih, tw, th, cnMax)));
   TplSpec ts;
   ts.p = p;
   ts.dh2 = dh2;
@@ -463,8 +472,28 @@ static int run_match(Engine& E, hipStream_t s, const uint8_t* d_img, int64_t ist
     CHK(E.tspec2.ensure(sizeof(float2) * tchan2 * cnMax));
     ts.spec2 = E.tspec2.as<float2>();
   }
-  CHK(build_tspec(E, s, E.st, d_tpl, tstride, tw, th, step, cnMax, p, dh2, pt, ts.spec, ts.spec2));
-  return corr_and_normalize(E, s, E.st, d_img, istride, iw, ih, tw, th, step, cnMax, ts, pt, d_map, E.res_d);
+  // fork: the call state, the template spectrum and the normalisation
+  // partials are built on the side stream while s transforms the parent rows
+  HIPCHK(hipEventRecord(E.fork, s));
+  HIPCHK(hipStreamWaitEvent(E.side, E.fork, 0));
+  hipStream_t q = E.side;
+  {
+    Stage S(E, "stats", q);
+    launch_state_init(E.st, q);
+    launch_tpl_stats(d_tpl, tstride, tw, th, cnMax, step, E.st, q);
+    if (cn_mode == 0) {
+      launch_alpha_scan(d_img, istride, iw, ih, E.st, 0, q);
+      launch_alpha_scan(d_tpl, tstride, tw, th, E.st, 1, q);
+    }
+    launch_state_finalize(E.st, tw, th, cn_mode, q);
+  }
+  CHK(build_tspec(E, q, E.st, d_tpl, tstride, tw, th, step, cnMax, p, dh2, pt, ts.spec, ts.spec2));
+  {
+    Stage S(E, "norm_prep", q);
+    launch_norm_prep(d_img, istride, iw, ih, step, tw, th, cnMax, E.st, E.nscratch.p, q);
+  }
+  HIPCHK(hipEventRecord(E.join, q));
+  return corr_and_normalize(E, s, E.st, d_img, istride, iw, ih, tw, th, step, cnMax, ts, pt, d_map, E.res_d, E.join);
 }
 
 static int finish_result(Engine& E, hipStream_t s, cvm_result* out) {
@@ -666,7 +695,7 @@ static int matcher_find_dev(Engine& E, hipStream_t s, cvm_matcher* m, const uint
   }
   matcher_state(E, s, E.st, m, d_img, istride, iw, ih);
   return corr_and_normalize(E, s, E.st, d_img, istride, iw, ih, m->w, m->h, m->step, m->step, g->ts, pt, nullptr,
-                            d_res);
+                            d_res, nullptr);
 }
 
 // --------------------------------------------------------------- Fleet --
@@ -912,6 +941,7 @@ static int fleet_find_dev(Engine& E, hipStream_t s, cvm_fleet* f, const uint8_t*
       CHK(E.nscratch.ensure(norm_scratch_bytes(pw, ph, m->w, m->h, cn)));
       {
         Stage S(E, "fleet_normalize", s);
+        launch_norm_prep(d_img, istride, pw, ph, f->step, m->w, m->h, cn, mst, E.nscratch.p, s);
         launch_normalize(d_img, istride, pw, ph, f->step, m->w, m->h, rw, rh, cn, mst, corr, nullptr,
                          E.parts.as<Extrema>(), E.nscratch.p, s);
         launch_reduce(E.parts.as<Extrema>(), nparts, mst, rw, d_res + mi, nullptr, rh, s);

@@ -312,10 +312,16 @@ void launch_twiddles(float2* tri, int32_t* brev, int n, hipStream_t s) {
 
 // ------------------------------------------------------ forward row FFTs --
 
-// Threads for a row transform of 2^LOG2N points.
+// Threads for a row transform of 2^LOG2N points (the forward kernel, with
+// no per-thread accumulators, takes 512 from 2048 points on: more waves per
+// CU behind the same LDS).
 template <int LOG2N>
 struct RowNT {
   static constexpr int value = LOG2N >= 10 ? 256 : 64;
+};
+template <int LOG2N>
+struct RowNTF {
+  static constexpr int value = LOG2N >= 11 ? 512 : RowNT<LOG2N>::value;
 };
 
 // Channels per workgroup for row transforms of length n: as many as fit a
@@ -333,18 +339,19 @@ static int row_cpg(int n, int cnMax) {
 // column kernel applies the bit reversal when it loads).  One workgroup
 // handles up to NVEC channels of one row pair.
 template <int LOG2N, int NVEC>
-__global__ __launch_bounds__(RowNT<LOG2N>::value) void k_row_fwd(
+__global__ __launch_bounds__(RowNTF<LOG2N>::value) void k_row_fwd(
     const uint8_t* __restrict__ img, int64_t istride, int step, const CallState* __restrict__ st, TileGrid g,
     const float2* __restrict__ triW, float2* __restrict__ spec, int64_t tileStride, int64_t chanStride, int cpg,
-    int ngroups, int npairs, int words) {
-  constexpr int n = 1 << LOG2N, NT = RowNT<LOG2N>::value, np = n + (n >> 5);
+    int ngroups, int npairs, int words, int cnAll) {
+  constexpr int n = 1 << LOG2N, NT = RowNTF<LOG2N>::value, np = n + (n >> 5);
   extern __shared__ float2 z[];
   const int L = xcd_remap(blockIdx.x, gridDim.x);
   const int cgi = L % ngroups, rest = L / ngroups;
   const int rp = rest % npairs, t = rest / npairs;
-  if (st->flat) return;
+  // without a call state every channel of the buffer is transformed
+  if (st && st->flat) return;
   const int k0 = cgi * cpg;
-  const int nv = min(cpg, st->cn - k0);
+  const int nv = min(cpg, (st ? st->cn : cnAll) - k0);
   if (nv <= 0) return;
   int x0, y0, bw, bh;
   tile_rect(g, t, x0, y0, bw, bh);
@@ -427,17 +434,17 @@ static void row_fwd_n(const uint8_t* img, int64_t istride, int step, int cnMax, 
   int ngroups = (cnMax + cpg - 1) / cpg;
   int npairs = g.dftH / 2;
   size_t lds = sizeof(float2) * (size_t)padded_len(n) * cpg;
-  dim3 grid(ngroups * npairs * ntiles), block(RowNT<LOG2N>::value);
+  dim3 grid(ngroups * npairs * ntiles), block(RowNTF<LOG2N>::value);
   int words = (step == 4 && ((((uintptr_t)img) | (uintptr_t)istride) & 3) == 0) ? 1 : 0;
   if (cpg <= 1)
     hipLaunchKernelGGL((k_row_fwd<LOG2N, 1>), grid, block, lds, s, img, istride, step, st, g, pt.triW, spec,
-                       tileStride, chanStride, cpg, ngroups, npairs, words);
+                       tileStride, chanStride, cpg, ngroups, npairs, words, cnMax);
   else if (cpg == 2)
     hipLaunchKernelGGL((k_row_fwd<LOG2N, 2>), grid, block, lds, s, img, istride, step, st, g, pt.triW, spec,
-                       tileStride, chanStride, cpg, ngroups, npairs, words);
+                       tileStride, chanStride, cpg, ngroups, npairs, words, cnMax);
   else
     hipLaunchKernelGGL((k_row_fwd<LOG2N, 4>), grid, block, lds, s, img, istride, step, st, g, pt.triW, spec,
-                       tileStride, chanStride, cpg, ngroups, npairs, words);
+                       tileStride, chanStride, cpg, ngroups, npairs, words, cnMax);
 }
 
 void launch_row_fwd(const uint8_t* img, int64_t istride, int step, int cnMax, const CallState* st,

@@ -97,6 +97,8 @@ enum ColMode { COL_TSPEC = 0, COL_CORR = 1, COL_FWD = 2, COL_MULINV = 3 };
 
 // Forward packed row FFTs of every loaded row pair of every tile/channel.
 // spec layout: [tile][channel][dftH][hwPad] complex (tile stride given).
+// With st == nullptr all cnMax channels are transformed and the call state
+// is not read (it may still be under construction on another stream).
 void launch_row_fwd(const uint8_t* img, int64_t istride, int step, int cnMax, const CallState* st,
                     const TileGrid& g, const PlanTables& pt, float2* spec, int64_t tileStride, int64_t chanStride,
                     int ntiles, hipStream_t s);
@@ -130,6 +132,11 @@ void launch_state_finalize(CallState* st, int tw, int th, int cn_mode, hipStream
 int norm_partials_count(int rw, int rh, int tw, int th, int cnMax);
 // bytes of scratch launch_normalize needs (chunk partial column sums)
 size_t norm_scratch_bytes(int iw, int ih, int tw, int th, int cnMax);
+// Per-chunk partial column sums the wave-form normalisation starts from
+// (nothing for the int64 form); scratch as for launch_normalize, which must
+// run after it.
+void launch_norm_prep(const uint8_t* img, int64_t istride, int iw, int ih, int step, int tw, int th, int cnMax,
+                      const CallState* st, void* scratch, hipStream_t s);
 void launch_normalize(const uint8_t* img, int64_t istride, int iw, int ih, int step, int tw, int th, int rw, int rh,
                       int cnMax, const CallState* st, const float* corr, float* map, Extrema* partials,
                       void* scratch, hipStream_t s);

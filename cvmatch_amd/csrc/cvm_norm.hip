@@ -827,16 +827,21 @@ __global__ __launch_bounds__(64) void k_norm_wave(const uint8_t* __restrict__ im
   }
 }
 
+void launch_norm_prep(const uint8_t* img, int64_t istride, int iw, int ih, int step, int tw, int th, int cnMax,
+                      const CallState* st, void* scratch, hipStream_t s) {
+  if (!norm_plan(tw, th, cnMax).wave || !scratch) return;
+  int nch = (ih + NB_ROWS - 1) / NB_ROWS;
+  int words = (step == 4 && ((((uintptr_t)img) | (uintptr_t)istride) & 3) == 0) ? 1 : 0;
+  hipLaunchKernelGGL(k_colsum_chunks, dim3((iw + 255) / 256, nch), dim3(256), 0, s, img, istride, iw, ih, step, words,
+                     st, (uint32_t*)scratch);
+}
+
 void launch_normalize(const uint8_t* img, int64_t istride, int iw, int ih, int step, int tw, int th, int rw, int rh,
                       int cnMax, const CallState* st, const float* corr, float* map, Extrema* partials,
                       void* scratch, hipStream_t s) {
   NormPlan np = norm_plan(tw, th, cnMax);
   if (np.wave && scratch) {
-    int nch = (ih + NB_ROWS - 1) / NB_ROWS;
     uint32_t* ps = (uint32_t*)scratch;
-    int words = (step == 4 && ((((uintptr_t)img) | (uintptr_t)istride) & 3) == 0) ? 1 : 0;
-    hipLaunchKernelGGL(k_colsum_chunks, dim3((iw + 255) / 256, nch), dim3(256), 0, s, img, istride, iw, ih, step,
-                       words, st, ps);
     const bool a16 = ((((uintptr_t)img) | (uintptr_t)istride) & 15) == 0;
     const bool a8 = ((((uintptr_t)img) | (uintptr_t)istride) & 7) == 0;
     int vmode = 0;

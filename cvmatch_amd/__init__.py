@@ -79,6 +79,11 @@ SIGNATURES = {
     "cvm_fleet_find_all": ([_P, _P, _I64, _I32, _I32, _P], C.c_int),
     "cvm_fleet_free": ([_P], None),
     "cvm_match_u8_device": ([_P, _I64, _I32, _I32, _P, _I64, _I32, _I32, _I32, _I32, _P, _RES, _P], C.c_int),
+    "cvm_match_u8_direct": ([_P, _I64, _I32, _I32, _P, _I64, _I32, _I32, _I32, _I32, _P, _RES], C.c_int),
+    "cvm_match_u8_direct_device": ([_P, _I64, _I32, _I32, _P, _I64, _I32, _I32, _I32, _I32, _P, _RES, _P],
+                                   C.c_int),
+    "cvm_matcher_find_direct": ([_P, _P, _I64, _I32, _I32, _RES], C.c_int),
+    "cvm_fleet_find_all_direct": ([_P, _P, _I64, _I32, _I32, _P], C.c_int),
     "cvm_device_alloc": ([_SZ, C.POINTER(_P)], C.c_int),
     "cvm_device_free": ([_P], C.c_int),
     "cvm_copy_h2d": ([_P, _P, _SZ], C.c_int),
@@ -139,15 +144,36 @@ def view(img: np.ndarray):
 # ---------------------------------------------------------------- one-shot
 
 
-def match_u8(img, istride, iw, ih, tpl, tstride, tw, th, cn, step, want_map=False):
-    """matchU8 impl.go:1126 on flat uint8 buffers; returns (map or None, six-tuple)."""
+def match_u8(img, istride, iw, ih, tpl, tstride, tw, th, cn, step, want_map=False, direct=False):
+    """matchU8 impl.go:1126 on flat uint8 buffers; returns (map or None, six-tuple).
+
+    direct=True: the direct form (exact int8-MFMA correlation, FP64 scores
+    rounded to float32); cn=0 then selects Match's RGBA alpha rule."""
     img = np.ascontiguousarray(img, dtype=np.uint8)
     tpl = np.ascontiguousarray(tpl, dtype=np.uint8)
     rw, rh = iw - tw + 1, ih - th + 1
     res = np.empty((rh, rw), dtype=np.float32) if want_map and rw > 0 and rh > 0 else None
     out = cvm_result()
-    _check(lib().cvm_match_u8(_ptr(img), istride, iw, ih, _ptr(tpl), tstride, tw, th, cn, step,
-                              _ptr(res) if res is not None else None, C.byref(out)))
+    fn = lib().cvm_match_u8_direct if direct else lib().cvm_match_u8
+    _check(fn(_ptr(img), istride, iw, ih, _ptr(tpl), tstride, tw, th, cn, step,
+              _ptr(res) if res is not None else None, C.byref(out)))
+    return res, out.tuple()
+
+
+def match_direct(parent, sub, want_map=False):
+    """The direct form of Match / MatchGray on (H, W, 4) RGBA (alpha rule) or
+    (H, W) gray arrays: (map or None, six-tuple)."""
+    p, ps, pw, ph, pst = view(parent)
+    s, ss, sw, sh, sst = view(sub)
+    if pst != sst:
+        raise CvmatchError(CVM_E_BAD_ARGS, "cvmatch: parent and template formats differ")
+    if sw > pw or sh > ph:
+        raise CvmatchError(CVM_E_BAD_ARGS, f"cvmatch: bad match arguments ({sw}x{sh} in {pw}x{ph})")
+    rw, rh = pw - sw + 1, ph - sh + 1
+    res = np.empty((rh, rw), dtype=np.float32) if want_map else None
+    out = cvm_result()
+    _check(lib().cvm_match_u8_direct(_ptr(p), ps, pw, ph, _ptr(s), ss, sw, sh, 0 if pst == 4 else 1, pst,
+                                     _ptr(res) if res is not None else None, C.byref(out)))
     return res, out.tuple()
 
 
@@ -237,7 +263,7 @@ class Matcher:
         _check(lib().cvm_matcher_new(_ptr(s), ss, sw, sh, sst, C.byref(h)))
         self._h = h
 
-    def find(self, parent):
+    def find(self, parent, direct=False):
         p, ps, pw, ph, pst = view(parent)
         if self.step == 1 and pst == 4:
             p = rgba_to_gray(p)
@@ -245,7 +271,8 @@ class Matcher:
         if pst != self.step:
             raise CvmatchError(CVM_E_BAD_ARGS, "cvmatch: parent format does not match the matcher")
         out = cvm_result()
-        _check(lib().cvm_matcher_find(self._h, _ptr(p), ps, pw, ph, C.byref(out)))
+        fn = lib().cvm_matcher_find_direct if direct else lib().cvm_matcher_find
+        _check(fn(self._h, _ptr(p), ps, pw, ph, C.byref(out)))
         return out.tuple()
 
     Find = find
@@ -281,7 +308,7 @@ class Fleet:
         self._h = h
         self.step = ms[0].step
 
-    def find_all(self, parent):
+    def find_all(self, parent, direct=False):
         p, ps, pw, ph, pst = view(parent)
         if self.step == 1 and pst == 4:
             p = rgba_to_gray(p)
@@ -290,7 +317,8 @@ class Fleet:
             raise CvmatchError(CVM_E_BAD_ARGS, "cvmatch: parent format does not match the fleet")
         n = len(self._ms)
         out = (cvm_result * n)()
-        _check(lib().cvm_fleet_find_all(self._h, _ptr(p), ps, pw, ph, C.cast(out, C.c_void_p)))
+        fn = lib().cvm_fleet_find_all_direct if direct else lib().cvm_fleet_find_all
+        _check(fn(self._h, _ptr(p), ps, pw, ph, C.cast(out, C.c_void_p)))
         return [Result(*out[i].tuple()) for i in range(n)]
 
     FindAll = find_all

@@ -188,7 +188,7 @@ struct Engine {
   hipStream_t side = nullptr;
   hipEvent_t fork = nullptr, join = nullptr;
   std::map<int, FftTables> tables;
-  DevBuf img, tpl, spec, tspec, tspec2, corr, parts, gray0, gray1, mapbuf, nscratch;
+  DevBuf img, tpl, spec, tspec, tspec2, corr, parts, gray0, gray1, mapbuf, nscratch, dbf;
   CallState* st = nullptr;
   Res6* res_d = nullptr;
   Res6* res_h = nullptr;
@@ -507,8 +507,12 @@ static int finish_result(Engine& E, hipStream_t s, cvm_result* out) {
 static int64_t img_bytes(int64_t stride, int w, int h, int step) { return (int64_t)(h - 1) * stride + (int64_t)w * step; }
 
 // Host-buffer front end shared by the one-shot entry points.
+static int run_match_direct(Engine& E, hipStream_t s, const uint8_t* d_img, int64_t istride, int iw, int ih,
+                            const uint8_t* d_tpl, int64_t tstride, int tw, int th, int cn_mode, int step,
+                            float* d_map);
+
 static int match_host(const uint8_t* img, int64_t istride, int iw, int ih, const uint8_t* tpl, int64_t tstride,
-                      int tw, int th, int cn_mode, int step, float* map, cvm_result* out) {
+                      int tw, int th, int cn_mode, int step, float* map, cvm_result* out, bool direct = false) {
   Engine& E = engine();
   std::lock_guard<std::recursive_mutex> lk(E.mu);
   CHK(E.init());
@@ -527,8 +531,12 @@ static int match_host(const uint8_t* img, int64_t istride, int iw, int ih, const
     CHK(E.mapbuf.ensure(sizeof(float) * (size_t)rw * rh));
     d_map = E.mapbuf.as<float>();
   }
-  CHK(run_match(E, s, E.img.as<uint8_t>(), istride, iw, ih, E.tpl.as<uint8_t>(), tstride, tw, th, cn_mode, step,
-                d_map));
+  if (direct)
+    CHK(run_match_direct(E, s, E.img.as<uint8_t>(), istride, iw, ih, E.tpl.as<uint8_t>(), tstride, tw, th, cn_mode,
+                         step, d_map));
+  else
+    CHK(run_match(E, s, E.img.as<uint8_t>(), istride, iw, ih, E.tpl.as<uint8_t>(), tstride, tw, th, cn_mode, step,
+                  d_map));
   if (map) {
     Stage S(E, "d2h", s);
     HIPCHK(hipMemcpyAsync(map, d_map, sizeof(float) * (size_t)rw * rh, hipMemcpyDeviceToHost, s));
@@ -582,6 +590,9 @@ struct cvm_matcher {
   uint8_t* d_pix = nullptr;  // compact copy, stride w*step
   std::mutex mu;
   std::shared_ptr<cvm::MatcherGeom> geom;
+  // direct-correlation B fragments per channel count (built on first use)
+  cvm::DevBuf dbf[5];
+  bool dbf_ok[5] = {false, false, false, false, false};
   std::atomic<int> refs{1};
 };
 
@@ -593,6 +604,7 @@ static void matcher_release(cvm_matcher* m) {
   if (!m) return;
   if (m->refs.fetch_sub(1) == 1) {
     if (m->d_pix) (void)hipFree(m->d_pix);
+    for (auto& b : m->dbf) b.release();
     delete m;
   }
 }
@@ -783,6 +795,10 @@ struct cvm_fleet {
   std::mutex mu;
   std::shared_ptr<cvm::FleetGeom> geom;
   cvm::DevBuf frame, mspec, corr, res;
+  // direct form: per-member call states for the parent alpha key dpConst,
+  // and the normalisation partials per channel count
+  cvm::DevBuf dstates, dscr[5];
+  int dpConst = -1;
 };
 
 namespace cvm {
@@ -952,6 +968,161 @@ static int fleet_find_dev(Engine& E, hipStream_t s, cvm_fleet* f, const uint8_t*
   return 0;
 }
 
+// ---------------------------------------------------- direct correlation --
+//
+// The mixed-precision form: the window x template sum exactly on int8 MFMA
+// (launch_direct), every score the FP64 one rounded to float32 (the exact
+// mode of the wave-form normalisation).  Not bit-identical to the FFT path
+// (whose float32 rounding it does not share); within float32 rounding of
+// the FP64 oracle instead.
+
+static int check_direct(int tw, int th, int cnMax, int step) {
+  if ((int64_t)tw * th * cnMax >= 131072)
+    return fail(CVM_E_BAD_ARGS, "cvmatch: template %dx%d (cn=%d) exceeds the direct-correlation bound area*cn < 131072",
+                tw, th, cnMax);
+  if (tw > 257 || th > 257 || !norm_wave_ok(tw, th, cnMax) || direct_lds_bytes(tw, step) > 160 * 1024 ||
+      direct_nk(tw, step) + 3 * step > 128)
+    return fail(CVM_E_BAD_ARGS, "cvmatch: template %dx%d is outside the direct-correlation shapes", tw, th);
+  return 0;
+}
+
+// Direct correlation + exact normalisation + extrema of one template whose
+// B fragments are ready; the call state is final on s.
+static int run_direct(Engine& E, hipStream_t s, const CallState* st, const uint8_t* d_img, int64_t istride, int iw,
+                      int ih, int tw, int th, int step, int cnMax, const int8_t* bf, void* nscratch, bool prep,
+                      float* d_map, Res6* d_res) {
+  int rw = iw - tw + 1, rh = ih - th + 1;
+  float* corr = d_map;
+  if (!corr) {
+    CHK(E.corr.ensure(sizeof(float) * (size_t)rw * rh));
+    corr = E.corr.as<float>();
+  }
+  int nparts = norm_partials_count(rw, rh, tw, th, cnMax);
+  CHK(E.parts.ensure(sizeof(Extrema) * (size_t)nparts));
+  {
+    Stage S(E, "direct", s);
+    if (launch_direct(d_img, istride, iw, ih, step, bf, tw, th, rw, rh, (int32_t*)corr, s) != 0)
+      return fail(CVM_E_BAD_ARGS, "cvmatch: template %dx%d is outside the direct-correlation shapes", tw, th);
+  }
+  if (prep) {
+    Stage S(E, "norm_prep", s);
+    launch_norm_prep(d_img, istride, iw, ih, step, tw, th, cnMax, st, nscratch, s);
+  }
+  {
+    Stage S(E, "normalize", s);
+    launch_normalize(d_img, istride, iw, ih, step, tw, th, rw, rh, cnMax, st, corr, d_map, E.parts.as<Extrema>(),
+                     nscratch, s, true);
+  }
+  {
+    Stage S(E, "reduce", s);
+    launch_reduce(E.parts.as<Extrema>(), nparts, st, rw, d_res, nullptr, rh, s);
+  }
+  HIPCHK(hipGetLastError());
+  return 0;
+}
+
+// matchU8 in the direct form on device buffers (cn_mode 0 = the RGBA alpha
+// rule, decided on the device; the B fragments then read cn from the state).
+static int run_match_direct(Engine& E, hipStream_t s, const uint8_t* d_img, int64_t istride, int iw, int ih,
+                            const uint8_t* d_tpl, int64_t tstride, int tw, int th, int cn_mode, int step,
+                            float* d_map) {
+  int cnMax = cn_mode ? cn_mode : 4;
+  CHK(check_direct(tw, th, cnMax, step));
+  CHK(E.nscratch.ensure(norm_scratch_bytes(iw, ih, tw, th, cnMax)));
+  CHK(E.dbf.ensure(direct_bfrag_bytes(tw, th, step)));
+  {
+    Stage S(E, "stats", s);
+    launch_state_init(E.st, s);
+    launch_tpl_stats(d_tpl, tstride, tw, th, cnMax, step, E.st, s);
+    if (cn_mode == 0) {
+      launch_alpha_scan(d_img, istride, iw, ih, E.st, 0, s);
+      launch_alpha_scan(d_tpl, tstride, tw, th, E.st, 1, s);
+    }
+    launch_state_finalize(E.st, tw, th, cn_mode, s);
+  }
+  {
+    Stage S(E, "bfrag", s);
+    launch_bfrag(d_tpl, tstride, tw, th, step, cnMax, E.st, E.dbf.as<int8_t>(), s);
+  }
+  return run_direct(E, s, E.st, d_img, istride, iw, ih, tw, th, step, cnMax, E.dbf.as<int8_t>(), E.nscratch.p, true,
+                    d_map, E.res_d);
+}
+
+// B fragments of a matcher for cn channels (cached; cn known on the host).
+static int matcher_bfrag(Engine& E, hipStream_t s, cvm_matcher* m, int cn, const int8_t** out) {
+  std::lock_guard<std::mutex> lk(m->mu);
+  if (!m->dbf_ok[cn]) {
+    CHK(m->dbf[cn].ensure(direct_bfrag_bytes(m->w, m->h, m->step)));
+    launch_bfrag(m->d_pix, (int64_t)m->w * m->step, m->w, m->h, m->step, cn, nullptr, m->dbf[cn].as<int8_t>(), s);
+    HIPCHK(hipGetLastError());
+    m->dbf_ok[cn] = true;
+  }
+  *out = m->dbf[cn].as<int8_t>();
+  return 0;
+}
+
+// Matcher.Find in the direct form.  With a constant-alpha RGBA template the
+// channel count depends on the parent's alpha, so its B fragments are built
+// for the call from the device state; otherwise they are cached.
+static int matcher_find_direct_dev(Engine& E, hipStream_t s, cvm_matcher* m, const uint8_t* d_img, int64_t istride,
+                                   int iw, int ih, Res6* d_res) {
+  int cnMax = m->step;
+  CHK(check_direct(m->w, m->h, cnMax, m->step));
+  CHK(E.nscratch.ensure(norm_scratch_bytes(iw, ih, m->w, m->h, cnMax)));
+  matcher_state(E, s, E.st, m, d_img, istride, iw, ih);
+  const int8_t* bf = nullptr;
+  if (m->step == 4 && m->aConst) {
+    CHK(E.dbf.ensure(direct_bfrag_bytes(m->w, m->h, m->step)));
+    launch_bfrag(m->d_pix, (int64_t)m->w * m->step, m->w, m->h, m->step, 4, E.st, E.dbf.as<int8_t>(), s);
+    bf = E.dbf.as<int8_t>();
+  } else {
+    CHK(matcher_bfrag(E, s, m, cnMax, &bf));
+  }
+  return run_direct(E, s, E.st, d_img, istride, iw, ih, m->w, m->h, m->step, cnMax, bf, E.nscratch.p, true, nullptr,
+                    d_res);
+}
+
+// Fleet.FindAll in the direct form: every non-flat member against the frame
+// (results independent of the bucketing, each equal to the member's direct
+// Find); the normalisation partials are built once per channel count.
+static int fleet_find_direct_dev(Engine& E, hipStream_t s, cvm_fleet* f, const uint8_t* d_img, int64_t istride,
+                                 int pw, int ph, int pConst, Res6* d_res) {
+  size_t n = f->ms.size();
+  for (size_t i = 0; i < n; i++)
+    if (f->ms[i]->varSum != 0) CHK(check_direct(f->ms[i]->w, f->ms[i]->h, f->ms[i]->step, f->ms[i]->step));
+  CHK(f->dstates.ensure(sizeof(CallState) * n));
+  CallState* states = f->dstates.as<CallState>();
+  if (f->dpConst != pConst) {
+    for (size_t i = 0; i < n; i++) {
+      cvm_matcher* m = f->ms[i];
+      if (m->varSum == 0) continue;
+      int cn = m->step == 4 ? ((m->aConst && pConst) ? 3 : 4) : 1;
+      launch_state_set(states + i, m->stats, 0, s);
+      launch_state_finalize(states + i, m->w, m->h, cn, s);
+    }
+    HIPCHK(hipGetLastError());
+    f->dpConst = pConst;
+  }
+  bool prepped[5] = {false, false, false, false, false};
+  for (size_t i = 0; i < n; i++) {
+    cvm_matcher* m = f->ms[i];
+    if (m->varSum == 0) continue;
+    int cn = m->step == 4 ? ((m->aConst && pConst) ? 3 : 4) : 1;
+    // partials depend on the frame and cn only; sized for the widest use
+    CHK(f->dscr[cn].ensure(norm_scratch_bytes(pw, ph, 1, 1, cn)));
+    if (!prepped[cn]) {
+      Stage S(E, "norm_prep", s);
+      launch_norm_prep(d_img, istride, pw, ph, f->step, m->w, m->h, cn, states + i, f->dscr[cn].p, s);
+      prepped[cn] = true;
+    }
+    const int8_t* bf = nullptr;
+    CHK(matcher_bfrag(E, s, m, cn, &bf));
+    CHK(run_direct(E, s, states + i, d_img, istride, pw, ph, m->w, m->h, m->step, cn, bf, f->dscr[cn].p, false,
+                   nullptr, d_res + i));
+  }
+  return 0;
+}
+
 }  // namespace cvm
 
 // ====================================================================== ABI
@@ -1047,9 +1218,40 @@ int cvm_match_gray(const uint8_t* img, int64_t istride, int32_t iw, int32_t ih, 
   return finish_result(E, s, out);
 }
 
+static int match_device(const uint8_t* d_img, int64_t istride, int32_t iw, int32_t ih, const uint8_t* d_tpl,
+                        int64_t tstride, int32_t tw, int32_t th, int32_t cn, int32_t step, float* d_map,
+                        cvm_result* out, void* stream, bool direct);
+
 int cvm_match_u8_device(const uint8_t* d_img, int64_t istride, int32_t iw, int32_t ih, const uint8_t* d_tpl,
                         int64_t tstride, int32_t tw, int32_t th, int32_t cn, int32_t step, float* d_map,
                         cvm_result* out, void* stream) {
+  return match_device(d_img, istride, iw, ih, d_tpl, tstride, tw, th, cn, step, d_map, out, stream, false);
+}
+
+int cvm_match_u8_direct_device(const uint8_t* d_img, int64_t istride, int32_t iw, int32_t ih, const uint8_t* d_tpl,
+                               int64_t tstride, int32_t tw, int32_t th, int32_t cn, int32_t step, float* d_map,
+                               cvm_result* out, void* stream) {
+  return match_device(d_img, istride, iw, ih, d_tpl, tstride, tw, th, cn, step, d_map, out, stream, true);
+}
+
+int cvm_match_u8_direct(const uint8_t* img, int64_t istride, int32_t iw, int32_t ih, const uint8_t* tpl,
+                        int64_t tstride, int32_t tw, int32_t th, int32_t cn, int32_t step, float* map,
+                        cvm_result* out) {
+  if (!img || !tpl) return fail(CVM_E_NIL_IMAGE, "cvmatch: nil image");
+  if (cn == 0) {
+    if (step != 4) return fail(CVM_E_BAD_ARGS, "cvmatch: the alpha rule (cn=0) needs step 4");
+    CHK(check_args(istride, iw, ih, tstride, tw, th, 4, 4));
+  } else {
+    CHK(check_args(istride, iw, ih, tstride, tw, th, cn, step));
+  }
+  return match_host(img, istride, iw, ih, tpl, tstride, tw, th, cn, step, map, out, true);
+}
+
+}  // extern "C"
+
+static int match_device(const uint8_t* d_img, int64_t istride, int32_t iw, int32_t ih, const uint8_t* d_tpl,
+                        int64_t tstride, int32_t tw, int32_t th, int32_t cn, int32_t step, float* d_map,
+                        cvm_result* out, void* stream, bool direct) {
   if (!d_img || !d_tpl) return fail(CVM_E_NIL_IMAGE, "cvmatch: nil image");
   if (cn == 0) {
     if (step != 4) return fail(CVM_E_BAD_ARGS, "cvmatch: the alpha rule (cn=0) needs step 4");
@@ -1061,9 +1263,14 @@ int cvm_match_u8_device(const uint8_t* d_img, int64_t istride, int32_t iw, int32
   std::lock_guard<std::recursive_mutex> lk(E.mu);
   CHK(E.init());
   hipStream_t s = stream ? (hipStream_t)stream : E.stream;
-  CHK(run_match(E, s, d_img, istride, iw, ih, d_tpl, tstride, tw, th, cn, step, d_map));
+  if (direct)
+    CHK(run_match_direct(E, s, d_img, istride, iw, ih, d_tpl, tstride, tw, th, cn, step, d_map));
+  else
+    CHK(run_match(E, s, d_img, istride, iw, ih, d_tpl, tstride, tw, th, cn, step, d_map));
   return finish_result(E, s, out);
 }
+
+extern "C" {
 
 int cvm_alpha_const(const uint8_t* pix, int64_t stride, int32_t w, int32_t h, int32_t* out) {
   if (!pix) return fail(CVM_E_NIL_IMAGE, "cvmatch: nil image");
@@ -1111,7 +1318,22 @@ int cvm_matcher_new(const uint8_t* tpl, int64_t stride, int32_t w, int32_t h, in
   return matcher_create(tpl, stride, w, h, step, out);
 }
 
+static int matcher_find(cvm_matcher* m, const uint8_t* img, int64_t stride, int32_t w, int32_t h, cvm_result* out,
+                        bool direct);
+
 int cvm_matcher_find(cvm_matcher* m, const uint8_t* img, int64_t stride, int32_t w, int32_t h, cvm_result* out) {
+  return matcher_find(m, img, stride, w, h, out, false);
+}
+
+int cvm_matcher_find_direct(cvm_matcher* m, const uint8_t* img, int64_t stride, int32_t w, int32_t h,
+                            cvm_result* out) {
+  return matcher_find(m, img, stride, w, h, out, true);
+}
+
+}  // extern "C"
+
+static int matcher_find(cvm_matcher* m, const uint8_t* img, int64_t stride, int32_t w, int32_t h, cvm_result* out,
+                        bool direct) {
   if (!m) return fail(CVM_E_BAD_ARGS, "cvmatch: null matcher");
   if (!img) return fail(CVM_E_NIL_IMAGE, "cvmatch: nil image");
   if (w <= 0 || h <= 0) return fail(CVM_E_EMPTY_IMAGE, "cvmatch: empty image");
@@ -1132,9 +1354,14 @@ int cvm_matcher_find(cvm_matcher* m, const uint8_t* img, int64_t stride, int32_t
     Stage S(E, "h2d", s);
     HIPCHK(hipMemcpyAsync(E.img.p, img, ib, hipMemcpyHostToDevice, s));
   }
-  CHK(matcher_find_dev(E, s, m, E.img.as<uint8_t>(), stride, w, h, E.res_d));
+  if (direct)
+    CHK(matcher_find_direct_dev(E, s, m, E.img.as<uint8_t>(), stride, w, h, E.res_d));
+  else
+    CHK(matcher_find_dev(E, s, m, E.img.as<uint8_t>(), stride, w, h, E.res_d));
   return finish_result(E, s, out);
 }
+
+extern "C" {
 
 void cvm_matcher_free(cvm_matcher* m) { matcher_release(m); }
 
@@ -1156,7 +1383,22 @@ int cvm_fleet_new(cvm_matcher* const* ms, int32_t n, cvm_fleet** out) {
   return 0;
 }
 
+static int fleet_find_all(cvm_fleet* f, const uint8_t* img, int64_t stride, int32_t w, int32_t h, cvm_result* out,
+                          bool direct);
+
 int cvm_fleet_find_all(cvm_fleet* f, const uint8_t* img, int64_t stride, int32_t w, int32_t h, cvm_result* out) {
+  return fleet_find_all(f, img, stride, w, h, out, false);
+}
+
+int cvm_fleet_find_all_direct(cvm_fleet* f, const uint8_t* img, int64_t stride, int32_t w, int32_t h,
+                              cvm_result* out) {
+  return fleet_find_all(f, img, stride, w, h, out, true);
+}
+
+}  // extern "C"
+
+static int fleet_find_all(cvm_fleet* f, const uint8_t* img, int64_t stride, int32_t w, int32_t h, cvm_result* out,
+                          bool direct) {
   if (!f) return fail(CVM_E_BAD_ARGS, "cvmatch: null fleet");
   if (!img) return fail(CVM_E_NIL_IMAGE, "cvmatch: nil image");
   if (w <= 0 || h <= 0) return fail(CVM_E_EMPTY_IMAGE, "cvmatch: empty image");
@@ -1195,7 +1437,10 @@ int cvm_fleet_find_all(cvm_fleet* f, const uint8_t* img, int64_t stride, int32_t
     pConst = hs.alpha_bad[0] == 0;
   }
   CHK(f->res.ensure(sizeof(Res6) * n));
-  CHK(fleet_find_dev(E, s, f, E.img.as<uint8_t>(), stride, w, h, pConst, f->res.as<Res6>(), flat));
+  if (direct)
+    CHK(fleet_find_direct_dev(E, s, f, E.img.as<uint8_t>(), stride, w, h, pConst, f->res.as<Res6>()));
+  else
+    CHK(fleet_find_dev(E, s, f, E.img.as<uint8_t>(), stride, w, h, pConst, f->res.as<Res6>(), flat));
   std::vector<Res6> hr(n);
   HIPCHK(hipMemcpyAsync(hr.data(), f->res.p, sizeof(Res6) * n, hipMemcpyDeviceToHost, s));
   HIPCHK(hipStreamSynchronize(s));
@@ -1204,6 +1449,8 @@ int cvm_fleet_find_all(cvm_fleet* f, const uint8_t* img, int64_t stride, int32_t
     if (f->ms[i]->varSum != 0) memcpy(&out[i], &hr[i], sizeof(cvm_result));
   return 0;
 }
+
+extern "C" {
 
 void cvm_fleet_free(cvm_fleet* f) {
   if (!f) return;

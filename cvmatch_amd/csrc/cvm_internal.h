@@ -137,9 +137,23 @@ size_t norm_scratch_bytes(int iw, int ih, int tw, int th, int cnMax);
 // run after it.
 void launch_norm_prep(const uint8_t* img, int64_t istride, int iw, int ih, int step, int tw, int th, int cnMax,
                       const CallState* st, void* scratch, hipStream_t s);
+// exact: corr holds the int32 C' of launch_direct, scores are FP64-grade
+// (needs the wave form: norm_wave_ok).
 void launch_normalize(const uint8_t* img, int64_t istride, int iw, int ih, int step, int tw, int th, int rw, int rh,
                       int cnMax, const CallState* st, const float* corr, float* map, Extrema* partials,
-                      void* scratch, hipStream_t s);
+                      void* scratch, hipStream_t s, bool exact = false);
+// The wave-form normalisation applies (it is the one with an exact mode).
+bool norm_wave_ok(int tw, int th, int cnMax);
+
+// Direct correlation on int8 MFMA (cvm_direct.hip).  B fragments of one
+// template (cn from st when st is set), then C' per window into out.
+int direct_nk(int tw, int step);
+size_t direct_bfrag_bytes(int tw, int th, int step);
+size_t direct_lds_bytes(int tw, int step);
+void launch_bfrag(const uint8_t* tpl, int64_t tstride, int tw, int th, int step, int cn, const CallState* st,
+                  int8_t* bf, hipStream_t s);
+int launch_direct(const uint8_t* img, int64_t istride, int iw, int ih, int step, const int8_t* bf, int tw, int th,
+                  int rw, int rh, int32_t* out, hipStream_t s);
 // Final first-occurrence reduction into res6 (device) = {min_v, min_x, min_y, max_v, max_x, max_y}.
 void launch_reduce(const Extrema* partials, int nparts, const CallState* st, int rw, void* res6, float* map_if_flat,
                    int rh, hipStream_t s);

@@ -248,6 +248,8 @@ static NormPlan norm_plan(int tw, int th, int cnMax) {
   return p;
 }
 
+bool norm_wave_ok(int tw, int th, int cnMax) { return norm_plan(tw, th, cnMax).wave != 0; }
+
 int norm_partials_count(int rw, int rh, int tw, int th, int cnMax) {
   NormPlan np = norm_plan(tw, th, cnMax);
   if (np.wave) return ((rw + 64 * np.q - 1) / (64 * np.q)) * ((rh + NB_ROWS - 1) / NB_ROWS);
@@ -586,10 +588,28 @@ struct NormRow {
   float c[Q];
 };
 
+// The FP64 score of one window from exact integers (the FP64 oracle's
+// formula): num = area * corr - sum_k S_k T_k, idiff = area * s2 - sum_k S_k^2,
+// every input an exact integer below 2^53.
+__device__ __forceinline__ double score_fp64(double numI, double idiff, double s2, double aread, double varSum) {
+  const double diff2 = idiff / aread;
+  double lim = 1.1920928955078125e-06 * s2;  // 10 * 2^-23 * s2
+  if (lim > 0.5) lim = 0.5;
+  if (!(diff2 > lim)) return 0.0;
+  const double denI = sqrt(idiff) * sqrt(varSum);
+  const double an = fabs(numI);
+  if (an < denI) return numI / denI;
+  if (an < denI * 1.125) return numI > 0.0 ? 1.0 : -1.0;
+  return 0.0;
+}
+
 // VMODE 1 / 2 (RGBA / gray rows with iw % 8 == 0, aligned for vector loads):
 // every lane loads eight whole columns, lanes past the image clamped to its
 // last eight, with no branch around any load; VMODE 0: byte loads.
-template <int Q, bool SQ32, int VMODE>
+// EXACT: corr holds the int32 C' = sum (P - 128)(T - 128) of the direct
+// correlation (bit pattern read through the float pointer) and every score
+// is the FP64 one rounded once to float32.
+template <int Q, bool SQ32, int VMODE, bool EXACT>
 __global__ __launch_bounds__(64) void k_norm_wave(const uint8_t* __restrict__ img, int64_t istride, int iw, int ih,
                                                   int step, int tw, int th, int rw, int rh,
                                                   const CallState* __restrict__ st, const uint32_t* __restrict__ ps,
@@ -634,6 +654,10 @@ __global__ __launch_bounds__(64) void k_norm_wave(const uint8_t* __restrict__ im
     const float numScale = st->numScale, varScale = st->varScale, templNorm = st->templNorm;
     double tsd[4];
     for (int k = 0; k < 4; k++) tsd[k] = k < cn ? (double)st->tsum[k] : 0.0;
+    // EXACT: sum P*T = C' + 128 (S + sum_k T_k) - 2^14 area cn
+    const double varSumD = (double)st->varSum;
+    const double tsumAll = tsd[0] + tsd[1] + tsd[2] + tsd[3];
+    const double cpOff = 16384.0 * aread * (double)cn;
     const int cbase = lane * CPT;
     // every in-image column of the lane is summed; columns past the strip
     // (or, clamped, repeats of the image's last eight) only reach prefix
@@ -759,9 +783,15 @@ __global__ __launch_bounds__(64) void k_norm_wave(const uint8_t* __restrict__ im
         const double s2 = (double)(sq_t)(P2[ib] - P2[ia]);
         const double sq = sd[0] * sd[0] + sd[1] * sd[1] + sd[2] * sd[2] + sd[3] * sd[3];
         double lane0 = sd[0] * tsd[0] + sd[1] * tsd[1] + sd[2] * tsd[2] + sd[3] * tsd[3];
-        if (one) lane0 = sd[0];
         const double idiff = aread * s2 - sq;
-        vq[q] = norm_one(r.c[q], (float)lane0, (float)idiff, (float)s2, numScale, varScale, templNorm);
+        if constexpr (EXACT) {
+          const double S = sd[0] + sd[1] + sd[2] + sd[3];
+          const double corrI = (double)__float_as_int(r.c[q]) + 128.0 * (S + tsumAll) - cpOff;
+          vq[q] = (float)score_fp64(aread * corrI - lane0, idiff, s2, aread, varSumD);
+        } else {
+          if (one) lane0 = sd[0];
+          vq[q] = norm_one(r.c[q], (float)lane0, (float)idiff, (float)s2, numScale, varScale, templNorm);
+        }
       }
       __syncthreads();
       // slide to row y+1: the leaving row is inside the current sums, so
@@ -838,7 +868,7 @@ void launch_norm_prep(const uint8_t* img, int64_t istride, int iw, int ih, int s
 
 void launch_normalize(const uint8_t* img, int64_t istride, int iw, int ih, int step, int tw, int th, int rw, int rh,
                       int cnMax, const CallState* st, const float* corr, float* map, Extrema* partials,
-                      void* scratch, hipStream_t s) {
+                      void* scratch, hipStream_t s, bool exact) {
   NormPlan np = norm_plan(tw, th, cnMax);
   if (np.wave && scratch) {
     uint32_t* ps = (uint32_t*)scratch;
@@ -848,16 +878,22 @@ void launch_normalize(const uint8_t* img, int64_t istride, int iw, int ih, int s
     if (iw % NORM_CPT == 0 && step == 4 && a16) vmode = 1;
     if (iw % NORM_CPT == 0 && step == 1 && a8) vmode = 2;
     dim3 grid((rw + 64 * np.q - 1) / (64 * np.q), (rh + NB_ROWS - 1) / NB_ROWS);
-#define CVM_NW(Q, S, V)                                                                                          \
-  hipLaunchKernelGGL((k_norm_wave<Q, S, V>), grid, dim3(64), 0, s, img, istride, iw, ih, step, tw, th, rw, rh, st, \
-                     ps, corr, map, partials)
+#define CVM_NW(Q, S, V, X)                                                                                       \
+  hipLaunchKernelGGL((k_norm_wave<Q, S, V, X>), grid, dim3(64), 0, s, img, istride, iw, ih, step, tw, th, rw, rh, \
+                     st, ps, corr, map, partials)
+#define CVM_NWX(Q, S, V) \
+  if (exact)             \
+    CVM_NW(Q, S, V, true); \
+  else                   \
+    CVM_NW(Q, S, V, false)
 #define CVM_NWV(Q, S)   \
-  if (vmode == 1)       \
-    CVM_NW(Q, S, 1);    \
-  else if (vmode == 2)  \
-    CVM_NW(Q, S, 2);    \
-  else                  \
-    CVM_NW(Q, S, 0)
+  if (vmode == 1) {     \
+    CVM_NWX(Q, S, 1);   \
+  } else if (vmode == 2) { \
+    CVM_NWX(Q, S, 2);   \
+  } else {              \
+    CVM_NWX(Q, S, 0);   \
+  }
     if (np.q == 4) {
       if (np.sq32) {
         CVM_NWV(4, true);
@@ -872,6 +908,7 @@ void launch_normalize(const uint8_t* img, int64_t istride, int iw, int ih, int s
       }
     }
 #undef CVM_NWV
+#undef CVM_NWX
 #undef CVM_NW
     return;
   }

@@ -120,6 +120,26 @@ int cvm_fleet_find_all(cvm_fleet *f, const uint8_t *img, int64_t stride, int32_t
 
 void cvm_fleet_free(cvm_fleet *f);
 
+/* ------------------------------------------------- direct (mixed) form -- */
+/* The same calls with the window x template sum computed exactly on int8
+ * MFMA (bytes offset by 128, int32 accumulation) and every score evaluated
+ * in FP64 from exact integers, then rounded once to float32: each score is
+ * the FP64 oracle's (the TM_CCOEFF_NORMED formula with OpenCV's guard ladder
+ * as in normOne impl.go:816-849, evaluated in binary64) rounded to float32.
+ * Not bit-identical to the plain calls, which reproduce the reference's
+ * float32 FFT rounding.  Applies when tw*th*cn < 131072 and tw, th <= 257;
+ * other shapes return CVM_E_BAD_ARGS.  Arguments and results as in the plain
+ * counterparts: matchU8 impl.go:1126-1155, Find matcher.go:83-130, FindAll
+ * fleet.go:176-332 (FindAll's members are not bucketed here: each result
+ * equals that member's direct Find). */
+int cvm_match_u8_direct(const uint8_t *img, int64_t istride, int32_t iw, int32_t ih, const uint8_t *tpl,
+                        int64_t tstride, int32_t tw, int32_t th, int32_t cn, int32_t step, float *map,
+                        cvm_result *out);
+int cvm_matcher_find_direct(cvm_matcher *m, const uint8_t *img, int64_t stride, int32_t w, int32_t h,
+                            cvm_result *out);
+int cvm_fleet_find_all_direct(cvm_fleet *f, const uint8_t *img, int64_t stride, int32_t w, int32_t h,
+                              cvm_result *out);
+
 /* ------------------------------------------------ device-resident calls -- */
 /* No reference counterpart: the same pipeline on buffers already in device
  * memory (the benchmark's "inputs resident in HBM" case and callers that
@@ -128,6 +148,10 @@ void cvm_fleet_free(cvm_fleet *f);
 int cvm_match_u8_device(const uint8_t *d_img, int64_t istride, int32_t iw, int32_t ih, const uint8_t *d_tpl,
                         int64_t tstride, int32_t tw, int32_t th, int32_t cn, int32_t step, float *d_map,
                         cvm_result *out, void *stream);
+/* The direct form of cvm_match_u8_device. */
+int cvm_match_u8_direct_device(const uint8_t *d_img, int64_t istride, int32_t iw, int32_t ih, const uint8_t *d_tpl,
+                               int64_t tstride, int32_t tw, int32_t th, int32_t cn, int32_t step, float *d_map,
+                               cvm_result *out, void *stream);
 
 /* Device memory helpers for callers without their own allocator. */
 int cvm_device_alloc(size_t bytes, void **out);

@@ -34,10 +34,16 @@ constexpr int DIR_T = 4;      // column tiles per wave (128 output columns)
 constexpr int DIR_WAVES = 2;  // waves per workgroup, 32 output rows each
 constexpr int DIR_ROWS = 32 * DIR_WAVES;
 constexpr int DIR_RING = DIR_ROWS + 8;
+constexpr int DIR_CMAX = 9;   // K chunks per residue class: nk <= step * DIR_CMAX
+constexpr int DIR_PF = 4;     // ring rows in flight (<= DIR_RING - DIR_ROWS)
 
 int direct_nk(int tw, int step) { return (step * (32 + tw - 1) + 31) / 32; }
 
-size_t direct_bfrag_bytes(int tw, int th, int step) { return (size_t)th * direct_nk(tw, step) * 64 * 16; }
+// B fragments plus DIR_CMAX of padding: a class load may run past the last
+// fragment (those registers are never used).
+size_t direct_bfrag_bytes(int tw, int th, int step) {
+  return ((size_t)th * direct_nk(tw, step) + DIR_CMAX) * 64 * 16;
+}
 
 static int direct_pitch(int nk, int step) {
   // staged row bytes, padded so consecutive rows start 16 bytes (4 banks)
@@ -112,34 +118,41 @@ __global__ __launch_bounds__(64 * DIR_WAVES) void k_direct_i8(const uint8_t* __r
   const int64_t xb0 = (int64_t)STEP * X0;
   const int64_t imgBytes = (int64_t)STEP * iw;
 
-  // 16 bytes of staged row rho (image row Y0 + rho), chunk c, as int8 (v ^ 0x80);
-  // bytes outside the image are 0 (they only reach discarded outputs)
-  auto fetch = [&](int rho, int c) -> uint4 {
+  // 16 bytes of staged row rho (image row Y0 + rho), chunk c.  VEC: the
+  // caller guarantees 16-byte alignment and STEP*iw % 16 == 0, so a chunk is
+  // wholly inside or wholly outside the image; the load is clamped into the
+  // image and `ok` says whether to keep it.  The int8 shift (v ^ 0x80) and
+  // the zeroing are applied when the chunk is written (cook), so the load
+  // itself is never waited on early.
+  auto fetch = [&](int rho, int c, bool& ok) -> uint4 {
     const int r = Y0 + rho;
     const int64_t b0 = xb0 + 16 * (int64_t)c;
-    uint4 v = make_uint4(0, 0, 0, 0);
+    if (VEC) {
+      ok = r < ih && b0 < imgBytes;
+      const int rr = r < ih ? r : ih - 1;
+      const int64_t bb = b0 < imgBytes ? b0 : imgBytes - 16;
+      return *(const uint4*)(img + (int64_t)rr * istride + bb);
+    }
+    ok = true;
+    uint32_t w[4] = {0x80808080u, 0x80808080u, 0x80808080u, 0x80808080u};
     if (r < ih) {
       const uint8_t* p = img + (int64_t)r * istride;
-      if (VEC && b0 + 16 <= imgBytes) {
-        v = *(const uint4*)(p + b0);
-        v.x ^= 0x80808080u;
-        v.y ^= 0x80808080u;
-        v.z ^= 0x80808080u;
-        v.w ^= 0x80808080u;
-      } else {
-        uint32_t w[4] = {0, 0, 0, 0};
-        for (int b = 0; b < 16; b++)
-          if (b0 + b < imgBytes) w[b >> 2] |= (uint32_t)(p[b0 + b] ^ 0x80u) << (8 * (b & 3));
-        v = make_uint4(w[0], w[1], w[2], w[3]);
-      }
+      for (int b = 0; b < 16; b++)
+        if (b0 + b < imgBytes) w[b >> 2] = (w[b >> 2] & ~(255u << (8 * (b & 3)))) | ((uint32_t)p[b0 + b] << (8 * (b & 3)));
     }
-    return v;
+    return make_uint4(w[0], w[1], w[2], w[3]);
+  };
+  auto cook = [](uint4 v, bool ok) -> uint4 {
+    if (!ok) return make_uint4(0, 0, 0, 0);
+    return make_uint4(v.x ^ 0x80808080u, v.y ^ 0x80808080u, v.z ^ 0x80808080u, v.w ^ 0x80808080u);
   };
 
   // initial rows [0, DIR_ROWS)
   for (int e = tid; e < DIR_ROWS * rowChunks; e += 64 * DIR_WAVES) {
     const int rho = e / rowChunks, c = e - rho * rowChunks;
-    *(uint4*)(ring + rho * pitch + 16 * c) = fetch(rho, c);
+    bool ok;
+    const uint4 v = fetch(rho, c, ok);
+    *(uint4*)(ring + rho * pitch + 16 * c) = cook(v, ok);
   }
   __syncthreads();
 
@@ -150,58 +163,99 @@ __global__ __launch_bounds__(64 * DIR_WAVES) void k_direct_i8(const uint8_t* __r
     for (int j = 0; j < 16; j++) acc[t][j] = 0;
 
   const int half = lane >> 5, m = lane & 31;
-  // B fragments stream in visit order through a 4-deep register queue that
-  // runs across template rows
+  // Per template row, K chunk residue class rc has c_rc = ceil((nk - rc) / STEP)
+  // chunks; its c_rc + T - 1 A fragments and c_rc B fragments sit in
+  // statically indexed registers, the next class's loads issued before this
+  // class's MFMAs, the next row's first B class before the barrier.
+  constexpr int NA = DIR_CMAX + T - 1;
   const dv4i* bq = (const dv4i*)bf + lane;
-  const int64_t gtot = (int64_t)th * nk;
-  int64_t g = 0;
-  dv4i q[4];
+  int cls[STEP], base[STEP];
+  {
+    int acc_s = 0;
 #pragma unroll
-  for (int p = 0; p < 4; p++) q[p] = p < gtot ? bq[(int64_t)p * 64] : dv4i{0, 0, 0, 0};
-  for (int i = 0; i < th; i++) {
-    // the row step i + 1 adds (staged row DIR_ROWS + i), loaded now and
-    // written after this step's MFMAs
-    const bool more = i + 1 < th;
-    uint4 nxt[2];
-    int nc[2];
-#pragma unroll
-    for (int p = 0; p < 2; p++) {
-      nc[p] = tid + p * 64 * DIR_WAVES;
-      nxt[p] = make_uint4(0, 0, 0, 0);
-      if (more && nc[p] < rowChunks) nxt[p] = fetch(DIR_ROWS + i, nc[p]);
+    for (int rc = 0; rc < STEP; rc++) {
+      cls[rc] = rc < nk ? (nk - rc + STEP - 1) / STEP : 0;
+      base[rc] = acc_s;
+      acc_s += cls[rc];
     }
+  }
+  dv4i A0[NA], A1[NA], B0[DIR_CMAX], B1[DIR_CMAX], BN[DIR_CMAX];
+  // unconditional loads (a fixed count the compiler can wait on exactly):
+  // A chunk indices clamp to the staged row, B runs into the padding
+  const int lastChunk = nk + STEP * (T - 1) - 1;
+  auto loadA = [&](dv4i* buf, const uint8_t* arow, int rc) {
+#pragma unroll
+    for (int j = 0; j < NA; j++) buf[j] = *(const dv4i*)(arow + 32 * min(rc + STEP * j, lastChunk));
+  };
+  auto loadB = [&](dv4i* buf, int i, int rc) {
+    const dv4i* p = bq + ((int64_t)i * nk + base[rc]) * 64;
+#pragma unroll
+    for (int j = 0; j < DIR_CMAX; j++) buf[j] = p[(int64_t)j * 64];
+  };
+  auto mma = [&](const dv4i* a, const dv4i* bb, int c) {
+#pragma unroll
+    for (int j = 0; j < DIR_CMAX; j++)
+      if (j < c)
+#pragma unroll
+        for (int t = 0; t < T; t++) acc[t] = __builtin_amdgcn_mfma_i32_32x32x32_i8(a[j + t], bb[j], acc[t], 0, 0, 0);
+  };
+
+  // Staged row DIR_ROWS + i (first read in step i + 1) is fetched in step
+  // i - DIR_PF + 1 into a register queue and written to its ring slot at the
+  // end of step i, so DIR_PF steps of MFMAs cover its load latency.
+  // chunk nc[p] of each staged row is this thread's (chunks past the row
+  // reload the last one and are not written)
+  int nc[2];
+#pragma unroll
+  for (int p = 0; p < 2; p++) nc[p] = tid + p * 64 * DIR_WAVES;
+  const int ncl[2] = {min(nc[0], rowChunks - 1), min(nc[1], rowChunks - 1)};
+  uint4 rq[DIR_PF][2];
+  bool rok[DIR_PF][2];
+#pragma unroll
+  for (int d = 0; d < DIR_PF - 1; d++)
+#pragma unroll
+    for (int p = 0; p < 2; p++) rq[d][p] = fetch(DIR_ROWS + d, ncl[p], rok[d][p]);
+
+  loadB(BN, 0, 0);
+  for (int i = 0; i < th; i++) {
+    const bool more = i + 1 < th;
+#pragma unroll
+    for (int p = 0; p < 2; p++) rq[DIR_PF - 1][p] = fetch(DIR_ROWS + i + DIR_PF - 1, ncl[p], rok[DIR_PF - 1][p]);
     // this wave's A rows: staged rows 32 wave + m + i
     const int slot = (32 * wave + m + i) % DIR_RING;
     const uint8_t* arow = ring + slot * pitch + 16 * half;
 #pragma unroll
+    for (int j = 0; j < DIR_CMAX; j++) B0[j] = BN[j];
+    loadA(A0, arow, 0);
+#pragma unroll
     for (int rc = 0; rc < STEP; rc++) {
-      if (rc >= nk) break;
-      dv4i aw[T];
-#pragma unroll
-      for (int t = 0; t < T; t++) aw[t] = *(const dv4i*)(arow + 32 * (rc + STEP * t));
-      for (int kc = rc; kc < nk; kc += STEP) {
-        const dv4i bcur = q[0];
-#pragma unroll
-        for (int p = 0; p < 3; p++) q[p] = q[p + 1];
-        q[3] = (g + 4 < gtot) ? bq[(g + 4) * 64] : dv4i{0, 0, 0, 0};
-        g++;
-        dv4i anew = aw[T - 1];
-        if (kc + STEP < nk) anew = *(const dv4i*)(arow + 32 * (kc + STEP * T));
-#pragma unroll
-        for (int t = 0; t < T; t++) acc[t] = __builtin_amdgcn_mfma_i32_32x32x32_i8(aw[t], bcur, acc[t], 0, 0, 0);
-#pragma unroll
-        for (int t = 0; t < T - 1; t++) aw[t] = aw[t + 1];
-        aw[T - 1] = anew;
+      dv4i* ca = (rc & 1) ? A1 : A0;
+      dv4i* cb = (rc & 1) ? B1 : B0;
+      dv4i* na = (rc & 1) ? A0 : A1;
+      dv4i* nb = (rc & 1) ? B0 : B1;
+      if (rc + 1 < STEP) {
+        loadA(na, arow, rc + 1);
+        loadB(nb, i, rc + 1);
+      } else if (more) {
+        loadB(BN, i + 1, 0);
       }
+      mma(ca, cb, cls[rc]);
     }
     if (more) {
-      // slot of staged row DIR_ROWS + i: it held row i - 8, last read in
-      // step i - 8, which every wave has left (barrier below)
+      // slot of staged row DIR_ROWS + i: it held row i + DIR_ROWS - DIR_RING,
+      // last read in that step, which every wave has left (barrier below)
       const int ns = (DIR_ROWS + i) % DIR_RING;
 #pragma unroll
       for (int p = 0; p < 2; p++)
-        if (nc[p] < rowChunks) *(uint4*)(ring + ns * pitch + 16 * nc[p]) = nxt[p];
+        if (nc[p] < rowChunks) *(uint4*)(ring + ns * pitch + 16 * nc[p]) = cook(rq[0][p], rok[0][p]);
     }
+#pragma unroll
+    for (int d = 0; d < DIR_PF - 1; d++)
+#pragma unroll
+      for (int p = 0; p < 2; p++) {
+        rq[d][p] = rq[d + 1][p];
+        rok[d][p] = rok[d + 1][p];
+      }
     __syncthreads();
   }
 
@@ -223,10 +277,12 @@ int launch_direct(const uint8_t* img, int64_t istride, int iw, int ih, int step,
                   int rw, int rh, int32_t* out, hipStream_t s) {
   const int nk = direct_nk(tw, step);
   const int pitch = direct_pitch(nk, step);
-  if (32 * (nk + step * (DIR_T - 1)) > pitch || 2 * 64 * DIR_WAVES < 2 * (nk + step * (DIR_T - 1))) return -1;
+  if (32 * (nk + step * (DIR_T - 1)) > pitch || 2 * 64 * DIR_WAVES < 2 * (nk + step * (DIR_T - 1)) ||
+      nk > step * DIR_CMAX)
+    return -1;
   const size_t lds = (size_t)DIR_RING * pitch;
   if (lds > 160 * 1024) return -1;
-  const bool vec = ((((uintptr_t)img) | (uintptr_t)istride) & 15) == 0;
+  const bool vec = ((((uintptr_t)img) | (uintptr_t)istride | (uintptr_t)(step * iw)) & 15) == 0;
   dim3 grid((rw + 32 * DIR_T - 1) / (32 * DIR_T), (rh + DIR_ROWS - 1) / DIR_ROWS), block(64 * DIR_WAVES);
 #define CVM_DI(S, V) \
   hipLaunchKernelGGL((k_direct_i8<S, V>), grid, block, lds, s, img, istride, iw, ih, bf, th, nk, pitch, rw, rh, out)

@@ -56,18 +56,25 @@ def tiles(tw, th, rw, rh, plan):
 
 
 def stage_bytes(iw, ih, tw, th, cn, plan):
-    """Algorithmic (compulsory) HBM bytes of one launch of each kernel."""
+    """Algorithmic (compulsory) HBM bytes of one call of each stage:
+    row_fwd: each tile's pixels (with halo) once + its spectrum rows written;
+    cols: spectrum rows in and out + the template spectrum once per channel
+    (and its shrunk-band copy); row_inv: spectrum rows in + the map written;
+    normalize: the parent once + the correlation read + the map written."""
     rw, rh = iw - tw + 1, ih - th + 1
     ts, dftW = tiles(tw, th, rw, rh, plan)
     hw = dftW // 2 + 1
     b = {"row_fwd": 0, "cols": 0, "row_inv": 0}
+    heights = set()
     for (x0, y0, bw, bh, n) in ts:
         loadW, loadH = bw + tw - 1, bh + th - 1
         loaded = min(n, (loadH + 1) & ~1)
         need = min(n, (bh + 1) & ~1)
-        b["row_fwd"] += cn * (loadW * loadH * 4 // cn + loaded * hw * 8)  # RGBA bytes once per tile + spectrum rows
-        b["cols"] += cn * (loaded * hw * 8 + n * hw * 8 + need * hw * 8)
+        b["row_fwd"] += loadW * loadH * 4 + cn * loaded * hw * 8
+        b["cols"] += cn * (loaded + need) * hw * 8
         b["row_inv"] += cn * need * hw * 8 + bw * bh * 4
+        heights.add(n)
+    b["cols"] += sum(cn * n * hw * 8 for n in heights)
     b["normalize"] = iw * ih * 4 + 2 * rw * rh * 4
     return b
 
@@ -99,6 +106,43 @@ def cpu_baseline(parent, sub, budget_s=3.0):
     }
 
 
+def fleet_leg(frames_timed=16, solo_frames=2):
+    """The batched Fleet over the seeded frame stream: 256 templates U[16,128]^2
+    (half with sigma-2 noise) on 2560x1440 RGBA frames, host frames in, six
+    results per member out.  Returns templates*frames/s for the direct
+    (mixed-precision: exact int8-MFMA correlation, FP64 scores) Fleet, the plain
+    Fleet (the reference's bucketed float32 FFT form, bit-exact to it) and 256
+    plain solo Finds per frame (the plain GPU form)."""
+    import cvmatch_amd as cvm
+    from cvmatch_amd import synth
+
+    frames, tpls = synth.workload_fleet()
+    ms = [cvm.NewMatcher(t) for t in tpls]
+    fl = cvm.NewFleet(*ms)
+    n = len(ms)
+    out = {"workload": "64-frame 2560x1440 RGBA flat-rectangle stream (alpha 255), 256 templates U[16,128]^2",
+           "frames_timed": frames_timed}
+    for key, direct in (("direct", True), ("plain_fleet", False)):
+        fl.find_all(frames[0], direct=direct)  # geometry / fragment caches
+        t0 = time.perf_counter()
+        for f in range(1, 1 + frames_timed):
+            fl.find_all(frames[f % len(frames)], direct=direct)
+        dt = (time.perf_counter() - t0) / frames_timed
+        out[key] = {"tpl_frames_per_s": round(n / dt, 1), "ms_per_frame": round(dt * 1e3, 3)}
+    t0 = time.perf_counter()
+    for f in range(1, 1 + solo_frames):
+        for m in ms:
+            m.find(frames[f])
+    dt = (time.perf_counter() - t0) / solo_frames
+    out["plain_solo"] = {"tpl_frames_per_s": round(n / dt, 1), "ms_per_frame": round(dt * 1e3, 3),
+                         "frames_timed": solo_frames}
+    out["direct_over_plain_solo"] = round(out["direct"]["tpl_frames_per_s"] / out["plain_solo"]["tpl_frames_per_s"], 3)
+    out["max_abs_delta_vs_fp64"] = ("direct scores equal the FP64 oracle rounded to float32 "
+                                    "(checked in tests/test_gpu_direct.py)")
+    del fl, ms
+    return out
+
+
 def copy_calibration(mib):
     import cvmatch_amd as cvm
 
@@ -128,11 +172,17 @@ def main():
     ap.add_argument("--steps", type=int, default=30)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--no-fleet", action="store_true", help="skip the Fleet leg")
+    ap.add_argument("--fleet-frames", type=int, default=16, help="frames timed per Fleet form")
+    ap.add_argument("--fleet-only", action="store_true", help="only run the Fleet leg (profiling)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run the plain copy kernel over MIB MiB (calibrates FETCH_SIZE/WRITE_SIZE)")
     args = ap.parse_args()
     if args.copy_calibration:
         copy_calibration(args.copy_calibration)
+        return
+    if args.fleet_only:
+        print(json.dumps({"fleet": fleet_leg(args.fleet_frames)}), flush=True)
         return
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (the path shards per frame or per "
@@ -232,6 +282,8 @@ def main():
                      "alg_bytes_per_launch": sb[dom], "avg_launch_ms": round(kern[dom]["avg_ms"], 5)},
         "stages_ms": {k: round(v["avg_ms"], 5) for k, v in stages.items()},
     }
+    if not args.no_fleet:
+        line["fleet"] = fleet_leg(args.fleet_frames)
     if not args.no_cpu:
         cb = cpu_baseline(parent, sub)
         line["cpu_baseline"] = cb

@@ -175,3 +175,36 @@ def test_direct_1440p_gray(cvm, orc):
         ys = np.concatenate([rng.integers(0, rh, 4096), [gt[2], gt[5]]]).astype(np.int32)
         ref = orc.fp64_points(img, img.shape[1], img.shape[1], img.shape[0], sub, tw, tw, th, 1, 1, xs, ys)
         assert np.array_equal(bits(gm[ys, xs]), bits(ref.astype(np.float32)))
+
+
+@pytest.mark.parametrize("form", ["plain", "direct"])
+def test_delta_vs_fp64_4k_and_ui(cvm, orc, form):
+    """Deviation from the FP64 oracle on the 4K RGBA MatchMap and the 1080p UI
+    gray workloads: 8192 seeded windows plus the extrema.  The direct form is
+    exact to float32 rounding; the plain (reference float32 FFT) form is
+    reported, with a loose bound."""
+    rng = np.random.default_rng(5)
+    parent, sub, _ = synth.workload_map_4k(seed=3, alpha="random")
+    gray, gsub, _ = synth.workload_gray_ui_1080p(seed=2)[:3]
+    cases = [("4k-rgba", parent, sub, 4, 4), ("1080p-ui-gray", gray, gsub, 1, 1)]
+    for label, p, s, cn, step in cases:
+        ph, pw = p.shape[:2]
+        sh, sw = s.shape[:2]
+        if form == "direct":
+            m, t = cvm.match_direct(p, s, want_map=True)
+        elif step == 4:
+            m, _, _ = cvm.match_map(p, s)
+            t = cvm.match(p, s)
+        else:
+            m, t = cvm.match_u8(p, pw, pw, ph, s, sw, sw, sh, 1, 1, want_map=True)
+        rh, rw = m.shape
+        xs = np.concatenate([rng.integers(0, rw, 8192), [t[1], t[4]]]).astype(np.int32)
+        ys = np.concatenate([rng.integers(0, rh, 8192), [t[2], t[5]]]).astype(np.int32)
+        ref = orc.fp64_points(np.ascontiguousarray(p), pw * step, pw, ph, np.ascontiguousarray(s), sw * step, sw, sh,
+                              cn, step, xs, ys)
+        d = np.abs(m[ys, xs].astype(np.float64) - ref)
+        say(f"[delta] {form} {label}: max |d| = {d.max():.3e}, mean |d| = {d.mean():.3e} over {d.size} windows")
+        if form == "direct":
+            assert np.array_equal(bits(m[ys, xs]), bits(ref.astype(np.float32)))
+        else:
+            assert d.max() < 1e-3

@@ -303,6 +303,57 @@ def test_device_resident_path(cvm):
             L.cvm_device_free(p)
 
 
+@pytest.mark.parametrize("direct", [False, True])
+def test_device_resident_replays(cvm, direct):
+    """Repeated device-resident calls with the same arguments replay a
+    captured call graph: every replay (interleaved with other calls, host
+    work and synchronisation) gives the eager call's map and tuple bitwise."""
+    L = cvm.lib()
+    fn = L.cvm_match_u8_direct_device if direct else L.cvm_match_u8_device
+    rng = np.random.default_rng(77)
+    parent = synth.box_blur5(rng.integers(0, 256, size=(360, 512, 4), dtype=np.uint8))
+    subs = [parent[40:88, 100:164].copy(), parent[200:231, 300:317].copy()]
+    ph, pw = parent.shape[:2]
+    bufs = []
+
+    def dalloc(n):
+        p = C.c_void_p()
+        assert L.cvm_device_alloc(n, C.byref(p)) == 0
+        bufs.append(p)
+        return p
+
+    try:
+        dp = dalloc(parent.nbytes)
+        L.cvm_copy_h2d(dp, parent.ctypes.data_as(C.c_void_p), parent.nbytes)
+        cases = []
+        for sub in subs:
+            th, tw = sub.shape[:2]
+            dt = dalloc(sub.nbytes)
+            L.cvm_copy_h2d(dt, sub.ctypes.data_as(C.c_void_p), sub.nbytes)
+            rw, rh = pw - tw + 1, ph - th + 1
+            dm = dalloc(rw * rh * 4)
+            if direct:
+                want_m, want_t = cvm.match_direct(parent, sub, want_map=True)
+            else:
+                want_m, _, _ = cvm.match_map(parent, sub)
+                want_t = cvm.match(parent, sub)
+            cases.append((dt, dm, tw, th, rw, rh, want_m, tuple(want_t)))
+        order = [0, 0, 0, 0, 1, 1, 1, 0, 0, 0, 1, 1, 0]
+        for k, ci in enumerate(order):
+            dt, dm, tw, th, rw, rh, want_m, want_t = cases[ci]
+            out = cvm.cvm_result()
+            assert fn(dp, pw * 4, pw, ph, dt, tw * 4, tw, th, 0, 4, dm, C.byref(out), None) == 0
+            m = np.empty((rh, rw), np.float32)
+            L.cvm_copy_d2h(m.ctypes.data_as(C.c_void_p), dm, m.nbytes)
+            assert out.tuple() == want_t, (k, ci)
+            assert np.array_equal(bits(m), bits(want_m)), (k, ci)
+            if k == 5:
+                cvm.match(parent, subs[1])  # other eager work between replays
+    finally:
+        for p in bufs:
+            L.cvm_device_free(p)
+
+
 def test_repeat_determinism(cvm):
     parent, sub, _ = synth.workload_match_640(seed=14)
     a, _, _ = cvm.match_map(parent, sub)

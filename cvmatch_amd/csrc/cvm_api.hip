@@ -12,6 +12,7 @@
 #include <string.h>
 
 #include <algorithm>
+#include <array>
 #include <atomic>
 #include <map>
 #include <memory>
@@ -827,6 +828,12 @@ struct cvm_fleet {
   // and the normalisation partials per channel count
   cvm::DevBuf dstates, dscr[5];
   int dpConst = -1;
+  // batched direct launches for one frame shape and parent-alpha key:
+  // job tables (device), all members' C' maps and extrema partials
+  cvm::DevBuf bj_direct, bj_dwg, bj_norm, bj_nwg, bj_cmaps, bj_parts;
+  int bj_pw = -1, bj_ph = -1, bj_pConst = -1, bj_nnorm = 0;
+  std::vector<std::array<int, 4>> bj_groups;   // normalisation: (q, sq32, first wg entry, wg count)
+  std::vector<std::array<int, 5>> bj_dgroups;  // direct: (kstep, cmax, first wg entry, wg count, max pitch)
 };
 
 namespace cvm {
@@ -1004,12 +1011,11 @@ static int fleet_find_dev(Engine& E, hipStream_t s, cvm_fleet* f, const uint8_t*
 // (whose float32 rounding it does not share); within float32 rounding of
 // the FP64 oracle instead.
 
-static int check_direct(int tw, int th, int cnMax, int step) {
+static int check_direct(int tw, int th, int cnMax, int kstep) {
   if ((int64_t)tw * th * cnMax >= 131072)
     return fail(CVM_E_BAD_ARGS, "cvmatch: template %dx%d (cn=%d) exceeds the direct-correlation bound area*cn < 131072",
                 tw, th, cnMax);
-  if (tw > 257 || th > 257 || !norm_wave_ok(tw, th, cnMax) || direct_lds_bytes(tw, step) > 160 * 1024 ||
-      direct_nk(tw, step) + 3 * step > 128)
+  if (tw > 257 || th > 257 || !norm_wave_ok(tw, th, cnMax) || !direct_shape_ok(tw, kstep))
     return fail(CVM_E_BAD_ARGS, "cvmatch: template %dx%d is outside the direct-correlation shapes", tw, th);
   return 0;
 }
@@ -1017,8 +1023,8 @@ static int check_direct(int tw, int th, int cnMax, int step) {
 // Direct correlation + exact normalisation + extrema of one template whose
 // B fragments are ready; the call state is final on s.
 static int run_direct(Engine& E, hipStream_t s, const CallState* st, const uint8_t* d_img, int64_t istride, int iw,
-                      int ih, int tw, int th, int step, int cnMax, const int8_t* bf, void* nscratch, bool prep,
-                      float* d_map, Res6* d_res) {
+                      int ih, int tw, int th, int step, int kstep, int cnMax, const int8_t* bf, void* nscratch,
+                      bool prep, float* d_map, Res6* d_res) {
   int rw = iw - tw + 1, rh = ih - th + 1;
   float* corr = d_map;
   if (!corr) {
@@ -1029,7 +1035,7 @@ static int run_direct(Engine& E, hipStream_t s, const CallState* st, const uint8
   CHK(E.parts.ensure(sizeof(Extrema) * (size_t)nparts));
   {
     Stage S(E, "direct", s);
-    if (launch_direct(d_img, istride, iw, ih, step, bf, tw, th, rw, rh, (int32_t*)corr, s) != 0)
+    if (launch_direct(d_img, istride, iw, ih, kstep, bf, tw, th, rw, rh, (int32_t*)corr, s) != 0)
       return fail(CVM_E_BAD_ARGS, "cvmatch: template %dx%d is outside the direct-correlation shapes", tw, th);
   }
   if (prep) {
@@ -1055,9 +1061,12 @@ static int run_match_direct(Engine& E, hipStream_t s, const uint8_t* d_img, int6
                             const uint8_t* d_tpl, int64_t tstride, int tw, int th, int cn_mode, int step,
                             float* d_map) {
   int cnMax = cn_mode ? cn_mode : 4;
-  CHK(check_direct(tw, th, cnMax, step));
+  // with the alpha rule cn is decided on the device: stage all four bytes
+  // (the fragments zero the alpha column when cn comes out 3)
+  const int kstep = direct_kstep(step, cnMax);
+  CHK(check_direct(tw, th, cnMax, kstep));
   CHK(E.nscratch.ensure(norm_scratch_bytes(iw, ih, tw, th, cnMax)));
-  CHK(E.dbf.ensure(direct_bfrag_bytes(tw, th, step)));
+  CHK(E.dbf.ensure(direct_bfrag_bytes(tw, th, kstep)));
   {
     Stage S(E, "stats", s);
     launch_state_init(E.st, s);
@@ -1070,18 +1079,19 @@ static int run_match_direct(Engine& E, hipStream_t s, const uint8_t* d_img, int6
   }
   {
     Stage S(E, "bfrag", s);
-    launch_bfrag(d_tpl, tstride, tw, th, step, cnMax, E.st, E.dbf.as<int8_t>(), s);
+    launch_bfrag(d_tpl, tstride, tw, th, kstep, cnMax, E.st, E.dbf.as<int8_t>(), s);
   }
-  return run_direct(E, s, E.st, d_img, istride, iw, ih, tw, th, step, cnMax, E.dbf.as<int8_t>(), E.nscratch.p, true,
-                    d_map, E.res_d);
+  return run_direct(E, s, E.st, d_img, istride, iw, ih, tw, th, step, kstep, cnMax, E.dbf.as<int8_t>(), E.nscratch.p,
+                    true, d_map, E.res_d);
 }
 
 // B fragments of a matcher for cn channels (cached; cn known on the host).
 static int matcher_bfrag(Engine& E, hipStream_t s, cvm_matcher* m, int cn, const int8_t** out) {
   std::lock_guard<std::mutex> lk(m->mu);
   if (!m->dbf_ok[cn]) {
-    CHK(m->dbf[cn].ensure(direct_bfrag_bytes(m->w, m->h, m->step)));
-    launch_bfrag(m->d_pix, (int64_t)m->w * m->step, m->w, m->h, m->step, cn, nullptr, m->dbf[cn].as<int8_t>(), s);
+    const int kstep = direct_kstep(m->step, cn);
+    CHK(m->dbf[cn].ensure(direct_bfrag_bytes(m->w, m->h, kstep)));
+    launch_bfrag(m->d_pix, (int64_t)m->w * m->step, m->w, m->h, kstep, cn, nullptr, m->dbf[cn].as<int8_t>(), s);
     HIPCHK(hipGetLastError());
     m->dbf_ok[cn] = true;
   }
@@ -1095,59 +1105,192 @@ static int matcher_bfrag(Engine& E, hipStream_t s, cvm_matcher* m, int cn, const
 static int matcher_find_direct_dev(Engine& E, hipStream_t s, cvm_matcher* m, const uint8_t* d_img, int64_t istride,
                                    int iw, int ih, Res6* d_res) {
   int cnMax = m->step;
-  CHK(check_direct(m->w, m->h, cnMax, m->step));
+  const int kstep = direct_kstep(m->step, cnMax);
+  CHK(check_direct(m->w, m->h, cnMax, kstep));
   CHK(E.nscratch.ensure(norm_scratch_bytes(iw, ih, m->w, m->h, cnMax)));
   matcher_state(E, s, E.st, m, d_img, istride, iw, ih);
   const int8_t* bf = nullptr;
   if (m->step == 4 && m->aConst) {
-    CHK(E.dbf.ensure(direct_bfrag_bytes(m->w, m->h, m->step)));
-    launch_bfrag(m->d_pix, (int64_t)m->w * m->step, m->w, m->h, m->step, 4, E.st, E.dbf.as<int8_t>(), s);
+    CHK(E.dbf.ensure(direct_bfrag_bytes(m->w, m->h, kstep)));
+    launch_bfrag(m->d_pix, (int64_t)m->w * m->step, m->w, m->h, kstep, 4, E.st, E.dbf.as<int8_t>(), s);
     bf = E.dbf.as<int8_t>();
   } else {
     CHK(matcher_bfrag(E, s, m, cnMax, &bf));
   }
-  return run_direct(E, s, E.st, d_img, istride, iw, ih, m->w, m->h, m->step, cnMax, bf, E.nscratch.p, true, nullptr,
-                    d_res);
+  return run_direct(E, s, E.st, d_img, istride, iw, ih, m->w, m->h, m->step, kstep, cnMax, bf, E.nscratch.p, true,
+                    nullptr, d_res);
 }
 
 // Fleet.FindAll in the direct form: every non-flat member against the frame
 // (results independent of the bucketing, each equal to the member's direct
-// Find); the normalisation partials are built once per channel count.
+// Find).  One launch per frame correlates all members (largest first), one
+// per wave-form variant normalises them, one reduces their extrema; the
+// normalisation partials are built once per channel count.
 static int fleet_find_direct_dev(Engine& E, hipStream_t s, cvm_fleet* f, const uint8_t* d_img, int64_t istride,
                                  int pw, int ph, int pConst, Res6* d_res) {
   size_t n = f->ms.size();
-  for (size_t i = 0; i < n; i++)
-    if (f->ms[i]->varSum != 0) CHK(check_direct(f->ms[i]->w, f->ms[i]->h, f->ms[i]->step, f->ms[i]->step));
   CHK(f->dstates.ensure(sizeof(CallState) * n));
   CallState* states = f->dstates.as<CallState>();
+  auto cn_of = [&](const cvm_matcher* m) { return m->step == 4 ? ((m->aConst && pConst) ? 3 : 4) : 1; };
+  for (size_t i = 0; i < n; i++)
+    if (f->ms[i]->varSum != 0) {
+      cvm_matcher* m = f->ms[i];
+      CHK(check_direct(m->w, m->h, cn_of(m), direct_kstep(m->step, cn_of(m))));
+    }
   if (f->dpConst != pConst) {
     for (size_t i = 0; i < n; i++) {
       cvm_matcher* m = f->ms[i];
       if (m->varSum == 0) continue;
-      int cn = m->step == 4 ? ((m->aConst && pConst) ? 3 : 4) : 1;
       launch_state_set(states + i, m->stats, 0, s);
-      launch_state_finalize(states + i, m->w, m->h, cn, s);
+      launch_state_finalize(states + i, m->w, m->h, cn_of(m), s);
     }
     HIPCHK(hipGetLastError());
     f->dpConst = pConst;
   }
-  bool prepped[5] = {false, false, false, false, false};
-  for (size_t i = 0; i < n; i++) {
-    cvm_matcher* m = f->ms[i];
-    if (m->varSum == 0) continue;
-    int cn = m->step == 4 ? ((m->aConst && pConst) ? 3 : 4) : 1;
-    // partials depend on the frame and cn only; sized for the widest use
-    CHK(f->dscr[cn].ensure(norm_scratch_bytes(pw, ph, 1, 1, cn)));
-    if (!prepped[cn]) {
-      Stage S(E, "norm_prep", s);
-      launch_norm_prep(d_img, istride, pw, ph, f->step, m->w, m->h, cn, states + i, f->dscr[cn].p, s);
-      prepped[cn] = true;
+  bool used[5] = {false, false, false, false, false};
+  std::vector<size_t> live;
+  for (size_t i = 0; i < n; i++)
+    if (f->ms[i]->varSum != 0) {
+      live.push_back(i);
+      used[cn_of(f->ms[i])] = true;
     }
-    const int8_t* bf = nullptr;
-    CHK(matcher_bfrag(E, s, m, cn, &bf));
-    CHK(run_direct(E, s, states + i, d_img, istride, pw, ph, m->w, m->h, m->step, cn, bf, f->dscr[cn].p, false,
-                   nullptr, d_res + i));
+  if (live.empty()) return 0;
+  for (int cn = 1; cn <= 4; cn++)
+    if (used[cn]) CHK(f->dscr[cn].ensure(norm_scratch_bytes(pw, ph, 1, 1, cn)));
+
+  if (f->bj_pw != pw || f->bj_ph != ph || f->bj_pConst != pConst) {
+    // job tables for this frame shape
+    std::vector<DirectJob> dj;
+    std::vector<NormJob> nj;
+    std::vector<int> order(live.size());
+    for (size_t k = 0; k < live.size(); k++) order[k] = (int)k;
+    std::vector<int64_t> cost(live.size());
+    size_t cm_total = 0, pt_total = 0;
+    std::vector<size_t> cm_off(live.size()), pt_off(live.size());
+    std::vector<int> qs(live.size()), sqs(live.size());
+    std::vector<int> ks(live.size()), cms(live.size());
+    for (size_t k = 0; k < live.size(); k++) {
+      cvm_matcher* m = f->ms[live[k]];
+      int rw = pw - m->w + 1, rh = ph - m->h + 1;
+      ks[k] = direct_kstep(m->step, cn_of(m));
+      cms[k] = direct_cmax(direct_nk(m->w, ks[k]), ks[k]);
+      cost[k] = (int64_t)m->h * direct_nk(m->w, ks[k]) * direct_grid_x(rw) * direct_grid_y(rh);
+      cm_off[k] = cm_total;
+      cm_total += (size_t)rw * rh;
+      norm_form(m->w, m->h, cn_of(m), &qs[k], &sqs[k]);
+      pt_off[k] = pt_total;
+      pt_total += (size_t)norm_grid_x(rw, qs[k]) * norm_grid_y(rh);
+    }
+    std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return cost[x] > cost[y]; });
+    CHK(f->bj_cmaps.ensure(sizeof(int32_t) * cm_total));
+    CHK(f->bj_parts.ensure(sizeof(Extrema) * pt_total));
+    int32_t* cmaps = f->bj_cmaps.as<int32_t>();
+    Extrema* parts = f->bj_parts.as<Extrema>();
+    // direct jobs grouped by (kstep, cmax), largest first inside a group
+    std::vector<int> dwg;
+    std::vector<std::array<int, 5>> dgroups;  // (kstep, cmax, first wg, wg count, max pitch)
+    for (int kst : {4, 3, 1})
+      for (int cm : {9, 6, 5, 4, 3, 2}) {
+        int first = (int)dwg.size(), max_pitch = 0;
+        for (int k : order) {
+          if (ks[k] != kst || cms[k] != cm) continue;
+          cvm_matcher* m = f->ms[live[k]];
+          int rw = pw - m->w + 1, rh = ph - m->h + 1;
+          const int8_t* bf = nullptr;
+          CHK(matcher_bfrag(E, s, m, cn_of(m), &bf));
+          DirectJob J{};
+          J.bf = bf;
+          J.out = cmaps + cm_off[k];
+          J.th = m->h;
+          J.nk = direct_nk(m->w, kst);
+          J.pitch = direct_pitch_of(m->w, kst);
+          J.rw = rw;
+          J.rh = rh;
+          J.gx = direct_grid_x(rw);
+          J.wg0 = (int)dwg.size() - first;
+          max_pitch = std::max(max_pitch, J.pitch);
+          int nb = J.gx * direct_grid_y(rh);
+          for (int b = 0; b < nb; b++) dwg.push_back((int)dj.size());
+          dj.push_back(J);
+        }
+        if ((int)dwg.size() > first) dgroups.push_back({kst, cm, first, (int)dwg.size() - first, max_pitch});
+      }
+    // normalisation jobs grouped by wave-form variant; the reduction takes
+    // the whole job array
+    std::vector<std::array<int, 4>> groups;
+    std::vector<int> nwg;
+    for (int q : {4, 2})
+      for (int sq : {1, 0}) {
+        int first = (int)nwg.size();
+        for (int k : order) {
+          if (qs[k] != q || sqs[k] != sq) continue;
+          cvm_matcher* m = f->ms[live[k]];
+          int cn = cn_of(m);
+          int rw = pw - m->w + 1, rh = ph - m->h + 1;
+          NormJob J{};
+          J.st = states + live[k];
+          J.ps = f->dscr[cn].as<uint32_t>();
+          J.corr = (const float*)(cmaps + cm_off[k]);
+          J.parts = parts + pt_off[k];
+          J.res = d_res + live[k];
+          J.tw = m->w;
+          J.th = m->h;
+          J.rw = rw;
+          J.rh = rh;
+          J.gx = norm_grid_x(rw, q);
+          J.gy = norm_grid_y(rh);
+          J.wg0 = (int)nwg.size() - first;
+          for (int b = 0; b < J.gx * J.gy; b++) nwg.push_back((int)nj.size());
+          nj.push_back(J);
+        }
+        if ((int)nwg.size() > first) groups.push_back({q, sq, first, (int)nwg.size() - first});
+      }
+    // the wg0 of a norm job counts from its group's first entry
+    CHK(f->bj_direct.ensure(sizeof(DirectJob) * dj.size()));
+    CHK(f->bj_dwg.ensure(sizeof(int) * dwg.size()));
+    CHK(f->bj_norm.ensure(sizeof(NormJob) * nj.size()));
+    CHK(f->bj_nwg.ensure(sizeof(int) * nwg.size()));
+    HIPCHK(hipMemcpyAsync(f->bj_direct.p, dj.data(), sizeof(DirectJob) * dj.size(), hipMemcpyHostToDevice, s));
+    HIPCHK(hipMemcpyAsync(f->bj_dwg.p, dwg.data(), sizeof(int) * dwg.size(), hipMemcpyHostToDevice, s));
+    HIPCHK(hipMemcpyAsync(f->bj_norm.p, nj.data(), sizeof(NormJob) * nj.size(), hipMemcpyHostToDevice, s));
+    HIPCHK(hipMemcpyAsync(f->bj_nwg.p, nwg.data(), sizeof(int) * nwg.size(), hipMemcpyHostToDevice, s));
+    HIPCHK(hipStreamSynchronize(s));  // the host vectors go out of scope
+    f->bj_dgroups = dgroups;
+    f->bj_nnorm = (int)nj.size();
+    f->bj_groups = groups;
+    f->bj_pw = pw;
+    f->bj_ph = ph;
+    f->bj_pConst = pConst;
   }
+
+  for (int cn = 1; cn <= 4; cn++) {
+    if (!used[cn]) continue;
+    for (size_t i : live)
+      if (cn_of(f->ms[i]) == cn) {
+        Stage S(E, "norm_prep", s);
+        launch_norm_prep(d_img, istride, pw, ph, f->step, f->ms[i]->w, f->ms[i]->h, cn, states + i, f->dscr[cn].p,
+                         s);
+        break;
+      }
+  }
+  {
+    Stage S(E, "direct", s);
+    for (auto& g : f->bj_dgroups)
+      if (launch_direct_batch(d_img, istride, pw, ph, g[0], g[1], f->bj_direct.as<DirectJob>(),
+                              f->bj_dwg.as<int>() + g[2], g[3], g[4], s) != 0)
+        return fail(CVM_E_BAD_ARGS, "cvmatch: fleet members outside the direct-correlation shapes");
+  }
+  {
+    Stage S(E, "normalize", s);
+    for (auto& g : f->bj_groups)
+      launch_norm_batch(d_img, istride, pw, ph, f->step, g[0], g[1], true, f->bj_norm.as<NormJob>(),
+                        f->bj_nwg.as<int>() + g[2], g[3], s);
+  }
+  {
+    Stage S(E, "reduce", s);
+    launch_reduce_batch(f->bj_norm.as<NormJob>(), f->bj_nnorm, s);
+  }
+  HIPCHK(hipGetLastError());
   return 0;
 }
 

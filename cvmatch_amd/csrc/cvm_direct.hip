@@ -8,17 +8,19 @@
 // from the exact window sums it already has.
 //
 // GEMM shape.  One MFMA tile is 32 output rows (M) x 32 output columns (N).
-// K runs over the raw bytes of one image row segment, channels interleaved:
-// for template row i, A_i[m][u] = P'[y0 + m + i][step*x0 + u] and
-//   B_i[u][n] = T'[i][u - step*n]   when 0 <= u - step*n < step*tw and the
+// K runs over the staged bytes of one image row segment, channels
+// interleaved: KSTEP bytes per pixel, 1 for gray, 4 for RGBA, 3 for RGBA
+// staged without alpha (cn = 3).  For template row i,
+//   A_i[m][u] = P'[y0 + m + i][KSTEP*x0 + u] and
+//   B_i[u][n] = T'[i][u - KSTEP*n]  when 0 <= u - KSTEP*n < KSTEP*tw and the
 //                                   byte's channel is below cn, else 0,
 // a banded Toeplitz matrix that is the same for every tile.  Summing the
 // th products A_i B_i into one accumulator gives the 32 x 32 block.  B_i is
 // built once per template in MFMA fragment order (k_bfrag).  The A fragment
-// of column tile t and K chunk kc is chunk kc + step*t of the row segment, so
-// a wave walking kc in steps of `step` slides a window of T A fragments by one
-// load per T MFMAs.  Image rows stream through an LDS ring, one new row per
-// template row.
+// of column tile t and K chunk kc is chunk kc + KSTEP*t of the row segment, so
+// a wave walking kc in steps of KSTEP (one residue class at a time) slides a
+// window of T A fragments by one load per T MFMAs.  Image rows stream through
+// an LDS ring, one new row per template row.
 //
 // v_mfma_i32_32x32x32_i8 lane layout: A byte b of lane l is A[l%32][16(l/32)+b],
 // B byte b of lane l is B[16(l/32)+b][l%32], and accumulator register j of
@@ -34,33 +36,44 @@ constexpr int DIR_T = 4;      // column tiles per wave (128 output columns)
 constexpr int DIR_WAVES = 2;  // waves per workgroup, 32 output rows each
 constexpr int DIR_ROWS = 32 * DIR_WAVES;
 constexpr int DIR_RING = DIR_ROWS + 8;
-constexpr int DIR_CMAX = 9;   // K chunks per residue class: nk <= step * DIR_CMAX
+constexpr int DIR_CMAX = 9;   // K chunks per residue class: nk <= kstep * DIR_CMAX
 constexpr int DIR_PF = 4;     // ring rows in flight (<= DIR_RING - DIR_ROWS)
 
-int direct_nk(int tw, int step) { return (step * (32 + tw - 1) + 31) / 32; }
+// Staged bytes per pixel: gray 1, RGBA 4, RGBA with cn = 3 staged as RGB.
+int direct_kstep(int step, int cn) { return step == 1 ? 1 : (cn == 3 ? 3 : 4); }
+
+int direct_nk(int tw, int kstep) { return (kstep * (32 + tw - 1) + 31) / 32; }
 
 // B fragments plus DIR_CMAX of padding: a class load may run past the last
 // fragment (those registers are never used).
-size_t direct_bfrag_bytes(int tw, int th, int step) {
-  return ((size_t)th * direct_nk(tw, step) + DIR_CMAX) * 64 * 16;
+size_t direct_bfrag_bytes(int tw, int th, int kstep) {
+  return ((size_t)th * direct_nk(tw, kstep) + DIR_CMAX) * 64 * 16;
 }
 
-static int direct_pitch(int nk, int step) {
+static int stage_unit(int kstep) { return kstep == 3 ? 12 : 16; }  // staged bytes per 16 source bytes
+
+static int direct_units(int nk, int kstep) {
+  const int bytes = 32 * (nk + kstep * (DIR_T - 1));
+  const int spu = stage_unit(kstep);
+  return (bytes + spu - 1) / spu;
+}
+
+static int direct_pitch(int nk, int kstep) {
   // staged row bytes, padded so consecutive rows start 16 bytes (4 banks)
   // apart modulo 128: the 32 rows of one A fragment spread over all banks
-  int bytes = 32 * (nk + step * (DIR_T - 1));
+  int bytes = direct_units(nk, kstep) * stage_unit(kstep);
   return (bytes + 127) / 128 * 128 + 16;
 }
 
-size_t direct_lds_bytes(int tw, int step) { return (size_t)DIR_RING * direct_pitch(direct_nk(tw, step), step); }
+size_t direct_lds_bytes(int tw, int kstep) { return (size_t)DIR_RING * direct_pitch(direct_nk(tw, kstep), kstep); }
 
 // Visit order of the K chunks within one template row: residue classes of
-// kc modulo step, each in increasing kc (the A window slides inside a class).
-__host__ __device__ inline void direct_visit(int s, int nk, int step, int& rc, int& kc) {
-  for (rc = 0; rc < step; rc++) {
-    const int c = (nk - rc + step - 1) / step;
+// kc modulo kstep, each in increasing kc (the A window slides inside a class).
+__host__ __device__ inline void direct_visit(int s, int nk, int kstep, int& rc, int& kc) {
+  for (rc = 0; rc < kstep; rc++) {
+    const int c = (nk - rc + kstep - 1) / kstep;
     if (s < c) {
-      kc = rc + step * s;
+      kc = rc + kstep * s;
       return;
     }
     s -= c;
@@ -70,8 +83,9 @@ __host__ __device__ inline void direct_visit(int s, int nk, int step, int& rc, i
 
 // B_i fragments in visit order: bf[((i * nk + s) * 64 + lane) * 16 + b] =
 // B_i[32 kc + 16 (lane/32) + b][lane % 32] with kc the s-th visited chunk.
-__global__ void k_bfrag(const uint8_t* __restrict__ tpl, int64_t tstride, int tw, int th, int step, int cn,
-                        const CallState* __restrict__ st, int nk, int8_t* __restrict__ bf) {
+// The template has sstep bytes per pixel (4 when kstep is 3 or 4).
+__global__ void k_bfrag(const uint8_t* __restrict__ tpl, int64_t tstride, int tw, int th, int kstep, int sstep,
+                        int cn, const CallState* __restrict__ st, int nk, int8_t* __restrict__ bf) {
   const int64_t id = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;  // one 16-byte fragment slice
   const int64_t total = (int64_t)th * nk * 64;
   if (id >= total) return;
@@ -80,50 +94,52 @@ __global__ void k_bfrag(const uint8_t* __restrict__ tpl, int64_t tstride, int tw
   const int64_t is = id >> 6;
   const int sv = (int)(is % nk), i = (int)(is / nk);
   int rc, kc;
-  direct_visit(sv, nk, step, rc, kc);
+  direct_visit(sv, nk, kstep, rc, kc);
   const int n = lane & 31, h = lane >> 5;
   uint32_t w[4] = {0, 0, 0, 0};
   for (int b = 0; b < 16; b++) {
     const int u = 32 * kc + 16 * h + b;
-    const int d = u - step * n;
+    const int d = u - kstep * n;
     uint32_t v = 0;
-    if (d >= 0 && d < step * tw && (d % step) < cn) v = (uint32_t)(tpl[(int64_t)i * tstride + d] ^ 0x80u) & 255u;
+    if (d >= 0 && d < kstep * tw) {
+      const int j = d / kstep, k = d - j * kstep;
+      if (k < cn) v = (uint32_t)(tpl[(int64_t)i * tstride + (int64_t)j * sstep + k] ^ 0x80u) & 255u;
+    }
     w[b >> 2] |= v << (8 * (b & 3));
   }
   uint4* o = (uint4*)(bf + id * 16);
   *o = make_uint4(w[0], w[1], w[2], w[3]);
 }
 
-void launch_bfrag(const uint8_t* tpl, int64_t tstride, int tw, int th, int step, int cn, const CallState* st,
+void launch_bfrag(const uint8_t* tpl, int64_t tstride, int tw, int th, int kstep, int cn, const CallState* st,
                   int8_t* bf, hipStream_t s) {
-  int nk = direct_nk(tw, step);
+  int nk = direct_nk(tw, kstep);
   int64_t total = (int64_t)th * nk * 64;
-  hipLaunchKernelGGL(k_bfrag, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, s, tpl, tstride, tw, th, step, cn,
-                     st, nk, bf);
+  hipLaunchKernelGGL(k_bfrag, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, s, tpl, tstride, tw, th, kstep,
+                     kstep == 1 ? 1 : 4, cn, st, nk, bf);
 }
 
 // One workgroup: output rows [Y0, Y0 + DIR_ROWS) x columns [X0, X0 + 32 T).
-// VEC: image rows 16-byte aligned (16-byte staging loads).
-template <int STEP, bool VEC>
-__global__ __launch_bounds__(64 * DIR_WAVES) void k_direct_i8(const uint8_t* __restrict__ img, int64_t istride,
-                                                              int iw, int ih, const int8_t* __restrict__ bf, int th,
-                                                              int nk, int pitch, int rw, int rh,
-                                                              int32_t* __restrict__ out) {
+// VEC: image rows 16-byte aligned with sstep*iw % 16 == 0 (16-byte loads).
+// CMAX: chunks per residue class, ceil(nk / KSTEP) <= CMAX.
+template <int KSTEP, bool VEC, int CMAX>
+__device__ __forceinline__ void direct_block(uint8_t* ring, const uint8_t* __restrict__ img, int64_t istride, int iw,
+                                             int ih, const int8_t* __restrict__ bf, int th, int nk, int pitch, int rw,
+                                             int rh, int32_t* __restrict__ out, int bx, int by) {
   constexpr int T = DIR_T;
-  extern __shared__ uint4 ring4[];
-  uint8_t* ring = (uint8_t*)ring4;
+  constexpr int SSTEP = KSTEP == 1 ? 1 : 4;  // source bytes per pixel
+  constexpr int SPU = KSTEP == 3 ? 12 : 16;  // staged bytes per 16 source bytes
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  const int X0 = blockIdx.x * 32 * T, Y0 = blockIdx.y * DIR_ROWS;
-  const int rowChunks = 2 * (nk + STEP * (T - 1));  // 16-byte chunks per staged row
-  const int64_t xb0 = (int64_t)STEP * X0;
-  const int64_t imgBytes = (int64_t)STEP * iw;
+  const int X0 = bx * 32 * T, Y0 = by * DIR_ROWS;
+  const int units = (32 * (nk + KSTEP * (T - 1)) + SPU - 1) / SPU;  // staging units per row
+  const int64_t xb0 = (int64_t)SSTEP * X0;
+  const int64_t imgBytes = (int64_t)SSTEP * iw;
 
-  // 16 bytes of staged row rho (image row Y0 + rho), chunk c.  VEC: the
-  // caller guarantees 16-byte alignment and STEP*iw % 16 == 0, so a chunk is
-  // wholly inside or wholly outside the image; the load is clamped into the
-  // image and `ok` says whether to keep it.  The int8 shift (v ^ 0x80) and
-  // the zeroing are applied when the chunk is written (cook), so the load
-  // itself is never waited on early.
+  // The 16 source bytes of staging unit c of staged row rho (image row
+  // Y0 + rho).  VEC: a unit is wholly inside or wholly outside the image; the
+  // load is clamped into the image and `ok` says whether to keep it.  The
+  // int8 shift, the zeroing and the RGB repacking are applied when the unit
+  // is written (put), so the load itself is never waited on early.
   auto fetch = [&](int rho, int c, bool& ok) -> uint4 {
     const int r = Y0 + rho;
     const int64_t b0 = xb0 + 16 * (int64_t)c;
@@ -138,21 +154,35 @@ __global__ __launch_bounds__(64 * DIR_WAVES) void k_direct_i8(const uint8_t* __r
     if (r < ih) {
       const uint8_t* p = img + (int64_t)r * istride;
       for (int b = 0; b < 16; b++)
-        if (b0 + b < imgBytes) w[b >> 2] = (w[b >> 2] & ~(255u << (8 * (b & 3)))) | ((uint32_t)p[b0 + b] << (8 * (b & 3)));
+        if (b0 + b < imgBytes)
+          w[b >> 2] = (w[b >> 2] & ~(255u << (8 * (b & 3)))) | ((uint32_t)p[b0 + b] << (8 * (b & 3)));
     }
     return make_uint4(w[0], w[1], w[2], w[3]);
   };
-  auto cook = [](uint4 v, bool ok) -> uint4 {
-    if (!ok) return make_uint4(0, 0, 0, 0);
-    return make_uint4(v.x ^ 0x80808080u, v.y ^ 0x80808080u, v.z ^ 0x80808080u, v.w ^ 0x80808080u);
+  // write unit c of a staged row whose first byte is dst (outside bytes: 0)
+  auto put = [&](uint8_t* dst, int c, uint4 v, bool ok) {
+    if (!ok) v = make_uint4(0x80808080u, 0x80808080u, 0x80808080u, 0x80808080u);
+    v.x ^= 0x80808080u;
+    v.y ^= 0x80808080u;
+    v.z ^= 0x80808080u;
+    v.w ^= 0x80808080u;
+    if (KSTEP == 3) {
+      // four RGBA pixels -> twelve RGB bytes
+      uint32_t* o = (uint32_t*)(dst + 12 * c);
+      o[0] = (v.x & 0x00FFFFFFu) | (v.y << 24);
+      o[1] = ((v.y >> 8) & 0x0000FFFFu) | (v.z << 16);
+      o[2] = ((v.z >> 16) & 0x000000FFu) | (v.w << 8);
+    } else {
+      *(uint4*)(dst + 16 * c) = v;
+    }
   };
 
   // initial rows [0, DIR_ROWS)
-  for (int e = tid; e < DIR_ROWS * rowChunks; e += 64 * DIR_WAVES) {
-    const int rho = e / rowChunks, c = e - rho * rowChunks;
+  for (int e = tid; e < DIR_ROWS * units; e += 64 * DIR_WAVES) {
+    const int rho = e / units, c = e - rho * units;
     bool ok;
     const uint4 v = fetch(rho, c, ok);
-    *(uint4*)(ring + rho * pitch + 16 * c) = cook(v, ok);
+    put(ring + rho * pitch, c, v, ok);
   }
   __syncthreads();
 
@@ -163,38 +193,38 @@ __global__ __launch_bounds__(64 * DIR_WAVES) void k_direct_i8(const uint8_t* __r
     for (int j = 0; j < 16; j++) acc[t][j] = 0;
 
   const int half = lane >> 5, m = lane & 31;
-  // Per template row, K chunk residue class rc has c_rc = ceil((nk - rc) / STEP)
-  // chunks; its c_rc + T - 1 A fragments and c_rc B fragments sit in
+  // Per template row, K chunk residue class rc has c_rc = ceil((nk - rc) / KSTEP)
+  // <= CMAX chunks; its c_rc + T - 1 A fragments and c_rc B fragments sit in
   // statically indexed registers, the next class's loads issued before this
-  // class's MFMAs, the next row's first B class before the barrier.
-  constexpr int NA = DIR_CMAX + T - 1;
+  // class's MFMAs, the next row's first B class before the barrier.  Loads
+  // are unconditional (A chunk indices clamp to the staged row, B runs into
+  // the padding), so the compiler's wait counts are exact.
+  constexpr int NA = CMAX + T - 1;
   const dv4i* bq = (const dv4i*)bf + lane;
-  int cls[STEP], base[STEP];
+  int cls[KSTEP], base[KSTEP];
   {
     int acc_s = 0;
 #pragma unroll
-    for (int rc = 0; rc < STEP; rc++) {
-      cls[rc] = rc < nk ? (nk - rc + STEP - 1) / STEP : 0;
+    for (int rc = 0; rc < KSTEP; rc++) {
+      cls[rc] = rc < nk ? (nk - rc + KSTEP - 1) / KSTEP : 0;
       base[rc] = acc_s;
       acc_s += cls[rc];
     }
   }
-  dv4i A0[NA], A1[NA], B0[DIR_CMAX], B1[DIR_CMAX], BN[DIR_CMAX];
-  // unconditional loads (a fixed count the compiler can wait on exactly):
-  // A chunk indices clamp to the staged row, B runs into the padding
-  const int lastChunk = nk + STEP * (T - 1) - 1;
+  dv4i A0[NA], A1[NA], B0[CMAX], B1[CMAX], BN[CMAX];
+  const int lastChunk = nk + KSTEP * (T - 1) - 1;
   auto loadA = [&](dv4i* buf, const uint8_t* arow, int rc) {
 #pragma unroll
-    for (int j = 0; j < NA; j++) buf[j] = *(const dv4i*)(arow + 32 * min(rc + STEP * j, lastChunk));
+    for (int j = 0; j < NA; j++) buf[j] = *(const dv4i*)(arow + 32 * min(rc + KSTEP * j, lastChunk));
   };
   auto loadB = [&](dv4i* buf, int i, int rc) {
     const dv4i* p = bq + ((int64_t)i * nk + base[rc]) * 64;
 #pragma unroll
-    for (int j = 0; j < DIR_CMAX; j++) buf[j] = p[(int64_t)j * 64];
+    for (int j = 0; j < CMAX; j++) buf[j] = p[(int64_t)j * 64];
   };
   auto mma = [&](const dv4i* a, const dv4i* bb, int c) {
 #pragma unroll
-    for (int j = 0; j < DIR_CMAX; j++)
+    for (int j = 0; j < CMAX; j++)
       if (j < c)
 #pragma unroll
         for (int t = 0; t < T; t++) acc[t] = __builtin_amdgcn_mfma_i32_32x32x32_i8(a[j + t], bb[j], acc[t], 0, 0, 0);
@@ -202,13 +232,13 @@ __global__ __launch_bounds__(64 * DIR_WAVES) void k_direct_i8(const uint8_t* __r
 
   // Staged row DIR_ROWS + i (first read in step i + 1) is fetched in step
   // i - DIR_PF + 1 into a register queue and written to its ring slot at the
-  // end of step i, so DIR_PF steps of MFMAs cover its load latency.
-  // chunk nc[p] of each staged row is this thread's (chunks past the row
-  // reload the last one and are not written)
+  // end of step i, so DIR_PF steps of MFMAs cover its load latency.  Unit
+  // nc[p] of each staged row is this thread's (units past the row reload the
+  // last one and are not written).
   int nc[2];
 #pragma unroll
   for (int p = 0; p < 2; p++) nc[p] = tid + p * 64 * DIR_WAVES;
-  const int ncl[2] = {min(nc[0], rowChunks - 1), min(nc[1], rowChunks - 1)};
+  const int ncl[2] = {min(nc[0], units - 1), min(nc[1], units - 1)};
   uint4 rq[DIR_PF][2];
   bool rok[DIR_PF][2];
 #pragma unroll
@@ -225,15 +255,15 @@ __global__ __launch_bounds__(64 * DIR_WAVES) void k_direct_i8(const uint8_t* __r
     const int slot = (32 * wave + m + i) % DIR_RING;
     const uint8_t* arow = ring + slot * pitch + 16 * half;
 #pragma unroll
-    for (int j = 0; j < DIR_CMAX; j++) B0[j] = BN[j];
+    for (int j = 0; j < CMAX; j++) B0[j] = BN[j];
     loadA(A0, arow, 0);
 #pragma unroll
-    for (int rc = 0; rc < STEP; rc++) {
+    for (int rc = 0; rc < KSTEP; rc++) {
       dv4i* ca = (rc & 1) ? A1 : A0;
       dv4i* cb = (rc & 1) ? B1 : B0;
       dv4i* na = (rc & 1) ? A0 : A1;
       dv4i* nb = (rc & 1) ? B0 : B1;
-      if (rc + 1 < STEP) {
+      if (rc + 1 < KSTEP) {
         loadA(na, arow, rc + 1);
         loadB(nb, i, rc + 1);
       } else if (more) {
@@ -247,7 +277,7 @@ __global__ __launch_bounds__(64 * DIR_WAVES) void k_direct_i8(const uint8_t* __r
       const int ns = (DIR_ROWS + i) % DIR_RING;
 #pragma unroll
       for (int p = 0; p < 2; p++)
-        if (nc[p] < rowChunks) *(uint4*)(ring + ns * pitch + 16 * nc[p]) = cook(rq[0][p], rok[0][p]);
+        if (nc[p] < units) put(ring + ns * pitch, nc[p], rq[0][p], rok[0][p]);
     }
 #pragma unroll
     for (int d = 0; d < DIR_PF - 1; d++)
@@ -273,34 +303,105 @@ __global__ __launch_bounds__(64 * DIR_WAVES) void k_direct_i8(const uint8_t* __r
   }
 }
 
-int launch_direct(const uint8_t* img, int64_t istride, int iw, int ih, int step, const int8_t* bf, int tw, int th,
-                  int rw, int rh, int32_t* out, hipStream_t s) {
-  const int nk = direct_nk(tw, step);
-  const int pitch = direct_pitch(nk, step);
-  if (32 * (nk + step * (DIR_T - 1)) > pitch || 2 * 64 * DIR_WAVES < 2 * (nk + step * (DIR_T - 1)) ||
-      nk > step * DIR_CMAX)
-    return -1;
-  const size_t lds = (size_t)DIR_RING * pitch;
-  if (lds > 160 * 1024) return -1;
-  const bool vec = ((((uintptr_t)img) | (uintptr_t)istride | (uintptr_t)(step * iw)) & 15) == 0;
-  dim3 grid((rw + 32 * DIR_T - 1) / (32 * DIR_T), (rh + DIR_ROWS - 1) / DIR_ROWS), block(64 * DIR_WAVES);
-#define CVM_DI(S, V) \
-  hipLaunchKernelGGL((k_direct_i8<S, V>), grid, block, lds, s, img, istride, iw, ih, bf, th, nk, pitch, rw, rh, out)
-  if (step == 4) {
-    if (vec)
-      CVM_DI(4, true);
-    else
-      CVM_DI(4, false);
-  } else if (step == 1) {
-    if (vec)
-      CVM_DI(1, true);
-    else
-      CVM_DI(1, false);
-  } else {
-    return -1;
+template <int KSTEP, bool VEC, int CMAX>
+__global__ __launch_bounds__(64 * DIR_WAVES) void k_direct_i8(const uint8_t* __restrict__ img, int64_t istride,
+                                                              int iw, int ih, const int8_t* __restrict__ bf, int th,
+                                                              int nk, int pitch, int rw, int rh,
+                                                              int32_t* __restrict__ out) {
+  extern __shared__ uint4 ring4[];
+  direct_block<KSTEP, VEC, CMAX>((uint8_t*)ring4, img, istride, iw, ih, bf, th, nk, pitch, rw, rh, out, blockIdx.x,
+                                 blockIdx.y);
+}
+
+// Many templates against one frame in one launch (a Fleet's members, all of
+// one (KSTEP, CMAX) form): workgroup b runs block b - job.wg0 of job wg_job[b].
+template <int KSTEP, bool VEC, int CMAX>
+__global__ __launch_bounds__(64 * DIR_WAVES) void k_direct_batch(const uint8_t* __restrict__ img, int64_t istride,
+                                                                 int iw, int ih, const DirectJob* __restrict__ jobs,
+                                                                 const int* __restrict__ wg_job) {
+  extern __shared__ uint4 ring4[];
+  const DirectJob J = jobs[wg_job[blockIdx.x]];
+  const int local = blockIdx.x - J.wg0;
+  direct_block<KSTEP, VEC, CMAX>((uint8_t*)ring4, img, istride, iw, ih, J.bf, J.th, J.nk, J.pitch, J.rw, J.rh,
+                                 J.out, local % J.gx, local / J.gx);
+}
+
+int direct_pitch_of(int tw, int kstep) { return direct_pitch(direct_nk(tw, kstep), kstep); }
+
+int direct_grid_x(int rw) { return (rw + 32 * DIR_T - 1) / (32 * DIR_T); }
+int direct_grid_y(int rh) { return (rh + DIR_ROWS - 1) / DIR_ROWS; }
+
+// The instantiated CMAX that covers ceil(nk / kstep), or 0 past DIR_CMAX.
+int direct_cmax(int nk, int kstep) {
+  const int need = (nk + kstep - 1) / kstep;
+  for (int c : {2, 3, 4, 5, 6, DIR_CMAX})
+    if (need <= c) return c;
+  return 0;
+}
+
+bool direct_shape_ok(int tw, int kstep) {
+  const int nk = direct_nk(tw, kstep);
+  return direct_cmax(nk, kstep) != 0 && direct_units(nk, kstep) <= 2 * 64 * DIR_WAVES &&
+         direct_lds_bytes(tw, kstep) <= 160 * 1024;
+}
+
+static bool direct_vec(const uint8_t* img, int64_t istride, int iw, int kstep) {
+  const int sstep = kstep == 1 ? 1 : 4;
+  return ((((uintptr_t)img) | (uintptr_t)istride | (uintptr_t)(sstep * iw)) & 15) == 0;
+}
+
+// dispatch over (kstep, cmax, vec) = key kstep * 100 + cmax * 2 + vec
+#define CVM_DIRECT_SWITCH(LAUNCH)                                                                         \
+  switch (kstep * 100 + cmax * 2 + (vec ? 1 : 0)) {                                                       \
+    LAUNCH(1, 2) LAUNCH(1, 3) LAUNCH(1, 4) LAUNCH(1, 5) LAUNCH(1, 6) LAUNCH(1, 9) LAUNCH(3, 2) LAUNCH(3, 3) \
+    LAUNCH(3, 4) LAUNCH(3, 5) LAUNCH(3, 6) LAUNCH(3, 9) LAUNCH(4, 2) LAUNCH(4, 3) LAUNCH(4, 4) LAUNCH(4, 5) \
+    LAUNCH(4, 6) LAUNCH(4, 9)                                                                             \
+    default:                                                                                              \
+      return -1;                                                                                          \
   }
+
+int launch_direct(const uint8_t* img, int64_t istride, int iw, int ih, int kstep, const int8_t* bf, int tw, int th,
+                  int rw, int rh, int32_t* out, hipStream_t s) {
+  if (!direct_shape_ok(tw, kstep)) return -1;
+  const int nk = direct_nk(tw, kstep);
+  const int pitch = direct_pitch(nk, kstep);
+  const int cmax = direct_cmax(nk, kstep);
+  const size_t lds = (size_t)DIR_RING * pitch;
+  const bool vec = direct_vec(img, istride, iw, kstep);
+  dim3 grid(direct_grid_x(rw), direct_grid_y(rh)), block(64 * DIR_WAVES);
+#define CVM_DI(KS, CM)                                                                                       \
+  case KS * 100 + CM * 2 + 1:                                                                                \
+    hipLaunchKernelGGL((k_direct_i8<KS, true, CM>), grid, block, lds, s, img, istride, iw, ih, bf, th, nk,   \
+                       pitch, rw, rh, out);                                                                  \
+    break;                                                                                                   \
+  case KS * 100 + CM * 2:                                                                                    \
+    hipLaunchKernelGGL((k_direct_i8<KS, false, CM>), grid, block, lds, s, img, istride, iw, ih, bf, th, nk,  \
+                       pitch, rw, rh, out);                                                                  \
+    break;
+  CVM_DIRECT_SWITCH(CVM_DI)
 #undef CVM_DI
   return 0;
 }
+
+int launch_direct_batch(const uint8_t* img, int64_t istride, int iw, int ih, int kstep, int cmax,
+                        const DirectJob* jobs, const int* wg_job, int nwg, int max_pitch, hipStream_t s) {
+  const size_t lds = (size_t)DIR_RING * max_pitch;
+  if (lds > 160 * 1024 || nwg <= 0) return -1;
+  const bool vec = direct_vec(img, istride, iw, kstep);
+#define CVM_DB(KS, CM)                                                                                          \
+  case KS * 100 + CM * 2 + 1:                                                                                   \
+    hipLaunchKernelGGL((k_direct_batch<KS, true, CM>), dim3(nwg), dim3(64 * DIR_WAVES), lds, s, img, istride,   \
+                       iw, ih, jobs, wg_job);                                                                   \
+    break;                                                                                                      \
+  case KS * 100 + CM * 2:                                                                                       \
+    hipLaunchKernelGGL((k_direct_batch<KS, false, CM>), dim3(nwg), dim3(64 * DIR_WAVES), lds, s, img, istride,  \
+                       iw, ih, jobs, wg_job);                                                                   \
+    break;
+  CVM_DIRECT_SWITCH(CVM_DB)
+#undef CVM_DB
+  return 0;
+}
+
+#undef CVM_DIRECT_SWITCH
 
 }  // namespace cvm

@@ -61,6 +61,27 @@ struct TemplStats {
   unsigned long long tsq[4];
 };
 
+// One template of a batched direct-correlation launch: its B fragments,
+// C' map, shape, ring pitch and grid (gx blocks along x), first workgroup.
+struct DirectJob {
+  const int8_t* bf;
+  int32_t* out;
+  int th, nk, pitch, rw, rh, gx, wg0, pad_;
+};
+
+struct Extrema;
+
+// One template of a batched normalisation launch: its call state, chunk
+// partials (per channel count), correlation, extrema partials and result.
+struct NormJob {
+  const CallState* st;
+  const uint32_t* ps;
+  const float* corr;
+  Extrema* parts;
+  void* res;  // Res6 of the batched reduction
+  int tw, th, rw, rh, gx, gy, wg0, pad_;
+};
+
 // Min/max partial of one normalisation workgroup: (value, raster index).
 struct Extrema {
   float min_v;
@@ -145,16 +166,33 @@ void launch_normalize(const uint8_t* img, int64_t istride, int iw, int ih, int s
 // The wave-form normalisation applies (it is the one with an exact mode).
 bool norm_wave_ok(int tw, int th, int cnMax);
 
-// Direct correlation on int8 MFMA (cvm_direct.hip).  B fragments of one
-// template (cn from st when st is set), then C' per window into out.
-int direct_nk(int tw, int step);
-size_t direct_bfrag_bytes(int tw, int th, int step);
-size_t direct_lds_bytes(int tw, int step);
-void launch_bfrag(const uint8_t* tpl, int64_t tstride, int tw, int th, int step, int cn, const CallState* st,
+// Direct correlation on int8 MFMA (cvm_direct.hip).  kstep: staged bytes per
+// pixel (1 gray, 4 RGBA, 3 RGBA with cn = 3 staged without alpha).  B
+// fragments of one template (cn from st when st is set), then C' per window.
+int direct_kstep(int step, int cn);
+int direct_nk(int tw, int kstep);
+size_t direct_bfrag_bytes(int tw, int th, int kstep);
+size_t direct_lds_bytes(int tw, int kstep);
+bool direct_shape_ok(int tw, int kstep);
+int direct_cmax(int nk, int kstep);
+void launch_bfrag(const uint8_t* tpl, int64_t tstride, int tw, int th, int kstep, int cn, const CallState* st,
                   int8_t* bf, hipStream_t s);
-int launch_direct(const uint8_t* img, int64_t istride, int iw, int ih, int step, const int8_t* bf, int tw, int th,
+int launch_direct(const uint8_t* img, int64_t istride, int iw, int ih, int kstep, const int8_t* bf, int tw, int th,
                   int rw, int rh, int32_t* out, hipStream_t s);
 // Final first-occurrence reduction into res6 (device) = {min_v, min_x, min_y, max_v, max_x, max_y}.
+// Batched forms (one launch for many templates against one frame).
+int direct_pitch_of(int tw, int kstep);
+int direct_grid_x(int rw);
+int direct_grid_y(int rh);
+int launch_direct_batch(const uint8_t* img, int64_t istride, int iw, int ih, int kstep, int cmax,
+                        const DirectJob* jobs, const int* wg_job, int nwg, int max_pitch, hipStream_t s);
+void norm_form(int tw, int th, int cnMax, int* q, int* sq32);
+int norm_grid_x(int rw, int q);
+int norm_grid_y(int rh);
+int launch_norm_batch(const uint8_t* img, int64_t istride, int iw, int ih, int step, int q, int sq32, bool exact,
+                      const NormJob* jobs, const int* wg_job, int nwg, hipStream_t s);
+// one workgroup per job: its extrema partials -> its Res6
+void launch_reduce_batch(const NormJob* jobs, int njobs, hipStream_t s);
 void launch_reduce(const Extrema* partials, int nparts, const CallState* st, int rw, void* res6, float* map_if_flat,
                    int rh, hipStream_t s);
 

@@ -610,10 +610,11 @@ __device__ __forceinline__ double score_fp64(double numI, double idiff, double s
 // correlation (bit pattern read through the float pointer) and every score
 // is the FP64 one rounded once to float32.
 template <int Q, bool SQ32, int VMODE, bool EXACT>
-__global__ __launch_bounds__(64) void k_norm_wave(const uint8_t* __restrict__ img, int64_t istride, int iw, int ih,
-                                                  int step, int tw, int th, int rw, int rh,
-                                                  const CallState* __restrict__ st, const uint32_t* __restrict__ ps,
-                                                  const float* corr, float* map, Extrema* __restrict__ parts) {
+__device__ __forceinline__ void norm_wave_block(const uint8_t* __restrict__ img, int64_t istride, int iw, int ih,
+                                                int step, int tw, int th, int rw, int rh,
+                                                const CallState* __restrict__ st, const uint32_t* __restrict__ ps,
+                                                const float* corr, float* map, Extrema* __restrict__ parts, int bx,
+                                                int by, int gx) {
   typedef typename std::conditional<SQ32, uint32_t, unsigned long long>::type sq_t;
   constexpr int CPT = NORM_CPT;
   constexpr int NW = 64 * CPT;
@@ -624,7 +625,7 @@ __global__ __launch_bounds__(64) void k_norm_wave(const uint8_t* __restrict__ im
   __shared__ uint32_t P[4][NWP];
   __shared__ sq_t P2[NWP];
   const int lane = threadIdx.x;
-  const int xs = blockIdx.x * SX, ys = blockIdx.y * NB_ROWS;
+  const int xs = bx * SX, ys = by * NB_ROWS;
   const int ye = min(rh, ys + NB_ROWS);
   float bmin = FLT_MAX, bmax = -FLT_MAX;
   int bmin_i = INT_MAX, bmax_i = INT_MAX;
@@ -677,7 +678,7 @@ __global__ __launch_bounds__(64) void k_norm_wave(const uint8_t* __restrict__ im
     }
     if (nload > 0) {
       const int nfull = th / NB_ROWS;
-      const uint32_t* pc = ps + (int64_t)blockIdx.y * 3 * iw + x0;
+      const uint32_t* pc = ps + (int64_t)by * 3 * iw + x0;
 #pragma unroll 2
       for (int i = 0; i < nfull; i++, pc += (int64_t)3 * iw) {
         uint32_t t0[CPT], t1[CPT], t2[CPT];
@@ -853,8 +854,29 @@ __global__ __launch_bounds__(64) void k_norm_wave(const uint8_t* __restrict__ im
     e.min_i = bmin_i;
     e.max_v = bmax;
     e.max_i = bmax_i;
-    parts[blockIdx.y * gridDim.x + blockIdx.x] = e;
+    parts[by * gx + bx] = e;
   }
+}
+
+template <int Q, bool SQ32, int VMODE, bool EXACT>
+__global__ __launch_bounds__(64) void k_norm_wave(const uint8_t* __restrict__ img, int64_t istride, int iw, int ih,
+                                                  int step, int tw, int th, int rw, int rh,
+                                                  const CallState* __restrict__ st, const uint32_t* __restrict__ ps,
+                                                  const float* corr, float* map, Extrema* __restrict__ parts) {
+  norm_wave_block<Q, SQ32, VMODE, EXACT>(img, istride, iw, ih, step, tw, th, rw, rh, st, ps, corr, map, parts,
+                                         blockIdx.x, blockIdx.y, gridDim.x);
+}
+
+// Many templates against one frame in one launch (a Fleet's members, all of
+// one (Q, SQ32) form): workgroup b runs block b - job.wg0 of job wg_job[b].
+template <int Q, bool SQ32, int VMODE, bool EXACT>
+__global__ __launch_bounds__(64) void k_norm_batch(const uint8_t* __restrict__ img, int64_t istride, int iw, int ih,
+                                                   int step, const NormJob* __restrict__ jobs,
+                                                   const int* __restrict__ wg_job) {
+  const NormJob J = jobs[wg_job[blockIdx.x]];
+  const int local = blockIdx.x - J.wg0;
+  norm_wave_block<Q, SQ32, VMODE, EXACT>(img, istride, iw, ih, step, J.tw, J.th, J.rw, J.rh, J.st, J.ps, J.corr,
+                                         nullptr, J.parts, local % J.gx, local / J.gx, J.gx);
 }
 
 void launch_norm_prep(const uint8_t* img, int64_t istride, int iw, int ih, int step, int tw, int th, int cnMax,
@@ -930,6 +952,60 @@ void launch_normalize(const uint8_t* img, int64_t istride, int iw, int ih, int s
 #undef CVM_NORM
 }
 
+// Wave-form variant of a shape: (q, sq32), or q = 0 when the int64 form applies.
+void norm_form(int tw, int th, int cnMax, int* q, int* sq32) {
+  NormPlan np = norm_plan(tw, th, cnMax);
+  *q = np.wave ? np.q : 0;
+  *sq32 = np.sq32;
+}
+
+int norm_grid_x(int rw, int q) { return (rw + 64 * q - 1) / (64 * q); }
+int norm_grid_y(int rh) { return (rh + NB_ROWS - 1) / NB_ROWS; }
+
+int launch_norm_batch(const uint8_t* img, int64_t istride, int iw, int ih, int step, int q, int sq32, bool exact,
+                      const NormJob* jobs, const int* wg_job, int nwg, hipStream_t s) {
+  if (nwg <= 0) return -1;
+  const bool a16 = ((((uintptr_t)img) | (uintptr_t)istride) & 15) == 0;
+  const bool a8 = ((((uintptr_t)img) | (uintptr_t)istride) & 7) == 0;
+  int vmode = 0;
+  if (iw % NORM_CPT == 0 && step == 4 && a16) vmode = 1;
+  if (iw % NORM_CPT == 0 && step == 1 && a8) vmode = 2;
+#define CVM_NB(Q, S, V, X) \
+  hipLaunchKernelGGL((k_norm_batch<Q, S, V, X>), dim3(nwg), dim3(64), 0, s, img, istride, iw, ih, step, jobs, wg_job)
+#define CVM_NBX(Q, S, V)  \
+  if (exact)              \
+    CVM_NB(Q, S, V, true); \
+  else                    \
+    CVM_NB(Q, S, V, false)
+#define CVM_NBV(Q, S)      \
+  if (vmode == 1) {        \
+    CVM_NBX(Q, S, 1);      \
+  } else if (vmode == 2) { \
+    CVM_NBX(Q, S, 2);      \
+  } else {                 \
+    CVM_NBX(Q, S, 0);      \
+  }
+  if (q == 4) {
+    if (sq32) {
+      CVM_NBV(4, true);
+    } else {
+      CVM_NBV(4, false);
+    }
+  } else if (q == 2) {
+    if (sq32) {
+      CVM_NBV(2, true);
+    } else {
+      CVM_NBV(2, false);
+    }
+  } else {
+    return -1;
+  }
+#undef CVM_NBV
+#undef CVM_NBX
+#undef CVM_NB
+  return 0;
+}
+
 // Final reduction of the workgroup partials and conversion to the
 // reference's six-tuple (min_v, min_x, min_y, max_v, max_x, max_y).
 struct Res6 {
@@ -939,7 +1015,22 @@ struct Res6 {
   int32_t max_x, max_y;
 };
 
+__device__ void reduce_block(const Extrema* __restrict__ parts, int n, int rw, Res6* out);
+
 __global__ __launch_bounds__(1024) void k_reduce(const Extrema* __restrict__ parts, int n, int rw, Res6* out) {
+  reduce_block(parts, n, rw, out);
+}
+
+__global__ __launch_bounds__(1024) void k_reduce_batch(const NormJob* __restrict__ jobs) {
+  const NormJob J = jobs[blockIdx.x];
+  reduce_block(J.parts, J.gx * J.gy, J.rw, (Res6*)J.res);
+}
+
+void launch_reduce_batch(const NormJob* jobs, int njobs, hipStream_t s) {
+  if (njobs > 0) hipLaunchKernelGGL(k_reduce_batch, dim3(njobs), dim3(1024), 0, s, jobs);
+}
+
+__device__ void reduce_block(const Extrema* __restrict__ parts, int n, int rw, Res6* out) {
   __shared__ float rv[2][16];
   __shared__ int ri[2][16];
   float bmin = FLT_MAX, bmax = -FLT_MAX;

@@ -131,15 +131,19 @@ def test_direct_flat_and_limits(cvm, orc):
         cvm.match_direct(tall, tall[:300, :10].copy())  # th > 257
 
 
-def test_direct_matcher_and_fleet(cvm, orc):
-    """Find and FindAll in the direct form equal the one-shot direct Match."""
-    rng = np.random.default_rng(21)
-    frame = synth.flat_rect_scene(320, 240, rng, n_rects=40, channels=4)
+@pytest.mark.parametrize("fw,fh", [(320, 240), (317, 201)])
+def test_direct_matcher_and_fleet(cvm, orc, fw, fh):
+    """Find and FindAll in the direct form equal the one-shot direct Match.
+    The Fleet stages constant-alpha members as RGB (three bytes a pixel) and
+    the others as RGBA, the one-shot call always RGBA; 317 columns take the
+    unaligned staging loads."""
+    rng = np.random.default_rng(21 + fw)
+    frame = synth.flat_rect_scene(fw, fh, rng, n_rects=40, channels=4)
     frame[..., 3] = 255
     frame = synth.add_noise(frame, 3.0, rng, channels=[0, 1, 2])
     subs = []
     for (tw, th) in [(16, 16), (24, 40), (57, 13), (64, 64), (9, 31), (100, 70)]:
-        x, y = int(rng.integers(0, 320 - tw + 1)), int(rng.integers(0, 240 - th + 1))
+        x, y = int(rng.integers(0, fw - tw + 1)), int(rng.integers(0, fh - th + 1))
         subs.append(frame[y:y + th, x:x + tw].copy())
     subs[2][..., 3] = 9  # constant alpha still
     subs[4][0, 0, 3] = 0  # non-constant template alpha: cn = 4

@@ -232,9 +232,11 @@ __device__ __forceinline__ void direct_block(uint8_t* ring, const uint8_t* __res
 
   // Staged row DIR_ROWS + i (first read in step i + 1) is fetched in step
   // i - DIR_PF + 1 into a register queue and written to its ring slot at the
-  // end of step i, so DIR_PF steps of MFMAs cover its load latency.  Unit
-  // nc[p] of each staged row is this thread's (units past the row reload the
-  // last one and are not written).
+  // start of step i; the barrier after step i's first class makes it visible
+  // before step i prefetches step i + 1's first class.  Its slot held row
+  // i + DIR_ROWS - DIR_RING, last read before the barrier of the step after
+  // that row's step.  Unit nc[p] of each staged row is this thread's (units
+  // past the row reload the last one and are not written).
   int nc[2];
 #pragma unroll
   for (int p = 0; p < 2; p++) nc[p] = tid + p * 64 * DIR_WAVES;
@@ -246,34 +248,19 @@ __device__ __forceinline__ void direct_block(uint8_t* ring, const uint8_t* __res
 #pragma unroll
     for (int p = 0; p < 2; p++) rq[d][p] = fetch(DIR_ROWS + d, ncl[p], rok[d][p]);
 
-  loadB(BN, 0, 0);
+  // step 0's first class
+  dv4i AN[NA];
+  {
+    const uint8_t* arow0 = ring + ((32 * wave + m) % DIR_RING) * pitch + 16 * half;
+    loadA(AN, arow0, 0);
+    loadB(BN, 0, 0);
+  }
   for (int i = 0; i < th; i++) {
     const bool more = i + 1 < th;
+    // queue: rq[d] holds staged row DIR_ROWS + i + d
 #pragma unroll
     for (int p = 0; p < 2; p++) rq[DIR_PF - 1][p] = fetch(DIR_ROWS + i + DIR_PF - 1, ncl[p], rok[DIR_PF - 1][p]);
-    // this wave's A rows: staged rows 32 wave + m + i
-    const int slot = (32 * wave + m + i) % DIR_RING;
-    const uint8_t* arow = ring + slot * pitch + 16 * half;
-#pragma unroll
-    for (int j = 0; j < CMAX; j++) B0[j] = BN[j];
-    loadA(A0, arow, 0);
-#pragma unroll
-    for (int rc = 0; rc < KSTEP; rc++) {
-      dv4i* ca = (rc & 1) ? A1 : A0;
-      dv4i* cb = (rc & 1) ? B1 : B0;
-      dv4i* na = (rc & 1) ? A0 : A1;
-      dv4i* nb = (rc & 1) ? B0 : B1;
-      if (rc + 1 < KSTEP) {
-        loadA(na, arow, rc + 1);
-        loadB(nb, i, rc + 1);
-      } else if (more) {
-        loadB(BN, i + 1, 0);
-      }
-      mma(ca, cb, cls[rc]);
-    }
     if (more) {
-      // slot of staged row DIR_ROWS + i: it held row i + DIR_ROWS - DIR_RING,
-      // last read in that step, which every wave has left (barrier below)
       const int ns = (DIR_ROWS + i) % DIR_RING;
 #pragma unroll
       for (int p = 0; p < 2; p++)
@@ -286,7 +273,34 @@ __device__ __forceinline__ void direct_block(uint8_t* ring, const uint8_t* __res
         rq[d][p] = rq[d + 1][p];
         rok[d][p] = rok[d + 1][p];
       }
-    __syncthreads();
+    // this wave's A rows: staged rows 32 wave + m + i
+    const uint8_t* arow = ring + ((32 * wave + m + i) % DIR_RING) * pitch + 16 * half;
+    const uint8_t* arowN = ring + ((32 * wave + m + i + 1) % DIR_RING) * pitch + 16 * half;
+#pragma unroll
+    for (int j = 0; j < NA; j++) A0[j] = AN[j];
+#pragma unroll
+    for (int j = 0; j < CMAX; j++) B0[j] = BN[j];
+    if (more) loadB(BN, i + 1, 0);
+#pragma unroll
+    for (int rc = 0; rc < KSTEP; rc++) {
+      dv4i* ca = (rc & 1) ? A1 : A0;
+      dv4i* cb = (rc & 1) ? B1 : B0;
+      dv4i* na = (rc & 1) ? A0 : A1;
+      dv4i* nb = (rc & 1) ? B0 : B1;
+      if (rc + 1 < KSTEP) {
+        loadA(na, arow, rc + 1);
+        loadB(nb, i, rc + 1);
+      }
+      if (rc == KSTEP - 1) {
+        // the next step's first class reads the row written at this step's
+        // start: after the barrier (before this class's MFMAs when there is
+        // only one class)
+        if (KSTEP == 1) __syncthreads();
+        if (more) loadA(AN, arowN, 0);
+      }
+      mma(ca, cb, cls[rc]);
+      if (rc == 0 && KSTEP > 1) __syncthreads();
+    }
   }
 
   const int yb = Y0 + 32 * wave + 4 * half;

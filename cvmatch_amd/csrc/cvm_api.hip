@@ -194,9 +194,9 @@ struct Engine {
   hipStream_t side = nullptr;
   hipEvent_t fork = nullptr, join = nullptr;
   // one captured device-resident call: replayed while the same arguments
-  // repeat and no buffer has moved (the first sightings run eagerly, the
-  // kGraphAfter-th captures -- a capture costs milliseconds, so only a
-  // stream of repeats pays for it -- later ones replay)
+  // repeat and no buffer has moved (the first sighting runs eagerly and
+  // allocates, the second captures -- a few milliseconds once -- later ones
+  // replay)
   struct CallKey {
     const void *img, *tpl, *map;
     int64_t istride, tstride;
@@ -1454,7 +1454,7 @@ static int match_device(const uint8_t* d_img, int64_t istride, int32_t iw, int32
     HIPCHK(hipGraphLaunch(E.gexec, s));
     return finish_result(E, s, out);
   }
-  constexpr int kGraphAfter = 4;
+  constexpr int kGraphAfter = 2;
   if (++E.gseen < kGraphAfter) {
     CHK(run());  // eager (the first one allocates what the call needs)
     return finish_result(E, s, out);

@@ -143,6 +143,38 @@ def fleet_leg(frames_timed=16, solo_frames=2):
     return out
 
 
+def host_leg(parent, sub, reps=10):
+    """The drop-in path: host buffers in, six-tuple (and map) out, PCIe
+    included.  MatchMap writes a freshly allocated result per call, as the
+    reference's does (cvmatch.go:100-117); the reused-result and pinned-input
+    rows isolate the page faults and the staging copy.  Never the headline."""
+    import cvmatch_amd as cvm
+
+    rw, rh = parent.shape[1] - sub.shape[1] + 1, parent.shape[0] - sub.shape[0] + 1
+    pin = cvm.host_array(parent.shape)
+    pin[:] = parent
+    res = np.empty((rh, rw), dtype=np.float32)
+    L = cvm._L()
+    p, ps, pw, ph, _ = cvm.view(parent)
+    s, ss, sw, sh, _ = cvm.view(sub)
+    out = cvm.cvm_result()
+
+    def map_reused():
+        cvm._check(L.cvm_match_map(cvm._ptr(p), ps, pw, ph, cvm._ptr(s), ss, sw, sh, cvm._ptr(res), C.byref(out)))
+
+    legs = {}
+    for key, fn in (("match_map", lambda: cvm.match_map(parent, sub)), ("match_map_reused_result", map_reused),
+                    ("match", lambda: cvm.match(parent, sub)), ("match_pinned_input", lambda: cvm.match(pin, sub))):
+        fn()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            fn()
+        dt = (time.perf_counter() - t0) / reps
+        legs[key] = {"ms": round(dt * 1e3, 3), "Gwindows_per_s": round(rw * rh / dt / 1e9, 3)}
+    legs["copy_threads"] = cvm.threads()
+    return legs
+
+
 def copy_calibration(mib):
     import cvmatch_amd as cvm
 
@@ -173,6 +205,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-fleet", action="store_true", help="skip the Fleet leg")
+    ap.add_argument("--no-host", action="store_true", help="skip the host-buffer (PCIe-inclusive) leg")
     ap.add_argument("--fleet-frames", type=int, default=16, help="frames timed per Fleet form")
     ap.add_argument("--fleet-only", action="store_true", help="only run the Fleet leg (profiling)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
@@ -282,6 +315,8 @@ def main():
                      "alg_bytes_per_launch": sb[dom], "avg_launch_ms": round(kern[dom]["avg_ms"], 5)},
         "stages_ms": {k: round(v["avg_ms"], 5) for k, v in stages.items()},
     }
+    if not args.no_host:
+        line["host_buffers"] = host_leg(parent, sub)
     if not args.no_fleet:
         line["fleet"] = fleet_leg(args.fleet_frames)
     if not args.no_cpu:

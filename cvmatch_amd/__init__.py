@@ -20,6 +20,7 @@ from __future__ import annotations
 
 import ctypes as C
 import os
+import weakref
 from collections import namedtuple
 
 import numpy as np
@@ -33,9 +34,15 @@ Result = namedtuple("Result", "MinV MinX MinY MaxV MaxX MaxY")
 CVM_OK, CVM_E_BAD_ARGS, CVM_E_STATS_CAP, CVM_E_EMPTY_FLEET, CVM_E_MIXED_FLEET = 0, 1, 2, 3, 4
 CVM_E_HIP, CVM_E_UNSUPPORTED, CVM_E_NIL_IMAGE, CVM_E_EMPTY_IMAGE = 5, 6, 7, 8
 
-# Worker-count knob of the reference (cvmatch.go:42-63).  Kept for API
-# compatibility; it has no effect on the GPU path and never changes bits.
-Threads = 0
+# Worker-count knob of the reference (cvmatch.go:36-63), with its
+# CVMATCH_THREADS environment variable read once at import.  Here it sizes the
+# host threads that copy pageable buffers through pinned staging (0 or less:
+# automatic; clamped to [1, 16]); it never changes an output bit.
+try:
+    Threads = max(int(os.environ.get("CVMATCH_THREADS", "0")), 0)
+except ValueError:
+    Threads = 0
+_threads_pushed = None
 
 
 class CvmatchError(RuntimeError):
@@ -95,6 +102,11 @@ SIGNATURES = {
     "cvm_profile_read": ([_P, _P, _P, _I32], C.c_int),
     "cvm_selftest_fp": ([_P, _P, _P, _I32, _P, _P, _P], C.c_int),
     "cvm_twiddles": ([_I32, _P, _P, C.POINTER(_I32)], C.c_int),
+    "cvm_set_threads": ([_I32], _I32),
+    "cvm_threads": ([], _I32),
+    "cvm_host_alloc": ([_SZ, C.POINTER(_P)], C.c_int),
+    "cvm_host_free": ([_P], C.c_int),
+    "cvm_selftest_copy": ([_P, _P, _SZ, _I32], C.c_int),
 }
 
 
@@ -111,6 +123,36 @@ def lib():
             fn.restype = res
         _lib = L
     return _lib
+
+
+def _L():
+    """lib(), with the module's Threads pushed to the library when it changed
+    (the reference reads its Threads variable at every call)."""
+    global _threads_pushed
+    L = lib()
+    if Threads != _threads_pushed:
+        L.cvm_set_threads(int(Threads))
+        _threads_pushed = Threads
+    return L
+
+
+def threads() -> int:
+    """threads() cvmatch.go:52-63: the resolved copy-thread count."""
+    return int(_L().cvm_threads())
+
+
+def host_array(shape, dtype=np.uint8) -> np.ndarray:
+    """A numpy array in pinned host memory (cvm_host_alloc), freed with the
+    array.  Frames filled here move to the GPU at the DMA rate, unstaged."""
+    dt = np.dtype(dtype)
+    count = int(np.prod(shape))
+    nbytes = max(count * dt.itemsize, 1)
+    p = C.c_void_p()
+    _check(lib().cvm_host_alloc(nbytes, C.byref(p)))
+    buf = (C.c_uint8 * nbytes).from_address(p.value)
+    fin = weakref.finalize(buf, lib().cvm_host_free, C.c_void_p(p.value))
+    fin.atexit = False  # the process's exit releases it
+    return np.frombuffer(buf, dtype=dt, count=count).reshape(shape)
 
 
 def _check(rc: int):
@@ -154,7 +196,7 @@ def match_u8(img, istride, iw, ih, tpl, tstride, tw, th, cn, step, want_map=Fals
     rw, rh = iw - tw + 1, ih - th + 1
     res = np.empty((rh, rw), dtype=np.float32) if want_map and rw > 0 and rh > 0 else None
     out = cvm_result()
-    fn = lib().cvm_match_u8_direct if direct else lib().cvm_match_u8
+    fn = _L().cvm_match_u8_direct if direct else _L().cvm_match_u8
     _check(fn(_ptr(img), istride, iw, ih, _ptr(tpl), tstride, tw, th, cn, step,
               _ptr(res) if res is not None else None, C.byref(out)))
     return res, out.tuple()
@@ -172,7 +214,7 @@ def match_direct(parent, sub, want_map=False):
     rw, rh = pw - sw + 1, ph - sh + 1
     res = np.empty((rh, rw), dtype=np.float32) if want_map else None
     out = cvm_result()
-    _check(lib().cvm_match_u8_direct(_ptr(p), ps, pw, ph, _ptr(s), ss, sw, sh, 0 if pst == 4 else 1, pst,
+    _check(_L().cvm_match_u8_direct(_ptr(p), ps, pw, ph, _ptr(s), ss, sw, sh, 0 if pst == 4 else 1, pst,
                                      _ptr(res) if res is not None else None, C.byref(out)))
     return res, out.tuple()
 
@@ -184,7 +226,7 @@ def match(parent, sub):
     if pst != 4 or sst != 4:
         raise CvmatchError(CVM_E_BAD_ARGS, "cvmatch: Match takes RGBA (H, W, 4) images")
     out = cvm_result()
-    _check(lib().cvm_match(_ptr(p), ps, pw, ph, _ptr(s), ss, sw, sh, C.byref(out)))
+    _check(_L().cvm_match(_ptr(p), ps, pw, ph, _ptr(s), ss, sw, sh, C.byref(out)))
     return out.tuple()
 
 
@@ -200,7 +242,7 @@ def match_map(parent, sub):
     w, h = pw - sw + 1, ph - sh + 1
     res = np.empty((h, w), dtype=np.float32)
     out = cvm_result()
-    _check(lib().cvm_match_map(_ptr(p), ps, pw, ph, _ptr(s), ss, sw, sh, _ptr(res), C.byref(out)))
+    _check(_L().cvm_match_map(_ptr(p), ps, pw, ph, _ptr(s), ss, sw, sh, _ptr(res), C.byref(out)))
     return res, w, h
 
 
@@ -209,7 +251,7 @@ def match_gray(parent, sub):
     p, ps, pw, ph, pst = view(parent)
     s, ss, sw, sh, sst = view(sub)
     out = cvm_result()
-    _check(lib().cvm_match_gray(_ptr(p), ps, pw, ph, pst, _ptr(s), ss, sw, sh, sst, C.byref(out)))
+    _check(_L().cvm_match_gray(_ptr(p), ps, pw, ph, pst, _ptr(s), ss, sw, sh, sst, C.byref(out)))
     return out.tuple()
 
 
@@ -219,7 +261,7 @@ def alpha_const(pix) -> bool:
     if st != 4:
         raise CvmatchError(CVM_E_BAD_ARGS, "cvmatch: alpha scan needs RGBA")
     out = C.c_int32()
-    _check(lib().cvm_alpha_const(_ptr(p), ps, w, h, C.byref(out)))
+    _check(_L().cvm_alpha_const(_ptr(p), ps, w, h, C.byref(out)))
     return bool(out.value)
 
 
@@ -229,7 +271,7 @@ def rgba_to_gray(pix) -> np.ndarray:
     if st != 4:
         raise CvmatchError(CVM_E_BAD_ARGS, "cvmatch: gray conversion needs RGBA")
     out = np.empty((h, w), dtype=np.uint8)
-    _check(lib().cvm_rgba_to_gray(_ptr(p), ps, w, h, _ptr(out)))
+    _check(_L().cvm_rgba_to_gray(_ptr(p), ps, w, h, _ptr(out)))
     return out
 
 
@@ -260,7 +302,7 @@ class Matcher:
             raise CvmatchError(CVM_E_BAD_ARGS, "cvmatch: NewMatcher takes an RGBA template")
         self.w, self.h, self.step = sw, sh, sst
         h = C.c_void_p()
-        _check(lib().cvm_matcher_new(_ptr(s), ss, sw, sh, sst, C.byref(h)))
+        _check(_L().cvm_matcher_new(_ptr(s), ss, sw, sh, sst, C.byref(h)))
         self._h = h
 
     def find(self, parent, direct=False):
@@ -271,7 +313,7 @@ class Matcher:
         if pst != self.step:
             raise CvmatchError(CVM_E_BAD_ARGS, "cvmatch: parent format does not match the matcher")
         out = cvm_result()
-        fn = lib().cvm_matcher_find_direct if direct else lib().cvm_matcher_find
+        fn = _L().cvm_matcher_find_direct if direct else _L().cvm_matcher_find
         _check(fn(self._h, _ptr(p), ps, pw, ph, C.byref(out)))
         return out.tuple()
 
@@ -304,7 +346,7 @@ class Fleet:
         self._ms = list(ms)
         arr = (C.c_void_p * max(len(ms), 1))(*[m._h.value for m in ms])
         h = C.c_void_p()
-        _check(lib().cvm_fleet_new(C.cast(arr, C.c_void_p), len(ms), C.byref(h)))
+        _check(_L().cvm_fleet_new(C.cast(arr, C.c_void_p), len(ms), C.byref(h)))
         self._h = h
         self.step = ms[0].step
 
@@ -317,7 +359,7 @@ class Fleet:
             raise CvmatchError(CVM_E_BAD_ARGS, "cvmatch: parent format does not match the fleet")
         n = len(self._ms)
         out = (cvm_result * n)()
-        fn = lib().cvm_fleet_find_all_direct if direct else lib().cvm_fleet_find_all
+        fn = _L().cvm_fleet_find_all_direct if direct else _L().cvm_fleet_find_all
         _check(fn(self._h, _ptr(p), ps, pw, ph, C.cast(out, C.c_void_p)))
         return [Result(*out[i].tuple()) for i in range(n)]
 

@@ -18,7 +18,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libcvmatch_hip.so")
-SOURCES = ["cvm_fft.hip", "cvm_norm.hip", "cvm_api.hip", "cvm_direct.hip", "cvm_fleet.hip"]
+SOURCES = ["cvm_fft.hip", "cvm_norm.hip", "cvm_api.hip", "cvm_direct.hip", "cvm_xfer.hip"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("CVM_OFFLOAD_ARCH", "gfx950")
 
@@ -51,7 +51,7 @@ def build(verbose: bool = False) -> str:
     with ThreadPoolExecutor(max_workers=min(len(srcs), 8)) as ex:
         objs = list(ex.map(_compile, srcs))
     tmp = LIB + ".tmp"
-    cmd = [HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp, *objs]
+    cmd = [HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-pthread", "-o", tmp, *objs]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError(f"link failed:\n{r.stderr[-6000:]}")

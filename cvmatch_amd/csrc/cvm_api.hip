@@ -553,8 +553,8 @@ static int match_host(const uint8_t* img, int64_t istride, int iw, int ih, const
   CHK(E.tpl.ensure(tb));
   {
     Stage S(E, "h2d", s);
-    HIPCHK(hipMemcpyAsync(E.img.p, img, ib, hipMemcpyHostToDevice, s));
-    HIPCHK(hipMemcpyAsync(E.tpl.p, tpl, tb, hipMemcpyHostToDevice, s));
+    HIPCHK(xfer_h2d(E.img.p, img, ib, s));
+    HIPCHK(xfer_h2d(E.tpl.p, tpl, tb, s));
   }
   float* d_map = nullptr;
   if (map) {
@@ -569,7 +569,7 @@ static int match_host(const uint8_t* img, int64_t istride, int iw, int ih, const
                   d_map));
   if (map) {
     Stage S(E, "d2h", s);
-    HIPCHK(hipMemcpyAsync(map, d_map, sizeof(float) * (size_t)rw * rh, hipMemcpyDeviceToHost, s));
+    HIPCHK(xfer_d2h(map, d_map, sizeof(float) * (size_t)rw * rh, s));
   }
   return finish_result(E, s, out);
 }
@@ -1365,8 +1365,8 @@ int cvm_match_gray(const uint8_t* img, int64_t istride, int32_t iw, int32_t ih, 
   CHK(E.tpl.ensure(tb));
   {
     Stage S(E, "h2d", s);
-    HIPCHK(hipMemcpyAsync(E.img.p, img, ib, hipMemcpyHostToDevice, s));
-    HIPCHK(hipMemcpyAsync(E.tpl.p, tpl, tb, hipMemcpyHostToDevice, s));
+    HIPCHK(xfer_h2d(E.img.p, img, ib, s));
+    HIPCHK(xfer_h2d(E.tpl.p, tpl, tb, s));
   }
   const uint8_t* gi = E.img.as<uint8_t>();
   const uint8_t* gt = E.tpl.as<uint8_t>();
@@ -1496,7 +1496,7 @@ int cvm_alpha_const(const uint8_t* pix, int64_t stride, int32_t w, int32_t h, in
   hipStream_t s = E.stream;
   size_t ib = (size_t)img_bytes(stride, w, h, 4);
   CHK(E.img.ensure(ib));
-  HIPCHK(hipMemcpyAsync(E.img.p, pix, ib, hipMemcpyHostToDevice, s));
+  HIPCHK(xfer_h2d(E.img.p, pix, ib, s));
   launch_state_init(E.st, s);
   launch_alpha_scan(E.img.as<uint8_t>(), stride, w, h, E.st, 0, s);
   CallState hs;
@@ -1517,9 +1517,9 @@ int cvm_rgba_to_gray(const uint8_t* pix, int64_t stride, int32_t w, int32_t h, u
   size_t ib = (size_t)img_bytes(stride, w, h, 4);
   CHK(E.img.ensure(ib));
   CHK(E.gray0.ensure((size_t)w * h));
-  HIPCHK(hipMemcpyAsync(E.img.p, pix, ib, hipMemcpyHostToDevice, s));
+  HIPCHK(xfer_h2d(E.img.p, pix, ib, s));
   launch_rgba_to_gray(E.img.as<uint8_t>(), stride, w, h, E.gray0.as<uint8_t>(), s);
-  HIPCHK(hipMemcpyAsync(out, E.gray0.p, (size_t)w * h, hipMemcpyDeviceToHost, s));
+  HIPCHK(xfer_d2h(out, E.gray0.p, (size_t)w * h, s));
   HIPCHK(hipStreamSynchronize(s));
   return 0;
 }
@@ -1566,7 +1566,7 @@ static int matcher_find(cvm_matcher* m, const uint8_t* img, int64_t stride, int3
   CHK(E.img.ensure(ib));
   {
     Stage S(E, "h2d", s);
-    HIPCHK(hipMemcpyAsync(E.img.p, img, ib, hipMemcpyHostToDevice, s));
+    HIPCHK(xfer_h2d(E.img.p, img, ib, s));
   }
   if (direct)
     CHK(matcher_find_direct_dev(E, s, m, E.img.as<uint8_t>(), stride, w, h, E.res_d));
@@ -1638,7 +1638,7 @@ static int fleet_find_all(cvm_fleet* f, const uint8_t* img, int64_t stride, int3
   CHK(E.img.ensure(ib));
   {
     Stage S(E, "h2d", s);
-    HIPCHK(hipMemcpyAsync(E.img.p, img, ib, hipMemcpyHostToDevice, s));
+    HIPCHK(xfer_h2d(E.img.p, img, ib, s));
   }
   int pConst = 0;
   if (f->step == 4) {
@@ -1700,12 +1700,38 @@ int cvm_device_free(void* p) {
 }
 
 int cvm_copy_h2d(void* dst, const void* src, size_t bytes) {
-  HIPCHK(hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice));
+  HIPCHK(xfer_h2d(dst, src, bytes, nullptr));
+  HIPCHK(hipStreamSynchronize(nullptr));
   return 0;
 }
 
 int cvm_copy_d2h(void* dst, const void* src, size_t bytes) {
-  HIPCHK(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost));
+  HIPCHK(xfer_d2h(dst, src, bytes, nullptr));
+  HIPCHK(hipStreamSynchronize(nullptr));
+  return 0;
+}
+
+int cvm_host_alloc(size_t bytes, void** out) {
+  if (!out) return fail(CVM_E_BAD_ARGS, "cvmatch: nil output pointer");
+  HIPCHK(host_alloc(bytes, out));
+  return 0;
+}
+
+int cvm_host_free(void* p) {
+  if (!p) return 0;
+  hipError_t e = host_free(p);
+  if (e == hipErrorInvalidValue) return fail(CVM_E_BAD_ARGS, "cvmatch: %p is not a cvm_host_alloc block", p);
+  HIPCHK(e);
+  return 0;
+}
+
+int32_t cvm_set_threads(int32_t n) { return set_host_threads(n); }
+
+int32_t cvm_threads(void) { return host_threads(); }
+
+int cvm_selftest_copy(void* dst, const void* src, size_t bytes, int32_t threads) {
+  if ((!dst || !src) && bytes) return fail(CVM_E_BAD_ARGS, "cvmatch: nil buffer");
+  host_copy(dst, src, bytes, threads);
   return 0;
 }
 

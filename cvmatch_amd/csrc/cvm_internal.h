@@ -198,4 +198,19 @@ void launch_reduce(const Extrema* partials, int nparts, const CallState* st, int
 
 void launch_rgba_to_gray(const uint8_t* pix, int64_t stride, int w, int h, uint8_t* out, hipStream_t s);
 
+// Host transfers (cvm_xfer.hip).  A pageable host buffer of 1 MiB or more
+// moves through pinned staging chunks with host_threads() copy threads;
+// buffers from host_alloc and small ones go straight to the driver.
+// xfer_h2d returns once the source may be reused; xfer_d2h returns with the
+// destination written, unless it is a host_alloc buffer (then the copy is left
+// in flight on s).
+int set_host_threads(int n);
+int host_threads();
+hipError_t host_alloc(size_t bytes, void** out);
+hipError_t host_free(void* p);
+hipError_t xfer_h2d(void* dst, const void* src, size_t n, hipStream_t s);
+hipError_t xfer_d2h(void* dst, const void* src, size_t n, hipStream_t s);
+// the copy-thread pool alone (host memory only), for its self-test
+void host_copy(void* dst, const void* src, size_t n, int threads);
+
 }  // namespace cvm

@@ -153,11 +153,36 @@ int cvm_match_u8_direct_device(const uint8_t *d_img, int64_t istride, int32_t iw
                                int64_t tstride, int32_t tw, int32_t th, int32_t cn, int32_t step, float *d_map,
                                cvm_result *out, void *stream);
 
-/* Device memory helpers for callers without their own allocator. */
+/* Device memory helpers for callers without their own allocator.  The two
+ * copies are synchronous and take the staged host path described below. */
 int cvm_device_alloc(size_t bytes, void **out);
 int cvm_device_free(void *p);
 int cvm_copy_h2d(void *dst, const void *src, size_t bytes);
 int cvm_copy_d2h(void *dst, const void *src, size_t bytes);
+
+/* ------------------------------------------------------- host transfers -- */
+/* Every host-buffer entry point moves a pageable buffer of 1 MiB or more
+ * through pinned staging chunks copied by host threads, so that the copy
+ * overlaps the DMA of the previous chunk and runs at the pinned-memory PCIe
+ * rate.  Memory from cvm_host_alloc is pinned already and skips the staging:
+ * a caller that fills its frames there (a screenshot stream) moves them at
+ * the DMA rate with no host copy.
+ *
+ * cvm_set_threads mirrors the reference's Threads variable (cvmatch.go:36-63)
+ * and its CVMATCH_THREADS environment variable (read once): n <= 0 means
+ * automatic (the CPUs this process may run on), other values are clamped to
+ * [1, 16]; returns the resolved count, as threads() cvmatch.go:52-63 does.
+ * Here it sizes the host copy threads only; no setting changes an output
+ * bit.  cvm_threads returns the resolved count. */
+int32_t cvm_set_threads(int32_t n);
+int32_t cvm_threads(void);
+/* Pinned host memory (no reference counterpart; Go frames live in the Go
+ * heap).  cvm_host_free rejects pointers that cvm_host_alloc did not return. */
+int cvm_host_alloc(size_t bytes, void **out);
+int cvm_host_free(void *p);
+/* Self-test of the host copy threads alone: copies bytes from src to dst
+ * (host memory) split over `threads` threads; touches no GPU.  Tests only. */
+int cvm_selftest_copy(void *dst, const void *src, size_t bytes, int32_t threads);
 int cvm_synchronize(void);
 /* Device-to-device copy by a plain streaming kernel; also the known-size
  * kernel that calibrates the HBM traffic counters. */

@@ -55,3 +55,40 @@ def test_stats_cap_matches(cvm, orc):
 
 def test_version(cvm):
     assert b"gfx950" in cvm.lib().cvm_version()
+
+
+def test_threads_knob(cvm):
+    """threads() cvmatch.go:52-63: n <= 0 is automatic, others clamp to
+    [1, 16].  It sizes host copy threads only (host-side, no GPU call)."""
+    L = cvm.lib()
+    try:
+        auto = L.cvm_set_threads(0)
+        assert 1 <= auto <= 16
+        assert L.cvm_set_threads(-7) == auto
+        assert L.cvm_set_threads(5) == 5 and L.cvm_threads() == 5
+        assert L.cvm_set_threads(100) == 16
+        assert L.cvm_set_threads(1) == 1
+    finally:
+        L.cvm_set_threads(0)
+        cvm._threads_pushed = None
+
+
+def test_copy_pool_splits_exactly(cvm):
+    """The host copy threads that feed the pinned staging chunks: every byte
+    of [off, off+n) arrives, nothing outside it is written, for sizes around
+    the piece boundaries and every thread count."""
+    L = cvm.lib()
+    rng = np.random.default_rng(8)
+    src = rng.integers(0, 256, size=(6 << 20) + 4096, dtype=np.uint8)
+    dst = np.empty_like(src)
+    piece = 192 << 10
+    sizes = [0, 1, 63, 64, 65, piece - 1, piece, piece + 1, 2 * piece + 7, (1 << 20) + 3, (4 << 20) + 5, 6 << 20]
+    sizes += [int(v) for v in rng.integers(0, 6 << 20, 20)]
+    for i, n in enumerate(sizes):
+        th = 1 + i % 16
+        off = int(rng.integers(0, 64))
+        dst.fill(0xEE)
+        rc = L.cvm_selftest_copy(ctypes.c_void_p(dst.ctypes.data + off), ctypes.c_void_p(src.ctypes.data + off), n, th)
+        assert rc == 0
+        assert np.array_equal(dst[off:off + n], src[off:off + n]), (n, th, off)
+        assert not np.any(dst[:off] != 0xEE) and not np.any(dst[off + n:] != 0xEE), (n, th, off)

@@ -175,6 +175,30 @@ def host_leg(parent, sub, reps=10):
     return legs
 
 
+def peaks_leg(L, d_map, rw, rh, tw, th, reps=20):
+    """Every occurrence over the device-resident 4K map: threshold 0.5 and the
+    template's half-size window (the recipe of cvmatch_test.go:346-365), peak
+    list and count back on the host."""
+    cap = 4096
+    d_xy, d_v = C.c_void_p(), C.c_void_p()
+    assert L.cvm_device_alloc(8 * cap, C.byref(d_xy)) == 0 and L.cvm_device_alloc(4 * cap, C.byref(d_v)) == 0
+    cnt = C.c_int32()
+
+    def call():
+        if L.cvm_find_peaks_device(d_map, rw, rh, 0.5, tw // 2, th // 2, d_xy, d_v, cap, C.byref(cnt), None) != 0:
+            raise RuntimeError(L.cvm_last_error_detail().decode())
+
+    call()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        call()
+    dt = (time.perf_counter() - t0) / reps
+    L.cvm_device_free(d_xy)
+    L.cvm_device_free(d_v)
+    return {"ms": round(dt * 1e3, 4), "thresh": 0.5, "rx": tw // 2, "ry": th // 2, "count": cnt.value,
+            "map_bytes": rw * rh * 4}
+
+
 def copy_calibration(mib):
     import cvmatch_amd as cvm
 
@@ -315,6 +339,7 @@ def main():
                      "alg_bytes_per_launch": sb[dom], "avg_launch_ms": round(kern[dom]["avg_ms"], 5)},
         "stages_ms": {k: round(v["avg_ms"], 5) for k, v in stages.items()},
     }
+    line["peaks"] = peaks_leg(L, d_map, rw, rh, tw, th)
     if not args.no_host:
         line["host_buffers"] = host_leg(parent, sub)
     if not args.no_fleet:

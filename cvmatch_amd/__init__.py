@@ -102,6 +102,9 @@ SIGNATURES = {
     "cvm_profile_read": ([_P, _P, _P, _I32], C.c_int),
     "cvm_selftest_fp": ([_P, _P, _P, _I32, _P, _P, _P], C.c_int),
     "cvm_twiddles": ([_I32, _P, _P, C.POINTER(_I32)], C.c_int),
+    "cvm_match_peaks": ([_P, _I64, _I32, _I32, _P, _I64, _I32, _I32, _I32, _I32, C.c_float, _I32, _I32, _P, _P,
+                         _I32, C.POINTER(_I32), _RES], C.c_int),
+    "cvm_find_peaks_device": ([_P, _I32, _I32, C.c_float, _I32, _I32, _P, _P, _I32, C.POINTER(_I32), _P], C.c_int),
     "cvm_set_threads": ([_I32], _I32),
     "cvm_threads": ([], _I32),
     "cvm_host_alloc": ([_SZ, C.POINTER(_P)], C.c_int),
@@ -244,6 +247,40 @@ def match_map(parent, sub):
     out = cvm_result()
     _check(_L().cvm_match_map(_ptr(p), ps, pw, ph, _ptr(s), ss, sw, sh, _ptr(res), C.byref(out)))
     return res, w, h
+
+
+def match_peaks(parent, sub, thresh=0.999, rx=None, ry=None, cap=4096):
+    """Every occurrence of sub in parent (README.md:82-90): threshold + local-
+    maximum suppression over the match map, the recipe TestFindAllOccurrences
+    (cvmatch_test.go:320-381) verifies, with the map kept on the GPU.  RGBA
+    inputs take Match's alpha rule, gray inputs MatchGray's single channel.
+    rx / ry default to half the template (the test's 16 x 12 for 32 x 24).
+    Returns (peaks, total, six-tuple): peaks = [(x, y, score)] in raster
+    order, at most cap of them; total counts them all."""
+    p, ps, pw, ph, pst = view(parent)
+    s, ss, sw, sh, sst = view(sub)
+    if pst != sst:
+        raise CvmatchError(CVM_E_BAD_ARGS, "cvmatch: parent and template must both be RGBA or both gray")
+    rx = sw // 2 if rx is None else rx
+    ry = sh // 2 if ry is None else ry
+    xy = np.zeros((max(cap, 1), 2), dtype=np.int32)
+    vals = np.zeros(max(cap, 1), dtype=np.float32)
+    count = C.c_int32()
+    out = cvm_result()
+    cn = 0 if pst == 4 else 1
+    _check(_L().cvm_match_peaks(_ptr(p), ps, pw, ph, _ptr(s), ss, sw, sh, cn, pst, float(thresh), rx, ry,
+                                _ptr(xy), _ptr(vals), cap, C.byref(count), C.byref(out)))
+    k = min(count.value, cap)
+    return [(int(xy[i, 0]), int(xy[i, 1]), float(vals[i])) for i in range(k)], count.value, out.tuple()
+
+
+def find_peaks_device(d_map, w, h, thresh, rx, ry, d_xy, d_vals, cap, stream=None) -> int:
+    """Peaks of a device-resident w x h map into device buffers d_xy (2*cap
+    int32) / d_vals (cap float32); returns the total count."""
+    count = C.c_int32()
+    _check(_L().cvm_find_peaks_device(d_map, w, h, float(thresh), rx, ry, d_xy, d_vals, cap, C.byref(count),
+                                      stream))
+    return count.value
 
 
 def match_gray(parent, sub):

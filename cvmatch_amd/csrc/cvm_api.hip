@@ -218,7 +218,7 @@ struct Engine {
     gseen = 0;
   }
   std::map<int, FftTables> tables;
-  DevBuf img, tpl, spec, tspec, tspec2, corr, parts, gray0, gray1, mapbuf, nscratch, dbf;
+  DevBuf img, tpl, spec, tspec, tspec2, corr, parts, gray0, gray1, mapbuf, nscratch, dbf, pkscr, pkout;
   CallState* st = nullptr;
   Res6* res_d = nullptr;
   Res6* res_h = nullptr;
@@ -541,8 +541,50 @@ static int run_match_direct(Engine& E, hipStream_t s, const uint8_t* d_img, int6
                             const uint8_t* d_tpl, int64_t tstride, int tw, int th, int cn_mode, int step,
                             float* d_map);
 
+// Every occurrence (threshold + local-maximum suppression, the recipe of
+// TestFindAllOccurrences cvmatch_test.go:346-365) of a call's map.
+struct PeakReq {
+  float thresh;
+  int rx, ry, cap;
+  int32_t* xy;  // host, 2 * cap
+  float* vals;  // host, cap
+  int32_t* count;
+};
+
+static int check_peaks(int rw, int rh, int& rx, int& ry, int cap, const void* xy, const void* vals,
+                       const void* count) {
+  if (!count || (cap > 0 && (!xy || !vals))) return fail(CVM_E_BAD_ARGS, "cvmatch: nil peak output");
+  if (rw <= 0 || rh <= 0) return fail(CVM_E_EMPTY_IMAGE, "cvmatch: empty image");
+  if (rx < 0 || ry < 0 || cap < 0)
+    return fail(CVM_E_BAD_ARGS, "cvmatch: bad peak arguments (rx=%d ry=%d cap=%d)", rx, ry, cap);
+  // a window wider than the map sees the same elements as one just as wide
+  rx = std::min(rx, rw - 1);
+  ry = std::min(ry, rh - 1);
+  if (rx > peaks_max_rx())
+    return fail(CVM_E_UNSUPPORTED, "cvmatch: peak window half-width %d exceeds this build's limit (%d)", rx,
+                peaks_max_rx());
+  return 0;
+}
+
+// Peaks of a device map into d_xy / d_vals; the total stays in the scratch.
+static int peaks_dev(Engine& E, hipStream_t s, const float* d_map, int rw, int rh, float thresh, int rx, int ry,
+                     int cap, int32_t* d_xy, float* d_vals) {
+  CHK(E.pkscr.ensure(peaks_scratch_bytes(rw, rh)));
+  Stage S(E, "peaks", s);
+  launch_peaks(d_map, rw, rh, thresh, rx, ry, E.pkscr.p, cap, d_xy, d_vals, s);
+  HIPCHK(hipGetLastError());
+  return 0;
+}
+
+static int peaks_total_host(Engine& E, hipStream_t s, int rw, int rh, int* tot) {
+  HIPCHK(hipMemcpyAsync(tot, peaks_total(E.pkscr.p, rw, rh), sizeof(int), hipMemcpyDeviceToHost, s));
+  HIPCHK(hipStreamSynchronize(s));
+  return 0;
+}
+
 static int match_host(const uint8_t* img, int64_t istride, int iw, int ih, const uint8_t* tpl, int64_t tstride,
-                      int tw, int th, int cn_mode, int step, float* map, cvm_result* out, bool direct = false) {
+                      int tw, int th, int cn_mode, int step, float* map, cvm_result* out, bool direct = false,
+                      const PeakReq* pk = nullptr) {
   Engine& E = engine();
   std::lock_guard<std::recursive_mutex> lk(E.mu);
   CHK(E.init());
@@ -557,9 +599,17 @@ static int match_host(const uint8_t* img, int64_t istride, int iw, int ih, const
     HIPCHK(xfer_h2d(E.tpl.p, tpl, tb, s));
   }
   float* d_map = nullptr;
-  if (map) {
+  if (map || pk) {
     CHK(E.mapbuf.ensure(sizeof(float) * (size_t)rw * rh));
     d_map = E.mapbuf.as<float>();
+  }
+  int32_t* d_xy = nullptr;
+  float* d_vals = nullptr;
+  if (pk) {
+    CHK(E.pkscr.ensure(peaks_scratch_bytes(rw, rh)));
+    CHK(E.pkout.ensure(sizeof(int32_t) * 3 * (size_t)pk->cap + 16));
+    d_xy = E.pkout.as<int32_t>();
+    d_vals = (float*)(d_xy + 2 * (size_t)pk->cap);
   }
   if (direct)
     CHK(run_match_direct(E, s, E.img.as<uint8_t>(), istride, iw, ih, E.tpl.as<uint8_t>(), tstride, tw, th, cn_mode,
@@ -570,6 +620,17 @@ static int match_host(const uint8_t* img, int64_t istride, int iw, int ih, const
   if (map) {
     Stage S(E, "d2h", s);
     HIPCHK(xfer_d2h(map, d_map, sizeof(float) * (size_t)rw * rh, s));
+  }
+  if (pk) {
+    CHK(peaks_dev(E, s, d_map, rw, rh, pk->thresh, pk->rx, pk->ry, pk->cap, d_xy, d_vals));
+    int tot = 0;
+    CHK(peaks_total_host(E, s, rw, rh, &tot));
+    int k = std::min(tot, pk->cap);
+    if (k > 0) {
+      HIPCHK(xfer_d2h(pk->xy, d_xy, sizeof(int32_t) * 2 * (size_t)k, s));
+      HIPCHK(xfer_d2h(pk->vals, d_vals, sizeof(float) * (size_t)k, s));
+    }
+    *pk->count = tot;
   }
   return finish_result(E, s, out);
 }
@@ -1346,6 +1407,37 @@ int cvm_match_map(const uint8_t* img, int64_t istride, int32_t iw, int32_t ih, c
   CHK(check_args(istride, iw, ih, tstride, tw, th, 4, 4));
   if (!map) return fail(CVM_E_BAD_ARGS, "cvmatch: MatchMap needs an output map");
   return match_host(img, istride, iw, ih, tpl, tstride, tw, th, 0, 4, map, out);
+}
+
+int cvm_match_peaks(const uint8_t* img, int64_t istride, int32_t iw, int32_t ih, const uint8_t* tpl, int64_t tstride,
+                    int32_t tw, int32_t th, int32_t cn, int32_t step, float thresh, int32_t rx, int32_t ry,
+                    int32_t* xy, float* vals, int32_t cap, int32_t* count, cvm_result* out) {
+  CHK(rgba_args(img, iw, ih, tpl, tw, th));
+  if (cn == 0) {
+    if (step != 4) return fail(CVM_E_BAD_ARGS, "cvmatch: the alpha rule (cn=0) needs step 4");
+    CHK(check_args(istride, iw, ih, tstride, tw, th, 4, 4));
+  } else {
+    CHK(check_args(istride, iw, ih, tstride, tw, th, cn, step));
+  }
+  PeakReq pk{thresh, rx, ry, cap, xy, vals, count};
+  CHK(check_peaks(iw - tw + 1, ih - th + 1, pk.rx, pk.ry, cap, xy, vals, count));
+  return match_host(img, istride, iw, ih, tpl, tstride, tw, th, cn, step, nullptr, out, false, &pk);
+}
+
+int cvm_find_peaks_device(const float* d_map, int32_t w, int32_t h, float thresh, int32_t rx, int32_t ry,
+                          int32_t* d_xy, float* d_vals, int32_t cap, int32_t* count, void* stream) {
+  if (!d_map) return fail(CVM_E_NIL_IMAGE, "cvmatch: nil image");
+  CHK(check_peaks(w, h, rx, ry, cap, d_xy, d_vals, count));
+  Engine& E = engine();
+  std::lock_guard<std::recursive_mutex> lk(E.mu);
+  CHK(E.init());
+  hipStream_t s = stream ? (hipStream_t)stream : E.stream;
+  CHK(peaks_dev(E, s, d_map, w, h, thresh, rx, ry, cap, d_xy, d_vals));
+  int tot = 0;
+  CHK(peaks_total_host(E, s, w, h, &tot));
+  E.stage_collect();
+  *count = tot;
+  return 0;
 }
 
 int cvm_match_gray(const uint8_t* img, int64_t istride, int32_t iw, int32_t ih, int32_t istep, const uint8_t* tpl,

@@ -210,6 +210,16 @@ hipError_t host_alloc(size_t bytes, void** out);
 hipError_t host_free(void* p);
 hipError_t xfer_h2d(void* dst, const void* src, size_t n, hipStream_t s);
 hipError_t xfer_d2h(void* dst, const void* src, size_t n, hipStream_t s);
+// Peaks (cvm_peaks.hip): threshold + local-maximum suppression over a map
+// with rx/ry already clamped to the map (rx <= peaks_max_rx()).  Writes up to
+// cap (x, y) pairs and values in raster order and the total count into the
+// scratch's peaks_total slot.
+size_t peaks_scratch_bytes(int rw, int rh);
+int peaks_max_rx();
+int* peaks_total(void* scratch, int rw, int rh);
+void launch_peaks(const float* map, int rw, int rh, float thresh, int rx, int ry, void* scratch, int cap,
+                  int32_t* xy, float* vals, hipStream_t s);
+
 // the copy-thread pool alone (host memory only), for its self-test
 void host_copy(void* dst, const void* src, size_t n, int threads);
 

@@ -120,6 +120,27 @@ int cvm_fleet_find_all(cvm_fleet *f, const uint8_t *img, int64_t stride, int32_t
 
 void cvm_fleet_free(cvm_fleet *f);
 
+/* -------------------------------------------------- every occurrence -- */
+/* The step after MatchMap that README.md:82-90 describes and
+ * TestFindAllOccurrences (cvmatch_test.go:320-381) verifies, run on the
+ * device so the map never crosses PCIe: (x, y) is a peak unless
+ * map[y*w+x] < thresh or an element within |dx| <= rx, |dy| <= ry (inside
+ * the map) is strictly greater; tied maxima are all peaks.  Peaks are
+ * written in raster order (the test's loop order): xy[2i], xy[2i+1] and
+ * vals[i] for i < min(count, cap); *count receives the total, which may
+ * exceed cap.  The test uses thresh 0.999 and rx, ry = half the template.
+ *
+ * cvm_match_peaks: matchU8 (impl.go:1126-1155) on host buffers, then the
+ * peaks of its map; cn = 0 selects Match's RGBA alpha rule (step 4). */
+int cvm_match_peaks(const uint8_t *img, int64_t istride, int32_t iw, int32_t ih, const uint8_t *tpl, int64_t tstride,
+                    int32_t tw, int32_t th, int32_t cn, int32_t step, float thresh, int32_t rx, int32_t ry,
+                    int32_t *xy, float *vals, int32_t cap, int32_t *count, cvm_result *out);
+/* The peaks of a w x h map already in device memory; d_xy / d_vals are
+ * device buffers, *count is written on the host when the call returns.
+ * stream: a hipStream_t or NULL for the library's stream. */
+int cvm_find_peaks_device(const float *d_map, int32_t w, int32_t h, float thresh, int32_t rx, int32_t ry,
+                          int32_t *d_xy, float *d_vals, int32_t cap, int32_t *count, void *stream);
+
 /* ------------------------------------------------- direct (mixed) form -- */
 /* The same calls with the window x template sum computed exactly on int8
  * MFMA (bytes offset by 128, int32 accumulation) and every score evaluated

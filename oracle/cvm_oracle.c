@@ -1365,3 +1365,40 @@ uint64_t orc_golden_hash(const float *res, size_t n, const orc_result *r) {
 #undef PUT32
   return h;
 }
+
+/* ------------------------------------------------------------- peaks -- */
+
+/* Every occurrence on a response map: the threshold + local-maximum
+ * suppression recipe of TestFindAllOccurrences (cvmatch_test.go:346-365),
+ * with the threshold and the neighbourhood half-sizes as parameters.  (x, y)
+ * is a peak unless resp < thresh or some in-bounds element within |dx| <= rx,
+ * |dy| <= ry is strictly greater.  Peaks are written in raster order, at most
+ * cap of them; returns the total count. */
+int orc_find_peaks(const float *resp, int w, int h, float thresh, int rx, int ry, int32_t *xy, float *vals, int cap) {
+  int count = 0;
+  for (int y = 0; y < h; y++) {
+    for (int x = 0; x < w; x++) {
+      float v = resp[(size_t)y * w + x];
+      if (v < thresh) continue;
+      int is_max = 1;
+      for (int dy = -ry; dy <= ry && is_max; dy++) {
+        for (int dx = -rx; dx <= rx; dx++) {
+          int nx = x + dx, ny = y + dy;
+          if (nx >= 0 && ny >= 0 && nx < w && ny < h && resp[(size_t)ny * w + nx] > v) {
+            is_max = 0;
+            break;
+          }
+        }
+      }
+      if (is_max) {
+        if (count < cap) {
+          xy[2 * count] = x;
+          xy[2 * count + 1] = y;
+          vals[count] = v;
+        }
+        count++;
+      }
+    }
+  }
+  return count;
+}

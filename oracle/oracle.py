@@ -96,6 +96,8 @@ def lib():
         L.orc_fp64_points.restype = C.c_int
         L.orc_golden_hash.argtypes = [P, C.c_size_t, C.POINTER(OrcResult)]
         L.orc_golden_hash.restype = C.c_uint64
+        L.orc_find_peaks.argtypes = [P, C.c_int, C.c_int, C.c_float, C.c_int, C.c_int, P, P, C.c_int]
+        L.orc_find_peaks.restype = C.c_int
         _lib = L
     return _lib
 
@@ -300,3 +302,18 @@ def golden_hash(res: np.ndarray, tup) -> int:
     res = np.ascontiguousarray(res, dtype=np.float32).ravel()
     r = OrcResult(*tup)
     return int(lib().orc_golden_hash(_ptr(res), res.size, C.byref(r)))
+
+
+def find_peaks(resp: np.ndarray, thresh: float, rx: int, ry: int, cap: int | None = None):
+    """Threshold + local-maximum suppression (cvmatch_test.go:346-365) on a
+    (h, w) float32 map: (count, xy[n, 2], vals[n]) in raster order, n =
+    min(count, cap)."""
+    resp = np.ascontiguousarray(resp, dtype=np.float32)
+    h, w = resp.shape
+    if cap is None:
+        cap = w * h
+    xy = np.zeros((max(cap, 1), 2), dtype=np.int32)
+    vals = np.zeros(max(cap, 1), dtype=np.float32)
+    n = lib().orc_find_peaks(_ptr(resp), w, h, float(thresh), rx, ry, _ptr(xy), _ptr(vals), cap)
+    k = min(n, cap)
+    return n, xy[:k], vals[:k]

@@ -92,3 +92,30 @@ def test_copy_pool_splits_exactly(cvm):
         assert rc == 0
         assert np.array_equal(dst[off:off + n], src[off:off + n]), (n, th, off)
         assert not np.any(dst[:off] != 0xEE) and not np.any(dst[off + n:] != 0xEE), (n, th, off)
+
+
+def test_peak_argument_errors(cvm):
+    """Argument checks of the peak entry points fail before any GPU work."""
+    L = cvm.lib()
+    img = np.zeros((40, 50, 4), np.uint8)
+    tpl = np.zeros((8, 8, 4), np.uint8)
+    xy = np.zeros(16, np.int32)
+    vals = np.zeros(8, np.float32)
+    cnt = ctypes.c_int32()
+    out = cvm.cvm_result()
+    P = ctypes.c_void_p
+
+    def call(rx=2, ry=2, cap=8, xyp=P(xy.ctypes.data), vp=P(vals.ctypes.data), cn=0, step=4, cntp=ctypes.byref(cnt)):
+        return L.cvm_match_peaks(P(img.ctypes.data), 200, 50, 40, P(tpl.ctypes.data), 32, 8, 8, cn, step, 0.5, rx, ry,
+                                 xyp, vp, cap, cntp, ctypes.byref(out))
+
+    assert call(rx=-1) == cvm.CVM_E_BAD_ARGS
+    assert call(ry=-3) == cvm.CVM_E_BAD_ARGS
+    assert call(cap=-1) == cvm.CVM_E_BAD_ARGS
+    assert call(xyp=None) == cvm.CVM_E_BAD_ARGS
+    assert call(cntp=None) == cvm.CVM_E_BAD_ARGS
+    assert call(cn=0, step=1) == cvm.CVM_E_BAD_ARGS
+    assert L.cvm_find_peaks_device(None, 10, 10, 0.5, 1, 1, None, None, 0, ctypes.byref(cnt), None) == \
+        cvm.CVM_E_NIL_IMAGE
+    assert L.cvm_find_peaks_device(P(8), 0, 10, 0.5, 1, 1, None, None, 0, ctypes.byref(cnt), None) == \
+        cvm.CVM_E_EMPTY_IMAGE

@@ -329,6 +329,34 @@ def test_find_all_occurrences(orc):
         if not np.any(nb > resp[y, x]):
             hits.append((int(x), int(y)))
     assert sorted(hits) == sorted(spots)
+    # the oracle's restatement of the same loop: raster order, same hits
+    n, xy, vals = orc.find_peaks(resp, 0.999, 16, 12)
+    assert n == 5 and [tuple(p) for p in xy] == sorted(spots, key=lambda p: (p[1], p[0]))
+    assert np.array_equal(vals, resp[xy[:, 1], xy[:, 0]])
+
+
+@pytest.mark.parametrize("rx,ry", [(0, 0), (1, 0), (0, 2), (3, 2), (7, 11), (40, 3), (80, 64)])
+def test_find_peaks_against_max_filter(orc, rx, ry):
+    """orc_find_peaks against an independent formulation: a peak is a point
+    >= thresh that equals the maximum of its clipped window (scipy's
+    maximum_filter with -inf outside the map).  Quantised values make ties
+    (both tied points stay peaks: suppression needs a strictly greater
+    neighbour); the truncated call keeps the raster-order prefix."""
+    from scipy.ndimage import maximum_filter
+
+    rng = np.random.default_rng(rx * 31 + ry)
+    for h, w, levels in [(37, 53, 5), (1, 70, 3), (64, 1, 4), (90, 120, 50)]:
+        resp = (rng.integers(0, levels, (h, w)).astype(np.float32) / (levels - 1) * 2 - 1).astype(np.float32)
+        mf = maximum_filter(resp, size=(2 * ry + 1, 2 * rx + 1), mode="constant", cval=-np.inf)
+        for thresh in (-2.0, 0.0, 0.5, 1.0):
+            want = np.argwhere((resp >= thresh) & (resp >= mf))  # (y, x), raster order
+            n, xy, vals = orc.find_peaks(resp, thresh, rx, ry)
+            assert n == len(want), (h, w, thresh)
+            assert np.array_equal(xy[:, ::-1], want), (h, w, thresh)
+            assert np.array_equal(vals, resp[want[:, 0], want[:, 1]])
+            if n > 2:
+                k, xy2, _ = orc.find_peaks(resp, thresh, rx, ry, cap=n // 2)
+                assert k == n and np.array_equal(xy2, xy[:n // 2])
 
 
 def test_matcher_find_equals_match(orc):

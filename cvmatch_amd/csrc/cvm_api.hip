@@ -555,6 +555,8 @@ static int check_peaks(int rw, int rh, int& rx, int& ry, int cap, const void* xy
                        const void* count) {
   if (!count || (cap > 0 && (!xy || !vals))) return fail(CVM_E_BAD_ARGS, "cvmatch: nil peak output");
   if (rw <= 0 || rh <= 0) return fail(CVM_E_EMPTY_IMAGE, "cvmatch: empty image");
+  if ((long long)rw * rh >= (1LL << 31))
+    return fail(CVM_E_UNSUPPORTED, "cvmatch: result map of %dx%d exceeds 2^31 elements", rw, rh);
   if (rx < 0 || ry < 0 || cap < 0)
     return fail(CVM_E_BAD_ARGS, "cvmatch: bad peak arguments (rx=%d ry=%d cap=%d)", rx, ry, cap);
   // a window wider than the map sees the same elements as one just as wide

@@ -8,14 +8,20 @@
 // max is formed by strict comparisons (it is then one of the window's
 // elements), so the window test is separable:
 //   k_peak_rowmax  row pass: max over [x-rx, x+rx] of each row, staged in LDS
-//                  (-inf outside the row stands for "no element there");
+//                  (-inf outside the row stands for "no element there"), by
+//                  doubling: maxima over 1, 2, 4 ... P <= 2rx+1 elements, then
+//                  the window is two overlapping length-P runs (log2 steps
+//                  per point instead of 2rx+1); very wide windows fall back to
+//                  a linear scan of the staged row;
 //   k_peak_mark    column pass over the row maxima, only for points that pass
-//                  the threshold, with early exit at the first greater one;
-//                  one 64-bit ballot per wave and a count per 256-point raster
-//                  segment;
-//   k_peak_scan    exclusive scan of the segment counts (one workgroup);
-//   k_peak_emit    each hit's rank = its segment's offset + the hits before it
-//                  in the segment, so peaks come out in raster order, as the
+//                  the threshold, stopping at the first greater one; four
+//                  passes of 64 consecutive points per wave, four 64-bit
+//                  ballots per wave and a count per 1024-point segment;
+//   k_peak_tilescan  exclusive scan of the segment counts within tiles of
+//                  1024 segments, and each tile's sum;
+//   k_peak_emit    each hit's rank = the sums of the tiles before its own +
+//                  its segment's offset in the tile + the hits before it in
+//                  the segment, so peaks come out in raster order, as the
 //                  reference's loop appends them.
 // The map is read about twice (once per pass) plus the window re-reads that
 // the caches absorb; the column pass touches only thresholded points.
@@ -27,146 +33,221 @@
 
 namespace cvm {
 
-constexpr int PK_SEG = 256;  // raster points per workgroup (4 waves)
+constexpr int PK_T = 256;               // threads per workgroup (4 waves)
+constexpr int PK_PTS = 4;               // points per lane, 64 apart
+constexpr int PK_SEG = PK_T * PK_PTS;   // raster points per segment
+constexpr int PK_TILE = 1024;           // segments per scan tile
+constexpr int PK_MASKS = (PK_T / 64) * PK_PTS;  // ballots per segment
 
-__global__ __launch_bounds__(PK_SEG) void k_peak_rowmax(const float* __restrict__ map, int rw, int rh, int rx,
-                                                        float* __restrict__ rmax) {
-  extern __shared__ float seg[];  // PK_SEG + 2 rx
-  const int x0 = blockIdx.x * PK_SEG;
-  const int n = PK_SEG + 2 * rx;
+// both ping-pong buffers fit 64 KiB
+constexpr int PK_DBL_MAX_N = 65536 / (2 * (int)sizeof(float));
+
+template <bool DOUBLING>
+__global__ __launch_bounds__(PK_T) void k_peak_rowmax(const float* __restrict__ map, int rw, int rh, int rx,
+                                                      float* __restrict__ rmax) {
+  extern __shared__ float lds[];  // DOUBLING: 2 x n, else n
+  const int x0 = blockIdx.x * PK_T;
+  const int n = PK_T + 2 * rx;
+  const int k = 2 * rx + 1;
   for (int y = blockIdx.y; y < rh; y += gridDim.y) {
     const float* row = map + (size_t)y * rw;
-    for (int i = threadIdx.x; i < n; i += PK_SEG) {
+    float* a = lds;
+    for (int i = threadIdx.x; i < n; i += PK_T) {
       const int x = x0 - rx + i;
-      seg[i] = (x >= 0 && x < rw) ? row[x] : -INFINITY;
+      a[i] = (x >= 0 && x < rw) ? row[x] : -INFINITY;
     }
     __syncthreads();
     const int x = x0 + threadIdx.x;
-    if (x < rw) {
-      float m = seg[threadIdx.x];
-      for (int i = 1; i <= 2 * rx; i++) {
-        const float v = seg[threadIdx.x + i];
+    float m;
+    if (DOUBLING) {
+      // a holds the maxima over runs of p elements starting at each index
+      // (valid while index + p <= n); b receives those over 2p
+      float* b = lds + n;
+      int p = 1;
+      while (2 * p <= k) {
+        for (int i = threadIdx.x; i + 2 * p <= n; i += PK_T) {
+          const float u = a[i], v = a[i + p];
+          b[i] = v > u ? v : u;
+        }
+        __syncthreads();
+        float* t = a;
+        a = b;
+        b = t;
+        p *= 2;
+      }
+      // window [t, t + k) = runs [t, t + p) and [t + k - p, t + k)
+      const float u = a[threadIdx.x], v = a[threadIdx.x + k - p];
+      m = v > u ? v : u;
+    } else {
+      m = a[threadIdx.x];
+      for (int i = 1; i < k; i++) {
+        const float v = a[threadIdx.x + i];
         m = v > m ? v : m;
       }
-      rmax[(size_t)y * rw + x] = m;
     }
+    if (x < rw) rmax[(size_t)y * rw + x] = m;
     __syncthreads();
   }
 }
 
-__global__ __launch_bounds__(PK_SEG) void k_peak_mark(const float* __restrict__ map, const float* __restrict__ rmax,
-                                                      int rw, int rh, int ry, float thresh,
-                                                      unsigned long long* __restrict__ masks, int* __restrict__ counts) {
-  __shared__ int wc[PK_SEG / 64];
-  const long long npts = (long long)rw * rh;
-  const long long idx = (long long)blockIdx.x * PK_SEG + threadIdx.x;
-  bool hit = false;
-  if (idx < npts) {
-    const float v = map[idx];
-    if (!(v < thresh)) {
-      const int y = (int)(idx / rw), x = (int)(idx - (long long)y * rw);
-      const int ya = max(0, y - ry), yb = min(rh - 1, y + ry);
-      hit = true;
-      for (int yy = ya; yy <= yb; yy++) {
-        if (rmax[(size_t)yy * rw + x] > v) {
-          hit = false;
-          break;
+// Point j of lane l in wave w of segment g is g*PK_SEG + w*256 + j*64 + l:
+// each of the four passes reads 64 consecutive points, and a horizontal run
+// of candidates spreads over lanes.  A candidate's column walk issues its row
+// maxima eight at a time, so a long window costs few memory round trips.
+__global__ __launch_bounds__(PK_T) void k_peak_mark(const float* __restrict__ map, const float* __restrict__ rmax,
+                                                    int rw, int rh, int ry, float thresh,
+                                                    unsigned long long* __restrict__ masks, int* __restrict__ counts,
+                                                    int* __restrict__ total) {
+  constexpr int kBatch = 8;
+  __shared__ int wc[PK_T / 64];
+  const unsigned npts = (unsigned)rw * (unsigned)rh;
+  const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+  const unsigned base = (unsigned)blockIdx.x * PK_SEG + (unsigned)w * 64 * PK_PTS + l;
+  int c = 0;
+#pragma unroll
+  for (int j = 0; j < PK_PTS; j++) {
+    const unsigned idx = base + j * 64;
+    bool hit = false;
+    if (idx < npts) {
+      const float v = map[idx];
+      if (!(v < thresh)) {
+        const int y = (int)(idx / (unsigned)rw), x = (int)(idx - (unsigned)y * (unsigned)rw);
+        const int ya = max(0, y - ry), yb = min(rh - 1, y + ry);
+        hit = true;
+        for (int y0 = ya; y0 <= yb && hit; y0 += kBatch) {
+          float r[kBatch];
+#pragma unroll
+          for (int k = 0; k < kBatch; k++) r[k] = rmax[(size_t)min(y0 + k, yb) * rw + x];
+#pragma unroll
+          for (int k = 0; k < kBatch; k++) hit = hit && !(r[k] > v);
         }
       }
     }
+    const unsigned long long b = __ballot(hit);
+    if (l == 0) masks[(size_t)blockIdx.x * PK_MASKS + w * PK_PTS + j] = b;
+    c += __popcll(b);
   }
-  const unsigned long long b = __ballot(hit);
-  const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
-  if (l == 0) {
-    masks[(size_t)blockIdx.x * (PK_SEG / 64) + w] = b;
-    wc[w] = __popcll(b);
-  }
+  if (l == 0) wc[w] = c;
   __syncthreads();
   if (threadIdx.x == 0) {
-    int c = 0;
-    for (int k = 0; k < PK_SEG / 64; k++) c += wc[k];
-    counts[blockIdx.x] = c;
+    int s = 0;
+    for (int i = 0; i < PK_T / 64; i++) s += wc[i];
+    counts[blockIdx.x] = s;
+    if (blockIdx.x == 0) *total = 0;  // k_peak_tilescan adds the tile sums
   }
 }
 
-__global__ __launch_bounds__(1024) void k_peak_scan(const int* __restrict__ counts, int nseg, int* __restrict__ offsets,
-                                                    int* __restrict__ total) {
-  __shared__ int sh[1024];
-  const int per = (nseg + 1023) / 1024;
-  const int a = min(nseg, (int)threadIdx.x * per), b = min(nseg, a + per);
-  int s = 0;
-  for (int i = a; i < b; i++) s += counts[i];
-  sh[threadIdx.x] = s;
+// One workgroup per tile of PK_TILE segments.
+__global__ __launch_bounds__(PK_TILE) void k_peak_tilescan(const int* __restrict__ counts, int nseg,
+                                                           int* __restrict__ offsets, int* __restrict__ tilesum,
+                                                           int* __restrict__ total) {
+  __shared__ int sh[PK_TILE];
+  const int i = blockIdx.x * PK_TILE + threadIdx.x;
+  const int c = i < nseg ? counts[i] : 0;
+  sh[threadIdx.x] = c;
   __syncthreads();
-  for (int off = 1; off < 1024; off <<= 1) {
+  for (int off = 1; off < PK_TILE; off <<= 1) {
     const int t = threadIdx.x >= (unsigned)off ? sh[threadIdx.x - off] : 0;
     __syncthreads();
     sh[threadIdx.x] += t;
     __syncthreads();
   }
-  int run = sh[threadIdx.x] - s;
-  for (int i = a; i < b; i++) {
-    offsets[i] = run;
-    run += counts[i];
+  if (i < nseg) offsets[i] = sh[threadIdx.x] - c;
+  if (threadIdx.x == PK_TILE - 1) {
+    tilesum[blockIdx.x] = sh[PK_TILE - 1];
+    atomicAdd(total, sh[PK_TILE - 1]);
   }
-  if (threadIdx.x == 1023) *total = sh[1023];
 }
 
-__global__ __launch_bounds__(PK_SEG) void k_peak_emit(const float* __restrict__ map, int rw,
-                                                      const unsigned long long* __restrict__ masks,
-                                                      const int* __restrict__ counts, const int* __restrict__ offsets,
-                                                      int cap, int32_t* __restrict__ xy, float* __restrict__ vals) {
+__global__ __launch_bounds__(PK_T) void k_peak_emit(const float* __restrict__ map, int rw,
+                                                    const unsigned long long* __restrict__ masks,
+                                                    const int* __restrict__ counts, const int* __restrict__ offsets,
+                                                    const int* __restrict__ tilesum, int cap, int32_t* __restrict__ xy,
+                                                    float* __restrict__ vals) {
+  __shared__ int part[PK_T / 64];
   const int sgi = blockIdx.x;
-  if (counts[sgi] == 0) return;
+  if (counts[sgi] == 0) return;  // uniform over the workgroup
+  // sum of the tiles before this segment's own
+  const int ntile = sgi / PK_TILE;
+  int acc = 0;
+  for (int i = threadIdx.x; i < ntile; i += PK_T) acc += tilesum[i];
+  for (int o = 32; o >= 1; o >>= 1) acc += __shfl_xor(acc, o);
   const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
-  const unsigned long long* m = masks + (size_t)sgi * (PK_SEG / 64);
-  const unsigned long long b = m[w];
-  if (!((b >> l) & 1ull)) return;
+  if (l == 0) part[w] = acc;
+  __syncthreads();
   int rank = offsets[sgi];
-  for (int k = 0; k < w; k++) rank += __popcll(m[k]);
-  rank += __popcll(b & ((1ull << l) - 1ull));
-  if (rank >= cap) return;
-  const long long idx = (long long)sgi * PK_SEG + threadIdx.x;
-  const int y = (int)(idx / rw), x = (int)(idx - (long long)y * rw);
-  xy[2 * rank] = x;
-  xy[2 * rank + 1] = y;
-  vals[rank] = map[idx];
+  for (int i = 0; i < PK_T / 64; i++) rank += part[i];
+  const unsigned long long* m = masks + (size_t)sgi * PK_MASKS;
+  for (int i = 0; i < w * PK_PTS; i++) rank += __popcll(m[i]);
+  const unsigned long long lt = (1ull << l) - 1ull;
+#pragma unroll
+  for (int j = 0; j < PK_PTS; j++) {
+    const unsigned long long b = m[w * PK_PTS + j];
+    if ((b >> l) & 1ull) {
+      const int r = rank + __popcll(b & lt);
+      if (r < cap) {
+        const unsigned idx = (unsigned)sgi * PK_SEG + (unsigned)w * 64 * PK_PTS + (unsigned)j * 64 + l;
+        const int y = (int)(idx / (unsigned)rw), x = (int)(idx - (unsigned)y * (unsigned)rw);
+        xy[2 * r] = x;
+        xy[2 * r + 1] = y;
+        vals[r] = map[idx];
+      }
+    }
+    rank += __popcll(b);
+  }
 }
 
-int peaks_segments(int rw, int rh) { return (int)(((long long)rw * rh + PK_SEG - 1) / PK_SEG); }
+static int peaks_segments(int rw, int rh) { return (int)(((long long)rw * rh + PK_SEG - 1) / PK_SEG); }
+static int peaks_tiles(int nseg) { return (nseg + PK_TILE - 1) / PK_TILE; }
+
+// scratch: row maxima [rh][rw] | (256-aligned) ballots [nseg][PK_MASKS] |
+// counts [nseg] | offsets [nseg] | tile sums [ntile] | total
+struct PeakScratch {
+  float* rmax;
+  unsigned long long* masks;
+  int *counts, *offsets, *tilesum, *total;
+};
+
+static PeakScratch peaks_layout(void* scratch, int rw, int rh) {
+  const int nseg = peaks_segments(rw, rh);
+  PeakScratch p;
+  p.rmax = (float*)scratch;
+  uintptr_t a = (uintptr_t)scratch + sizeof(float) * (size_t)rw * rh;
+  a = (a + 255) & ~(uintptr_t)255;
+  p.masks = (unsigned long long*)a;
+  p.counts = (int*)(a + sizeof(unsigned long long) * (size_t)nseg * PK_MASKS);
+  p.offsets = p.counts + nseg;
+  p.tilesum = p.offsets + nseg;
+  p.total = p.tilesum + peaks_tiles(nseg);
+  return p;
+}
 
 size_t peaks_scratch_bytes(int rw, int rh) {
   const size_t nseg = (size_t)peaks_segments(rw, rh);
-  return sizeof(float) * (size_t)rw * rh + sizeof(unsigned long long) * nseg * (PK_SEG / 64) +
-         sizeof(int) * (2 * nseg + 1) + 256;
+  return sizeof(float) * (size_t)rw * rh + 256 + sizeof(unsigned long long) * nseg * PK_MASKS +
+         sizeof(int) * (2 * nseg + (size_t)peaks_tiles((int)nseg) + 1);
 }
 
-int peaks_max_rx() { return (65536 / (int)sizeof(float) - PK_SEG) / 2; }
+int peaks_max_rx() { return (65536 / (int)sizeof(float) - PK_T) / 2; }
 
-// scratch: row maxima [rh][rw] | (256-aligned) ballots [nseg][4] | counts
-// [nseg] | offsets [nseg] | total
-static int* peaks_counts(void* scratch, int rw, int rh) {
-  const int nseg = peaks_segments(rw, rh);
-  uintptr_t p = (uintptr_t)scratch + sizeof(float) * (size_t)rw * rh;
-  p = (p + 255) & ~(uintptr_t)255;
-  return (int*)(p + sizeof(unsigned long long) * (size_t)nseg * (PK_SEG / 64));
-}
-
-int* peaks_total(void* scratch, int rw, int rh) { return peaks_counts(scratch, rw, rh) + 2 * peaks_segments(rw, rh); }
+int* peaks_total(void* scratch, int rw, int rh) { return peaks_layout(scratch, rw, rh).total; }
 
 void launch_peaks(const float* map, int rw, int rh, float thresh, int rx, int ry, void* scratch, int cap,
                   int32_t* xy, float* vals, hipStream_t s) {
   const int nseg = peaks_segments(rw, rh);
-  float* rmax = (float*)scratch;
-  int* counts = peaks_counts(scratch, rw, rh);
-  unsigned long long* masks = (unsigned long long*)((uint8_t*)counts - sizeof(unsigned long long) * (size_t)nseg * (PK_SEG / 64));
-  int* offsets = counts + nseg;
-  int* total = offsets + nseg;
-  const dim3 grow((rw + PK_SEG - 1) / PK_SEG, (unsigned)std::min(rh, 65535));
-  hipLaunchKernelGGL(k_peak_rowmax, grow, dim3(PK_SEG), sizeof(float) * (PK_SEG + 2 * rx), s, map, rw, rh, rx, rmax);
-  hipLaunchKernelGGL(k_peak_mark, dim3(nseg), dim3(PK_SEG), 0, s, map, rmax, rw, rh, ry, thresh, masks, counts);
-  hipLaunchKernelGGL(k_peak_scan, dim3(1), dim3(1024), 0, s, counts, nseg, offsets, total);
-  hipLaunchKernelGGL(k_peak_emit, dim3(nseg), dim3(PK_SEG), 0, s, map, rw, masks, counts, offsets, cap, xy, vals);
+  const PeakScratch p = peaks_layout(scratch, rw, rh);
+  const int n = PK_T + 2 * rx;
+  const dim3 grow((rw + PK_T - 1) / PK_T, (unsigned)std::min(rh, 65535));
+  if (n <= PK_DBL_MAX_N)
+    hipLaunchKernelGGL(k_peak_rowmax<true>, grow, dim3(PK_T), 2 * sizeof(float) * n, s, map, rw, rh, rx, p.rmax);
+  else
+    hipLaunchKernelGGL(k_peak_rowmax<false>, grow, dim3(PK_T), sizeof(float) * n, s, map, rw, rh, rx, p.rmax);
+  hipLaunchKernelGGL(k_peak_mark, dim3(nseg), dim3(PK_T), 0, s, map, p.rmax, rw, rh, ry, thresh, p.masks, p.counts,
+                     p.total);
+  hipLaunchKernelGGL(k_peak_tilescan, dim3(peaks_tiles(nseg)), dim3(PK_TILE), 0, s, p.counts, nseg, p.offsets,
+                     p.tilesum, p.total);
+  hipLaunchKernelGGL(k_peak_emit, dim3(nseg), dim3(PK_T), 0, s, map, rw, p.masks, p.counts, p.offsets, p.tilesum,
+                     cap, xy, vals);
 }
 
 }  // namespace cvm

@@ -83,6 +83,26 @@ def test_device_peaks_equal_oracle(cvm, orc, h, w, levels):
                 assert device_peaks(cvm, resp, thresh, rx, ry, 0)[0] == want[0]  # count only
 
 
+def test_device_peaks_wide_windows(cvm, orc):
+    """Row windows wide enough to leave the doubling form for the linear scan
+    of the staged row (more than 8192 staged elements), and just below."""
+    rng = np.random.default_rng(9000)
+    x = np.arange(9000)
+    # a slow quantised wave (period 2300, decaying): the peak set depends on
+    # the window width (2 to 30 peaks for the windows below)
+    resp = np.stack([0.9 * np.sin(2 * np.pi * x / 2300.0 + 1.3 * y) * (1 - x / 20000.0) for y in range(3)])
+    resp = (np.round((resp + rng.normal(0, 0.01, resp.shape)) * 100) / 100).astype(np.float32)
+    for rx, ry in [(5000, 1), (3969, 0), (3967, 2), (8064, 2), (2000, 0)]:
+        for thresh in (-5.0, 0.5):
+            say(f"[peaks] wide map 9000x3 rx {rx} ry {ry} thresh {thresh}")
+            want = orc.find_peaks(resp, thresh, rx, ry)
+            same(device_peaks(cvm, resp, thresh, rx, ry, max(want[0], 1)), want, (rx, ry, thresh))
+    # past the staged row's limit (64 KiB of LDS): refused, not truncated
+    with pytest.raises(cvm.CvmatchError) as e:
+        device_peaks(cvm, resp, 0.5, 8065, 0, 4)
+    assert e.value.code == cvm.CVM_E_UNSUPPORTED
+
+
 def test_find_all_occurrences(cvm, orc):
     """TestFindAllOccurrences cvmatch_test.go:320-381 with this suite's RNG:
     five stamped copies of a 32x24 template on weak noise are recovered

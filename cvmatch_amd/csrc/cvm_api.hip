@@ -14,6 +14,7 @@
 #include <algorithm>
 #include <array>
 #include <atomic>
+#include <functional>
 #include <map>
 #include <memory>
 #include <mutex>
@@ -189,10 +190,11 @@ struct Engine {
   std::recursive_mutex mu;
   bool ready = false;
   hipStream_t stream = nullptr;
-  // side stream of a MatchMap call, forked from and joined back into the
-  // caller's stream by the two events
+  // side stream of a MatchMap call, forked from the caller's stream and
+  // joined back into it twice: before the column pass (call state, template
+  // spectrum) and before the normalisation (its partials)
   hipStream_t side = nullptr;
-  hipEvent_t fork = nullptr, join = nullptr;
+  hipEvent_t fork = nullptr, join = nullptr, join2 = nullptr;
   // one captured device-resident call: replayed while the same arguments
   // repeat and no buffer has moved (the first sighting runs eagerly and
   // allocates, the second captures -- a few milliseconds once -- later ones
@@ -236,6 +238,7 @@ struct Engine {
     HIPCHK(hipStreamCreateWithFlags(&side, hipStreamNonBlocking));
     HIPCHK(hipEventCreateWithFlags(&fork, hipEventDisableTiming));
     HIPCHK(hipEventCreateWithFlags(&join, hipEventDisableTiming));
+    HIPCHK(hipEventCreateWithFlags(&join2, hipEventDisableTiming));
     HIPCHK(hipMalloc(&st, sizeof(CallState)));
     HIPCHK(hipMalloc(&res_d, sizeof(Res6) * 2));
     HIPCHK(hipHostMalloc(&res_h, sizeof(Res6) * 2, hipHostMallocDefault));
@@ -399,9 +402,16 @@ static int build_tspec(Engine& E, hipStream_t s, const CallState* st, const uint
 // records join when done: the parent's row transforms (which need none of
 // them) run first, then s waits for join.  Otherwise all of them are ready
 // on s, except the partials, built here.
+// side (may be null): enqueues the call's side-stream work on E.side and
+// records E.join once the call state and the template spectrum are ready and
+// E.join2 once the normalisation partials are.  It runs after the parent's row pass has
+// been submitted on s, so that pass (which needs none of it) is first in
+// line and overlaps the side work.
+using SideWork = std::function<int()>;
+
 static int corr_and_normalize(Engine& E, hipStream_t s, const CallState* st, const uint8_t* d_img, int64_t istride,
                               int iw, int ih, int tw, int th, int step, int cnMax, const TplSpec& ts,
-                              const PlanTables& pt, float* d_map, Res6* d_res, hipEvent_t join) {
+                              const PlanTables& pt, float* d_map, Res6* d_res, const SideWork* side) {
   const Plan& p = ts.p;
   int rw = iw - tw + 1, rh = ih - th + 1;
   int hwPad = ts.hwPad;
@@ -426,31 +436,41 @@ static int corr_and_normalize(Engine& E, hipStream_t s, const CallState* st, con
   g.log2H2 = ts.dh2 ? ilog2(ts.dh2) : 0;
   int64_t chan = (int64_t)p.dftH * hwPad, tileStride = chan * cnMax;
   int64_t tchan = chan, tchan2 = (int64_t)ts.dh2 * hwPad;
+  // every buffer first: no allocation lands between fork and join
   CHK(E.spec.ensure(sizeof(float2) * (size_t)tileStride * ntiles));
   float2* spec = E.spec.as<float2>();
-  {
-    Stage S(E, "row_fwd", s);
-    launch_row_fwd(d_img, istride, step, cnMax, join ? nullptr : st, g, pt, spec, tileStride, chan, ntiles, s);
-  }
-  if (join) HIPCHK(hipStreamWaitEvent(s, join, 0));
-  {
-    Stage S(E, "cols", s);
-    launch_cols(COL_CORR, spec, spec, tileStride, tileStride, chan, cnMax, st, g, pt, ts.spec, ts.spec2, tchan,
-                tchan2, 1.0f, ntiles, s);
-  }
   float* corr = d_map;
   if (!corr) {
     CHK(E.corr.ensure(sizeof(float) * (size_t)rw * rh));
     corr = E.corr.as<float>();
   }
+  int nparts = norm_partials_count(rw, rh, tw, th, cnMax);
+  CHK(E.parts.ensure(sizeof(Extrema) * (size_t)nparts));
+  if (!side) CHK(E.nscratch.ensure(norm_scratch_bytes(iw, ih, tw, th, cnMax)));
+  if (side) {
+    HIPCHK(hipEventRecord(E.fork, s));
+    HIPCHK(hipStreamWaitEvent(E.side, E.fork, 0));
+  }
+  {
+    Stage S(E, "row_fwd", s);
+    launch_row_fwd(d_img, istride, step, cnMax, side ? nullptr : st, g, pt, spec, tileStride, chan, ntiles, s);
+  }
+  if (side) {
+    CHK((*side)());
+    HIPCHK(hipStreamWaitEvent(s, E.join, 0));
+  }
+  {
+    Stage S(E, "cols", s);
+    launch_cols(COL_CORR, spec, spec, tileStride, tileStride, chan, cnMax, st, g, pt, ts.spec, ts.spec2, tchan,
+                tchan2, 1.0f, ntiles, s);
+  }
   {
     Stage S(E, "row_inv", s);
     launch_row_inv(spec, tileStride, chan, st, cnMax, g, pt, corr, rw, ntiles, s);
   }
-  int nparts = norm_partials_count(rw, rh, tw, th, cnMax);
-  CHK(E.parts.ensure(sizeof(Extrema) * (size_t)nparts));
-  if (!join) {
-    CHK(E.nscratch.ensure(norm_scratch_bytes(iw, ih, tw, th, cnMax)));
+  if (side) {
+    HIPCHK(hipStreamWaitEvent(s, E.join2, 0));
+  } else {
     Stage S(E, "norm_prep", s);
     launch_norm_prep(d_img, istride, iw, ih, step, tw, th, cnMax, st, E.nscratch.p, s);
   }
@@ -502,28 +522,30 @@ static int run_match(Engine& E, hipStream_t s, const uint8_t* d_img, int64_t ist
     CHK(E.tspec2.ensure(sizeof(float2) * tchan2 * cnMax));
     ts.spec2 = E.tspec2.as<float2>();
   }
-  // fork: the call state, the template spectrum and the normalisation
-  // partials are built on the side stream while s transforms the parent rows
-  HIPCHK(hipEventRecord(E.fork, s));
-  HIPCHK(hipStreamWaitEvent(E.side, E.fork, 0));
-  hipStream_t q = E.side;
-  {
-    Stage S(E, "stats", q);
-    launch_state_init(E.st, q);
-    launch_tpl_stats(d_tpl, tstride, tw, th, cnMax, step, E.st, q);
-    if (cn_mode == 0) {
-      launch_alpha_scan(d_img, istride, iw, ih, E.st, 0, q);
-      launch_alpha_scan(d_tpl, tstride, tw, th, E.st, 1, q);
+  // the call state, the template spectrum and the normalisation partials
+  // are built on the side stream while s transforms the parent rows
+  SideWork side = [&]() -> int {
+    hipStream_t q = E.side;
+    {
+      Stage S(E, "stats", q);
+      launch_state_init(E.st, q);
+      launch_tpl_stats(d_tpl, tstride, tw, th, cnMax, step, E.st, q);
+      if (cn_mode == 0) {
+        launch_alpha_scan(d_img, istride, iw, ih, E.st, 0, q);
+        launch_alpha_scan(d_tpl, tstride, tw, th, E.st, 1, q);
+      }
+      launch_state_finalize(E.st, tw, th, cn_mode, q);
     }
-    launch_state_finalize(E.st, tw, th, cn_mode, q);
-  }
-  CHK(build_tspec(E, q, E.st, d_tpl, tstride, tw, th, step, cnMax, p, dh2, pt, ts.spec, ts.spec2));
-  {
-    Stage S(E, "norm_prep", q);
-    launch_norm_prep(d_img, istride, iw, ih, step, tw, th, cnMax, E.st, E.nscratch.p, q);
-  }
-  HIPCHK(hipEventRecord(E.join, q));
-  return corr_and_normalize(E, s, E.st, d_img, istride, iw, ih, tw, th, step, cnMax, ts, pt, d_map, E.res_d, E.join);
+    CHK(build_tspec(E, q, E.st, d_tpl, tstride, tw, th, step, cnMax, p, dh2, pt, ts.spec, ts.spec2));
+    HIPCHK(hipEventRecord(E.join, q));
+    {
+      Stage S(E, "norm_prep", q);
+      launch_norm_prep(d_img, istride, iw, ih, step, tw, th, cnMax, E.st, E.nscratch.p, q);
+    }
+    HIPCHK(hipEventRecord(E.join2, q));
+    return 0;
+  };
+  return corr_and_normalize(E, s, E.st, d_img, istride, iw, ih, tw, th, step, cnMax, ts, pt, d_map, E.res_d, &side);
 }
 
 static int finish_result(Engine& E, hipStream_t s, cvm_result* out) {

@@ -199,6 +199,53 @@ def peaks_leg(L, d_map, rw, rh, tw, th, reps=20):
             "map_bytes": rw * rh * 4}
 
 
+def windowrules_leg(reps=20):
+    """The window-rules engine over the reference example's demo window
+    (1280x800 RGBA, 4 rules in 3 (ROI, gray) groups, 7 templates): one Run =
+    one frame upload, one gray conversion, three device-view FindAlls and the
+    verdicts.  Beside it, the same groups run as the reference's Run does,
+    each FindAll on its own host view (one upload per group, gray views
+    converted separately)."""
+    import cvmatch_amd as cvm
+    from cvmatch_amd import synth
+    from cvmatch_amd import windowrules as W
+
+    base = synth.window_frame()
+    R = synth.WINDOW_RECTS
+
+    def tpl(i, r):
+        return W.Template(i, synth.crop(base, r))
+
+    panel = tpl(105, R["panel"])
+    rules = [
+        W.Rule(1, "confirm-button", Mode=W.ModeFirst, Threshold=0.95,
+               Templates=[W.Template(100, synth.decoy_widget(120, 36, 99)), tpl(101, R["ok"])]),
+        W.Rule(2, "tray-status", ROI=(1150, 0, 1280, 48), Gray=True, Mode=W.ModeAll, Threshold=0.95,
+               Templates=[tpl(102, R["wifi"]), tpl(103, R["mute"]), W.Template(104, synth.decoy_widget(24, 24, 77))]),
+        W.Rule(3, "main-panel", ROI=(180, 180, 520, 420), Threshold=0.95, Templates=[panel]),
+        W.Rule(4, "oversized-roi", ROI=(0, 0, 40, 40), Threshold=0.95, Templates=[panel]),
+    ]
+    eng = W.NewEngine(rules)
+    frames = [base, synth.window_frame(move_button=True)]
+
+    def host_groups(frame):
+        for g in eng._groups:
+            x0, y0, x1, y1 = (0, 0, frame.shape[1], frame.shape[0]) if g.roi == (0, 0, 0, 0) else g.roi
+            fleet, _ = g.fit(x1 - x0, y1 - y0)
+            if fleet is not None:
+                fleet.find_all(frame[y0:y1, x0:x1])
+
+    out = {"workload": "windowrules demo window 1280x800 RGBA, 4 rules, 3 groups, 7 templates"}
+    for key, fn in (("engine_run", eng.run), ("per_group_host_findall", host_groups)):
+        fn(frames[0])
+        fn(frames[1])
+        t0 = time.perf_counter()
+        for i in range(reps):
+            fn(frames[i % 2])
+        out[key + "_ms"] = round((time.perf_counter() - t0) / reps * 1e3, 3)
+    return out
+
+
 def copy_calibration(mib):
     import cvmatch_amd as cvm
 
@@ -342,6 +389,7 @@ def main():
     line["peaks"] = peaks_leg(L, d_map, rw, rh, tw, th)
     if not args.no_host:
         line["host_buffers"] = host_leg(parent, sub)
+        line["windowrules"] = windowrules_leg()
     if not args.no_fleet:
         line["fleet"] = fleet_leg(args.fleet_frames)
     if not args.no_cpu:

@@ -105,6 +105,10 @@ SIGNATURES = {
     "cvm_match_peaks": ([_P, _I64, _I32, _I32, _P, _I64, _I32, _I32, _I32, _I32, C.c_float, _I32, _I32, _P, _P,
                          _I32, C.POINTER(_I32), _RES], C.c_int),
     "cvm_find_peaks_device": ([_P, _I32, _I32, C.c_float, _I32, _I32, _P, _P, _I32, C.POINTER(_I32), _P], C.c_int),
+    "cvm_matcher_find_device": ([_P, _P, _I64, _I32, _I32, _RES, _P], C.c_int),
+    "cvm_fleet_find_all_device": ([_P, _P, _I64, _I32, _I32, _P, _P], C.c_int),
+    "cvm_fleet_find_all_direct_device": ([_P, _P, _I64, _I32, _I32, _P, _P], C.c_int),
+    "cvm_rgba_to_gray_device": ([_P, _I64, _I32, _I32, _P, _P], C.c_int),
     "cvm_set_threads": ([_I32], _I32),
     "cvm_threads": ([], _I32),
     "cvm_host_alloc": ([_SZ, C.POINTER(_P)], C.c_int),
@@ -401,6 +405,16 @@ class Fleet:
         return [Result(*out[i].tuple()) for i in range(n)]
 
     FindAll = find_all
+
+    def find_all_device(self, d_img, stride, w, h, direct=False, stream=None):
+        """FindAll on a frame (or a region of one) already in device memory:
+        d_img is the device address of the view's first pixel, stride its
+        row pitch in bytes; the view has the fleet's format."""
+        n = len(self._ms)
+        out = (cvm_result * n)()
+        fn = _L().cvm_fleet_find_all_direct_device if direct else _L().cvm_fleet_find_all_device
+        _check(fn(self._h, C.c_void_p(int(d_img)), stride, w, h, C.cast(out, C.c_void_p), stream))
+        return [Result(*out[i].tuple()) for i in range(n)]
 
     def __del__(self):
         h = getattr(self, "_h", None)

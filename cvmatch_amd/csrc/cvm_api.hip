@@ -1622,6 +1622,21 @@ int cvm_alpha_const(const uint8_t* pix, int64_t stride, int32_t w, int32_t h, in
   return 0;
 }
 
+int cvm_rgba_to_gray_device(const uint8_t* d_pix, int64_t stride, int32_t w, int32_t h, uint8_t* d_out,
+                            void* stream) {
+  if (!d_pix || !d_out) return fail(CVM_E_NIL_IMAGE, "cvmatch: nil image");
+  if (w <= 0 || h <= 0) return fail(CVM_E_EMPTY_IMAGE, "cvmatch: empty image");
+  if (stride < (int64_t)w * 4) return fail(CVM_E_BAD_ARGS, "cvmatch: stride %lld below 4*width", (long long)stride);
+  Engine& E = engine();
+  std::lock_guard<std::recursive_mutex> lk(E.mu);
+  CHK(E.init());
+  hipStream_t s = stream ? (hipStream_t)stream : E.stream;
+  launch_rgba_to_gray(d_pix, stride, w, h, d_out, s);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipStreamSynchronize(s));
+  return 0;
+}
+
 int cvm_rgba_to_gray(const uint8_t* pix, int64_t stride, int32_t w, int32_t h, uint8_t* out) {
   if (!pix || !out) return fail(CVM_E_NIL_IMAGE, "cvmatch: nil image");
   if (w <= 0 || h <= 0) return fail(CVM_E_EMPTY_IMAGE, "cvmatch: empty image");
@@ -1648,8 +1663,10 @@ int cvm_matcher_new(const uint8_t* tpl, int64_t stride, int32_t w, int32_t h, in
   return matcher_create(tpl, stride, w, h, step, out);
 }
 
+// img is a host buffer, or with on_device a device buffer (then stream, or
+// the library's stream when null, orders the call)
 static int matcher_find(cvm_matcher* m, const uint8_t* img, int64_t stride, int32_t w, int32_t h, cvm_result* out,
-                        bool direct);
+                        bool direct, bool on_device = false, void* stream = nullptr);
 
 int cvm_matcher_find(cvm_matcher* m, const uint8_t* img, int64_t stride, int32_t w, int32_t h, cvm_result* out) {
   return matcher_find(m, img, stride, w, h, out, false);
@@ -1660,10 +1677,15 @@ int cvm_matcher_find_direct(cvm_matcher* m, const uint8_t* img, int64_t stride, 
   return matcher_find(m, img, stride, w, h, out, true);
 }
 
+int cvm_matcher_find_device(cvm_matcher* m, const uint8_t* d_img, int64_t stride, int32_t w, int32_t h,
+                            cvm_result* out, void* stream) {
+  return matcher_find(m, d_img, stride, w, h, out, false, true, stream);
+}
+
 }  // extern "C"
 
 static int matcher_find(cvm_matcher* m, const uint8_t* img, int64_t stride, int32_t w, int32_t h, cvm_result* out,
-                        bool direct) {
+                        bool direct, bool on_device, void* stream) {
   if (!m) return fail(CVM_E_BAD_ARGS, "cvmatch: null matcher");
   if (!img) return fail(CVM_E_NIL_IMAGE, "cvmatch: nil image");
   if (w <= 0 || h <= 0) return fail(CVM_E_EMPTY_IMAGE, "cvmatch: empty image");
@@ -1677,17 +1699,19 @@ static int matcher_find(cvm_matcher* m, const uint8_t* img, int64_t stride, int3
   Engine& E = engine();
   std::lock_guard<std::recursive_mutex> lk(E.mu);
   CHK(E.init());
-  hipStream_t s = E.stream;
-  size_t ib = (size_t)img_bytes(stride, w, h, m->step);
-  CHK(E.img.ensure(ib));
-  {
+  hipStream_t s = stream ? (hipStream_t)stream : E.stream;
+  const uint8_t* d_img = img;
+  if (!on_device) {
+    size_t ib = (size_t)img_bytes(stride, w, h, m->step);
+    CHK(E.img.ensure(ib));
     Stage S(E, "h2d", s);
     HIPCHK(xfer_h2d(E.img.p, img, ib, s));
+    d_img = E.img.as<uint8_t>();
   }
   if (direct)
-    CHK(matcher_find_direct_dev(E, s, m, E.img.as<uint8_t>(), stride, w, h, E.res_d));
+    CHK(matcher_find_direct_dev(E, s, m, d_img, stride, w, h, E.res_d));
   else
-    CHK(matcher_find_dev(E, s, m, E.img.as<uint8_t>(), stride, w, h, E.res_d));
+    CHK(matcher_find_dev(E, s, m, d_img, stride, w, h, E.res_d));
   return finish_result(E, s, out);
 }
 
@@ -1713,8 +1737,10 @@ int cvm_fleet_new(cvm_matcher* const* ms, int32_t n, cvm_fleet** out) {
   return 0;
 }
 
+// img is a host buffer, or with on_device a device buffer (then stream, or
+// the library's stream when null, orders the call)
 static int fleet_find_all(cvm_fleet* f, const uint8_t* img, int64_t stride, int32_t w, int32_t h, cvm_result* out,
-                          bool direct);
+                          bool direct, bool on_device = false, void* stream = nullptr);
 
 int cvm_fleet_find_all(cvm_fleet* f, const uint8_t* img, int64_t stride, int32_t w, int32_t h, cvm_result* out) {
   return fleet_find_all(f, img, stride, w, h, out, false);
@@ -1725,10 +1751,20 @@ int cvm_fleet_find_all_direct(cvm_fleet* f, const uint8_t* img, int64_t stride, 
   return fleet_find_all(f, img, stride, w, h, out, true);
 }
 
+int cvm_fleet_find_all_device(cvm_fleet* f, const uint8_t* d_img, int64_t stride, int32_t w, int32_t h,
+                              cvm_result* out, void* stream) {
+  return fleet_find_all(f, d_img, stride, w, h, out, false, true, stream);
+}
+
+int cvm_fleet_find_all_direct_device(cvm_fleet* f, const uint8_t* d_img, int64_t stride, int32_t w, int32_t h,
+                                     cvm_result* out, void* stream) {
+  return fleet_find_all(f, d_img, stride, w, h, out, true, true, stream);
+}
+
 }  // extern "C"
 
 static int fleet_find_all(cvm_fleet* f, const uint8_t* img, int64_t stride, int32_t w, int32_t h, cvm_result* out,
-                          bool direct) {
+                          bool direct, bool on_device, void* stream) {
   if (!f) return fail(CVM_E_BAD_ARGS, "cvmatch: null fleet");
   if (!img) return fail(CVM_E_NIL_IMAGE, "cvmatch: nil image");
   if (w <= 0 || h <= 0) return fail(CVM_E_EMPTY_IMAGE, "cvmatch: empty image");
@@ -1749,18 +1785,20 @@ static int fleet_find_all(cvm_fleet* f, const uint8_t* img, int64_t stride, int3
   Engine& E = engine();
   std::lock_guard<std::recursive_mutex> lk(E.mu);
   CHK(E.init());
-  hipStream_t s = E.stream;
-  size_t ib = (size_t)img_bytes(stride, w, h, f->step);
-  CHK(E.img.ensure(ib));
-  {
+  hipStream_t s = stream ? (hipStream_t)stream : E.stream;
+  const uint8_t* d_img = img;
+  if (!on_device) {
+    size_t ib = (size_t)img_bytes(stride, w, h, f->step);
+    CHK(E.img.ensure(ib));
     Stage S(E, "h2d", s);
     HIPCHK(xfer_h2d(E.img.p, img, ib, s));
+    d_img = E.img.as<uint8_t>();
   }
   int pConst = 0;
   if (f->step == 4) {
     // the parent's alpha constancy keys the geometry cache (fleet.go:202-212)
     launch_state_init(E.st, s);
-    launch_alpha_scan(E.img.as<uint8_t>(), stride, w, h, E.st, 0, s);
+    launch_alpha_scan(d_img, stride, w, h, E.st, 0, s);
     CallState hs;
     HIPCHK(hipMemcpyAsync(&hs, E.st, sizeof(CallState), hipMemcpyDeviceToHost, s));
     HIPCHK(hipStreamSynchronize(s));
@@ -1768,9 +1806,9 @@ static int fleet_find_all(cvm_fleet* f, const uint8_t* img, int64_t stride, int3
   }
   CHK(f->res.ensure(sizeof(Res6) * n));
   if (direct)
-    CHK(fleet_find_direct_dev(E, s, f, E.img.as<uint8_t>(), stride, w, h, pConst, f->res.as<Res6>()));
+    CHK(fleet_find_direct_dev(E, s, f, d_img, stride, w, h, pConst, f->res.as<Res6>()));
   else
-    CHK(fleet_find_dev(E, s, f, E.img.as<uint8_t>(), stride, w, h, pConst, f->res.as<Res6>(), flat));
+    CHK(fleet_find_dev(E, s, f, d_img, stride, w, h, pConst, f->res.as<Res6>(), flat));
   std::vector<Res6> hr(n);
   HIPCHK(hipMemcpyAsync(hr.data(), f->res.p, sizeof(Res6) * n, hipMemcpyDeviceToHost, s));
   HIPCHK(hipStreamSynchronize(s));

@@ -150,3 +150,70 @@ def workload_fleet(seed: int = 5, frames: int = 64, members: int = 256, w: int =
             t = add_noise(t, 2.0, rng, channels=[0, 1, 2])
         tpls.append(t)
     return frame_list, tpls
+
+
+# ---------------------------------------------------------- window scene
+# The synthetic application window of the reference's windowrules example
+# (examples/windowrules/main.go:32-105): a 1280x800 gradient with bordered
+# widgets whose interiors are textured by a 32-bit LCG, so the scene is the
+# same on every run.
+
+WINDOW_RECTS = {  # (x0, y0, x1, y1), window coordinates
+    "ok": (860, 700, 980, 736),
+    "ok_moved": (600, 700, 720, 736),
+    "wifi": (1180, 12, 1204, 36),
+    "mute": (1216, 12, 1240, 36),
+    "panel": (200, 200, 500, 400),
+}
+
+
+def draw_widget(img: np.ndarray, rect, seed: int) -> None:
+    """A dark one-pixel border and an interior textured by the LCG
+    s = 1664525 s + 1013904223 (mod 2^32), advanced once per pixel in raster
+    order (border pixels included)."""
+    x0, y0, x1, y1 = rect
+    w, h = x1 - x0, y1 - y0
+    n = w * h
+    s = np.empty(n, dtype=np.uint64)
+    v = seed & 0xFFFFFFFF
+    for i in range(n):
+        v = (v * 1664525 + 1013904223) & 0xFFFFFFFF
+        s[i] = v
+    s = s.reshape(h, w)
+    blk = np.empty((h, w, 4), dtype=np.uint8)
+    blk[..., 0] = 140 + ((s >> 8) & 63)
+    blk[..., 1] = 120 + ((s >> 16) & 63)
+    blk[..., 2] = 90 + ((s >> 24) & 63)
+    blk[..., 3] = 255
+    blk[0, :] = blk[-1, :] = (20, 20, 24, 255)
+    blk[:, 0] = blk[:, -1] = (20, 20, 24, 255)
+    img[y0:y1, x0:x1] = blk
+
+
+def window_frame(move_button: bool = False) -> np.ndarray:
+    """The 1280x800 RGBA window: R = 30 + 50x/1280, G = 40 + 50y/800 (integer
+    division), B = 64, opaque; wifi, mute, panel and the confirm button
+    (moved left when move_button)."""
+    h, w = 800, 1280
+    img = np.empty((h, w, 4), dtype=np.uint8)
+    img[..., 0] = (30 + np.arange(w) * 50 // 1280)[None, :]
+    img[..., 1] = (40 + np.arange(h) * 50 // 800)[:, None]
+    img[..., 2] = 64
+    img[..., 3] = 255
+    draw_widget(img, WINDOW_RECTS["wifi"], 7)
+    draw_widget(img, WINDOW_RECTS["mute"], 9)
+    draw_widget(img, WINDOW_RECTS["panel"], 33)
+    draw_widget(img, WINDOW_RECTS["ok_moved" if move_button else "ok"], 21)
+    return img
+
+
+def crop(img: np.ndarray, rect) -> np.ndarray:
+    x0, y0, x1, y1 = rect
+    return np.ascontiguousarray(img[y0:y1, x0:x1])
+
+
+def decoy_widget(w: int, h: int, seed: int) -> np.ndarray:
+    """A widget drawn nowhere on the window."""
+    img = np.zeros((h, w, 4), dtype=np.uint8)
+    draw_widget(img, (0, 0, w, h), seed)
+    return img

@@ -174,6 +174,23 @@ int cvm_match_u8_direct_device(const uint8_t *d_img, int64_t istride, int32_t iw
                                int64_t tstride, int32_t tw, int32_t th, int32_t cn, int32_t step, float *d_map,
                                cvm_result *out, void *stream);
 
+/* Find, FindAll and the gray conversion on frames already in device memory
+ * (no reference counterpart): a caller that runs several fleets over one
+ * captured frame -- one per region of interest, as the windowrules example
+ * does (examples/windowrules/engine.go:276-310) -- uploads the frame once and
+ * passes each region as a device view (pointer + row stride), converting to
+ * gray once for all gray fleets.  Arguments and results as in
+ * (*Matcher).Find matcher.go:83-130, (*Fleet).FindAll fleet.go:176-332 and
+ * rgbToGray cvmatch.go:222-240; out is host memory, written on return. */
+int cvm_matcher_find_device(cvm_matcher *m, const uint8_t *d_img, int64_t stride, int32_t w, int32_t h,
+                            cvm_result *out, void *stream);
+int cvm_fleet_find_all_device(cvm_fleet *f, const uint8_t *d_img, int64_t stride, int32_t w, int32_t h,
+                              cvm_result *out, void *stream);
+int cvm_fleet_find_all_direct_device(cvm_fleet *f, const uint8_t *d_img, int64_t stride, int32_t w, int32_t h,
+                                     cvm_result *out, void *stream);
+int cvm_rgba_to_gray_device(const uint8_t *d_pix, int64_t stride, int32_t w, int32_t h, uint8_t *d_out,
+                            void *stream);
+
 /* Device memory helpers for callers without their own allocator.  The two
  * copies are synchronous and take the staged host path described below. */
 int cvm_device_alloc(size_t bytes, void **out);

@@ -18,6 +18,7 @@ groups.  Each group's results equal FindAll on the host view bit for bit.
 from __future__ import annotations
 
 import ctypes as C
+import threading
 from dataclasses import dataclass, field
 from typing import List, Optional, Tuple
 
@@ -184,6 +185,10 @@ class Engine:
             g.all_idx = list(range(len(g.members)))
         self._dev = C.c_void_p()
         self._dev_cap = 0
+        # Run is safe for concurrent use (engine.go:123): runs of one engine
+        # share its device frame buffer and sub-fleet memos, so they take
+        # turns (the GPU work of every call is ordered by the library anyway)
+        self._lock = threading.Lock()
 
     def _device(self, nbytes: int) -> int:
         L = lib()
@@ -199,6 +204,10 @@ class Engine:
     def run(self, frame) -> List[RuleResult]:
         """Run engine.go:276-345 on an (h, w, 4) RGBA frame (window origin at
         (0, 0)): verdicts in rule order."""
+        with self._lock:
+            return self._run(frame)
+
+    def _run(self, frame) -> List[RuleResult]:
         p, stride, fw, fh, step = view(frame)
         if step != 4:
             raise EngineError("windowrules: frames are RGBA (h, w, 4) arrays")

@@ -1010,6 +1010,10 @@ static int fleet_find_dev(Engine& E, hipStream_t s, cvm_fleet* f, const uint8_t*
     f->geom = ng;
     g = ng;
   }
+  // The normalisation's chunk prefix sums depend on the frame and the
+  // channel count only: built once per channel count (from the first
+  // non-flat member that takes the wave form) and shared by every member.
+  void* prep[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
   for (size_t bi = 0; bi < f->buckets.size(); bi++) {
     const FleetBucket& b = f->buckets[bi];
     BucketGeom& bg = *g->bg[bi];
@@ -1075,12 +1079,17 @@ static int fleet_find_dev(Engine& E, hipStream_t s, cvm_fleet* f, const uint8_t*
       }
       int nparts = norm_partials_count(rw, rh, m->w, m->h, cn);
       CHK(E.parts.ensure(sizeof(Extrema) * (size_t)nparts));
-      CHK(E.nscratch.ensure(norm_scratch_bytes(pw, ph, m->w, m->h, cn)));
+      if (!prep[cn] && m->varSum != 0 && norm_wave_ok(m->w, m->h, cn)) {
+        // sized as the direct form sizes it, so neither form reallocates it
+        CHK(f->dscr[cn].ensure(norm_scratch_bytes(pw, ph, 1, 1, cn)));
+        Stage S(E, "norm_prep", s);
+        launch_norm_prep(d_img, istride, pw, ph, f->step, m->w, m->h, cn, mst, f->dscr[cn].p, s);
+        prep[cn] = f->dscr[cn].p;
+      }
       {
         Stage S(E, "fleet_normalize", s);
-        launch_norm_prep(d_img, istride, pw, ph, f->step, m->w, m->h, cn, mst, E.nscratch.p, s);
         launch_normalize(d_img, istride, pw, ph, f->step, m->w, m->h, rw, rh, cn, mst, corr, nullptr,
-                         E.parts.as<Extrema>(), E.nscratch.p, s);
+                         E.parts.as<Extrema>(), prep[cn], s);
         launch_reduce(E.parts.as<Extrema>(), nparts, mst, rw, d_res + mi, nullptr, rh, s);
       }
     }

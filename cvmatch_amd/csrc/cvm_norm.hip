@@ -260,7 +260,7 @@ int norm_partials_count(int rw, int rh, int tw, int th, int cnMax) {
 size_t norm_scratch_bytes(int iw, int ih, int tw, int th, int cnMax) {
   if (!norm_plan(tw, th, cnMax).wave) return 0;
   size_t nch = (size_t)(ih + NB_ROWS - 1) / NB_ROWS;
-  return nch * (size_t)iw * 3 * sizeof(uint32_t);
+  return (nch + 1) * (size_t)iw * 3 * sizeof(uint32_t);
 }
 
 // One workgroup: result columns [xs, xs+256) x rows [ys, ye).  Each thread
@@ -468,15 +468,17 @@ __global__ __launch_bounds__(256) void k_normalize(const uint8_t* __restrict__ i
 // One wave (a 64-thread workgroup) owns result columns [xs, xs + 64*Q) and
 // rows [ys, ys + NB_ROWS).  Lane l keeps the exact column sums of CPT
 // consecutive image columns of the strip over the current th window rows.
-// The sums at the band start come from per-chunk partial column sums of
-// NB_ROWS rows (k_colsum_chunks reads every pixel once) plus th % NB_ROWS
-// rows read directly; the window then slides down one row per step.
+// The sums at the band start are the difference of two column prefix sums
+// over NB_ROWS-row chunks (k_colsum_chunks reads every pixel once,
+// k_colsum_prefix accumulates the chunks) plus th % NB_ROWS rows read
+// directly; the window then slides down one row per step.
 // Horizontal window sums are differences of a wave-level prefix scan in LDS.
 
 // Channel-byte mask of a packed pixel word: bytes of channels >= cn cleared.
 __device__ __forceinline__ uint32_t chan_mask(int cn) { return cn >= 4 ? 0xFFFFFFFFu : ((1u << (8 * cn)) - 1u); }
 
 // Partial column sums of chunk c (rows [c*NB_ROWS, c*NB_ROWS + NB_ROWS)),
+// stored at row c + 1 of the scratch (row 0 is the empty prefix),
 // three words per image column: ch0 | ch2 << 16, ch1 | ch3 << 16 (each at
 // most NB_ROWS * 255) and the sum of squares over the first cn channels.
 __global__ __launch_bounds__(256) void k_colsum_chunks(const uint8_t* __restrict__ img, int64_t istride, int iw,
@@ -502,10 +504,39 @@ __global__ __launch_bounds__(256) void k_colsum_chunks(const uint8_t* __restrict
     s13 += (w >> 8) & 0x00FF00FFu;
     q = __builtin_amdgcn_udot4(w, w, q, false);
   }
-  uint32_t* o = ps + (int64_t)c * 3 * iw + x;
+  uint32_t* o = ps + (int64_t)(c + 1) * 3 * iw + x;
   o[0] = s02;
   o[iw] = s13;
   o[2 * iw] = q;
+}
+
+// Running sums of the chunk partials down each column, modulo 2^32: row c
+// becomes the sum of chunks [0, c).  A window of whole chunks is then the
+// difference of two rows; the difference is exact because every window sum
+// fits its field (each packed 16-bit channel half <= 255 * 257, the square
+// sum < 2^32), so the wrap of the running sums cancels.
+__global__ __launch_bounds__(256) void k_colsum_prefix(int nwords, int nch, const CallState* __restrict__ st,
+                                                       uint32_t* __restrict__ ps) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nwords || st->flat) return;
+  uint32_t* p = ps + i;
+  p[0] = 0u;
+  uint32_t acc = 0u;
+  int c = 1;
+  for (; c + 8 <= nch + 1; c += 8) {
+    uint32_t v[8];
+#pragma unroll
+    for (int k = 0; k < 8; k++) v[k] = p[(int64_t)(c + k) * nwords];
+#pragma unroll
+    for (int k = 0; k < 8; k++) {
+      acc += v[k];
+      p[(int64_t)(c + k) * nwords] = acc;
+    }
+  }
+  for (; c <= nch; c++) {
+    acc += p[(int64_t)c * nwords];
+    p[(int64_t)c * nwords] = acc;
+  }
 }
 
 // Inclusive add-scan of a uint32 over the 64 lanes of a wave with DPP moves
@@ -677,20 +708,23 @@ __device__ __forceinline__ void norm_wave_block(const uint8_t* __restrict__ img,
       c2[j] = 0;
     }
     if (nload > 0) {
+      // whole chunks [by, by + nfull): prefix row by + nfull minus row by
+      // (ys + nfull * NB_ROWS <= ys + th <= ih, so both rows exist)
       const int nfull = th / NB_ROWS;
-      const uint32_t* pc = ps + (int64_t)by * 3 * iw + x0;
-#pragma unroll 2
-      for (int i = 0; i < nfull; i++, pc += (int64_t)3 * iw) {
-        uint32_t t0[CPT], t1[CPT], t2[CPT];
-        load_u32x8<VMODE != 0>(pc, nload, t0);
-        load_u32x8<VMODE != 0>(pc + iw, nload, t1);
-        load_u32x8<VMODE != 0>(pc + 2 * iw, nload, t2);
+      const uint32_t* p0 = ps + (int64_t)by * 3 * iw + x0;
+      const uint32_t* p1 = ps + (int64_t)(by + nfull) * 3 * iw + x0;
+      uint32_t a0[CPT], a1[CPT], a2[CPT], b0[CPT], b1[CPT], b2[CPT];
+      load_u32x8<VMODE != 0>(p0, nload, a0);
+      load_u32x8<VMODE != 0>(p0 + iw, nload, a1);
+      load_u32x8<VMODE != 0>(p0 + 2 * iw, nload, a2);
+      load_u32x8<VMODE != 0>(p1, nload, b0);
+      load_u32x8<VMODE != 0>(p1 + iw, nload, b1);
+      load_u32x8<VMODE != 0>(p1 + 2 * iw, nload, b2);
 #pragma unroll
-        for (int j = 0; j < CPT; j++) {
-          cs02[j] += t0[j];
-          cs13[j] += t1[j];
-          c2[j] += t2[j];
-        }
+      for (int j = 0; j < CPT; j++) {
+        cs02[j] = b0[j] - a0[j];
+        cs13[j] = b1[j] - a1[j];
+        c2[j] = (sq_t)(uint32_t)(b2[j] - a2[j]);
       }
       for (int y = ys + nfull * NB_ROWS; y < ys + th; y++) {
         uint32_t w[CPT];
@@ -886,6 +920,9 @@ void launch_norm_prep(const uint8_t* img, int64_t istride, int iw, int ih, int s
   int words = (step == 4 && ((((uintptr_t)img) | (uintptr_t)istride) & 3) == 0) ? 1 : 0;
   hipLaunchKernelGGL(k_colsum_chunks, dim3((iw + 255) / 256, nch), dim3(256), 0, s, img, istride, iw, ih, step, words,
                      st, (uint32_t*)scratch);
+  const int nwords = 3 * iw;
+  hipLaunchKernelGGL(k_colsum_prefix, dim3((nwords + 255) / 256), dim3(256), 0, s, nwords, nch, st,
+                     (uint32_t*)scratch);
 }
 
 void launch_normalize(const uint8_t* img, int64_t istride, int iw, int ih, int step, int tw, int th, int rw, int rh,

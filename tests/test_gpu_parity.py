@@ -377,3 +377,61 @@ def test_alpha_scan_positions(cvm, orc, w, h):
         padded = np.zeros((h, w + 1, 4), np.uint8)
         padded[:, :w] = img
         assert cvm.alpha_const(padded[:, :w]) is False
+
+
+def _random_cases_large(n=12, seed=7):
+    """Templates past the wave-form normalisation (th > 257, or wide enough to
+    take the int64 form), odd strides, thin images."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for i in range(n):
+        cn, step = [(1, 1), (4, 4), (3, 4), (2, 2)][i % 4]
+        iw, ih = int(rng.integers(300, 1400)), int(rng.integers(270, 900))
+        tw = int(rng.integers(150, min(iw, 700) + 1))
+        th = int(rng.integers(100, min(ih, 420) + 1))
+        if i % 3 == 0:
+            iw, tw = int(rng.integers(2, 40)), 1  # thin column images
+        out.append((100 + i, iw, ih, tw, th, cn, step, int(rng.integers(0, 7)), int(rng.integers(0, 5))))
+    return out
+
+
+@pytest.mark.parametrize("case", _random_cases_large(), ids=lambda c: f"L{c[0]}")
+def test_random_large_templates_bitwise(cvm, orc, case):
+    i, iw, ih, tw, th, cn, step, ipad, tpad = case
+    if tw * th > orc.stats_cap(cn):
+        pytest.skip("template above the exact-statistics bound (rejected by both)")
+    say("random large template", case)
+    rng = np.random.default_rng(2000 + i)
+    istride, tstride = (iw + ipad) * step, (tw + tpad) * step
+    img = rng.integers(0, 256, istride * ih, dtype=np.uint8)
+    tpl = rng.integers(0, 256, tstride * th, dtype=np.uint8)
+    same(cvm.match_u8(img, istride, iw, ih, tpl, tstride, tw, th, cn, step, want_map=True),
+         orc.match_u8(img, istride, iw, ih, tpl, tstride, tw, th, cn, step, threads=8))
+
+
+@pytest.mark.parametrize("seed", [21, 22, 23])
+def test_random_fleets_bitwise(cvm, orc, seed):
+    """Random fleets against the oracle's FindAll: 6-12 members of random
+    sizes (1 to 220 per side), some with varying alpha (their own cn = 4 on a
+    constant-alpha frame), some flat; the same members as a gray fleet."""
+    rng = np.random.default_rng(seed)
+    fw, fh = int(rng.integers(300, 900)), int(rng.integers(200, 600))
+    frame = synth.add_noise(synth.flat_rect_scene(fw, fh, rng, channels=4), 4.0, rng, channels=[0, 1, 2])
+    frame[..., 3] = 255
+    subs = []
+    for k in range(int(rng.integers(6, 13))):
+        w, h = int(rng.integers(1, min(fw, 220) + 1)), int(rng.integers(1, min(fh, 220) + 1))
+        x, y = int(rng.integers(0, fw - w + 1)), int(rng.integers(0, fh - h + 1))
+        s = frame[y:y + h, x:x + w].copy()
+        if k % 4 == 1:
+            s[..., 3] = rng.integers(0, 256, (h, w), dtype=np.uint8)  # varying alpha
+        if k % 5 == 4:
+            s[...] = 77  # flat
+        subs.append(s)
+    say(f"[fleet] seed {seed}: frame {fw}x{fh}, {len(subs)} members")
+    got = [tuple(r) for r in cvm.Fleet(*[cvm.Matcher(s) for s in subs]).find_all(frame)]
+    assert got == orc.fleet_find_all(frame, subs)
+    g = cvm.rgba_to_gray(frame)
+    gsubs = [cvm.rgba_to_gray(s) for s in subs]
+    assert [tuple(r) for r in cvm.Fleet(*[cvm.Matcher(s, gray=True) for s in gsubs]).find_all(g)] == \
+        orc.fleet_find_all(g, gsubs)

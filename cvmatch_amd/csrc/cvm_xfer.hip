@@ -181,16 +181,23 @@ struct Staging {
   CopyPool pool;
   std::map<uintptr_t, size_t> pinned;  // cvm_host_alloc blocks: start -> bytes
 
+  bool unavailable = false;  // the host refused the pinned chunks once: not retried
+
   hipError_t init() {
     if (pin[0]) return hipSuccess;
+    if (unavailable) return hipErrorOutOfMemory;
     for (int k = 0; k < kSlots; k++) {
       hipError_t e = hipHostMalloc((void**)&pin[k], kChunk, hipHostMallocDefault);
       if (e == hipSuccess) e = hipEventCreateWithFlags(&ev[k], hipEventDisableTiming);
       if (e != hipSuccess) {
         for (int i = 0; i <= k; i++) {
           if (pin[i]) (void)hipHostFree(pin[i]);
+          if (ev[i]) (void)hipEventDestroy(ev[i]);
           pin[i] = nullptr;
+          ev[i] = nullptr;
         }
+        unavailable = true;
+        (void)hipGetLastError();  // not a launch failure for a later check to report
         return e;
       }
     }
@@ -252,8 +259,10 @@ void host_copy(void* dst, const void* src, size_t n, int threads) {
 hipError_t xfer_h2d(void* dst, const void* src, size_t n, hipStream_t s) {
   Staging& S = staging();
   std::lock_guard<std::mutex> lk(S.mu);
-  if (n < Staging::kDirect || S.covers(src, n)) return hipMemcpyAsync(dst, src, n, hipMemcpyHostToDevice, s);
-  XCHK(S.init());
+  // without pinned staging (the host refused the allocation) the driver's
+  // pageable path still moves the buffer, only more slowly
+  if (n < Staging::kDirect || S.covers(src, n) || S.init() != hipSuccess)
+    return hipMemcpyAsync(dst, src, n, hipMemcpyHostToDevice, s);
   const int threads = host_threads();
   size_t c = 0;
   for (size_t off = 0; off < n; off += Staging::kChunk, c++) {
@@ -274,11 +283,10 @@ hipError_t xfer_d2h(void* dst, const void* src, size_t n, hipStream_t s) {
   Staging& S = staging();
   std::lock_guard<std::mutex> lk(S.mu);
   if (S.covers(dst, n)) return hipMemcpyAsync(dst, src, n, hipMemcpyDeviceToHost, s);
-  if (n < Staging::kDirect) {
+  if (n < Staging::kDirect || S.init() != hipSuccess) {
     XCHK(hipMemcpyAsync(dst, src, n, hipMemcpyDeviceToHost, s));
     return hipStreamSynchronize(s);
   }
-  XCHK(S.init());
   const int threads = host_threads();
   const size_t nc = (n + Staging::kChunk - 1) / Staging::kChunk;
   size_t issued = 0;

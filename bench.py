@@ -246,6 +246,73 @@ def windowrules_leg(reps=20):
     return out
 
 
+def configs_leg(reps=15):
+    """The other configurations, device resident (inputs in HBM, the map
+    written on the device, graphs replayed like the headline's step):
+    640x480 RGBA Match 48x48 (cn = 4), the 1920x1080 gray UI MatchGray 96x32,
+    and the 2560x1440 gray direct-vs-FFT sweep (ms per call with the map for
+    square templates 8..256; the direct form is the FP64-exact one)."""
+    import cvmatch_amd as cvm
+    from cvmatch_amd import synth
+
+    L = cvm.lib()
+    out = cvm.cvm_result()
+    bufs = []
+
+    def dev(a):
+        p = C.c_void_p()
+        assert L.cvm_device_alloc(max(a.nbytes, 4), C.byref(p)) == 0
+        assert L.cvm_copy_h2d(p, a.ctypes.data_as(C.c_void_p), a.nbytes) == 0
+        bufs.append(p)
+        return p
+
+    def timed(fn):
+        for _ in range(3):  # eager, capture, replay
+            fn()
+        L.cvm_synchronize()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            fn()
+        L.cvm_synchronize()
+        return (time.perf_counter() - t0) / reps
+
+    def call(direct, d_img, istride, iw, ih, d_tpl, tstride, tw, th, cn, step, d_map):
+        f = L.cvm_match_u8_direct_device if direct else L.cvm_match_u8_device
+        rc = f(d_img, istride, iw, ih, d_tpl, tstride, tw, th, cn, step, d_map, C.byref(out), None)
+        if rc != 0:
+            raise RuntimeError(L.cvm_last_error_detail().decode())
+
+    res = {}
+    parent, sub, _ = synth.workload_match_640(seed=1, alpha="random")
+    rw, rh = 640 - 48 + 1, 480 - 48 + 1
+    dp, ds = dev(parent), dev(sub)
+    dm = dev(np.zeros(rw * rh, np.float32))
+    dt = timed(lambda: call(False, dp, 640 * 4, 640, 480, ds, 48 * 4, 48, 48, 0, 4, dm))
+    res["match_640x480_rgba_48"] = {"ms": round(dt * 1e3, 4), "Gwindows_per_s": round(rw * rh / dt / 1e9, 3)}
+    gray, gsub = synth.workload_gray_ui_1080p(seed=2)[:2]
+    rw, rh = 1920 - 96 + 1, 1080 - 32 + 1
+    dg, dgs = dev(gray), dev(gsub)
+    dm = dev(np.zeros(rw * rh, np.float32))
+    dt = timed(lambda: call(False, dg, 1920, 1920, 1080, dgs, 96, 96, 32, 1, 1, dm))
+    res["match_gray_1920x1080_96x32"] = {"ms": round(dt * 1e3, 4), "Gwindows_per_s": round(rw * rh / dt / 1e9, 3)}
+    img, _ = synth.workload_direct_1440p(seed=4, sizes=())
+    di = dev(img)
+    dm = dev(np.zeros(2560 * 1440, np.float32))
+    sweep = []
+    for sz in (8, 16, 24, 32, 48, 64, 96, 128, 192, 256):
+        t = np.ascontiguousarray(img[100:100 + sz, 200:200 + sz])
+        dtp = dev(t)
+        fft = timed(lambda: call(False, di, 2560, 2560, 1440, dtp, sz, sz, sz, 1, 1, dm))
+        dirt = timed(lambda: call(True, di, 2560, 2560, 1440, dtp, sz, sz, sz, 1, 1, dm))
+        sweep.append({"template": sz, "fft_ms": round(fft * 1e3, 4), "direct_ms": round(dirt * 1e3, 4)})
+    res["direct_vs_fft_gray_2560x1440"] = sweep
+    cross = [r["template"] for r in sweep if r["direct_ms"] > r["fft_ms"]]
+    res["crossover_template"] = cross[0] if cross else None
+    for p in bufs:
+        L.cvm_device_free(p)
+    return res
+
+
 def copy_calibration(mib):
     import cvmatch_amd as cvm
 
@@ -387,6 +454,8 @@ def main():
         "stages_ms": {k: round(v["avg_ms"], 5) for k, v in stages.items()},
     }
     line["peaks"] = peaks_leg(L, d_map, rw, rh, tw, th)
+    if not args.no_host:
+        line["configs"] = configs_leg()
     if not args.no_host:
         line["host_buffers"] = host_leg(parent, sub)
         line["windowrules"] = windowrules_leg()

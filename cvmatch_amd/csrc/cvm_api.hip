@@ -529,12 +529,7 @@ static int run_match(Engine& E, hipStream_t s, const uint8_t* d_img, int64_t ist
     {
       Stage S(E, "stats", q);
       launch_state_init(E.st, q);
-      launch_tpl_stats(d_tpl, tstride, tw, th, cnMax, step, E.st, q);
-      if (cn_mode == 0) {
-        launch_alpha_scan(d_img, istride, iw, ih, E.st, 0, q);
-        launch_alpha_scan(d_tpl, tstride, tw, th, E.st, 1, q);
-      }
-      launch_state_finalize(E.st, tw, th, cn_mode, q);
+      launch_call_state(d_img, istride, iw, ih, d_tpl, tstride, tw, th, cnMax, step, cn_mode, E.st, q);
     }
     CHK(build_tspec(E, q, E.st, d_tpl, tstride, tw, th, step, cnMax, p, dh2, pt, ts.spec, ts.spec2));
     HIPCHK(hipEventRecord(E.join, q));
@@ -1165,12 +1160,7 @@ static int run_match_direct(Engine& E, hipStream_t s, const uint8_t* d_img, int6
   {
     Stage S(E, "stats", s);
     launch_state_init(E.st, s);
-    launch_tpl_stats(d_tpl, tstride, tw, th, cnMax, step, E.st, s);
-    if (cn_mode == 0) {
-      launch_alpha_scan(d_img, istride, iw, ih, E.st, 0, s);
-      launch_alpha_scan(d_tpl, tstride, tw, th, E.st, 1, s);
-    }
-    launch_state_finalize(E.st, tw, th, cn_mode, s);
+    launch_call_state(d_img, istride, iw, ih, d_tpl, tstride, tw, th, cnMax, step, cn_mode, E.st, s);
   }
   {
     Stage S(E, "bfrag", s);

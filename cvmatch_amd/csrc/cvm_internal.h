@@ -146,6 +146,12 @@ void launch_tpl_stats(const uint8_t* tpl, int64_t tstride, int tw, int th, int c
 void launch_alpha_scan(const uint8_t* pix, int64_t stride, int w, int h, CallState* st, int which, hipStream_t s);
 // cn_mode: 1..4 fixed; 0 = RGBA auto (3 iff both alpha planes constant)
 void launch_state_finalize(CallState* st, int tw, int th, int cn_mode, hipStream_t s);
+// A one-shot call's state after launch_state_init in three launches instead
+// of six: the template statistics, the template's alpha scan and the
+// parent's first rows together, then the rest of the parent's alpha scan
+// (cn_mode 0), then the finalisation.
+void launch_call_state(const uint8_t* img, int64_t istride, int iw, int ih, const uint8_t* tpl, int64_t tstride,
+                       int tw, int th, int cnMax, int step, int cn_mode, CallState* st, hipStream_t s);
 
 // Exact-statistics normalisation + tail + per-workgroup extrema.
 // map may alias corr (in place) or be null (extrema only).

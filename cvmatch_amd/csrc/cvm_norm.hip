@@ -16,6 +16,8 @@
 
 #include <type_traits>
 
+#include <algorithm>
+
 #include "cvm_internal.h"
 
 namespace cvm {
@@ -45,13 +47,14 @@ __device__ __forceinline__ unsigned long long wave_sum_u64(unsigned long long v)
 
 // templStats impl.go:1063-1079 (sums and square sums per channel, exact).
 // grid.y walks template rows, threads walk columns; no divisions.
-__global__ __launch_bounds__(256) void k_tpl_stats(const uint8_t* __restrict__ tpl, int64_t tstride, int tw, int th,
-                                                   int cnMax, int step, CallState* st) {
+// One workgroup (bx, by) of a gx x gy grid over the template.
+__device__ __forceinline__ void tpl_stats_block(const uint8_t* __restrict__ tpl, int64_t tstride, int tw, int th,
+                                                int cnMax, int step, CallState* st, int bx, int by, int gx, int gy) {
   __shared__ unsigned long long red[4][8];
   unsigned long long s[4] = {0, 0, 0, 0}, q[4] = {0, 0, 0, 0};
-  for (int y = blockIdx.y; y < th; y += gridDim.y) {
+  for (int y = by; y < th; y += gy) {
     const uint8_t* row = tpl + (int64_t)y * tstride;
-    for (int x = blockIdx.x * blockDim.x + threadIdx.x; x < tw; x += gridDim.x * blockDim.x) {
+    for (int x = bx * blockDim.x + threadIdx.x; x < tw; x += gx * blockDim.x) {
       const uint8_t* p = row + (int64_t)x * step;
       for (int k = 0; k < 4; k++)
         if (k < cnMax) {
@@ -82,6 +85,11 @@ __global__ __launch_bounds__(256) void k_tpl_stats(const uint8_t* __restrict__ t
   }
 }
 
+__global__ __launch_bounds__(256) void k_tpl_stats(const uint8_t* __restrict__ tpl, int64_t tstride, int tw, int th,
+                                                   int cnMax, int step, CallState* st) {
+  tpl_stats_block(tpl, tstride, tw, th, cnMax, step, st, blockIdx.x, blockIdx.y, gridDim.x, gridDim.y);
+}
+
 void launch_tpl_stats(const uint8_t* tpl, int64_t tstride, int tw, int th, int cnMax, int step, CallState* st,
                       hipStream_t s) {
   int gx = (tw + 255) / 256;
@@ -96,26 +104,33 @@ void launch_tpl_stats(const uint8_t* tpl, int64_t tstride, int tw, int th, int c
 // answer is then "not constant" whatever the rest of the image holds).
 // grid.y walks rows, grid.x x threads cover a row in 16-byte groups of four
 // pixels when rows are 16-byte aligned, else in single pixels.
-__global__ __launch_bounds__(256) void k_alpha_scan(const uint8_t* __restrict__ pix, int64_t stride, int w, int y_begin,
-                                                    int h, CallState* st, int which, int vec) {
+// One workgroup (bx, by) of a gx x gy grid over rows [y_begin, h).
+__device__ __forceinline__ void alpha_scan_block(const uint8_t* __restrict__ pix, int64_t stride, int w, int y_begin,
+                                                 int h, CallState* st, int which, int vec, int bx, int by, int gx,
+                                                 int gy) {
   const uint32_t a0 = pix[3];
   volatile unsigned int* flag = &st->alpha_bad[which];
   int bad = 0;
   const int w4 = vec ? (w >> 2) : 0;
   const uint32_t want = a0 * 0x01010101u;
-  for (int y = y_begin + blockIdx.y; y < h; y += gridDim.y) {
+  for (int y = y_begin + by; y < h; y += gy) {
     if (*flag) break;
     const uint8_t* row = pix + (int64_t)y * stride;
-    for (int q = blockIdx.x * blockDim.x + threadIdx.x; q < w4; q += gridDim.x * blockDim.x) {
+    for (int q = bx * blockDim.x + threadIdx.x; q < w4; q += gx * blockDim.x) {
       uint4 v = *(const uint4*)(row + (int64_t)q * 16);
       uint32_t al = (v.x >> 24) | ((v.y >> 24) << 8) | ((v.z >> 24) << 16) | ((v.w >> 24) << 24);
       if (al != want) bad = 1;
     }
-    for (int x = 4 * w4 + blockIdx.x * blockDim.x + threadIdx.x; x < w; x += gridDim.x * blockDim.x)
+    for (int x = 4 * w4 + bx * blockDim.x + threadIdx.x; x < w; x += gx * blockDim.x)
       if (row[(int64_t)x * 4 + 3] != a0) bad = 1;
     if (bad) break;
   }
   if (__syncthreads_or(bad) && threadIdx.x == 0) atomicOr(&st->alpha_bad[which], 1u);
+}
+
+__global__ __launch_bounds__(256) void k_alpha_scan(const uint8_t* __restrict__ pix, int64_t stride, int w, int y_begin,
+                                                    int h, CallState* st, int which, int vec) {
+  alpha_scan_block(pix, stride, w, y_begin, h, st, which, vec, blockIdx.x, blockIdx.y, gridDim.x, gridDim.y);
 }
 
 void launch_alpha_scan(const uint8_t* pix, int64_t stride, int w, int h, CallState* st, int which, hipStream_t s) {
@@ -156,6 +171,51 @@ __global__ void k_state_finalize(CallState* st, int tw, int th, int cn_mode) {
 
 void launch_state_finalize(CallState* st, int tw, int th, int cn_mode, hipStream_t s) {
   hipLaunchKernelGGL(k_state_finalize, dim3(1), dim3(64), 0, s, st, tw, th, cn_mode);
+}
+
+// One launch for the independent parts of a call state: workgroups [0, nS)
+// take the template statistics (an sgx x sgy grid), the next pgx x pgy the
+// first ph0 rows of the parent's alpha scan (a varying alpha plane shows up
+// there, and the rest of the scan then leaves at its first flag check), the
+// last tgx x tgy the template's alpha scan.
+__global__ __launch_bounds__(256) void k_call_state(const uint8_t* __restrict__ img, int64_t istride, int iw, int ph0,
+                                                    int ivec, const uint8_t* __restrict__ tpl, int64_t tstride, int tw,
+                                                    int th, int tvec, int cnMax, int step, int sgx, int sgy, int pgx,
+                                                    int pgy, int tgx, int tgy, CallState* st) {
+  const int nS = sgx * sgy, nP = pgx * pgy;
+  int b = blockIdx.x;
+  if (b < nS) {
+    tpl_stats_block(tpl, tstride, tw, th, cnMax, step, st, b % sgx, b / sgx, sgx, sgy);
+  } else if (b < nS + nP) {
+    b -= nS;
+    alpha_scan_block(img, istride, iw, 0, ph0, st, 0, ivec, b % pgx, b / pgx, pgx, pgy);
+  } else {
+    b -= nS + nP;
+    alpha_scan_block(tpl, tstride, tw, 0, th, st, 1, tvec, b % tgx, b / tgx, tgx, tgy);
+  }
+}
+
+void launch_call_state(const uint8_t* img, int64_t istride, int iw, int ih, const uint8_t* tpl, int64_t tstride,
+                       int tw, int th, int cnMax, int step, int cn_mode, CallState* st, hipStream_t s) {
+  const int sgx = std::min(16, (tw + 255) / 256), sgy = std::min(256, th);
+  int pgx = 0, pgy = 0, tgx = 0, tgy = 0, ivec = 0, tvec = 0, ph0 = 0;
+  if (cn_mode == 0) {
+    ivec = ((((uintptr_t)img) | (uintptr_t)istride) & 15) == 0;
+    tvec = ((((uintptr_t)tpl) | (uintptr_t)tstride) & 15) == 0;
+    ph0 = std::min(ih, 8);
+    pgx = std::min(16, ((ivec ? iw / 4 + 1 : iw) + 255) / 256);
+    pgy = ph0;
+    tgx = std::min(16, ((tvec ? tw / 4 + 1 : tw) + 255) / 256);
+    tgy = std::min(512, th);
+  }
+  const int nb = sgx * sgy + pgx * pgy + tgx * tgy;
+  hipLaunchKernelGGL(k_call_state, dim3(nb), dim3(256), 0, s, img, istride, iw, ph0, ivec, tpl, tstride, tw, th, tvec,
+                     cnMax, step, sgx, sgy, pgx, pgy, tgx, tgy, st);
+  if (cn_mode == 0 && ih > ph0) {
+    const int gy = std::min(512, ih - ph0);
+    hipLaunchKernelGGL(k_alpha_scan, dim3(pgx, gy), dim3(256), 0, s, img, istride, iw, ph0, ih, st, 0, ivec);
+  }
+  launch_state_finalize(st, tw, th, cn_mode, s);
 }
 
 // rgbToGray cvmatch.go:222-240: (4899 R + 9617 G + 1868 B + 8192) >> 14.

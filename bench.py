@@ -357,7 +357,7 @@ def main():
         return
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (the path shards per frame or per "
-              f"fleet member; see DESIGN.md)", file=sys.stderr)
+              f"fleet member)", file=sys.stderr)
         sys.exit(2)
 
     import cvmatch_amd as cvm

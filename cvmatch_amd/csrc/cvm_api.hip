@@ -130,7 +130,7 @@ static int64_t stats_cap(int cn) {
   return (cn >= 1 && cn <= 4) ? caps[cn] : 0;
 }
 
-// Kernel limits of this build: row transforms keep two dftW rows in LDS and
+// Kernel limits: row transforms keep two dftW rows in LDS and
 // column slabs dftH x CG complex, both <= 64 KiB; the normalisation strip
 // holds 256 + tw - 1 <= 2048 columns.
 constexpr int kMaxFftW = 4096;
@@ -348,10 +348,10 @@ static int plan_tables(Engine& E, const Plan& p, int dh2, PlanTables& pt) {
 
 static int check_limits(const Plan& p, int tw, int rw, int rh) {
   if (p.dftW > kMaxFftW || p.dftH > kMaxFftH)
-    return fail(CVM_E_UNSUPPORTED, "cvmatch: transform %dx%d exceeds this build's LDS limits (%d x %d)", p.dftW,
+    return fail(CVM_E_UNSUPPORTED, "cvmatch: transform %dx%d exceeds the kernels' LDS limits (%d x %d)", p.dftW,
                 p.dftH, kMaxFftW, kMaxFftH);
   if (tw > kMaxNormTw)
-    return fail(CVM_E_UNSUPPORTED, "cvmatch: template width %d exceeds this build's normalisation strip (%d)", tw,
+    return fail(CVM_E_UNSUPPORTED, "cvmatch: template width %d exceeds the normalisation strip (%d)", tw,
                 kMaxNormTw);
   if ((long long)rw * rh >= (1LL << 31))
     return fail(CVM_E_UNSUPPORTED, "cvmatch: result map of %dx%d exceeds 2^31 elements", rw, rh);
@@ -580,7 +580,7 @@ static int check_peaks(int rw, int rh, int& rx, int& ry, int cap, const void* xy
   rx = std::min(rx, rw - 1);
   ry = std::min(ry, rh - 1);
   if (rx > peaks_max_rx())
-    return fail(CVM_E_UNSUPPORTED, "cvmatch: peak window half-width %d exceeds this build's limit (%d)", rx,
+    return fail(CVM_E_UNSUPPORTED, "cvmatch: peak window half-width %d exceeds the staged row's limit (%d)", rx,
                 peaks_max_rx());
   return 0;
 }

@@ -42,7 +42,7 @@ enum {
   CVM_E_EMPTY_FLEET = 3, /* "cvmatch: empty fleet" fleet.go:87 */
   CVM_E_MIXED_FLEET = 4, /* "cvmatch: fleet mixes color and gray matchers" fleet.go:95 */
   CVM_E_HIP = 5,         /* HIP runtime failure (no reference counterpart) */
-  CVM_E_UNSUPPORTED = 6, /* geometry beyond this build's kernel limits (no reference counterpart) */
+  CVM_E_UNSUPPORTED = 6, /* geometry beyond the library's kernel limits (no reference counterpart) */
   CVM_E_NIL_IMAGE = 7,   /* "cvmatch: nil image" cvmatch.go:172 */
   CVM_E_EMPTY_IMAGE = 8  /* "cvmatch: empty image" cvmatch.go:176 */
 };

@@ -435,3 +435,16 @@ def test_random_fleets_bitwise(cvm, orc, seed):
     gsubs = [cvm.rgba_to_gray(s) for s in subs]
     assert [tuple(r) for r in cvm.Fleet(*[cvm.Matcher(s, gray=True) for s in gsubs]).find_all(g)] == \
         orc.fleet_find_all(g, gsubs)
+
+
+def test_8k_map_bitwise(cvm, orc):
+    """A 7680x4320 RGBA MatchMap (4x the headline's pixels, 3x8+ tiles, a
+    133 MB parent): the whole map bit-identical to the oracle's."""
+    rng = np.random.default_rng(88)
+    parent = synth.box_blur5(rng.integers(0, 256, size=(4320, 7680, 4), dtype=np.uint8))
+    sub = synth.add_noise(parent[2000:2048, 5000:5064].copy(), 2.0, rng, channels=[0, 1, 2])
+    say("[8k] MatchMap 7680x4320 RGBA vs 64x48")
+    m, _, _ = cvm.match_map(parent, sub)
+    om, ot = orc.match_rgba(parent, sub, threads=16)
+    assert np.array_equal(m.view(np.uint32), om.view(np.uint32))
+    assert cvm.match(parent, sub) == ot

@@ -313,6 +313,37 @@ def configs_leg(reps=15):
     return res
 
 
+def frames_leg(parent, sub, n=20):
+    """A stream of 4K RGBA host frames through a prepared Matcher (128x128):
+    synchronous Find per frame against the frame stream (cvm_frames_*), where
+    the upload of frame i+1 overlaps the matching of frame i."""
+    import cvmatch_amd as cvm
+
+    m = cvm.NewMatcher(sub)
+    shifted = np.ascontiguousarray(np.roll(parent, 3, axis=1))
+    fr = [parent, shifted]
+    out = {"workload": "4K RGBA host frames (two alternating), 128x128 Matcher", "frames": n}
+    m.find(fr[0])
+    m.find(fr[1])
+    t0 = time.perf_counter()
+    for i in range(n):
+        m.find(fr[i % 2])
+    out["sync_find_ms_per_frame"] = round((time.perf_counter() - t0) / n * 1e3, 3)
+    for depth in (2, 3):
+        q = cvm.FrameStream(depth)
+        q.find(m, fr[0])
+        q.result()
+        t0 = time.perf_counter()
+        for i in range(n):
+            q.find(m, fr[i % 2], tag=i)
+            if q.pending() >= depth:
+                q.result()
+        while q.pending():
+            q.result()
+        out[f"stream_depth{depth}_ms_per_frame"] = round((time.perf_counter() - t0) / n * 1e3, 3)
+    return out
+
+
 def copy_calibration(mib):
     import cvmatch_amd as cvm
 
@@ -458,6 +489,7 @@ def main():
         line["configs"] = configs_leg()
     if not args.no_host:
         line["host_buffers"] = host_leg(parent, sub)
+        line["frame_stream"] = frames_leg(parent, sub)
         line["windowrules"] = windowrules_leg()
     if not args.no_fleet:
         line["fleet"] = fleet_leg(args.fleet_frames)

@@ -109,6 +109,11 @@ SIGNATURES = {
     "cvm_fleet_find_all_device": ([_P, _P, _I64, _I32, _I32, _P, _P], C.c_int),
     "cvm_fleet_find_all_direct_device": ([_P, _P, _I64, _I32, _I32, _P, _P], C.c_int),
     "cvm_rgba_to_gray_device": ([_P, _I64, _I32, _I32, _P, _P], C.c_int),
+    "cvm_frames_new": ([_I32, C.POINTER(_P)], C.c_int),
+    "cvm_frames_find": ([_P, _P, _P, _I64, _I32, _I32, _I32, _I64], C.c_int),
+    "cvm_frames_result": ([_P, C.POINTER(_I64), _RES], C.c_int),
+    "cvm_frames_pending": ([_P], _I32),
+    "cvm_frames_free": ([_P], None),
     "cvm_set_threads": ([_I32], _I32),
     "cvm_threads": ([], _I32),
     "cvm_host_alloc": ([_SZ, C.POINTER(_P)], C.c_int),
@@ -420,6 +425,45 @@ class Fleet:
         h = getattr(self, "_h", None)
         if h is not None and h.value and _lib is not None:
             _lib.cvm_fleet_free(h)
+            self._h = None
+
+
+class FrameStream:
+    """Find over a stream of frames with the upload of each frame overlapping
+    the matching of the previous one (cvm_frames_*): ``find`` submits and
+    returns once the frame has been copied out of its buffer; ``result``
+    returns (tag, six-tuple) of the oldest submitted frame.  Each result
+    equals ``matcher.find(frame)``."""
+
+    def __init__(self, depth: int = 2):
+        h = C.c_void_p()
+        _check(_L().cvm_frames_new(depth, C.byref(h)))
+        self._h = h
+        self._keep = {}
+
+    def find(self, matcher: Matcher, frame, tag: int = 0, direct: bool = False):
+        p, ps, pw, ph, pst = view(frame)
+        if matcher.step == 1 and pst == 4:
+            p = rgba_to_gray(p)
+            ps, pst = pw, 1
+        if pst != matcher.step:
+            raise CvmatchError(CVM_E_BAD_ARGS, "cvmatch: parent format does not match the matcher")
+        _check(_L().cvm_frames_find(self._h, matcher._h, _ptr(p), ps, pw, ph, int(direct), int(tag)))
+        self._keep[id(matcher)] = matcher  # the Python object outlives its frames
+
+    def result(self):
+        tag = C.c_int64()
+        out = cvm_result()
+        _check(_L().cvm_frames_result(self._h, C.byref(tag), C.byref(out)))
+        return int(tag.value), out.tuple()
+
+    def pending(self) -> int:
+        return int(_L().cvm_frames_pending(self._h))
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h is not None and h.value and _lib is not None:
+            _lib.cvm_frames_free(h)
             self._h = None
 
 

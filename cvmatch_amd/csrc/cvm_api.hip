@@ -14,6 +14,7 @@
 #include <algorithm>
 #include <array>
 #include <atomic>
+#include <deque>
 #include <functional>
 #include <map>
 #include <memory>
@@ -1714,7 +1715,164 @@ static int matcher_find(cvm_matcher* m, const uint8_t* img, int64_t stride, int3
   return finish_result(E, s, out);
 }
 
+// ------------------------------------------------------- frame streams --
+//
+// Find over a stream of host frames with the upload of frame i + 1 under way
+// while frame i is matched: every slot owns a device frame and a pinned
+// result; uploads go through the staged transfers on the stream's own copy
+// stream, the library's stream waits for a slot's upload, matches it and
+// copies the six-tuple into the slot's pinned result.  Each frame's result
+// is the one cvm_matcher_find(_direct) returns for it.
+
+struct cvm_frames {
+  struct Slot {
+    cvm::DevBuf img;
+    cvm::Res6* res = nullptr;  // pinned
+    hipEvent_t up = nullptr, done = nullptr;
+    bool busy = false;
+  };
+  struct Entry {
+    int64_t tag;
+    int slot;  // -1 once the result is in r
+    cvm_matcher* m;  // reference held while in flight
+    cvm_result r;
+  };
+  std::mutex mu;
+  hipStream_t copy = nullptr;
+  std::vector<Slot> slots;
+  std::deque<Entry> fifo;
+};
+
+namespace cvm {
+
+// Moves the oldest in-flight entry's result out of its slot (waiting for it).
+static int frames_complete(cvm_frames* q, cvm_frames::Entry& e) {
+  if (e.slot < 0) return 0;
+  cvm_frames::Slot& sl = q->slots[e.slot];
+  HIPCHK(hipEventSynchronize(sl.done));
+  memcpy(&e.r, sl.res, sizeof(cvm_result));
+  sl.busy = false;
+  e.slot = -1;
+  matcher_release(e.m);
+  e.m = nullptr;
+  return 0;
+}
+
+}  // namespace cvm
+
 extern "C" {
+
+int cvm_frames_new(int32_t depth, cvm_frames** out) {
+  if (!out) return fail(CVM_E_BAD_ARGS, "cvmatch: null output handle");
+  if (depth < 2 || depth > 8) return fail(CVM_E_BAD_ARGS, "cvmatch: frame stream depth %d outside [2, 8]", depth);
+  Engine& E = engine();
+  {
+    std::lock_guard<std::recursive_mutex> lk(E.mu);
+    CHK(E.init());
+  }
+  std::unique_ptr<cvm_frames> q(new cvm_frames());
+  HIPCHK(hipStreamCreateWithFlags(&q->copy, hipStreamNonBlocking));
+  q->slots.resize(depth);
+  for (auto& sl : q->slots) {
+    HIPCHK(hipHostMalloc((void**)&sl.res, sizeof(Res6), hipHostMallocDefault));
+    HIPCHK(hipEventCreateWithFlags(&sl.up, hipEventDisableTiming));
+    HIPCHK(hipEventCreateWithFlags(&sl.done, hipEventDisableTiming));
+  }
+  *out = q.release();
+  return 0;
+}
+
+int cvm_frames_find(cvm_frames* q, cvm_matcher* m, const uint8_t* img, int64_t stride, int32_t w, int32_t h,
+                    int32_t direct, int64_t tag) {
+  if (!q || !m) return fail(CVM_E_BAD_ARGS, "cvmatch: null frame stream or matcher");
+  if (!img) return fail(CVM_E_NIL_IMAGE, "cvmatch: nil image");
+  if (w <= 0 || h <= 0) return fail(CVM_E_EMPTY_IMAGE, "cvmatch: empty image");
+  if (m->w > w || m->h > h) return fail(CVM_E_BAD_ARGS, "cvmatch: bad match arguments (%dx%d in %dx%d)", m->w, m->h, w, h);
+  if (stride < (int64_t)w * m->step) return fail(CVM_E_BAD_ARGS, "cvmatch: stride below width*step");
+  std::lock_guard<std::mutex> ql(q->mu);
+  cvm_frames::Entry e{tag, -1, nullptr, {}};
+  if (m->varSum == 0) {  // exactly flat template matcher.go:100-102: no GPU work
+    e.r = cvm_result{1.0f, 0, 0, 1.0f, 0, 0};
+    q->fifo.push_back(e);
+    return 0;
+  }
+  int si = -1;
+  for (int i = 0; i < (int)q->slots.size(); i++)
+    if (!q->slots[i].busy) {
+      si = i;
+      break;
+    }
+  if (si < 0) {  // every slot in flight: finish the oldest first
+    for (auto& o : q->fifo)
+      if (o.slot >= 0) {
+        si = o.slot;
+        CHK(frames_complete(q, o));
+        break;
+      }
+  }
+  cvm_frames::Slot& sl = q->slots[si];
+  Engine& E = engine();
+  std::lock_guard<std::recursive_mutex> lk(E.mu);
+  hipStream_t s = E.stream;
+  size_t ib = (size_t)img_bytes(stride, w, h, m->step);
+  CHK(sl.img.ensure(ib));
+  {
+    Stage S(E, "h2d", q->copy);
+    HIPCHK(xfer_h2d(sl.img.p, img, ib, q->copy));
+  }
+  HIPCHK(hipEventRecord(sl.up, q->copy));
+  HIPCHK(hipStreamWaitEvent(s, sl.up, 0));
+  if (direct)
+    CHK(matcher_find_direct_dev(E, s, m, sl.img.as<uint8_t>(), stride, w, h, E.res_d));
+  else
+    CHK(matcher_find_dev(E, s, m, sl.img.as<uint8_t>(), stride, w, h, E.res_d));
+  HIPCHK(hipMemcpyAsync(sl.res, E.res_d, sizeof(Res6), hipMemcpyDeviceToHost, s));
+  HIPCHK(hipEventRecord(sl.done, s));
+  sl.busy = true;
+  m->refs.fetch_add(1);
+  e.slot = si;
+  e.m = m;
+  q->fifo.push_back(e);
+  return 0;
+}
+
+int cvm_frames_result(cvm_frames* q, int64_t* tag, cvm_result* out) {
+  if (!q || !out) return fail(CVM_E_BAD_ARGS, "cvmatch: null frame stream or output");
+  std::lock_guard<std::mutex> ql(q->mu);
+  if (q->fifo.empty()) return fail(CVM_E_BAD_ARGS, "cvmatch: no frame in flight");
+  cvm_frames::Entry& e = q->fifo.front();
+  CHK(frames_complete(q, e));
+  if (tag) *tag = e.tag;
+  *out = e.r;
+  q->fifo.pop_front();
+  return 0;
+}
+
+int32_t cvm_frames_pending(cvm_frames* q) {
+  if (!q) return 0;
+  std::lock_guard<std::mutex> ql(q->mu);
+  return (int32_t)q->fifo.size();
+}
+
+void cvm_frames_free(cvm_frames* q) {
+  if (!q) return;
+  {
+    std::lock_guard<std::mutex> ql(q->mu);
+    for (auto& e : q->fifo) (void)frames_complete(q, e);
+    q->fifo.clear();
+  }
+  Engine& E = engine();
+  std::lock_guard<std::recursive_mutex> lk(E.mu);
+  if (q->copy) (void)hipStreamSynchronize(q->copy);
+  for (auto& sl : q->slots) {
+    sl.img.release();
+    if (sl.res) (void)hipHostFree(sl.res);
+    if (sl.up) (void)hipEventDestroy(sl.up);
+    if (sl.done) (void)hipEventDestroy(sl.done);
+  }
+  if (q->copy) (void)hipStreamDestroy(q->copy);
+  delete q;
+}
 
 void cvm_matcher_free(cvm_matcher* m) { matcher_release(m); }
 

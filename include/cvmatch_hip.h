@@ -108,6 +108,27 @@ int cvm_matcher_find(cvm_matcher *m, const uint8_t *img, int64_t stride, int32_t
  * from a finalizer).  A fleet keeps its own reference to its members. */
 void cvm_matcher_free(cvm_matcher *m);
 
+/* ------------------------------------------------------- frame streams -- */
+/* No reference counterpart: Find over a stream of captured frames (a
+ * screenshot loop) with the upload of one frame overlapping the matching of
+ * the previous one.  cvm_frames_new: up to depth (2..8) frames in flight.
+ * cvm_frames_find submits Find (matcher.go:83-130; direct != 0: the direct
+ * form of cvm_matcher_find_direct) of m on a host frame and returns once the
+ * frame has been copied out of the caller's buffer, which may then be
+ * reused; with depth frames already in flight it first completes the
+ * oldest.  cvm_frames_result returns the oldest submitted frame's six-tuple
+ * and its tag, waiting for it; CVM_E_BAD_ARGS when nothing is in flight.
+ * Each result equals cvm_matcher_find (or _direct) on that frame, bit for
+ * bit.  A stream holds a reference to each matcher while its frames are in
+ * flight.  One stream is used by one thread at a time. */
+typedef struct cvm_frames cvm_frames;
+int cvm_frames_new(int32_t depth, cvm_frames **out);
+int cvm_frames_find(cvm_frames *q, cvm_matcher *m, const uint8_t *img, int64_t stride, int32_t w, int32_t h,
+                    int32_t direct, int64_t tag);
+int cvm_frames_result(cvm_frames *q, int64_t *tag, cvm_result *out);
+int32_t cvm_frames_pending(cvm_frames *q);
+void cvm_frames_free(cvm_frames *q);
+
 /* --------------------------------------------------------------- Fleet -- */
 
 /* NewFleet fleet.go:85-100: members bucketed by shape dominance

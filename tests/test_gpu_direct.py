@@ -212,3 +212,19 @@ def test_delta_vs_fp64_4k_and_ui(cvm, orc, form):
             assert np.array_equal(bits(m[ys, xs]), bits(ref.astype(np.float32)))
         else:
             assert d.max() < 1e-3
+
+
+@pytest.mark.parametrize("size", [16, 24, 32, 48])
+def test_direct_1440p_full_map_fp64(cvm, orc, size):
+    """The mixed-precision configuration over the whole map: 2560x1440 gray,
+    the seeded size x size crop; every one of the ~3.5 M scores equals the
+    FP64 oracle's rounded to float32 (max |d| is float32 rounding alone)."""
+    img, subs = synth.workload_direct_1440p(sizes=(size,))
+    sub, (x, y) = subs[0]
+    say(f"[direct] 1440p gray {size}x{size}: full map vs FP64")
+    gm, gt = cvm.match_direct(img, sub, want_map=True)
+    om = orc.fp64_map(img, 2560, 2560, 1440, sub, size, size, size, 1, 1)
+    d = np.abs(gm.astype(np.float64) - om)
+    say(f"[direct] {size}x{size}: max |d| = {d.max():.3e} over {d.size} windows")
+    assert np.array_equal(bits(gm), bits(om.astype(np.float32)))
+    assert gt[4:] == (x, y) and tuple(gt) == first_extrema(om.astype(np.float32))

@@ -914,6 +914,10 @@ struct cvm_fleet {
   // job tables (device), all members' C' maps and extrema partials
   cvm::DevBuf bj_direct, bj_dwg, bj_norm, bj_nwg, bj_cmaps, bj_parts;
   int bj_pw = -1, bj_ph = -1, bj_pConst = -1, bj_nnorm = 0;
+  // the per-channel-count scratch the tables point into (the plain FindAll
+  // grows the same buffers for larger frames): tables are rebuilt when any
+  // of them has moved
+  void* bj_ps[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
   std::vector<std::array<int, 4>> bj_groups;   // normalisation: (q, sq32, first wg entry, wg count)
   std::vector<std::array<int, 5>> bj_dgroups;  // direct: (kstep, cmax, first wg entry, wg count, max pitch)
 };
@@ -1244,7 +1248,10 @@ static int fleet_find_direct_dev(Engine& E, hipStream_t s, cvm_fleet* f, const u
   for (int cn = 1; cn <= 4; cn++)
     if (used[cn]) CHK(f->dscr[cn].ensure(norm_scratch_bytes(pw, ph, 1, 1, cn)));
 
-  if (f->bj_pw != pw || f->bj_ph != ph || f->bj_pConst != pConst) {
+  bool moved = false;
+  for (int cn = 1; cn <= 4; cn++)
+    if (used[cn] && f->bj_ps[cn] != f->dscr[cn].p) moved = true;
+  if (moved || f->bj_pw != pw || f->bj_ph != ph || f->bj_pConst != pConst) {
     // job tables for this frame shape
     std::vector<DirectJob> dj;
     std::vector<NormJob> nj;
@@ -1347,6 +1354,7 @@ static int fleet_find_direct_dev(Engine& E, hipStream_t s, cvm_fleet* f, const u
     f->bj_pw = pw;
     f->bj_ph = ph;
     f->bj_pConst = pConst;
+    for (int cn = 1; cn <= 4; cn++) f->bj_ps[cn] = f->dscr[cn].p;
   }
 
   for (int cn = 1; cn <= 4; cn++) {

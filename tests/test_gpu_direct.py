@@ -228,3 +228,24 @@ def test_direct_1440p_full_map_fp64(cvm, orc, size):
     say(f"[direct] {size}x{size}: max |d| = {d.max():.3e} over {d.size} windows")
     assert np.array_equal(bits(gm), bits(om.astype(np.float32)))
     assert gt[4:] == (x, y) and tuple(gt) == first_extrema(om.astype(np.float32))
+
+
+def test_direct_fleet_tables_survive_scratch_growth(cvm, orc):
+    """The direct FindAll's cached job tables point into the fleet's
+    normalisation scratch, which a plain FindAll on a larger frame grows (and
+    so moves): the next direct FindAll on the small frame must rebuild its
+    tables and give the same results."""
+    rng = np.random.default_rng(77)
+    small = synth.flat_rect_scene(400, 300, rng, n_rects=20, channels=4)
+    small[..., 3] = 255
+    big = synth.flat_rect_scene(1600, 1000, rng, n_rects=60, channels=4)
+    big[..., 3] = 255
+    subs = [small[50:82, 60:100].copy(), small[200:224, 300:330].copy()]
+    fl = cvm.NewFleet(*[cvm.NewMatcher(s) for s in subs])
+    first = [tuple(r) for r in fl.find_all(small, direct=True)]
+    fl.find_all(big)  # the plain form grows the shared scratch
+    assert [tuple(r) for r in fl.find_all(small, direct=True)] == first
+    fl.find_all(big, direct=True)  # tables for the large frame, then back
+    assert [tuple(r) for r in fl.find_all(small, direct=True)] == first
+    for s, w in zip(subs, first):
+        assert w == first_extrema(oracle_map(orc, small, s, 3))

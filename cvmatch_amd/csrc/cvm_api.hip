@@ -886,11 +886,35 @@ struct BucketGeom {
   }
 };
 
+// The plain FindAll's normalisations, batched: members (in loop order) whose
+// correlation maps together fit the batch buffer are normalised by one
+// launch per strip form and reduced by one launch once the last of them has
+// its map.  Members past the wave form are normalised on their own.
+struct PlainNormPlan {
+  std::vector<int64_t> corr_off;  // per member: float offset in the batch buffer, -1: on its own
+  std::vector<int> last_of;       // per member: batch index it closes, -1 otherwise
+  struct Batch {
+    std::vector<std::array<int, 4>> groups;  // (q, sq32, first entry in the wg table, workgroups)
+    int job0 = 0, njobs = 0;
+  };
+  std::vector<Batch> batches;
+  DevBuf jobs, wg;
+  bool used_cn[5] = {false, false, false, false, false};
+  int prep_member[5] = {-1, -1, -1, -1, -1};  // a batched member of each channel count
+  std::array<void*, 7> ptrs{};  // batch corr, batch parts, results, scratch per cn at build time
+  bool built = false;
+  ~PlainNormPlan() {
+    jobs.release();
+    wg.release();
+  }
+};
+
 struct FleetGeom {
   int pw = 0, ph = 0, pConst = 0;
   std::vector<std::unique_ptr<BucketGeom>> bg;
   CallState* states = nullptr;  // per member
   CallState* frame_state = nullptr;
+  PlainNormPlan norm;
   ~FleetGeom() {
     if (states) (void)hipFree(states);
     if (frame_state) (void)hipFree(frame_state);
@@ -906,6 +930,8 @@ struct cvm_fleet {
   std::mutex mu;
   std::shared_ptr<cvm::FleetGeom> geom;
   cvm::DevBuf frame, mspec, corr, res;
+  // plain FindAll: correlation maps and extrema partials of one batch
+  cvm::DevBuf pcorr, pparts;
   // direct form: per-member call states for the parent alpha key dpConst,
   // and the normalisation partials per channel count
   cvm::DevBuf dstates, dscr[5];
@@ -995,6 +1021,118 @@ static int fleet_build_geom(Engine& E, hipStream_t s, cvm_fleet* f, int pw, int 
   return 0;
 }
 
+// The plain FindAll's batched normalisation plan for geometry g: built on
+// first use, rebuilt when a buffer its job tables point into has moved.
+static int plain_norm_plan(Engine& E, hipStream_t s, cvm_fleet* f, FleetGeom& g, int pw, int ph, Res6* d_res) {
+  PlainNormPlan& P = g.norm;
+  constexpr size_t kBatchFloats = (size_t)48 << 20;  // 192 MiB of correlation maps per batch
+  const size_t n = f->ms.size();
+  struct Item {
+    int mi, cn, q, sq, rw, rh;
+    int64_t off;
+  };
+  std::vector<std::vector<Item>> bat;
+  std::vector<int> order, cnOf(n, 0);
+  for (size_t bi = 0; bi < f->buckets.size(); bi++)
+    for (size_t j = 0; j < f->buckets[bi].members.size(); j++) {
+      int mi = f->buckets[bi].members[j];
+      order.push_back(mi);
+      cnOf[mi] = g.bg[bi]->cns[j];
+    }
+  size_t maxMap = 0;
+  for (int mi : order)
+    maxMap = std::max(maxMap, (size_t)(pw - f->ms[mi]->w + 1) * (size_t)(ph - f->ms[mi]->h + 1));
+  const size_t capF = std::max(kBatchFloats, (maxMap + 63) / 64 * 64);
+  std::vector<int64_t> off(n, -1);
+  std::vector<int> last(n, -1);
+  bool usedCn[5] = {false, false, false, false, false};
+  int prepM[5] = {-1, -1, -1, -1, -1};
+  size_t fill = 0;
+  for (int mi : order) {
+    cvm_matcher* m = f->ms[mi];
+    const int cn = cnOf[mi];
+    if (!norm_wave_ok(m->w, m->h, cn)) continue;  // normalised on its own (int64 form)
+    Item it{mi, cn, 0, 0, pw - m->w + 1, ph - m->h + 1, 0};
+    norm_form(m->w, m->h, cn, &it.q, &it.sq);
+    const size_t sz = ((size_t)it.rw * it.rh + 63) / 64 * 64;
+    if (bat.empty() || fill + sz > capF) {
+      bat.emplace_back();
+      fill = 0;
+    }
+    it.off = (int64_t)fill;
+    off[mi] = it.off;
+    fill += sz;
+    bat.back().push_back(it);
+    usedCn[cn] = true;
+    if (prepM[cn] < 0) prepM[cn] = mi;
+  }
+  size_t maxParts = 1;
+  for (auto& b : bat) {
+    size_t t = 0;
+    for (auto& it : b) t += (size_t)norm_grid_x(it.rw, it.q) * norm_grid_y(it.rh);
+    maxParts = std::max(maxParts, t);
+  }
+  CHK(f->pcorr.ensure(sizeof(float) * capF));
+  CHK(f->pparts.ensure(sizeof(Extrema) * maxParts));
+  for (int cn = 1; cn <= 4; cn++)
+    if (usedCn[cn]) CHK(f->dscr[cn].ensure(norm_scratch_bytes(pw, ph, 1, 1, cn)));
+  const std::array<void*, 7> now = {f->pcorr.p, f->pparts.p, (void*)d_res, f->dscr[1].p, f->dscr[2].p,
+                                    f->dscr[3].p, f->dscr[4].p};
+  if (P.built && P.ptrs == now) return 0;
+  std::vector<NormJob> jobs;
+  std::vector<int> wgs;
+  P.batches.clear();
+  for (size_t b = 0; b < bat.size(); b++) {
+    PlainNormPlan::Batch B;
+    B.job0 = (int)jobs.size();
+    size_t poff = 0;
+    for (int q : {4, 2})
+      for (int sq : {1, 0}) {
+        const int first = (int)wgs.size();
+        for (auto& it : bat[b]) {
+          if (it.q != q || it.sq != sq) continue;
+          cvm_matcher* m = f->ms[it.mi];
+          NormJob J{};
+          J.st = g.states + it.mi;
+          J.ps = f->dscr[it.cn].as<uint32_t>();
+          J.corr = f->pcorr.as<float>() + it.off;
+          J.parts = f->pparts.as<Extrema>() + poff;
+          J.res = d_res + it.mi;
+          J.tw = m->w;
+          J.th = m->h;
+          J.rw = it.rw;
+          J.rh = it.rh;
+          J.gx = norm_grid_x(it.rw, q);
+          J.gy = norm_grid_y(it.rh);
+          J.wg0 = (int)wgs.size() - first;
+          poff += (size_t)J.gx * J.gy;
+          for (int w = 0; w < J.gx * J.gy; w++) wgs.push_back((int)jobs.size());
+          jobs.push_back(J);
+        }
+        if ((int)wgs.size() > first) B.groups.push_back({q, sq, first, (int)wgs.size() - first});
+      }
+    B.njobs = (int)jobs.size() - B.job0;
+    P.batches.push_back(B);
+    last[bat[b].back().mi] = (int)b;
+  }
+  if (!jobs.empty()) {
+    CHK(P.jobs.ensure(sizeof(NormJob) * jobs.size()));
+    CHK(P.wg.ensure(sizeof(int) * wgs.size()));
+    HIPCHK(hipMemcpyAsync(P.jobs.p, jobs.data(), sizeof(NormJob) * jobs.size(), hipMemcpyHostToDevice, s));
+    HIPCHK(hipMemcpyAsync(P.wg.p, wgs.data(), sizeof(int) * wgs.size(), hipMemcpyHostToDevice, s));
+    HIPCHK(hipStreamSynchronize(s));  // the host vectors go out of scope
+  }
+  P.corr_off = off;
+  P.last_of = last;
+  for (int cn = 0; cn < 5; cn++) {
+    P.used_cn[cn] = usedCn[cn];
+    P.prep_member[cn] = prepM[cn];
+  }
+  P.ptrs = now;
+  P.built = true;
+  return 0;
+}
+
 // FindAll fleet.go:176-332 on a device-resident parent.
 static int fleet_find_dev(Engine& E, hipStream_t s, cvm_fleet* f, const uint8_t* d_img, int64_t istride, int pw,
                           int ph, int pConst, Res6* d_res, std::vector<int>& flat_members) {
@@ -1010,10 +1148,18 @@ static int fleet_find_dev(Engine& E, hipStream_t s, cvm_fleet* f, const uint8_t*
     f->geom = ng;
     g = ng;
   }
-  // The normalisation's chunk prefix sums depend on the frame and the
-  // channel count only: built once per channel count (from the first
-  // non-flat member that takes the wave form) and shared by every member.
-  void* prep[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
+  // Batched normalisations (members taking the wave form): the chunk prefix
+  // sums depend on the frame and the channel count only, built once per
+  // channel count before the members' tails.
+  CHK(plain_norm_plan(E, s, f, *g, pw, ph, d_res));
+  const PlainNormPlan& NP = g->norm;
+  for (int cn = 1; cn <= 4; cn++) {
+    if (!NP.used_cn[cn]) continue;
+    const int mi = NP.prep_member[cn];
+    Stage S(E, "norm_prep", s);
+    launch_norm_prep(d_img, istride, pw, ph, f->step, f->ms[mi]->w, f->ms[mi]->h, cn, g->states + mi,
+                     f->dscr[cn].p, s);
+  }
   for (size_t bi = 0; bi < f->buckets.size(); bi++) {
     const FleetBucket& b = f->buckets[bi];
     BucketGeom& bg = *g->bg[bi];
@@ -1066,8 +1212,14 @@ static int fleet_find_dev(Engine& E, hipStream_t s, cvm_fleet* f, const uint8_t*
       gM.extH = rh;
       const TplSpec& ts = bg.ts[j];
       int64_t tchan2 = (int64_t)bg.dh2 * bg.hwPad;
-      CHK(f->corr.ensure(sizeof(float) * (size_t)rw * rh));
-      float* corr = f->corr.as<float>();
+      const bool batched = NP.corr_off[mi] >= 0;
+      float* corr;
+      if (batched) {
+        corr = f->pcorr.as<float>() + NP.corr_off[mi];
+      } else {
+        CHK(f->corr.ensure(sizeof(float) * (size_t)rw * rh));
+        corr = f->corr.as<float>();
+      }
       {
         Stage S(E, "fleet_cols", s);
         launch_cols(COL_MULINV, frame, mspec, ftile, ftile, chan, cn, mst, gM, pt, ts.spec, ts.spec2, chan, tchan2,
@@ -1077,20 +1229,21 @@ static int fleet_find_dev(Engine& E, hipStream_t s, cvm_fleet* f, const uint8_t*
         Stage S(E, "fleet_row_inv", s);
         launch_row_inv(mspec, ftile, chan, mst, cn, gM, pt, corr, rw, ntiles, s);
       }
-      int nparts = norm_partials_count(rw, rh, m->w, m->h, cn);
-      CHK(E.parts.ensure(sizeof(Extrema) * (size_t)nparts));
-      if (!prep[cn] && m->varSum != 0 && norm_wave_ok(m->w, m->h, cn)) {
-        // sized as the direct form sizes it, so neither form reallocates it
-        CHK(f->dscr[cn].ensure(norm_scratch_bytes(pw, ph, 1, 1, cn)));
-        Stage S(E, "norm_prep", s);
-        launch_norm_prep(d_img, istride, pw, ph, f->step, m->w, m->h, cn, mst, f->dscr[cn].p, s);
-        prep[cn] = f->dscr[cn].p;
-      }
-      {
+      if (!batched) {  // past the wave form: the int64 normalisation, on its own
+        int nparts = norm_partials_count(rw, rh, m->w, m->h, cn);
+        CHK(E.parts.ensure(sizeof(Extrema) * (size_t)nparts));
         Stage S(E, "fleet_normalize", s);
         launch_normalize(d_img, istride, pw, ph, f->step, m->w, m->h, rw, rh, cn, mst, corr, nullptr,
-                         E.parts.as<Extrema>(), prep[cn], s);
+                         E.parts.as<Extrema>(), nullptr, s);
         launch_reduce(E.parts.as<Extrema>(), nparts, mst, rw, d_res + mi, nullptr, rh, s);
+      }
+      if (NP.last_of[mi] >= 0) {  // this member's map completes a batch
+        const PlainNormPlan::Batch& B = NP.batches[NP.last_of[mi]];
+        Stage S(E, "fleet_normalize", s);
+        for (auto& gq : B.groups)
+          launch_norm_batch(d_img, istride, pw, ph, f->step, gq[0], gq[1], false, NP.jobs.as<NormJob>(),
+                            NP.wg.as<int>() + gq[2], gq[3], s);
+        launch_reduce_batch(NP.jobs.as<NormJob>() + B.job0, B.njobs, s);
       }
     }
   }
@@ -1996,6 +2149,8 @@ void cvm_fleet_free(cvm_fleet* f) {
     f->mspec.release();
     f->corr.release();
     f->res.release();
+    f->pcorr.release();
+    f->pparts.release();
   }
   for (auto* m : f->ms) matcher_release(m);
   delete f;

@@ -298,6 +298,69 @@ def fp64_points(img, istride, iw, ih, tpl, tstride, tw, th, cn, step, xs, ys) ->
     return out
 
 
+def _window_sums(a: np.ndarray, th: int, tw: int) -> np.ndarray:
+    """Exact int64 sums of every th x tw window of a (h, w) int64 array."""
+    c = np.zeros((a.shape[0] + 1, a.shape[1] + 1), dtype=np.int64)
+    np.cumsum(np.cumsum(a, axis=0), axis=1, out=c[1:, 1:])
+    return c[th:, tw:] - c[:-th, tw:] - c[th:, :-tw] + c[:-th, :-tw]
+
+
+def fp64_map_fft(parent: np.ndarray, sub: np.ndarray, cn: int) -> np.ndarray:
+    """The FP64 oracle's full map (same values, bit for bit, as fp64_map) at
+    sizes the brute force cannot reach: the window x template sums are the
+    exact integers, obtained by rounding a float64 FFT correlation whose
+    rounding residue is asserted to stay below 1/8 (the products are < 2^32,
+    so float64 leaves a margin of many orders); window sums come from exact
+    int64 integral images; the score is the C oracle's fp64_score, op for op
+    (the TM_CCOEFF_NORMED guard ladder of cvmatch_test.go:57-76).
+    parent / sub: (h, w) gray (cn 1) or (h, w, 4) RGBA (first cn channels)."""
+    p = parent if parent.ndim == 3 else parent[..., None]
+    t = sub if sub.ndim == 3 else sub[..., None]
+    ih, iw = p.shape[:2]
+    th, tw = t.shape[:2]
+    rh, rw = ih - th + 1, iw - tw + 1
+    area = tw * th
+    spec = None
+    s = []
+    s2 = np.zeros((rh, rw), dtype=np.int64)
+    tsum = []
+    for k in range(cn):
+        pk = p[..., k].astype(np.int64)
+        tk = t[..., k].astype(np.int64)
+        fp = np.fft.rfft2(pk.astype(np.float64))
+        ft = np.fft.rfft2(tk.astype(np.float64), s=(ih, iw))
+        spec = fp * np.conj(ft) if spec is None else spec + fp * np.conj(ft)
+        s.append(_window_sums(pk, th, tw))
+        s2 += _window_sums(pk * pk, th, tw)
+        tsum.append(int(tk.sum()))
+    cf = np.fft.irfft2(spec, s=(ih, iw))[:rh, :rw]
+    corr = np.rint(cf)
+    resid = float(np.max(np.abs(cf - corr))) if corr.size else 0.0
+    assert resid < 0.125, f"float64 correlation residue {resid} too close to 1/2 for exact rounding"
+    corr = corr.astype(np.int64)
+    tt = [t[..., k].astype(np.int64) for k in range(cn)]
+    varSum = sum(area * int((x * x).sum()) - int(x.sum()) ** 2 for x in tt)
+    if varSum == 0:
+        return np.ones((rh, rw), dtype=np.float64)
+    cross = np.zeros((rh, rw), dtype=np.int64)
+    sq = np.zeros((rh, rw), dtype=np.int64)
+    for k in range(cn):
+        cross += s[k] * tsum[k]
+        sq += s[k] * s[k]
+    numI = area * corr - cross
+    idiff = area * s2 - sq
+    diff2 = idiff.astype(np.float64) / float(area)
+    lim = np.minimum(10.0 * 1.1920928955078125e-07 * s2.astype(np.float64), 0.5)
+    denI = np.sqrt(idiff.astype(np.float64)) * np.sqrt(float(varSum))
+    nf = numI.astype(np.float64)
+    an = np.abs(nf)
+    with np.errstate(divide="ignore", invalid="ignore"):
+        q = nf / denI
+    out = np.where(an < denI, q, np.where(an < denI * 1.125, np.where(numI > 0, 1.0, -1.0), 0.0))
+    out[~(diff2 > lim)] = 0.0
+    return out
+
+
 def golden_hash(res: np.ndarray, tup) -> int:
     res = np.ascontiguousarray(res, dtype=np.float32).ravel()
     r = OrcResult(*tup)

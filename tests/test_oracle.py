@@ -13,6 +13,7 @@ What fixes each part of the oracle:
   plumbing         gray formula, alpha predicate, cn3 == cn4 on constant alpha
   Matcher / Fleet  Find == Match bitwise, single-member fleet == Find bitwise,
                    bucketize cases (fleet_internal_test.go:27-51)
+  FP64 FFT route   fp64_map_fft == the brute-force FP64 map bit for bit
 Inputs come from numpy's PCG64, not Go's math/rand, so golden hashes
 #101-#109 cannot be reproduced here (their inputs need Go's generator).
 """
@@ -417,3 +418,52 @@ def test_fleet_matches_solo(orc):
     # order independence: reversing the members reverses the answers
     rev = orc.fleet_find_all(desk, subs[::-1])
     assert rev[::-1] == got
+
+
+@pytest.mark.parametrize("case", [
+    # (iw, ih, tw, th, cn, planted)
+    (97, 61, 17, 13, 1, False),
+    (80, 50, 9, 31, 3, True),
+    (64, 48, 16, 16, 4, True),
+    (40, 30, 40, 30, 4, False),
+    (33, 20, 1, 1, 1, False),
+    (70, 45, 12, 10, 3, False),
+])
+def test_fp64_map_fft_equals_brute_force(orc, case):
+    """The FFT-route FP64 map equals the brute-force FP64 map bit for bit:
+    random bytes, a planted exact crop (the +-1 band), flat regions in the
+    parent (den = 0) and a flat template (all ones)."""
+    iw, ih, tw, th, cn, planted = case
+    rng = np.random.default_rng(iw * 3 + th)
+    ch = 1 if cn == 1 else 4
+    parent = rng.integers(0, 256, (ih, iw, ch), dtype=np.uint8)
+    parent[: ih // 3, : iw // 2] = 77  # flat block: windows inside it have den = 0
+    if ch == 4 and cn == 3:
+        parent[..., 3] = 255
+    sub = rng.integers(0, 256, (th, tw, ch), dtype=np.uint8)
+    if planted:
+        sub = parent[ih - th:, iw - tw:].copy()
+    if ch == 4 and cn == 3:
+        sub[..., 3] = 255
+    p2 = parent[..., 0] if ch == 1 else parent
+    s2 = sub[..., 0] if ch == 1 else sub
+    want = orc.fp64_map(np.ascontiguousarray(p2), iw * ch, iw, ih, np.ascontiguousarray(s2), tw * ch, tw, th, cn, ch)
+    got = orc.fp64_map_fft(p2, s2, cn)
+    assert got.shape == want.shape
+    assert np.array_equal(got.view(np.uint64), want.view(np.uint64))
+    flat = np.full_like(s2, 9)
+    assert np.all(orc.fp64_map_fft(p2, flat, cn) == 1.0)
+
+
+def test_fp64_map_fft_at_frame_scale(orc):
+    """A 2560x1440 RGBA frame against a 128x128 template (products near
+    2^31): the FFT route still rounds to the exact integers (its residue
+    assertion) and matches the brute-force oracle at seeded windows."""
+    rng = np.random.default_rng(77)
+    parent = rng.integers(200, 256, (1440, 2560, 4), dtype=np.uint8)
+    sub = rng.integers(200, 256, (128, 128, 4), dtype=np.uint8)
+    got = orc.fp64_map_fft(parent, sub, 4)
+    xs = rng.integers(0, got.shape[1], 64).astype(np.int32)
+    ys = rng.integers(0, got.shape[0], 64).astype(np.int32)
+    want = orc.fp64_points(parent, 2560 * 4, 2560, 1440, sub, 128 * 4, 128, 128, 4, 4, xs, ys)
+    assert np.array_equal(got[ys, xs].view(np.uint64), want.view(np.uint64))

@@ -513,14 +513,15 @@ __device__ __forceinline__ void col_mid(float2* a, const float2* __restrict__ tr
 // pass writes spatial rows straight to the spectrum.  The template spectrum
 // values of the last forward pass are loaded before the first pass starts.
 // Logical workgroup order is tile-fastest, so the tiles that read one
-// template-spectrum slab run together on one XCD.  One launch covers tiles
-// [t0, t0 + ntiles), all of column height 2^LOG2N.
+// template-spectrum slab run together on one XCD.  A body covers tiles
+// [t0, t0 + ntiles), all of column height 2^LOG2N; L is its logical
+// workgroup index.
 template <int LOG2N, int MODE>
-__global__ __launch_bounds__(256) void k_cols(const float2* __restrict__ src, float2* __restrict__ dst,
-                                              int64_t srcTileStride, int64_t dstTileStride, int64_t chanStride,
-                                              const CallState* __restrict__ st, TileGrid g,
-                                              const float2* __restrict__ tri, const float2* __restrict__ tspec,
-                                              int64_t tchanStride, float scale, int t0, int ntiles, int nslabs) {
+__device__ __forceinline__ void cols_body(float2* a, int L, const float2* __restrict__ src,
+                                          float2* __restrict__ dst, int64_t srcTileStride, int64_t dstTileStride,
+                                          int64_t chanStride, const CallState* __restrict__ st, const TileGrid& g,
+                                          const float2* __restrict__ tri, const float2* __restrict__ tspec,
+                                          int64_t tchanStride, float scale, int t0, int ntiles, int nslabs) {
   constexpr int n = 1 << LOG2N, CG = ColCG<LOG2N>::value, LCG = ceil_log2c(CG), NT = 256;
   using F = ColFirst<LOG2N>;
   constexpr int NB1 = F::NB, E1 = 1 << NB1;
@@ -531,8 +532,6 @@ __global__ __launch_bounds__(256) void k_cols(const float2* __restrict__ src, fl
   constexpr int NSTL = SINGLE ? F::NST : (LOG2N - LOL) / 2;
   constexpr int NBL = HEADL + 2 * NSTL, EL = 1 << NBL;
   constexpr int GL = CG << (LOG2N - NBL), ITL = (GL + NT - 1) / NT;
-  extern __shared__ float2 a[];
-  const int L = xcd_remap(blockIdx.x, gridDim.x);
   const int t = t0 + L % ntiles, rest = L / ntiles;
   const int slab = rest % nslabs, k = rest / nslabs;
   const int c0 = slab * CG;
@@ -696,6 +695,46 @@ __global__ __launch_bounds__(256) void k_cols(const float2* __restrict__ src, fl
   }
 }
 
+// One band of tiles of a column launch: its twiddles, template spectrum and
+// tile range, and the workgroups given to it (a multiple of 8 when another
+// band follows it in the launch, so the next band's first workgroup starts
+// on XCD 0; workgroups past the band's real count find no channel and
+// return).
+struct ColBand {
+  const float2* tri;
+  const float2* tspec;
+  int64_t tchanStride;
+  int t0, ntiles, nwg;
+};
+
+template <int LOG2N, int MODE>
+__global__ __launch_bounds__(256) void k_cols(const float2* __restrict__ src, float2* __restrict__ dst,
+                                              int64_t srcTileStride, int64_t dstTileStride, int64_t chanStride,
+                                              const CallState* __restrict__ st, TileGrid g, float scale, int nslabs,
+                                              ColBand b) {
+  extern __shared__ float2 a[];
+  cols_body<LOG2N, MODE>(a, xcd_remap(blockIdx.x, gridDim.x), src, dst, srcTileStride, dstTileStride, chanStride,
+                         st, g, b.tri, b.tspec, b.tchanStride, scale, b.t0, b.ntiles, nslabs);
+}
+
+// Full-height tiles and the shrunk last band (column height 2^LOG2N2) in
+// one launch: the band's workgroups come after every full-height one, spread
+// over all XCDs, and fill the tail of the full-height work.
+template <int LOG2N, int LOG2N2, int MODE>
+__global__ __launch_bounds__(256) void k_cols2(const float2* __restrict__ src, float2* __restrict__ dst,
+                                               int64_t srcTileStride, int64_t dstTileStride, int64_t chanStride,
+                                               const CallState* __restrict__ st, TileGrid g, float scale, int nslabs,
+                                               ColBand b1, ColBand b2) {
+  extern __shared__ float2 a[];
+  const int p = blockIdx.x;
+  if (p < b1.nwg)
+    cols_body<LOG2N, MODE>(a, xcd_remap(p, b1.nwg), src, dst, srcTileStride, dstTileStride, chanStride, st, g,
+                           b1.tri, b1.tspec, b1.tchanStride, scale, b1.t0, b1.ntiles, nslabs);
+  else
+    cols_body<LOG2N2, MODE>(a, xcd_remap(p - b1.nwg, b2.nwg), src, dst, srcTileStride, dstTileStride, chanStride,
+                            st, g, b2.tri, b2.tspec, b2.tchanStride, scale, b2.t0, b2.ntiles, nslabs);
+}
+
 template <int LOG2N>
 static void cols_n(ColMode mode, const float2* src, float2* dst, int64_t srcTileStride, int64_t dstTileStride,
                    int64_t chanStride, int cnMax, const CallState* st, const TileGrid& g, const float2* tri,
@@ -704,9 +743,10 @@ static void cols_n(ColMode mode, const float2* src, float2* dst, int64_t srcTile
   size_t lds = sizeof(float2) * (size_t)(1 << LOG2N) * CG;
   int nslabs = (g.hw + CG - 1) / CG;
   dim3 grid(nslabs * ntiles * cnMax), block(256);
+  ColBand b{tri, tspec, tchanStride, t0, ntiles, (int)grid.x};
 #define CVM_CM(M)                                                                                                \
   hipLaunchKernelGGL((k_cols<LOG2N, M>), grid, block, lds, s, src, dst, srcTileStride, dstTileStride, chanStride, \
-                     st, g, tri, tspec, tchanStride, scale, t0, ntiles, nslabs)
+                     st, g, scale, nslabs, b)
   switch (mode) {
     case COL_TSPEC: CVM_CM(COL_TSPEC); break;
     case COL_FWD: CVM_CM(COL_FWD); break;
@@ -714,6 +754,30 @@ static void cols_n(ColMode mode, const float2* src, float2* dst, int64_t srcTile
     case COL_MULINV: CVM_CM(COL_MULINV); break;
   }
 #undef CVM_CM
+}
+
+// Both bands in one launch (the hot modes; shrunk band one or two sizes
+// below the full height).  Returns false when no such kernel exists.
+template <int LOG2N, int LOG2N2>
+static bool cols2_n(ColMode mode, const float2* src, float2* dst, int64_t srcTileStride, int64_t dstTileStride,
+                    int64_t chanStride, int cnMax, const CallState* st, const TileGrid& g, ColBand b1, ColBand b2,
+                    float scale, hipStream_t s) {
+  constexpr int CG = ColCG<LOG2N>::value;
+  static_assert(ColCG<LOG2N2>::value == CG, "both bands share the column slabs");
+  size_t lds = sizeof(float2) * (size_t)(1 << LOG2N) * CG;
+  int nslabs = (g.hw + CG - 1) / CG;
+  b1.nwg = (nslabs * b1.ntiles * cnMax + 7) & ~7;
+  b2.nwg = nslabs * b2.ntiles * cnMax;
+  dim3 grid(b1.nwg + b2.nwg), block(256);
+  if (mode == COL_CORR)
+    hipLaunchKernelGGL((k_cols2<LOG2N, LOG2N2, COL_CORR>), grid, block, lds, s, src, dst, srcTileStride,
+                       dstTileStride, chanStride, st, g, scale, nslabs, b1, b2);
+  else if (mode == COL_MULINV)
+    hipLaunchKernelGGL((k_cols2<LOG2N, LOG2N2, COL_MULINV>), grid, block, lds, s, src, dst, srcTileStride,
+                       dstTileStride, chanStride, st, g, scale, nslabs, b1, b2);
+  else
+    return false;
+  return true;
 }
 
 static void cols_dispatch(int log2n, ColMode mode, const float2* src, float2* dst, int64_t srcTileStride,
@@ -741,6 +805,17 @@ void launch_cols(ColMode mode, const float2* src, float2* dst, int64_t srcTileSt
   int nshr = g.dftH2 ? g.ntx : 0;
   if (nshr > ntiles) nshr = 0;
   int nfull = ntiles - nshr;
+  if (nfull > 0 && nshr > 0 && (mode == COL_CORR || mode == COL_MULINV)) {
+    ColBand b1{pt.triH, tspec, tchanStride, 0, nfull, 0}, b2{pt.triH2, tspec2, tchanStride2, nfull, nshr, 0};
+    bool done = false;
+#define CVM_C2(L, L2)                                                                                             \
+  if (g.log2H == L && g.log2H2 == L2)                                                                             \
+  done = cols2_n<L, L2>(mode, src, dst, srcTileStride, dstTileStride, chanStride, cnMax, st, g, b1, b2, scale, s)
+    CVM_C2(9, 8); CVM_C2(9, 7); CVM_C2(8, 7); CVM_C2(8, 6); CVM_C2(7, 6); CVM_C2(7, 5); CVM_C2(6, 5);
+    CVM_C2(6, 4);
+#undef CVM_C2
+    if (done) return;
+  }
   if (nfull > 0)
     cols_dispatch(g.log2H, mode, src, dst, srcTileStride, dstTileStride, chanStride, cnMax, st, g, pt.triH, tspec,
                   tchanStride, scale, 0, nfull, s);

@@ -360,23 +360,26 @@ __global__ __launch_bounds__(RowNTF<LOG2N>::value) void k_row_fwd(
   const int r = 2 * rp;
   if (r >= loadH || r >= dftH) return;
   const int tid = threadIdx.x;
-  const uint8_t* ra = img + (int64_t)(y0 + r) * istride + (int64_t)x0 * step + k0;
+  const uint8_t* rpx = img + (int64_t)(y0 + r) * istride + (int64_t)x0 * step;
+  const uint8_t* ra = rpx + k0;
   const bool two = r + 1 < loadH;
   // all of this thread's pixels are loaded before any is stored
   constexpr int MAXI = (n + NT - 1) / NT;
   uint32_t wa[MAXI], wb[MAXI];
-  if (NVEC == 4 && words && k0 == 0) {
-    // four-byte pixels, one dword load each (bytes of channels >= nv are
+  if (words) {
+    // four-byte pixels, one dword load each, shifted so the group's first
+    // channel sits in the low byte (bytes of channels past the group are
     // never used)
     const int64_t rb = two ? istride : 0;
+    const int sh = 8 * k0;
 #pragma unroll
     for (int i = 0; i < MAXI; i++) {
       int x = tid + i * NT;
       wa[i] = wb[i] = 0;
       if (x < loadW) {
-        const uint8_t* pa = ra + (int64_t)x * 4;
-        wa[i] = *(const uint32_t*)pa;
-        wb[i] = *(const uint32_t*)(pa + rb);
+        const uint8_t* pa = rpx + (int64_t)x * 4;
+        wa[i] = *(const uint32_t*)pa >> sh;
+        wb[i] = *(const uint32_t*)(pa + rb) >> sh;
       }
     }
     if (!two)

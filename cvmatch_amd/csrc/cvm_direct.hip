@@ -31,6 +31,10 @@ namespace cvm {
 
 typedef int dv4i __attribute__((ext_vector_type(4)));
 typedef int dv16i __attribute__((ext_vector_type(16)));
+// B fragments are read through a global-address-space pointer: a job table
+// hands the kernel generic pointers, and generic (flat) loads are counted
+// with the LDS reads, so every wait for an A fragment would drain them too.
+typedef const dv4i __attribute__((address_space(1)))* gdv4p;
 
 constexpr int DIR_T = 4;      // column tiles per wave (128 output columns)
 constexpr int DIR_WAVES = 2;  // waves per workgroup, 32 output rows each
@@ -200,7 +204,7 @@ __device__ __forceinline__ void direct_block(uint8_t* ring, const uint8_t* __res
   // are unconditional (A chunk indices clamp to the staged row, B runs into
   // the padding), so the compiler's wait counts are exact.
   constexpr int NA = CMAX + T - 1;
-  const dv4i* bq = (const dv4i*)bf + lane;
+  const gdv4p bq = (gdv4p)(const dv4i*)bf + lane;
   int cls[KSTEP], base[KSTEP];
   {
     int acc_s = 0;
@@ -218,7 +222,7 @@ __device__ __forceinline__ void direct_block(uint8_t* ring, const uint8_t* __res
     for (int j = 0; j < NA; j++) buf[j] = *(const dv4i*)(arow + 32 * min(rc + KSTEP * j, lastChunk));
   };
   auto loadB = [&](dv4i* buf, int i, int rc) {
-    const dv4i* p = bq + ((int64_t)i * nk + base[rc]) * 64;
+    const gdv4p p = bq + ((int64_t)i * nk + base[rc]) * 64;
 #pragma unroll
     for (int j = 0; j < CMAX; j++) buf[j] = p[(int64_t)j * 64];
   };
@@ -230,9 +234,10 @@ __device__ __forceinline__ void direct_block(uint8_t* ring, const uint8_t* __res
         for (int t = 0; t < T; t++) acc[t] = __builtin_amdgcn_mfma_i32_32x32x32_i8(a[j + t], bb[j], acc[t], 0, 0, 0);
   };
 
-  // Staged row DIR_ROWS + i (first read in step i + 1) is fetched in step
-  // i - DIR_PF + 1 into a register queue and written to its ring slot at the
-  // start of step i; the barrier after step i's first class makes it visible
+  // Staged row DIR_ROWS + i (first read in step i + 1) is fetched at the end
+  // of step i - DIR_PF (before the loop for i < DIR_PF) into a register
+  // queue and written to its ring slot at the start of step i; the barrier
+  // after step i's first class makes it visible
   // before step i prefetches step i + 1's first class.  Its slot held row
   // i + DIR_ROWS - DIR_RING, last read before the barrier of the step after
   // that row's step.  Unit nc[p] of each staged row is this thread's (units
@@ -255,11 +260,13 @@ __device__ __forceinline__ void direct_block(uint8_t* ring, const uint8_t* __res
     loadA(AN, arow0, 0);
     loadB(BN, 0, 0);
   }
+#pragma unroll
+  for (int p = 0; p < 2; p++) rq[DIR_PF - 1][p] = fetch(DIR_ROWS + DIR_PF - 1, ncl[p], rok[DIR_PF - 1][p]);
   for (int i = 0; i < th; i++) {
     const bool more = i + 1 < th;
-    // queue: rq[d] holds staged row DIR_ROWS + i + d
-#pragma unroll
-    for (int p = 0; p < 2; p++) rq[DIR_PF - 1][p] = fetch(DIR_ROWS + i + DIR_PF - 1, ncl[p], rok[DIR_PF - 1][p]);
+    // queue: rq[d] holds staged row DIR_ROWS + i + d; the newest one was
+    // fetched at the end of the previous step, so the shift below never
+    // waits on a load issued in this step
     if (more) {
       const int ns = (DIR_ROWS + i) % DIR_RING;
 #pragma unroll
@@ -301,6 +308,8 @@ __device__ __forceinline__ void direct_block(uint8_t* ring, const uint8_t* __res
       mma(ca, cb, cls[rc]);
       if (rc == 0 && KSTEP > 1) __syncthreads();
     }
+#pragma unroll
+    for (int p = 0; p < 2; p++) rq[DIR_PF - 1][p] = fetch(DIR_ROWS + i + DIR_PF, ncl[p], rok[DIR_PF - 1][p]);
   }
 
   const int yb = Y0 + 32 * wave + 4 * half;

@@ -199,10 +199,15 @@ __device__ __forceinline__ void direct_block(uint8_t* ring, const uint8_t* __res
   const int half = lane >> 5, m = lane & 31;
   // Per template row, K chunk residue class rc has c_rc = ceil((nk - rc) / KSTEP)
   // <= CMAX chunks; its c_rc + T - 1 A fragments and c_rc B fragments sit in
-  // statically indexed registers, the next class's loads issued before this
-  // class's MFMAs, the next row's first B class before the barrier.  Loads
-  // are unconditional (A chunk indices clamp to the staged row, B runs into
-  // the padding), so the compiler's wait counts are exact.
+  // statically indexed registers.  A fragments (LDS) are loaded one class
+  // ahead.  B fragments (global, L2) have one buffer per class when KSTEP > 1:
+  // each class loads the B of the class KSTEP - 1 ahead (the next row's when
+  // past this row) into the buffer the previous class has just used, so a B
+  // load has KSTEP - 1 classes of MFMAs to arrive; gray rows (one class) keep
+  // the next row's B in a second buffer loaded at the row start.  Loads are
+  // unconditional (A chunk indices clamp to the staged row, B rows clamp to
+  // the template and run into the padding), so the compiler's wait counts
+  // are exact.
   constexpr int NA = CMAX + T - 1;
   const gdv4p bq = (gdv4p)(const dv4i*)bf + lane;
   int cls[KSTEP], base[KSTEP];
@@ -215,7 +220,8 @@ __device__ __forceinline__ void direct_block(uint8_t* ring, const uint8_t* __res
       acc_s += cls[rc];
     }
   }
-  dv4i A0[NA], A1[NA], B0[CMAX], B1[CMAX], BN[CMAX];
+  constexpr int NBUF = KSTEP == 1 ? 2 : KSTEP;
+  dv4i A0[NA], A1[NA], Bk[NBUF][CMAX];
   const int lastChunk = nk + KSTEP * (T - 1) - 1;
   auto loadA = [&](dv4i* buf, const uint8_t* arow, int rc) {
 #pragma unroll
@@ -258,7 +264,12 @@ __device__ __forceinline__ void direct_block(uint8_t* ring, const uint8_t* __res
   {
     const uint8_t* arow0 = ring + ((32 * wave + m) % DIR_RING) * pitch + 16 * half;
     loadA(AN, arow0, 0);
-    loadB(BN, 0, 0);
+    if (KSTEP == 1) {
+      loadB(Bk[1], 0, 0);
+    } else {
+#pragma unroll
+      for (int rc = 0; rc + 1 < KSTEP; rc++) loadB(Bk[rc], 0, rc);
+    }
   }
 #pragma unroll
   for (int p = 0; p < 2; p++) rq[DIR_PF - 1][p] = fetch(DIR_ROWS + DIR_PF - 1, ncl[p], rok[DIR_PF - 1][p]);
@@ -283,21 +294,26 @@ __device__ __forceinline__ void direct_block(uint8_t* ring, const uint8_t* __res
     // this wave's A rows: staged rows 32 wave + m + i
     const uint8_t* arow = ring + ((32 * wave + m + i) % DIR_RING) * pitch + 16 * half;
     const uint8_t* arowN = ring + ((32 * wave + m + i + 1) % DIR_RING) * pitch + 16 * half;
+    const int inext = more ? i + 1 : i;
 #pragma unroll
     for (int j = 0; j < NA; j++) A0[j] = AN[j];
+    if (KSTEP == 1) {
 #pragma unroll
-    for (int j = 0; j < CMAX; j++) B0[j] = BN[j];
-    if (more) loadB(BN, i + 1, 0);
+      for (int j = 0; j < CMAX; j++) Bk[0][j] = Bk[1][j];
+      loadB(Bk[1], inext, 0);
+    }
 #pragma unroll
     for (int rc = 0; rc < KSTEP; rc++) {
       dv4i* ca = (rc & 1) ? A1 : A0;
-      dv4i* cb = (rc & 1) ? B1 : B0;
       dv4i* na = (rc & 1) ? A0 : A1;
-      dv4i* nb = (rc & 1) ? B0 : B1;
-      if (rc + 1 < KSTEP) {
-        loadA(na, arow, rc + 1);
-        loadB(nb, i, rc + 1);
+      if (KSTEP > 1) {
+        const int lc = rc + KSTEP - 1;
+        if (lc < KSTEP)
+          loadB(Bk[lc], i, lc);
+        else
+          loadB(Bk[lc - KSTEP], inext, lc - KSTEP);
       }
+      if (rc + 1 < KSTEP) loadA(na, arow, rc + 1);
       if (rc == KSTEP - 1) {
         // the next step's first class reads the row written at this step's
         // start: after the barrier (before this class's MFMAs when there is
@@ -305,7 +321,7 @@ __device__ __forceinline__ void direct_block(uint8_t* ring, const uint8_t* __res
         if (KSTEP == 1) __syncthreads();
         if (more) loadA(AN, arowN, 0);
       }
-      mma(ca, cb, cls[rc]);
+      mma(ca, Bk[KSTEP == 1 ? 0 : rc], cls[rc]);
       if (rc == 0 && KSTEP > 1) __syncthreads();
     }
 #pragma unroll

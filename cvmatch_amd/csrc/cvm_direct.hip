@@ -37,7 +37,7 @@ typedef int dv16i __attribute__((ext_vector_type(16)));
 typedef const dv4i __attribute__((address_space(1)))* gdv4p;
 
 constexpr int DIR_T = 4;      // column tiles per wave (128 output columns)
-constexpr int DIR_WAVES = 2;  // waves per workgroup, 32 output rows each
+constexpr int DIR_WAVES = 1;  // waves per workgroup, 32 output rows each
 constexpr int DIR_ROWS = 32 * DIR_WAVES;
 constexpr int DIR_RING = DIR_ROWS + 8;
 constexpr int DIR_CMAX = 9;   // K chunks per residue class: nk <= kstep * DIR_CMAX

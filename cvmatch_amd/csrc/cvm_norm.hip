@@ -291,6 +291,10 @@ struct NormPlan {
 // Each lane of the wave form owns 8 consecutive image columns of the strip
 // (64*q + tw - 1 <= 512 columns), loaded with 16-byte vector loads.
 constexpr int NORM_CPT = 8;
+// Prefix row length in LDS: prefix index c lives at c + c/32 (lane l writes
+// columns 8l..8l+7, and the pad word spreads the 64 lanes of one store over
+// all banks).
+constexpr int NORM_NWP = 64 * NORM_CPT + 1 + (64 * NORM_CPT + 1) / 32 + 1;
 
 static NormPlan norm_plan(int tw, int th, int cnMax) {
   NormPlan p{0, 0, 0};
@@ -700,21 +704,17 @@ __device__ __forceinline__ double score_fp64(double numI, double idiff, double s
 // EXACT: corr holds the int32 C' = sum (P - 128)(T - 128) of the direct
 // correlation (bit pattern read through the float pointer) and every score
 // is the FP64 one rounded once to float32.
-template <int Q, bool SQ32, int VMODE, bool EXACT>
+// NCH: channel sums kept (1 gray, 3 RGB, 4 any cn <= 4); channels at or past
+// cn are zero, so dropping them changes no sum.  P / P2: the workgroup's
+// prefix rows in LDS.
+template <int Q, bool SQ32, int VMODE, bool EXACT, int NCH, typename sq_t>
 __device__ __forceinline__ void norm_wave_block(const uint8_t* __restrict__ img, int64_t istride, int iw, int ih,
                                                 int step, int tw, int th, int rw, int rh,
                                                 const CallState* __restrict__ st, const uint32_t* __restrict__ ps,
                                                 const float* corr, float* map, Extrema* __restrict__ parts, int bx,
-                                                int by, int gx) {
-  typedef typename std::conditional<SQ32, uint32_t, unsigned long long>::type sq_t;
+                                                int by, int gx, uint32_t (*P)[NORM_NWP], sq_t* P2) {
   constexpr int CPT = NORM_CPT;
-  constexpr int NW = 64 * CPT;
   constexpr int SX = 64 * Q;
-  // prefix index c lives at c + c/32: lane l writes columns 8l..8l+7, and
-  // the pad word spreads the 64 lanes of one store over all banks
-  constexpr int NWP = NW + 1 + (NW + 1) / 32 + 1;
-  __shared__ uint32_t P[4][NWP];
-  __shared__ sq_t P2[NWP];
   const int lane = threadIdx.x;
   const int xs = bx * SX, ys = by * NB_ROWS;
   const int ye = min(rh, ys + NB_ROWS);
@@ -748,7 +748,9 @@ __device__ __forceinline__ void norm_wave_block(const uint8_t* __restrict__ img,
     for (int k = 0; k < 4; k++) tsd[k] = k < cn ? (double)st->tsum[k] : 0.0;
     // EXACT: sum P*T = C' + 128 (S + sum_k T_k) - 2^14 area cn
     const double varSumD = (double)st->varSum;
-    const double tsumAll = tsd[0] + tsd[1] + tsd[2] + tsd[3];
+    double tsumAll = tsd[0];
+#pragma unroll
+    for (int k = 1; k < NCH; k++) tsumAll += tsd[k];
     const double cpOff = 16384.0 * aread * (double)cn;
     const int cbase = lane * CPT;
     // every in-image column of the lane is summed; columns past the strip
@@ -823,24 +825,26 @@ __device__ __forceinline__ void norm_wave_block(const uint8_t* __restrict__ img,
     };
 
     auto row = [&](const NormRow<Q>& r, int y) {
-      uint32_t run[4] = {0, 0, 0, 0};
+      uint32_t run[NCH];
+#pragma unroll
+      for (int k = 0; k < NCH; k++) run[k] = 0;
       sq_t run2 = 0;
-      uint32_t loc[CPT][4];
+      uint32_t loc[CPT][NCH];
       sq_t loc2[CPT];
 #pragma unroll
       for (int j = 0; j < CPT; j++) {
         const uint32_t cv[4] = {cs02[j] & 0xFFFFu, cs13[j] & 0xFFFFu, cs02[j] >> 16, cs13[j] >> 16};
 #pragma unroll
-        for (int k = 0; k < 4; k++) {
+        for (int k = 0; k < NCH; k++) {
           run[k] += cv[k];
           loc[j][k] = run[k];
         }
         run2 += c2[j];
         loc2[j] = run2;
       }
-      uint32_t off[4];
+      uint32_t off[NCH];
 #pragma unroll
-      for (int k = 0; k < 4; k++) off[k] = wave_incl_scan_u32(run[k], lane) - run[k];
+      for (int k = 0; k < NCH; k++) off[k] = wave_incl_scan_u32(run[k], lane) - run[k];
       sq_t off2;
       if (SQ32) {
         off2 = (sq_t)wave_incl_scan_u32((uint32_t)run2, lane) - run2;
@@ -857,12 +861,12 @@ __device__ __forceinline__ void norm_wave_block(const uint8_t* __restrict__ img,
         const int pc = cbase + j + 1;
         const int pi = pc + (pc >> 5);
 #pragma unroll
-        for (int k = 0; k < 4; k++) P[k][pi] = off[k] + loc[j][k];
+        for (int k = 0; k < NCH; k++) P[k][pi] = off[k] + loc[j][k];
         P2[pi] = off2 + loc2[j];
       }
       if (lane == 0) {
 #pragma unroll
-        for (int k = 0; k < 4; k++) P[k][0] = 0;
+        for (int k = 0; k < NCH; k++) P[k][0] = 0;
         P2[0] = 0;
       }
       __syncthreads();
@@ -871,16 +875,23 @@ __device__ __forceinline__ void norm_wave_block(const uint8_t* __restrict__ img,
       for (int q = 0; q < Q; q++) {
         const int xl = lane + 64 * q, xb = xl + tw;
         const int ia = xl + (xl >> 5), ib = xb + (xb >> 5);
-        // exact integers carried in binary64 (all below 2^53)
-        double sd[4];
+        // exact integers carried in binary64 (all below 2^53, so every sum
+        // below is exact in any order)
+        double sd[NCH];
 #pragma unroll
-        for (int k = 0; k < 4; k++) sd[k] = (double)(uint32_t)(P[k][ib] - P[k][ia]);
+        for (int k = 0; k < NCH; k++) sd[k] = (double)(uint32_t)(P[k][ib] - P[k][ia]);
         const double s2 = (double)(sq_t)(P2[ib] - P2[ia]);
-        const double sq = sd[0] * sd[0] + sd[1] * sd[1] + sd[2] * sd[2] + sd[3] * sd[3];
-        double lane0 = sd[0] * tsd[0] + sd[1] * tsd[1] + sd[2] * tsd[2] + sd[3] * tsd[3];
+        double sq = sd[0] * sd[0], lane0 = sd[0] * tsd[0];
+#pragma unroll
+        for (int k = 1; k < NCH; k++) {
+          sq = sq + sd[k] * sd[k];
+          lane0 = lane0 + sd[k] * tsd[k];
+        }
         const double idiff = aread * s2 - sq;
         if constexpr (EXACT) {
-          const double S = sd[0] + sd[1] + sd[2] + sd[3];
+          double S = sd[0];
+#pragma unroll
+          for (int k = 1; k < NCH; k++) S += sd[k];
           const double corrI = (double)__float_as_int(r.c[q]) + 128.0 * (S + tsumAll) - cpOff;
           vq[q] = (float)score_fp64(aread * corrI - lane0, idiff, s2, aread, varSumD);
         } else {
@@ -957,8 +968,20 @@ __global__ __launch_bounds__(64) void k_norm_wave(const uint8_t* __restrict__ im
                                                   int step, int tw, int th, int rw, int rh,
                                                   const CallState* __restrict__ st, const uint32_t* __restrict__ ps,
                                                   const float* corr, float* map, Extrema* __restrict__ parts) {
-  norm_wave_block<Q, SQ32, VMODE, EXACT>(img, istride, iw, ih, step, tw, th, rw, rh, st, ps, corr, map, parts,
-                                         blockIdx.x, blockIdx.y, gridDim.x);
+  typedef typename std::conditional<SQ32, uint32_t, unsigned long long>::type sq_t;
+  __shared__ uint32_t P[4][NORM_NWP];
+  __shared__ sq_t P2[NORM_NWP];
+  const int cn = st->cn;
+#define CVM_NWB(N)                                                                                              \
+  norm_wave_block<Q, SQ32, VMODE, EXACT, N, sq_t>(img, istride, iw, ih, step, tw, th, rw, rh, st, ps, corr, map, \
+                                                  parts, blockIdx.x, blockIdx.y, gridDim.x, P, P2)
+  if (VMODE == 2 || (VMODE == 0 && cn == 1))
+    CVM_NWB(1);
+  else if (VMODE == 1 && cn == 3)
+    CVM_NWB(3);
+  else
+    CVM_NWB(4);
+#undef CVM_NWB
 }
 
 // Many templates against one frame in one launch (a Fleet's members, all of
@@ -967,10 +990,22 @@ template <int Q, bool SQ32, int VMODE, bool EXACT>
 __global__ __launch_bounds__(64) void k_norm_batch(const uint8_t* __restrict__ img, int64_t istride, int iw, int ih,
                                                    int step, const NormJob* __restrict__ jobs,
                                                    const int* __restrict__ wg_job) {
+  typedef typename std::conditional<SQ32, uint32_t, unsigned long long>::type sq_t;
+  __shared__ uint32_t P[4][NORM_NWP];
+  __shared__ sq_t P2[NORM_NWP];
   const NormJob J = jobs[wg_job[blockIdx.x]];
   const int local = blockIdx.x - J.wg0;
-  norm_wave_block<Q, SQ32, VMODE, EXACT>(img, istride, iw, ih, step, J.tw, J.th, J.rw, J.rh, J.st, J.ps, J.corr,
-                                         nullptr, J.parts, local % J.gx, local / J.gx, J.gx);
+  const int cn = J.st->cn;
+#define CVM_NBB(N)                                                                                              \
+  norm_wave_block<Q, SQ32, VMODE, EXACT, N, sq_t>(img, istride, iw, ih, step, J.tw, J.th, J.rw, J.rh, J.st, J.ps, \
+                                                  J.corr, nullptr, J.parts, local % J.gx, local / J.gx, J.gx, P, P2)
+  if (VMODE == 2 || (VMODE == 0 && cn == 1))
+    CVM_NBB(1);
+  else if (VMODE == 1 && cn == 3)
+    CVM_NBB(3);
+  else
+    CVM_NBB(4);
+#undef CVM_NBB
 }
 
 void launch_norm_prep(const uint8_t* img, int64_t istride, int iw, int ih, int step, int tw, int th, int cnMax,

@@ -138,7 +138,7 @@ def fleet_leg(frames_timed=16, solo_frames=2):
                          "frames_timed": solo_frames}
     out["direct_over_plain_solo"] = round(out["direct"]["tpl_frames_per_s"] / out["plain_solo"]["tpl_frames_per_s"], 3)
     out["max_abs_delta_vs_fp64"] = ("direct scores equal the FP64 oracle rounded to float32 "
-                                    "(checked in tests/test_gpu_direct.py)")
+                                    "(checked on this workload in tests/test_gpu_fleet_fp64.py)")
     del fl, ms
     return out
 
@@ -250,8 +250,9 @@ def configs_leg(reps=15):
     """The other configurations, device resident (inputs in HBM, the map
     written on the device, graphs replayed like the headline's step):
     640x480 RGBA Match 48x48 (cn = 4), the 1920x1080 gray UI MatchGray 96x32,
-    and the 2560x1440 gray direct-vs-FFT sweep (ms per call with the map for
-    square templates 8..256; the direct form is the FP64-exact one)."""
+    the 2560x1440 gray direct-vs-FFT sweep (ms per call with the map for
+    square templates 8..256; the direct form is the FP64-exact one) and the
+    4K RGBA 128x128 call in both forms."""
     import cvmatch_amd as cvm
     from cvmatch_amd import synth
 
@@ -308,6 +309,14 @@ def configs_leg(reps=15):
     res["direct_vs_fft_gray_2560x1440"] = sweep
     cross = [r["template"] for r in sweep if r["direct_ms"] > r["fft_ms"]]
     res["crossover_template"] = cross[0] if cross else None
+    # the headline's 4K RGBA shape (128x128, cn = 4) in both forms
+    parent, sub, _ = synth.workload_map_4k(seed=3, alpha="random")
+    rw, rh = 3840 - 128 + 1, 2160 - 128 + 1
+    dp, ds = dev(parent), dev(sub)
+    dm = dev(np.zeros(rw * rh, np.float32))
+    fft = timed(lambda: call(False, dp, 3840 * 4, 3840, 2160, ds, 128 * 4, 128, 128, 0, 4, dm))
+    dirt = timed(lambda: call(True, dp, 3840 * 4, 3840, 2160, ds, 128 * 4, 128, 128, 0, 4, dm))
+    res["direct_vs_fft_rgba_3840x2160_128"] = {"fft_ms": round(fft * 1e3, 4), "direct_ms": round(dirt * 1e3, 4)}
     for p in bufs:
         L.cvm_device_free(p)
     return res

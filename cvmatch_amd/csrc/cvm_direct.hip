@@ -39,9 +39,13 @@ typedef const dv4i __attribute__((address_space(1)))* gdv4p;
 constexpr int DIR_T = 4;      // column tiles per wave (128 output columns)
 constexpr int DIR_WAVES = 1;  // waves per workgroup, 32 output rows each
 constexpr int DIR_ROWS = 32 * DIR_WAVES;
-constexpr int DIR_RING = DIR_ROWS + 8;
+// Ring rows: the DIR_ROWS a step reads plus the one written at its start (a
+// single wave's LDS accesses run in program order; queued rows wait in
+// registers, not in the ring).
+constexpr int DIR_RING = DIR_ROWS + 1;
+static_assert(DIR_WAVES == 1, "a ring of DIR_ROWS + 1 rows relies on one wave's program order");
 constexpr int DIR_CMAX = 9;   // K chunks per residue class: nk <= kstep * DIR_CMAX
-constexpr int DIR_PF = 4;     // ring rows in flight (<= DIR_RING - DIR_ROWS)
+constexpr int DIR_PF = 4;     // staged rows in flight in the register queue
 
 // Staged bytes per pixel: gray 1, RGBA 4, RGBA with cn = 3 staged as RGB.
 int direct_kstep(int step, int cn) { return step == 1 ? 1 : (cn == 3 ? 3 : 4); }
@@ -242,12 +246,12 @@ __device__ __forceinline__ void direct_block(uint8_t* ring, const uint8_t* __res
 
   // Staged row DIR_ROWS + i (first read in step i + 1) is fetched at the end
   // of step i - DIR_PF (before the loop for i < DIR_PF) into a register
-  // queue and written to its ring slot at the start of step i; the barrier
-  // after step i's first class makes it visible
-  // before step i prefetches step i + 1's first class.  Its slot held row
-  // i + DIR_ROWS - DIR_RING, last read before the barrier of the step after
-  // that row's step.  Unit nc[p] of each staged row is this thread's (units
-  // past the row reload the last one and are not written).
+  // queue and written to its ring slot at the start of step i, before step i
+  // prefetches step i + 1's first class (the workgroup is one wave, so its
+  // LDS accesses take effect in program order).  Its slot held row
+  // i + DIR_ROWS - DIR_RING = i - 1, last read in step i - 1.  Unit nc[p] of
+  // each staged row is this thread's (units past the row reload the last
+  // one and are not written).
   int nc[2];
 #pragma unroll
   for (int p = 0; p < 2; p++) nc[p] = tid + p * 64 * DIR_WAVES;

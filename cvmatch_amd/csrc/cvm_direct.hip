@@ -25,6 +25,8 @@
 // v_mfma_i32_32x32x32_i8 lane layout: A byte b of lane l is A[l%32][16(l/32)+b],
 // B byte b of lane l is B[16(l/32)+b][l%32], and accumulator register j of
 // lane l is D[8(j/4) + 4(l/32) + j%4][l%32].
+#include <type_traits>
+
 #include "cvm_internal.h"
 
 namespace cvm {
@@ -263,13 +265,23 @@ __device__ __forceinline__ void direct_block(uint8_t* ring, const uint8_t* __res
 #pragma unroll
     for (int p = 0; p < 2; p++) rq[d][p] = fetch(DIR_ROWS + d, ncl[p], rok[d][p]);
 
-  // step 0's first class
-  dv4i AN[NA];
+  // Odd KSTEP: class c of the whole walk (row i, class rc: c = i * KSTEP + rc)
+  // uses A buffer c & 1 and loads class c + 1's A into the other one, so a
+  // row's first class takes buffer i & 1 and the rows are walked in pairs,
+  // every buffer index static (gray rows keep their B the same way: row i
+  // in Bk[i & 1]).  Even KSTEP: every row starts in A0, and the next row's
+  // first class is loaded into a third buffer AN (skipped after the last
+  // row), copied at the row start.  That keeps AN live beside A0 and the
+  // RGBA kernels at one wave per SIMD, which measured faster than two (4K
+  // RGBA 128x128: 0.66 against 0.81 ms; the 33-row ring of these rows lets
+  // four single-wave workgroups share a CU).
+  constexpr bool PAIRED = (KSTEP & 1) != 0;
+  dv4i AN[PAIRED ? 1 : NA];
   {
     const uint8_t* arow0 = ring + ((32 * wave + m) % DIR_RING) * pitch + 16 * half;
-    loadA(AN, arow0, 0);
+    loadA(PAIRED ? A0 : AN, arow0, 0);
     if (KSTEP == 1) {
-      loadB(Bk[1], 0, 0);
+      loadB(Bk[0], 0, 0);
     } else {
 #pragma unroll
       for (int rc = 0; rc + 1 < KSTEP; rc++) loadB(Bk[rc], 0, rc);
@@ -277,7 +289,9 @@ __device__ __forceinline__ void direct_block(uint8_t* ring, const uint8_t* __res
   }
 #pragma unroll
   for (int p = 0; p < 2; p++) rq[DIR_PF - 1][p] = fetch(DIR_ROWS + DIR_PF - 1, ncl[p], rok[DIR_PF - 1][p]);
-  for (int i = 0; i < th; i++) {
+  // one template row; P0 = parity of its first class
+  auto step = [&](int i, auto P0) {
+    constexpr int p0 = decltype(P0)::value;
     const bool more = i + 1 < th;
     // queue: rq[d] holds staged row DIR_ROWS + i + d; the newest one was
     // fetched at the end of the previous step, so the shift below never
@@ -299,17 +313,15 @@ __device__ __forceinline__ void direct_block(uint8_t* ring, const uint8_t* __res
     const uint8_t* arow = ring + ((32 * wave + m + i) % DIR_RING) * pitch + 16 * half;
     const uint8_t* arowN = ring + ((32 * wave + m + i + 1) % DIR_RING) * pitch + 16 * half;
     const int inext = more ? i + 1 : i;
+    if (!PAIRED) {
 #pragma unroll
-    for (int j = 0; j < NA; j++) A0[j] = AN[j];
-    if (KSTEP == 1) {
-#pragma unroll
-      for (int j = 0; j < CMAX; j++) Bk[0][j] = Bk[1][j];
-      loadB(Bk[1], inext, 0);
+      for (int j = 0; j < NA; j++) A0[j] = AN[j];
     }
+    if (KSTEP == 1) loadB(Bk[(p0 + 1) & 1], inext, 0);
 #pragma unroll
     for (int rc = 0; rc < KSTEP; rc++) {
-      dv4i* ca = (rc & 1) ? A1 : A0;
-      dv4i* na = (rc & 1) ? A0 : A1;
+      dv4i* ca = ((p0 + rc) & 1) ? A1 : A0;
+      dv4i* na = ((p0 + rc) & 1) ? A0 : A1;
       if (KSTEP > 1) {
         const int lc = rc + KSTEP - 1;
         if (lc < KSTEP)
@@ -320,16 +332,27 @@ __device__ __forceinline__ void direct_block(uint8_t* ring, const uint8_t* __res
       if (rc + 1 < KSTEP) loadA(na, arow, rc + 1);
       if (rc == KSTEP - 1) {
         // the next step's first class reads the row written at this step's
-        // start: after the barrier (before this class's MFMAs when there is
-        // only one class)
+        // start (program order within the wave; the barrier is a compiler
+        // fence here); past the last row it reloads an unused ring row
         if (KSTEP == 1) __syncthreads();
-        if (more) loadA(AN, arowN, 0);
+        if (PAIRED)
+          loadA(na, arowN, 0);
+        else if (more)
+          loadA(AN, arowN, 0);
       }
-      mma(ca, Bk[KSTEP == 1 ? 0 : rc], cls[rc]);
+      mma(ca, Bk[KSTEP == 1 ? (p0 & 1) : rc], cls[rc]);
       if (rc == 0 && KSTEP > 1) __syncthreads();
     }
 #pragma unroll
     for (int p = 0; p < 2; p++) rq[DIR_PF - 1][p] = fetch(DIR_ROWS + i + DIR_PF, ncl[p], rok[DIR_PF - 1][p]);
+  };
+  if constexpr (PAIRED) {
+    for (int i = 0; i < th; i += 2) {
+      step(i, std::integral_constant<int, 0>());
+      if (i + 1 < th) step(i + 1, std::integral_constant<int, 1>());
+    }
+  } else {
+    for (int i = 0; i < th; i++) step(i, std::integral_constant<int, 0>());
   }
 
   const int yb = Y0 + 32 * wave + 4 * half;

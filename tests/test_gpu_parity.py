@@ -272,6 +272,30 @@ def test_fleet_bitwise(cvm, orc):
     assert [tuple(r) for r in fg.find_all(g)] == orc.fleet_find_all(g, gsubs)
 
 
+def test_fleet_mixed_normalisation_forms(cvm, orc):
+    """A FindAll whose members take both normalisation forms: most are
+    scored in the batched wave-form launches, a 400-wide and a 300-tall
+    member take the int64 form on their own; varying-alpha members (cn = 4)
+    sit beside cn = 3 ones.  Two frame sizes in turn (the batch plan is
+    rebuilt for the second), each bit for bit against the oracle's FindAll."""
+    rng = np.random.default_rng(31)
+    big = synth.add_noise(synth.flat_rect_scene(960, 720, rng, channels=4), 3.0, rng, channels=[0, 1, 2])
+    big[..., 3] = 255
+    sizes = [(400, 90), (60, 300), (40, 40), (120, 64), (17, 23), (200, 150), (90, 90)]
+    subs = []
+    for k, (w, h) in enumerate(sizes):
+        x, y = int(rng.integers(0, 960 - w)), int(rng.integers(0, 720 - h))
+        s = big[y:y + h, x:x + w].copy()
+        if k in (2, 5):
+            s[..., 3] = rng.integers(0, 256, (h, w), dtype=np.uint8)  # varying alpha: cn = 4
+        subs.append(s)
+    f = cvm.Fleet(*[cvm.Matcher(s) for s in subs])
+    for fw, fh in [(960, 720), (700, 520), (960, 720)]:
+        frame = np.ascontiguousarray(big[:fh, :fw])
+        say(f"[fleet] mixed forms on {fw}x{fh}")
+        assert [tuple(r) for r in f.find_all(frame)] == orc.fleet_find_all(frame, subs)
+
+
 def test_fleet_single_equals_find(cvm):
     parent, sub, _ = synth.workload_match_640(seed=12, alpha="opaque")
     m = cvm.Matcher(sub)

@@ -96,6 +96,18 @@ def cpu_baseline(parent, sub, budget_s=3.0):
     th, tw = sub.shape[:2]
     windows = (iw - tw + 1) * (ih - th + 1)
     best = min(times)
+    # the same call on one thread, best of 3
+    one = []
+    for _ in range(3):
+        t0 = time.perf_counter()
+        O.match_rgba(parent, sub, threads=1, want_map=True)
+        one.append(time.perf_counter() - t0)
+    host = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            host = next((ln.split(":", 1)[1].strip() for ln in f if ln.startswith("model name")), host)
+    except OSError:
+        pass
     return {
         "value": windows / best / 1e9,
         "unit": "Gwindows/s",
@@ -103,6 +115,8 @@ def cpu_baseline(parent, sub, budget_s=3.0):
         "kind": "port",
         "sample": f"full 3840x2160 RGBA 128x128 MatchMap, best of {len(times)} runs after 1 warm-up "
                   f"({best * 1e3:.1f} ms/run) with {threads} threads",
+        "single_thread": {"value": windows / min(one) / 1e9, "ms": round(min(one) * 1e3, 1), "runs": 3},
+        "host_cpu": host,
     }
 
 

@@ -1047,7 +1047,7 @@ static int plain_norm_plan(Engine& E, hipStream_t s, cvm_fleet* f, FleetGeom& g,
   std::vector<int> last(n, -1);
   bool usedCn[5] = {false, false, false, false, false};
   int prepM[5] = {-1, -1, -1, -1, -1};
-  size_t fill = 0;
+  size_t fill = 0, maxFill = 64;  // the region is sized to the largest batch, not the cap
   for (int mi : order) {
     cvm_matcher* m = f->ms[mi];
     const int cn = cnOf[mi];
@@ -1062,6 +1062,7 @@ static int plain_norm_plan(Engine& E, hipStream_t s, cvm_fleet* f, FleetGeom& g,
     it.off = (int64_t)fill;
     off[mi] = it.off;
     fill += sz;
+    maxFill = std::max(maxFill, fill);
     bat.back().push_back(it);
     usedCn[cn] = true;
     if (prepM[cn] < 0) prepM[cn] = mi;
@@ -1072,7 +1073,7 @@ static int plain_norm_plan(Engine& E, hipStream_t s, cvm_fleet* f, FleetGeom& g,
     for (auto& it : b) t += (size_t)norm_grid_x(it.rw, it.q) * norm_grid_y(it.rh);
     maxParts = std::max(maxParts, t);
   }
-  CHK(f->pcorr.ensure(sizeof(float) * capF));
+  CHK(f->pcorr.ensure(sizeof(float) * maxFill));
   CHK(f->pparts.ensure(sizeof(Extrema) * maxParts));
   for (int cn = 1; cn <= 4; cn++)
     if (usedCn[cn]) CHK(f->dscr[cn].ensure(norm_scratch_bytes(pw, ph, 1, 1, cn)));

@@ -930,8 +930,9 @@ struct cvm_fleet {
   std::mutex mu;
   std::shared_ptr<cvm::FleetGeom> geom;
   cvm::DevBuf frame, mspec, corr, res;
-  // plain FindAll: correlation maps and extrema partials of one batch
-  cvm::DevBuf pcorr, pparts;
+  // plain FindAll: correlation maps and extrema partials of one batch, and
+  // the second member's product spectra of a paired column launch
+  cvm::DevBuf pcorr, pparts, mspec2;
   // direct form: per-member call states for the parent alpha key dpConst,
   // and the normalisation partials per channel count
   cvm::DevBuf dstates, dscr[5];
@@ -1199,20 +1200,34 @@ static int fleet_find_dev(Engine& E, hipStream_t s, cvm_fleet* f, const uint8_t*
       launch_cols(COL_FWD, frame, frame, ftile, ftile, chan, bg.cnMax, fst, gF, pt, nullptr, nullptr, 0, 0, 1.0f,
                   ntiles, s);
     }
-    // Phase 2 fleet.go:265-328: each member's tail
+    // Phase 2 fleet.go:265-328: each member's tail.  Members go in pairs
+    // through one column launch (the frame slab read once for both) when the
+    // bucket's tiles are full height; each member's row pass and scoring
+    // then follow in member order.
+    const bool pairs = b.members.size() > 1 && bg.dh2 == 0;
     CHK(f->mspec.ensure(sizeof(float2) * (size_t)ftile * ntiles));
-    float2* mspec = f->mspec.as<float2>();
-    for (size_t j = 0; j < b.members.size(); j++) {
-      int mi = b.members[j];
-      cvm_matcher* m = f->ms[mi];
-      int cn = bg.cns[j];
-      int rw = pw - m->w + 1, rh = ph - m->h + 1;
-      CallState* mst = g->states + mi;
+    if (pairs) CHK(f->mspec2.ensure(sizeof(float2) * (size_t)ftile * ntiles));
+    float2* mspecs[2] = {f->mspec.as<float2>(), pairs ? f->mspec2.as<float2>() : nullptr};
+    int64_t tchan2 = (int64_t)bg.dh2 * bg.hwPad;
+    auto grid_of = [&](size_t j) {
       TileGrid gM = gF;
-      gM.extW = rw;
-      gM.extH = rh;
-      const TplSpec& ts = bg.ts[j];
-      int64_t tchan2 = (int64_t)bg.dh2 * bg.hwPad;
+      gM.extW = pw - f->ms[b.members[j]]->w + 1;
+      gM.extH = ph - f->ms[b.members[j]]->h + 1;
+      return gM;
+    };
+    auto cols_one = [&](size_t j, float2* mspec) {
+      const int mi = b.members[j];
+      Stage S(E, "fleet_cols", s);
+      launch_cols(COL_MULINV, frame, mspec, ftile, ftile, chan, bg.cns[j], g->states + mi, grid_of(j), pt,
+                  bg.ts[j].spec, bg.ts[j].spec2, chan, tchan2, 1.0f, ntiles, s);
+    };
+    auto tail = [&](size_t j, const float2* mspec) -> int {
+      const int mi = b.members[j];
+      cvm_matcher* m = f->ms[mi];
+      const int cn = bg.cns[j];
+      const int rw = pw - m->w + 1, rh = ph - m->h + 1;
+      CallState* mst = g->states + mi;
+      const TileGrid gM = grid_of(j);
       const bool batched = NP.corr_off[mi] >= 0;
       float* corr;
       if (batched) {
@@ -1220,11 +1235,6 @@ static int fleet_find_dev(Engine& E, hipStream_t s, cvm_fleet* f, const uint8_t*
       } else {
         CHK(f->corr.ensure(sizeof(float) * (size_t)rw * rh));
         corr = f->corr.as<float>();
-      }
-      {
-        Stage S(E, "fleet_cols", s);
-        launch_cols(COL_MULINV, frame, mspec, ftile, ftile, chan, cn, mst, gM, pt, ts.spec, ts.spec2, chan, tchan2,
-                    1.0f, ntiles, s);
       }
       {
         Stage S(E, "fleet_row_inv", s);
@@ -1245,6 +1255,27 @@ static int fleet_find_dev(Engine& E, hipStream_t s, cvm_fleet* f, const uint8_t*
           launch_norm_batch(d_img, istride, pw, ph, f->step, gq[0], gq[1], false, NP.jobs.as<NormJob>(),
                             NP.wg.as<int>() + gq[2], gq[3], s);
         launch_reduce_batch(NP.jobs.as<NormJob>() + B.job0, B.njobs, s);
+      }
+      return 0;
+    };
+    for (size_t j = 0; j < b.members.size();) {
+      bool paired = false;
+      if (pairs && j + 1 < b.members.size()) {
+        const TileGrid gA = grid_of(j), gB = grid_of(j + 1);
+        const ColPairMember A{g->states + b.members[j], bg.ts[j].spec, mspecs[0], gA.extW, gA.extH};
+        const ColPairMember Bm{g->states + b.members[j + 1], bg.ts[j + 1].spec, mspecs[1], gB.extW, gB.extH};
+        Stage S(E, "fleet_cols", s);
+        paired = launch_cols_pair(frame, ftile, chan, std::max(bg.cns[j], bg.cns[j + 1]), gF, pt, chan, ntiles, A,
+                                  Bm, s);
+      }
+      if (paired) {
+        CHK(tail(j, mspecs[0]));
+        CHK(tail(j + 1, mspecs[1]));
+        j += 2;
+      } else {
+        cols_one(j, mspecs[0]);
+        CHK(tail(j, mspecs[0]));
+        j += 1;
       }
     }
   }
@@ -2151,6 +2182,7 @@ void cvm_fleet_free(cvm_fleet* f) {
     f->corr.release();
     f->res.release();
     f->pcorr.release();
+    f->mspec2.release();
     f->pparts.release();
   }
   for (auto* m : f->ms) matcher_release(m);

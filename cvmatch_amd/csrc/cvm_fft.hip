@@ -501,6 +501,56 @@ __device__ __forceinline__ void col_mid(float2* a, const float2* __restrict__ tr
   }
 }
 
+// The inverse column transform of the LDS slab (input at the bit-reversed
+// rows, as the products are stored) and the store of its first `need`
+// spatial rows: the passes in LDS, the last one in registers straight to the
+// spectrum rows.  The caller has synchronised after filling the slab.
+__device__ __forceinline__ void store_f2(float2* p, float2 v) { *p = v; }
+__device__ __forceinline__ void store_f2(float __attribute__((ext_vector_type(2))) __attribute__((address_space(1)))* p,
+                                         float2 v) {
+  float __attribute__((ext_vector_type(2))) u;
+  u.x = v.x;
+  u.y = v.y;
+  *p = u;
+}
+
+template <int LOG2N, typename DstPtr>
+__device__ __forceinline__ void col_inverse_store(float2* a, const float2* __restrict__ tri, DstPtr dp, int hwPad,
+                                                  int ncol, int need) {
+  constexpr int CG = ColCG<LOG2N>::value, LCG = ceil_log2c(CG), NT = 256;
+  using F = ColFirst<LOG2N>;
+  constexpr int NB1 = F::NB;
+  constexpr bool SINGLE = NB1 == LOG2N;
+  constexpr int LOL = SINGLE ? 0 : col_last_lo(LOG2N, NB1);
+  constexpr int HEADL = SINGLE ? F::HEAD : 0;
+  constexpr int NSTL = SINGLE ? F::NST : (LOG2N - LOL) / 2;
+  constexpr int NBL = HEADL + 2 * NSTL, EL = 1 << NBL;
+  constexpr int GL = CG << (LOG2N - NBL), ITL = (GL + NT - 1) / NT;
+  const int tid = threadIdx.x;
+  if constexpr (!SINGLE) {
+    fft_pass<LOG2N, 0, F::HEAD, F::NST, CG, NT, CG, false, true>(a, CG, 1, tri, true);
+    __syncthreads();
+    col_mid<LOG2N, NB1, LOL, CG>(a, tri, true);
+  }
+#pragma unroll
+  for (int it = 0; it < ITL; it++) {
+    const int gi = tid + it * NT, v = gi & (CG - 1), gg = gi >> LCG;
+    const int base = grp_base<LOL, NBL>(gg);
+    if (GL % NT == 0 || gi < GL) {
+      float2 r[EL];
+#pragma unroll
+      for (int e = 0; e < EL; e++) r[e] = a[(base | (e << LOL)) * CG + v];
+      bfly_regs<LOG2N, LOL, HEADL, NSTL>(r, gg & ((1 << LOL) - 1), tri, true);
+      if (v < ncol)
+#pragma unroll
+        for (int e = 0; e < EL; e++) {
+          const int y = base | (e << LOL);
+          if (y < need) store_f2(dp + (int64_t)y * hwPad + v, r[e]);
+        }
+    }
+  }
+}
+
 // One workgroup owns CG spectrum columns of one tile/channel for the whole
 // column height; threads take (group, column) pairs with the column fastest,
 // so each row's CG columns are one contiguous access.  Modes:
@@ -672,30 +722,89 @@ __device__ __forceinline__ void cols_body(float2* a, int L, const float2* __rest
     }
   }
   __syncthreads();
+  col_inverse_store<LOG2N>(a, tri, dp, hwPad, ncol, need);
+}
 
-  // inverse: the passes in LDS, the last one to the spectrum rows
-  if constexpr (!SINGLE) {
-    fft_pass<LOG2N, 0, F::HEAD, F::NST, CG, NT, CG, false, true>(a, CG, 1, tri, true);
-    __syncthreads();
-    col_mid<LOG2N, NB1, LOL, CG>(a, tri, true);
+// Two members of one Fleet bucket against the same frame spectrum in one
+// launch (COL_MULINV for both): each workgroup reads its frame slab once,
+// keeps it in registers, and for member A then member B forms the product
+// with that member's template spectrum in LDS, runs the inverse passes and
+// stores that member's needed rows.  Members share the plan, tiles and slabs;
+// each has its own call state (channel count), template spectrum, product
+// spectra and tile extents.  Full-height tiles only (no shrunk band).
+// Global-address-space views of the members' pointers: they arrive inside a
+// by-value struct, and as generic (flat) pointers their loads and stores
+// could alias the LDS slab and would be kept in order with its accesses.
+typedef float gfv2 __attribute__((ext_vector_type(2)));
+typedef const gfv2 __attribute__((address_space(1)))* gcf2p;
+typedef gfv2 __attribute__((address_space(1)))* gf2p;
+
+template <int LOG2N>
+__global__ __launch_bounds__(256) void k_cols_pair(const float2* __restrict__ src, int64_t tileStride,
+                                                   int64_t chanStride, TileGrid g, const float2* __restrict__ tri,
+                                                   int64_t tchanStride, int ntiles, int nslabs, ColPairMember A,
+                                                   ColPairMember B) {
+  constexpr int n = 1 << LOG2N, CG = ColCG<LOG2N>::value, LCG = ceil_log2c(CG), NT = 256;
+  constexpr int TOT = n * CG, PER = (TOT + NT - 1) / NT;
+  extern __shared__ float2 a[];
+  const int L = xcd_remap(blockIdx.x, gridDim.x);
+  const int t = L % ntiles, rest = L / ntiles;
+  const int slab = rest % nslabs, k = rest / nslabs;
+  const int c0 = slab * CG;
+  const bool onA = !A.st->flat && k < A.st->cn, onB = !B.st->flat && k < B.st->cn;
+  if (c0 >= g.hw || !(onA || onB)) return;
+  const int tid = threadIdx.x, hwPad = g.hwPad;
+  const int ncol = (g.hw - c0) < CG ? (g.hw - c0) : CG;
+  const float2* sp = src + (int64_t)t * tileStride + (int64_t)k * chanStride + c0;
+  // the frame slab, once: element u of this thread is row (u NT + tid) >> LCG
+  float2 x[PER];
+#pragma unroll
+  for (int u = 0; u < PER; u++) {
+    const int idx = u * NT + tid, r = idx >> LCG, c = idx & (CG - 1);
+    x[u] = make_float2(0.f, 0.f);
+    if ((TOT % NT == 0 || idx < TOT) && c < ncol) x[u] = sp[(int64_t)r * hwPad + c];
   }
+  auto member = [&](const ColPairMember& M) {
+    const gcf2p ts = (gcf2p)M.tspec + (int64_t)k * tchanStride + c0;
+    // template values in batches of LB, all loads of a batch before its
+    // products are stored (so they are in flight together)
+    constexpr int LB = PER < 16 ? PER : 16;
 #pragma unroll
-  for (int it = 0; it < ITL; it++) {
-    const int gi = tid + it * NT, v = gi & (CG - 1), gg = gi >> LCG;
-    const int base = grp_base<LOL, NBL>(gg);
-    if (GL % NT == 0 || gi < GL) {
-      float2 r[EL];
+    for (int b0 = 0; b0 < PER; b0 += LB) {
+      float2 w[LB];
 #pragma unroll
-      for (int e = 0; e < EL; e++) r[e] = a[(base | (e << LOL)) * CG + v];
-      bfly_regs<LOG2N, LOL, HEADL, NSTL>(r, gg & ((1 << LOL) - 1), tri, true);
-      if (v < ncol)
-#pragma unroll
-        for (int e = 0; e < EL; e++) {
-          const int y = base | (e << LOL);
-          if (y < need) dp[(int64_t)y * hwPad + v] = r[e];
+      for (int u = 0; u < LB; u++) {
+        const int idx = (b0 + u) * NT + tid, r = idx >> LCG, c = idx & (CG - 1);
+        w[u] = make_float2(0.f, 0.f);
+        if ((TOT % NT == 0 || idx < TOT) && c < ncol) {
+          const gfv2 tv = ts[(int64_t)r * hwPad + c];
+          w[u] = make_float2(tv.x, tv.y);
         }
+      }
+#pragma unroll
+      for (int u = 0; u < LB; u++) {
+        const int idx = (b0 + u) * NT + tid, r = idx >> LCG, c = idx & (CG - 1);
+        if (TOT % NT == 0 || idx < TOT) {
+          const float2 xv = x[b0 + u];
+          const float p0 = xv.x * w[u].x, p1 = xv.y * w[u].y, p2 = xv.y * w[u].x, p3 = xv.x * w[u].y;
+          a[bitrev_idx(r, LOG2N) * CG + c] = make_float2(p0 + p1, p2 - p3);
+        }
+      }
     }
-  }
+    __syncthreads();
+    TileGrid gm = g;
+    gm.extW = M.extW;
+    gm.extH = M.extH;
+    int x0, y0, bw, bh;
+    tile_rect(gm, t, x0, y0, bw, bh);
+    int need = (bh + 1) & ~1;
+    if (need > n) need = n;
+    const gf2p dp = (gf2p)M.dst + (int64_t)t * tileStride + (int64_t)k * chanStride + c0;
+    col_inverse_store<LOG2N>(a, tri, dp, hwPad, ncol, need);
+  };
+  if (onA) member(A);
+  if (onA && onB) __syncthreads();  // the slab is rewritten for B
+  if (onB) member(B);
 }
 
 // One band of tiles of a column launch: its twiddles, template spectrum and
@@ -825,6 +934,32 @@ void launch_cols(ColMode mode, const float2* src, float2* dst, int64_t srcTileSt
   if (nshr > 0)
     cols_dispatch(g.log2H2, mode, src, dst, srcTileStride, dstTileStride, chanStride, cnMax, st, g, pt.triH2, tspec2,
                   tchanStride2, scale, nfull, nshr, s);
+}
+
+template <int LOG2N>
+static void cols_pair_n(const float2* src, int64_t tileStride, int64_t chanStride, int cnMax, const TileGrid& g,
+                        const float2* tri, int64_t tchanStride, int ntiles, const ColPairMember& A,
+                        const ColPairMember& B, hipStream_t s) {
+  constexpr int CG = ColCG<LOG2N>::value;
+  const size_t lds = sizeof(float2) * (size_t)(1 << LOG2N) * CG;
+  const int nslabs = (g.hw + CG - 1) / CG;
+  hipLaunchKernelGGL(k_cols_pair<LOG2N>, dim3(nslabs * ntiles * cnMax), dim3(256), lds, s, src, tileStride,
+                     chanStride, g, tri, tchanStride, ntiles, nslabs, A, B);
+}
+
+bool launch_cols_pair(const float2* src, int64_t tileStride, int64_t chanStride, int cnMax, const TileGrid& g,
+                      const PlanTables& pt, int64_t tchanStride, int ntiles, const ColPairMember& A,
+                      const ColPairMember& B, hipStream_t s) {
+  if (g.dftH2 != 0) return false;  // full-height tiles only
+#define CVM_CP(L)                                                                                   \
+  case L:                                                                                           \
+    cols_pair_n<L>(src, tileStride, chanStride, cnMax, g, pt.triH, tchanStride, ntiles, A, B, s); \
+    return true
+  switch (g.log2H) {
+    CVM_CP(6); CVM_CP(7); CVM_CP(8); CVM_CP(9); CVM_CP(10); CVM_CP(11);
+    default: return false;
+  }
+#undef CVM_CP
 }
 
 // shrinkInto impl.go:591-610: the shrunk band's template spectrum is the

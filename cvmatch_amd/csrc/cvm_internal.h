@@ -116,6 +116,21 @@ void launch_twiddles(float2* tri, int32_t* brev, int n, hipStream_t s);
 // Spectrum column modes.
 enum ColMode { COL_TSPEC = 0, COL_CORR = 1, COL_FWD = 2, COL_MULINV = 3 };
 
+// One member of a paired column launch (two Fleet members of one bucket,
+// COL_MULINV, full-height tiles): its call state, template spectrum, product
+// spectra (same layout as the frame's) and result extents.
+struct ColPairMember {
+  const CallState* st;
+  const float2* tspec;
+  float2* dst;
+  int extW, extH;
+};
+// false when the shape has no paired kernel (a shrunk band, or a column
+// height outside 2^6..2^11): launch the members one by one instead.
+bool launch_cols_pair(const float2* src, int64_t tileStride, int64_t chanStride, int cnMax, const TileGrid& g,
+                      const PlanTables& pt, int64_t tchanStride, int ntiles, const ColPairMember& A,
+                      const ColPairMember& B, hipStream_t s);
+
 // Forward packed row FFTs of every loaded row pair of every tile/channel.
 // spec layout: [tile][channel][dftH][hwPad] complex (tile stride given).
 // With st == nullptr all cnMax channels are transformed and the call state

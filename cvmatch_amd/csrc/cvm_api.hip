@@ -225,6 +225,11 @@ struct Engine {
   CallState* st = nullptr;
   Res6* res_d = nullptr;
   Res6* res_h = nullptr;
+  // the Match family's six-tuple in coherent host memory mapped for the
+  // device: the final reduction writes it there, so no copy follows the call
+  // (null when the mapping is unavailable: res_d and a copy instead)
+  Res6* rmap_h = nullptr;
+  Res6* rmap_d = nullptr;
   bool prof = false;
   std::vector<hipEvent_t> evpool;
   std::vector<ProfStage> pending;
@@ -243,6 +248,16 @@ struct Engine {
     HIPCHK(hipMalloc(&st, sizeof(CallState)));
     HIPCHK(hipMalloc(&res_d, sizeof(Res6) * 2));
     HIPCHK(hipHostMalloc(&res_h, sizeof(Res6) * 2, hipHostMallocDefault));
+    if (hipHostMalloc((void**)&rmap_h, sizeof(Res6), hipHostMallocCoherent | hipHostMallocMapped) == hipSuccess) {
+      void* dp = nullptr;
+      if (hipHostGetDevicePointer(&dp, rmap_h, 0) == hipSuccess && dp) {
+        rmap_d = (Res6*)dp;
+      } else {
+        (void)hipHostFree(rmap_h);
+        rmap_h = nullptr;
+      }
+    }
+    (void)hipGetLastError();
     ready = true;
     return 0;
   }
@@ -541,10 +556,18 @@ static int run_match(Engine& E, hipStream_t s, const uint8_t* d_img, int64_t ist
     HIPCHK(hipEventRecord(E.join2, q));
     return 0;
   };
-  return corr_and_normalize(E, s, E.st, d_img, istride, iw, ih, tw, th, step, cnMax, ts, pt, d_map, E.res_d, &side);
+  return corr_and_normalize(E, s, E.st, d_img, istride, iw, ih, tw, th, step, cnMax, ts, pt, d_map,
+                            E.rmap_d ? E.rmap_d : E.res_d, &side);
 }
 
-static int finish_result(Engine& E, hipStream_t s, cvm_result* out) {
+// mapped: the call's result went to E.rmap (run_match), else to E.res_d.
+static int finish_result(Engine& E, hipStream_t s, cvm_result* out, bool mapped = false) {
+  if (mapped && E.rmap_d) {
+    HIPCHK(hipStreamSynchronize(s));
+    E.stage_collect();
+    memcpy(out, E.rmap_h, sizeof(cvm_result));
+    return 0;
+  }
   HIPCHK(hipMemcpyAsync(E.res_h, E.res_d, sizeof(Res6), hipMemcpyDeviceToHost, s));
   HIPCHK(hipStreamSynchronize(s));
   E.stage_collect();
@@ -652,7 +675,7 @@ static int match_host(const uint8_t* img, int64_t istride, int iw, int ih, const
     }
     *pk->count = tot;
   }
-  return finish_result(E, s, out);
+  return finish_result(E, s, out, !direct);
 }
 
 
@@ -1696,7 +1719,7 @@ int cvm_match_gray(const uint8_t* img, int64_t istride, int32_t iw, int32_t ih, 
     }
   }
   CHK(run_match(E, s, gi, gis, iw, ih, gt, gts, tw, th, 1, 1, nullptr));
-  return finish_result(E, s, out);
+  return finish_result(E, s, out, true);
 }
 
 static int match_device(const uint8_t* d_img, int64_t istride, int32_t iw, int32_t ih, const uint8_t* d_tpl,
@@ -1751,7 +1774,7 @@ static int match_device(const uint8_t* d_img, int64_t istride, int32_t iw, int32
   if (E.prof) {  // stage events need the eager launches
     E.drop_graph();
     CHK(run());
-    return finish_result(E, s, out);
+    return finish_result(E, s, out, !direct);
   }
   Engine::CallKey key{d_img, d_tpl, d_map, istride, tstride, iw, ih, tw, th, cn, step, direct ? 1 : 0, s,
                       g_alloc_epoch.load()};
@@ -1761,12 +1784,12 @@ static int match_device(const uint8_t* d_img, int64_t istride, int32_t iw, int32
   }
   if (E.gexec) {
     HIPCHK(hipGraphLaunch(E.gexec, s));
-    return finish_result(E, s, out);
+    return finish_result(E, s, out, !direct);
   }
   constexpr int kGraphAfter = 2;
   if (++E.gseen < kGraphAfter) {
     CHK(run());  // eager (the first one allocates what the call needs)
-    return finish_result(E, s, out);
+    return finish_result(E, s, out, !direct);
   }
   // a run of repeats with nothing reallocated since: capture the call
   hipGraph_t graph = nullptr;
@@ -1779,7 +1802,7 @@ static int match_device(const uint8_t* d_img, int64_t istride, int32_t iw, int32
     (void)hipGetLastError();
     if (rc != 0) return rc;
     CHK(run());  // run eagerly instead
-    return finish_result(E, s, out);
+    return finish_result(E, s, out, !direct);
   }
   hipError_t ie = hipGraphInstantiate(&E.gexec, graph, nullptr, nullptr, 0);
   (void)hipGraphDestroy(graph);
@@ -1787,10 +1810,10 @@ static int match_device(const uint8_t* d_img, int64_t istride, int32_t iw, int32
     E.gexec = nullptr;
     (void)hipGetLastError();
     CHK(run());
-    return finish_result(E, s, out);
+    return finish_result(E, s, out, !direct);
   }
   HIPCHK(hipGraphLaunch(E.gexec, s));
-  return finish_result(E, s, out);
+  return finish_result(E, s, out, !direct);
 }
 
 extern "C" {

@@ -13,7 +13,9 @@ Coverage per checked frame:
     rounded to float32 bit for bit, and the six-tuple is its first-occurrence
     extrema.
 The plain float32 path (the reference's op sequence) is reported beside it:
-max |delta| of its scores against FP64 at the same windows.
+max |delta| of its scores against FP64 at the same windows.  The plain
+FindAll itself is checked bit for bit against the oracle's FindAll on the same
+stream with 30 members (several normalisation batches).
 """
 import os
 import sys
@@ -96,3 +98,17 @@ def test_direct_fleet_equals_fp64(cvm, orc, workload, fi):
             plain_delta = max(plain_delta, float(np.max(np.abs(pv.astype(np.float64) - ov))))
     say(f"[fleet-fp64] frame {fi}: direct == FP64 (float32) at {len(tpls) * (SAMPLES + 2)} windows; "
         f"plain float32 path max |delta| vs FP64 {plain_delta:.3g}")
+
+
+def test_plain_fleet_multi_batch_bitwise(cvm, orc):
+    """The plain FindAll on the 2560x1440 stream with 30 members: their maps
+    (~103 M floats) span three normalisation batches of at most 192 MiB, and
+    the bucket members go through the paired column launch; every six-tuple
+    bit for bit against the oracle's FindAll, on two frames."""
+    frames, tpls = synth.workload_fleet(frames=24, members=30)
+    fl = cvm.NewFleet(*[cvm.NewMatcher(t) for t in tpls])
+    for fi in (0, 23):
+        frame = np.ascontiguousarray(frames[fi])
+        say(f"[fleet] plain FindAll, 30 members, frame {fi}")
+        got = [tuple(r) for r in fl.find_all(frame)]
+        assert got == orc.fleet_find_all(frame, tpls), fi

@@ -433,7 +433,7 @@ def test_random_large_templates_bitwise(cvm, orc, case):
          orc.match_u8(img, istride, iw, ih, tpl, tstride, tw, th, cn, step, threads=8))
 
 
-@pytest.mark.parametrize("seed", [21, 22, 23])
+@pytest.mark.parametrize("seed", [21, 22, 23, 24, 25, 26, 27, 28])
 def test_random_fleets_bitwise(cvm, orc, seed):
     """Random fleets against the oracle's FindAll: 6-12 members of random
     sizes (1 to 220 per side), some with varying alpha (their own cn = 4 on a

@@ -225,9 +225,10 @@ struct Engine {
   CallState* st = nullptr;
   Res6* res_d = nullptr;
   Res6* res_h = nullptr;
-  // the Match family's six-tuple in coherent host memory mapped for the
-  // device: the final reduction writes it there, so no copy follows the call
-  // (null when the mapping is unavailable: res_d and a copy instead)
+  // a single-result call's six-tuple (Match family, Find) in coherent host
+  // memory mapped for the device: the final reduction writes it there, so
+  // no copy follows the call (null when the mapping is unavailable: res_d
+  // and a copy instead)
   Res6* rmap_h = nullptr;
   Res6* rmap_d = nullptr;
   bool prof = false;
@@ -410,6 +411,10 @@ static int build_tspec(Engine& E, hipStream_t s, const CallState* st, const uint
   return 0;
 }
 
+// Where a single-result call (Match family, Find) leaves its six-tuple: the
+// mapped host result when there is one, else the device one.
+static Res6* res_out(Engine& E) { return E.rmap_d ? E.rmap_d : E.res_d; }
+
 // Correlation + normalisation + extrema for one parent against a prepared
 // template spectrum (crossCorrGo + normalizeParallelGo, impl.go:1150-1154).
 // CallState must be finalised on the stream before this runs.
@@ -557,12 +562,12 @@ static int run_match(Engine& E, hipStream_t s, const uint8_t* d_img, int64_t ist
     return 0;
   };
   return corr_and_normalize(E, s, E.st, d_img, istride, iw, ih, tw, th, step, cnMax, ts, pt, d_map,
-                            E.rmap_d ? E.rmap_d : E.res_d, &side);
+                            res_out(E), &side);
 }
 
-// mapped: the call's result went to E.rmap (run_match), else to E.res_d.
-static int finish_result(Engine& E, hipStream_t s, cvm_result* out, bool mapped = false) {
-  if (mapped && E.rmap_d) {
+// The six-tuple of a call that wrote it to res_out(E).
+static int finish_result(Engine& E, hipStream_t s, cvm_result* out) {
+  if (E.rmap_d) {
     HIPCHK(hipStreamSynchronize(s));
     E.stage_collect();
     memcpy(out, E.rmap_h, sizeof(cvm_result));
@@ -675,7 +680,7 @@ static int match_host(const uint8_t* img, int64_t istride, int iw, int ih, const
     }
     *pk->count = tot;
   }
-  return finish_result(E, s, out, !direct);
+  return finish_result(E, s, out);
 }
 
 
@@ -1380,7 +1385,7 @@ static int run_match_direct(Engine& E, hipStream_t s, const uint8_t* d_img, int6
     launch_bfrag(d_tpl, tstride, tw, th, kstep, cnMax, E.st, E.dbf.as<int8_t>(), s);
   }
   return run_direct(E, s, E.st, d_img, istride, iw, ih, tw, th, step, kstep, cnMax, E.dbf.as<int8_t>(), E.nscratch.p,
-                    true, d_map, E.res_d);
+                    true, d_map, res_out(E));
 }
 
 // B fragments of a matcher for cn channels (cached; cn known on the host).
@@ -1719,7 +1724,7 @@ int cvm_match_gray(const uint8_t* img, int64_t istride, int32_t iw, int32_t ih, 
     }
   }
   CHK(run_match(E, s, gi, gis, iw, ih, gt, gts, tw, th, 1, 1, nullptr));
-  return finish_result(E, s, out, true);
+  return finish_result(E, s, out);
 }
 
 static int match_device(const uint8_t* d_img, int64_t istride, int32_t iw, int32_t ih, const uint8_t* d_tpl,
@@ -1774,7 +1779,7 @@ static int match_device(const uint8_t* d_img, int64_t istride, int32_t iw, int32
   if (E.prof) {  // stage events need the eager launches
     E.drop_graph();
     CHK(run());
-    return finish_result(E, s, out, !direct);
+    return finish_result(E, s, out);
   }
   Engine::CallKey key{d_img, d_tpl, d_map, istride, tstride, iw, ih, tw, th, cn, step, direct ? 1 : 0, s,
                       g_alloc_epoch.load()};
@@ -1784,12 +1789,12 @@ static int match_device(const uint8_t* d_img, int64_t istride, int32_t iw, int32
   }
   if (E.gexec) {
     HIPCHK(hipGraphLaunch(E.gexec, s));
-    return finish_result(E, s, out, !direct);
+    return finish_result(E, s, out);
   }
   constexpr int kGraphAfter = 2;
   if (++E.gseen < kGraphAfter) {
     CHK(run());  // eager (the first one allocates what the call needs)
-    return finish_result(E, s, out, !direct);
+    return finish_result(E, s, out);
   }
   // a run of repeats with nothing reallocated since: capture the call
   hipGraph_t graph = nullptr;
@@ -1802,7 +1807,7 @@ static int match_device(const uint8_t* d_img, int64_t istride, int32_t iw, int32
     (void)hipGetLastError();
     if (rc != 0) return rc;
     CHK(run());  // run eagerly instead
-    return finish_result(E, s, out, !direct);
+    return finish_result(E, s, out);
   }
   hipError_t ie = hipGraphInstantiate(&E.gexec, graph, nullptr, nullptr, 0);
   (void)hipGraphDestroy(graph);
@@ -1810,10 +1815,10 @@ static int match_device(const uint8_t* d_img, int64_t istride, int32_t iw, int32
     E.gexec = nullptr;
     (void)hipGetLastError();
     CHK(run());
-    return finish_result(E, s, out, !direct);
+    return finish_result(E, s, out);
   }
   HIPCHK(hipGraphLaunch(E.gexec, s));
-  return finish_result(E, s, out, !direct);
+  return finish_result(E, s, out);
 }
 
 extern "C" {
@@ -1925,9 +1930,9 @@ static int matcher_find(cvm_matcher* m, const uint8_t* img, int64_t stride, int3
     d_img = E.img.as<uint8_t>();
   }
   if (direct)
-    CHK(matcher_find_direct_dev(E, s, m, d_img, stride, w, h, E.res_d));
+    CHK(matcher_find_direct_dev(E, s, m, d_img, stride, w, h, res_out(E)));
   else
-    CHK(matcher_find_dev(E, s, m, d_img, stride, w, h, E.res_d));
+    CHK(matcher_find_dev(E, s, m, d_img, stride, w, h, res_out(E)));
   return finish_result(E, s, out);
 }
 

@@ -1368,6 +1368,11 @@ static int run_direct(Engine& E, hipStream_t s, const CallState* st, const uint8
 static int run_match_direct(Engine& E, hipStream_t s, const uint8_t* d_img, int64_t istride, int iw, int ih,
                             const uint8_t* d_tpl, int64_t tstride, int tw, int th, int cn_mode, int step,
                             float* d_map) {
+  // the staging loads read 1 byte per pixel (gray) or 4 (RGBA): a packed
+  // 2- or 3-byte layout would be read past its rows
+  if (!((step == 1 && cn_mode == 1) || (step == 4 && (cn_mode == 0 || cn_mode == 3 || cn_mode == 4))))
+    return fail(CVM_E_BAD_ARGS, "cvmatch: the direct form takes cn=1 step=1 or cn=0/3/4 step=4 (got cn=%d step=%d)",
+                cn_mode, step);
   int cnMax = cn_mode ? cn_mode : 4;
   // with the alpha rule cn is decided on the device: stage all four bytes
   // (the fragments zero the alpha column when cn comes out 3)

@@ -169,8 +169,10 @@ int cvm_find_peaks_device(const float *d_map, int32_t w, int32_t h, float thresh
  * the FP64 oracle's (the TM_CCOEFF_NORMED formula with OpenCV's guard ladder
  * as in normOne impl.go:816-849, evaluated in binary64) rounded to float32.
  * Not bit-identical to the plain calls, which reproduce the reference's
- * float32 FFT rounding.  Applies when tw*th*cn < 131072 and tw, th <= 257;
- * other shapes return CVM_E_BAD_ARGS.  Arguments and results as in the plain
+ * float32 FFT rounding.  Applies when tw*th*cn < 131072 and tw, th <= 257,
+ * on gray (cn = 1, step = 1) or RGBA (step = 4, cn = 3, 4 or 0 for the alpha
+ * rule) pixels; other shapes and layouts return CVM_E_BAD_ARGS.  Arguments
+ * and results as in the plain
  * counterparts: matchU8 impl.go:1126-1155, Find matcher.go:83-130, FindAll
  * fleet.go:176-332 (FindAll's members are not bucketed here: each result
  * equals that member's direct Find). */

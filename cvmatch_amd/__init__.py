@@ -365,6 +365,14 @@ class Matcher:
 
     Find = find
 
+    def find_device(self, d_img, stride, w, h, stream=None):
+        """Find on a frame (or a region of one) already in device memory:
+        d_img is the device address of the view's first pixel, stride its
+        row pitch in bytes; the view has the matcher's format."""
+        out = cvm_result()
+        _check(_L().cvm_matcher_find_device(self._h, C.c_void_p(int(d_img)), stride, w, h, C.byref(out), stream))
+        return out.tuple()
+
     def __del__(self):
         h = getattr(self, "_h", None)
         if h is not None and h.value and _lib is not None:

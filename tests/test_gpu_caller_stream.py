@@ -60,6 +60,20 @@ def test_match_and_fleet_on_caller_stream(cvm):
         ptr = df.value + (y0 * 640 + x0) * 4
         got_roi = [tuple(r) for r in fl.find_all_device(ptr, 640 * 4, x1 - x0, y1 - y0, stream=stream)]
         assert got_roi == want_roi
+        say("[stream] direct FindAll and Find on the device view")
+        want_direct = [tuple(r) for r in fl.find_all(frame[y0:y1, x0:x1], direct=True)]
+        got_direct = [tuple(r) for r in fl.find_all_device(ptr, 640 * 4, x1 - x0, y1 - y0, direct=True, stream=stream)]
+        assert got_direct == want_direct
+        for s in subs:
+            m = cvm.NewMatcher(s)
+            assert m.find_device(ptr, 640 * 4, x1 - x0, y1 - y0, stream=stream) == m.find(frame[y0:y1, x0:x1])
+        # a gray Matcher on a gray device view (row pitch not a multiple of the width)
+        gray = np.zeros((400, 656), np.uint8)
+        gray[:, :640] = cvm.rgba_to_gray(frame)
+        dg = dev(gray)
+        gm = cvm.NewGrayMatcher(subs[0])
+        gptr = dg.value + y0 * 656 + x0
+        assert gm.find_device(gptr, 656, x1 - x0, y1 - y0, stream=stream) == gm.find(gray[y0:y1, x0:x1])
     finally:
         for p in bufs:
             L.cvm_device_free(p)

@@ -687,10 +687,17 @@ struct NormRow {
 // formula): num = area * corr - sum_k S_k T_k, idiff = area * s2 - sum_k S_k^2,
 // every input an exact integer below 2^53.
 __device__ __forceinline__ double score_fp64(double numI, double idiff, double s2, double aread, double varSum) {
-  const double diff2 = idiff / aread;
-  double lim = 1.1920928955078125e-06 * s2;  // 10 * 2^-23 * s2
-  if (lim > 0.5) lim = 0.5;
-  if (!(diff2 > lim)) return 0.0;
+  // The guard is fl(idiff / area) > lim with lim = min(0.5, 10 * 2^-23 * s2)
+  // >= 0.  idiff and area are exact non-negative integers, so idiff > area
+  // (a quotient above 1) always passes it and idiff == 0 (a quotient of +0)
+  // always fails it: only 0 < idiff <= area needs the division.
+  if (!(idiff > aread)) {
+    if (idiff == 0.0) return 0.0;
+    const double diff2 = idiff / aread;
+    double lim = 1.1920928955078125e-06 * s2;  // 10 * 2^-23 * s2
+    if (lim > 0.5) lim = 0.5;
+    if (!(diff2 > lim)) return 0.0;
+  }
   const double denI = sqrt(idiff) * sqrt(varSum);
   const double an = fabs(numI);
   if (an < denI) return numI / denI;

@@ -444,3 +444,20 @@ def test_fp64_map_fft_at_frame_scale(orc):
     ys = rng.integers(0, got.shape[0], 64).astype(np.int32)
     want = orc.fp64_points(parent, 2560 * 4, 2560, 1440, sub, 128 * 4, 128, 128, 4, 4, xs, ys)
     assert np.array_equal(got[ys, xs].view(np.uint64), want.view(np.uint64))
+
+
+def test_guard_shortcut_identity():
+    """The FP64 scoring skips the guard's division fl(idiff / area) > lim
+    (lim = min(0.5, 10 * 2^-23 * s2)) when idiff > area (always passes) or
+    idiff == 0 (always fails); the identity, on exact integers in float64."""
+    rng = np.random.default_rng(17)
+    area = rng.integers(1, 131072, 200000).astype(np.float64)
+    s2 = rng.integers(0, 1 << 40, 200000).astype(np.float64)
+    pick = rng.integers(0, 4, 200000)
+    idiff = np.where(pick == 0, 0.0, np.where(pick == 1, area, np.where(
+        pick == 2, np.floor(area * rng.random(200000)), rng.integers(0, 1 << 44, 200000).astype(np.float64))))
+    s2[:1000] = 0.0
+    lim = np.minimum(0.5, 1.1920928955078125e-06 * s2)
+    full = idiff / area > lim
+    short = (idiff > area) | ((idiff != 0) & (idiff / area > lim))
+    assert np.array_equal(full, short)

@@ -14,6 +14,7 @@
 #include <algorithm>
 #include <array>
 #include <atomic>
+#include <condition_variable>
 #include <deque>
 #include <functional>
 #include <map>
@@ -132,11 +133,11 @@ static int64_t stats_cap(int cn) {
 }
 
 // Kernel limits: row transforms keep two dftW rows in LDS and
-// column slabs dftH x CG complex, both <= 64 KiB; the normalisation strip
-// holds 256 + tw - 1 <= 2048 columns.
+// column slabs dftH x CG complex, both <= 64 KiB.  Templates are at most
+// dftW wide, which the normalisation's strips take (up to 256 + 4096 - 1
+// columns).
 constexpr int kMaxFftW = 4096;
 constexpr int kMaxFftH = 8192;
-constexpr int kMaxNormTw = 2048 - 255;
 
 static int round_up(int v, int m) { return (v + m - 1) / m * m; }
 
@@ -187,9 +188,76 @@ struct ProfStage {
   hipEvent_t a, b;
 };
 
+// Process-wide state: the per-size twiddle tables (generated on the device
+// once, read-only afterwards), the pool of call contexts and the stage
+// timings.  Nothing here is touched by a call's kernels except the tables.
+struct Ctx;
 struct Engine {
-  std::recursive_mutex mu;
+  std::mutex mu;  // ready, tables, the context pool
   bool ready = false;
+  hipStream_t tstream = nullptr;  // table generation
+  std::map<int, FftTables> tables;
+  std::vector<Ctx*> idle;  // most recently released last
+  int created = 0;
+  std::condition_variable cv;
+  std::atomic<bool> prof{false};
+  std::mutex pmu;
+  std::map<std::string, std::pair<double, long long>> prof_acc;
+
+  int init() {
+    if (ready) return 0;
+    int ndev = 0;
+    HIPCHK(hipGetDeviceCount(&ndev));
+    if (ndev < 1) return fail(CVM_E_HIP, "cvmatch: no HIP device available");
+    HIPCHK(hipStreamCreateWithFlags(&tstream, hipStreamNonBlocking));
+    ready = true;
+    return 0;
+  }
+
+  int tab(int n, const FftTables** out) {
+    std::lock_guard<std::mutex> lk(mu);
+    CHK(init());
+    auto it = tables.find(n);
+    if (it == tables.end()) {
+      FftTables t;
+      t.n = n;
+      int len = tri_len(n);
+      HIPCHK(hipMalloc(&t.tri, sizeof(float2) * (size_t)std::max(len, 1)));
+      HIPCHK(hipMalloc(&t.brev, sizeof(int32_t) * (size_t)n));
+      launch_twiddles(t.tri, t.brev, n, tstream);
+      HIPCHK(hipGetLastError());
+      // tables are shared by every stream a caller may pass in
+      HIPCHK(hipStreamSynchronize(tstream));
+      it = tables.emplace(n, t).first;
+    }
+    *out = &it->second;
+    return 0;
+  }
+
+  int acquire(Ctx** out);
+  void release(Ctx* x);
+  void add_times(const std::vector<std::pair<std::string, float>>& t) {
+    std::lock_guard<std::mutex> lk(pmu);
+    for (auto& p : t) {
+      auto& acc = prof_acc[p.first];
+      acc.first += p.second;
+      acc.second += 1;
+    }
+  }
+};
+
+static Engine& engine() {
+  static Engine* e = new Engine();
+  return *e;
+}
+
+// One call context: the streams, events and grow-only scratch of one call in
+// flight.  A call takes a context from the engine's pool for its duration
+// (calls return with their work complete, so the next user of the context
+// finds its buffers free); a frame stream owns one of its own, since its
+// frames stay in flight on it between calls.  Concurrent calls therefore
+// share no scratch and overlap on the device.
+struct Ctx {
   hipStream_t stream = nullptr;
   // side stream of a MatchMap call, forked from the caller's stream and
   // joined back into it twice: before the column pass (call state, template
@@ -220,7 +288,6 @@ struct Engine {
     gexec = nullptr;
     gseen = 0;
   }
-  std::map<int, FftTables> tables;
   DevBuf img, tpl, spec, tspec, tspec2, corr, parts, gray0, gray1, mapbuf, nscratch, dbf, pkscr, pkout;
   CallState* st = nullptr;
   Res6* res_d = nullptr;
@@ -231,16 +298,10 @@ struct Engine {
   // and a copy instead)
   Res6* rmap_h = nullptr;
   Res6* rmap_d = nullptr;
-  bool prof = false;
   std::vector<hipEvent_t> evpool;
   std::vector<ProfStage> pending;
-  std::map<std::string, std::pair<double, long long>> prof_acc;
 
   int init() {
-    if (ready) return 0;
-    int ndev = 0;
-    HIPCHK(hipGetDeviceCount(&ndev));
-    if (ndev < 1) return fail(CVM_E_HIP, "cvmatch: no HIP device available");
     HIPCHK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
     HIPCHK(hipStreamCreateWithFlags(&side, hipStreamNonBlocking));
     HIPCHK(hipEventCreateWithFlags(&fork, hipEventDisableTiming));
@@ -259,27 +320,27 @@ struct Engine {
       }
     }
     (void)hipGetLastError();
-    ready = true;
     return 0;
   }
 
-  int tab(int n, const FftTables** out) {
-    auto it = tables.find(n);
-    if (it == tables.end()) {
-      FftTables t;
-      t.n = n;
-      int len = tri_len(n);
-      HIPCHK(hipMalloc(&t.tri, sizeof(float2) * (size_t)std::max(len, 1)));
-      HIPCHK(hipMalloc(&t.brev, sizeof(int32_t) * (size_t)n));
-      launch_twiddles(t.tri, t.brev, n, stream);
-      HIPCHK(hipGetLastError());
-      // tables are shared by every stream a caller may pass in
-      HIPCHK(hipStreamSynchronize(stream));
-      it = tables.emplace(n, t).first;
-    }
-    *out = &it->second;
-    return 0;
+  // Frees everything; the context's streams must be idle.
+  void destroy() {
+    drop_graph();
+    for (DevBuf* b : {&img, &tpl, &spec, &tspec, &tspec2, &corr, &parts, &gray0, &gray1, &mapbuf, &nscratch, &dbf,
+                      &pkscr, &pkout})
+      b->release();
+    if (st) (void)hipFree(st);
+    if (res_d) (void)hipFree(res_d);
+    if (res_h) (void)hipHostFree(res_h);
+    if (rmap_h) (void)hipHostFree(rmap_h);
+    for (hipEvent_t e : evpool) (void)hipEventDestroy(e);
+    for (hipEvent_t e : {fork, join, join2})
+      if (e) (void)hipEventDestroy(e);
+    for (hipStream_t q : {stream, side})
+      if (q) (void)hipStreamDestroy(q);
   }
+
+  int tab(int n, const FftTables** out) { return engine().tab(n, out); }
 
   hipEvent_t ev() {
     if (!evpool.empty()) {
@@ -293,7 +354,7 @@ struct Engine {
   }
 
   int stage_begin(const char* name, hipStream_t s) {
-    if (!prof) return -1;
+    if (!engine().prof.load()) return -1;
     ProfStage p;
     p.name = name;
     p.a = ev();
@@ -309,31 +370,80 @@ struct Engine {
   }
   // after the stream has been synchronised
   void stage_collect() {
+    if (pending.empty()) return;
+    std::vector<std::pair<std::string, float>> t;
     for (auto& p : pending) {
       float ms = 0.f;
-      if (hipEventElapsedTime(&ms, p.a, p.b) == hipSuccess) {
-        auto& acc = prof_acc[p.name];
-        acc.first += ms;
-        acc.second += 1;
-      }
+      if (hipEventElapsedTime(&ms, p.a, p.b) == hipSuccess) t.emplace_back(p.name, ms);
       evpool.push_back(p.a);
       evpool.push_back(p.b);
     }
     pending.clear();
+    engine().add_times(t);
   }
 };
 
-static Engine& engine() {
-  static Engine* e = new Engine();
-  return *e;
+// Up to kMaxCtx contexts exist at once; a caller beyond that waits for one.
+constexpr int kMaxCtx = 8;
+
+int Engine::acquire(Ctx** out) {
+  std::unique_lock<std::mutex> lk(mu);
+  CHK(init());
+  for (;;) {
+    if (!idle.empty()) {
+      *out = idle.back();  // the most recently used: its buffers and graph are warm
+      idle.pop_back();
+      return 0;
+    }
+    if (created < kMaxCtx) break;
+    cv.wait(lk);
+  }
+  created++;
+  lk.unlock();
+  std::unique_ptr<Ctx> x(new Ctx());
+  int rc = x->init();
+  if (rc) {
+    x->destroy();
+    std::lock_guard<std::mutex> g(mu);
+    created--;
+    cv.notify_one();
+    return rc;
+  }
+  *out = x.release();
+  return 0;
 }
 
+void Engine::release(Ctx* x) {
+  {
+    std::lock_guard<std::mutex> lk(mu);
+    idle.push_back(x);
+  }
+  cv.notify_one();
+}
+
+// A pool context held for one call.
+struct Lease {
+  Ctx* x = nullptr;
+  int rc = 0;
+  Lease() { rc = engine().acquire(&x); }
+  ~Lease() {
+    if (x) engine().release(x);
+  }
+  Lease(const Lease&) = delete;
+  Lease& operator=(const Lease&) = delete;
+};
+
+#define LEASE(name)      \
+  Lease lease_;          \
+  CHK(lease_.rc);        \
+  Ctx& name = *lease_.x
+
 struct Stage {
-  Engine& E;
+  Ctx& X;
   int i;
   hipStream_t s;
-  Stage(Engine& e, const char* name, hipStream_t st) : E(e), s(st) { i = E.stage_begin(name, st); }
-  ~Stage() { E.stage_end(i, s); }
+  Stage(Ctx& x, const char* name, hipStream_t st) : X(x), s(st) { i = X.stage_begin(name, st); }
+  ~Stage() { X.stage_end(i, s); }
 };
 
 // ------------------------------------------------------------ pipeline --
@@ -349,11 +459,11 @@ struct TplSpec {
   float2* spec2 = nullptr;
 };
 
-static int plan_tables(Engine& E, const Plan& p, int dh2, PlanTables& pt) {
+static int plan_tables(Ctx& X, const Plan& p, int dh2, PlanTables& pt) {
   const FftTables *tw, *th, *th2 = nullptr;
-  CHK(E.tab(p.dftW, &tw));
-  CHK(E.tab(p.dftH, &th));
-  if (dh2) CHK(E.tab(dh2, &th2));
+  CHK(X.tab(p.dftW, &tw));
+  CHK(X.tab(p.dftH, &th));
+  if (dh2) CHK(X.tab(dh2, &th2));
   pt.triW = tw->tri;
   pt.brevW = tw->brev;
   pt.triH = th->tri;
@@ -364,12 +474,10 @@ static int plan_tables(Engine& E, const Plan& p, int dh2, PlanTables& pt) {
 }
 
 static int check_limits(const Plan& p, int tw, int rw, int rh) {
+  (void)tw;
   if (p.dftW > kMaxFftW || p.dftH > kMaxFftH)
     return fail(CVM_E_UNSUPPORTED, "cvmatch: transform %dx%d exceeds the kernels' LDS limits (%d x %d)", p.dftW,
                 p.dftH, kMaxFftW, kMaxFftH);
-  if (tw > kMaxNormTw)
-    return fail(CVM_E_UNSUPPORTED, "cvmatch: template width %d exceeds the normalisation strip (%d)", tw,
-                kMaxNormTw);
   if ((long long)rw * rh >= (1LL << 31))
     return fail(CVM_E_UNSUPPORTED, "cvmatch: result map of %dx%d exceeds 2^31 elements", rw, rh);
   return 0;
@@ -378,10 +486,10 @@ static int check_limits(const Plan& p, int tw, int rw, int rh) {
 // Template spectrum on plan p (buildTSpecSet impl.go:520-561): forward
 // transform of the template at (0,0), scaled by 1/(dftW*dftH), plus the
 // shrunk band (shrinkInto impl.go:591-610).  dst buffers sized by caller.
-static int build_tspec(Engine& E, hipStream_t s, const CallState* st, const uint8_t* d_tpl, int64_t tstride, int tw,
+static int build_tspec(Ctx& X, hipStream_t s, const CallState* st, const uint8_t* d_tpl, int64_t tstride, int tw,
                        int th, int step, int cnMax, const Plan& p, int dh2, const PlanTables& pt, float2* spec,
                        float2* spec2) {
-  Stage S(E, "tspec", s);
+  Stage S(X, "tspec", s);
   int hwPad = round_up(p.hw, 16);
   int64_t tchan = (int64_t)p.dftH * hwPad, tchan2 = (int64_t)dh2 * hwPad;
   TileGrid gt{};
@@ -413,7 +521,7 @@ static int build_tspec(Engine& E, hipStream_t s, const CallState* st, const uint
 
 // Where a single-result call (Match family, Find) leaves its six-tuple: the
 // mapped host result when there is one, else the device one.
-static Res6* res_out(Engine& E) { return E.rmap_d ? E.rmap_d : E.res_d; }
+static Res6* res_out(Ctx& X) { return X.rmap_d ? X.rmap_d : X.res_d; }
 
 // Correlation + normalisation + extrema for one parent against a prepared
 // template spectrum (crossCorrGo + normalizeParallelGo, impl.go:1150-1154).
@@ -423,14 +531,14 @@ static Res6* res_out(Engine& E) { return E.rmap_d ? E.rmap_d : E.res_d; }
 // records join when done: the parent's row transforms (which need none of
 // them) run first, then s waits for join.  Otherwise all of them are ready
 // on s, except the partials, built here.
-// side (may be null): enqueues the call's side-stream work on E.side and
-// records E.join once the call state and the template spectrum are ready and
-// E.join2 once the normalisation partials are.  It runs after the parent's row pass has
+// side (may be null): enqueues the call's side-stream work on X.side and
+// records X.join once the call state and the template spectrum are ready and
+// X.join2 once the normalisation partials are.  It runs after the parent's row pass has
 // been submitted on s, so that pass (which needs none of it) is first in
 // line and overlaps the side work.
 using SideWork = std::function<int()>;
 
-static int corr_and_normalize(Engine& E, hipStream_t s, const CallState* st, const uint8_t* d_img, int64_t istride,
+static int corr_and_normalize(Ctx& X, hipStream_t s, const CallState* st, const uint8_t* d_img, int64_t istride,
                               int iw, int ih, int tw, int th, int step, int cnMax, const TplSpec& ts,
                               const PlanTables& pt, float* d_map, Res6* d_res, const SideWork* side) {
   const Plan& p = ts.p;
@@ -458,132 +566,164 @@ static int corr_and_normalize(Engine& E, hipStream_t s, const CallState* st, con
   int64_t chan = (int64_t)p.dftH * hwPad, tileStride = chan * cnMax;
   int64_t tchan = chan, tchan2 = (int64_t)ts.dh2 * hwPad;
   // every buffer first: no allocation lands between fork and join
-  CHK(E.spec.ensure(sizeof(float2) * (size_t)tileStride * ntiles));
-  float2* spec = E.spec.as<float2>();
+  CHK(X.spec.ensure(sizeof(float2) * (size_t)tileStride * ntiles));
+  float2* spec = X.spec.as<float2>();
   float* corr = d_map;
   if (!corr) {
-    CHK(E.corr.ensure(sizeof(float) * (size_t)rw * rh));
-    corr = E.corr.as<float>();
+    CHK(X.corr.ensure(sizeof(float) * (size_t)rw * rh));
+    corr = X.corr.as<float>();
   }
   int nparts = norm_partials_count(rw, rh, tw, th, cnMax);
-  CHK(E.parts.ensure(sizeof(Extrema) * (size_t)nparts));
-  if (!side) CHK(E.nscratch.ensure(norm_scratch_bytes(iw, ih, tw, th, cnMax)));
+  CHK(X.parts.ensure(sizeof(Extrema) * (size_t)nparts));
+  if (!side) CHK(X.nscratch.ensure(norm_scratch_bytes(iw, ih, tw, th, cnMax)));
   if (side) {
-    HIPCHK(hipEventRecord(E.fork, s));
-    HIPCHK(hipStreamWaitEvent(E.side, E.fork, 0));
+    HIPCHK(hipEventRecord(X.fork, s));
+    HIPCHK(hipStreamWaitEvent(X.side, X.fork, 0));
   }
   {
-    Stage S(E, "row_fwd", s);
+    Stage S(X, "row_fwd", s);
     launch_row_fwd(d_img, istride, step, cnMax, side ? nullptr : st, g, pt, spec, tileStride, chan, ntiles, s);
   }
   if (side) {
     CHK((*side)());
-    HIPCHK(hipStreamWaitEvent(s, E.join, 0));
+    HIPCHK(hipStreamWaitEvent(s, X.join, 0));
   }
   {
-    Stage S(E, "cols", s);
+    Stage S(X, "cols", s);
     launch_cols(COL_CORR, spec, spec, tileStride, tileStride, chan, cnMax, st, g, pt, ts.spec, ts.spec2, tchan,
                 tchan2, 1.0f, ntiles, s);
   }
   {
-    Stage S(E, "row_inv", s);
+    Stage S(X, "row_inv", s);
     launch_row_inv(spec, tileStride, chan, st, cnMax, g, pt, corr, rw, ntiles, s);
   }
   if (side) {
-    HIPCHK(hipStreamWaitEvent(s, E.join2, 0));
+    HIPCHK(hipStreamWaitEvent(s, X.join2, 0));
   } else {
-    Stage S(E, "norm_prep", s);
-    launch_norm_prep(d_img, istride, iw, ih, step, tw, th, cnMax, st, E.nscratch.p, s);
+    Stage S(X, "norm_prep", s);
+    launch_norm_prep(d_img, istride, iw, ih, step, tw, th, cnMax, st, X.nscratch.p, s);
   }
   {
-    Stage S(E, "normalize", s);
-    launch_normalize(d_img, istride, iw, ih, step, tw, th, rw, rh, cnMax, st, corr, d_map, E.parts.as<Extrema>(),
-                     E.nscratch.p, s);
+    Stage S(X, "normalize", s);
+    launch_normalize(d_img, istride, iw, ih, step, tw, th, rw, rh, cnMax, st, corr, d_map, X.parts.as<Extrema>(),
+                     X.nscratch.p, s);
   }
   {
-    Stage S(E, "reduce", s);
-    launch_reduce(E.parts.as<Extrema>(), nparts, st, rw, d_res, nullptr, rh, s);
+    Stage S(X, "reduce", s);
+    launch_reduce(X.parts.as<Extrema>(), nparts, st, rw, d_res, nullptr, rh, s);
   }
   HIPCHK(hipGetLastError());
   return 0;
 }
 
-static int check_args(int64_t istride, int iw, int ih, int64_t tstride, int tw, int th, int cn, int step) {
+// matchU8's argument checks impl.go:1127-1133: the shape, then the
+// exact-statistics bound of the channel count.
+static int check_shape(int64_t istride, int iw, int ih, int64_t tstride, int tw, int th, int cn, int step) {
   if (cn < 1 || cn > 4 || step < cn || tw < 1 || th < 1 || tw > iw || th > ih || istride < (int64_t)iw * step ||
       tstride < (int64_t)tw * step)
     return fail(CVM_E_BAD_ARGS, "cvmatch: bad match arguments (%dx%d in %dx%d, cn=%d step=%d)", tw, th, iw, ih, cn,
                 step);
+  return 0;
+}
+
+static int check_cap(int tw, int th, int cn) {
   if ((int64_t)tw * th > stats_cap(cn))
     return fail(CVM_E_STATS_CAP, "cvmatch: template area %dx%d exceeds the exact-statistics bound for cn=%d", tw, th,
                 cn);
   return 0;
 }
 
+static int check_args(int64_t istride, int iw, int ih, int64_t tstride, int tw, int th, int cn, int step) {
+  CHK(check_shape(istride, iw, ih, tstride, tw, th, cn, step));
+  return check_cap(tw, th, cn);
+}
+
+// Match's alpha rule (cvmatch.go:82-86) for templates above statsCap(4):
+// only cn = 3 (both alpha planes constant) can take them (impl.go:1131), so
+// the rule is settled here, before the call, from device scans of both
+// alpha planes; *cn_mode becomes 3, or the call fails with the bound of the
+// channel count the rule picked.  Smaller templates keep cn_mode 0 and the
+// rule is decided on the device inside the call.
+static int alpha_rule_cn(Ctx& X, hipStream_t s, const uint8_t* d_img, int64_t istride, int iw, int ih,
+                         const uint8_t* d_tpl, int64_t tstride, int tw, int th, int* cn_mode) {
+  if ((int64_t)tw * th <= stats_cap(4)) return 0;
+  launch_state_init(X.st, s);
+  launch_alpha_scan(d_img, istride, iw, ih, X.st, 0, s);
+  launch_alpha_scan(d_tpl, tstride, tw, th, X.st, 1, s);
+  CallState hs;
+  HIPCHK(hipMemcpyAsync(&hs, X.st, sizeof(CallState), hipMemcpyDeviceToHost, s));
+  HIPCHK(hipStreamSynchronize(s));
+  const int cn = (hs.alpha_bad[0] == 0 && hs.alpha_bad[1] == 0) ? 3 : 4;
+  CHK(check_cap(tw, th, cn));
+  *cn_mode = cn;
+  return 0;
+}
+
 // matchU8 on device buffers.  cn_mode 0 = Match's RGBA alpha rule.
-static int run_match(Engine& E, hipStream_t s, const uint8_t* d_img, int64_t istride, int iw, int ih,
+static int run_match(Ctx& X, hipStream_t s, const uint8_t* d_img, int64_t istride, int iw, int ih,
                      const uint8_t* d_tpl, int64_t tstride, int tw, int th, int cn_mode, int step, float* d_map) {
   int rw = iw - tw + 1, rh = ih - th + 1;
   Plan p = make_plan(tw, th, rw, rh);
   CHK(check_limits(p, tw, rw, rh));
   int dh2 = shrink_dh(p, rh, th);
   PlanTables pt;
-  CHK(plan_tables(E, p, dh2, pt));
+  CHK(plan_tables(X, p, dh2, pt));
   int cnMax = cn_mode ? cn_mode : 4;
   // side-stream buffers first: no allocation lands between fork and join
-  CHK(E.nscratch.ensure(norm_scratch_bytes(iw, ih, tw, th, cnMax)));
+  CHK(X.nscratch.ensure(norm_scratch_bytes(iw, ih, tw, th, cnMax)));
   TplSpec ts;
   ts.p = p;
   ts.dh2 = dh2;
   ts.hwPad = round_up(p.hw, 16);
   ts.cnMax = cnMax;
   size_t tchan = (size_t)p.dftH * ts.hwPad, tchan2 = (size_t)dh2 * ts.hwPad;
-  CHK(E.tspec.ensure(sizeof(float2) * tchan * cnMax));
-  ts.spec = E.tspec.as<float2>();
+  CHK(X.tspec.ensure(sizeof(float2) * tchan * cnMax));
+  ts.spec = X.tspec.as<float2>();
   if (dh2) {
-    CHK(E.tspec2.ensure(sizeof(float2) * tchan2 * cnMax));
-    ts.spec2 = E.tspec2.as<float2>();
+    CHK(X.tspec2.ensure(sizeof(float2) * tchan2 * cnMax));
+    ts.spec2 = X.tspec2.as<float2>();
   }
   // the call state, the template spectrum and the normalisation partials
   // are built on the side stream while s transforms the parent rows
   SideWork side = [&]() -> int {
-    hipStream_t q = E.side;
+    hipStream_t q = X.side;
     {
-      Stage S(E, "stats", q);
-      launch_state_init(E.st, q);
-      launch_call_state(d_img, istride, iw, ih, d_tpl, tstride, tw, th, cnMax, step, cn_mode, E.st, q);
+      Stage S(X, "stats", q);
+      launch_state_init(X.st, q);
+      launch_call_state(d_img, istride, iw, ih, d_tpl, tstride, tw, th, cnMax, step, cn_mode, X.st, q);
     }
-    CHK(build_tspec(E, q, E.st, d_tpl, tstride, tw, th, step, cnMax, p, dh2, pt, ts.spec, ts.spec2));
-    HIPCHK(hipEventRecord(E.join, q));
+    CHK(build_tspec(X, q, X.st, d_tpl, tstride, tw, th, step, cnMax, p, dh2, pt, ts.spec, ts.spec2));
+    HIPCHK(hipEventRecord(X.join, q));
     {
-      Stage S(E, "norm_prep", q);
-      launch_norm_prep(d_img, istride, iw, ih, step, tw, th, cnMax, E.st, E.nscratch.p, q);
+      Stage S(X, "norm_prep", q);
+      launch_norm_prep(d_img, istride, iw, ih, step, tw, th, cnMax, X.st, X.nscratch.p, q);
     }
-    HIPCHK(hipEventRecord(E.join2, q));
+    HIPCHK(hipEventRecord(X.join2, q));
     return 0;
   };
-  return corr_and_normalize(E, s, E.st, d_img, istride, iw, ih, tw, th, step, cnMax, ts, pt, d_map,
-                            res_out(E), &side);
+  return corr_and_normalize(X, s, X.st, d_img, istride, iw, ih, tw, th, step, cnMax, ts, pt, d_map,
+                            res_out(X), &side);
 }
 
-// The six-tuple of a call that wrote it to res_out(E).
-static int finish_result(Engine& E, hipStream_t s, cvm_result* out) {
-  if (E.rmap_d) {
+// The six-tuple of a call that wrote it to res_out(X).
+static int finish_result(Ctx& X, hipStream_t s, cvm_result* out) {
+  if (X.rmap_d) {
     HIPCHK(hipStreamSynchronize(s));
-    E.stage_collect();
-    memcpy(out, E.rmap_h, sizeof(cvm_result));
+    X.stage_collect();
+    memcpy(out, X.rmap_h, sizeof(cvm_result));
     return 0;
   }
-  HIPCHK(hipMemcpyAsync(E.res_h, E.res_d, sizeof(Res6), hipMemcpyDeviceToHost, s));
+  HIPCHK(hipMemcpyAsync(X.res_h, X.res_d, sizeof(Res6), hipMemcpyDeviceToHost, s));
   HIPCHK(hipStreamSynchronize(s));
-  E.stage_collect();
-  memcpy(out, E.res_h, sizeof(cvm_result));
+  X.stage_collect();
+  memcpy(out, X.res_h, sizeof(cvm_result));
   return 0;
 }
 
 static int64_t img_bytes(int64_t stride, int w, int h, int step) { return (int64_t)(h - 1) * stride + (int64_t)w * step; }
 
 // Host-buffer front end shared by the one-shot entry points.
-static int run_match_direct(Engine& E, hipStream_t s, const uint8_t* d_img, int64_t istride, int iw, int ih,
+static int run_match_direct(Ctx& X, hipStream_t s, const uint8_t* d_img, int64_t istride, int iw, int ih,
                             const uint8_t* d_tpl, int64_t tstride, int tw, int th, int cn_mode, int step,
                             float* d_map);
 
@@ -615,17 +755,17 @@ static int check_peaks(int rw, int rh, int& rx, int& ry, int cap, const void* xy
 }
 
 // Peaks of a device map into d_xy / d_vals; the total stays in the scratch.
-static int peaks_dev(Engine& E, hipStream_t s, const float* d_map, int rw, int rh, float thresh, int rx, int ry,
+static int peaks_dev(Ctx& X, hipStream_t s, const float* d_map, int rw, int rh, float thresh, int rx, int ry,
                      int cap, int32_t* d_xy, float* d_vals) {
-  CHK(E.pkscr.ensure(peaks_scratch_bytes(rw, rh)));
-  Stage S(E, "peaks", s);
-  launch_peaks(d_map, rw, rh, thresh, rx, ry, E.pkscr.p, cap, d_xy, d_vals, s);
+  CHK(X.pkscr.ensure(peaks_scratch_bytes(rw, rh)));
+  Stage S(X, "peaks", s);
+  launch_peaks(d_map, rw, rh, thresh, rx, ry, X.pkscr.p, cap, d_xy, d_vals, s);
   HIPCHK(hipGetLastError());
   return 0;
 }
 
-static int peaks_total_host(Engine& E, hipStream_t s, int rw, int rh, int* tot) {
-  HIPCHK(hipMemcpyAsync(tot, peaks_total(E.pkscr.p, rw, rh), sizeof(int), hipMemcpyDeviceToHost, s));
+static int peaks_total_host(Ctx& X, hipStream_t s, int rw, int rh, int* tot) {
+  HIPCHK(hipMemcpyAsync(tot, peaks_total(X.pkscr.p, rw, rh), sizeof(int), hipMemcpyDeviceToHost, s));
   HIPCHK(hipStreamSynchronize(s));
   return 0;
 }
@@ -633,46 +773,46 @@ static int peaks_total_host(Engine& E, hipStream_t s, int rw, int rh, int* tot) 
 static int match_host(const uint8_t* img, int64_t istride, int iw, int ih, const uint8_t* tpl, int64_t tstride,
                       int tw, int th, int cn_mode, int step, float* map, cvm_result* out, bool direct = false,
                       const PeakReq* pk = nullptr) {
-  Engine& E = engine();
-  std::lock_guard<std::recursive_mutex> lk(E.mu);
-  CHK(E.init());
-  hipStream_t s = E.stream;
+  LEASE(X);
+  hipStream_t s = X.stream;
   int rw = iw - tw + 1, rh = ih - th + 1;
   size_t ib = (size_t)img_bytes(istride, iw, ih, step), tb = (size_t)img_bytes(tstride, tw, th, step);
-  CHK(E.img.ensure(ib));
-  CHK(E.tpl.ensure(tb));
+  CHK(X.img.ensure(ib));
+  CHK(X.tpl.ensure(tb));
   {
-    Stage S(E, "h2d", s);
-    HIPCHK(xfer_h2d(E.img.p, img, ib, s));
-    HIPCHK(xfer_h2d(E.tpl.p, tpl, tb, s));
+    Stage S(X, "h2d", s);
+    HIPCHK(xfer_h2d(X.img.p, img, ib, s));
+    HIPCHK(xfer_h2d(X.tpl.p, tpl, tb, s));
   }
   float* d_map = nullptr;
   if (map || pk) {
-    CHK(E.mapbuf.ensure(sizeof(float) * (size_t)rw * rh));
-    d_map = E.mapbuf.as<float>();
+    CHK(X.mapbuf.ensure(sizeof(float) * (size_t)rw * rh));
+    d_map = X.mapbuf.as<float>();
   }
   int32_t* d_xy = nullptr;
   float* d_vals = nullptr;
   if (pk) {
-    CHK(E.pkscr.ensure(peaks_scratch_bytes(rw, rh)));
-    CHK(E.pkout.ensure(sizeof(int32_t) * 3 * (size_t)pk->cap + 16));
-    d_xy = E.pkout.as<int32_t>();
+    CHK(X.pkscr.ensure(peaks_scratch_bytes(rw, rh)));
+    CHK(X.pkout.ensure(sizeof(int32_t) * 3 * (size_t)pk->cap + 16));
+    d_xy = X.pkout.as<int32_t>();
     d_vals = (float*)(d_xy + 2 * (size_t)pk->cap);
   }
+  if (cn_mode == 0)
+    CHK(alpha_rule_cn(X, s, X.img.as<uint8_t>(), istride, iw, ih, X.tpl.as<uint8_t>(), tstride, tw, th, &cn_mode));
   if (direct)
-    CHK(run_match_direct(E, s, E.img.as<uint8_t>(), istride, iw, ih, E.tpl.as<uint8_t>(), tstride, tw, th, cn_mode,
+    CHK(run_match_direct(X, s, X.img.as<uint8_t>(), istride, iw, ih, X.tpl.as<uint8_t>(), tstride, tw, th, cn_mode,
                          step, d_map));
   else
-    CHK(run_match(E, s, E.img.as<uint8_t>(), istride, iw, ih, E.tpl.as<uint8_t>(), tstride, tw, th, cn_mode, step,
+    CHK(run_match(X, s, X.img.as<uint8_t>(), istride, iw, ih, X.tpl.as<uint8_t>(), tstride, tw, th, cn_mode, step,
                   d_map));
   if (map) {
-    Stage S(E, "d2h", s);
+    Stage S(X, "d2h", s);
     HIPCHK(xfer_d2h(map, d_map, sizeof(float) * (size_t)rw * rh, s));
   }
   if (pk) {
-    CHK(peaks_dev(E, s, d_map, rw, rh, pk->thresh, pk->rx, pk->ry, pk->cap, d_xy, d_vals));
+    CHK(peaks_dev(X, s, d_map, rw, rh, pk->thresh, pk->rx, pk->ry, pk->cap, d_xy, d_vals));
     int tot = 0;
-    CHK(peaks_total_host(E, s, rw, rh, &tot));
+    CHK(peaks_total_host(X, s, rw, rh, &tot));
     int k = std::min(tot, pk->cap);
     if (k > 0) {
       HIPCHK(xfer_d2h(pk->xy, d_xy, sizeof(int32_t) * 2 * (size_t)k, s));
@@ -680,7 +820,7 @@ static int match_host(const uint8_t* img, int64_t istride, int iw, int ih, const
     }
     *pk->count = tot;
   }
-  return finish_result(E, s, out);
+  return finish_result(X, s, out);
 }
 
 
@@ -757,10 +897,8 @@ static int matcher_create(const uint8_t* tpl, int64_t stride, int w, int h, int 
   if (stride < (int64_t)w * step) return fail(CVM_E_BAD_ARGS, "cvmatch: stride below width*step");
   if ((int64_t)w * h > stats_cap(step))
     return fail(CVM_E_STATS_CAP, "cvmatch: template area %dx%d exceeds the exact-statistics bound", w, h);
-  Engine& E = engine();
-  std::lock_guard<std::recursive_mutex> lk(E.mu);
-  CHK(E.init());
-  hipStream_t s = E.stream;
+  LEASE(X);
+  hipStream_t s = X.stream;
   std::unique_ptr<cvm_matcher> m(new cvm_matcher());
   m->w = w;
   m->h = h;
@@ -768,11 +906,11 @@ static int matcher_create(const uint8_t* tpl, int64_t stride, int w, int h, int 
   HIPCHK(hipMalloc(&m->d_pix, (size_t)w * h * step));
   HIPCHK(hipMemcpy2DAsync(m->d_pix, (size_t)w * step, tpl, (size_t)stride, (size_t)w * step, h, hipMemcpyHostToDevice,
                           s));
-  launch_state_init(E.st, s);
-  launch_tpl_stats(m->d_pix, (int64_t)w * step, w, h, step, step, E.st, s);
-  if (step == 4) launch_alpha_scan(m->d_pix, (int64_t)w * 4, w, h, E.st, 1, s);
+  launch_state_init(X.st, s);
+  launch_tpl_stats(m->d_pix, (int64_t)w * step, w, h, step, step, X.st, s);
+  if (step == 4) launch_alpha_scan(m->d_pix, (int64_t)w * 4, w, h, X.st, 1, s);
   CallState hs;
-  HIPCHK(hipMemcpyAsync(&hs, E.st, sizeof(CallState), hipMemcpyDeviceToHost, s));
+  HIPCHK(hipMemcpyAsync(&hs, X.st, sizeof(CallState), hipMemcpyDeviceToHost, s));
   HIPCHK(hipStreamSynchronize(s));
   long long area = (long long)w * h, var = 0;
   for (int k = 0; k < 4; k++) {
@@ -789,7 +927,7 @@ static int matcher_create(const uint8_t* tpl, int64_t stride, int w, int h, int 
 // Sets the call state from a matcher's cached statistics and decides cn on
 // the device (Find matcher.go:103-112: 3 only when both alpha planes are
 // constant; the parent is scanned only when the template's alpha is).
-static void matcher_state(Engine& E, hipStream_t s, CallState* st, const cvm_matcher* m, const uint8_t* d_img,
+static void matcher_state(Ctx& X, hipStream_t s, CallState* st, const cvm_matcher* m, const uint8_t* d_img,
                           int64_t istride, int iw, int ih) {
   launch_state_set(st, m->stats, m->step == 4 && !m->aConst, s);
   if (m->step == 4 && m->aConst) launch_alpha_scan(d_img, istride, iw, ih, st, 0, s);
@@ -797,7 +935,7 @@ static void matcher_state(Engine& E, hipStream_t s, CallState* st, const cvm_mat
 }
 
 // Template spectrum of m on plan p built for all `step` channels.
-static int matcher_spectrum(Engine& E, hipStream_t s, CallState* st, const cvm_matcher* m, const Plan& p, int dh2,
+static int matcher_spectrum(Ctx& X, hipStream_t s, CallState* st, const cvm_matcher* m, const Plan& p, int dh2,
                             const PlanTables& pt, TplSpec& ts, DevBuf& spec, DevBuf& spec2) {
   ts.p = p;
   ts.dh2 = dh2;
@@ -814,11 +952,11 @@ static int matcher_spectrum(Engine& E, hipStream_t s, CallState* st, const cvm_m
   // build every channel: a state whose cn is the template's channel count
   launch_state_set(st, m->stats, 0, s);
   launch_state_finalize(st, m->w, m->h, m->step, s);
-  return build_tspec(E, s, st, m->d_pix, (int64_t)m->w * m->step, m->w, m->h, m->step, m->step, p, dh2, pt, ts.spec,
+  return build_tspec(X, s, st, m->d_pix, (int64_t)m->w * m->step, m->w, m->h, m->step, m->step, p, dh2, pt, ts.spec,
                      ts.spec2);
 }
 
-static int matcher_find_dev(Engine& E, hipStream_t s, cvm_matcher* m, const uint8_t* d_img, int64_t istride, int iw,
+static int matcher_find_dev(Ctx& X, hipStream_t s, cvm_matcher* m, const uint8_t* d_img, int64_t istride, int iw,
                             int ih, Res6* d_res) {
   int rw = iw - m->w + 1, rh = ih - m->h + 1;
   std::shared_ptr<MatcherGeom> g;
@@ -833,19 +971,21 @@ static int matcher_find_dev(Engine& E, hipStream_t s, cvm_matcher* m, const uint
     p = make_plan(m->w, m->h, rw, rh);
     CHK(check_limits(p, m->w, rw, rh));
     dh2 = shrink_dh(p, rh, m->h);
-    CHK(plan_tables(E, p, dh2, pt));
+    CHK(plan_tables(X, p, dh2, pt));
     auto ng = std::make_shared<MatcherGeom>();
     ng->rw = rw;
     ng->rh = rh;
-    CHK(matcher_spectrum(E, s, E.st, m, p, dh2, pt, ng->ts, ng->spec, ng->spec2));
+    CHK(matcher_spectrum(X, s, X.st, m, p, dh2, pt, ng->ts, ng->spec, ng->spec2));
+    // published only once built: a call on another stream may use it next
+    HIPCHK(hipStreamSynchronize(s));
     std::lock_guard<std::mutex> lk(m->mu);
     m->geom = ng;
     g = ng;
   } else {
-    CHK(plan_tables(E, g->ts.p, g->ts.dh2, pt));
+    CHK(plan_tables(X, g->ts.p, g->ts.dh2, pt));
   }
-  matcher_state(E, s, E.st, m, d_img, istride, iw, ih);
-  return corr_and_normalize(E, s, E.st, d_img, istride, iw, ih, m->w, m->h, m->step, m->step, g->ts, pt, nullptr,
+  matcher_state(X, s, X.st, m, d_img, istride, iw, ih);
+  return corr_and_normalize(X, s, X.st, d_img, istride, iw, ih, m->w, m->h, m->step, m->step, g->ts, pt, nullptr,
                             d_res, nullptr);
 }
 
@@ -956,6 +1096,7 @@ struct cvm_fleet {
   int step = 0;
   std::vector<cvm::FleetBucket> buckets;
   std::mutex mu;
+  std::mutex call_mu;  // held by FindAll
   std::shared_ptr<cvm::FleetGeom> geom;
   cvm::DevBuf frame, mspec, corr, res;
   // plain FindAll: correlation maps and extrema partials of one batch, and
@@ -992,7 +1133,7 @@ __global__ void k_state_frame(CallState* st, int cn) {
   st->templNorm = st->numScale = st->varScale = 0.f;
 }
 
-static int fleet_build_geom(Engine& E, hipStream_t s, cvm_fleet* f, int pw, int ph, int pConst,
+static int fleet_build_geom(Ctx& X, hipStream_t s, cvm_fleet* f, int pw, int ph, int pConst,
                             std::shared_ptr<FleetGeom>& out) {
   auto g = std::make_shared<FleetGeom>();
   g->pw = pw;
@@ -1010,7 +1151,7 @@ static int fleet_build_geom(Engine& E, hipStream_t s, cvm_fleet* f, int pw, int 
     bg->dh2 = shrink_dh(bg->p, rhF, b.th);
     bg->hwPad = round_up(bg->p.hw, 16);
     PlanTables pt;
-    CHK(plan_tables(E, bg->p, bg->dh2, pt));
+    CHK(plan_tables(X, bg->p, bg->dh2, pt));
     size_t nm = b.members.size();
     bg->cns.resize(nm);
     bg->ts.resize(nm);
@@ -1039,7 +1180,7 @@ static int fleet_build_geom(Engine& E, hipStream_t s, cvm_fleet* f, int pw, int 
       CallState* mst = g->states + b.members[j];
       launch_state_set(mst, m->stats, 0, s);
       launch_state_finalize(mst, m->w, m->h, cn, s);
-      CHK(build_tspec(E, s, mst, m->d_pix, (int64_t)m->w * m->step, m->w, m->h, m->step, cn, bg->p, bg->dh2, pt,
+      CHK(build_tspec(X, s, mst, m->d_pix, (int64_t)m->w * m->step, m->w, m->h, m->step, cn, bg->p, bg->dh2, pt,
                       ts.spec, ts.spec2));
     }
     hipLaunchKernelGGL(k_state_frame, dim3(1), dim3(64), 0, s, g->frame_state + bi, bg->cnMax);
@@ -1052,7 +1193,7 @@ static int fleet_build_geom(Engine& E, hipStream_t s, cvm_fleet* f, int pw, int 
 
 // The plain FindAll's batched normalisation plan for geometry g: built on
 // first use, rebuilt when a buffer its job tables point into has moved.
-static int plain_norm_plan(Engine& E, hipStream_t s, cvm_fleet* f, FleetGeom& g, int pw, int ph, Res6* d_res) {
+static int plain_norm_plan(Ctx& X, hipStream_t s, cvm_fleet* f, FleetGeom& g, int pw, int ph, Res6* d_res) {
   PlainNormPlan& P = g.norm;
   constexpr size_t kBatchFloats = (size_t)48 << 20;  // 192 MiB of correlation maps per batch
   const size_t n = f->ms.size();
@@ -1164,7 +1305,7 @@ static int plain_norm_plan(Engine& E, hipStream_t s, cvm_fleet* f, FleetGeom& g,
 }
 
 // FindAll fleet.go:176-332 on a device-resident parent.
-static int fleet_find_dev(Engine& E, hipStream_t s, cvm_fleet* f, const uint8_t* d_img, int64_t istride, int pw,
+static int fleet_find_dev(Ctx& X, hipStream_t s, cvm_fleet* f, const uint8_t* d_img, int64_t istride, int pw,
                           int ph, int pConst, Res6* d_res, std::vector<int>& flat_members) {
   std::shared_ptr<FleetGeom> g;
   {
@@ -1173,7 +1314,7 @@ static int fleet_find_dev(Engine& E, hipStream_t s, cvm_fleet* f, const uint8_t*
   }
   if (!g || g->pw != pw || g->ph != ph || g->pConst != pConst) {
     std::shared_ptr<FleetGeom> ng;
-    CHK(fleet_build_geom(E, s, f, pw, ph, pConst, ng));
+    CHK(fleet_build_geom(X, s, f, pw, ph, pConst, ng));
     std::lock_guard<std::mutex> lk(f->mu);
     f->geom = ng;
     g = ng;
@@ -1181,12 +1322,12 @@ static int fleet_find_dev(Engine& E, hipStream_t s, cvm_fleet* f, const uint8_t*
   // Batched normalisations (members taking the wave form): the chunk prefix
   // sums depend on the frame and the channel count only, built once per
   // channel count before the members' tails.
-  CHK(plain_norm_plan(E, s, f, *g, pw, ph, d_res));
+  CHK(plain_norm_plan(X, s, f, *g, pw, ph, d_res));
   const PlainNormPlan& NP = g->norm;
   for (int cn = 1; cn <= 4; cn++) {
     if (!NP.used_cn[cn]) continue;
     const int mi = NP.prep_member[cn];
-    Stage S(E, "norm_prep", s);
+    Stage S(X, "norm_prep", s);
     launch_norm_prep(d_img, istride, pw, ph, f->step, f->ms[mi]->w, f->ms[mi]->h, cn, g->states + mi,
                      f->dscr[cn].p, s);
   }
@@ -1195,7 +1336,7 @@ static int fleet_find_dev(Engine& E, hipStream_t s, cvm_fleet* f, const uint8_t*
     BucketGeom& bg = *g->bg[bi];
     const Plan& p = bg.p;
     PlanTables pt;
-    CHK(plan_tables(E, p, bg.dh2, pt));
+    CHK(plan_tables(X, p, bg.dh2, pt));
     int rwF = pw - b.tw + 1, rhF = ph - b.th + 1;
     int ntx = (rwF + p.blockW - 1) / p.blockW, nty = (rhF + p.blockH - 1) / p.blockH, ntiles = ntx * nty;
     TileGrid gF{};
@@ -1223,7 +1364,7 @@ static int fleet_find_dev(Engine& E, hipStream_t s, cvm_fleet* f, const uint8_t*
     CallState* fst = g->frame_state + bi;
     {
       // Phase 1 fleet.go:240-263: the parent's spectrum per tile, once
-      Stage S(E, "fleet_fwd", s);
+      Stage S(X, "fleet_fwd", s);
       launch_row_fwd(d_img, istride, f->step, bg.cnMax, fst, gF, pt, frame, ftile, chan, ntiles, s);
       launch_cols(COL_FWD, frame, frame, ftile, ftile, chan, bg.cnMax, fst, gF, pt, nullptr, nullptr, 0, 0, 1.0f,
                   ntiles, s);
@@ -1245,7 +1386,7 @@ static int fleet_find_dev(Engine& E, hipStream_t s, cvm_fleet* f, const uint8_t*
     };
     auto cols_one = [&](size_t j, float2* mspec) {
       const int mi = b.members[j];
-      Stage S(E, "fleet_cols", s);
+      Stage S(X, "fleet_cols", s);
       launch_cols(COL_MULINV, frame, mspec, ftile, ftile, chan, bg.cns[j], g->states + mi, grid_of(j), pt,
                   bg.ts[j].spec, bg.ts[j].spec2, chan, tchan2, 1.0f, ntiles, s);
     };
@@ -1265,20 +1406,20 @@ static int fleet_find_dev(Engine& E, hipStream_t s, cvm_fleet* f, const uint8_t*
         corr = f->corr.as<float>();
       }
       {
-        Stage S(E, "fleet_row_inv", s);
+        Stage S(X, "fleet_row_inv", s);
         launch_row_inv(mspec, ftile, chan, mst, cn, gM, pt, corr, rw, ntiles, s);
       }
       if (!batched) {  // past the wave form: the int64 normalisation, on its own
         int nparts = norm_partials_count(rw, rh, m->w, m->h, cn);
-        CHK(E.parts.ensure(sizeof(Extrema) * (size_t)nparts));
-        Stage S(E, "fleet_normalize", s);
+        CHK(X.parts.ensure(sizeof(Extrema) * (size_t)nparts));
+        Stage S(X, "fleet_normalize", s);
         launch_normalize(d_img, istride, pw, ph, f->step, m->w, m->h, rw, rh, cn, mst, corr, nullptr,
-                         E.parts.as<Extrema>(), nullptr, s);
-        launch_reduce(E.parts.as<Extrema>(), nparts, mst, rw, d_res + mi, nullptr, rh, s);
+                         X.parts.as<Extrema>(), nullptr, s);
+        launch_reduce(X.parts.as<Extrema>(), nparts, mst, rw, d_res + mi, nullptr, rh, s);
       }
       if (NP.last_of[mi] >= 0) {  // this member's map completes a batch
         const PlainNormPlan::Batch& B = NP.batches[NP.last_of[mi]];
-        Stage S(E, "fleet_normalize", s);
+        Stage S(X, "fleet_normalize", s);
         for (auto& gq : B.groups)
           launch_norm_batch(d_img, istride, pw, ph, f->step, gq[0], gq[1], false, NP.jobs.as<NormJob>(),
                             NP.wg.as<int>() + gq[2], gq[3], s);
@@ -1292,7 +1433,7 @@ static int fleet_find_dev(Engine& E, hipStream_t s, cvm_fleet* f, const uint8_t*
         const TileGrid gA = grid_of(j), gB = grid_of(j + 1);
         const ColPairMember A{g->states + b.members[j], bg.ts[j].spec, mspecs[0], gA.extW, gA.extH};
         const ColPairMember Bm{g->states + b.members[j + 1], bg.ts[j + 1].spec, mspecs[1], gB.extW, gB.extH};
-        Stage S(E, "fleet_cols", s);
+        Stage S(X, "fleet_cols", s);
         paired = launch_cols_pair(frame, ftile, chan, std::max(bg.cns[j], bg.cns[j + 1]), gF, pt, chan, ntiles, A,
                                   Bm, s);
       }
@@ -1330,34 +1471,34 @@ static int check_direct(int tw, int th, int cnMax, int kstep) {
 
 // Direct correlation + exact normalisation + extrema of one template whose
 // B fragments are ready; the call state is final on s.
-static int run_direct(Engine& E, hipStream_t s, const CallState* st, const uint8_t* d_img, int64_t istride, int iw,
+static int run_direct(Ctx& X, hipStream_t s, const CallState* st, const uint8_t* d_img, int64_t istride, int iw,
                       int ih, int tw, int th, int step, int kstep, int cnMax, const int8_t* bf, void* nscratch,
                       bool prep, float* d_map, Res6* d_res) {
   int rw = iw - tw + 1, rh = ih - th + 1;
   float* corr = d_map;
   if (!corr) {
-    CHK(E.corr.ensure(sizeof(float) * (size_t)rw * rh));
-    corr = E.corr.as<float>();
+    CHK(X.corr.ensure(sizeof(float) * (size_t)rw * rh));
+    corr = X.corr.as<float>();
   }
   int nparts = norm_partials_count(rw, rh, tw, th, cnMax);
-  CHK(E.parts.ensure(sizeof(Extrema) * (size_t)nparts));
+  CHK(X.parts.ensure(sizeof(Extrema) * (size_t)nparts));
   {
-    Stage S(E, "direct", s);
+    Stage S(X, "direct", s);
     if (launch_direct(d_img, istride, iw, ih, kstep, bf, tw, th, rw, rh, (int32_t*)corr, s) != 0)
       return fail(CVM_E_BAD_ARGS, "cvmatch: template %dx%d is outside the direct-correlation shapes", tw, th);
   }
   if (prep) {
-    Stage S(E, "norm_prep", s);
+    Stage S(X, "norm_prep", s);
     launch_norm_prep(d_img, istride, iw, ih, step, tw, th, cnMax, st, nscratch, s);
   }
   {
-    Stage S(E, "normalize", s);
-    launch_normalize(d_img, istride, iw, ih, step, tw, th, rw, rh, cnMax, st, corr, d_map, E.parts.as<Extrema>(),
+    Stage S(X, "normalize", s);
+    launch_normalize(d_img, istride, iw, ih, step, tw, th, rw, rh, cnMax, st, corr, d_map, X.parts.as<Extrema>(),
                      nscratch, s, true);
   }
   {
-    Stage S(E, "reduce", s);
-    launch_reduce(E.parts.as<Extrema>(), nparts, st, rw, d_res, nullptr, rh, s);
+    Stage S(X, "reduce", s);
+    launch_reduce(X.parts.as<Extrema>(), nparts, st, rw, d_res, nullptr, rh, s);
   }
   HIPCHK(hipGetLastError());
   return 0;
@@ -1365,7 +1506,7 @@ static int run_direct(Engine& E, hipStream_t s, const CallState* st, const uint8
 
 // matchU8 in the direct form on device buffers (cn_mode 0 = the RGBA alpha
 // rule, decided on the device; the B fragments then read cn from the state).
-static int run_match_direct(Engine& E, hipStream_t s, const uint8_t* d_img, int64_t istride, int iw, int ih,
+static int run_match_direct(Ctx& X, hipStream_t s, const uint8_t* d_img, int64_t istride, int iw, int ih,
                             const uint8_t* d_tpl, int64_t tstride, int tw, int th, int cn_mode, int step,
                             float* d_map) {
   // the staging loads read 1 byte per pixel (gray) or 4 (RGBA): a packed
@@ -1378,29 +1519,30 @@ static int run_match_direct(Engine& E, hipStream_t s, const uint8_t* d_img, int6
   // (the fragments zero the alpha column when cn comes out 3)
   const int kstep = direct_kstep(step, cnMax);
   CHK(check_direct(tw, th, cnMax, kstep));
-  CHK(E.nscratch.ensure(norm_scratch_bytes(iw, ih, tw, th, cnMax)));
-  CHK(E.dbf.ensure(direct_bfrag_bytes(tw, th, kstep)));
+  CHK(X.nscratch.ensure(norm_scratch_bytes(iw, ih, tw, th, cnMax)));
+  CHK(X.dbf.ensure(direct_bfrag_bytes(tw, th, kstep)));
   {
-    Stage S(E, "stats", s);
-    launch_state_init(E.st, s);
-    launch_call_state(d_img, istride, iw, ih, d_tpl, tstride, tw, th, cnMax, step, cn_mode, E.st, s);
+    Stage S(X, "stats", s);
+    launch_state_init(X.st, s);
+    launch_call_state(d_img, istride, iw, ih, d_tpl, tstride, tw, th, cnMax, step, cn_mode, X.st, s);
   }
   {
-    Stage S(E, "bfrag", s);
-    launch_bfrag(d_tpl, tstride, tw, th, kstep, cnMax, E.st, E.dbf.as<int8_t>(), s);
+    Stage S(X, "bfrag", s);
+    launch_bfrag(d_tpl, tstride, tw, th, kstep, cnMax, X.st, X.dbf.as<int8_t>(), s);
   }
-  return run_direct(E, s, E.st, d_img, istride, iw, ih, tw, th, step, kstep, cnMax, E.dbf.as<int8_t>(), E.nscratch.p,
-                    true, d_map, res_out(E));
+  return run_direct(X, s, X.st, d_img, istride, iw, ih, tw, th, step, kstep, cnMax, X.dbf.as<int8_t>(), X.nscratch.p,
+                    true, d_map, res_out(X));
 }
 
 // B fragments of a matcher for cn channels (cached; cn known on the host).
-static int matcher_bfrag(Engine& E, hipStream_t s, cvm_matcher* m, int cn, const int8_t** out) {
+static int matcher_bfrag(Ctx& X, hipStream_t s, cvm_matcher* m, int cn, const int8_t** out) {
   std::lock_guard<std::mutex> lk(m->mu);
   if (!m->dbf_ok[cn]) {
     const int kstep = direct_kstep(m->step, cn);
     CHK(m->dbf[cn].ensure(direct_bfrag_bytes(m->w, m->h, kstep)));
     launch_bfrag(m->d_pix, (int64_t)m->w * m->step, m->w, m->h, kstep, cn, nullptr, m->dbf[cn].as<int8_t>(), s);
     HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(s));  // built before another stream may read it
     m->dbf_ok[cn] = true;
   }
   *out = m->dbf[cn].as<int8_t>();
@@ -1410,22 +1552,22 @@ static int matcher_bfrag(Engine& E, hipStream_t s, cvm_matcher* m, int cn, const
 // Matcher.Find in the direct form.  With a constant-alpha RGBA template the
 // channel count depends on the parent's alpha, so its B fragments are built
 // for the call from the device state; otherwise they are cached.
-static int matcher_find_direct_dev(Engine& E, hipStream_t s, cvm_matcher* m, const uint8_t* d_img, int64_t istride,
+static int matcher_find_direct_dev(Ctx& X, hipStream_t s, cvm_matcher* m, const uint8_t* d_img, int64_t istride,
                                    int iw, int ih, Res6* d_res) {
   int cnMax = m->step;
   const int kstep = direct_kstep(m->step, cnMax);
   CHK(check_direct(m->w, m->h, cnMax, kstep));
-  CHK(E.nscratch.ensure(norm_scratch_bytes(iw, ih, m->w, m->h, cnMax)));
-  matcher_state(E, s, E.st, m, d_img, istride, iw, ih);
+  CHK(X.nscratch.ensure(norm_scratch_bytes(iw, ih, m->w, m->h, cnMax)));
+  matcher_state(X, s, X.st, m, d_img, istride, iw, ih);
   const int8_t* bf = nullptr;
   if (m->step == 4 && m->aConst) {
-    CHK(E.dbf.ensure(direct_bfrag_bytes(m->w, m->h, kstep)));
-    launch_bfrag(m->d_pix, (int64_t)m->w * m->step, m->w, m->h, kstep, 4, E.st, E.dbf.as<int8_t>(), s);
-    bf = E.dbf.as<int8_t>();
+    CHK(X.dbf.ensure(direct_bfrag_bytes(m->w, m->h, kstep)));
+    launch_bfrag(m->d_pix, (int64_t)m->w * m->step, m->w, m->h, kstep, 4, X.st, X.dbf.as<int8_t>(), s);
+    bf = X.dbf.as<int8_t>();
   } else {
-    CHK(matcher_bfrag(E, s, m, cnMax, &bf));
+    CHK(matcher_bfrag(X, s, m, cnMax, &bf));
   }
-  return run_direct(E, s, E.st, d_img, istride, iw, ih, m->w, m->h, m->step, kstep, cnMax, bf, E.nscratch.p, true,
+  return run_direct(X, s, X.st, d_img, istride, iw, ih, m->w, m->h, m->step, kstep, cnMax, bf, X.nscratch.p, true,
                     nullptr, d_res);
 }
 
@@ -1434,7 +1576,7 @@ static int matcher_find_direct_dev(Engine& E, hipStream_t s, cvm_matcher* m, con
 // Find).  One launch per frame correlates all members (largest first), one
 // per wave-form variant normalises them, one reduces their extrema; the
 // normalisation partials are built once per channel count.
-static int fleet_find_direct_dev(Engine& E, hipStream_t s, cvm_fleet* f, const uint8_t* d_img, int64_t istride,
+static int fleet_find_direct_dev(Ctx& X, hipStream_t s, cvm_fleet* f, const uint8_t* d_img, int64_t istride,
                                  int pw, int ph, int pConst, Res6* d_res) {
   size_t n = f->ms.size();
   CHK(f->dstates.ensure(sizeof(CallState) * n));
@@ -1508,7 +1650,7 @@ static int fleet_find_direct_dev(Engine& E, hipStream_t s, cvm_fleet* f, const u
           cvm_matcher* m = f->ms[live[k]];
           int rw = pw - m->w + 1, rh = ph - m->h + 1;
           const int8_t* bf = nullptr;
-          CHK(matcher_bfrag(E, s, m, cn_of(m), &bf));
+          CHK(matcher_bfrag(X, s, m, cn_of(m), &bf));
           DirectJob J{};
           J.bf = bf;
           J.out = cmaps + cm_off[k];
@@ -1579,27 +1721,27 @@ static int fleet_find_direct_dev(Engine& E, hipStream_t s, cvm_fleet* f, const u
     if (!used[cn]) continue;
     for (size_t i : live)
       if (cn_of(f->ms[i]) == cn) {
-        Stage S(E, "norm_prep", s);
+        Stage S(X, "norm_prep", s);
         launch_norm_prep(d_img, istride, pw, ph, f->step, f->ms[i]->w, f->ms[i]->h, cn, states + i, f->dscr[cn].p,
                          s);
         break;
       }
   }
   {
-    Stage S(E, "direct", s);
+    Stage S(X, "direct", s);
     for (auto& g : f->bj_dgroups)
       if (launch_direct_batch(d_img, istride, pw, ph, g[0], g[1], f->bj_direct.as<DirectJob>(),
                               f->bj_dwg.as<int>() + g[2], g[3], g[4], s) != 0)
         return fail(CVM_E_BAD_ARGS, "cvmatch: fleet members outside the direct-correlation shapes");
   }
   {
-    Stage S(E, "normalize", s);
+    Stage S(X, "normalize", s);
     for (auto& g : f->bj_groups)
       launch_norm_batch(d_img, istride, pw, ph, f->step, g[0], g[1], true, f->bj_norm.as<NormJob>(),
                         f->bj_nwg.as<int>() + g[2], g[3], s);
   }
   {
-    Stage S(E, "reduce", s);
+    Stage S(X, "reduce", s);
     launch_reduce_batch(f->bj_norm.as<NormJob>(), f->bj_nnorm, s);
   }
   HIPCHK(hipGetLastError());
@@ -1645,16 +1787,16 @@ static int rgba_args(const uint8_t* img, int32_t iw, int32_t ih, const uint8_t* 
 int cvm_match(const uint8_t* img, int64_t istride, int32_t iw, int32_t ih, const uint8_t* tpl, int64_t tstride,
               int32_t tw, int32_t th, cvm_result* out) {
   CHK(rgba_args(img, iw, ih, tpl, tw, th));
-  // the cn=4 bound is the one that applies before the alpha rule is known
-  // on the device; cn=3 and cn=4 share the argument checks otherwise
-  CHK(check_args(istride, iw, ih, tstride, tw, th, 4, 4));
+  // the shape checks of cn=4; the exact-statistics bound follows the alpha
+  // rule (alpha_rule_cn)
+  CHK(check_shape(istride, iw, ih, tstride, tw, th, 4, 4));
   return match_host(img, istride, iw, ih, tpl, tstride, tw, th, 0, 4, nullptr, out);
 }
 
 int cvm_match_map(const uint8_t* img, int64_t istride, int32_t iw, int32_t ih, const uint8_t* tpl, int64_t tstride,
                   int32_t tw, int32_t th, float* map, cvm_result* out) {
   CHK(rgba_args(img, iw, ih, tpl, tw, th));
-  CHK(check_args(istride, iw, ih, tstride, tw, th, 4, 4));
+  CHK(check_shape(istride, iw, ih, tstride, tw, th, 4, 4));
   if (!map) return fail(CVM_E_BAD_ARGS, "cvmatch: MatchMap needs an output map");
   return match_host(img, istride, iw, ih, tpl, tstride, tw, th, 0, 4, map, out);
 }
@@ -1665,7 +1807,7 @@ int cvm_match_peaks(const uint8_t* img, int64_t istride, int32_t iw, int32_t ih,
   CHK(rgba_args(img, iw, ih, tpl, tw, th));
   if (cn == 0) {
     if (step != 4) return fail(CVM_E_BAD_ARGS, "cvmatch: the alpha rule (cn=0) needs step 4");
-    CHK(check_args(istride, iw, ih, tstride, tw, th, 4, 4));
+    CHK(check_shape(istride, iw, ih, tstride, tw, th, 4, 4));
   } else {
     CHK(check_args(istride, iw, ih, tstride, tw, th, cn, step));
   }
@@ -1678,14 +1820,12 @@ int cvm_find_peaks_device(const float* d_map, int32_t w, int32_t h, float thresh
                           int32_t* d_xy, float* d_vals, int32_t cap, int32_t* count, void* stream) {
   if (!d_map) return fail(CVM_E_NIL_IMAGE, "cvmatch: nil image");
   CHK(check_peaks(w, h, rx, ry, cap, d_xy, d_vals, count));
-  Engine& E = engine();
-  std::lock_guard<std::recursive_mutex> lk(E.mu);
-  CHK(E.init());
-  hipStream_t s = stream ? (hipStream_t)stream : E.stream;
-  CHK(peaks_dev(E, s, d_map, w, h, thresh, rx, ry, cap, d_xy, d_vals));
+  LEASE(X);
+  hipStream_t s = stream ? (hipStream_t)stream : X.stream;
+  CHK(peaks_dev(X, s, d_map, w, h, thresh, rx, ry, cap, d_xy, d_vals));
   int tot = 0;
-  CHK(peaks_total_host(E, s, w, h, &tot));
-  E.stage_collect();
+  CHK(peaks_total_host(X, s, w, h, &tot));
+  X.stage_collect();
   *count = tot;
   return 0;
 }
@@ -1698,38 +1838,36 @@ int cvm_match_gray(const uint8_t* img, int64_t istride, int32_t iw, int32_t ih, 
   if (istride < (int64_t)iw * istep || tstride < (int64_t)tw * tstep)
     return fail(CVM_E_BAD_ARGS, "cvmatch: bad match arguments (%dx%d in %dx%d, cn=%d step=%d)", tw, th, iw, ih, 1, 1);
   CHK(check_args(iw, iw, ih, tw, tw, th, 1, 1));
-  Engine& E = engine();
-  std::lock_guard<std::recursive_mutex> lk(E.mu);
-  CHK(E.init());
-  hipStream_t s = E.stream;
+  LEASE(X);
+  hipStream_t s = X.stream;
   size_t ib = (size_t)img_bytes(istride, iw, ih, istep), tb = (size_t)img_bytes(tstride, tw, th, tstep);
-  CHK(E.img.ensure(ib));
-  CHK(E.tpl.ensure(tb));
+  CHK(X.img.ensure(ib));
+  CHK(X.tpl.ensure(tb));
   {
-    Stage S(E, "h2d", s);
-    HIPCHK(xfer_h2d(E.img.p, img, ib, s));
-    HIPCHK(xfer_h2d(E.tpl.p, tpl, tb, s));
+    Stage S(X, "h2d", s);
+    HIPCHK(xfer_h2d(X.img.p, img, ib, s));
+    HIPCHK(xfer_h2d(X.tpl.p, tpl, tb, s));
   }
-  const uint8_t* gi = E.img.as<uint8_t>();
-  const uint8_t* gt = E.tpl.as<uint8_t>();
+  const uint8_t* gi = X.img.as<uint8_t>();
+  const uint8_t* gt = X.tpl.as<uint8_t>();
   int64_t gis = istride, gts = tstride;
   if (istep == 4 || tstep == 4) {
-    Stage S(E, "gray", s);
+    Stage S(X, "gray", s);
     if (istep == 4) {
-      CHK(E.gray0.ensure((size_t)iw * ih));
-      launch_rgba_to_gray(E.img.as<uint8_t>(), istride, iw, ih, E.gray0.as<uint8_t>(), s);
-      gi = E.gray0.as<uint8_t>();
+      CHK(X.gray0.ensure((size_t)iw * ih));
+      launch_rgba_to_gray(X.img.as<uint8_t>(), istride, iw, ih, X.gray0.as<uint8_t>(), s);
+      gi = X.gray0.as<uint8_t>();
       gis = iw;
     }
     if (tstep == 4) {
-      CHK(E.gray1.ensure((size_t)tw * th));
-      launch_rgba_to_gray(E.tpl.as<uint8_t>(), tstride, tw, th, E.gray1.as<uint8_t>(), s);
-      gt = E.gray1.as<uint8_t>();
+      CHK(X.gray1.ensure((size_t)tw * th));
+      launch_rgba_to_gray(X.tpl.as<uint8_t>(), tstride, tw, th, X.gray1.as<uint8_t>(), s);
+      gt = X.gray1.as<uint8_t>();
       gts = tw;
     }
   }
-  CHK(run_match(E, s, gi, gis, iw, ih, gt, gts, tw, th, 1, 1, nullptr));
-  return finish_result(E, s, out);
+  CHK(run_match(X, s, gi, gis, iw, ih, gt, gts, tw, th, 1, 1, nullptr));
+  return finish_result(X, s, out);
 }
 
 static int match_device(const uint8_t* d_img, int64_t istride, int32_t iw, int32_t ih, const uint8_t* d_tpl,
@@ -1754,7 +1892,7 @@ int cvm_match_u8_direct(const uint8_t* img, int64_t istride, int32_t iw, int32_t
   if (!img || !tpl) return fail(CVM_E_NIL_IMAGE, "cvmatch: nil image");
   if (cn == 0) {
     if (step != 4) return fail(CVM_E_BAD_ARGS, "cvmatch: the alpha rule (cn=0) needs step 4");
-    CHK(check_args(istride, iw, ih, tstride, tw, th, 4, 4));
+    CHK(check_shape(istride, iw, ih, tstride, tw, th, 4, 4));
   } else {
     CHK(check_args(istride, iw, ih, tstride, tw, th, cn, step));
   }
@@ -1769,37 +1907,36 @@ static int match_device(const uint8_t* d_img, int64_t istride, int32_t iw, int32
   if (!d_img || !d_tpl) return fail(CVM_E_NIL_IMAGE, "cvmatch: nil image");
   if (cn == 0) {
     if (step != 4) return fail(CVM_E_BAD_ARGS, "cvmatch: the alpha rule (cn=0) needs step 4");
-    CHK(check_args(istride, iw, ih, tstride, tw, th, 4, 4));
+    CHK(check_shape(istride, iw, ih, tstride, tw, th, 4, 4));
   } else {
     CHK(check_args(istride, iw, ih, tstride, tw, th, cn, step));
   }
-  Engine& E = engine();
-  std::lock_guard<std::recursive_mutex> lk(E.mu);
-  CHK(E.init());
-  hipStream_t s = stream ? (hipStream_t)stream : E.stream;
+  LEASE(X);
+  hipStream_t s = stream ? (hipStream_t)stream : X.stream;
+  if (cn == 0) CHK(alpha_rule_cn(X, s, d_img, istride, iw, ih, d_tpl, tstride, tw, th, &cn));
   auto run = [&]() -> int {
-    if (direct) return run_match_direct(E, s, d_img, istride, iw, ih, d_tpl, tstride, tw, th, cn, step, d_map);
-    return run_match(E, s, d_img, istride, iw, ih, d_tpl, tstride, tw, th, cn, step, d_map);
+    if (direct) return run_match_direct(X, s, d_img, istride, iw, ih, d_tpl, tstride, tw, th, cn, step, d_map);
+    return run_match(X, s, d_img, istride, iw, ih, d_tpl, tstride, tw, th, cn, step, d_map);
   };
-  if (E.prof) {  // stage events need the eager launches
-    E.drop_graph();
+  if (engine().prof.load()) {  // stage events need the eager launches
+    X.drop_graph();
     CHK(run());
-    return finish_result(E, s, out);
+    return finish_result(X, s, out);
   }
-  Engine::CallKey key{d_img, d_tpl, d_map, istride, tstride, iw, ih, tw, th, cn, step, direct ? 1 : 0, s,
+  Ctx::CallKey key{d_img, d_tpl, d_map, istride, tstride, iw, ih, tw, th, cn, step, direct ? 1 : 0, s,
                       g_alloc_epoch.load()};
-  if (!(key == E.gkey)) {
-    E.drop_graph();
-    E.gkey = key;
+  if (!(key == X.gkey)) {
+    X.drop_graph();
+    X.gkey = key;
   }
-  if (E.gexec) {
-    HIPCHK(hipGraphLaunch(E.gexec, s));
-    return finish_result(E, s, out);
+  if (X.gexec) {
+    HIPCHK(hipGraphLaunch(X.gexec, s));
+    return finish_result(X, s, out);
   }
   constexpr int kGraphAfter = 2;
-  if (++E.gseen < kGraphAfter) {
+  if (++X.gseen < kGraphAfter) {
     CHK(run());  // eager (the first one allocates what the call needs)
-    return finish_result(E, s, out);
+    return finish_result(X, s, out);
   }
   // a run of repeats with nothing reallocated since: capture the call
   hipGraph_t graph = nullptr;
@@ -1808,22 +1945,22 @@ static int match_device(const uint8_t* d_img, int64_t istride, int32_t iw, int32
   hipError_t ce = hipStreamEndCapture(s, &graph);
   if (rc != 0 || ce != hipSuccess || g_alloc_epoch.load() != key.epoch) {
     if (graph) (void)hipGraphDestroy(graph);
-    E.drop_graph();
+    X.drop_graph();
     (void)hipGetLastError();
     if (rc != 0) return rc;
     CHK(run());  // run eagerly instead
-    return finish_result(E, s, out);
+    return finish_result(X, s, out);
   }
-  hipError_t ie = hipGraphInstantiate(&E.gexec, graph, nullptr, nullptr, 0);
+  hipError_t ie = hipGraphInstantiate(&X.gexec, graph, nullptr, nullptr, 0);
   (void)hipGraphDestroy(graph);
   if (ie != hipSuccess) {
-    E.gexec = nullptr;
+    X.gexec = nullptr;
     (void)hipGetLastError();
     CHK(run());
-    return finish_result(E, s, out);
+    return finish_result(X, s, out);
   }
-  HIPCHK(hipGraphLaunch(E.gexec, s));
-  return finish_result(E, s, out);
+  HIPCHK(hipGraphLaunch(X.gexec, s));
+  return finish_result(X, s, out);
 }
 
 extern "C" {
@@ -1832,17 +1969,15 @@ int cvm_alpha_const(const uint8_t* pix, int64_t stride, int32_t w, int32_t h, in
   if (!pix) return fail(CVM_E_NIL_IMAGE, "cvmatch: nil image");
   if (w <= 0 || h <= 0) return fail(CVM_E_EMPTY_IMAGE, "cvmatch: empty image");
   if (stride < (int64_t)w * 4) return fail(CVM_E_BAD_ARGS, "cvmatch: stride %lld below 4*width", (long long)stride);
-  Engine& E = engine();
-  std::lock_guard<std::recursive_mutex> lk(E.mu);
-  CHK(E.init());
-  hipStream_t s = E.stream;
+  LEASE(X);
+  hipStream_t s = X.stream;
   size_t ib = (size_t)img_bytes(stride, w, h, 4);
-  CHK(E.img.ensure(ib));
-  HIPCHK(xfer_h2d(E.img.p, pix, ib, s));
-  launch_state_init(E.st, s);
-  launch_alpha_scan(E.img.as<uint8_t>(), stride, w, h, E.st, 0, s);
+  CHK(X.img.ensure(ib));
+  HIPCHK(xfer_h2d(X.img.p, pix, ib, s));
+  launch_state_init(X.st, s);
+  launch_alpha_scan(X.img.as<uint8_t>(), stride, w, h, X.st, 0, s);
   CallState hs;
-  HIPCHK(hipMemcpyAsync(&hs, E.st, sizeof(CallState), hipMemcpyDeviceToHost, s));
+  HIPCHK(hipMemcpyAsync(&hs, X.st, sizeof(CallState), hipMemcpyDeviceToHost, s));
   HIPCHK(hipStreamSynchronize(s));
   *out = hs.alpha_bad[0] == 0;
   return 0;
@@ -1853,10 +1988,8 @@ int cvm_rgba_to_gray_device(const uint8_t* d_pix, int64_t stride, int32_t w, int
   if (!d_pix || !d_out) return fail(CVM_E_NIL_IMAGE, "cvmatch: nil image");
   if (w <= 0 || h <= 0) return fail(CVM_E_EMPTY_IMAGE, "cvmatch: empty image");
   if (stride < (int64_t)w * 4) return fail(CVM_E_BAD_ARGS, "cvmatch: stride %lld below 4*width", (long long)stride);
-  Engine& E = engine();
-  std::lock_guard<std::recursive_mutex> lk(E.mu);
-  CHK(E.init());
-  hipStream_t s = stream ? (hipStream_t)stream : E.stream;
+  LEASE(X);
+  hipStream_t s = stream ? (hipStream_t)stream : X.stream;
   launch_rgba_to_gray(d_pix, stride, w, h, d_out, s);
   HIPCHK(hipGetLastError());
   HIPCHK(hipStreamSynchronize(s));
@@ -1867,16 +2000,14 @@ int cvm_rgba_to_gray(const uint8_t* pix, int64_t stride, int32_t w, int32_t h, u
   if (!pix || !out) return fail(CVM_E_NIL_IMAGE, "cvmatch: nil image");
   if (w <= 0 || h <= 0) return fail(CVM_E_EMPTY_IMAGE, "cvmatch: empty image");
   if (stride < (int64_t)w * 4) return fail(CVM_E_BAD_ARGS, "cvmatch: stride %lld below 4*width", (long long)stride);
-  Engine& E = engine();
-  std::lock_guard<std::recursive_mutex> lk(E.mu);
-  CHK(E.init());
-  hipStream_t s = E.stream;
+  LEASE(X);
+  hipStream_t s = X.stream;
   size_t ib = (size_t)img_bytes(stride, w, h, 4);
-  CHK(E.img.ensure(ib));
-  CHK(E.gray0.ensure((size_t)w * h));
-  HIPCHK(xfer_h2d(E.img.p, pix, ib, s));
-  launch_rgba_to_gray(E.img.as<uint8_t>(), stride, w, h, E.gray0.as<uint8_t>(), s);
-  HIPCHK(xfer_d2h(out, E.gray0.p, (size_t)w * h, s));
+  CHK(X.img.ensure(ib));
+  CHK(X.gray0.ensure((size_t)w * h));
+  HIPCHK(xfer_h2d(X.img.p, pix, ib, s));
+  launch_rgba_to_gray(X.img.as<uint8_t>(), stride, w, h, X.gray0.as<uint8_t>(), s);
+  HIPCHK(xfer_d2h(out, X.gray0.p, (size_t)w * h, s));
   HIPCHK(hipStreamSynchronize(s));
   return 0;
 }
@@ -1922,23 +2053,21 @@ static int matcher_find(cvm_matcher* m, const uint8_t* img, int64_t stride, int3
     out->min_x = out->min_y = out->max_x = out->max_y = 0;
     return 0;
   }
-  Engine& E = engine();
-  std::lock_guard<std::recursive_mutex> lk(E.mu);
-  CHK(E.init());
-  hipStream_t s = stream ? (hipStream_t)stream : E.stream;
+  LEASE(X);
+  hipStream_t s = stream ? (hipStream_t)stream : X.stream;
   const uint8_t* d_img = img;
   if (!on_device) {
     size_t ib = (size_t)img_bytes(stride, w, h, m->step);
-    CHK(E.img.ensure(ib));
-    Stage S(E, "h2d", s);
-    HIPCHK(xfer_h2d(E.img.p, img, ib, s));
-    d_img = E.img.as<uint8_t>();
+    CHK(X.img.ensure(ib));
+    Stage S(X, "h2d", s);
+    HIPCHK(xfer_h2d(X.img.p, img, ib, s));
+    d_img = X.img.as<uint8_t>();
   }
   if (direct)
-    CHK(matcher_find_direct_dev(E, s, m, d_img, stride, w, h, res_out(E)));
+    CHK(matcher_find_direct_dev(X, s, m, d_img, stride, w, h, res_out(X)));
   else
-    CHK(matcher_find_dev(E, s, m, d_img, stride, w, h, res_out(E)));
-  return finish_result(E, s, out);
+    CHK(matcher_find_dev(X, s, m, d_img, stride, w, h, res_out(X)));
+  return finish_result(X, s, out);
 }
 
 // ------------------------------------------------------- frame streams --
@@ -1964,6 +2093,7 @@ struct cvm_frames {
     cvm_result r;
   };
   std::mutex mu;
+  cvm::Ctx* ctx = nullptr;  // owned: scratch and stream of this stream's frames
   hipStream_t copy = nullptr;
   std::vector<Slot> slots;
   std::deque<Entry> fifo;
@@ -1991,12 +2121,14 @@ extern "C" {
 int cvm_frames_new(int32_t depth, cvm_frames** out) {
   if (!out) return fail(CVM_E_BAD_ARGS, "cvmatch: null output handle");
   if (depth < 2 || depth > 8) return fail(CVM_E_BAD_ARGS, "cvmatch: frame stream depth %d outside [2, 8]", depth);
-  Engine& E = engine();
   {
-    std::lock_guard<std::recursive_mutex> lk(E.mu);
+    Engine& E = engine();
+    std::lock_guard<std::mutex> lk(E.mu);
     CHK(E.init());
   }
   std::unique_ptr<cvm_frames> q(new cvm_frames());
+  q->ctx = new Ctx();
+  HIPCHK(q->ctx->init() == 0 ? hipSuccess : hipErrorOutOfMemory);
   HIPCHK(hipStreamCreateWithFlags(&q->copy, hipStreamNonBlocking));
   q->slots.resize(depth);
   for (auto& sl : q->slots) {
@@ -2037,22 +2169,25 @@ int cvm_frames_find(cvm_frames* q, cvm_matcher* m, const uint8_t* img, int64_t s
       }
   }
   cvm_frames::Slot& sl = q->slots[si];
-  Engine& E = engine();
-  std::lock_guard<std::recursive_mutex> lk(E.mu);
-  hipStream_t s = E.stream;
+  // the stream's own context: its frames stay in flight on it after return
+  Ctx& X = *q->ctx;
+  hipStream_t s = X.stream;
   size_t ib = (size_t)img_bytes(stride, w, h, m->step);
   CHK(sl.img.ensure(ib));
   {
-    Stage S(E, "h2d", q->copy);
+    Stage S(X, "h2d", q->copy);
     HIPCHK(xfer_h2d(sl.img.p, img, ib, q->copy));
   }
   HIPCHK(hipEventRecord(sl.up, q->copy));
+  // a pinned (or small) source is read by the DMA engine after xfer_h2d
+  // returns: the caller may reuse its buffer only once the upload is done
+  if (xfer_h2d_source_busy(img, ib)) HIPCHK(hipEventSynchronize(sl.up));
   HIPCHK(hipStreamWaitEvent(s, sl.up, 0));
   if (direct)
-    CHK(matcher_find_direct_dev(E, s, m, sl.img.as<uint8_t>(), stride, w, h, E.res_d));
+    CHK(matcher_find_direct_dev(X, s, m, sl.img.as<uint8_t>(), stride, w, h, X.res_d));
   else
-    CHK(matcher_find_dev(E, s, m, sl.img.as<uint8_t>(), stride, w, h, E.res_d));
-  HIPCHK(hipMemcpyAsync(sl.res, E.res_d, sizeof(Res6), hipMemcpyDeviceToHost, s));
+    CHK(matcher_find_dev(X, s, m, sl.img.as<uint8_t>(), stride, w, h, X.res_d));
+  HIPCHK(hipMemcpyAsync(sl.res, X.res_d, sizeof(Res6), hipMemcpyDeviceToHost, s));
   HIPCHK(hipEventRecord(sl.done, s));
   sl.busy = true;
   m->refs.fetch_add(1);
@@ -2071,6 +2206,9 @@ int cvm_frames_result(cvm_frames* q, int64_t* tag, cvm_result* out) {
   if (tag) *tag = e.tag;
   *out = e.r;
   q->fifo.pop_front();
+  bool idle = true;
+  for (auto& sl : q->slots) idle = idle && !sl.busy;
+  if (idle) q->ctx->stage_collect();  // every stage event of the stream has completed
   return 0;
 }
 
@@ -2087,9 +2225,13 @@ void cvm_frames_free(cvm_frames* q) {
     for (auto& e : q->fifo) (void)frames_complete(q, e);
     q->fifo.clear();
   }
-  Engine& E = engine();
-  std::lock_guard<std::recursive_mutex> lk(E.mu);
   if (q->copy) (void)hipStreamSynchronize(q->copy);
+  if (q->ctx) {
+    (void)hipStreamSynchronize(q->ctx->stream);
+    q->ctx->stage_collect();
+    q->ctx->destroy();
+    delete q->ctx;
+  }
   for (auto& sl : q->slots) {
     sl.img.release();
     if (sl.res) (void)hipHostFree(sl.res);
@@ -2165,37 +2307,37 @@ static int fleet_find_all(cvm_fleet* f, const uint8_t* img, int64_t stride, int3
     }
   }
   if (f->buckets.empty()) return 0;
-  Engine& E = engine();
-  std::lock_guard<std::recursive_mutex> lk(E.mu);
-  CHK(E.init());
-  hipStream_t s = stream ? (hipStream_t)stream : E.stream;
+  // one FindAll of a fleet at a time: its caches and buffers are the fleet's
+  std::lock_guard<std::mutex> fl(f->call_mu);
+  LEASE(X);
+  hipStream_t s = stream ? (hipStream_t)stream : X.stream;
   const uint8_t* d_img = img;
   if (!on_device) {
     size_t ib = (size_t)img_bytes(stride, w, h, f->step);
-    CHK(E.img.ensure(ib));
-    Stage S(E, "h2d", s);
-    HIPCHK(xfer_h2d(E.img.p, img, ib, s));
-    d_img = E.img.as<uint8_t>();
+    CHK(X.img.ensure(ib));
+    Stage S(X, "h2d", s);
+    HIPCHK(xfer_h2d(X.img.p, img, ib, s));
+    d_img = X.img.as<uint8_t>();
   }
   int pConst = 0;
   if (f->step == 4) {
     // the parent's alpha constancy keys the geometry cache (fleet.go:202-212)
-    launch_state_init(E.st, s);
-    launch_alpha_scan(d_img, stride, w, h, E.st, 0, s);
+    launch_state_init(X.st, s);
+    launch_alpha_scan(d_img, stride, w, h, X.st, 0, s);
     CallState hs;
-    HIPCHK(hipMemcpyAsync(&hs, E.st, sizeof(CallState), hipMemcpyDeviceToHost, s));
+    HIPCHK(hipMemcpyAsync(&hs, X.st, sizeof(CallState), hipMemcpyDeviceToHost, s));
     HIPCHK(hipStreamSynchronize(s));
     pConst = hs.alpha_bad[0] == 0;
   }
   CHK(f->res.ensure(sizeof(Res6) * n));
   if (direct)
-    CHK(fleet_find_direct_dev(E, s, f, d_img, stride, w, h, pConst, f->res.as<Res6>()));
+    CHK(fleet_find_direct_dev(X, s, f, d_img, stride, w, h, pConst, f->res.as<Res6>()));
   else
-    CHK(fleet_find_dev(E, s, f, d_img, stride, w, h, pConst, f->res.as<Res6>(), flat));
+    CHK(fleet_find_dev(X, s, f, d_img, stride, w, h, pConst, f->res.as<Res6>(), flat));
   std::vector<Res6> hr(n);
   HIPCHK(hipMemcpyAsync(hr.data(), f->res.p, sizeof(Res6) * n, hipMemcpyDeviceToHost, s));
   HIPCHK(hipStreamSynchronize(s));
-  E.stage_collect();
+  X.stage_collect();
   for (size_t i = 0; i < n; i++)
     if (f->ms[i]->varSum != 0) memcpy(&out[i], &hr[i], sizeof(cvm_result));
   return 0;
@@ -2206,17 +2348,14 @@ extern "C" {
 void cvm_fleet_free(cvm_fleet* f) {
   if (!f) return;
   {
-    Engine& E = engine();
-    std::lock_guard<std::recursive_mutex> lk(E.mu);
-    if (E.ready) (void)hipStreamSynchronize(E.stream);
+    // FindAll returns with its work complete; this waits for a call still
+    // running in another thread
+    std::lock_guard<std::mutex> fl(f->call_mu);
     f->geom.reset();
-    f->frame.release();
-    f->mspec.release();
-    f->corr.release();
-    f->res.release();
-    f->pcorr.release();
-    f->mspec2.release();
-    f->pparts.release();
+    for (DevBuf* b : {&f->frame, &f->mspec, &f->corr, &f->res, &f->pcorr, &f->pparts, &f->mspec2, &f->dstates,
+                      &f->bj_direct, &f->bj_dwg, &f->bj_norm, &f->bj_nwg, &f->bj_cmaps, &f->bj_parts})
+      b->release();
+    for (DevBuf& b : f->dscr) b.release();
   }
   for (auto* m : f->ms) matcher_release(m);
   delete f;
@@ -2225,9 +2364,9 @@ void cvm_fleet_free(cvm_fleet* f) {
 // ------------------------------------------------------------- helpers --
 
 int cvm_device_alloc(size_t bytes, void** out) {
-  Engine& E = engine();
   {
-    std::lock_guard<std::recursive_mutex> lk(E.mu);
+    Engine& E = engine();
+    std::lock_guard<std::mutex> lk(E.mu);
     CHK(E.init());
   }
   HIPCHK(hipMalloc(out, bytes));
@@ -2299,14 +2438,12 @@ __global__ void k_copy1(const uint8_t* __restrict__ src, uint8_t* __restrict__ d
 extern "C" {
 
 int cvm_device_copy(void* dst, const void* src, size_t bytes) {
-  Engine& E = engine();
-  std::lock_guard<std::recursive_mutex> lk(E.mu);
-  CHK(E.init());
-  hipStream_t s = E.stream;
+  LEASE(X);
+  hipStream_t s = X.stream;
   bool aligned = (((uintptr_t)dst | (uintptr_t)src) & 15) == 0;
   size_t n16 = aligned ? bytes / 16 : 0;
   if (n16) {
-    Stage S(E, "copy", s);
+    Stage S(X, "copy", s);
     hipLaunchKernelGGL(k_copy16, dim3((unsigned)((n16 + 1023) / 1024)), dim3(256), 0, s, (const uint4*)src,
                        (uint4*)dst, n16);
   }
@@ -2315,7 +2452,7 @@ int cvm_device_copy(void* dst, const void* src, size_t bytes) {
                        bytes - n16 * 16);
   HIPCHK(hipGetLastError());
   HIPCHK(hipStreamSynchronize(s));
-  E.stage_collect();
+  X.stage_collect();
   return 0;
 }
 
@@ -2325,22 +2462,20 @@ int cvm_synchronize(void) {
 }
 
 int cvm_profile_enable(int32_t on) {
-  Engine& E = engine();
-  std::lock_guard<std::recursive_mutex> lk(E.mu);
-  E.prof = on != 0;
+  engine().prof.store(on != 0);
   return 0;
 }
 
 int cvm_profile_reset(void) {
   Engine& E = engine();
-  std::lock_guard<std::recursive_mutex> lk(E.mu);
+  std::lock_guard<std::mutex> lk(E.pmu);
   E.prof_acc.clear();
   return 0;
 }
 
 int cvm_profile_read(char* names, double* total_ms, int64_t* count, int32_t max) {
   Engine& E = engine();
-  std::lock_guard<std::recursive_mutex> lk(E.mu);
+  std::lock_guard<std::mutex> lk(E.pmu);
   int i = 0;
   for (auto& kv : E.prof_acc) {
     if (i >= max) break;
@@ -2369,10 +2504,8 @@ __global__ void k_selftest_fp(const float* a, const float* b, const long long* i
 
 extern "C" int cvm_selftest_fp(const float* a, const float* b, const int64_t* ints, int32_t n, float* out_sqrt,
                                float* out_div, float* out_cvt) {
-  Engine& E = engine();
-  std::lock_guard<std::recursive_mutex> lk(E.mu);
-  CHK(E.init());
-  hipStream_t s = E.stream;
+  LEASE(X);
+  hipStream_t s = X.stream;
   DevBuf da, db, di, o1, o2, o3;
   size_t fb = sizeof(float) * (size_t)n, ibytes = sizeof(int64_t) * (size_t)n;
   CHK(da.ensure(fb));
@@ -2402,15 +2535,13 @@ extern "C" int cvm_selftest_fp(const float* a, const float* b, const int64_t* in
 
 extern "C" int cvm_twiddles(int32_t n, float* tri, int32_t* brev, int32_t* tri_len_out) {
   if (n < 1 || (n & (n - 1)) || n > (1 << 22)) return fail(CVM_E_BAD_ARGS, "cvmatch: twiddle size %d", n);
-  Engine& E = engine();
-  std::lock_guard<std::recursive_mutex> lk(E.mu);
-  CHK(E.init());
+  LEASE(X);
   const FftTables* t;
-  CHK(E.tab(n, &t));
+  CHK(X.tab(n, &t));
   int len = tri_len(n);
-  HIPCHK(hipMemcpyAsync(tri, t->tri, sizeof(float2) * (size_t)len, hipMemcpyDeviceToHost, E.stream));
-  HIPCHK(hipMemcpyAsync(brev, t->brev, sizeof(int32_t) * (size_t)n, hipMemcpyDeviceToHost, E.stream));
-  HIPCHK(hipStreamSynchronize(E.stream));
+  HIPCHK(hipMemcpyAsync(tri, t->tri, sizeof(float2) * (size_t)len, hipMemcpyDeviceToHost, X.stream));
+  HIPCHK(hipMemcpyAsync(brev, t->brev, sizeof(int32_t) * (size_t)n, hipMemcpyDeviceToHost, X.stream));
+  HIPCHK(hipStreamSynchronize(X.stream));
   *tri_len_out = len;
   return 0;
 }

@@ -222,14 +222,18 @@ void launch_rgba_to_gray(const uint8_t* pix, int64_t stride, int w, int h, uint8
 // Host transfers (cvm_xfer.hip).  A pageable host buffer of 1 MiB or more
 // moves through pinned staging chunks with host_threads() copy threads;
 // buffers from host_alloc and small ones go straight to the driver.
-// xfer_h2d returns once the source may be reused; xfer_d2h returns with the
-// destination written, unless it is a host_alloc buffer (then the copy is left
-// in flight on s).
+// xfer_h2d returns with a staged (pageable, 1 MiB or more) source already
+// copied out; a host_alloc or small source is still read by the copy in
+// flight on s (xfer_h2d_source_busy says which), so a caller that returns
+// before s completes waits for that copy first.  xfer_d2h returns with the
+// destination written, unless it is a host_alloc buffer (then the copy is
+// left in flight on s).
 int set_host_threads(int n);
 int host_threads();
 hipError_t host_alloc(size_t bytes, void** out);
 hipError_t host_free(void* p);
 hipError_t xfer_h2d(void* dst, const void* src, size_t n, hipStream_t s);
+bool xfer_h2d_source_busy(const void* src, size_t n);
 hipError_t xfer_d2h(void* dst, const void* src, size_t n, hipStream_t s);
 // Peaks (cvm_peaks.hip): threshold + local-maximum suppression over a map
 // with rx/ry already clamped to the map (rx <= peaks_max_rx()).  Writes up to

@@ -330,14 +330,17 @@ size_t norm_scratch_bytes(int iw, int ih, int tw, int th, int cnMax) {
 // One workgroup: result columns [xs, xs+256) x rows [ys, ye).  Each thread
 // owns CPT consecutive image columns of the strip [xs, xs+256+tw-1) and keeps
 // their exact column sums over the current th window rows in registers.
+// Result column xs + t takes its window sums from the strip's prefix at
+// columns t and t + tw, so of the whole prefix only the two 256-entry
+// windows [0, 256) (PL) and [tw, tw + 256) (PH) are kept in LDS, whatever
+// the template width.
 template <int CPT>
 __global__ __launch_bounds__(256) void k_normalize(const uint8_t* __restrict__ img, int64_t istride, int iw, int step,
                                                    int tw, int th, int rw, int rh, int bandRows,
                                                    const CallState* __restrict__ st, const float* corr, float* map,
                                                    Extrema* __restrict__ parts) {
-  constexpr int NW = NORM_NX * CPT;
-  __shared__ uint32_t P[4][NW + 1];
-  __shared__ unsigned long long P2[NW + 1];
+  __shared__ uint32_t PL[4][NORM_NX], PH[4][NORM_NX];
+  __shared__ unsigned long long PL2[NORM_NX], PH2[NORM_NX];
   __shared__ uint32_t wtot[4][4];
   __shared__ unsigned long long wtot2[4];
   __shared__ float red_v[2][4];
@@ -425,33 +428,42 @@ __global__ __launch_bounds__(256) void k_normalize(const uint8_t* __restrict__ i
       }
       for (int k = 0; k < 4; k++) off[k] += inc[k] - run[k];  // exclusive within the wave
       off2 += inc2 - run2;
+      // prefix entry c (the sums of strip columns [0, c)) is kept when a
+      // result reads it: c < 256 (PL) or tw <= c < tw + 256 (PH)
       for (int j = 0; j < CPT; j++) {
-        for (int k = 0; k < 4; k++) P[k][cbase + j + 1] = off[k] + loc[j][k];
-        P2[cbase + j + 1] = off2 + loc2[j];
+        const int c = cbase + j + 1;
+        if (c < NORM_NX) {
+          for (int k = 0; k < 4; k++) PL[k][c] = off[k] + loc[j][k];
+          PL2[c] = off2 + loc2[j];
+        }
+        if (c >= tw && c < tw + NORM_NX) {
+          for (int k = 0; k < 4; k++) PH[k][c - tw] = off[k] + loc[j][k];
+          PH2[c - tw] = off2 + loc2[j];
+        }
       }
       if (tid == 0) {
-        for (int k = 0; k < 4; k++) P[k][0] = 0;
-        P2[0] = 0;
+        for (int k = 0; k < 4; k++) PL[k][0] = 0;
+        PL2[0] = 0;
       }
       __syncthreads();
       if (active) {
         long long s[4];
         long long lane0 = 0, sq = 0;
         for (int k = 0; k < 4; k++) {
-          s[k] = (long long)(uint32_t)(P[k][tid + tw] - P[k][tid]);
+          s[k] = (long long)(uint32_t)(PH[k][tid] - PL[k][tid]);
           if (k < cn) {
             lane0 += s[k] * tsum[k];
             sq += s[k] * s[k];
           }
         }
-        long long s2 = (long long)(P2[tid + tw] - P2[tid]);
+        long long s2 = (long long)(PH2[tid] - PL2[tid]);
         if (cn == 1) lane0 = s[0];
         float l0 = (float)(double)lane0;
         float l1 = (float)(double)(area * s2 - sq);
         float l2 = (float)(double)s2;
         v = norm_one(corr[(int64_t)y * rw + x], l0, l1, l2, numScale, varScale, templNorm);
       }
-      __syncthreads();  // P is rewritten for the next row
+      __syncthreads();  // PL / PH are rewritten for the next row
       // slide the column sums down one row (colSlideGo impl.go:761-805)
       if (y + 1 < ye) {
         const uint8_t* rsub = img + (int64_t)y * istride;
@@ -1080,14 +1092,19 @@ void launch_normalize(const uint8_t* img, int64_t istride, int iw, int ih, int s
 #define CVM_NORM(C)                                                                                              \
   hipLaunchKernelGGL(k_normalize<C>, grid, dim3(NORM_NX), 0, s, img, istride, iw, step, tw, th, rw, rh, br, st, \
                      corr, map, partials)
+  // templates up to the widest row transform (4096): cpt <= 17
   if (cpt <= 1)
     CVM_NORM(1);
   else if (cpt <= 2)
     CVM_NORM(2);
   else if (cpt <= 4)
     CVM_NORM(4);
-  else
+  else if (cpt <= 8)
     CVM_NORM(8);
+  else if (cpt <= 12)
+    CVM_NORM(12);
+  else
+    CVM_NORM(17);
 #undef CVM_NORM
 }
 

@@ -254,8 +254,17 @@ void host_copy(void* dst, const void* src, size_t n, int threads) {
   S.pool.copy(dst, src, n, std::min(16, std::max(1, threads)));
 }
 
-// Returns once the source may be reused: the last chunks may still be in
-// flight on stream s, which orders them before later work on s.
+// Whether xfer_h2d leaves the copy reading src after it returns: true for
+// the direct routes (a pinned block, or a buffer below the staging size).
+bool xfer_h2d_source_busy(const void* src, size_t n) {
+  Staging& S = staging();
+  std::lock_guard<std::mutex> lk(S.mu);
+  return n < Staging::kDirect || S.covers(src, n) || S.init() != hipSuccess;
+}
+
+// A staged source may be reused on return: the last chunks may still be in
+// flight on stream s (from the pinned chunks), which orders them before
+// later work on s.  On the direct routes the copy reads src itself.
 hipError_t xfer_h2d(void* dst, const void* src, size_t n, hipStream_t s) {
   Staging& S = staging();
   std::lock_guard<std::mutex> lk(S.mu);

@@ -71,3 +71,98 @@ def test_stream_4k_matches_sync(cvm):
         q.find(m, parent, tag=k)
     for k in range(6):
         assert q.result() == (k, want)
+
+
+@pytest.mark.parametrize("pinned,size", [(True, (1280, 720)), (False, (320, 200))])
+def test_frame_buffer_reused_after_submit(cvm, pinned, size):
+    """The caller's frame buffer may be reused as soon as find returns, also
+    when the upload reads it directly (a cvm_host_alloc frame, or a frame
+    below the staging size): it is overwritten right after each submission
+    and every result still equals the synchronous Find of the frame it held."""
+    w, h = size
+    base, fr = frames(6, 77, w=w, h=h)
+    m = cvm.NewMatcher(base[50:90, 60:110].copy())
+    want = [(i, m.find(f)) for i, f in enumerate(fr)]
+    buf = cvm.host_array(fr[0].shape) if pinned else np.empty_like(fr[0])
+    q = cvm.FrameStream(2)
+    got = []
+    say(f"[frames] {'pinned' if pinned else 'pageable'} {w}x{h} buffer overwritten after each find")
+    for i, f in enumerate(fr):
+        buf[...] = f
+        q.find(m, buf, tag=i)
+        buf[...] = 0
+        if q.pending() >= 2:
+            got.append(q.result())
+    while q.pending():
+        got.append(q.result())
+    assert got == want
+
+
+def test_frames_in_flight_beside_caller_stream_calls(cvm):
+    """4K frames in flight on a frame stream while device-resident calls run
+    on a caller's HIP stream (Match with its map, FindAll plain and direct,
+    Find): the two share no device scratch, so every result equals the
+    synchronous call's bit for bit (README.md:419-431)."""
+    import ctypes as C
+
+    L = cvm.lib()
+    hip = C.CDLL("libamdhip64.so.7")  # the runtime the library has loaded
+    stream = C.c_void_p()
+    assert hip.hipStreamCreateWithFlags(C.byref(stream), C.c_uint(1)) == 0
+    bufs = []
+
+    def dev(a):
+        p = C.c_void_p()
+        assert L.cvm_device_alloc(max(a.nbytes, 4), C.byref(p)) == 0
+        assert L.cvm_copy_h2d(p, C.c_void_p(a.ctypes.data), a.nbytes) == 0
+        bufs.append(p)
+        return p
+
+    try:
+        big, sub4k, _ = synth.workload_map_4k(seed=3, alpha="random")
+        shifted = np.ascontiguousarray(np.roll(big, 5, axis=1))
+        m4 = cvm.NewMatcher(sub4k)
+        want_frames = [m4.find(big), m4.find(shifted)]
+        parent, sub, _ = synth.workload_match_640(seed=9, alpha="random")
+        want_map, _, _ = cvm.match_map(parent, sub)
+        want_t = cvm.match(parent, sub)
+        rh, rw = want_map.shape
+        frame = synth.flat_rect_scene(640, 400, np.random.default_rng(3), n_rects=25, channels=4)
+        frame[..., 3] = 255
+        subs = [frame[y:y + hh, x:x + ww].copy() for (x, y, ww, hh) in [(300, 150, 40, 30), (420, 220, 24, 24)]]
+        ms = [cvm.NewMatcher(s) for s in subs]
+        fl = cvm.NewFleet(*ms)
+        want_all = [tuple(r) for r in fl.find_all(frame)]
+        want_direct = [tuple(r) for r in fl.find_all(frame, direct=True)]
+        want_find = [m.find(frame) for m in ms]
+        dp, ds, df = dev(parent), dev(sub), dev(frame)
+        dm = dev(np.zeros(rw * rh, np.float32))
+        out = cvm.cvm_result()
+        q = cvm.FrameStream(3)
+        got = []
+        tags = 0
+        say("[frames] 4K frames in flight beside caller-stream calls")
+        for r in range(4):
+            for f in (big, shifted):
+                q.find(m4, f, tag=tags)
+                tags += 1
+            rc = L.cvm_match_u8_device(dp, 640 * 4, 640, 480, ds, 48 * 4, 48, 48, 0, 4, dm, C.byref(out), stream)
+            assert rc == 0, L.cvm_last_error_detail()
+            gm = np.empty(rw * rh, np.float32)
+            assert hip.hipStreamSynchronize(stream) == 0
+            assert L.cvm_copy_d2h(C.c_void_p(gm.ctypes.data), dm, gm.nbytes) == 0
+            assert np.array_equal(gm.view(np.uint32), want_map.ravel().view(np.uint32)), r
+            assert out.tuple() == want_t, r
+            assert [tuple(x) for x in fl.find_all_device(df.value, 640 * 4, 640, 400, stream=stream)] == want_all
+            assert [tuple(x) for x in fl.find_all_device(df.value, 640 * 4, 640, 400, direct=True,
+                                                         stream=stream)] == want_direct
+            assert [m.find_device(df.value, 640 * 4, 640, 400, stream=stream) for m in ms] == want_find
+            while q.pending() > 3:
+                got.append(q.result())
+        while q.pending():
+            got.append(q.result())
+        assert got == [(k, want_frames[k % 2]) for k in range(tags)]
+    finally:
+        for p in bufs:
+            L.cvm_device_free(p)
+        hip.hipStreamDestroy(stream)

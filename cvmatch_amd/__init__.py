@@ -89,6 +89,8 @@ SIGNATURES = {
     "cvm_match_u8_direct": ([_P, _I64, _I32, _I32, _P, _I64, _I32, _I32, _I32, _I32, _P, _RES], C.c_int),
     "cvm_match_u8_direct_device": ([_P, _I64, _I32, _I32, _P, _I64, _I32, _I32, _I32, _I32, _P, _RES, _P],
                                    C.c_int),
+    "cvm_match_u8_direct_device_noalloc": ([_P, _I64, _I32, _I32, _P, _I64, _I32, _I32, _I32, _I32, _P, _RES, _P],
+                                           C.c_int),
     "cvm_matcher_find_direct": ([_P, _P, _I64, _I32, _I32, _RES], C.c_int),
     "cvm_fleet_find_all_direct": ([_P, _P, _I64, _I32, _I32, _P], C.c_int),
     "cvm_device_alloc": ([_SZ, C.POINTER(_P)], C.c_int),

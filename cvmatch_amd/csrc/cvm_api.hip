@@ -300,6 +300,9 @@ struct Ctx {
   Res6* rmap_d = nullptr;
   std::vector<hipEvent_t> evpool;
   std::vector<ProfStage> pending;
+  // a context that never records stage events (the allocation-free call,
+  // which has no streams, events or buffers of its own)
+  bool noprof = false;
 
   int init() {
     HIPCHK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
@@ -354,7 +357,7 @@ struct Ctx {
   }
 
   int stage_begin(const char* name, hipStream_t s) {
-    if (!engine().prof.load()) return -1;
+    if (noprof || !engine().prof.load()) return -1;
     ProfStage p;
     p.name = name;
     p.a = ev();
@@ -1470,18 +1473,14 @@ static int check_direct(int tw, int th, int cnMax, int kstep) {
 }
 
 // Direct correlation + exact normalisation + extrema of one template whose
-// B fragments are ready; the call state is final on s.
+// B fragments are ready; the call state is final on s.  corr receives the
+// int32 C' map (the caller's map, rescored in place, or scratch) and parts
+// the extrema partials (norm_partials_count entries).
 static int run_direct(Ctx& X, hipStream_t s, const CallState* st, const uint8_t* d_img, int64_t istride, int iw,
                       int ih, int tw, int th, int step, int kstep, int cnMax, const int8_t* bf, void* nscratch,
-                      bool prep, float* d_map, Res6* d_res) {
+                      bool prep, float* d_map, float* corr, Extrema* parts, Res6* d_res) {
   int rw = iw - tw + 1, rh = ih - th + 1;
-  float* corr = d_map;
-  if (!corr) {
-    CHK(X.corr.ensure(sizeof(float) * (size_t)rw * rh));
-    corr = X.corr.as<float>();
-  }
   int nparts = norm_partials_count(rw, rh, tw, th, cnMax);
-  CHK(X.parts.ensure(sizeof(Extrema) * (size_t)nparts));
   {
     Stage S(X, "direct", s);
     if (launch_direct(d_img, istride, iw, ih, kstep, bf, tw, th, rw, rh, (int32_t*)corr, s) != 0)
@@ -1493,45 +1492,87 @@ static int run_direct(Ctx& X, hipStream_t s, const CallState* st, const uint8_t*
   }
   {
     Stage S(X, "normalize", s);
-    launch_normalize(d_img, istride, iw, ih, step, tw, th, rw, rh, cnMax, st, corr, d_map, X.parts.as<Extrema>(),
-                     nscratch, s, true);
+    launch_normalize(d_img, istride, iw, ih, step, tw, th, rw, rh, cnMax, st, corr, d_map, parts, nscratch, s, true);
   }
   {
     Stage S(X, "reduce", s);
-    launch_reduce(X.parts.as<Extrema>(), nparts, st, rw, d_res, nullptr, rh, s);
+    launch_reduce(parts, nparts, st, rw, d_res, nullptr, rh, s);
   }
   HIPCHK(hipGetLastError());
   return 0;
 }
 
-// matchU8 in the direct form on device buffers (cn_mode 0 = the RGBA alpha
-// rule, decided on the device; the B fragments then read cn from the state).
-static int run_match_direct(Ctx& X, hipStream_t s, const uint8_t* d_img, int64_t istride, int iw, int ih,
-                            const uint8_t* d_tpl, int64_t tstride, int tw, int th, int cn_mode, int step,
-                            float* d_map) {
+// The scratch of one direct-form matchU8 call (run_match_direct): its call
+// state, the template's B fragments, the normalisation's chunk sums, the C'
+// map when the caller passes none, the extrema partials and the six-tuple.
+struct DirectScratch {
+  CallState* st;
+  int8_t* bf;
+  void* nscratch;
+  float* corr;
+  Extrema* parts;
+  Res6* res;
+};
+
+// Byte sizes of DirectScratch's buffers after the state (which is
+// sizeof(CallState)) in the order bf, nscratch, corr, parts; 0 = none.
+static void direct_scratch_sizes(int iw, int ih, int tw, int th, int cnMax, int kstep, bool want_corr,
+                                 size_t out[4]) {
+  const int rw = iw - tw + 1, rh = ih - th + 1;
+  out[0] = direct_bfrag_bytes(tw, th, kstep);
+  out[1] = norm_scratch_bytes(iw, ih, tw, th, cnMax);
+  out[2] = want_corr ? sizeof(float) * (size_t)rw * rh : 0;
+  out[3] = sizeof(Extrema) * (size_t)norm_partials_count(rw, rh, tw, th, cnMax);
+}
+
+// The argument rules of the direct form; sets cnMax and kstep.
+static int direct_args(int tw, int th, int cn_mode, int step, int* cnMax, int* kstep) {
   // the staging loads read 1 byte per pixel (gray) or 4 (RGBA): a packed
   // 2- or 3-byte layout would be read past its rows
   if (!((step == 1 && cn_mode == 1) || (step == 4 && (cn_mode == 0 || cn_mode == 3 || cn_mode == 4))))
     return fail(CVM_E_BAD_ARGS, "cvmatch: the direct form takes cn=1 step=1 or cn=0/3/4 step=4 (got cn=%d step=%d)",
                 cn_mode, step);
-  int cnMax = cn_mode ? cn_mode : 4;
+  *cnMax = cn_mode ? cn_mode : 4;
   // with the alpha rule cn is decided on the device: stage all four bytes
   // (the fragments zero the alpha column when cn comes out 3)
-  const int kstep = direct_kstep(step, cnMax);
-  CHK(check_direct(tw, th, cnMax, kstep));
-  CHK(X.nscratch.ensure(norm_scratch_bytes(iw, ih, tw, th, cnMax)));
-  CHK(X.dbf.ensure(direct_bfrag_bytes(tw, th, kstep)));
+  *kstep = direct_kstep(step, *cnMax);
+  return check_direct(tw, th, *cnMax, *kstep);
+}
+
+// matchU8 in the direct form on device buffers with its scratch given
+// (cn_mode 0 = the RGBA alpha rule, decided on the device; the B fragments
+// then read cn from the state).  Allocates nothing.
+static int direct_pipeline(Ctx& X, hipStream_t s, const uint8_t* d_img, int64_t istride, int iw, int ih,
+                           const uint8_t* d_tpl, int64_t tstride, int tw, int th, int cn_mode, int step, int cnMax,
+                           int kstep, const DirectScratch& B, float* d_map) {
   {
     Stage S(X, "stats", s);
-    launch_state_init(X.st, s);
-    launch_call_state(d_img, istride, iw, ih, d_tpl, tstride, tw, th, cnMax, step, cn_mode, X.st, s);
+    launch_state_init(B.st, s);
+    launch_call_state(d_img, istride, iw, ih, d_tpl, tstride, tw, th, cnMax, step, cn_mode, B.st, s);
   }
   {
     Stage S(X, "bfrag", s);
-    launch_bfrag(d_tpl, tstride, tw, th, kstep, cnMax, X.st, X.dbf.as<int8_t>(), s);
+    launch_bfrag(d_tpl, tstride, tw, th, kstep, cnMax, B.st, B.bf, s);
   }
-  return run_direct(X, s, X.st, d_img, istride, iw, ih, tw, th, step, kstep, cnMax, X.dbf.as<int8_t>(), X.nscratch.p,
-                    true, d_map, res_out(X));
+  return run_direct(X, s, B.st, d_img, istride, iw, ih, tw, th, step, kstep, cnMax, B.bf, B.nscratch, true, d_map,
+                    d_map ? d_map : B.corr, B.parts, B.res);
+}
+
+// matchU8 in the direct form on device buffers, scratch from the context.
+static int run_match_direct(Ctx& X, hipStream_t s, const uint8_t* d_img, int64_t istride, int iw, int ih,
+                            const uint8_t* d_tpl, int64_t tstride, int tw, int th, int cn_mode, int step,
+                            float* d_map) {
+  int cnMax, kstep;
+  CHK(direct_args(tw, th, cn_mode, step, &cnMax, &kstep));
+  size_t sz[4];
+  direct_scratch_sizes(iw, ih, tw, th, cnMax, kstep, d_map == nullptr, sz);
+  CHK(X.dbf.ensure(sz[0]));
+  CHK(X.nscratch.ensure(sz[1]));
+  if (sz[2]) CHK(X.corr.ensure(sz[2]));
+  CHK(X.parts.ensure(sz[3]));
+  const DirectScratch B{X.st, X.dbf.as<int8_t>(), X.nscratch.p, X.corr.as<float>(), X.parts.as<Extrema>(),
+                        res_out(X)};
+  return direct_pipeline(X, s, d_img, istride, iw, ih, d_tpl, tstride, tw, th, cn_mode, step, cnMax, kstep, B, d_map);
 }
 
 // B fragments of a matcher for cn channels (cached; cn known on the host).
@@ -1567,8 +1608,11 @@ static int matcher_find_direct_dev(Ctx& X, hipStream_t s, cvm_matcher* m, const 
   } else {
     CHK(matcher_bfrag(X, s, m, cnMax, &bf));
   }
+  const int rw = iw - m->w + 1, rh = ih - m->h + 1;
+  CHK(X.corr.ensure(sizeof(float) * (size_t)rw * rh));
+  CHK(X.parts.ensure(sizeof(Extrema) * (size_t)norm_partials_count(rw, rh, m->w, m->h, cnMax)));
   return run_direct(X, s, X.st, d_img, istride, iw, ih, m->w, m->h, m->step, kstep, cnMax, bf, X.nscratch.p, true,
-                    nullptr, d_res);
+                    nullptr, X.corr.as<float>(), X.parts.as<Extrema>(), d_res);
 }
 
 // Fleet.FindAll in the direct form: every non-flat member against the frame
@@ -1963,7 +2007,87 @@ static int match_device(const uint8_t* d_img, int64_t istride, int32_t iw, int32
   return finish_result(X, s, out);
 }
 
+// ---------------------------------------------------- allocation-free call --
+//
+// cvm_match_u8_direct_device_noalloc: the direct form's device-resident call
+// for callers that cannot allocate (code under stream capture, serving loops
+// that forbid allocation).  Its scratch -- call state, B fragments, chunk
+// sums, the C' map when no map is passed, extrema partials, six-tuple -- is
+// carved out of one file-scope device array of kNoAllocBytes; each device
+// holds its own copy, whose address is looked up once per device with
+// hipGetSymbolAddress (the array's name in host code is only a stand-in).
+// Calls on one device are serialised by a lock held until their work has
+// completed, and each call rewrites the state it reads (the call state is
+// cleared by a kernel), so nothing carries over from the previous call.
+
+constexpr size_t kNoAllocBytes = (size_t)64 << 20;
+static __device__ __align__(256) uint8_t scratch[kNoAllocBytes];
+
+namespace {
+constexpr int kNoAllocDevices = 64;
+struct NoAllocDevice {
+  std::mutex mu;
+  uint8_t* ptr = nullptr;  // this device's copy of scratch
+};
+NoAllocDevice g_noalloc[kNoAllocDevices];
+}  // namespace
+
+static int noalloc_call(const uint8_t* d_img, int64_t istride, int32_t iw, int32_t ih, const uint8_t* d_tpl,
+                        int64_t tstride, int32_t tw, int32_t th, int32_t cn, int32_t step, float* d_map,
+                        cvm_result* out, void* stream) {
+  if (!d_img || !d_tpl) return fail(CVM_E_NIL_IMAGE, "cvmatch: nil image");
+  if (!out) return fail(CVM_E_BAD_ARGS, "cvmatch: nil result");
+  if (cn == 0) {
+    if (step != 4) return fail(CVM_E_BAD_ARGS, "cvmatch: the alpha rule (cn=0) needs step 4");
+    CHK(check_shape(istride, iw, ih, tstride, tw, th, 4, 4));
+  } else {
+    CHK(check_args(istride, iw, ih, tstride, tw, th, cn, step));
+  }
+  int cnMax, kstep;
+  CHK(direct_args(tw, th, cn, step, &cnMax, &kstep));
+  size_t sz[4];
+  direct_scratch_sizes(iw, ih, tw, th, cnMax, kstep, d_map == nullptr, sz);
+  auto up = [](size_t v) { return (v + 255) & ~(size_t)255; };
+  const size_t off_res = up(sizeof(CallState)), off_bf = off_res + up(sizeof(Res6));
+  const size_t off_ns = off_bf + up(sz[0]), off_corr = off_ns + up(sz[1]), off_parts = off_corr + up(sz[2]);
+  const size_t total = off_parts + up(sz[3]);
+  if (total > kNoAllocBytes)
+    return fail(CVM_E_UNSUPPORTED,
+                "cvmatch: the allocation-free call needs %zu bytes of scratch for %dx%d in %dx%d; its limit is %zu",
+                total, tw, th, iw, ih, kNoAllocBytes);
+  int dev = 0;
+  HIPCHK(hipGetDevice(&dev));
+  if (dev < 0 || dev >= kNoAllocDevices) return fail(CVM_E_UNSUPPORTED, "cvmatch: device ordinal %d", dev);
+  NoAllocDevice& D = g_noalloc[dev];
+  std::lock_guard<std::mutex> lk(D.mu);
+  if (!D.ptr) {
+    void* p = nullptr;
+    HIPCHK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    if (!p) return fail(CVM_E_HIP, "cvmatch: no device address for the allocation-free scratch");
+    D.ptr = (uint8_t*)p;
+  }
+  uint8_t* base = D.ptr;
+  const DirectScratch B{(CallState*)base,          (int8_t*)(base + off_bf),       base + off_ns,
+                        (float*)(base + off_corr), (Extrema*)(base + off_parts), (Res6*)(base + off_res)};
+  static Ctx quiet = [] {
+    Ctx c;
+    c.noprof = true;
+    return c;
+  }();
+  hipStream_t s = (hipStream_t)stream;
+  CHK(direct_pipeline(quiet, s, d_img, istride, iw, ih, d_tpl, tstride, tw, th, cn, step, cnMax, kstep, B, d_map));
+  HIPCHK(hipMemcpyAsync(out, B.res, sizeof(cvm_result), hipMemcpyDeviceToHost, s));
+  HIPCHK(hipStreamSynchronize(s));
+  return 0;
+}
+
 extern "C" {
+
+int cvm_match_u8_direct_device_noalloc(const uint8_t* d_img, int64_t istride, int32_t iw, int32_t ih,
+                                       const uint8_t* d_tpl, int64_t tstride, int32_t tw, int32_t th, int32_t cn,
+                                       int32_t step, float* d_map, cvm_result* out, void* stream) {
+  return noalloc_call(d_img, istride, iw, ih, d_tpl, tstride, tw, th, cn, step, d_map, out, stream);
+}
 
 int cvm_alpha_const(const uint8_t* pix, int64_t stride, int32_t w, int32_t h, int32_t* out) {
   if (!pix) return fail(CVM_E_NIL_IMAGE, "cvmatch: nil image");

@@ -196,6 +196,18 @@ int cvm_match_u8_device(const uint8_t *d_img, int64_t istride, int32_t iw, int32
 int cvm_match_u8_direct_device(const uint8_t *d_img, int64_t istride, int32_t iw, int32_t ih, const uint8_t *d_tpl,
                                int64_t tstride, int32_t tw, int32_t th, int32_t cn, int32_t step, float *d_map,
                                cvm_result *out, void *stream);
+/* No counterpart in the reference: cvm_match_u8_direct_device for callers
+ * that cannot allocate device memory (code under HIP stream capture, serving
+ * loops or harnesses that forbid hipMalloc).  Same arguments and results; it
+ * allocates nothing: its scratch is one static device array of 64 MiB per
+ * device (held until the process ends), which bounds the call to
+ * direct-form shapes whose scratch fits it (a 3840x2160 RGBA frame with a
+ * 128x128 template fits with or without a map); larger ones return
+ * CVM_E_UNSUPPORTED.  stream NULL: the null stream.  Calls on one device
+ * share the array and are serialised (each returns with its work done). */
+int cvm_match_u8_direct_device_noalloc(const uint8_t *d_img, int64_t istride, int32_t iw, int32_t ih,
+                                       const uint8_t *d_tpl, int64_t tstride, int32_t tw, int32_t th, int32_t cn,
+                                       int32_t step, float *d_map, cvm_result *out, void *stream);
 
 /* Find, FindAll and the gray conversion on frames already in device memory
  * (no reference counterpart): a caller that runs several fleets over one

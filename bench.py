@@ -8,9 +8,12 @@ resident in HBM before the timed region; each step is one complete call
 24-byte result back to the host).
 
 Prints ONE JSON line.  value = Gwindows/s (windows = result elements).
-roofline: the dominant kernel's algorithmic HBM bytes per launch / its
-average duration from HIP events recorded on the library's streams (in a
-profiling pass after the timed one).
+roofline: the call's algorithmic HBM bytes (parent + template + map, 8.40 B
+per window) / the average duration of its dominant kernel, from HIP events
+recorded on the library's streams (in a profiling pass after the timed one);
+kernel_stream_efficiency: that kernel's own compulsory bytes (the spectra it
+must stream) / the same duration.  vs_fp64 fields: |score - FP64 score| at
+seeded windows and at the reported extrema.
 cpu_baseline: the CPU oracle (a C restatement of the reference's op
 sequence) timed on this host on the same workload.
 """
@@ -19,7 +22,9 @@ from __future__ import annotations
 import argparse
 import ctypes as C
 import json
+import math
 import os
+import threading
 import sys
 import time
 
@@ -79,6 +84,51 @@ def stage_bytes(iw, ih, tw, th, cn, plan):
     return b
 
 
+def fp64_scores(img, tpl, cn, pts):
+    """The FP64 reference the deviations are measured against: TM_CCOEFF_NORMED
+    from exact integer window statistics (Python integers), the numerator,
+    denominator and normOne's guard ladder (impl.go:816-849) evaluated in
+    binary64.  img (H, W) or (H, W, C) uint8, tpl likewise; cn channels;
+    pts = [(x, y)] window origins."""
+    th, tw = tpl.shape[:2]
+    area = tw * th
+    T = tpl.reshape(th, tw, -1)[..., :cn].astype(np.int64)
+    tsum = T.sum(axis=(0, 1))
+    var_sum = int((area * (T * T).sum(axis=(0, 1)) - tsum * tsum).sum())
+    out = np.empty(len(pts), np.float64)
+    for i, (x, y) in enumerate(pts):
+        if var_sum == 0:
+            out[i] = 1.0
+            continue
+        P = img[y:y + th, x:x + tw].reshape(th, tw, -1)[..., :cn].astype(np.int64)
+        s = P.sum(axis=(0, 1))
+        num = area * int((P * T).sum()) - int((s * tsum).sum())
+        idiff = area * int((P * P).sum()) - int((s * s).sum())
+        s2 = int((P * P).sum())
+        lim = min(10.0 * 2.0 ** -23 * float(s2), 0.5)
+        if not float(idiff) / float(area) > lim:
+            out[i] = 0.0
+            continue
+        den = math.sqrt(float(idiff)) * math.sqrt(float(var_sum))
+        an = abs(float(num))
+        out[i] = float(num) / den if an < den else ((1.0 if num > 0 else -1.0) if an < den * 1.125 else 0.0)
+    return out
+
+
+def delta_stats(got, ref):
+    d = np.abs(np.asarray(got, np.float64) - np.asarray(ref, np.float64))
+    return {"max": float(d.max()) if d.size else 0.0, "mean": float(d.mean()) if d.size else 0.0, "n": int(d.size)}
+
+
+def map_delta_vs_fp64(m, img, tpl, cn, samples=4096, seed=0, extra=()):
+    """max / mean |map - FP64| over seeded windows plus the given ones."""
+    rh, rw = m.shape
+    rng = np.random.default_rng(seed)
+    pts = list(zip(rng.integers(0, rw, samples).tolist(), rng.integers(0, rh, samples).tolist())) + list(extra)
+    ref = fp64_scores(img, tpl, cn, pts)
+    return delta_stats([m[y, x] for (x, y) in pts], ref)
+
+
 def cpu_baseline(parent, sub, budget_s=3.0):
     from oracle import oracle as O
 
@@ -120,39 +170,93 @@ def cpu_baseline(parent, sub, budget_s=3.0):
     }
 
 
-def fleet_leg(frames_timed=16, solo_frames=2):
+def fleet_leg(frames_timed=63, solo_frames=8, fp64_frames=(0, 21, 42, 63)):
     """The batched Fleet over the seeded frame stream: 256 templates U[16,128]^2
-    (half with sigma-2 noise) on 2560x1440 RGBA frames, host frames in, six
-    results per member out.  Returns templates*frames/s for the direct
-    (mixed-precision: exact int8-MFMA correlation, FP64 scores) Fleet, the plain
-    Fleet (the reference's bucketed float32 FFT form, bit-exact to it) and 256
-    plain solo Finds per frame (the plain GPU form)."""
+    (half with sigma-2 noise) on 64 frames of 2560x1440 RGBA, six results per
+    member out.  templates*frames/s of the direct (mixed-precision: exact
+    int8-MFMA correlation, FP64 scores) Fleet and of the plain Fleet (the
+    reference's bucketed float32 FFT form, bit-exact to it), with the frames
+    resident in HBM (the headline form) and as host frames; the plain GPU form
+    is 256 solo Finds per frame on the same device-resident frames (host-frame
+    solo Finds, which upload the frame once per member, are reported apart).
+    Deviations: both forms' reported extrema against FP64 scores on four
+    frames, and direct against plain over every timed frame."""
     import cvmatch_amd as cvm
     from cvmatch_amd import synth
 
+    L = cvm.lib()
     frames, tpls = synth.workload_fleet()
+    frames = [np.ascontiguousarray(f) for f in frames]
     ms = [cvm.NewMatcher(t) for t in tpls]
     fl = cvm.NewFleet(*ms)
     n = len(ms)
+    fh, fw = frames[0].shape[:2]
+    dfr = []
+    for f in frames:
+        p = C.c_void_p()
+        assert L.cvm_device_alloc(f.nbytes, C.byref(p)) == 0
+        assert L.cvm_copy_h2d(p, f.ctypes.data_as(C.c_void_p), f.nbytes) == 0
+        dfr.append(p)
+    nt = min(frames_timed, len(frames) - 1)
     out = {"workload": "64-frame 2560x1440 RGBA flat-rectangle stream (alpha 255), 256 templates U[16,128]^2",
-           "frames_timed": frames_timed}
+           "frames_timed": nt}
+    res = {}
     for key, direct in (("direct", True), ("plain_fleet", False)):
-        fl.find_all(frames[0], direct=direct)  # geometry / fragment caches
-        t0 = time.perf_counter()
-        for f in range(1, 1 + frames_timed):
-            fl.find_all(frames[f % len(frames)], direct=direct)
-        dt = (time.perf_counter() - t0) / frames_timed
-        out[key] = {"tpl_frames_per_s": round(n / dt, 1), "ms_per_frame": round(dt * 1e3, 3)}
+        for where in ("device", "host"):
+            if where == "device":
+                def call(f):
+                    return fl.find_all_device(dfr[f].value, fw * 4, fw, fh, direct=direct)
+            else:
+                def call(f):
+                    return fl.find_all(frames[f], direct=direct)
+            call(0)  # geometry / fragment caches
+            got = {0: call(0)}
+            t0 = time.perf_counter()
+            for f in range(1, 1 + nt):
+                got[f] = call(f)
+            dt = (time.perf_counter() - t0) / nt
+            out[f"{key}_{where}"] = {"tpl_frames_per_s": round(n / dt, 1), "ms_per_frame": round(dt * 1e3, 3)}
+            if where == "device":
+                res[key] = got
     t0 = time.perf_counter()
     for f in range(1, 1 + solo_frames):
         for m in ms:
-            m.find(frames[f])
+            m.find_device(dfr[f].value, fw * 4, fw, fh)
     dt = (time.perf_counter() - t0) / solo_frames
-    out["plain_solo"] = {"tpl_frames_per_s": round(n / dt, 1), "ms_per_frame": round(dt * 1e3, 3),
-                         "frames_timed": solo_frames}
-    out["direct_over_plain_solo"] = round(out["direct"]["tpl_frames_per_s"] / out["plain_solo"]["tpl_frames_per_s"], 3)
-    out["max_abs_delta_vs_fp64"] = ("direct scores equal the FP64 oracle rounded to float32 "
-                                    "(checked on this workload in tests/test_gpu_fleet_fp64.py)")
+    out["plain_solo_device"] = {"tpl_frames_per_s": round(n / dt, 1), "ms_per_frame": round(dt * 1e3, 3),
+                                "frames_timed": solo_frames}
+    t0 = time.perf_counter()
+    for f in range(1, 3):
+        for m in ms:
+            m.find(frames[f])
+    dt = (time.perf_counter() - t0) / 2
+    out["plain_solo_host_frames"] = {"tpl_frames_per_s": round(n / dt, 1), "ms_per_frame": round(dt * 1e3, 3),
+                                     "frames_timed": 2, "note": "each Find uploads the 14.7 MB frame again"}
+    solo = out["plain_solo_device"]["tpl_frames_per_s"]
+    out["direct_over_plain_solo"] = round(out["direct_device"]["tpl_frames_per_s"] / solo, 3)
+    out["plain_fleet_over_plain_solo"] = round(out["plain_fleet_device"]["tpl_frames_per_s"] / solo, 3)
+    # deviations
+    live = [i for i, t in enumerate(tpls) if t[..., :3].std() > 0 or t[..., 3].std() > 0]
+    dvp = [abs(res["direct"][f][i][k] - res["plain_fleet"][f][i][k])
+           for f in res["direct"] for i in live for k in (0, 3)]
+    same_loc = np.mean([res["direct"][f][i][4:6] == res["plain_fleet"][f][i][4:6]
+                        for f in res["direct"] for i in live])
+    out["direct_vs_plain"] = {"max_abs_delta_extrema": float(max(dvp)), "mean_abs_delta_extrema": float(np.mean(dvp)),
+                              "max_location_agreement": round(float(same_loc), 4), "frames": len(res["direct"])}
+    for key in ("direct", "plain_fleet"):
+        got, ref = [], []
+        for f in fp64_frames:
+            if f not in res[key]:
+                continue
+            for i in live:
+                r = res[key][f][i]
+                cn = 3  # alpha 255 in every frame and template: the alpha rule picks 3
+                v = fp64_scores(frames[f], tpls[i], cn, [(r[1], r[2]), (r[4], r[5])])
+                got += [r[0], r[3]]
+                ref += list(v)
+        out[f"{key}_vs_fp64_at_extrema"] = dict(delta_stats(got, ref), frames=list(fp64_frames))
+    for p in dfr:
+        L.cvm_device_free(p)
     del fl, ms
     return out
 
@@ -297,19 +401,31 @@ def configs_leg(reps=15):
         if rc != 0:
             raise RuntimeError(L.cvm_last_error_detail().decode())
 
+    def host_map(dm, rw, rh):
+        m = np.empty((rh, rw), np.float32)
+        assert L.cvm_copy_d2h(m.ctypes.data_as(C.c_void_p), dm, m.nbytes) == 0
+        return m
+
+    def extrema_pts():
+        return [(out.min_x, out.min_y), (out.max_x, out.max_y)]
+
     res = {}
     parent, sub, _ = synth.workload_match_640(seed=1, alpha="random")
     rw, rh = 640 - 48 + 1, 480 - 48 + 1
     dp, ds = dev(parent), dev(sub)
     dm = dev(np.zeros(rw * rh, np.float32))
     dt = timed(lambda: call(False, dp, 640 * 4, 640, 480, ds, 48 * 4, 48, 48, 0, 4, dm))
-    res["match_640x480_rgba_48"] = {"ms": round(dt * 1e3, 4), "Gwindows_per_s": round(rw * rh / dt / 1e9, 3)}
+    res["match_640x480_rgba_48"] = {"ms": round(dt * 1e3, 4), "Gwindows_per_s": round(rw * rh / dt / 1e9, 3),
+                                    "vs_fp64": map_delta_vs_fp64(host_map(dm, rw, rh), parent, sub, 4, seed=11,
+                                                                 extra=extrema_pts())}
     gray, gsub = synth.workload_gray_ui_1080p(seed=2)[:2]
     rw, rh = 1920 - 96 + 1, 1080 - 32 + 1
     dg, dgs = dev(gray), dev(gsub)
     dm = dev(np.zeros(rw * rh, np.float32))
     dt = timed(lambda: call(False, dg, 1920, 1920, 1080, dgs, 96, 96, 32, 1, 1, dm))
-    res["match_gray_1920x1080_96x32"] = {"ms": round(dt * 1e3, 4), "Gwindows_per_s": round(rw * rh / dt / 1e9, 3)}
+    res["match_gray_1920x1080_96x32"] = {"ms": round(dt * 1e3, 4), "Gwindows_per_s": round(rw * rh / dt / 1e9, 3),
+                                         "vs_fp64": map_delta_vs_fp64(host_map(dm, rw, rh), gray, gsub, 1, seed=12,
+                                                                      extra=extrema_pts())}
     img, _ = synth.workload_direct_1440p(seed=4, sizes=())
     di = dev(img)
     dm = dev(np.zeros(2560 * 1440, np.float32))
@@ -319,7 +435,12 @@ def configs_leg(reps=15):
         dtp = dev(t)
         fft = timed(lambda: call(False, di, 2560, 2560, 1440, dtp, sz, sz, sz, 1, 1, dm))
         dirt = timed(lambda: call(True, di, 2560, 2560, 1440, dtp, sz, sz, sz, 1, 1, dm))
-        sweep.append({"template": sz, "fft_ms": round(fft * 1e3, 4), "direct_ms": round(dirt * 1e3, 4)})
+        row = {"template": sz, "fft_ms": round(fft * 1e3, 4), "direct_ms": round(dirt * 1e3, 4)}
+        if sz in (16, 24, 32, 48):  # the mixed-precision configuration: the direct map against FP64
+            rw, rh = 2560 - sz + 1, 1440 - sz + 1
+            row["direct_vs_fp64"] = map_delta_vs_fp64(host_map(dm, rw, rh), img, t, 1, samples=2048, seed=sz,
+                                                      extra=extrema_pts())
+        sweep.append(row)
     res["direct_vs_fft_gray_2560x1440"] = sweep
     cross = [r["template"] for r in sweep if r["direct_ms"] > r["fft_ms"]]
     res["crossover_template"] = cross[0] if cross else None
@@ -334,6 +455,75 @@ def configs_leg(reps=15):
     for p in bufs:
         L.cvm_device_free(p)
     return res
+
+
+def concurrency_leg(calls=480, threads=8):
+    """640x480 RGBA Match 48x48 (device resident, the six-tuple back on the
+    host) from one host thread, then from `threads` threads at once: calls/s.
+    Every call is synchronous; concurrent calls take separate contexts of the
+    library's pool (own streams, scratch and captured graph), so they overlap
+    on the device instead of queueing behind one lock."""
+    import cvmatch_amd as cvm
+    from cvmatch_amd import synth
+
+    L = cvm.lib()
+    parent, sub, _ = synth.workload_match_640(seed=1, alpha="random")
+    rw, rh = 640 - 48 + 1, 480 - 48 + 1
+    bufs = []
+
+    def dev(nbytes, src=None):
+        p = C.c_void_p()
+        assert L.cvm_device_alloc(nbytes, C.byref(p)) == 0
+        if src is not None:
+            assert L.cvm_copy_h2d(p, src.ctypes.data_as(C.c_void_p), nbytes) == 0
+        bufs.append(p)
+        return p
+
+    dp, ds = dev(parent.nbytes, parent), dev(sub.nbytes, sub)
+    maps = [dev(rw * rh * 4) for _ in range(threads)]
+    outs = [cvm.cvm_result() for _ in range(threads)]
+
+    def call(i):
+        rc = L.cvm_match_u8_device(dp, 640 * 4, 640, 480, ds, 48 * 4, 48, 48, 0, 4, maps[i], C.byref(outs[i]), None)
+        if rc != 0:
+            raise RuntimeError(L.cvm_last_error_detail().decode())
+
+    for _ in range(3):
+        call(0)
+    t0 = time.perf_counter()
+    for _ in range(calls):
+        call(0)
+    single = calls / (time.perf_counter() - t0)
+    want = outs[0].tuple()
+    gate = threading.Barrier(threads + 1)
+    errors = []
+
+    def worker(i):
+        try:
+            for _ in range(3):  # this thread's context: eager, capture, replay
+                call(i)
+            gate.wait()
+            for _ in range(calls // threads):
+                call(i)
+        except Exception as e:  # reported in the leg's output
+            errors.append(repr(e))
+            gate.abort()
+
+    ts = [threading.Thread(target=worker, args=(i,)) for i in range(threads)]
+    for t in ts:
+        t.start()
+    gate.wait()
+    t0 = time.perf_counter()
+    for t in ts:
+        t.join()
+    multi = (calls // threads) * threads / (time.perf_counter() - t0)
+    same = all(o.tuple() == want for o in outs)
+    for p in bufs:
+        L.cvm_device_free(p)
+    return {"workload": "640x480 RGBA Match 48x48, device resident, synchronous calls",
+            "single_thread_calls_per_s": round(single, 1), "threads": threads,
+            "multi_thread_calls_per_s": round(multi, 1), "speedup": round(multi / single, 3),
+            "results_identical": same, "errors": errors}
 
 
 def frames_leg(parent, sub, n=20):
@@ -398,7 +588,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-fleet", action="store_true", help="skip the Fleet leg")
     ap.add_argument("--no-host", action="store_true", help="skip the host-buffer (PCIe-inclusive) leg")
-    ap.add_argument("--fleet-frames", type=int, default=16, help="frames timed per Fleet form")
+    ap.add_argument("--fleet-frames", type=int, default=63, help="frames timed per Fleet form")
     ap.add_argument("--fleet-only", action="store_true", help="only run the Fleet leg (profiling)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run the plain copy kernel over MIB MiB (calibrates FETCH_SIZE/WRITE_SIZE)")
@@ -477,15 +667,24 @@ def main():
     sb = stage_bytes(iw, ih, tw, th, cn, plan)
     kern = {k: v for k, v in stages.items() if k in sb}
     dom = max(kern, key=lambda k: kern[k]["avg_ms"])
-    achieved = sb[dom] / (kern[dom]["avg_ms"] * 1e-3) / 1e9
+    dom_s = kern[dom]["avg_ms"] * 1e-3
+    # algorithmic bytes of one call (8.40 B per window here): the parent and
+    # the template read once, the map written once
+    bytes_alg = iw * ih * 4 + tw * th * 4 + windows * 4
+    achieved = bytes_alg / dom_s / 1e9
     traffic = None
+    traffic_src = "profiles/pipeline_4k_rgba_matchmap.md (rocprofv3 --pmc FETCH_SIZE, WRITE_SIZE; calibrated)"
     tf = os.path.join(ROOT, "profiles", "traffic_4k_rgba.json")
     if os.path.exists(tf):
         try:
             traffic = json.load(open(tf)).get(dom)
         except Exception:
             traffic = None
-    bytes_alg = iw * ih * 4 + tw * th * 4 + windows * 4
+    # the headline map against FP64 at seeded windows and both extrema
+    hmap = np.empty((rh, rw), np.float32)
+    L.cvm_copy_d2h(hmap.ctypes.data_as(C.c_void_p), d_map, hmap.nbytes)
+    vs64 = map_delta_vs_fp64(hmap, parent, sub, cn, samples=2048, seed=3,
+                             extra=[(out.min_x, out.min_y), (out.max_x, out.max_y)])
     line = {
         "metric": "Gwindows/s + %HBM roofline, 4K RGBA Match; Fleet tpl*frames/s at |d|<=1e-5",
         "value": round(value, 4),
@@ -504,7 +703,14 @@ def main():
         "hbm_frac_pipeline": round(bytes_alg / (ms * 1e-3) / (HBM_PEAK_GBS * 1e9), 5),
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": traffic,
-                     "alg_bytes_per_launch": sb[dom], "avg_launch_ms": round(kern[dom]["avg_ms"], 5)},
+                     "traffic_source": traffic_src, "alg_bytes_per_launch": bytes_alg,
+                     "alg_bytes_def": "parent + template + map of the call (8.40 B/window x 7,548,529 windows)",
+                     "avg_launch_ms": round(kern[dom]["avg_ms"], 5)},
+        "kernel_stream_efficiency": {"kernel": dom, "stage_bytes": sb[dom],
+                                     "GBps": round(sb[dom] / dom_s / 1e9, 1),
+                                     "frac": round(sb[dom] / dom_s / 1e9 / HBM_PEAK_GBS, 4),
+                                     "def": "the kernel's own compulsory spectrum/pixel/map bytes per launch"},
+        "vs_fp64": vs64,
         "stages_ms": {k: round(v["avg_ms"], 5) for k, v in stages.items()},
     }
     line["peaks"] = peaks_leg(L, d_map, rw, rh, tw, th)
@@ -514,6 +720,8 @@ def main():
         line["host_buffers"] = host_leg(parent, sub)
         line["frame_stream"] = frames_leg(parent, sub)
         line["windowrules"] = windowrules_leg()
+    if not args.no_host:
+        line["concurrency"] = concurrency_leg()
     if not args.no_fleet:
         line["fleet"] = fleet_leg(args.fleet_frames)
     if not args.no_cpu:

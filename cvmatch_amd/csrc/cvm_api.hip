@@ -21,6 +21,7 @@
 #include <memory>
 #include <mutex>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "../../include/cvmatch_hip.h"
@@ -303,6 +304,7 @@ struct Ctx {
   // a context that never records stage events (the allocation-free call,
   // which has no streams, events or buffers of its own)
   bool noprof = false;
+  std::thread::id owner;  // the thread that acquired it last
 
   int init() {
     HIPCHK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
@@ -392,10 +394,17 @@ constexpr int kMaxCtx = 8;
 int Engine::acquire(Ctx** out) {
   std::unique_lock<std::mutex> lk(mu);
   CHK(init());
+  const std::thread::id me = std::this_thread::get_id();
   for (;;) {
     if (!idle.empty()) {
-      *out = idle.back();  // the most recently used: its buffers and graph are warm
-      idle.pop_back();
+      // the context this thread used last (its buffers and captured call
+      // are sized and keyed for this caller), else the most recently used
+      size_t k = idle.size() - 1;
+      for (size_t i = 0; i < idle.size(); i++)
+        if (idle[i]->owner == me) k = i;
+      *out = idle[k];
+      idle.erase(idle.begin() + (long)k);
+      (*out)->owner = me;
       return 0;
     }
     if (created < kMaxCtx) break;
@@ -412,6 +421,7 @@ int Engine::acquire(Ctx** out) {
     cv.notify_one();
     return rc;
   }
+  x->owner = me;
   *out = x.release();
   return 0;
 }

@@ -9,6 +9,7 @@
 // which is pure integer arithmetic (newGoPlan impl.go:246-295).
 #include <stdarg.h>
 #include <stdio.h>
+#include <stdlib.h>
 #include <string.h>
 
 #include <algorithm>
@@ -252,6 +253,48 @@ static Engine& engine() {
   return *e;
 }
 
+// Captured calls: up to kGraphKeys argument keys (a caller that cycles
+// through a few frame buffers keeps each one's graph), each with its
+// sightings and, from the second one on, its executable graph; the least
+// recently used key goes first.
+constexpr int kGraphKeys = 4;
+struct GraphSlot {
+  struct Entry {
+    std::vector<uint64_t> key;
+    int seen = 0;
+    hipGraphExec_t exec = nullptr;
+    unsigned long long used = 0;
+  };
+  std::vector<Entry> e;
+  unsigned long long tick = 0;
+  Entry& find(const std::vector<uint64_t>& key) {
+    for (auto& x : e)
+      if (x.key == key) {
+        x.used = ++tick;
+        return x;
+      }
+    Entry* slot;
+    if ((int)e.size() < kGraphKeys) {
+      e.emplace_back();
+      slot = &e.back();
+    } else {
+      slot = &e[0];
+      for (auto& x : e)
+        if (x.used < slot->used) slot = &x;
+      if (slot->exec) (void)hipGraphExecDestroy(slot->exec);
+      *slot = Entry();
+    }
+    slot->key = key;
+    slot->used = ++tick;
+    return *slot;
+  }
+  void drop() {
+    for (auto& x : e)
+      if (x.exec) (void)hipGraphExecDestroy(x.exec);
+    e.clear();
+  }
+};
+
 // One call context: the streams, events and grow-only scratch of one call in
 // flight.  A call takes a context from the engine's pool for its duration
 // (calls return with their work complete, so the next user of the context
@@ -265,30 +308,8 @@ struct Ctx {
   // spectrum) and before the normalisation (its partials)
   hipStream_t side = nullptr;
   hipEvent_t fork = nullptr, join = nullptr, join2 = nullptr;
-  // one captured device-resident call: replayed while the same arguments
-  // repeat and no buffer has moved (the first sighting runs eagerly and
-  // allocates, the second captures -- a few milliseconds once -- later ones
-  // replay)
-  struct CallKey {
-    const void *img, *tpl, *map;
-    int64_t istride, tstride;
-    int iw, ih, tw, th, cn, step, direct;
-    hipStream_t s;
-    unsigned long long epoch;
-    bool operator==(const CallKey& o) const {
-      return img == o.img && tpl == o.tpl && map == o.map && istride == o.istride && tstride == o.tstride &&
-             iw == o.iw && ih == o.ih && tw == o.tw && th == o.th && cn == o.cn && step == o.step &&
-             direct == o.direct && s == o.s && epoch == o.epoch;
-    }
-  };
-  CallKey gkey{};
-  int gseen = 0;  // consecutive sightings of gkey
-  hipGraphExec_t gexec = nullptr;
-  void drop_graph() {
-    if (gexec) (void)hipGraphExecDestroy(gexec);
-    gexec = nullptr;
-    gseen = 0;
-  }
+  // the captured device-resident call (run_graphed)
+  GraphSlot graph;
   DevBuf img, tpl, spec, tspec, tspec2, corr, parts, gray0, gray1, mapbuf, nscratch, dbf, pkscr, pkout;
   CallState* st = nullptr;
   Res6* res_d = nullptr;
@@ -330,7 +351,7 @@ struct Ctx {
 
   // Frees everything; the context's streams must be idle.
   void destroy() {
-    drop_graph();
+    graph.drop();
     for (DevBuf* b : {&img, &tpl, &spec, &tspec, &tspec2, &corr, &parts, &gray0, &gray1, &mapbuf, &nscratch, &dbf,
                       &pkscr, &pkout})
       b->release();
@@ -458,6 +479,54 @@ struct Stage {
   Stage(Ctx& x, const char* name, hipStream_t st) : X(x), s(st) { i = X.stage_begin(name, st); }
   ~Stage() { X.stage_end(i, s); }
 };
+
+// Runs `run`, which enqueues one call's work on stream s, through a graph
+// slot: eagerly the first time its key is seen (that run allocates whatever
+// the call needs), captured into a hipGraph on the second consecutive
+// sighting, and replayed from then on, which removes the host cost of every
+// launch in the call.  The key must name everything the launches depend on
+// (pointers, shapes, stream, context); any device reallocation since the
+// capture changes the allocation epoch appended to it.  A call that cannot be
+// captured (it allocates or synchronises) runs eagerly instead.  Nothing
+// inside a call fills memory with hipMemsetAsync (fills are kernels).
+template <class F>
+static int run_graphed(GraphSlot& slot, std::vector<uint64_t> key, hipStream_t s, F&& run) {
+  key.push_back(g_alloc_epoch.load());
+  key.push_back((uint64_t)(uintptr_t)s);
+  if (engine().prof.load()) {  // stage events need the eager launches
+    slot.drop();
+    return run();
+  }
+  GraphSlot::Entry& g = slot.find(key);
+  if (g.exec) {
+    HIPCHK(hipGraphLaunch(g.exec, s));
+    return 0;
+  }
+  if (++g.seen < 2) return run();
+  hipGraph_t graph = nullptr;
+  HIPCHK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+  const int rc = run();
+  const hipError_t ce = hipStreamEndCapture(s, &graph);
+  const bool moved = g_alloc_epoch.load() != key[key.size() - 2];
+  if (rc != 0 || ce != hipSuccess || moved) {
+    if (graph) (void)hipGraphDestroy(graph);
+    g.seen = 0;
+    (void)hipGetLastError();
+    return run();  // eagerly instead (and again with its own error, if any)
+  }
+  const hipError_t ie = hipGraphInstantiate(&g.exec, graph, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(graph);
+  if (ie != hipSuccess) {
+    g.exec = nullptr;
+    g.seen = 0;
+    (void)hipGetLastError();
+    return run();
+  }
+  HIPCHK(hipGraphLaunch(g.exec, s));
+  return 0;
+}
+
+static uint64_t u64p(const void* p) { return (uint64_t)(uintptr_t)p; }
 
 // ------------------------------------------------------------ pipeline --
 
@@ -895,6 +964,9 @@ namespace cvm {
 static void matcher_release(cvm_matcher* m) {
   if (!m) return;
   if (m->refs.fetch_sub(1) == 1) {
+    // a later matcher may reuse this address: no captured call may replay
+    // with this one's buffers
+    g_alloc_epoch.fetch_add(1);
     if (m->d_pix) (void)hipFree(m->d_pix);
     for (auto& b : m->dbf) b.release();
     delete m;
@@ -1129,6 +1201,7 @@ struct cvm_fleet {
   void* bj_ps[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
   std::vector<std::array<int, 4>> bj_groups;   // normalisation: (q, sq32, first wg entry, wg count)
   std::vector<std::array<int, 5>> bj_dgroups;  // direct: (kstep, cmax, first wg entry, wg count, max pitch)
+  cvm::GraphSlot gplain, gdirect;  // the captured FindAll of each form
 };
 
 namespace cvm {
@@ -1473,6 +1546,15 @@ static int fleet_find_dev(Ctx& X, hipStream_t s, cvm_fleet* f, const uint8_t* d_
 // (whose float32 rounding it does not share); within float32 rounding of
 // the FP64 oracle instead.
 
+static int direct_waves_env() {
+  static int w = [] {
+    const char* v = getenv("CVMATCH_DIRECT_WAVES");
+    int x = v ? atoi(v) : 2;
+    return direct_waves_ok(x) ? x : 2;
+  }();
+  return w;
+}
+
 static int check_direct(int tw, int th, int cnMax, int kstep) {
   if ((int64_t)tw * th * cnMax >= 131072)
     return fail(CVM_E_BAD_ARGS, "cvmatch: template %dx%d (cn=%d) exceeds the direct-correlation bound area*cn < 131072",
@@ -1493,7 +1575,7 @@ static int run_direct(Ctx& X, hipStream_t s, const CallState* st, const uint8_t*
   int nparts = norm_partials_count(rw, rh, tw, th, cnMax);
   {
     Stage S(X, "direct", s);
-    if (launch_direct(d_img, istride, iw, ih, kstep, bf, tw, th, rw, rh, (int32_t*)corr, s) != 0)
+    if (launch_direct(d_img, istride, iw, ih, kstep, bf, tw, th, rw, rh, (int32_t*)corr, direct_waves_env(), s) != 0)
       return fail(CVM_E_BAD_ARGS, "cvmatch: template %dx%d is outside the direct-correlation shapes", tw, th);
   }
   if (prep) {
@@ -1785,7 +1867,7 @@ static int fleet_find_direct_dev(Ctx& X, hipStream_t s, cvm_fleet* f, const uint
     Stage S(X, "direct", s);
     for (auto& g : f->bj_dgroups)
       if (launch_direct_batch(d_img, istride, pw, ph, g[0], g[1], f->bj_direct.as<DirectJob>(),
-                              f->bj_dwg.as<int>() + g[2], g[3], g[4], s) != 0)
+                              f->bj_dwg.as<int>() + g[2], g[3], g[4], direct_waves_env(), s) != 0)
         return fail(CVM_E_BAD_ARGS, "cvmatch: fleet members outside the direct-correlation shapes");
   }
   {
@@ -1972,48 +2054,10 @@ static int match_device(const uint8_t* d_img, int64_t istride, int32_t iw, int32
     if (direct) return run_match_direct(X, s, d_img, istride, iw, ih, d_tpl, tstride, tw, th, cn, step, d_map);
     return run_match(X, s, d_img, istride, iw, ih, d_tpl, tstride, tw, th, cn, step, d_map);
   };
-  if (engine().prof.load()) {  // stage events need the eager launches
-    X.drop_graph();
-    CHK(run());
-    return finish_result(X, s, out);
-  }
-  Ctx::CallKey key{d_img, d_tpl, d_map, istride, tstride, iw, ih, tw, th, cn, step, direct ? 1 : 0, s,
-                      g_alloc_epoch.load()};
-  if (!(key == X.gkey)) {
-    X.drop_graph();
-    X.gkey = key;
-  }
-  if (X.gexec) {
-    HIPCHK(hipGraphLaunch(X.gexec, s));
-    return finish_result(X, s, out);
-  }
-  constexpr int kGraphAfter = 2;
-  if (++X.gseen < kGraphAfter) {
-    CHK(run());  // eager (the first one allocates what the call needs)
-    return finish_result(X, s, out);
-  }
-  // a run of repeats with nothing reallocated since: capture the call
-  hipGraph_t graph = nullptr;
-  HIPCHK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-  int rc = run();
-  hipError_t ce = hipStreamEndCapture(s, &graph);
-  if (rc != 0 || ce != hipSuccess || g_alloc_epoch.load() != key.epoch) {
-    if (graph) (void)hipGraphDestroy(graph);
-    X.drop_graph();
-    (void)hipGetLastError();
-    if (rc != 0) return rc;
-    CHK(run());  // run eagerly instead
-    return finish_result(X, s, out);
-  }
-  hipError_t ie = hipGraphInstantiate(&X.gexec, graph, nullptr, nullptr, 0);
-  (void)hipGraphDestroy(graph);
-  if (ie != hipSuccess) {
-    X.gexec = nullptr;
-    (void)hipGetLastError();
-    CHK(run());
-    return finish_result(X, s, out);
-  }
-  HIPCHK(hipGraphLaunch(X.gexec, s));
+  const std::vector<uint64_t> key = {u64p(d_img), u64p(d_tpl), u64p(d_map), (uint64_t)istride, (uint64_t)tstride,
+                                     (uint64_t)iw, (uint64_t)ih, (uint64_t)tw, (uint64_t)th, (uint64_t)cn,
+                                     (uint64_t)step, direct ? 1u : 0u};
+  CHK(run_graphed(X.graph, key, s, run));
   return finish_result(X, s, out);
 }
 
@@ -2219,6 +2263,7 @@ struct cvm_frames {
     cvm::Res6* res = nullptr;  // pinned
     hipEvent_t up = nullptr, done = nullptr;
     bool busy = false;
+    cvm::GraphSlot graph;  // the captured Find of this slot's frame
   };
   struct Entry {
     int64_t tag;
@@ -2317,11 +2362,19 @@ int cvm_frames_find(cvm_frames* q, cvm_matcher* m, const uint8_t* img, int64_t s
   // returns: the caller may reuse its buffer only once the upload is done
   if (xfer_h2d_source_busy(img, ib)) HIPCHK(hipEventSynchronize(sl.up));
   HIPCHK(hipStreamWaitEvent(s, sl.up, 0));
-  if (direct)
-    CHK(matcher_find_direct_dev(X, s, m, sl.img.as<uint8_t>(), stride, w, h, X.res_d));
-  else
-    CHK(matcher_find_dev(X, s, m, sl.img.as<uint8_t>(), stride, w, h, X.res_d));
-  HIPCHK(hipMemcpyAsync(sl.res, X.res_d, sizeof(Res6), hipMemcpyDeviceToHost, s));
+  // the frame's Find, result copy included, replays a captured graph per
+  // slot once the same matcher and frame shape repeat
+  auto run = [&]() -> int {
+    if (direct)
+      CHK(matcher_find_direct_dev(X, s, m, sl.img.as<uint8_t>(), stride, w, h, X.res_d));
+    else
+      CHK(matcher_find_dev(X, s, m, sl.img.as<uint8_t>(), stride, w, h, X.res_d));
+    HIPCHK(hipMemcpyAsync(sl.res, X.res_d, sizeof(Res6), hipMemcpyDeviceToHost, s));
+    return 0;
+  };
+  const std::vector<uint64_t> key = {u64p(m), u64p(sl.img.p), (uint64_t)stride, (uint64_t)w, (uint64_t)h,
+                                     direct ? 1u : 0u};
+  CHK(run_graphed(sl.graph, key, s, run));
   HIPCHK(hipEventRecord(sl.done, s));
   sl.busy = true;
   m->refs.fetch_add(1);
@@ -2367,6 +2420,7 @@ void cvm_frames_free(cvm_frames* q) {
     delete q->ctx;
   }
   for (auto& sl : q->slots) {
+    sl.graph.drop();
     sl.img.release();
     if (sl.res) (void)hipHostFree(sl.res);
     if (sl.up) (void)hipEventDestroy(sl.up);
@@ -2464,10 +2518,15 @@ static int fleet_find_all(cvm_fleet* f, const uint8_t* img, int64_t stride, int3
     pConst = hs.alpha_bad[0] == 0;
   }
   CHK(f->res.ensure(sizeof(Res6) * n));
-  if (direct)
-    CHK(fleet_find_direct_dev(X, s, f, d_img, stride, w, h, pConst, f->res.as<Res6>()));
-  else
-    CHK(fleet_find_dev(X, s, f, d_img, stride, w, h, pConst, f->res.as<Res6>(), flat));
+  // the whole FindAll is one captured graph from its second sighting on
+  // (per frame buffer, shape, parent-alpha key, form, context and stream)
+  auto run = [&]() -> int {
+    if (direct) return fleet_find_direct_dev(X, s, f, d_img, stride, w, h, pConst, f->res.as<Res6>());
+    return fleet_find_dev(X, s, f, d_img, stride, w, h, pConst, f->res.as<Res6>(), flat);
+  };
+  const std::vector<uint64_t> key = {u64p(&X), u64p(d_img), (uint64_t)stride, (uint64_t)w, (uint64_t)h,
+                                     (uint64_t)pConst};
+  CHK(run_graphed(direct ? f->gdirect : f->gplain, key, s, run));
   std::vector<Res6> hr(n);
   HIPCHK(hipMemcpyAsync(hr.data(), f->res.p, sizeof(Res6) * n, hipMemcpyDeviceToHost, s));
   HIPCHK(hipStreamSynchronize(s));
@@ -2485,6 +2544,8 @@ void cvm_fleet_free(cvm_fleet* f) {
     // FindAll returns with its work complete; this waits for a call still
     // running in another thread
     std::lock_guard<std::mutex> fl(f->call_mu);
+    f->gplain.drop();
+    f->gdirect.drop();
     f->geom.reset();
     for (DevBuf* b : {&f->frame, &f->mspec, &f->corr, &f->res, &f->pcorr, &f->pparts, &f->mspec2, &f->dstates,
                       &f->bj_direct, &f->bj_dwg, &f->bj_norm, &f->bj_nwg, &f->bj_cmaps, &f->bj_parts})

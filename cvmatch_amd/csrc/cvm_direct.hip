@@ -75,7 +75,6 @@ static int direct_pitch(int nk, int kstep) {
   return (bytes + 127) / 128 * 128 + 16;
 }
 
-size_t direct_lds_bytes(int tw, int kstep) { return (size_t)DIR_RING * direct_pitch(direct_nk(tw, kstep), kstep); }
 
 // Visit order of the K chunks within one template row: residue classes of
 // kc modulo kstep, each in increasing kc (the A window slides inside a class).
@@ -129,28 +128,24 @@ void launch_bfrag(const uint8_t* tpl, int64_t tstride, int tw, int th, int kstep
                      kstep == 1 ? 1 : 4, cn, st, nk, bf);
 }
 
-// One workgroup: output rows [Y0, Y0 + DIR_ROWS) x columns [X0, X0 + 32 T).
-// VEC: image rows 16-byte aligned with sstep*iw % 16 == 0 (16-byte loads).
-// CMAX: chunks per residue class, ceil(nk / KSTEP) <= CMAX.
-template <int KSTEP, bool VEC, int CMAX>
-__device__ __forceinline__ void direct_block(uint8_t* ring, const uint8_t* __restrict__ img, int64_t istride, int iw,
-                                             int ih, const int8_t* __restrict__ bf, int th, int nk, int pitch, int rw,
-                                             int rh, int32_t* __restrict__ out, int bx, int by) {
-  constexpr int T = DIR_T;
-  constexpr int SSTEP = KSTEP == 1 ? 1 : 4;  // source bytes per pixel
-  constexpr int SPU = KSTEP == 3 ? 12 : 16;  // staged bytes per 16 source bytes
-  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  const int X0 = bx * 32 * T, Y0 = by * DIR_ROWS;
-  const int units = (32 * (nk + KSTEP * (T - 1)) + SPU - 1) / SPU;  // staging units per row
-  const int64_t xb0 = (int64_t)SSTEP * X0;
-  const int64_t imgBytes = (int64_t)SSTEP * iw;
+// Staging of image rows for one output block (columns from X0, rows from
+// Y0): the 16 source bytes of staging unit c of staged row rho (image row
+// Y0 + rho), and their write into a staged LDS row.  VEC: a unit is wholly
+// inside or wholly outside the image; the load is clamped into the image and
+// `ok` says whether to keep it.  The int8 shift, the zeroing and the RGB
+// repacking are applied when the unit is written (put), so the load itself
+// is never waited on early.
+template <int KSTEP, bool VEC>
+struct RowStager {
+  static constexpr int SSTEP = KSTEP == 1 ? 1 : 4;  // source bytes per pixel
+  const uint8_t* __restrict__ img;
+  int64_t istride;
+  int ih, Y0;
+  int64_t xb0, imgBytes;
+  __device__ RowStager(const uint8_t* im, int64_t is, int iw, int ih_, int X0, int Y0_)
+      : img(im), istride(is), ih(ih_), Y0(Y0_), xb0((int64_t)SSTEP * X0), imgBytes((int64_t)SSTEP * iw) {}
 
-  // The 16 source bytes of staging unit c of staged row rho (image row
-  // Y0 + rho).  VEC: a unit is wholly inside or wholly outside the image; the
-  // load is clamped into the image and `ok` says whether to keep it.  The
-  // int8 shift, the zeroing and the RGB repacking are applied when the unit
-  // is written (put), so the load itself is never waited on early.
-  auto fetch = [&](int rho, int c, bool& ok) -> uint4 {
+  __device__ __forceinline__ uint4 fetch(int rho, int c, bool& ok) const {
     const int r = Y0 + rho;
     const int64_t b0 = xb0 + 16 * (int64_t)c;
     if (VEC) {
@@ -168,9 +163,9 @@ __device__ __forceinline__ void direct_block(uint8_t* ring, const uint8_t* __res
           w[b >> 2] = (w[b >> 2] & ~(255u << (8 * (b & 3)))) | ((uint32_t)p[b0 + b] << (8 * (b & 3)));
     }
     return make_uint4(w[0], w[1], w[2], w[3]);
-  };
+  }
   // write unit c of a staged row whose first byte is dst (outside bytes: 0)
-  auto put = [&](uint8_t* dst, int c, uint4 v, bool ok) {
+  __device__ static __forceinline__ void put(uint8_t* dst, int c, uint4 v, bool ok) {
     if (!ok) v = make_uint4(0x80808080u, 0x80808080u, 0x80808080u, 0x80808080u);
     v.x ^= 0x80808080u;
     v.y ^= 0x80808080u;
@@ -185,7 +180,24 @@ __device__ __forceinline__ void direct_block(uint8_t* ring, const uint8_t* __res
     } else {
       *(uint4*)(dst + 16 * c) = v;
     }
-  };
+  }
+};
+
+// One workgroup: output rows [Y0, Y0 + DIR_ROWS) x columns [X0, X0 + 32 T).
+// VEC: image rows 16-byte aligned with sstep*iw % 16 == 0 (16-byte loads).
+// CMAX: chunks per residue class, ceil(nk / KSTEP) <= CMAX.
+template <int KSTEP, bool VEC, int CMAX>
+__device__ __forceinline__ void direct_block(uint8_t* ring, const uint8_t* __restrict__ img, int64_t istride, int iw,
+                                             int ih, const int8_t* __restrict__ bf, int th, int nk, int pitch, int rw,
+                                             int rh, int32_t* __restrict__ out, int bx, int by) {
+  constexpr int T = DIR_T;
+  constexpr int SPU = KSTEP == 3 ? 12 : 16;  // staged bytes per 16 source bytes
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int X0 = bx * 32 * T, Y0 = by * DIR_ROWS;
+  const int units = (32 * (nk + KSTEP * (T - 1)) + SPU - 1) / SPU;  // staging units per row
+  const RowStager<KSTEP, VEC> stg(img, istride, iw, ih, X0, Y0);
+  auto fetch = [&](int rho, int c, bool& ok) { return stg.fetch(rho, c, ok); };
+  auto put = [&](uint8_t* dst, int c, uint4 v, bool ok) { RowStager<KSTEP, VEC>::put(dst, c, v, ok); };
 
   // initial rows [0, DIR_ROWS)
   for (int e = tid; e < DIR_ROWS * units; e += 64 * DIR_WAVES) {
@@ -369,27 +381,175 @@ __device__ __forceinline__ void direct_block(uint8_t* ring, const uint8_t* __res
   }
 }
 
-template <int KSTEP, bool VEC, int CMAX>
-__global__ __launch_bounds__(64 * DIR_WAVES) void k_direct_i8(const uint8_t* __restrict__ img, int64_t istride,
-                                                              int iw, int ih, const int8_t* __restrict__ bf, int th,
-                                                              int nk, int pitch, int rw, int rh,
-                                                              int32_t* __restrict__ out) {
+// The same output block (32 rows x 32 T columns) computed by W waves that
+// share one ring of staged rows: wave w takes the template rows i = w mod W,
+// so the W waves of a step read the overlapping row windows [W s + w,
+// W s + w + 32) and the ring holds 2 W + 31 rows (those of the step plus the W
+// being written for the next one) -- about (2 W + 31) / W rows per wave
+// instead of 33, which lets more waves share a CU.  Each step: the waves
+// write the next step's W rows (fetched a step earlier), run their template
+// row (residue classes with double-buffered A and B fragments), and meet at
+// one barrier; the slot a step writes held a row only the previous step read.
+// The W partial sums (each a sum of exact int8 products, bounded like the
+// whole) are added through LDS at the end and wave 0 stores the block.
+template <int W, int KSTEP, bool VEC, int CMAX>
+__device__ __forceinline__ void direct_block_mw(uint8_t* ring, const uint8_t* __restrict__ img, int64_t istride,
+                                                int iw, int ih, const int8_t* __restrict__ bf, int th, int nk,
+                                                int pitch, int rw, int rh, int32_t* __restrict__ out, int bx,
+                                                int by) {
+  constexpr int T = DIR_T;
+  constexpr int R = 2 * W + 31;  // ring rows
+  constexpr int SPU = KSTEP == 3 ? 12 : 16;
+  constexpr int NA = CMAX + T - 1;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int X0 = bx * 32 * T, Y0 = by * 32;
+  const int units = (32 * (nk + KSTEP * (T - 1)) + SPU - 1) / SPU;
+  const int nrows = th + 31;  // staged rows the block reads
+  const RowStager<KSTEP, VEC> stg(img, istride, iw, ih, X0, Y0);
+
+  // step 0 reads staged rows [0, W + 31)
+  for (int e = tid; e < (W + 31) * units; e += 64 * W) {
+    const int rho = e / units, c = e - rho * units;
+    bool ok;
+    const uint4 v = stg.fetch(rho, c, ok);
+    RowStager<KSTEP, VEC>::put(ring + rho * pitch, c, v, ok);
+  }
+  // this wave's row of step s is staged row W s + W + 31 + wave, fetched
+  // during step s - 1 (units past the row reload the last one, not written)
+  const int nc0 = lane, nc1 = lane + 64;
+  const int ncl0 = min(nc0, units - 1), ncl1 = min(nc1, units - 1);
+  bool k0, k1;
+  uint4 q0 = stg.fetch(W + 31 + wave, ncl0, k0), q1 = stg.fetch(W + 31 + wave, ncl1, k1);
+
+  dv16i acc[T];
+#pragma unroll
+  for (int t = 0; t < T; t++)
+#pragma unroll
+    for (int j = 0; j < 16; j++) acc[t][j] = 0;
+  const int half = lane >> 5, m = lane & 31;
+  const gdv4p bq = (gdv4p)(const dv4i*)bf + lane;
+  int cls[KSTEP], base[KSTEP];
+  {
+    int acc_s = 0;
+#pragma unroll
+    for (int rc = 0; rc < KSTEP; rc++) {
+      cls[rc] = rc < nk ? (nk - rc + KSTEP - 1) / KSTEP : 0;
+      base[rc] = acc_s;
+      acc_s += cls[rc];
+    }
+  }
+  const int lastChunk = nk + KSTEP * (T - 1) - 1;
+  dv4i A0[NA], A1[NA], B0[CMAX], B1[CMAX];
+  auto loadA = [&](dv4i* buf, const uint8_t* arow, int rc) {
+#pragma unroll
+    for (int j = 0; j < NA; j++) buf[j] = *(const dv4i*)(arow + 32 * min(rc + KSTEP * j, lastChunk));
+  };
+  auto loadB = [&](dv4i* buf, int i, int rc) {
+    const gdv4p p = bq + ((int64_t)i * nk + base[rc]) * 64;
+#pragma unroll
+    for (int j = 0; j < CMAX; j++) buf[j] = p[(int64_t)j * 64];
+  };
+  auto mma = [&](const dv4i* a, const dv4i* bb, int c) {
+#pragma unroll
+    for (int j = 0; j < CMAX; j++)
+      if (j < c)
+#pragma unroll
+        for (int t = 0; t < T; t++) acc[t] = __builtin_amdgcn_mfma_i32_32x32x32_i8(a[j + t], bb[j], acc[t], 0, 0, 0);
+  };
+  __syncthreads();
+
+  const int steps = (th + W - 1) / W;
+  for (int st = 0; st < steps; st++) {
+    const int wr = W * st + W + 31 + wave;
+    if (wr < nrows) {
+      uint8_t* dst = ring + (wr % R) * pitch;
+      if (nc0 < units) RowStager<KSTEP, VEC>::put(dst, nc0, q0, k0);
+      if (nc1 < units) RowStager<KSTEP, VEC>::put(dst, nc1, q1, k1);
+    }
+    q0 = stg.fetch(wr + W, ncl0, k0);
+    q1 = stg.fetch(wr + W, ncl1, k1);
+    const int i = W * st + wave;
+    if (i < th) {
+      const uint8_t* arow = ring + ((m + i) % R) * pitch + 16 * half;
+      loadA(A0, arow, 0);
+      loadB(B0, i, 0);
+#pragma unroll
+      for (int rc = 0; rc < KSTEP; rc++) {
+        dv4i* ca = (rc & 1) ? A1 : A0;
+        dv4i* na = (rc & 1) ? A0 : A1;
+        dv4i* cb = (rc & 1) ? B1 : B0;
+        dv4i* nb = (rc & 1) ? B0 : B1;
+        if (rc + 1 < KSTEP) {
+          loadA(na, arow, rc + 1);
+          loadB(nb, i, rc + 1);
+        }
+        mma(ca, cb, cls[rc]);
+      }
+    }
+    __syncthreads();
+  }
+
+  // add the partial sums of waves 1..W-1 into wave 0's (the ring is free)
+  if constexpr (W > 1) {
+    int32_t* red = (int32_t*)ring;  // (W - 1) x 16 x 64 words, lane-minor
+#pragma unroll
+    for (int t = 0; t < T; t++) {
+      if (wave > 0)
+#pragma unroll
+        for (int j = 0; j < 16; j++) red[((wave - 1) * 16 + j) * 64 + lane] = acc[t][j];
+      __syncthreads();
+      if (wave == 0)
+        for (int w = 1; w < W; w++)
+#pragma unroll
+          for (int j = 0; j < 16; j++) acc[t][j] += red[((w - 1) * 16 + j) * 64 + lane];
+      __syncthreads();
+    }
+  }
+  if (wave != 0) return;
+  const int yb = Y0 + 4 * half;
+#pragma unroll
+  for (int t = 0; t < T; t++) {
+    const int x = X0 + 32 * t + m;
+    if (x < rw) {
+#pragma unroll
+      for (int j = 0; j < 16; j++) {
+        const int y = yb + 8 * (j >> 2) + (j & 3);
+        if (y < rh) out[(int64_t)y * rw + x] = acc[t][j];
+      }
+    }
+  }
+}
+
+template <int W, int KSTEP, bool VEC, int CMAX>
+__device__ __forceinline__ void direct_block_w(uint8_t* ring, const uint8_t* __restrict__ img, int64_t istride,
+                                               int iw, int ih, const int8_t* __restrict__ bf, int th, int nk,
+                                               int pitch, int rw, int rh, int32_t* __restrict__ out, int bx, int by) {
+  if constexpr (W == 1)
+    direct_block<KSTEP, VEC, CMAX>(ring, img, istride, iw, ih, bf, th, nk, pitch, rw, rh, out, bx, by);
+  else
+    direct_block_mw<W, KSTEP, VEC, CMAX>(ring, img, istride, iw, ih, bf, th, nk, pitch, rw, rh, out, bx, by);
+}
+
+template <int W, int KSTEP, bool VEC, int CMAX>
+__global__ __launch_bounds__(64 * W) void k_direct_i8(const uint8_t* __restrict__ img, int64_t istride, int iw,
+                                                      int ih, const int8_t* __restrict__ bf, int th, int nk,
+                                                      int pitch, int rw, int rh, int32_t* __restrict__ out) {
   extern __shared__ uint4 ring4[];
-  direct_block<KSTEP, VEC, CMAX>((uint8_t*)ring4, img, istride, iw, ih, bf, th, nk, pitch, rw, rh, out, blockIdx.x,
-                                 blockIdx.y);
+  direct_block_w<W, KSTEP, VEC, CMAX>((uint8_t*)ring4, img, istride, iw, ih, bf, th, nk, pitch, rw, rh, out,
+                                      blockIdx.x, blockIdx.y);
 }
 
 // Many templates against one frame in one launch (a Fleet's members, all of
 // one (KSTEP, CMAX) form): workgroup b runs block b - job.wg0 of job wg_job[b].
-template <int KSTEP, bool VEC, int CMAX>
-__global__ __launch_bounds__(64 * DIR_WAVES) void k_direct_batch(const uint8_t* __restrict__ img, int64_t istride,
-                                                                 int iw, int ih, const DirectJob* __restrict__ jobs,
-                                                                 const int* __restrict__ wg_job) {
+template <int W, int KSTEP, bool VEC, int CMAX>
+__global__ __launch_bounds__(64 * W) void k_direct_batch(const uint8_t* __restrict__ img, int64_t istride, int iw,
+                                                         int ih, const DirectJob* __restrict__ jobs,
+                                                         const int* __restrict__ wg_job) {
   extern __shared__ uint4 ring4[];
   const DirectJob J = jobs[wg_job[blockIdx.x]];
   const int local = blockIdx.x - J.wg0;
-  direct_block<KSTEP, VEC, CMAX>((uint8_t*)ring4, img, istride, iw, ih, J.bf, J.th, J.nk, J.pitch, J.rw, J.rh,
-                                 J.out, local % J.gx, local / J.gx);
+  direct_block_w<W, KSTEP, VEC, CMAX>((uint8_t*)ring4, img, istride, iw, ih, J.bf, J.th, J.nk, J.pitch, J.rw, J.rh,
+                                      J.out, local % J.gx, local / J.gx);
 }
 
 int direct_pitch_of(int tw, int kstep) { return direct_pitch(direct_nk(tw, kstep), kstep); }
@@ -405,10 +565,22 @@ int direct_cmax(int nk, int kstep) {
   return 0;
 }
 
+// LDS of one workgroup of `waves` waves on rows of `pitch` bytes: the ring,
+// and for several waves also the partial sums added at the end.
+static size_t direct_lds_of(int pitch, int waves) {
+  if (waves == 1) return (size_t)DIR_RING * pitch;
+  return std::max((size_t)(2 * waves + 31) * pitch, (size_t)(waves - 1) * 16 * 64 * sizeof(int32_t));
+}
+
+size_t direct_lds_bytes(int tw, int kstep, int waves) {
+  return direct_lds_of(direct_pitch(direct_nk(tw, kstep), kstep), waves);
+}
+
+bool direct_waves_ok(int waves) { return waves == 1 || waves == 2 || waves == 4; }
+
 bool direct_shape_ok(int tw, int kstep) {
   const int nk = direct_nk(tw, kstep);
-  return direct_cmax(nk, kstep) != 0 && direct_units(nk, kstep) <= 2 * 64 * DIR_WAVES &&
-         direct_lds_bytes(tw, kstep) <= 160 * 1024;
+  return direct_cmax(nk, kstep) != 0 && direct_units(nk, kstep) <= 2 * 64 && direct_lds_bytes(tw, kstep, 1) <= 160 * 1024;
 }
 
 static bool direct_vec(const uint8_t* img, int64_t istride, int iw, int kstep) {
@@ -416,33 +588,36 @@ static bool direct_vec(const uint8_t* img, int64_t istride, int iw, int kstep) {
   return ((((uintptr_t)img) | (uintptr_t)istride | (uintptr_t)(sstep * iw)) & 15) == 0;
 }
 
-// dispatch over (kstep, cmax, vec) = key kstep * 100 + cmax * 2 + vec
-#define CVM_DIRECT_SWITCH(LAUNCH)                                                                         \
-  switch (kstep * 100 + cmax * 2 + (vec ? 1 : 0)) {                                                       \
-    LAUNCH(1, 2) LAUNCH(1, 3) LAUNCH(1, 4) LAUNCH(1, 5) LAUNCH(1, 6) LAUNCH(1, 9) LAUNCH(3, 2) LAUNCH(3, 3) \
-    LAUNCH(3, 4) LAUNCH(3, 5) LAUNCH(3, 6) LAUNCH(3, 9) LAUNCH(4, 2) LAUNCH(4, 3) LAUNCH(4, 4) LAUNCH(4, 5) \
-    LAUNCH(4, 6) LAUNCH(4, 9)                                                                             \
-    default:                                                                                              \
-      return -1;                                                                                          \
+// dispatch over (waves, kstep, cmax, vec) = key waves * 1000 + kstep * 100 + cmax * 2 + vec
+#define CVM_DIRECT_KEYS(LAUNCH, W)                                                                         \
+  LAUNCH(W, 1, 2) LAUNCH(W, 1, 3) LAUNCH(W, 1, 4) LAUNCH(W, 1, 5) LAUNCH(W, 1, 6) LAUNCH(W, 1, 9)          \
+  LAUNCH(W, 3, 2) LAUNCH(W, 3, 3) LAUNCH(W, 3, 4) LAUNCH(W, 3, 5) LAUNCH(W, 3, 6) LAUNCH(W, 3, 9)          \
+  LAUNCH(W, 4, 2) LAUNCH(W, 4, 3) LAUNCH(W, 4, 4) LAUNCH(W, 4, 5) LAUNCH(W, 4, 6) LAUNCH(W, 4, 9)
+#define CVM_DIRECT_SWITCH(LAUNCH)                                                      \
+  switch (waves * 1000 + kstep * 100 + cmax * 2 + (vec ? 1 : 0)) {                     \
+    CVM_DIRECT_KEYS(LAUNCH, 1) CVM_DIRECT_KEYS(LAUNCH, 2) CVM_DIRECT_KEYS(LAUNCH, 4) \
+    default:                                                                           \
+      return -1;                                                                       \
   }
 
 int launch_direct(const uint8_t* img, int64_t istride, int iw, int ih, int kstep, const int8_t* bf, int tw, int th,
-                  int rw, int rh, int32_t* out, hipStream_t s) {
-  if (!direct_shape_ok(tw, kstep)) return -1;
+                  int rw, int rh, int32_t* out, int waves, hipStream_t s) {
+  if (!direct_shape_ok(tw, kstep) || !direct_waves_ok(waves)) return -1;
   const int nk = direct_nk(tw, kstep);
   const int pitch = direct_pitch(nk, kstep);
   const int cmax = direct_cmax(nk, kstep);
-  const size_t lds = (size_t)DIR_RING * pitch;
+  const size_t lds = direct_lds_of(pitch, waves);
+  if (lds > 160 * 1024) return -1;
   const bool vec = direct_vec(img, istride, iw, kstep);
-  dim3 grid(direct_grid_x(rw), direct_grid_y(rh)), block(64 * DIR_WAVES);
-#define CVM_DI(KS, CM)                                                                                       \
-  case KS * 100 + CM * 2 + 1:                                                                                \
-    hipLaunchKernelGGL((k_direct_i8<KS, true, CM>), grid, block, lds, s, img, istride, iw, ih, bf, th, nk,   \
-                       pitch, rw, rh, out);                                                                  \
-    break;                                                                                                   \
-  case KS * 100 + CM * 2:                                                                                    \
-    hipLaunchKernelGGL((k_direct_i8<KS, false, CM>), grid, block, lds, s, img, istride, iw, ih, bf, th, nk,  \
-                       pitch, rw, rh, out);                                                                  \
+  dim3 grid(direct_grid_x(rw), direct_grid_y(rh)), block(64 * waves);
+#define CVM_DI(W, KS, CM)                                                                                       \
+  case W * 1000 + KS * 100 + CM * 2 + 1:                                                                        \
+    hipLaunchKernelGGL((k_direct_i8<W, KS, true, CM>), grid, block, lds, s, img, istride, iw, ih, bf, th, nk,   \
+                       pitch, rw, rh, out);                                                                     \
+    break;                                                                                                      \
+  case W * 1000 + KS * 100 + CM * 2:                                                                            \
+    hipLaunchKernelGGL((k_direct_i8<W, KS, false, CM>), grid, block, lds, s, img, istride, iw, ih, bf, th, nk,  \
+                       pitch, rw, rh, out);                                                                     \
     break;
   CVM_DIRECT_SWITCH(CVM_DI)
 #undef CVM_DI
@@ -450,24 +625,26 @@ int launch_direct(const uint8_t* img, int64_t istride, int iw, int ih, int kstep
 }
 
 int launch_direct_batch(const uint8_t* img, int64_t istride, int iw, int ih, int kstep, int cmax,
-                        const DirectJob* jobs, const int* wg_job, int nwg, int max_pitch, hipStream_t s) {
-  const size_t lds = (size_t)DIR_RING * max_pitch;
+                        const DirectJob* jobs, const int* wg_job, int nwg, int max_pitch, int waves, hipStream_t s) {
+  if (!direct_waves_ok(waves)) return -1;
+  const size_t lds = direct_lds_of(max_pitch, waves);
   if (lds > 160 * 1024 || nwg <= 0) return -1;
   const bool vec = direct_vec(img, istride, iw, kstep);
-#define CVM_DB(KS, CM)                                                                                          \
-  case KS * 100 + CM * 2 + 1:                                                                                   \
-    hipLaunchKernelGGL((k_direct_batch<KS, true, CM>), dim3(nwg), dim3(64 * DIR_WAVES), lds, s, img, istride,   \
-                       iw, ih, jobs, wg_job);                                                                   \
-    break;                                                                                                      \
-  case KS * 100 + CM * 2:                                                                                       \
-    hipLaunchKernelGGL((k_direct_batch<KS, false, CM>), dim3(nwg), dim3(64 * DIR_WAVES), lds, s, img, istride,  \
-                       iw, ih, jobs, wg_job);                                                                   \
+#define CVM_DB(W, KS, CM)                                                                                           \
+  case W * 1000 + KS * 100 + CM * 2 + 1:                                                                            \
+    hipLaunchKernelGGL((k_direct_batch<W, KS, true, CM>), dim3(nwg), dim3(64 * W), lds, s, img, istride, iw, ih,   \
+                       jobs, wg_job);                                                                               \
+    break;                                                                                                          \
+  case W * 1000 + KS * 100 + CM * 2:                                                                                \
+    hipLaunchKernelGGL((k_direct_batch<W, KS, false, CM>), dim3(nwg), dim3(64 * W), lds, s, img, istride, iw, ih,  \
+                       jobs, wg_job);                                                                               \
     break;
   CVM_DIRECT_SWITCH(CVM_DB)
 #undef CVM_DB
   return 0;
 }
 
+#undef CVM_DIRECT_KEYS
 #undef CVM_DIRECT_SWITCH
 
 }  // namespace cvm

@@ -193,20 +193,22 @@ bool norm_wave_ok(int tw, int th, int cnMax);
 int direct_kstep(int step, int cn);
 int direct_nk(int tw, int kstep);
 size_t direct_bfrag_bytes(int tw, int th, int kstep);
-size_t direct_lds_bytes(int tw, int kstep);
+// LDS of a workgroup of `waves` waves (1, 2 or 4: direct_waves_ok)
+size_t direct_lds_bytes(int tw, int kstep, int waves);
+bool direct_waves_ok(int waves);
 bool direct_shape_ok(int tw, int kstep);
 int direct_cmax(int nk, int kstep);
 void launch_bfrag(const uint8_t* tpl, int64_t tstride, int tw, int th, int kstep, int cn, const CallState* st,
                   int8_t* bf, hipStream_t s);
 int launch_direct(const uint8_t* img, int64_t istride, int iw, int ih, int kstep, const int8_t* bf, int tw, int th,
-                  int rw, int rh, int32_t* out, hipStream_t s);
+                  int rw, int rh, int32_t* out, int waves, hipStream_t s);
 // Final first-occurrence reduction into res6 (device) = {min_v, min_x, min_y, max_v, max_x, max_y}.
 // Batched forms (one launch for many templates against one frame).
 int direct_pitch_of(int tw, int kstep);
 int direct_grid_x(int rw);
 int direct_grid_y(int rh);
 int launch_direct_batch(const uint8_t* img, int64_t istride, int iw, int ih, int kstep, int cmax,
-                        const DirectJob* jobs, const int* wg_job, int nwg, int max_pitch, hipStream_t s);
+                        const DirectJob* jobs, const int* wg_job, int nwg, int max_pitch, int waves, hipStream_t s);
 void norm_form(int tw, int th, int cnMax, int* q, int* sq32);
 int norm_grid_x(int rw, int q);
 int norm_grid_y(int rh);

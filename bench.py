@@ -201,11 +201,13 @@ def fleet_leg(frames_timed=63, solo_frames=8, fp64_frames=(0, 21, 42, 63)):
     out = {"workload": "64-frame 2560x1440 RGBA flat-rectangle stream (alpha 255), 256 templates U[16,128]^2",
            "frames_timed": nt}
     res = {}
-    for key, direct in (("direct", True), ("plain_fleet", False)):
+    for key, direct, mixed in (("direct", True, False), ("plain_fleet", False, False), ("mixed", False, True)):
         for where in ("device", "host"):
+            if mixed and where == "host":
+                continue
             if where == "device":
                 def call(f):
-                    return fl.find_all_device(dfr[f].value, fw * 4, fw, fh, direct=direct)
+                    return fl.find_all_device(dfr[f].value, fw * 4, fw, fh, direct=direct, mixed=mixed)
             else:
                 def call(f):
                     return fl.find_all(frames[f], direct=direct)
@@ -232,9 +234,17 @@ def fleet_leg(frames_timed=63, solo_frames=8, fp64_frames=(0, 21, 42, 63)):
     dt = (time.perf_counter() - t0) / 2
     out["plain_solo_host_frames"] = {"tpl_frames_per_s": round(n / dt, 1), "ms_per_frame": round(dt * 1e3, 3),
                                      "frames_timed": 2, "note": "each Find uploads the 14.7 MB frame again"}
+    routed, gate, xo = fl.mixed_plan()
+    cand = [g for g in gate if g >= 0]
+    out["mixed_plan"] = {"crossover_side": xo, "routed_to_fft": int(sum(routed)), "candidates": len(cand),
+                         "gate_bound": 1e-5,
+                         "candidate_fft_dev_min": min(cand) if cand else None,
+                         "candidate_fft_dev_median": float(np.median(cand)) if cand else None,
+                         "mixed_equals_direct": res["mixed"] == res["direct"]}
     solo = out["plain_solo_device"]["tpl_frames_per_s"]
     out["direct_over_plain_solo"] = round(out["direct_device"]["tpl_frames_per_s"] / solo, 3)
     out["plain_fleet_over_plain_solo"] = round(out["plain_fleet_device"]["tpl_frames_per_s"] / solo, 3)
+    out["mixed_over_plain_solo"] = round(out["mixed_device"]["tpl_frames_per_s"] / solo, 3)
     # deviations
     live = [i for i, t in enumerate(tpls) if t[..., :3].std() > 0 or t[..., 3].std() > 0]
     dvp = [abs(res["direct"][f][i][k] - res["plain_fleet"][f][i][k])
@@ -713,17 +723,39 @@ def main():
         "vs_fp64": vs64,
         "stages_ms": {k: round(v["avg_ms"], 5) for k, v in stages.items()},
     }
+    def say(msg):  # progress on stderr; the JSON line is the only stdout
+        print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+    say(f"headline {ms:.4f} ms/step")
     line["peaks"] = peaks_leg(L, d_map, rw, rh, tw, th)
+    say("peaks leg done")
     if not args.no_host:
         line["configs"] = configs_leg()
+        say("configs leg done")
     if not args.no_host:
         line["host_buffers"] = host_leg(parent, sub)
+        say("host-buffer leg done")
         line["frame_stream"] = frames_leg(parent, sub)
+        say("frame-stream leg done")
         line["windowrules"] = windowrules_leg()
+        say("windowrules leg done")
     if not args.no_host:
         line["concurrency"] = concurrency_leg()
+        say("concurrency leg done")
     if not args.no_fleet:
         line["fleet"] = fleet_leg(args.fleet_frames)
+        say("fleet leg done")
+    rep = cvm.tune_report()
+    kinds = {}
+    for k, v in rep.items():
+        kind = k.split("/")[0]
+        kinds.setdefault(kind, {}).setdefault(str(v["choice"]), 0)
+        kinds[kind][str(v["choice"])] += 1
+    line["tuner"] = {"decisions": len(rep), "choice_counts_by_kind": kinds,
+                     "fleet_2560x1440": {k.split("@")[0]: {"choice": v["choice"], "ms": v["ms"], "source": v["source"]}
+                                         for k, v in rep.items() if k.startswith("fleet.") and "2560x1440" in k},
+                     "headline_4k": {k.split("@")[0]: v["choice"] for k, v in rep.items() if "3840x2160" in k},
+                     "sources": sorted({v["source"] for v in rep.values()})}
     if not args.no_cpu:
         cb = cpu_baseline(parent, sub)
         line["cpu_baseline"] = cb

@@ -91,6 +91,10 @@ SIGNATURES = {
                                    C.c_int),
     "cvm_match_u8_direct_device_noalloc": ([_P, _I64, _I32, _I32, _P, _I64, _I32, _I32, _I32, _I32, _P, _RES, _P],
                                            C.c_int),
+    "cvm_fleet_find_all_mixed": ([_P, _P, _I64, _I32, _I32, _P], C.c_int),
+    "cvm_fleet_find_all_mixed_device": ([_P, _P, _I64, _I32, _I32, _P, _P], C.c_int),
+    "cvm_fleet_mixed_plan": ([_P, _P, _P, _I32, C.POINTER(_I32)], C.c_int),
+    "cvm_tune_report": ([_P, _I32], C.c_int),
     "cvm_matcher_find_direct": ([_P, _P, _I64, _I32, _I32, _RES], C.c_int),
     "cvm_fleet_find_all_direct": ([_P, _P, _I64, _I32, _I32, _P], C.c_int),
     "cvm_device_alloc": ([_SZ, C.POINTER(_P)], C.c_int),
@@ -121,6 +125,7 @@ SIGNATURES = {
     "cvm_host_alloc": ([_SZ, C.POINTER(_P)], C.c_int),
     "cvm_host_free": ([_P], C.c_int),
     "cvm_selftest_copy": ([_P, _P, _SZ, _I32], C.c_int),
+    "cvm_selftest_tuner": ([], C.c_int),
 }
 
 
@@ -167,6 +172,17 @@ def host_array(shape, dtype=np.uint8) -> np.ndarray:
     fin = weakref.finalize(buf, lib().cvm_host_free, C.c_void_p(p.value))
     fin.atexit = False  # the process's exit releases it
     return np.frombuffer(buf, dtype=dt, count=count).reshape(shape)
+
+
+def tune_report() -> dict:
+    """The autotuner's decisions so far (cvm_tune_report)."""
+    import json
+
+    L = lib()
+    n = L.cvm_tune_report(None, 0)
+    buf = C.create_string_buffer(n + 1)
+    L.cvm_tune_report(C.cast(buf, C.c_void_p), n + 1)
+    return json.loads(buf.value.decode() or "{}")
 
 
 def _check(rc: int):
@@ -406,7 +422,9 @@ class Fleet:
         self._h = h
         self.step = ms[0].step
 
-    def find_all(self, parent, direct=False):
+    def find_all(self, parent, direct=False, mixed=False):
+        """FindAll; direct=True: the direct form, mixed=True: each member on
+        the form the autotuner picked for this frame geometry."""
         p, ps, pw, ph, pst = view(parent)
         if self.step == 1 and pst == 4:
             p = rgba_to_gray(p)
@@ -415,21 +433,35 @@ class Fleet:
             raise CvmatchError(CVM_E_BAD_ARGS, "cvmatch: parent format does not match the fleet")
         n = len(self._ms)
         out = (cvm_result * n)()
-        fn = _L().cvm_fleet_find_all_direct if direct else _L().cvm_fleet_find_all
+        L = _L()
+        fn = L.cvm_fleet_find_all_mixed if mixed else L.cvm_fleet_find_all_direct if direct else L.cvm_fleet_find_all
         _check(fn(self._h, _ptr(p), ps, pw, ph, C.cast(out, C.c_void_p)))
         return [Result(*out[i].tuple()) for i in range(n)]
 
     FindAll = find_all
 
-    def find_all_device(self, d_img, stride, w, h, direct=False, stream=None):
+    def find_all_device(self, d_img, stride, w, h, direct=False, stream=None, mixed=False):
         """FindAll on a frame (or a region of one) already in device memory:
         d_img is the device address of the view's first pixel, stride its
         row pitch in bytes; the view has the fleet's format."""
         n = len(self._ms)
         out = (cvm_result * n)()
-        fn = _L().cvm_fleet_find_all_direct_device if direct else _L().cvm_fleet_find_all_device
+        L = _L()
+        fn = (L.cvm_fleet_find_all_mixed_device if mixed else
+              L.cvm_fleet_find_all_direct_device if direct else L.cvm_fleet_find_all_device)
         _check(fn(self._h, C.c_void_p(int(d_img)), stride, w, h, C.cast(out, C.c_void_p), stream))
         return [Result(*out[i].tuple()) for i in range(n)]
+
+    def mixed_plan(self):
+        """The last mixed FindAll's routing: (routed flags, gate deviations,
+        crossover side) -- see cvm_fleet_mixed_plan."""
+        n = len(self._ms)
+        routed = (C.c_int32 * n)()
+        gate = (C.c_float * n)()
+        xo = C.c_int32()
+        _check(_L().cvm_fleet_mixed_plan(self._h, C.cast(routed, C.c_void_p), C.cast(gate, C.c_void_p), n,
+                                         C.byref(xo)))
+        return [bool(v) for v in routed], [float(v) for v in gate], int(xo.value)
 
     def __del__(self):
         h = getattr(self, "_h", None)

@@ -18,7 +18,8 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libcvmatch_hip.so")
-SOURCES = ["cvm_fft.hip", "cvm_norm.hip", "cvm_api.hip", "cvm_direct.hip", "cvm_xfer.hip", "cvm_peaks.hip"]
+SOURCES = ["cvm_fft.hip", "cvm_norm.hip", "cvm_api.hip", "cvm_direct.hip", "cvm_xfer.hip", "cvm_peaks.hip",
+           "cvm_tune.hip"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("CVM_OFFLOAD_ARCH", "gfx950")
 

@@ -311,6 +311,7 @@ struct Ctx {
   // the captured device-resident call (run_graphed)
   GraphSlot graph;
   DevBuf img, tpl, spec, tspec, tspec2, corr, parts, gray0, gray1, mapbuf, nscratch, dbf, pkscr, pkout;
+  DevBuf gate;  // the mixed form's two trial maps
   CallState* st = nullptr;
   Res6* res_d = nullptr;
   Res6* res_h = nullptr;
@@ -353,7 +354,7 @@ struct Ctx {
   void destroy() {
     graph.drop();
     for (DevBuf* b : {&img, &tpl, &spec, &tspec, &tspec2, &corr, &parts, &gray0, &gray1, &mapbuf, &nscratch, &dbf,
-                      &pkscr, &pkout})
+                      &pkscr, &pkout, &gate})
       b->release();
     if (st) (void)hipFree(st);
     if (res_d) (void)hipFree(res_d);
@@ -1200,8 +1201,22 @@ struct cvm_fleet {
   // of them has moved
   void* bj_ps[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
   std::vector<std::array<int, 4>> bj_groups;   // normalisation: (q, sq32, first wg entry, wg count)
-  std::vector<std::array<int, 5>> bj_dgroups;  // direct: (kstep, cmax, first wg entry, wg count, max pitch)
-  cvm::GraphSlot gplain, gdirect;  // the captured FindAll of each form
+  std::vector<std::array<int, 6>> bj_dgroups;  // direct: (kstep, cmax, first wg entry, wg count, max pitch, waves)
+  cvm::GraphSlot gplain, gdirect, gmixed;  // the captured FindAll of each form
+  // The mixed form's routing for one frame geometry (pw, ph, pConst): the
+  // non-flat members split into a direct fleet of their own (didx: their
+  // indices here) and the members routed to their plain Find (ridx); gate[i] is
+  // member i's measured max |FFT score - exact score| on the tuning frame
+  // (-1: not a candidate), crossover the template side from which members
+  // were candidates (0: none).
+  struct Mixed {
+    int pw = -1, ph = -1, pConst = -1, crossover = 0;
+    cvm_fleet* dpart = nullptr;
+    std::vector<int> didx, ridx;
+    std::vector<float> gate;
+    cvm::DevBuf idx;  // didx on the device
+  };
+  std::unique_ptr<Mixed> mixed;
 };
 
 namespace cvm {
@@ -1546,11 +1561,13 @@ static int fleet_find_dev(Ctx& X, hipStream_t s, cvm_fleet* f, const uint8_t* d_
 // (whose float32 rounding it does not share); within float32 rounding of
 // the FP64 oracle instead.
 
-static int direct_waves_env() {
+// Waves per direct workgroup when nothing is measured: CVMATCH_DIRECT_WAVES
+// (1, 2 or 4), else 1.
+static int direct_waves_default() {
   static int w = [] {
     const char* v = getenv("CVMATCH_DIRECT_WAVES");
-    int x = v ? atoi(v) : 2;
-    return direct_waves_ok(x) ? x : 2;
+    int x = v ? atoi(v) : 1;
+    return direct_waves_ok(x) ? x : 1;
   }();
   return w;
 }
@@ -1573,9 +1590,23 @@ static int run_direct(Ctx& X, hipStream_t s, const CallState* st, const uint8_t*
                       bool prep, float* d_map, float* corr, Extrema* parts, Res6* d_res) {
   int rw = iw - tw + 1, rh = ih - th + 1;
   int nparts = norm_partials_count(rw, rh, tw, th, cnMax);
+  // waves per workgroup: measured per shape on the first call (the C' map
+  // the trial launches write is rewritten below); the allocation-free call
+  // takes the default
+  int waves = direct_waves_default();
+  if (!X.noprof) {
+    char what[96];
+    snprintf(what, sizeof(what), "direct.waves/k%d/c%d/%dx%d/%dx%d", kstep, direct_cmax(direct_nk(tw, kstep), kstep),
+             iw, ih, tw, th);
+    waves = tune_choice(what, {1, 2, 4}, "CVMATCH_DIRECT_WAVES", [&](int w) {
+      return time_launches(
+          s, [&] { return launch_direct(d_img, istride, iw, ih, kstep, bf, tw, th, rw, rh, (int32_t*)corr, w, s); },
+          1);
+    });
+  }
   {
     Stage S(X, "direct", s);
-    if (launch_direct(d_img, istride, iw, ih, kstep, bf, tw, th, rw, rh, (int32_t*)corr, direct_waves_env(), s) != 0)
+    if (launch_direct(d_img, istride, iw, ih, kstep, bf, tw, th, rw, rh, (int32_t*)corr, waves, s) != 0)
       return fail(CVM_E_BAD_ARGS, "cvmatch: template %dx%d is outside the direct-correlation shapes", tw, th);
   }
   if (prep) {
@@ -1750,13 +1781,11 @@ static int fleet_find_direct_dev(Ctx& X, hipStream_t s, cvm_fleet* f, const uint
   if (moved || f->bj_pw != pw || f->bj_ph != ph || f->bj_pConst != pConst) {
     // job tables for this frame shape
     std::vector<DirectJob> dj;
-    std::vector<NormJob> nj;
     std::vector<int> order(live.size());
     for (size_t k = 0; k < live.size(); k++) order[k] = (int)k;
     std::vector<int64_t> cost(live.size());
-    size_t cm_total = 0, pt_total = 0;
-    std::vector<size_t> cm_off(live.size()), pt_off(live.size());
-    std::vector<int> qs(live.size()), sqs(live.size());
+    size_t cm_total = 0;
+    std::vector<size_t> cm_off(live.size());
     std::vector<int> ks(live.size()), cms(live.size());
     for (size_t k = 0; k < live.size(); k++) {
       cvm_matcher* m = f->ms[live[k]];
@@ -1766,18 +1795,13 @@ static int fleet_find_direct_dev(Ctx& X, hipStream_t s, cvm_fleet* f, const uint
       cost[k] = (int64_t)m->h * direct_nk(m->w, ks[k]) * direct_grid_x(rw) * direct_grid_y(rh);
       cm_off[k] = cm_total;
       cm_total += (size_t)rw * rh;
-      norm_form(m->w, m->h, cn_of(m), &qs[k], &sqs[k]);
-      pt_off[k] = pt_total;
-      pt_total += (size_t)norm_grid_x(rw, qs[k]) * norm_grid_y(rh);
     }
     std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return cost[x] > cost[y]; });
     CHK(f->bj_cmaps.ensure(sizeof(int32_t) * cm_total));
-    CHK(f->bj_parts.ensure(sizeof(Extrema) * pt_total));
     int32_t* cmaps = f->bj_cmaps.as<int32_t>();
-    Extrema* parts = f->bj_parts.as<Extrema>();
     // direct jobs grouped by (kstep, cmax), largest first inside a group
     std::vector<int> dwg;
-    std::vector<std::array<int, 5>> dgroups;  // (kstep, cmax, first wg, wg count, max pitch)
+    std::vector<std::array<int, 6>> dgroups;  // (kstep, cmax, first wg, wg count, max pitch, waves)
     for (int kst : {4, 3, 1})
       for (int cm : {9, 6, 5, 4, 3, 2}) {
         int first = (int)dwg.size(), max_pitch = 0;
@@ -1802,48 +1826,120 @@ static int fleet_find_direct_dev(Ctx& X, hipStream_t s, cvm_fleet* f, const uint
           for (int b = 0; b < nb; b++) dwg.push_back((int)dj.size());
           dj.push_back(J);
         }
-        if ((int)dwg.size() > first) dgroups.push_back({kst, cm, first, (int)dwg.size() - first, max_pitch});
+        if ((int)dwg.size() > first) dgroups.push_back({kst, cm, first, (int)dwg.size() - first, max_pitch, 1});
       }
-    // normalisation jobs grouped by wave-form variant; the reduction takes
-    // the whole job array
-    std::vector<std::array<int, 4>> groups;
-    std::vector<int> nwg;
-    for (int q : {4, 2})
-      for (int sq : {1, 0}) {
-        int first = (int)nwg.size();
-        for (int k : order) {
-          if (qs[k] != q || sqs[k] != sq) continue;
-          cvm_matcher* m = f->ms[live[k]];
-          int cn = cn_of(m);
-          int rw = pw - m->w + 1, rh = ph - m->h + 1;
-          NormJob J{};
-          J.st = states + live[k];
-          J.ps = f->dscr[cn].as<uint32_t>();
-          J.corr = (const float*)(cmaps + cm_off[k]);
-          J.parts = parts + pt_off[k];
-          J.res = d_res + live[k];
-          J.tw = m->w;
-          J.th = m->h;
-          J.rw = rw;
-          J.rh = rh;
-          J.gx = norm_grid_x(rw, q);
-          J.gy = norm_grid_y(rh);
-          J.wg0 = (int)nwg.size() - first;
-          for (int b = 0; b < J.gx * J.gy; b++) nwg.push_back((int)nj.size());
-          nj.push_back(J);
-        }
-        if ((int)nwg.size() > first) groups.push_back({q, sq, first, (int)nwg.size() - first});
-      }
-    // the wg0 of a norm job counts from its group's first entry
     CHK(f->bj_direct.ensure(sizeof(DirectJob) * dj.size()));
     CHK(f->bj_dwg.ensure(sizeof(int) * dwg.size()));
-    CHK(f->bj_norm.ensure(sizeof(NormJob) * nj.size()));
-    CHK(f->bj_nwg.ensure(sizeof(int) * nwg.size()));
     HIPCHK(hipMemcpyAsync(f->bj_direct.p, dj.data(), sizeof(DirectJob) * dj.size(), hipMemcpyHostToDevice, s));
     HIPCHK(hipMemcpyAsync(f->bj_dwg.p, dwg.data(), sizeof(int) * dwg.size(), hipMemcpyHostToDevice, s));
-    HIPCHK(hipMemcpyAsync(f->bj_norm.p, nj.data(), sizeof(NormJob) * nj.size(), hipMemcpyHostToDevice, s));
-    HIPCHK(hipMemcpyAsync(f->bj_nwg.p, nwg.data(), sizeof(int) * nwg.size(), hipMemcpyHostToDevice, s));
     HIPCHK(hipStreamSynchronize(s));  // the host vectors go out of scope
+    // waves per workgroup of each group, measured on this frame (the C'
+    // maps these trial launches write are rewritten by every FindAll)
+    for (auto& g : dgroups) {
+      char what[96];
+      snprintf(what, sizeof(what), "fleet.direct.waves/k%d/c%d/%dx%d/n%d", g[0], g[1], pw, ph, g[3]);
+      g[5] = tune_choice(what, {1, 2, 4}, "CVMATCH_DIRECT_WAVES", [&](int w) {
+        return time_launches(
+            s,
+            [&] {
+              return launch_direct_batch(d_img, istride, pw, ph, g[0], g[1], f->bj_direct.as<DirectJob>(),
+                                         f->bj_dwg.as<int>() + g[2], g[3], g[4], w, s);
+            },
+            1);
+      });
+    }
+
+    // Normalisation jobs grouped by wave-form variant (the reduction takes
+    // the whole job array; a job's wg0 counts from its group's first entry).
+    // qcap 4 gives each member the widest strip its template allows, qcap 2
+    // half-width strips; the tuner times both on this frame.
+    auto build_norm = [&](int qcap, std::vector<NormJob>& nj, std::vector<int>& nwg,
+                          std::vector<std::array<int, 4>>& groups) -> size_t {
+      nj.clear();
+      nwg.clear();
+      groups.clear();
+      std::vector<int> qs(live.size()), sqs(live.size());
+      std::vector<size_t> pt_off(live.size());
+      size_t pt_total = 0;
+      for (size_t k = 0; k < live.size(); k++) {
+        cvm_matcher* m = f->ms[live[k]];
+        norm_form(m->w, m->h, cn_of(m), &qs[k], &sqs[k]);
+        qs[k] = std::min(qs[k], qcap);
+        pt_off[k] = pt_total;
+        pt_total += (size_t)norm_grid_x(pw - m->w + 1, qs[k]) * norm_grid_y(ph - m->h + 1);
+      }
+      Extrema* parts = f->bj_parts.as<Extrema>();
+      for (int q : {4, 2})
+        for (int sq : {1, 0}) {
+          int first = (int)nwg.size();
+          for (int k : order) {
+            if (qs[k] != q || sqs[k] != sq) continue;
+            cvm_matcher* m = f->ms[live[k]];
+            int cn = cn_of(m);
+            int rw = pw - m->w + 1, rh = ph - m->h + 1;
+            NormJob J{};
+            J.st = states + live[k];
+            J.ps = f->dscr[cn].as<uint32_t>();
+            J.corr = (const float*)(cmaps + cm_off[k]);
+            J.parts = parts ? parts + pt_off[k] : nullptr;
+            J.res = d_res + live[k];
+            J.tw = m->w;
+            J.th = m->h;
+            J.rw = rw;
+            J.rh = rh;
+            J.gx = norm_grid_x(rw, q);
+            J.gy = norm_grid_y(rh);
+            J.wg0 = (int)nwg.size() - first;
+            for (int b = 0; b < J.gx * J.gy; b++) nwg.push_back((int)nj.size());
+            nj.push_back(J);
+          }
+          if ((int)nwg.size() > first) groups.push_back({q, sq, first, (int)nwg.size() - first});
+        }
+      return pt_total;
+    };
+    std::vector<NormJob> nj;
+    std::vector<int> nwg;
+    std::vector<std::array<int, 4>> groups;
+    size_t pts = std::max(build_norm(4, nj, nwg, groups), build_norm(2, nj, nwg, groups));
+    CHK(f->bj_parts.ensure(sizeof(Extrema) * pts));
+    auto upload_norm = [&](int qcap) -> int {
+      build_norm(qcap, nj, nwg, groups);
+      CHK(f->bj_norm.ensure(sizeof(NormJob) * nj.size()));
+      CHK(f->bj_nwg.ensure(sizeof(int) * nwg.size()));
+      HIPCHK(hipMemcpyAsync(f->bj_norm.p, nj.data(), sizeof(NormJob) * nj.size(), hipMemcpyHostToDevice, s));
+      HIPCHK(hipMemcpyAsync(f->bj_nwg.p, nwg.data(), sizeof(int) * nwg.size(), hipMemcpyHostToDevice, s));
+      HIPCHK(hipStreamSynchronize(s));
+      return 0;
+    };
+    for (int cn = 1; cn <= 4; cn++) {  // the strips read the chunk sums
+      if (!used[cn]) continue;
+      for (size_t i : live)
+        if (cn_of(f->ms[i]) == cn) {
+          launch_norm_prep(d_img, istride, pw, ph, f->step, f->ms[i]->w, f->ms[i]->h, cn, states + i,
+                           f->dscr[cn].p, s);
+          break;
+        }
+    }
+    char what[96];
+    snprintf(what, sizeof(what), "fleet.norm.q/%dx%d/n%zu", pw, ph, live.size());
+    int rc = 0;
+    const int qcap = tune_choice(what, {4, 2}, "CVMATCH_NORM_Q", [&](int q) -> float {
+      if (upload_norm(q) != 0) {
+        rc = -1;
+        return -1.0f;
+      }
+      return time_launches(
+          s,
+          [&] {
+            for (auto& g : groups)
+              launch_norm_batch(d_img, istride, pw, ph, f->step, g[0], g[1], true, f->bj_norm.as<NormJob>(),
+                                f->bj_nwg.as<int>() + g[2], g[3], s);
+            return 0;
+          },
+          1);
+    });
+    if (rc) return rc;
+    CHK(upload_norm(qcap));
     f->bj_dgroups = dgroups;
     f->bj_nnorm = (int)nj.size();
     f->bj_groups = groups;
@@ -1867,7 +1963,7 @@ static int fleet_find_direct_dev(Ctx& X, hipStream_t s, cvm_fleet* f, const uint
     Stage S(X, "direct", s);
     for (auto& g : f->bj_dgroups)
       if (launch_direct_batch(d_img, istride, pw, ph, g[0], g[1], f->bj_direct.as<DirectJob>(),
-                              f->bj_dwg.as<int>() + g[2], g[3], g[4], direct_waves_env(), s) != 0)
+                              f->bj_dwg.as<int>() + g[2], g[3], g[4], g[5], s) != 0)
         return fail(CVM_E_BAD_ARGS, "cvmatch: fleet members outside the direct-correlation shapes");
   }
   {
@@ -2432,6 +2528,23 @@ void cvm_frames_free(cvm_frames* q) {
 
 void cvm_matcher_free(cvm_matcher* m) { matcher_release(m); }
 
+}  // extern "C"
+
+// NewFleet fleet.go:85-100 on a non-empty member list of one step: a private
+// member list (each matcher's reference count raised) and its buckets.
+static cvm_fleet* fleet_make(const std::vector<cvm_matcher*>& ms) {
+  cvm_fleet* f = new cvm_fleet();
+  f->step = ms[0]->step;
+  for (cvm_matcher* m : ms) {
+    m->refs.fetch_add(1);
+    f->ms.push_back(m);
+  }
+  f->buckets = bucketize(f->ms);
+  return f;
+}
+
+extern "C" {
+
 int cvm_fleet_new(cvm_matcher* const* ms, int32_t n, cvm_fleet** out) {
   if (!out) return fail(CVM_E_BAD_ARGS, "cvmatch: null output handle");
   if (n <= 0 || !ms) return fail(CVM_E_EMPTY_FLEET, "cvmatch: empty fleet");
@@ -2439,45 +2552,201 @@ int cvm_fleet_new(cvm_matcher* const* ms, int32_t n, cvm_fleet** out) {
     if (!ms[i]) return fail(CVM_E_BAD_ARGS, "cvmatch: null matcher in fleet");
   for (int i = 0; i < n; i++)
     if (ms[i]->step != ms[0]->step) return fail(CVM_E_MIXED_FLEET, "cvmatch: fleet mixes color and gray matchers");
-  cvm_fleet* f = new cvm_fleet();
-  f->step = ms[0]->step;
-  for (int i = 0; i < n; i++) {
-    ms[i]->refs.fetch_add(1);
-    f->ms.push_back(ms[i]);
-  }
-  f->buckets = bucketize(f->ms);
-  *out = f;
+  *out = fleet_make(std::vector<cvm_matcher*>(ms, ms + n));
   return 0;
 }
 
 // img is a host buffer, or with on_device a device buffer (then stream, or
 // the library's stream when null, orders the call)
+// form: 0 plain, 1 direct, 2 mixed
 static int fleet_find_all(cvm_fleet* f, const uint8_t* img, int64_t stride, int32_t w, int32_t h, cvm_result* out,
-                          bool direct, bool on_device = false, void* stream = nullptr);
+                          int form, bool on_device = false, void* stream = nullptr);
 
 int cvm_fleet_find_all(cvm_fleet* f, const uint8_t* img, int64_t stride, int32_t w, int32_t h, cvm_result* out) {
-  return fleet_find_all(f, img, stride, w, h, out, false);
+  return fleet_find_all(f, img, stride, w, h, out, 0);
 }
 
 int cvm_fleet_find_all_direct(cvm_fleet* f, const uint8_t* img, int64_t stride, int32_t w, int32_t h,
                               cvm_result* out) {
-  return fleet_find_all(f, img, stride, w, h, out, true);
+  return fleet_find_all(f, img, stride, w, h, out, 1);
 }
 
 int cvm_fleet_find_all_device(cvm_fleet* f, const uint8_t* d_img, int64_t stride, int32_t w, int32_t h,
                               cvm_result* out, void* stream) {
-  return fleet_find_all(f, d_img, stride, w, h, out, false, true, stream);
+  return fleet_find_all(f, d_img, stride, w, h, out, 0, true, stream);
 }
 
 int cvm_fleet_find_all_direct_device(cvm_fleet* f, const uint8_t* d_img, int64_t stride, int32_t w, int32_t h,
                                      cvm_result* out, void* stream) {
-  return fleet_find_all(f, d_img, stride, w, h, out, true, true, stream);
+  return fleet_find_all(f, d_img, stride, w, h, out, 1, true, stream);
 }
+
+int cvm_fleet_find_all_mixed(cvm_fleet* f, const uint8_t* img, int64_t stride, int32_t w, int32_t h,
+                             cvm_result* out) {
+  return fleet_find_all(f, img, stride, w, h, out, 2);
+}
+
+int cvm_fleet_find_all_mixed_device(cvm_fleet* f, const uint8_t* d_img, int64_t stride, int32_t w, int32_t h,
+                                    cvm_result* out, void* stream) {
+  return fleet_find_all(f, d_img, stride, w, h, out, 2, true, stream);
+}
+
+int cvm_fleet_mixed_plan(cvm_fleet* f, int32_t* routed, float* gate, int32_t n, int32_t* crossover) {
+  if (!f) return fail(CVM_E_BAD_ARGS, "cvmatch: null fleet");
+  std::lock_guard<std::mutex> fl(f->call_mu);
+  if (!f->mixed) return fail(CVM_E_BAD_ARGS, "cvmatch: no mixed FindAll has run on this fleet");
+  if (n != (int32_t)f->ms.size()) return fail(CVM_E_BAD_ARGS, "cvmatch: the fleet has %zu members", f->ms.size());
+  const auto& M = *f->mixed;
+  for (int i = 0; i < n; i++) {
+    if (routed) routed[i] = 0;
+    if (gate) gate[i] = M.gate[i];
+  }
+  if (routed)
+    for (int i : M.ridx) routed[i] = 1;
+  if (crossover) *crossover = M.crossover;
+  return 0;
+}
+
+int cvm_tune_report(char* buf, int32_t cap) { return tune_report(buf, cap); }
 
 }  // extern "C"
 
+namespace cvm {
+
+__global__ void k_scatter_res(const Res6* __restrict__ src, const int* __restrict__ idx, int n,
+                              Res6* __restrict__ dst) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) dst[idx[i]] = src[i];
+}
+
+static void launch_scatter_res(const Res6* src, const int* idx, int n, Res6* dst, hipStream_t s) {
+  if (n > 0) hipLaunchKernelGGL(k_scatter_res, dim3((n + 255) / 256), dim3(256), 0, s, src, idx, n, dst);
+}
+
+__global__ void k_fill_u32(uint32_t* p, uint32_t v) {
+  if (threadIdx.x == 0) *p = v;
+}
+
+// max |a - b| over n floats as the bits of a non-negative float (which order
+// like the floats themselves; a NaN difference orders above every number)
+__global__ __launch_bounds__(256) void k_max_abs_diff(const float* __restrict__ a, const float* __restrict__ b,
+                                                      size_t n, uint32_t* out) {
+  uint32_t m = 0;
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+    m = max(m, __float_as_uint(fabsf(a[i] - b[i])) & 0x7FFFFFFFu);
+  for (int o = 32; o > 0; o >>= 1) m = max(m, (uint32_t)__shfl_xor((int)m, o, 64));
+  if ((threadIdx.x & 63) == 0) atomicMax(out, m);
+}
+
+// The largest |plain score - exact score| over member m's whole map on this
+// frame: the plain float32 FFT form (the reference's op sequence) against the
+// direct form (the FP64 score rounded once to float32).
+static int gate_delta(Ctx& X, hipStream_t s, const uint8_t* d_img, int64_t istride, int pw, int ph, cvm_matcher* m,
+                      int cn, float* out) {
+  const size_t nmap = (size_t)(pw - m->w + 1) * (size_t)(ph - m->h + 1);
+  CHK(X.gate.ensure(sizeof(float) * nmap * 2 + 256));
+  float* A = X.gate.as<float>();
+  float* B = A + nmap;
+  uint32_t* r = (uint32_t*)(B + nmap);
+  const int64_t ts = (int64_t)m->w * m->step;
+  CHK(run_match(X, s, d_img, istride, pw, ph, m->d_pix, ts, m->w, m->h, cn, m->step, A));
+  CHK(run_match_direct(X, s, d_img, istride, pw, ph, m->d_pix, ts, m->w, m->h, cn, m->step, B));
+  hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, s, r, 0u);
+  hipLaunchKernelGGL(k_max_abs_diff, dim3(512), dim3(256), 0, s, A, B, nmap, r);
+  HIPCHK(hipGetLastError());
+  uint32_t bits = 0;
+  HIPCHK(hipMemcpyAsync(&bits, r, sizeof(bits), hipMemcpyDeviceToHost, s));
+  HIPCHK(hipStreamSynchronize(s));
+  memcpy(out, &bits, sizeof(float));
+  return 0;
+}
+
+// Score bound of the mixed form: a member leaves the direct form only when
+// its FFT scores stay this close to the exact ones on the tuning frame
+// (1e-5; CVMATCH_MIXED_GATE sets another bound).
+static float mixed_gate() {
+  const char* v = getenv("CVMATCH_MIXED_GATE");
+  return v && *v ? (float)atof(v) : 1e-5f;
+}
+
+}  // namespace cvm
+
+// The mixed form's routing for the frame geometry (pw, ph, pConst), built on
+// its first FindAll: the tuner's crossover -- the smallest square template
+// side at which the plain FFT call on this frame beats the direct call -- and,
+// for each non-flat member at least that wide or tall, the gate: its FFT
+// form's largest deviation from the exact scores over its whole map on this
+// frame.  Members within mixed_gate() take the FFT form (each its plain
+// Find, the path the gate measured); members outside the direct form's
+// shapes take it regardless; the rest stay direct (as a direct fleet of their
+// own).
+static int mixed_plan(Ctx& X, hipStream_t s, cvm_fleet* f, const uint8_t* d_img, int64_t istride, int pw, int ph,
+                      int pConst) {
+  if (f->mixed && f->mixed->pw == pw && f->mixed->ph == ph && f->mixed->pConst == pConst) return 0;
+  f->gmixed.drop();
+  if (f->mixed) {
+    cvm_fleet_free(f->mixed->dpart);
+    f->mixed->idx.release();
+    f->mixed.reset();
+  }
+  const size_t n = f->ms.size();
+  const int step = f->step;
+  auto M = std::make_unique<cvm_fleet::Mixed>();
+  M->pw = pw;
+  M->ph = ph;
+  M->pConst = pConst;
+  M->gate.assign(n, -1.0f);
+  const int cnF = step == 1 ? 1 : (pConst ? 3 : 4);
+  char what[96];
+  snprintf(what, sizeof(what), "fleet.crossover/s%d/cn%d/%dx%d", step, cnF, pw, ph);
+  M->crossover = tune_value(what, "CVMATCH_CROSSOVER", 0, [&](std::vector<float>& ms) -> int {
+    for (int S : {32, 48, 64, 80, 96, 112, 128}) {
+      if (S > pw || S > ph) break;
+      const uint8_t* t = d_img + (int64_t)((ph - S) / 2) * istride + (int64_t)((pw - S) / 2) * step;
+      const float td = time_launches(
+          s, [&] { return run_match_direct(X, s, d_img, istride, pw, ph, t, istride, S, S, cnF, step, nullptr); }, 2);
+      const float tf = time_launches(
+          s, [&] { return run_match(X, s, d_img, istride, pw, ph, t, istride, S, S, cnF, step, nullptr); }, 2);
+      ms.push_back(td);
+      ms.push_back(tf);
+      if (td >= 0.0f && tf >= 0.0f && tf < td) return S;
+    }
+    return 0;
+  });
+  std::vector<cvm_matcher*> dms;
+  for (size_t i = 0; i < n; i++) {
+    cvm_matcher* m = f->ms[i];
+    if (m->varSum == 0) continue;
+    const int cn = step == 4 ? ((m->aConst && pConst) ? 3 : 4) : 1;
+    bool route = false;
+    if (check_direct(m->w, m->h, cn, direct_kstep(step, cn)) != 0) {
+      route = true;  // the FFT form is the only one this member has
+      M->gate[i] = -2.0f;
+    } else if (M->crossover > 0 && std::max(m->w, m->h) >= M->crossover) {
+      float d = 0.0f;
+      CHK(gate_delta(X, s, d_img, istride, pw, ph, m, cn, &d));
+      M->gate[i] = d;
+      route = d <= mixed_gate();
+    }
+    (route ? M->ridx : M->didx).push_back((int)i);
+    if (!route) dms.push_back(m);
+  }
+  if (!dms.empty()) {
+    M->dpart = fleet_make(dms);
+    CHK(M->dpart->res.ensure(sizeof(Res6) * dms.size()));
+  }
+  const std::vector<int>& idx = M->didx;
+  CHK(M->idx.ensure(sizeof(int) * std::max<size_t>(idx.size(), 1)));
+  if (!idx.empty()) {
+    HIPCHK(hipMemcpyAsync(M->idx.p, idx.data(), sizeof(int) * idx.size(), hipMemcpyHostToDevice, s));
+    HIPCHK(hipStreamSynchronize(s));
+  }
+  f->mixed = std::move(M);
+  return 0;
+}
+
 static int fleet_find_all(cvm_fleet* f, const uint8_t* img, int64_t stride, int32_t w, int32_t h, cvm_result* out,
-                          bool direct, bool on_device, void* stream) {
+                          int form, bool on_device, void* stream) {
   if (!f) return fail(CVM_E_BAD_ARGS, "cvmatch: null fleet");
   if (!img) return fail(CVM_E_NIL_IMAGE, "cvmatch: nil image");
   if (w <= 0 || h <= 0) return fail(CVM_E_EMPTY_IMAGE, "cvmatch: empty image");
@@ -2518,15 +2787,26 @@ static int fleet_find_all(cvm_fleet* f, const uint8_t* img, int64_t stride, int3
     pConst = hs.alpha_bad[0] == 0;
   }
   CHK(f->res.ensure(sizeof(Res6) * n));
+  if (form == 2) CHK(mixed_plan(X, s, f, d_img, stride, w, h, pConst));
   // the whole FindAll is one captured graph from its second sighting on
   // (per frame buffer, shape, parent-alpha key, form, context and stream)
   auto run = [&]() -> int {
-    if (direct) return fleet_find_direct_dev(X, s, f, d_img, stride, w, h, pConst, f->res.as<Res6>());
-    return fleet_find_dev(X, s, f, d_img, stride, w, h, pConst, f->res.as<Res6>(), flat);
+    if (form == 1) return fleet_find_direct_dev(X, s, f, d_img, stride, w, h, pConst, f->res.as<Res6>());
+    if (form == 0) return fleet_find_dev(X, s, f, d_img, stride, w, h, pConst, f->res.as<Res6>(), flat);
+    const cvm_fleet::Mixed& M = *f->mixed;
+    const int* idx = M.idx.as<int>();
+    if (M.dpart) {
+      CHK(fleet_find_direct_dev(X, s, M.dpart, d_img, stride, w, h, pConst, M.dpart->res.as<Res6>()));
+      launch_scatter_res(M.dpart->res.as<Res6>(), idx, (int)M.didx.size(), f->res.as<Res6>(), s);
+    }
+    // routed members: the plain Find each (the form their gate measured)
+    for (int i : M.ridx) CHK(matcher_find_dev(X, s, f->ms[i], d_img, stride, w, h, f->res.as<Res6>() + i));
+    HIPCHK(hipGetLastError());
+    return 0;
   };
   const std::vector<uint64_t> key = {u64p(&X), u64p(d_img), (uint64_t)stride, (uint64_t)w, (uint64_t)h,
                                      (uint64_t)pConst};
-  CHK(run_graphed(direct ? f->gdirect : f->gplain, key, s, run));
+  CHK(run_graphed(form == 1 ? f->gdirect : form == 0 ? f->gplain : f->gmixed, key, s, run));
   std::vector<Res6> hr(n);
   HIPCHK(hipMemcpyAsync(hr.data(), f->res.p, sizeof(Res6) * n, hipMemcpyDeviceToHost, s));
   HIPCHK(hipStreamSynchronize(s));
@@ -2546,6 +2826,12 @@ void cvm_fleet_free(cvm_fleet* f) {
     std::lock_guard<std::mutex> fl(f->call_mu);
     f->gplain.drop();
     f->gdirect.drop();
+    f->gmixed.drop();
+    if (f->mixed) {
+      cvm_fleet_free(f->mixed->dpart);
+      f->mixed->idx.release();
+      f->mixed.reset();
+    }
     f->geom.reset();
     for (DevBuf* b : {&f->frame, &f->mspec, &f->corr, &f->res, &f->pcorr, &f->pparts, &f->mspec2, &f->dstates,
                       &f->bj_direct, &f->bj_dwg, &f->bj_norm, &f->bj_nwg, &f->bj_cmaps, &f->bj_parts})

@@ -10,6 +10,10 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <functional>
+#include <string>
+#include <vector>
+
 namespace cvm {
 
 // ------------------------------------------------------------ geometry --
@@ -249,5 +253,23 @@ void launch_peaks(const float* map, int rw, int rh, float thresh, int rx, int ry
 
 // the copy-thread pool alone (host memory only), for its self-test
 void host_copy(void* dst, const void* src, size_t n, int threads);
+
+// Autotuner (cvm_tune.hip): measured choices among candidates that change no
+// output bit, per key, cached for the process and in a file.
+// tune_choice: the override environment variable's value when it names a
+// candidate, else the cached choice, else each candidate timed with time_of
+// (ms; negative = not applicable) and the fastest taken (the first candidate
+// when nothing was measured).  tune_note records a choice made elsewhere.
+bool tuning_enabled();
+int tune_choice(const std::string& what, const std::vector<int>& cands, const char* override_env,
+                const std::function<float(int)>& time_of);
+// tune_value: a measured value rather than a choice (the override's value,
+// else the cached one, else measure(times) -- dflt when tuning is off)
+int tune_value(const std::string& what, const char* override_env, int dflt,
+               const std::function<int(std::vector<float>&)>& measure);
+void tune_note(const std::string& what, int choice, const std::vector<float>& ms, const char* source);
+int tune_report(char* buf, int cap);
+// average ms of one run of `launch` on s (after one untimed run); -1 on failure
+float time_launches(hipStream_t s, const std::function<int()>& launch, int reps);
 
 }  // namespace cvm

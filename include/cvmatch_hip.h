@@ -184,6 +184,35 @@ int cvm_matcher_find_direct(cvm_matcher *m, const uint8_t *img, int64_t stride, 
 int cvm_fleet_find_all_direct(cvm_fleet *f, const uint8_t *img, int64_t stride, int32_t w, int32_t h,
                               cvm_result *out);
 
+/* ------------------------------------------------------ the mixed form -- */
+/* No reference counterpart: FindAll (fleet.go:176-332) with each member on
+ * the form the autotuner picks for this frame geometry.  On the first call of
+ * a geometry the tuner measures the crossover -- the smallest square template
+ * side at which the plain FFT call on this frame is faster than the direct
+ * one -- and, for every non-flat member at least that wide or tall, the
+ * largest |FFT score - exact score| over its whole map on that frame; a
+ * member within 1e-5 (CVMATCH_MIXED_GATE sets another bound) takes the FFT form (its result is then the plain Find's,
+ * bit for bit), every other member the direct form (its result is the direct
+ * Find's); members outside the direct form's shapes take the FFT form.
+ * cvm_fleet_mixed_plan reports the last plan: routed[i] = 1 for members on
+ * the FFT form, gate[i] their measured deviation (-1: not a candidate, -2:
+ * outside the direct shapes), *crossover the side (0: no member routed by
+ * speed).  n is the member count. */
+int cvm_fleet_find_all_mixed(cvm_fleet *f, const uint8_t *img, int64_t stride, int32_t w, int32_t h,
+                             cvm_result *out);
+int cvm_fleet_find_all_mixed_device(cvm_fleet *f, const uint8_t *d_img, int64_t stride, int32_t w, int32_t h,
+                                    cvm_result *out, void *stream);
+int cvm_fleet_mixed_plan(cvm_fleet *f, int32_t *routed, float *gate, int32_t n, int32_t *crossover);
+
+/* No reference counterpart: the autotuner's decisions so far as a JSON
+ * object (key -> choice, source, candidate times in ms).  Writes at most cap
+ * bytes (NUL included) and returns the full length.  The choices it makes
+ * (waves per direct workgroup, scoring strip width, the mixed form's
+ * crossover) change no output bit; CVMATCH_TUNE=0 turns measuring off,
+ * CVMATCH_TUNE_FILE names its cache file, CVMATCH_DIRECT_WAVES,
+ * CVMATCH_NORM_Q and CVMATCH_CROSSOVER force a choice. */
+int cvm_tune_report(char *buf, int32_t cap);
+
 /* ------------------------------------------------ device-resident calls -- */
 /* No reference counterpart: the same pipeline on buffers already in device
  * memory (the benchmark's "inputs resident in HBM" case and callers that
@@ -256,6 +285,8 @@ int cvm_host_free(void *p);
 /* Self-test of the host copy threads alone: copies bytes from src to dst
  * (host memory) split over `threads` threads; touches no GPU.  Tests only. */
 int cvm_selftest_copy(void *dst, const void *src, size_t bytes, int32_t threads);
+/* Self-test of the autotuner's locking (host only; tests): returns 2. */
+int cvm_selftest_tuner(void);
 int cvm_synchronize(void);
 /* Device-to-device copy by a plain streaming kernel; also the known-size
  * kernel that calibrates the HBM traffic counters. */

@@ -119,3 +119,19 @@ def test_peak_argument_errors(cvm):
         cvm.CVM_E_NIL_IMAGE
     assert L.cvm_find_peaks_device(P(8), 0, 10, 0.5, 1, 1, None, None, 0, ctypes.byref(cnt), None) == \
         cvm.CVM_E_EMPTY_IMAGE
+
+
+def test_tuner_nested_measurement_does_not_deadlock(cvm, tmp_path, monkeypatch):
+    """A tuner measurement that consults the tuner for another key (the mixed
+    form's crossover runs direct calls, which pick their own waves) must not
+    wait on the tuner's lock: run in a thread with a deadline."""
+    import threading
+
+    monkeypatch.setenv("CVMATCH_TUNE_FILE", str(tmp_path / "tune.txt"))
+    L = cvm.lib()
+    out = []
+    t = threading.Thread(target=lambda: out.append(L.cvm_selftest_tuner()), daemon=True)
+    t.start()
+    t.join(timeout=20)
+    assert not t.is_alive(), "the tuner deadlocked on a nested measurement"
+    assert out == [2]

@@ -619,6 +619,42 @@ static Res6* res_out(Ctx& X) { return X.rmap_d ? X.rmap_d : X.res_d; }
 // X.join2 once the normalisation partials are.  It runs after the parent's row pass has
 // been submitted on s, so that pass (which needs none of it) is first in
 // line and overlaps the side work.
+// Partials of the widest range of strip widths a call's normalisation may take.
+static int norm_partials_max(int rw, int rh, int tw, int th, int cnMax) {
+  int n = norm_partials_count(rw, rh, tw, th, cnMax);
+  for (int q : {4, 2, 1})
+    if (norm_q_ok(tw, th, cnMax, q)) n = std::max(n, norm_partials_count(rw, rh, tw, th, cnMax, q));
+  return n;
+}
+
+// Strip width (result columns per lane) of a call's wave-form normalisation:
+// each width that holds the template is timed on the call's own correlation
+// -- extrema only, so the map is left alone -- the first time the shape comes
+// up (a small map fills the chip only with narrow strips).  0 (the default
+// width) for the int64 form and for the allocation-free call.
+static int norm_q_for(Ctx& X, hipStream_t s, const uint8_t* d_img, int64_t istride, int iw, int ih, int step, int tw,
+                      int th, int cnMax, const CallState* st, const float* corr, Extrema* parts, void* nscratch,
+                      bool exact) {
+  if (X.noprof || !norm_wave_ok(tw, th, cnMax)) return 0;
+  std::vector<int> cands;
+  for (int q : {4, 2, 1})
+    if (norm_q_ok(tw, th, cnMax, q)) cands.push_back(q);
+  if (cands.size() < 2) return cands.empty() ? 0 : cands[0];
+  const int rw = iw - tw + 1, rh = ih - th + 1;
+  char what[112];
+  snprintf(what, sizeof(what), "norm.q/%dx%d/%dx%d/c%d/%s", iw, ih, tw, th, cnMax, exact ? "exact" : "plain");
+  return tune_choice(what, cands, "CVMATCH_NORM_Q", [&](int q) {
+    return time_launches(
+        s,
+        [&] {
+          launch_normalize(d_img, istride, iw, ih, step, tw, th, rw, rh, cnMax, st, corr, nullptr, parts, nscratch, s,
+                           exact, q);
+          return 0;
+        },
+        2);
+  });
+}
+
 using SideWork = std::function<int()>;
 
 static int corr_and_normalize(Ctx& X, hipStream_t s, const CallState* st, const uint8_t* d_img, int64_t istride,
@@ -656,8 +692,8 @@ static int corr_and_normalize(Ctx& X, hipStream_t s, const CallState* st, const 
     CHK(X.corr.ensure(sizeof(float) * (size_t)rw * rh));
     corr = X.corr.as<float>();
   }
-  int nparts = norm_partials_count(rw, rh, tw, th, cnMax);
-  CHK(X.parts.ensure(sizeof(Extrema) * (size_t)nparts));
+  // room for the partials of any strip width the tuner may try
+  CHK(X.parts.ensure(sizeof(Extrema) * (size_t)norm_partials_max(rw, rh, tw, th, cnMax)));
   if (!side) CHK(X.nscratch.ensure(norm_scratch_bytes(iw, ih, tw, th, cnMax)));
   if (side) {
     HIPCHK(hipEventRecord(X.fork, s));
@@ -686,14 +722,17 @@ static int corr_and_normalize(Ctx& X, hipStream_t s, const CallState* st, const 
     Stage S(X, "norm_prep", s);
     launch_norm_prep(d_img, istride, iw, ih, step, tw, th, cnMax, st, X.nscratch.p, s);
   }
+  const int q = norm_q_for(X, s, d_img, istride, iw, ih, step, tw, th, cnMax, st, corr, X.parts.as<Extrema>(),
+                           X.nscratch.p, false);
   {
     Stage S(X, "normalize", s);
     launch_normalize(d_img, istride, iw, ih, step, tw, th, rw, rh, cnMax, st, corr, d_map, X.parts.as<Extrema>(),
-                     X.nscratch.p, s);
+                     X.nscratch.p, s, false, q);
   }
   {
     Stage S(X, "reduce", s);
-    launch_reduce(X.parts.as<Extrema>(), nparts, st, rw, d_res, nullptr, rh, s);
+    launch_reduce(X.parts.as<Extrema>(), norm_partials_count(rw, rh, tw, th, cnMax, q), st, rw, d_res, nullptr, rh,
+                  s);
   }
   HIPCHK(hipGetLastError());
   return 0;
@@ -1584,12 +1623,11 @@ static int check_direct(int tw, int th, int cnMax, int kstep) {
 // Direct correlation + exact normalisation + extrema of one template whose
 // B fragments are ready; the call state is final on s.  corr receives the
 // int32 C' map (the caller's map, rescored in place, or scratch) and parts
-// the extrema partials (norm_partials_count entries).
+// the extrema partials (norm_partials_max entries).
 static int run_direct(Ctx& X, hipStream_t s, const CallState* st, const uint8_t* d_img, int64_t istride, int iw,
                       int ih, int tw, int th, int step, int kstep, int cnMax, const int8_t* bf, void* nscratch,
                       bool prep, float* d_map, float* corr, Extrema* parts, Res6* d_res) {
   int rw = iw - tw + 1, rh = ih - th + 1;
-  int nparts = norm_partials_count(rw, rh, tw, th, cnMax);
   // waves per workgroup: measured per shape on the first call (the C' map
   // the trial launches write is rewritten below); the allocation-free call
   // takes the default
@@ -1613,13 +1651,15 @@ static int run_direct(Ctx& X, hipStream_t s, const CallState* st, const uint8_t*
     Stage S(X, "norm_prep", s);
     launch_norm_prep(d_img, istride, iw, ih, step, tw, th, cnMax, st, nscratch, s);
   }
+  const int q = norm_q_for(X, s, d_img, istride, iw, ih, step, tw, th, cnMax, st, corr, parts, nscratch, true);
   {
     Stage S(X, "normalize", s);
-    launch_normalize(d_img, istride, iw, ih, step, tw, th, rw, rh, cnMax, st, corr, d_map, parts, nscratch, s, true);
+    launch_normalize(d_img, istride, iw, ih, step, tw, th, rw, rh, cnMax, st, corr, d_map, parts, nscratch, s, true,
+                     q);
   }
   {
     Stage S(X, "reduce", s);
-    launch_reduce(parts, nparts, st, rw, d_res, nullptr, rh, s);
+    launch_reduce(parts, norm_partials_count(rw, rh, tw, th, cnMax, q), st, rw, d_res, nullptr, rh, s);
   }
   HIPCHK(hipGetLastError());
   return 0;
@@ -1645,7 +1685,7 @@ static void direct_scratch_sizes(int iw, int ih, int tw, int th, int cnMax, int 
   out[0] = direct_bfrag_bytes(tw, th, kstep);
   out[1] = norm_scratch_bytes(iw, ih, tw, th, cnMax);
   out[2] = want_corr ? sizeof(float) * (size_t)rw * rh : 0;
-  out[3] = sizeof(Extrema) * (size_t)norm_partials_count(rw, rh, tw, th, cnMax);
+  out[3] = sizeof(Extrema) * (size_t)norm_partials_max(rw, rh, tw, th, cnMax);
 }
 
 // The argument rules of the direct form; sets cnMax and kstep.
@@ -1733,7 +1773,7 @@ static int matcher_find_direct_dev(Ctx& X, hipStream_t s, cvm_matcher* m, const 
   }
   const int rw = iw - m->w + 1, rh = ih - m->h + 1;
   CHK(X.corr.ensure(sizeof(float) * (size_t)rw * rh));
-  CHK(X.parts.ensure(sizeof(Extrema) * (size_t)norm_partials_count(rw, rh, m->w, m->h, cnMax)));
+  CHK(X.parts.ensure(sizeof(Extrema) * (size_t)norm_partials_max(rw, rh, m->w, m->h, cnMax)));
   return run_direct(X, s, X.st, d_img, istride, iw, ih, m->w, m->h, m->step, kstep, cnMax, bf, X.nscratch.p, true,
                     nullptr, X.corr.as<float>(), X.parts.as<Extrema>(), d_res);
 }

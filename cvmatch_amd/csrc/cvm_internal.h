@@ -175,7 +175,10 @@ void launch_call_state(const uint8_t* img, int64_t istride, int iw, int ih, cons
 // Exact-statistics normalisation + tail + per-workgroup extrema.
 // map may alias corr (in place) or be null (extrema only).
 // cnMax: the largest channel count the call can take (the device decides).
-int norm_partials_count(int rw, int rh, int tw, int th, int cnMax);
+// qpref (0, or 4, 2, 1 result columns per lane of the wave form's strips)
+// selects the strip width; a width that cannot hold the template is ignored.
+int norm_partials_count(int rw, int rh, int tw, int th, int cnMax, int qpref = 0);
+bool norm_q_ok(int tw, int th, int cnMax, int q);
 // bytes of scratch launch_normalize needs (chunk partial column sums)
 size_t norm_scratch_bytes(int iw, int ih, int tw, int th, int cnMax);
 // Per-chunk partial column sums the wave-form normalisation starts from
@@ -187,7 +190,7 @@ void launch_norm_prep(const uint8_t* img, int64_t istride, int iw, int ih, int s
 // (needs the wave form: norm_wave_ok).
 void launch_normalize(const uint8_t* img, int64_t istride, int iw, int ih, int step, int tw, int th, int rw, int rh,
                       int cnMax, const CallState* st, const float* corr, float* map, Extrema* partials,
-                      void* scratch, hipStream_t s, bool exact = false);
+                      void* scratch, hipStream_t s, bool exact = false, int qpref = 0);
 // The wave-form normalisation applies (it is the one with an exact mode).
 bool norm_wave_ok(int tw, int th, int cnMax);
 

@@ -296,12 +296,15 @@ constexpr int NORM_CPT = 8;
 // all banks).
 constexpr int NORM_NWP = 64 * NORM_CPT + 1 + (64 * NORM_CPT + 1) / 32 + 1;
 
-static NormPlan norm_plan(int tw, int th, int cnMax) {
+// qpref: 4, 2 or 1 result columns per lane when that strip holds the
+// template (64 q + tw - 1 <= 512 columns), else (or 0) the widest that does.
+static NormPlan norm_plan(int tw, int th, int cnMax, int qpref = 0) {
   NormPlan p{0, 0, 0};
   double area = (double)tw * th;
   if ((double)cnMax * 65025.0 * area * area >= 9007199254740992.0) return p;
   if (th > 257) return p;  // packed 16-bit channel column sums: 255 * th < 2^16
-  for (int q : {4, 2}) {
+  for (int q : {qpref, 4, 2}) {
+    if (q != 1 && q != 2 && q != 4) continue;
     if (64 * q + tw - 1 <= 64 * NORM_CPT) {
       p.wave = 1;
       p.q = q;
@@ -314,8 +317,13 @@ static NormPlan norm_plan(int tw, int th, int cnMax) {
 
 bool norm_wave_ok(int tw, int th, int cnMax) { return norm_plan(tw, th, cnMax).wave != 0; }
 
-int norm_partials_count(int rw, int rh, int tw, int th, int cnMax) {
-  NormPlan np = norm_plan(tw, th, cnMax);
+bool norm_q_ok(int tw, int th, int cnMax, int q) {
+  const NormPlan np = norm_plan(tw, th, cnMax, q);
+  return np.wave && np.q == q;
+}
+
+int norm_partials_count(int rw, int rh, int tw, int th, int cnMax, int qpref) {
+  NormPlan np = norm_plan(tw, th, cnMax, qpref);
   if (np.wave) return ((rw + 64 * np.q - 1) / (64 * np.q)) * ((rh + NB_ROWS - 1) / NB_ROWS);
   int br = norm_band_rows(rh);
   return ((rw + NORM_NX - 1) / NORM_NX) * ((rh + br - 1) / br);
@@ -1041,8 +1049,8 @@ void launch_norm_prep(const uint8_t* img, int64_t istride, int iw, int ih, int s
 
 void launch_normalize(const uint8_t* img, int64_t istride, int iw, int ih, int step, int tw, int th, int rw, int rh,
                       int cnMax, const CallState* st, const float* corr, float* map, Extrema* partials,
-                      void* scratch, hipStream_t s, bool exact) {
-  NormPlan np = norm_plan(tw, th, cnMax);
+                      void* scratch, hipStream_t s, bool exact, int qpref) {
+  NormPlan np = norm_plan(tw, th, cnMax, qpref);
   if (np.wave && scratch) {
     uint32_t* ps = (uint32_t*)scratch;
     const bool a16 = ((((uintptr_t)img) | (uintptr_t)istride) & 15) == 0;
@@ -1073,11 +1081,17 @@ void launch_normalize(const uint8_t* img, int64_t istride, int iw, int ih, int s
       } else {
         CVM_NWV(4, false);
       }
-    } else {
+    } else if (np.q == 2) {
       if (np.sq32) {
         CVM_NWV(2, true);
       } else {
         CVM_NWV(2, false);
+      }
+    } else {
+      if (np.sq32) {
+        CVM_NWV(1, true);
+      } else {
+        CVM_NWV(1, false);
       }
     }
 #undef CVM_NWV

@@ -522,7 +522,10 @@ def concurrency_leg(calls=480, threads=8):
     ts = [threading.Thread(target=worker, args=(i,)) for i in range(threads)]
     for t in ts:
         t.start()
-    gate.wait()
+    try:
+        gate.wait()
+    except threading.BrokenBarrierError:  # a worker failed: its error is reported below
+        pass
     t0 = time.perf_counter()
     for t in ts:
         t.join()

@@ -360,6 +360,8 @@ struct Ctx {
     if (res_d) (void)hipFree(res_d);
     if (res_h) (void)hipHostFree(res_h);
     if (rmap_h) (void)hipHostFree(rmap_h);
+    for (hipStream_t q : {stream, side})
+      if (q) (void)xfer_release_stream(q);
     for (hipEvent_t e : evpool) (void)hipEventDestroy(e);
     for (hipEvent_t e : {fork, join, join2})
       if (e) (void)hipEventDestroy(e);
@@ -2548,7 +2550,10 @@ void cvm_frames_free(cvm_frames* q) {
     for (auto& e : q->fifo) (void)frames_complete(q, e);
     q->fifo.clear();
   }
-  if (q->copy) (void)hipStreamSynchronize(q->copy);
+  if (q->copy) {
+    (void)hipStreamSynchronize(q->copy);
+    (void)xfer_release_stream(q->copy);  // staging slots last used by its uploads
+  }
   if (q->ctx) {
     (void)hipStreamSynchronize(q->ctx->stream);
     q->ctx->stage_collect();

@@ -243,6 +243,7 @@ hipError_t host_alloc(size_t bytes, void** out);
 hipError_t host_free(void* p);
 hipError_t xfer_h2d(void* dst, const void* src, size_t n, hipStream_t s);
 bool xfer_h2d_source_busy(const void* src, size_t n);
+hipError_t xfer_release_stream(hipStream_t s);
 hipError_t xfer_d2h(void* dst, const void* src, size_t n, hipStream_t s);
 // Peaks (cvm_peaks.hip): threshold + local-maximum suppression over a map
 // with rx/ry already clamped to the map (rx <= peaks_max_rx()).  Writes up to

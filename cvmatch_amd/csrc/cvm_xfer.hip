@@ -178,6 +178,7 @@ struct Staging {
   uint8_t* pin[kSlots] = {};
   hipEvent_t ev[kSlots] = {};
   bool live[kSlots] = {};
+  hipStream_t on[kSlots] = {};  // the stream ev[k] was last recorded on
   CopyPool pool;
   std::map<uintptr_t, size_t> pinned;  // cvm_host_alloc blocks: start -> bytes
 
@@ -254,6 +255,21 @@ void host_copy(void* dst, const void* src, size_t n, int threads) {
   S.pool.copy(dst, src, n, std::min(16, std::max(1, threads)));
 }
 
+// Before a stream is destroyed: every staging slot whose event was last
+// recorded on it is waited for and retired, so no later transfer waits on
+// an event of a stream that no longer exists.
+hipError_t xfer_release_stream(hipStream_t s) {
+  Staging& S = staging();
+  std::lock_guard<std::mutex> lk(S.mu);
+  hipError_t first = hipSuccess;
+  for (int k = 0; k < Staging::kSlots; k++)
+    if (S.live[k] && S.on[k] == s) {
+      const hipError_t e = S.slot_free(k);
+      if (first == hipSuccess) first = e;
+    }
+  return first;
+}
+
 // Whether xfer_h2d leaves the copy reading src after it returns: true for
 // the direct routes (a pinned block, or a buffer below the staging size).
 bool xfer_h2d_source_busy(const void* src, size_t n) {
@@ -282,6 +298,7 @@ hipError_t xfer_h2d(void* dst, const void* src, size_t n, hipStream_t s) {
     XCHK(hipMemcpyAsync((uint8_t*)dst + off, S.pin[k], len, hipMemcpyHostToDevice, s));
     XCHK(hipEventRecord(S.ev[k], s));
     S.live[k] = true;
+    S.on[k] = s;
   }
   return hipSuccess;
 }
@@ -310,6 +327,7 @@ hipError_t xfer_d2h(void* dst, const void* src, size_t n, hipStream_t s) {
                           hipMemcpyDeviceToHost, s));
       XCHK(hipEventRecord(S.ev[k], s));
       S.live[k] = true;
+      S.on[k] = s;
     }
     int k = (int)(c % Staging::kSlots);
     size_t off = c * Staging::kChunk;

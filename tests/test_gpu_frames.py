@@ -166,3 +166,31 @@ def test_frames_in_flight_beside_caller_stream_calls(cvm):
         for p in bufs:
             L.cvm_device_free(p)
         hip.hipStreamDestroy(stream)
+
+
+def test_staged_uploads_after_a_stream_is_freed(cvm):
+    """A frame stream's uploads go through the shared pinned staging ring on
+    the stream's own copy stream; freeing the stream retires those staging
+    slots first, so later host-buffer calls (whose staged copies wait on the
+    slots) work: several streams created, used with 14.7 MB pageable frames
+    and freed, each followed by staged host calls."""
+    import gc
+
+    frames, tpls = synth.workload_fleet(seed=14, frames=3, members=2)
+    frames = [np.ascontiguousarray(f) for f in frames]
+    m = cvm.NewMatcher(tpls[0])
+    want = [m.find(f) for f in frames]
+    say("[frames] create / use / free frame streams between staged host calls")
+    for rep in range(4):
+        q = cvm.FrameStream(2 + rep % 2)
+        for i, f in enumerate(frames):
+            q.find(m, f, tag=i)
+        got = []
+        while q.pending():
+            got.append(q.result())
+        assert got == list(enumerate(want)), rep
+        del q
+        gc.collect()
+        assert [m.find(f) for f in frames] == want, rep
+        fl = cvm.NewFleet(m)
+        assert tuple(fl.find_all(frames[rep % 3])[0]) == want[rep % 3], rep

@@ -220,6 +220,29 @@ def fleet_leg(frames_timed=63, solo_frames=8, fp64_frames=(0, 21, 42, 63)):
             out[f"{key}_{where}"] = {"tpl_frames_per_s": round(n / dt, 1), "ms_per_frame": round(dt * 1e3, 3)}
             if where == "device":
                 res[key] = got
+    rep = cvm.tune_report()
+    out["direct_form"] = {k.split("@")[0]: {"fused": v["choice"], "ms_fused_unfused": v["ms"], "source": v["source"]}
+                          for k, v in rep.items() if k.startswith("fleet.fused/") and f"{fw}x{fh}" in k}
+    # the direct form's unfused launches (each member's C' map written to HBM
+    # and read back by the scoring launch), forced on a Fleet of its own
+    os.environ["CVMATCH_FLEET_FUSED"] = "0"
+    try:
+        flu = cvm.NewFleet(*ms)
+        nu = min(nt, 16)
+        flu.find_all_device(dfr[0].value, fw * 4, fw, fh, direct=True)
+        flu.find_all_device(dfr[0].value, fw * 4, fw, fh, direct=True)
+        same = True
+        t0 = time.perf_counter()
+        for f in range(1, 1 + nu):
+            same &= flu.find_all_device(dfr[f].value, fw * 4, fw, fh, direct=True) == res["direct"][f]
+        dt = (time.perf_counter() - t0) / nu
+        out["direct_unfused_device"] = {"tpl_frames_per_s": round(n / dt, 1), "ms_per_frame": round(dt * 1e3, 3),
+                                        "frames_timed": nu, "equals_fused": bool(same)}
+        del flu
+    finally:
+        del os.environ["CVMATCH_FLEET_FUSED"]
+    for m in ms:  # each member's first Find tunes and caches its shape's choices: untimed
+        m.find_device(dfr[0].value, fw * 4, fw, fh)
     t0 = time.perf_counter()
     for f in range(1, 1 + solo_frames):
         for m in ms:

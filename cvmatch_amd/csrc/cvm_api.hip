@@ -1243,6 +1243,12 @@ struct cvm_fleet {
   void* bj_ps[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
   std::vector<std::array<int, 4>> bj_groups;   // normalisation: (q, sq32, first wg entry, wg count)
   std::vector<std::array<int, 6>> bj_dgroups;  // direct: (kstep, cmax, first wg entry, wg count, max pitch, waves)
+  // the fused form (correlation and scoring in one launch per group, no C'
+  // maps): job tables, reduction jobs, per-block extrema partials, groups
+  // (kstep, cmax, sq32, first wg entry, wg count, max pitch, waves)
+  cvm::DevBuf bj_fused, bj_fwg, bj_fred, bj_fparts;
+  std::vector<std::array<int, 7>> bj_fgroups;
+  int bj_nfred = 0, bj_use_fused = 0;
   cvm::GraphSlot gplain, gdirect, gmixed;  // the captured FindAll of each form
   // The mixed form's routing for one frame geometry (pw, ph, pConst): the
   // non-flat members split into a direct fleet of their own (didx: their
@@ -1782,9 +1788,12 @@ static int matcher_find_direct_dev(Ctx& X, hipStream_t s, cvm_matcher* m, const 
 
 // Fleet.FindAll in the direct form: every non-flat member against the frame
 // (results independent of the bucketing, each equal to the member's direct
-// Find).  One launch per frame correlates all members (largest first), one
-// per wave-form variant normalises them, one reduces their extrema; the
-// normalisation partials are built once per channel count.
+// Find).  The frame's chunk column sums are built once per channel count.
+// Fused form: one launch per (kstep, cmax, sq32) group correlates and scores
+// every member's blocks (largest member first), one reduces their extrema.
+// Unfused form (frames whose layout the fused kernel does not take, or when
+// it measures slower): one launch per (kstep, cmax) group writes each
+// member's C' map, one per wave-form variant scores them, one reduces.
 static int fleet_find_direct_dev(Ctx& X, hipStream_t s, cvm_fleet* f, const uint8_t* d_img, int64_t istride,
                                  int pw, int ph, int pConst, Res6* d_res) {
   size_t n = f->ms.size();
@@ -1822,138 +1831,21 @@ static int fleet_find_direct_dev(Ctx& X, hipStream_t s, cvm_fleet* f, const uint
     if (used[cn] && f->bj_ps[cn] != f->dscr[cn].p) moved = true;
   if (moved || f->bj_pw != pw || f->bj_ph != ph || f->bj_pConst != pConst) {
     // job tables for this frame shape
-    std::vector<DirectJob> dj;
     std::vector<int> order(live.size());
     for (size_t k = 0; k < live.size(); k++) order[k] = (int)k;
     std::vector<int64_t> cost(live.size());
-    size_t cm_total = 0;
-    std::vector<size_t> cm_off(live.size());
-    std::vector<int> ks(live.size()), cms(live.size());
+    std::vector<int> ks(live.size()), cms(live.size()), sq32s(live.size());
     for (size_t k = 0; k < live.size(); k++) {
       cvm_matcher* m = f->ms[live[k]];
       int rw = pw - m->w + 1, rh = ph - m->h + 1;
       ks[k] = direct_kstep(m->step, cn_of(m));
       cms[k] = direct_cmax(direct_nk(m->w, ks[k]), ks[k]);
+      int q = 0;
+      norm_form(m->w, m->h, cn_of(m), &q, &sq32s[k]);
       cost[k] = (int64_t)m->h * direct_nk(m->w, ks[k]) * direct_grid_x(rw) * direct_grid_y(rh);
-      cm_off[k] = cm_total;
-      cm_total += (size_t)rw * rh;
     }
     std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return cost[x] > cost[y]; });
-    CHK(f->bj_cmaps.ensure(sizeof(int32_t) * cm_total));
-    int32_t* cmaps = f->bj_cmaps.as<int32_t>();
-    // direct jobs grouped by (kstep, cmax), largest first inside a group
-    std::vector<int> dwg;
-    std::vector<std::array<int, 6>> dgroups;  // (kstep, cmax, first wg, wg count, max pitch, waves)
-    for (int kst : {4, 3, 1})
-      for (int cm : {9, 6, 5, 4, 3, 2}) {
-        int first = (int)dwg.size(), max_pitch = 0;
-        for (int k : order) {
-          if (ks[k] != kst || cms[k] != cm) continue;
-          cvm_matcher* m = f->ms[live[k]];
-          int rw = pw - m->w + 1, rh = ph - m->h + 1;
-          const int8_t* bf = nullptr;
-          CHK(matcher_bfrag(X, s, m, cn_of(m), &bf));
-          DirectJob J{};
-          J.bf = bf;
-          J.out = cmaps + cm_off[k];
-          J.th = m->h;
-          J.nk = direct_nk(m->w, kst);
-          J.pitch = direct_pitch_of(m->w, kst);
-          J.rw = rw;
-          J.rh = rh;
-          J.gx = direct_grid_x(rw);
-          J.wg0 = (int)dwg.size() - first;
-          max_pitch = std::max(max_pitch, J.pitch);
-          int nb = J.gx * direct_grid_y(rh);
-          for (int b = 0; b < nb; b++) dwg.push_back((int)dj.size());
-          dj.push_back(J);
-        }
-        if ((int)dwg.size() > first) dgroups.push_back({kst, cm, first, (int)dwg.size() - first, max_pitch, 1});
-      }
-    CHK(f->bj_direct.ensure(sizeof(DirectJob) * dj.size()));
-    CHK(f->bj_dwg.ensure(sizeof(int) * dwg.size()));
-    HIPCHK(hipMemcpyAsync(f->bj_direct.p, dj.data(), sizeof(DirectJob) * dj.size(), hipMemcpyHostToDevice, s));
-    HIPCHK(hipMemcpyAsync(f->bj_dwg.p, dwg.data(), sizeof(int) * dwg.size(), hipMemcpyHostToDevice, s));
-    HIPCHK(hipStreamSynchronize(s));  // the host vectors go out of scope
-    // waves per workgroup of each group, measured on this frame (the C'
-    // maps these trial launches write are rewritten by every FindAll)
-    for (auto& g : dgroups) {
-      char what[96];
-      snprintf(what, sizeof(what), "fleet.direct.waves/k%d/c%d/%dx%d/n%d", g[0], g[1], pw, ph, g[3]);
-      g[5] = tune_choice(what, {1, 2, 4}, "CVMATCH_DIRECT_WAVES", [&](int w) {
-        return time_launches(
-            s,
-            [&] {
-              return launch_direct_batch(d_img, istride, pw, ph, g[0], g[1], f->bj_direct.as<DirectJob>(),
-                                         f->bj_dwg.as<int>() + g[2], g[3], g[4], w, s);
-            },
-            1);
-      });
-    }
-
-    // Normalisation jobs grouped by wave-form variant (the reduction takes
-    // the whole job array; a job's wg0 counts from its group's first entry).
-    // qcap 4 gives each member the widest strip its template allows, qcap 2
-    // half-width strips; the tuner times both on this frame.
-    auto build_norm = [&](int qcap, std::vector<NormJob>& nj, std::vector<int>& nwg,
-                          std::vector<std::array<int, 4>>& groups) -> size_t {
-      nj.clear();
-      nwg.clear();
-      groups.clear();
-      std::vector<int> qs(live.size()), sqs(live.size());
-      std::vector<size_t> pt_off(live.size());
-      size_t pt_total = 0;
-      for (size_t k = 0; k < live.size(); k++) {
-        cvm_matcher* m = f->ms[live[k]];
-        norm_form(m->w, m->h, cn_of(m), &qs[k], &sqs[k]);
-        qs[k] = std::min(qs[k], qcap);
-        pt_off[k] = pt_total;
-        pt_total += (size_t)norm_grid_x(pw - m->w + 1, qs[k]) * norm_grid_y(ph - m->h + 1);
-      }
-      Extrema* parts = f->bj_parts.as<Extrema>();
-      for (int q : {4, 2})
-        for (int sq : {1, 0}) {
-          int first = (int)nwg.size();
-          for (int k : order) {
-            if (qs[k] != q || sqs[k] != sq) continue;
-            cvm_matcher* m = f->ms[live[k]];
-            int cn = cn_of(m);
-            int rw = pw - m->w + 1, rh = ph - m->h + 1;
-            NormJob J{};
-            J.st = states + live[k];
-            J.ps = f->dscr[cn].as<uint32_t>();
-            J.corr = (const float*)(cmaps + cm_off[k]);
-            J.parts = parts ? parts + pt_off[k] : nullptr;
-            J.res = d_res + live[k];
-            J.tw = m->w;
-            J.th = m->h;
-            J.rw = rw;
-            J.rh = rh;
-            J.gx = norm_grid_x(rw, q);
-            J.gy = norm_grid_y(rh);
-            J.wg0 = (int)nwg.size() - first;
-            for (int b = 0; b < J.gx * J.gy; b++) nwg.push_back((int)nj.size());
-            nj.push_back(J);
-          }
-          if ((int)nwg.size() > first) groups.push_back({q, sq, first, (int)nwg.size() - first});
-        }
-      return pt_total;
-    };
-    std::vector<NormJob> nj;
-    std::vector<int> nwg;
-    std::vector<std::array<int, 4>> groups;
-    size_t pts = std::max(build_norm(4, nj, nwg, groups), build_norm(2, nj, nwg, groups));
-    CHK(f->bj_parts.ensure(sizeof(Extrema) * pts));
-    auto upload_norm = [&](int qcap) -> int {
-      build_norm(qcap, nj, nwg, groups);
-      CHK(f->bj_norm.ensure(sizeof(NormJob) * nj.size()));
-      CHK(f->bj_nwg.ensure(sizeof(int) * nwg.size()));
-      HIPCHK(hipMemcpyAsync(f->bj_norm.p, nj.data(), sizeof(NormJob) * nj.size(), hipMemcpyHostToDevice, s));
-      HIPCHK(hipMemcpyAsync(f->bj_nwg.p, nwg.data(), sizeof(int) * nwg.size(), hipMemcpyHostToDevice, s));
-      HIPCHK(hipStreamSynchronize(s));
-      return 0;
-    };
-    for (int cn = 1; cn <= 4; cn++) {  // the strips read the chunk sums
+    for (int cn = 1; cn <= 4; cn++) {  // the scoring reads the chunk sums
       if (!used[cn]) continue;
       for (size_t i : live)
         if (cn_of(f->ms[i]) == cn) {
@@ -1962,29 +1854,305 @@ static int fleet_find_direct_dev(Ctx& X, hipStream_t s, cvm_fleet* f, const uint
           break;
         }
     }
-    char what[96];
-    snprintf(what, sizeof(what), "fleet.norm.q/%dx%d/n%zu", pw, ph, live.size());
-    int rc = 0;
-    const int qcap = tune_choice(what, {4, 2}, "CVMATCH_NORM_Q", [&](int q) -> float {
-      if (upload_norm(q) != 0) {
-        rc = -1;
-        return -1.0f;
+
+    // the chunk sums of a group's channel count (every member of a group has
+    // the same kstep, hence the same channel count)
+    auto ps_of = [&](int kstep) -> const uint32_t* {
+      for (size_t k = 0; k < live.size(); k++)
+        if (ks[k] == kstep) return f->dscr[cn_of(f->ms[live[k]])].as<uint32_t>();
+      return nullptr;
+    };
+
+    // The fused form: every member's blocks correlated and scored in one
+    // launch per (kstep, cmax, sq32) group; no C' maps.  Available when the
+    // frame's layout allows it for every member.
+    bool fusable = true;
+    for (size_t k = 0; k < live.size(); k++)
+      if (!fused_ok(d_img, istride, pw, ks[k], cms[k], sq32s[k] != 0)) fusable = false;
+    auto build_fused = [&]() -> int {
+      std::vector<FusedJob> fj;
+      std::vector<NormJob> rj(live.size());
+      std::vector<int> fwg;
+      std::vector<std::array<int, 7>> fgroups;  // (kstep, cmax, sq32, first wg, wg count, max pitch, waves)
+      std::vector<size_t> pt_off(live.size());
+      size_t pt_total = 0;
+      for (size_t k = 0; k < live.size(); k++) {
+        cvm_matcher* m = f->ms[live[k]];
+        pt_off[k] = pt_total;
+        pt_total += (size_t)direct_grid_x(pw - m->w + 1) * direct_grid_y(ph - m->h + 1);
       }
-      return time_launches(
-          s,
-          [&] {
-            for (auto& g : groups)
-              launch_norm_batch(d_img, istride, pw, ph, f->step, g[0], g[1], true, f->bj_norm.as<NormJob>(),
-                                f->bj_nwg.as<int>() + g[2], g[3], s);
-            return 0;
-          },
-          1);
-    });
-    if (rc) return rc;
-    CHK(upload_norm(qcap));
-    f->bj_dgroups = dgroups;
-    f->bj_nnorm = (int)nj.size();
-    f->bj_groups = groups;
+      CHK(f->bj_fparts.ensure(sizeof(Extrema) * pt_total));
+      Extrema* parts = f->bj_fparts.as<Extrema>();
+      for (int kst : {4, 3, 1})
+        for (int cm : {9, 6, 5, 4, 3, 2})
+          for (int sq : {1, 0}) {
+            int first = (int)fwg.size(), max_pitch = 0;
+            for (int k : order) {
+              if (ks[k] != kst || cms[k] != cm || sq32s[k] != sq) continue;
+              cvm_matcher* m = f->ms[live[k]];
+              int rw = pw - m->w + 1, rh = ph - m->h + 1;
+              const int8_t* bf = nullptr;
+              CHK(matcher_bfrag(X, s, m, cn_of(m), &bf));
+              FusedJob J{};
+              J.bf = bf;
+              J.st = states + live[k];
+              J.parts = parts + pt_off[k];
+              J.tw = m->w;
+              J.th = m->h;
+              J.nk = direct_nk(m->w, kst);
+              J.pitch = direct_pitch_of(m->w, kst);
+              J.rw = rw;
+              J.rh = rh;
+              J.gx = direct_grid_x(rw);
+              J.wg0 = (int)fwg.size() - first;
+              max_pitch = std::max(max_pitch, J.pitch);
+              const int nb = J.gx * direct_grid_y(rh);
+              for (int b = 0; b < nb; b++) fwg.push_back((int)fj.size());
+              fj.push_back(J);
+              NormJob R{};
+              R.st = J.st;
+              R.parts = J.parts;
+              R.res = d_res + live[k];
+              R.tw = m->w;
+              R.th = m->h;
+              R.rw = rw;
+              R.rh = rh;
+              R.gx = J.gx;
+              R.gy = direct_grid_y(rh);
+              rj[k] = R;
+            }
+            if ((int)fwg.size() > first)
+              fgroups.push_back({kst, cm, sq, first, (int)fwg.size() - first, max_pitch, 1});
+          }
+      CHK(f->bj_fused.ensure(sizeof(FusedJob) * fj.size()));
+      CHK(f->bj_fwg.ensure(sizeof(int) * fwg.size()));
+      CHK(f->bj_fred.ensure(sizeof(NormJob) * rj.size()));
+      HIPCHK(hipMemcpyAsync(f->bj_fused.p, fj.data(), sizeof(FusedJob) * fj.size(), hipMemcpyHostToDevice, s));
+      HIPCHK(hipMemcpyAsync(f->bj_fwg.p, fwg.data(), sizeof(int) * fwg.size(), hipMemcpyHostToDevice, s));
+      HIPCHK(hipMemcpyAsync(f->bj_fred.p, rj.data(), sizeof(NormJob) * rj.size(), hipMemcpyHostToDevice, s));
+      HIPCHK(hipStreamSynchronize(s));  // the host vectors go out of scope
+      // waves per workgroup of each group, measured on this frame
+      for (auto& g : fgroups) {
+        char what[112];
+        snprintf(what, sizeof(what), "fleet.fused.waves/k%d/c%d/s%d/%dx%d/n%d", g[0], g[1], g[2], pw, ph, g[4]);
+        g[6] = tune_choice(what, {1, 2, 4}, "CVMATCH_DIRECT_WAVES", [&](int w) {
+          return time_launches(
+              s,
+              [&] {
+                return launch_direct_fused(d_img, istride, pw, ph, ps_of(g[0]), g[0], g[1], g[2] != 0,
+                                           f->bj_fused.as<FusedJob>(), f->bj_fwg.as<int>() + g[3], g[4], g[5], w,
+                                           s);
+              },
+              1);
+        });
+      }
+      f->bj_fgroups = fgroups;
+      f->bj_nfred = (int)rj.size();
+      return 0;
+    };
+
+    auto build_unfused = [&]() -> int {
+      std::vector<DirectJob> dj;
+      size_t cm_total = 0;
+      std::vector<size_t> cm_off(live.size());
+      for (size_t k = 0; k < live.size(); k++) {
+        cvm_matcher* m = f->ms[live[k]];
+        cm_off[k] = cm_total;
+        cm_total += (size_t)(pw - m->w + 1) * (ph - m->h + 1);
+      }
+      CHK(f->bj_cmaps.ensure(sizeof(int32_t) * cm_total));
+      int32_t* cmaps = f->bj_cmaps.as<int32_t>();
+      // direct jobs grouped by (kstep, cmax), largest first inside a group
+      std::vector<int> dwg;
+      std::vector<std::array<int, 6>> dgroups;  // (kstep, cmax, first wg, wg count, max pitch, waves)
+      for (int kst : {4, 3, 1})
+        for (int cm : {9, 6, 5, 4, 3, 2}) {
+          int first = (int)dwg.size(), max_pitch = 0;
+          for (int k : order) {
+            if (ks[k] != kst || cms[k] != cm) continue;
+            cvm_matcher* m = f->ms[live[k]];
+            int rw = pw - m->w + 1, rh = ph - m->h + 1;
+            const int8_t* bf = nullptr;
+            CHK(matcher_bfrag(X, s, m, cn_of(m), &bf));
+            DirectJob J{};
+            J.bf = bf;
+            J.out = cmaps + cm_off[k];
+            J.th = m->h;
+            J.nk = direct_nk(m->w, kst);
+            J.pitch = direct_pitch_of(m->w, kst);
+            J.rw = rw;
+            J.rh = rh;
+            J.gx = direct_grid_x(rw);
+            J.wg0 = (int)dwg.size() - first;
+            max_pitch = std::max(max_pitch, J.pitch);
+            int nb = J.gx * direct_grid_y(rh);
+            for (int b = 0; b < nb; b++) dwg.push_back((int)dj.size());
+            dj.push_back(J);
+          }
+          if ((int)dwg.size() > first) dgroups.push_back({kst, cm, first, (int)dwg.size() - first, max_pitch, 1});
+        }
+      CHK(f->bj_direct.ensure(sizeof(DirectJob) * dj.size()));
+      CHK(f->bj_dwg.ensure(sizeof(int) * dwg.size()));
+      HIPCHK(hipMemcpyAsync(f->bj_direct.p, dj.data(), sizeof(DirectJob) * dj.size(), hipMemcpyHostToDevice, s));
+      HIPCHK(hipMemcpyAsync(f->bj_dwg.p, dwg.data(), sizeof(int) * dwg.size(), hipMemcpyHostToDevice, s));
+      HIPCHK(hipStreamSynchronize(s));  // the host vectors go out of scope
+      // waves per workgroup of each group, measured on this frame (the C'
+      // maps these trial launches write are rewritten by every FindAll)
+      for (auto& g : dgroups) {
+        char what[96];
+        snprintf(what, sizeof(what), "fleet.direct.waves/k%d/c%d/%dx%d/n%d", g[0], g[1], pw, ph, g[3]);
+        g[5] = tune_choice(what, {1, 2, 4}, "CVMATCH_DIRECT_WAVES", [&](int w) {
+          return time_launches(
+              s,
+              [&] {
+                return launch_direct_batch(d_img, istride, pw, ph, g[0], g[1], f->bj_direct.as<DirectJob>(),
+                                           f->bj_dwg.as<int>() + g[2], g[3], g[4], w, s);
+              },
+              1);
+        });
+      }
+
+      // Normalisation jobs grouped by wave-form variant (the reduction takes
+      // the whole job array; a job's wg0 counts from its group's first entry).
+      // qcap 4 gives each member the widest strip its template allows, qcap 2
+      // half-width strips; the tuner times both on this frame.
+      auto build_norm = [&](int qcap, std::vector<NormJob>& nj, std::vector<int>& nwg,
+                            std::vector<std::array<int, 4>>& groups) -> size_t {
+        nj.clear();
+        nwg.clear();
+        groups.clear();
+        std::vector<int> qs(live.size()), sqs(live.size());
+        std::vector<size_t> pt_off(live.size());
+        size_t pt_total = 0;
+        for (size_t k = 0; k < live.size(); k++) {
+          cvm_matcher* m = f->ms[live[k]];
+          norm_form(m->w, m->h, cn_of(m), &qs[k], &sqs[k]);
+          qs[k] = std::min(qs[k], qcap);
+          pt_off[k] = pt_total;
+          pt_total += (size_t)norm_grid_x(pw - m->w + 1, qs[k]) * norm_grid_y(ph - m->h + 1);
+        }
+        Extrema* parts = f->bj_parts.as<Extrema>();
+        for (int q : {4, 2})
+          for (int sq : {1, 0}) {
+            int first = (int)nwg.size();
+            for (int k : order) {
+              if (qs[k] != q || sqs[k] != sq) continue;
+              cvm_matcher* m = f->ms[live[k]];
+              int cn = cn_of(m);
+              int rw = pw - m->w + 1, rh = ph - m->h + 1;
+              NormJob J{};
+              J.st = states + live[k];
+              J.ps = f->dscr[cn].as<uint32_t>();
+              J.corr = (const float*)(cmaps + cm_off[k]);
+              J.parts = parts ? parts + pt_off[k] : nullptr;
+              J.res = d_res + live[k];
+              J.tw = m->w;
+              J.th = m->h;
+              J.rw = rw;
+              J.rh = rh;
+              J.gx = norm_grid_x(rw, q);
+              J.gy = norm_grid_y(rh);
+              J.wg0 = (int)nwg.size() - first;
+              for (int b = 0; b < J.gx * J.gy; b++) nwg.push_back((int)nj.size());
+              nj.push_back(J);
+            }
+            if ((int)nwg.size() > first) groups.push_back({q, sq, first, (int)nwg.size() - first});
+          }
+        return pt_total;
+      };
+      std::vector<NormJob> nj;
+      std::vector<int> nwg;
+      std::vector<std::array<int, 4>> groups;
+      size_t pts = std::max(build_norm(4, nj, nwg, groups), build_norm(2, nj, nwg, groups));
+      CHK(f->bj_parts.ensure(sizeof(Extrema) * pts));
+      auto upload_norm = [&](int qcap) -> int {
+        build_norm(qcap, nj, nwg, groups);
+        CHK(f->bj_norm.ensure(sizeof(NormJob) * nj.size()));
+        CHK(f->bj_nwg.ensure(sizeof(int) * nwg.size()));
+        HIPCHK(hipMemcpyAsync(f->bj_norm.p, nj.data(), sizeof(NormJob) * nj.size(), hipMemcpyHostToDevice, s));
+        HIPCHK(hipMemcpyAsync(f->bj_nwg.p, nwg.data(), sizeof(int) * nwg.size(), hipMemcpyHostToDevice, s));
+        HIPCHK(hipStreamSynchronize(s));
+        return 0;
+      };
+      char what[96];
+      snprintf(what, sizeof(what), "fleet.norm.q/%dx%d/n%zu", pw, ph, live.size());
+      int rc = 0;
+      const int qcap = tune_choice(what, {4, 2}, "CVMATCH_NORM_Q", [&](int q) -> float {
+        if (upload_norm(q) != 0) {
+          rc = -1;
+          return -1.0f;
+        }
+        return time_launches(
+            s,
+            [&] {
+              for (auto& g : groups)
+                launch_norm_batch(d_img, istride, pw, ph, f->step, g[0], g[1], true, f->bj_norm.as<NormJob>(),
+                                  f->bj_nwg.as<int>() + g[2], g[3], s);
+              return 0;
+            },
+            1);
+      });
+      if (rc) return rc;
+      CHK(upload_norm(qcap));
+      f->bj_dgroups = dgroups;
+      f->bj_nnorm = (int)nj.size();
+      f->bj_groups = groups;
+      return 0;
+    };
+    auto run_fused = [&]() -> int {
+      for (auto& g : f->bj_fgroups)
+        if (launch_direct_fused(d_img, istride, pw, ph, ps_of(g[0]), g[0], g[1], g[2] != 0,
+                                f->bj_fused.as<FusedJob>(), f->bj_fwg.as<int>() + g[3], g[4], g[5], g[6], s) != 0)
+          return -1;
+      launch_reduce_batch(f->bj_fred.as<NormJob>(), f->bj_nfred, s);
+      return 0;
+    };
+    auto run_unfused = [&]() -> int {
+      for (auto& g : f->bj_dgroups)
+        if (launch_direct_batch(d_img, istride, pw, ph, g[0], g[1], f->bj_direct.as<DirectJob>(),
+                                f->bj_dwg.as<int>() + g[2], g[3], g[4], g[5], s) != 0)
+          return -1;
+      for (auto& g : f->bj_groups)
+        launch_norm_batch(d_img, istride, pw, ph, f->step, g[0], g[1], true, f->bj_norm.as<NormJob>(),
+                          f->bj_nwg.as<int>() + g[2], g[3], s);
+      launch_reduce_batch(f->bj_norm.as<NormJob>(), f->bj_nnorm, s);
+      return 0;
+    };
+    // fused or unfused: both give the same bits; the tuner times a whole
+    // frame of each (building the unfused form's C' maps only to time it)
+    int use_fused = 0;
+    bool unfused_built = false;  // the unfused tables of this frame shape
+    if (fusable) {
+      CHK(build_fused());
+      char what[96];
+      snprintf(what, sizeof(what), "fleet.fused/%dx%d/n%zu", pw, ph, live.size());
+      int rc = 0;
+      use_fused = tune_choice(what, {1, 0}, "CVMATCH_FLEET_FUSED", [&](int fz) -> float {
+        if (fz == 0) {
+          if (build_unfused() != 0) {
+            rc = -1;
+            return -1.0f;
+          }
+          unfused_built = true;
+        }
+        return time_launches(s, fz ? std::function<int()>(run_fused) : std::function<int()>(run_unfused), 1);
+      });
+      if (rc) return rc;
+    }
+    if (use_fused) {
+      // the unfused form's per-member maps are not kept
+      for (DevBuf* b : {&f->bj_direct, &f->bj_dwg, &f->bj_norm, &f->bj_nwg, &f->bj_cmaps, &f->bj_parts})
+        b->release();
+      f->bj_dgroups.clear();
+      f->bj_groups.clear();
+      f->bj_nnorm = 0;
+    } else {
+      if (!unfused_built) CHK(build_unfused());
+      for (DevBuf* b : {&f->bj_fused, &f->bj_fwg, &f->bj_fred, &f->bj_fparts}) b->release();
+      f->bj_fgroups.clear();
+      f->bj_nfred = 0;
+    }
+    f->bj_use_fused = use_fused;
     f->bj_pw = pw;
     f->bj_ph = ph;
     f->bj_pConst = pConst;
@@ -2001,22 +2169,40 @@ static int fleet_find_direct_dev(Ctx& X, hipStream_t s, cvm_fleet* f, const uint
         break;
       }
   }
-  {
-    Stage S(X, "direct", s);
-    for (auto& g : f->bj_dgroups)
-      if (launch_direct_batch(d_img, istride, pw, ph, g[0], g[1], f->bj_direct.as<DirectJob>(),
-                              f->bj_dwg.as<int>() + g[2], g[3], g[4], g[5], s) != 0)
+  if (f->bj_use_fused) {
+    Stage S(X, "fused", s);
+    for (auto& g : f->bj_fgroups) {
+      const uint32_t* ps = nullptr;  // the chunk sums of the group's channel count
+      for (size_t i : live)
+        if (direct_kstep(f->ms[i]->step, cn_of(f->ms[i])) == g[0]) {
+          ps = f->dscr[cn_of(f->ms[i])].as<uint32_t>();
+          break;
+        }
+      if (launch_direct_fused(d_img, istride, pw, ph, ps, g[0], g[1], g[2] != 0, f->bj_fused.as<FusedJob>(),
+                              f->bj_fwg.as<int>() + g[3], g[4], g[5], g[6], s) != 0)
         return fail(CVM_E_BAD_ARGS, "cvmatch: fleet members outside the direct-correlation shapes");
-  }
-  {
-    Stage S(X, "normalize", s);
-    for (auto& g : f->bj_groups)
-      launch_norm_batch(d_img, istride, pw, ph, f->step, g[0], g[1], true, f->bj_norm.as<NormJob>(),
-                        f->bj_nwg.as<int>() + g[2], g[3], s);
+    }
+  } else {
+    {
+      Stage S(X, "direct", s);
+      for (auto& g : f->bj_dgroups)
+        if (launch_direct_batch(d_img, istride, pw, ph, g[0], g[1], f->bj_direct.as<DirectJob>(),
+                                f->bj_dwg.as<int>() + g[2], g[3], g[4], g[5], s) != 0)
+          return fail(CVM_E_BAD_ARGS, "cvmatch: fleet members outside the direct-correlation shapes");
+    }
+    {
+      Stage S(X, "normalize", s);
+      for (auto& g : f->bj_groups)
+        launch_norm_batch(d_img, istride, pw, ph, f->step, g[0], g[1], true, f->bj_norm.as<NormJob>(),
+                          f->bj_nwg.as<int>() + g[2], g[3], s);
+    }
   }
   {
     Stage S(X, "reduce", s);
-    launch_reduce_batch(f->bj_norm.as<NormJob>(), f->bj_nnorm, s);
+    if (f->bj_use_fused)
+      launch_reduce_batch(f->bj_fred.as<NormJob>(), f->bj_nfred, s);
+    else
+      launch_reduce_batch(f->bj_norm.as<NormJob>(), f->bj_nnorm, s);
   }
   HIPCHK(hipGetLastError());
   return 0;
@@ -2879,7 +3065,8 @@ void cvm_fleet_free(cvm_fleet* f) {
     }
     f->geom.reset();
     for (DevBuf* b : {&f->frame, &f->mspec, &f->corr, &f->res, &f->pcorr, &f->pparts, &f->mspec2, &f->dstates,
-                      &f->bj_direct, &f->bj_dwg, &f->bj_norm, &f->bj_nwg, &f->bj_cmaps, &f->bj_parts})
+                      &f->bj_direct, &f->bj_dwg, &f->bj_norm, &f->bj_nwg, &f->bj_cmaps, &f->bj_parts,
+                      &f->bj_fused, &f->bj_fwg, &f->bj_fred, &f->bj_fparts})
       b->release();
     for (DevBuf& b : f->dscr) b.release();
   }

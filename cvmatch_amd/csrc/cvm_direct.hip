@@ -25,9 +25,13 @@
 // v_mfma_i32_32x32x32_i8 lane layout: A byte b of lane l is A[l%32][16(l/32)+b],
 // B byte b of lane l is B[16(l/32)+b][l%32], and accumulator register j of
 // lane l is D[8(j/4) + 4(l/32) + j%4][l%32].
+#include <float.h>
+#include <limits.h>
+
 #include <type_traits>
 
 #include "cvm_internal.h"
+#include "cvm_score.h"
 
 namespace cvm {
 
@@ -188,8 +192,8 @@ struct RowStager {
 // CMAX: chunks per residue class, ceil(nk / KSTEP) <= CMAX.
 template <int KSTEP, bool VEC, int CMAX>
 __device__ __forceinline__ void direct_block(uint8_t* ring, const uint8_t* __restrict__ img, int64_t istride, int iw,
-                                             int ih, const int8_t* __restrict__ bf, int th, int nk, int pitch, int rw,
-                                             int rh, int32_t* __restrict__ out, int bx, int by) {
+                                             int ih, const int8_t* __restrict__ bf, int th, int nk, int pitch, int bx,
+                                             int by, dv16i (&acc)[DIR_T]) {
   constexpr int T = DIR_T;
   constexpr int SPU = KSTEP == 3 ? 12 : 16;  // staged bytes per 16 source bytes
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
@@ -208,7 +212,6 @@ __device__ __forceinline__ void direct_block(uint8_t* ring, const uint8_t* __res
   }
   __syncthreads();
 
-  dv16i acc[T];
 #pragma unroll
   for (int t = 0; t < T; t++)
 #pragma unroll
@@ -366,19 +369,6 @@ __device__ __forceinline__ void direct_block(uint8_t* ring, const uint8_t* __res
   } else {
     for (int i = 0; i < th; i++) step(i, std::integral_constant<int, 0>());
   }
-
-  const int yb = Y0 + 32 * wave + 4 * half;
-#pragma unroll
-  for (int t = 0; t < T; t++) {
-    const int x = X0 + 32 * t + m;
-    if (x < rw) {
-#pragma unroll
-      for (int j = 0; j < 16; j++) {
-        const int y = yb + 8 * (j >> 2) + (j & 3);
-        if (y < rh) out[(int64_t)y * rw + x] = acc[t][j];
-      }
-    }
-  }
 }
 
 // The same output block (32 rows x 32 T columns) computed by W waves that
@@ -390,13 +380,11 @@ __device__ __forceinline__ void direct_block(uint8_t* ring, const uint8_t* __res
 // write the next step's W rows (fetched a step earlier), run their template
 // row (residue classes with double-buffered A and B fragments), and meet at
 // one barrier; the slot a step writes held a row only the previous step read.
-// The W partial sums (each a sum of exact int8 products, bounded like the
-// whole) are added through LDS at the end and wave 0 stores the block.
+// Each wave ends with its partial sum of the block (the caller adds them).
 template <int W, int KSTEP, bool VEC, int CMAX>
 __device__ __forceinline__ void direct_block_mw(uint8_t* ring, const uint8_t* __restrict__ img, int64_t istride,
                                                 int iw, int ih, const int8_t* __restrict__ bf, int th, int nk,
-                                                int pitch, int rw, int rh, int32_t* __restrict__ out, int bx,
-                                                int by) {
+                                                int pitch, int bx, int by, dv16i (&acc)[DIR_T]) {
   constexpr int T = DIR_T;
   constexpr int R = 2 * W + 31;  // ring rows
   constexpr int SPU = KSTEP == 3 ? 12 : 16;
@@ -421,7 +409,6 @@ __device__ __forceinline__ void direct_block_mw(uint8_t* ring, const uint8_t* __
   bool k0, k1;
   uint4 q0 = stg.fetch(W + 31 + wave, ncl0, k0), q1 = stg.fetch(W + 31 + wave, ncl1, k1);
 
-  dv16i acc[T];
 #pragma unroll
   for (int t = 0; t < T; t++)
 #pragma unroll
@@ -488,10 +475,33 @@ __device__ __forceinline__ void direct_block_mw(uint8_t* ring, const uint8_t* __
     }
     __syncthreads();
   }
+}
 
-  // add the partial sums of waves 1..W-1 into wave 0's (the ring is free)
+// The block's sum over the template rows: one wave's whole sum (W = 1) or
+// each of the W waves' partial sums.
+template <int W, int KSTEP, bool VEC, int CMAX>
+__device__ __forceinline__ void direct_accumulate(uint8_t* ring, const uint8_t* __restrict__ img, int64_t istride,
+                                                  int iw, int ih, const int8_t* __restrict__ bf, int th, int nk,
+                                                  int pitch, int bx, int by, dv16i (&acc)[DIR_T]) {
+  if constexpr (W == 1)
+    direct_block<KSTEP, VEC, CMAX>(ring, img, istride, iw, ih, bf, th, nk, pitch, bx, by, acc);
+  else
+    direct_block_mw<W, KSTEP, VEC, CMAX>(ring, img, istride, iw, ih, bf, th, nk, pitch, bx, by, acc);
+}
+
+// One block into the C' map: the W waves' partial sums (each a sum of exact
+// int8 products, bounded like the whole) are added through LDS into wave 0's,
+// which stores the block.
+template <int W, int KSTEP, bool VEC, int CMAX>
+__device__ __forceinline__ void direct_block_w(uint8_t* ring, const uint8_t* __restrict__ img, int64_t istride,
+                                               int iw, int ih, const int8_t* __restrict__ bf, int th, int nk,
+                                               int pitch, int rw, int rh, int32_t* __restrict__ out, int bx, int by) {
+  constexpr int T = DIR_T;
+  dv16i acc[T];
+  direct_accumulate<W, KSTEP, VEC, CMAX>(ring, img, istride, iw, ih, bf, th, nk, pitch, bx, by, acc);
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   if constexpr (W > 1) {
-    int32_t* red = (int32_t*)ring;  // (W - 1) x 16 x 64 words, lane-minor
+    int32_t* red = (int32_t*)ring;  // (W - 1) x 16 x 64 words, lane-minor; the ring is free
 #pragma unroll
     for (int t = 0; t < T; t++) {
       if (wave > 0)
@@ -506,7 +516,8 @@ __device__ __forceinline__ void direct_block_mw(uint8_t* ring, const uint8_t* __
     }
   }
   if (wave != 0) return;
-  const int yb = Y0 + 4 * half;
+  const int half = lane >> 5, m = lane & 31;
+  const int X0 = bx * 32 * T, yb = by * 32 + 4 * half;
 #pragma unroll
   for (int t = 0; t < T; t++) {
     const int x = X0 + 32 * t + m;
@@ -518,16 +529,6 @@ __device__ __forceinline__ void direct_block_mw(uint8_t* ring, const uint8_t* __
       }
     }
   }
-}
-
-template <int W, int KSTEP, bool VEC, int CMAX>
-__device__ __forceinline__ void direct_block_w(uint8_t* ring, const uint8_t* __restrict__ img, int64_t istride,
-                                               int iw, int ih, const int8_t* __restrict__ bf, int th, int nk,
-                                               int pitch, int rw, int rh, int32_t* __restrict__ out, int bx, int by) {
-  if constexpr (W == 1)
-    direct_block<KSTEP, VEC, CMAX>(ring, img, istride, iw, ih, bf, th, nk, pitch, rw, rh, out, bx, by);
-  else
-    direct_block_mw<W, KSTEP, VEC, CMAX>(ring, img, istride, iw, ih, bf, th, nk, pitch, rw, rh, out, bx, by);
 }
 
 template <int W, int KSTEP, bool VEC, int CMAX>
@@ -646,5 +647,429 @@ int launch_direct_batch(const uint8_t* img, int64_t istride, int iw, int ih, int
 
 #undef CVM_DIRECT_KEYS
 #undef CVM_DIRECT_SWITCH
+
+
+// ------------------------------------------- fused correlation and scoring
+//
+// One workgroup correlates its 32 x 128 output block on the MFMA (as above),
+// parks the block's C' values in LDS and scores them there, so no C' map
+// reaches HBM: a member costs one extrema partial per block.  The scoring is
+// the wave-form exact normalisation (window sums from the 16-row chunk
+// prefix sums of the frame, slid down the rows in registers, horizontal sums
+// from a DPP prefix scan) followed by the FP64 score rounded once to float32,
+// op for op the unfused k_norm_batch in its exact mode, so every score and
+// extremum is bitwise the same.  Scoring waves: one wave takes all 32 rows
+// x 128 columns (Q = 2 results per lane); two waves take 16 rows each; four
+// waves take 16 rows x 64 columns each (Q = 1).  Each wave keeps its own
+// prefix rows, so a scoring wave synchronises only with itself.
+
+constexpr int FB_PITCH = 32 * DIR_T + 8;  // C' words per block row in LDS (rows r and r + 4 on other banks)
+constexpr int FB_CBYTES = 32 * FB_PITCH * 4;
+constexpr int FB_CHUNK = 16;  // rows per chunk of the frame's prefix column sums (the normalisation's band)
+
+__host__ __device__ constexpr int fused_nwp(int cpt) { return 64 * cpt + 1 + (64 * cpt + 1) / 32 + 1; }
+__host__ __device__ constexpr int fused_cpt(int cmax) { return cmax >= 6 ? 8 : 4; }
+__host__ __device__ constexpr int fused_nch(int kstep) { return kstep == 1 ? 1 : (kstep == 3 ? 3 : 4); }
+// LDS bytes of one scoring wave's prefix rows (NCH channel rows + the square row)
+__host__ __device__ constexpr int fused_pbytes(int cpt, int nch, bool sq32) {
+  return fused_nwp(cpt) * (4 * nch + (sq32 ? 4 : 8));
+}
+
+// CPT consecutive pixels x0.. of one row as packed channel words (channel k
+// in bits 8k..), all inside the row: one 16-byte load per four RGBA pixels,
+// one 4- or 8-byte load of gray bytes.
+template <int CPT, int STEP>
+__device__ __forceinline__ void fused_px(const uint8_t* row, int x0, uint32_t (&out)[CPT]) {
+  if constexpr (STEP == 4) {
+#pragma unroll
+    for (int h = 0; h < CPT / 4; h++) {
+      const uint4 a = *(const uint4*)(row + (int64_t)(x0 + 4 * h) * 4);
+      out[4 * h] = a.x;
+      out[4 * h + 1] = a.y;
+      out[4 * h + 2] = a.z;
+      out[4 * h + 3] = a.w;
+    }
+  } else if constexpr (CPT == 4) {
+    const uint32_t v = *(const uint32_t*)(row + x0);
+#pragma unroll
+    for (int j = 0; j < 4; j++) out[j] = (v >> (8 * j)) & 255u;
+  } else {
+    const uint2 v = *(const uint2*)(row + x0);
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+      out[j] = (v.x >> (8 * j)) & 255u;
+      out[4 + j] = (v.y >> (8 * j)) & 255u;
+    }
+  }
+}
+
+template <int CPT>
+__device__ __forceinline__ void fused_u32(const uint32_t* p, uint32_t (&out)[CPT]) {
+#pragma unroll
+  for (int h = 0; h < CPT / 4; h++) {
+    const uint4 a = ((const uint4*)p)[h];
+    out[4 * h] = a.x;
+    out[4 * h + 1] = a.y;
+    out[4 * h + 2] = a.z;
+    out[4 * h + 3] = a.w;
+  }
+}
+
+template <int Q, int CPT>
+struct FusedRow {
+  uint32_t a[CPT], b[CPT];
+  int32_t c[Q];
+};
+
+// Scores of result rows [ys, ye) x columns [xs, xs + 64 Q) of one member by
+// one wave, folded into (bmin, bmax).  cb: the C' value of (ys, xs) in LDS,
+// rows FB_PITCH words apart.  Requires iw % CPT == 0, xs % CPT == 0,
+// ys % FB_CHUNK == 0, ys < ye <= rh and 64 CPT >= 64 Q + tw - 1.
+template <int Q, int CPT, int NCH, int STEP, typename sq_t>
+__device__ __forceinline__ void fused_score(const uint8_t* __restrict__ img, int64_t istride, int iw, int ih, int tw,
+                                            int th, int rw, const CallState* __restrict__ st,
+                                            const uint32_t* __restrict__ ps, const int32_t* cb, int xs, int ys,
+                                            int ye, uint32_t* pl, float& bmin, int& bmin_i, float& bmax,
+                                            int& bmax_i) {
+  constexpr int NWP = fused_nwp(CPT);
+  constexpr bool SQ32 = sizeof(sq_t) == 4;
+  uint32_t(*P)[NWP] = (uint32_t(*)[NWP])pl;
+  sq_t* P2 = (sq_t*)(pl + NCH * NWP);
+  const int lane = threadIdx.x & 63;
+  const int cn = st->cn;
+  const double aread = (double)tw * (double)th;
+  double tsd[NCH];
+#pragma unroll
+  for (int k = 0; k < NCH; k++) tsd[k] = k < cn ? (double)st->tsum[k] : 0.0;
+  const double varSumD = (double)st->varSum;
+  double tsumAll = tsd[0];
+#pragma unroll
+  for (int k = 1; k < NCH; k++) tsumAll += tsd[k];
+  // sum P*T = C' + 128 (S + sum_k T_k) - 2^14 area cn
+  const double cpOff = 16384.0 * aread * (double)cn;
+  const int cbase = lane * CPT;
+  // lanes past the image load its last CPT columns: whole lanes only (iw and
+  // xs are multiples of CPT), and their prefix entries lie past every one a
+  // result of the strip reads
+  const int x0 = min(xs + cbase, iw - CPT);
+  const uint32_t cm = chan_mask(cn);
+
+  // column sums over the th window rows, channels packed in 16-bit halves:
+  // cs02 = ch0 | ch2 << 16, cs13 = ch1 | ch3 << 16 (each <= 255 * th < 2^16)
+  uint32_t cs02[CPT], cs13[CPT];
+  sq_t c2[CPT];
+  {
+    const int c0 = ys / FB_CHUNK, nfull = th / FB_CHUNK;
+    const uint32_t* p0 = ps + (int64_t)c0 * 3 * iw + x0;
+    const uint32_t* p1 = ps + (int64_t)(c0 + nfull) * 3 * iw + x0;
+    uint32_t a0[CPT], a1[CPT], a2[CPT], b0[CPT], b1[CPT], b2[CPT];
+    fused_u32<CPT>(p0, a0);
+    fused_u32<CPT>(p0 + iw, a1);
+    fused_u32<CPT>(p0 + 2 * iw, a2);
+    fused_u32<CPT>(p1, b0);
+    fused_u32<CPT>(p1 + iw, b1);
+    fused_u32<CPT>(p1 + 2 * iw, b2);
+#pragma unroll
+    for (int j = 0; j < CPT; j++) {
+      cs02[j] = b0[j] - a0[j];
+      cs13[j] = b1[j] - a1[j];
+      c2[j] = (sq_t)(uint32_t)(b2[j] - a2[j]);
+    }
+    for (int y = ys + nfull * FB_CHUNK; y < ys + th; y++) {
+      uint32_t w[CPT];
+      fused_px<CPT, STEP>(img + (int64_t)y * istride, x0, w);
+#pragma unroll
+      for (int j = 0; j < CPT; j++) {
+        const uint32_t v = w[j] & cm;
+        cs02[j] += v & 0x00FF00FFu;
+        cs13[j] += (v >> 8) & 0x00FF00FFu;
+        c2[j] += (sq_t)__builtin_amdgcn_udot4(v, v, 0u, false);
+      }
+    }
+  }
+
+  // row yy's inputs (rows past the band clamped into the image: loaded, not used)
+  auto fetch = [&](FusedRow<Q, CPT>& r, int yy) {
+    const int yl = min(yy, ye - 1);
+    const int ya = min(yl + th, ih - 1);
+    fused_px<CPT, STEP>(img + (int64_t)ya * istride, x0, r.a);
+    fused_px<CPT, STEP>(img + (int64_t)yl * istride, x0, r.b);
+#pragma unroll
+    for (int q = 0; q < Q; q++) r.c[q] = cb[(yl - ys) * FB_PITCH + lane + 64 * q];
+  };
+
+  auto row = [&](const FusedRow<Q, CPT>& r, int y) {
+    uint32_t run[NCH];
+#pragma unroll
+    for (int k = 0; k < NCH; k++) run[k] = 0;
+    sq_t run2 = 0;
+    uint32_t loc[CPT][NCH];
+    sq_t loc2[CPT];
+#pragma unroll
+    for (int j = 0; j < CPT; j++) {
+      const uint32_t cv[4] = {cs02[j] & 0xFFFFu, cs13[j] & 0xFFFFu, cs02[j] >> 16, cs13[j] >> 16};
+#pragma unroll
+      for (int k = 0; k < NCH; k++) {
+        run[k] += cv[k];
+        loc[j][k] = run[k];
+      }
+      run2 += c2[j];
+      loc2[j] = run2;
+    }
+    uint32_t off[NCH];
+#pragma unroll
+    for (int k = 0; k < NCH; k++) off[k] = wave_incl_scan_u32(run[k], lane) - run[k];
+    sq_t off2;
+    if constexpr (SQ32) {
+      off2 = (sq_t)wave_incl_scan_u32((uint32_t)run2, lane) - run2;
+    } else {
+      sq_t inc2 = run2;
+      for (int o = 1; o < 64; o <<= 1) {
+        const sq_t up2 = __shfl_up(inc2, o, 64);
+        if (lane >= o) inc2 += up2;
+      }
+      off2 = inc2 - run2;
+    }
+#pragma unroll
+    for (int j = 0; j < CPT; j++) {
+      const int pc = cbase + j + 1;
+      const int pi = pc + (pc >> 5);
+#pragma unroll
+      for (int k = 0; k < NCH; k++) P[k][pi] = off[k] + loc[j][k];
+      P2[pi] = off2 + loc2[j];
+    }
+    if (lane == 0) {
+#pragma unroll
+      for (int k = 0; k < NCH; k++) P[k][0] = 0;
+      P2[0] = 0;
+    }
+    wave_lds_sync();
+    float vq[Q];
+#pragma unroll
+    for (int q = 0; q < Q; q++) {
+      const int xl = lane + 64 * q, xb = xl + tw;
+      const int ia = xl + (xl >> 5), ib = xb + (xb >> 5);
+      // exact integers carried in binary64 (all below 2^53)
+      double sd[NCH];
+#pragma unroll
+      for (int k = 0; k < NCH; k++) sd[k] = (double)(uint32_t)(P[k][ib] - P[k][ia]);
+      const double s2 = (double)(sq_t)(P2[ib] - P2[ia]);
+      double sq = sd[0] * sd[0], lane0 = sd[0] * tsd[0];
+#pragma unroll
+      for (int k = 1; k < NCH; k++) {
+        sq = sq + sd[k] * sd[k];
+        lane0 = lane0 + sd[k] * tsd[k];
+      }
+      const double idiff = aread * s2 - sq;
+      double S = sd[0];
+#pragma unroll
+      for (int k = 1; k < NCH; k++) S += sd[k];
+      const double corrI = (double)r.c[q] + 128.0 * (S + tsumAll) - cpOff;
+      vq[q] = (float)score_fp64(aread * corrI - lane0, idiff, s2, aread, varSumD);
+    }
+    wave_lds_sync();
+    // slide to row y+1: the leaving row is inside the current sums, so each
+    // 16-bit half stays in [0, 2^16) and no borrow crosses halves
+#pragma unroll
+    for (int j = 0; j < CPT; j++) {
+      const uint32_t a = r.a[j] & cm, b = r.b[j] & cm;
+      cs02[j] = cs02[j] - (b & 0x00FF00FFu) + (a & 0x00FF00FFu);
+      cs13[j] = cs13[j] - ((b >> 8) & 0x00FF00FFu) + ((a >> 8) & 0x00FF00FFu);
+      c2[j] += (sq_t)__builtin_amdgcn_udot4(a, a, 0u, false) - (sq_t)__builtin_amdgcn_udot4(b, b, 0u, false);
+    }
+#pragma unroll
+    for (int q = 0; q < Q; q++) {
+      const int x = xs + lane + 64 * q;
+      if (x < rw) {
+        const int idx = y * rw + x;
+        if (better_min(vq[q], idx, bmin, bmin_i)) {
+          bmin = vq[q];
+          bmin_i = idx;
+        }
+        if (better_max(vq[q], idx, bmax, bmax_i)) {
+          bmax = vq[q];
+          bmax_i = idx;
+        }
+      }
+    }
+  };
+
+  // two register sets in turn: row y+1's loads are in flight while row y is scored
+  FusedRow<Q, CPT> ra, rb;
+  fetch(ra, ys);
+  for (int y = ys; y < ye; y += 2) {
+    fetch(rb, y + 1);
+    row(ra, y);
+    if (y + 1 >= ye) break;
+    fetch(ra, y + 2);
+    row(rb, y + 1);
+  }
+}
+
+// Many members against one frame, correlated and scored in one launch (all
+// of one (KSTEP, CMAX, SQ32) form): workgroup b runs block b - job.wg0 of job
+// wg_job[b] and writes that block's extrema partial.  The frame's rows are
+// 16-byte aligned (direct_vec) and iw % CPT == 0.
+template <int W, int KSTEP, int CMAX, bool SQ32>
+__global__ __launch_bounds__(64 * W) void k_direct_fused(const uint8_t* __restrict__ img, int64_t istride, int iw,
+                                                         int ih, const uint32_t* __restrict__ ps,
+                                                         const FusedJob* __restrict__ jobs,
+                                                         const int* __restrict__ wg_job) {
+  typedef typename std::conditional<SQ32, uint32_t, unsigned long long>::type sq_t;
+  constexpr int T = DIR_T;
+  constexpr int CPT = fused_cpt(CMAX), NCH = fused_nch(KSTEP), STEP = KSTEP == 1 ? 1 : 4;
+  constexpr int Q = W == 4 ? 1 : 2;
+  constexpr int PB = fused_pbytes(CPT, NCH, SQ32);
+  extern __shared__ uint4 lds4[];
+  __shared__ Extrema wred[W];
+  uint8_t* lds = (uint8_t*)lds4;
+  const FusedJob J = jobs[wg_job[blockIdx.x]];
+  const int local = blockIdx.x - J.wg0;
+  const int bx = local % J.gx, by = local / J.gx;
+  const int X0 = bx * 32 * T, Y0 = by * 32;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const CallState* st = J.st;
+  if (st->flat) {
+    // templNorm below the flat threshold: every score is 1
+    if (tid == 0) {
+      Extrema e;
+      e.min_v = e.max_v = 1.0f;
+      e.min_i = e.max_i = Y0 * J.rw + X0;
+      J.parts[local] = e;
+    }
+    return;
+  }
+
+  dv16i acc[T];
+  direct_accumulate<W, KSTEP, true, CMAX>(lds, img, istride, iw, ih, J.bf, J.th, J.nk, J.pitch, bx, by, acc);
+  __syncthreads();  // the ring is no longer read
+  int32_t* cbuf = (int32_t*)lds;
+  {
+    const int half = lane >> 5, m = lane & 31;
+    // wave 0 stores its sum, the other waves then add their partial sums
+    for (int w = 0; w < W; w++) {
+      if (wave == w)
+#pragma unroll
+        for (int t = 0; t < T; t++)
+#pragma unroll
+          for (int j = 0; j < 16; j++) {
+            int32_t* c = cbuf + (8 * (j >> 2) + 4 * half + (j & 3)) * FB_PITCH + 32 * t + m;
+            if (w == 0)
+              *c = acc[t][j];
+            else
+              atomicAdd(c, acc[t][j]);
+          }
+      if (w == 0 || w == W - 1) __syncthreads();
+    }
+  }
+
+  float bmin = FLT_MAX, bmax = -FLT_MAX;
+  int bmin_i = INT_MAX, bmax_i = INT_MAX;
+  {
+    const int ys = Y0 + (W == 1 ? 0 : 16 * (wave & 1));
+    const int xs = X0 + (W == 4 ? 64 * (wave >> 1) : 0);
+    const int ye = min(J.rh, ys + (W == 1 ? 32 : 16));
+    uint32_t* pl = (uint32_t*)(lds + FB_CBYTES + wave * PB);
+    if (ys < ye && xs < J.rw)
+      fused_score<Q, CPT, NCH, STEP, sq_t>(img, istride, iw, ih, J.tw, J.th, J.rw, st, ps,
+                                           cbuf + (ys - Y0) * FB_PITCH + (xs - X0), xs, ys, ye, pl, bmin, bmin_i,
+                                           bmax, bmax_i);
+  }
+  for (int o = 32; o > 0; o >>= 1) {
+    float ov = __shfl_xor(bmin, o, 64);
+    int oi = __shfl_xor(bmin_i, o, 64);
+    if (better_min(ov, oi, bmin, bmin_i)) {
+      bmin = ov;
+      bmin_i = oi;
+    }
+    ov = __shfl_xor(bmax, o, 64);
+    oi = __shfl_xor(bmax_i, o, 64);
+    if (better_max(ov, oi, bmax, bmax_i)) {
+      bmax = ov;
+      bmax_i = oi;
+    }
+  }
+  if (W > 1) {
+    if (lane == 0) {
+      wred[wave].min_v = bmin;
+      wred[wave].min_i = bmin_i;
+      wred[wave].max_v = bmax;
+      wred[wave].max_i = bmax_i;
+    }
+    __syncthreads();
+    if (tid == 0)
+      for (int w = 1; w < W; w++) {
+        if (better_min(wred[w].min_v, wred[w].min_i, bmin, bmin_i)) {
+          bmin = wred[w].min_v;
+          bmin_i = wred[w].min_i;
+        }
+        if (better_max(wred[w].max_v, wred[w].max_i, bmax, bmax_i)) {
+          bmax = wred[w].max_v;
+          bmax_i = wred[w].max_i;
+        }
+      }
+  }
+  if (tid == 0) {
+    Extrema e;
+    e.min_v = bmin;
+    e.min_i = bmin_i;
+    e.max_v = bmax;
+    e.max_i = bmax_i;
+    J.parts[local] = e;
+  }
+}
+
+// CPT of the fused scoring for a chunk count, and whether a (kstep, cmax,
+// sq32) form is built: the 64-bit square sums only where a window's square
+// sum can pass 2^32 (cn * 65025 * area >= 2^32 needs tw >= 86 at cn 3,
+// tw >= 65 at cn 4; never at cn 1 within th <= 257).
+int fused_cpt_of(int cmax) { return fused_cpt(cmax); }
+
+static bool fused_form_built(int kstep, int cmax, bool sq32) {
+  if (sq32) return true;
+  if (kstep == 3) return cmax >= 4;
+  if (kstep == 4) return cmax >= 3;
+  return false;
+}
+
+size_t fused_lds_bytes(int max_pitch, int kstep, int cmax, bool sq32, int waves) {
+  const size_t ring = (size_t)(waves == 1 ? DIR_RING : 2 * waves + 31) * max_pitch;
+  const size_t score = (size_t)FB_CBYTES + (size_t)waves * fused_pbytes(fused_cpt(cmax), fused_nch(kstep), sq32);
+  return std::max(ring, score);
+}
+
+bool fused_ok(const uint8_t* img, int64_t istride, int iw, int kstep, int cmax, bool sq32) {
+  return fused_form_built(kstep, cmax, sq32) && iw % fused_cpt(cmax) == 0 && direct_vec(img, istride, iw, kstep);
+}
+
+int launch_direct_fused(const uint8_t* img, int64_t istride, int iw, int ih, const uint32_t* ps, int kstep, int cmax,
+                        bool sq32, const FusedJob* jobs, const int* wg_job, int nwg, int max_pitch, int waves,
+                        hipStream_t s) {
+  if (!direct_waves_ok(waves) || nwg <= 0 || !fused_ok(img, istride, iw, kstep, cmax, sq32)) return -1;
+  const size_t lds = fused_lds_bytes(max_pitch, kstep, cmax, sq32, waves);
+  if (lds + 64 > 160 * 1024) return -1;
+#define CVM_DF(W, KS, CM, SQ)                                                                                    \
+  case W * 10000 + KS * 1000 + CM * 10 + SQ:                                                                    \
+    hipLaunchKernelGGL((k_direct_fused<W, KS, CM, SQ == 1>), dim3(nwg), dim3(64 * W), lds, s, img, istride, iw, \
+                       ih, ps, jobs, wg_job);                                                                   \
+    break;
+#define CVM_DF_W(W)                                                                                         \
+  CVM_DF(W, 1, 2, 1) CVM_DF(W, 1, 3, 1) CVM_DF(W, 1, 4, 1) CVM_DF(W, 1, 5, 1) CVM_DF(W, 1, 6, 1)              \
+  CVM_DF(W, 1, 9, 1) CVM_DF(W, 3, 2, 1) CVM_DF(W, 3, 3, 1) CVM_DF(W, 3, 4, 1) CVM_DF(W, 3, 4, 0)              \
+  CVM_DF(W, 3, 5, 1) CVM_DF(W, 3, 5, 0) CVM_DF(W, 3, 6, 1) CVM_DF(W, 3, 6, 0) CVM_DF(W, 3, 9, 1)              \
+  CVM_DF(W, 3, 9, 0) CVM_DF(W, 4, 2, 1) CVM_DF(W, 4, 3, 1) CVM_DF(W, 4, 3, 0) CVM_DF(W, 4, 4, 1)              \
+  CVM_DF(W, 4, 4, 0) CVM_DF(W, 4, 5, 1) CVM_DF(W, 4, 5, 0) CVM_DF(W, 4, 6, 1) CVM_DF(W, 4, 6, 0)              \
+  CVM_DF(W, 4, 9, 1) CVM_DF(W, 4, 9, 0)
+  switch (waves * 10000 + kstep * 1000 + cmax * 10 + (sq32 ? 1 : 0)) {
+    CVM_DF_W(1)
+    CVM_DF_W(2)
+    CVM_DF_W(4)
+    default:
+      return -1;
+  }
+#undef CVM_DF_W
+#undef CVM_DF
+  return 0;
+}
 
 }  // namespace cvm

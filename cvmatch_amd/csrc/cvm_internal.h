@@ -75,6 +75,16 @@ struct DirectJob {
 
 struct Extrema;
 
+// One template of a fused correlation + scoring launch: its B fragments,
+// call state, one extrema partial per 32 x 128 block, shape, ring pitch and
+// grid (gx blocks along x), first workgroup.
+struct FusedJob {
+  const int8_t* bf;
+  const CallState* st;
+  Extrema* parts;
+  int tw, th, nk, pitch, rw, rh, gx, wg0;
+};
+
 // One template of a batched normalisation launch: its call state, chunk
 // partials (per channel count), correlation, extrema partials and result.
 struct NormJob {
@@ -216,6 +226,18 @@ int direct_grid_x(int rw);
 int direct_grid_y(int rh);
 int launch_direct_batch(const uint8_t* img, int64_t istride, int iw, int ih, int kstep, int cmax,
                         const DirectJob* jobs, const int* wg_job, int nwg, int max_pitch, int waves, hipStream_t s);
+// Fused correlation + FP64 scoring of many members against one frame (no
+// C' map; one extrema partial per direct block; reduce with
+// launch_reduce_batch over jobs whose gx x gy is the direct grid).  ps: the
+// frame's chunk column sums (launch_norm_prep) for the members' channel
+// count.  fused_ok: the form is built and the frame's layout allows it
+// (rows 16-byte aligned for the staging, width a multiple of fused_cpt_of).
+int fused_cpt_of(int cmax);
+bool fused_ok(const uint8_t* img, int64_t istride, int iw, int kstep, int cmax, bool sq32);
+size_t fused_lds_bytes(int max_pitch, int kstep, int cmax, bool sq32, int waves);
+int launch_direct_fused(const uint8_t* img, int64_t istride, int iw, int ih, const uint32_t* ps, int kstep, int cmax,
+                        bool sq32, const FusedJob* jobs, const int* wg_job, int nwg, int max_pitch, int waves,
+                        hipStream_t s);
 void norm_form(int tw, int th, int cnMax, int* q, int* sq32);
 int norm_grid_x(int rw, int q);
 int norm_grid_y(int rh);

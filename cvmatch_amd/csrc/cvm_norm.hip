@@ -19,6 +19,7 @@
 #include <algorithm>
 
 #include "cvm_internal.h"
+#include "cvm_score.h"
 
 namespace cvm {
 
@@ -261,8 +262,6 @@ __device__ __forceinline__ float norm_one(float num, float lane0, float idiff, f
   return 0.0f;
 }
 
-__device__ __forceinline__ bool better_min(float v, int i, float bv, int bi) { return v < bv || (v == bv && i < bi); }
-__device__ __forceinline__ bool better_max(float v, int i, float bv, int bi) { return v > bv || (v == bv && i < bi); }
 
 constexpr int NORM_NX = 256;  // output columns per workgroup (one per thread)
 
@@ -558,8 +557,6 @@ __global__ __launch_bounds__(256) void k_normalize(const uint8_t* __restrict__ i
 // directly; the window then slides down one row per step.
 // Horizontal window sums are differences of a wave-level prefix scan in LDS.
 
-// Channel-byte mask of a packed pixel word: bytes of channels >= cn cleared.
-__device__ __forceinline__ uint32_t chan_mask(int cn) { return cn >= 4 ? 0xFFFFFFFFu : ((1u << (8 * cn)) - 1u); }
 
 // Partial column sums of chunk c (rows [c*NB_ROWS, c*NB_ROWS + NB_ROWS)),
 // stored at row c + 1 of the scratch (row 0 is the empty prefix),
@@ -623,32 +620,6 @@ __global__ __launch_bounds__(256) void k_colsum_prefix(int nwords, int nch, cons
   }
 }
 
-// Inclusive add-scan of a uint32 over the 64 lanes of a wave with DPP moves
-// (row_shr 1/2/4/8 inside each 16-lane row, then row_bcast 15/31 across
-// rows): VALU-latency steps instead of LDS permutes.
-template <int CTRL>
-__device__ __forceinline__ uint32_t dpp_mov(uint32_t v) {
-  return (uint32_t)__builtin_amdgcn_mov_dpp((int)v, CTRL, 0xf, 0xf, false);
-}
-
-__device__ __forceinline__ uint32_t wave_incl_scan_u32(uint32_t x, int lane) {
-  const int rl = lane & 15;
-  uint32_t t;
-  t = dpp_mov<0x111>(x);
-  if (rl >= 1) x += t;
-  t = dpp_mov<0x112>(x);
-  if (rl >= 2) x += t;
-  t = dpp_mov<0x114>(x);
-  if (rl >= 4) x += t;
-  t = dpp_mov<0x118>(x);
-  if (rl >= 8) x += t;
-  t = dpp_mov<0x142>(x);
-  if ((lane & 31) >= 16) x += t;
-  t = dpp_mov<0x143>(x);
-  if (lane >= 32) x += t;
-  return x;
-}
-
 // Eight consecutive pixels x0..x0+7 of one row as packed channel words
 // (channel k in bits 8k..).  VMODE 1: RGBA rows 16-byte aligned, two 16-byte
 // loads; VMODE 2: gray rows 8-byte aligned, one 8-byte load (both with all
@@ -702,28 +673,6 @@ struct NormRow {
   uint32_t a[NORM_CPT], b[NORM_CPT];
   float c[Q];
 };
-
-// The FP64 score of one window from exact integers (the FP64 oracle's
-// formula): num = area * corr - sum_k S_k T_k, idiff = area * s2 - sum_k S_k^2,
-// every input an exact integer below 2^53.
-__device__ __forceinline__ double score_fp64(double numI, double idiff, double s2, double aread, double varSum) {
-  // The guard is fl(idiff / area) > lim with lim = min(0.5, 10 * 2^-23 * s2)
-  // >= 0.  idiff and area are exact non-negative integers, so idiff > area
-  // (a quotient above 1) always passes it and idiff == 0 (a quotient of +0)
-  // always fails it: only 0 < idiff <= area needs the division.
-  if (!(idiff > aread)) {
-    if (idiff == 0.0) return 0.0;
-    const double diff2 = idiff / aread;
-    double lim = 1.1920928955078125e-06 * s2;  // 10 * 2^-23 * s2
-    if (lim > 0.5) lim = 0.5;
-    if (!(diff2 > lim)) return 0.0;
-  }
-  const double denI = sqrt(idiff) * sqrt(varSum);
-  const double an = fabs(numI);
-  if (an < denI) return numI / denI;
-  if (an < denI * 1.125) return numI > 0.0 ? 1.0 : -1.0;
-  return 0.0;
-}
 
 // VMODE 1 / 2 (RGBA / gray rows with iw % 8 == 0, aligned for vector loads):
 // every lane loads eight whole columns, lanes past the image clamped to its

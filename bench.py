@@ -778,8 +778,12 @@ def main():
         kinds.setdefault(kind, {}).setdefault(str(v["choice"]), 0)
         kinds[kind][str(v["choice"])] += 1
     line["tuner"] = {"decisions": len(rep), "choice_counts_by_kind": kinds,
+                     # the fused-or-not decision is in fleet.direct_form (the fleet leg's
+                     # forced unfused run overrides the key afterwards)
                      "fleet_2560x1440": {k.split("@")[0]: {"choice": v["choice"], "ms": v["ms"], "source": v["source"]}
-                                         for k, v in rep.items() if k.startswith("fleet.") and "2560x1440" in k},
+                                         for k, v in rep.items()
+                                         if k.startswith("fleet.") and "2560x1440" in k
+                                         and not k.startswith("fleet.fused/")},
                      "headline_4k": {k.split("@")[0]: v["choice"] for k, v in rep.items() if "3840x2160" in k},
                      "sources": sorted({v["source"] for v in rep.values()})}
     if not args.no_cpu:

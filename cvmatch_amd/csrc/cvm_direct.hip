@@ -42,6 +42,13 @@ typedef int dv16i __attribute__((ext_vector_type(16)));
 // with the LDS reads, so every wait for an A fragment would drain them too.
 typedef const dv4i __attribute__((address_space(1)))* gdv4p;
 
+// The direct kernels run two waves per SIMD (their LDS ring and accumulators
+// allow no more): the register budget of two waves lets the compiler keep
+// loads in flight in registers of their own instead of reusing registers
+// that a load may still be writing (each reuse is a wait for that load).
+// (Six or more chunks per class need more than the two-wave budget.)
+#define CVM_DIRECT_OCC __attribute__((amdgpu_waves_per_eu(CMAX >= 6 ? 1 : 2, 2)))
+
 constexpr int DIR_T = 4;      // column tiles per wave (128 output columns)
 constexpr int DIR_WAVES = 1;  // waves per workgroup, 32 output rows each
 constexpr int DIR_ROWS = 32 * DIR_WAVES;
@@ -50,8 +57,9 @@ constexpr int DIR_ROWS = 32 * DIR_WAVES;
 // registers, not in the ring).
 constexpr int DIR_RING = DIR_ROWS + 1;
 static_assert(DIR_WAVES == 1, "a ring of DIR_ROWS + 1 rows relies on one wave's program order");
+// Ring rows of a workgroup of `waves` waves (the multi-wave form below).
+__host__ __device__ constexpr int direct_ring_rows(int waves) { return waves == 1 ? DIR_RING : 3 * waves + 31; }
 constexpr int DIR_CMAX = 9;   // K chunks per residue class: nk <= kstep * DIR_CMAX
-constexpr int DIR_PF = 4;     // staged rows in flight in the register queue
 
 // Staged bytes per pixel: gray 1, RGBA 4, RGBA with cn = 3 staged as RGB.
 int direct_kstep(int step, int cn) { return step == 1 ? 1 : (cn == 3 ? 3 : 4); }
@@ -248,8 +256,9 @@ __device__ __forceinline__ void direct_block(uint8_t* ring, const uint8_t* __res
 #pragma unroll
     for (int j = 0; j < NA; j++) buf[j] = *(const dv4i*)(arow + 32 * min(rc + KSTEP * j, lastChunk));
   };
+  // template rows past the last clamp to it (loaded, not used)
   auto loadB = [&](dv4i* buf, int i, int rc) {
-    const gdv4p p = bq + ((int64_t)i * nk + base[rc]) * 64;
+    const gdv4p p = bq + ((int64_t)min(i, th - 1) * nk + base[rc]) * 64;
 #pragma unroll
     for (int j = 0; j < CMAX; j++) buf[j] = p[(int64_t)j * 64];
   };
@@ -261,24 +270,23 @@ __device__ __forceinline__ void direct_block(uint8_t* ring, const uint8_t* __res
         for (int t = 0; t < T; t++) acc[t] = __builtin_amdgcn_mfma_i32_32x32x32_i8(a[j + t], bb[j], acc[t], 0, 0, 0);
   };
 
-  // Staged row DIR_ROWS + i (first read in step i + 1) is fetched at the end
-  // of step i - DIR_PF (before the loop for i < DIR_PF) into a register
-  // queue and written to its ring slot at the start of step i, before step i
+  // Staged row DIR_ROWS + i (first read in step i + 1) is fetched at the
+  // start of step i - 2 (before the loop for i < 2) into register slot i & 1
+  // and written to its ring slot at the start of step i, before step i
   // prefetches step i + 1's first class (the workgroup is one wave, so its
   // LDS accesses take effect in program order).  Its slot held row
-  // i + DIR_ROWS - DIR_RING = i - 1, last read in step i - 1.  Unit nc[p] of
-  // each staged row is this thread's (units past the row reload the last
-  // one and are not written).
+  // i + DIR_ROWS - DIR_RING = i - 1, last read in step i - 1.  The rows are
+  // walked in pairs, so the two register slots are static: no copies of
+  // registers a load may still be writing.  Unit nc[p] of each staged row is
+  // this thread's (units past the row reload the last one, not written).
   int nc[2];
 #pragma unroll
   for (int p = 0; p < 2; p++) nc[p] = tid + p * 64 * DIR_WAVES;
   const int ncl[2] = {min(nc[0], units - 1), min(nc[1], units - 1)};
-  uint4 rq[DIR_PF][2];
-  bool rok[DIR_PF][2];
+  uint4 rq[2][2];
+  bool rok[2][2];
 #pragma unroll
-  for (int d = 0; d < DIR_PF - 1; d++)
-#pragma unroll
-    for (int p = 0; p < 2; p++) rq[d][p] = fetch(DIR_ROWS + d, ncl[p], rok[d][p]);
+  for (int p = 0; p < 2; p++) rq[0][p] = fetch(DIR_ROWS, ncl[p], rok[0][p]);
 
   // Odd KSTEP: class c of the whole walk (row i, class rc: c = i * KSTEP + rc)
   // uses A buffer c & 1 and loads class c + 1's A into the other one, so a
@@ -303,36 +311,37 @@ __device__ __forceinline__ void direct_block(uint8_t* ring, const uint8_t* __res
     }
   }
 #pragma unroll
-  for (int p = 0; p < 2; p++) rq[DIR_PF - 1][p] = fetch(DIR_ROWS + DIR_PF - 1, ncl[p], rok[DIR_PF - 1][p]);
-  // one template row; P0 = parity of its first class
-  auto step = [&](int i, auto P0) {
+  for (int p = 0; p < 2; p++) rq[1][p] = fetch(DIR_ROWS + 1, ncl[p], rok[1][p]);
+  // one template row; P0 = parity of its first class, QS = i & 1 (its
+  // register slot).  The rows run in pairs; with odd th the pair's second
+  // step is a row past the template that issues its loads and stores (into
+  // ring slots no real step reads again) but no MFMA, so every step has the
+  // same loads and the compiler counts the loads in flight exactly.  For the
+  // same reason the row stores are unconditional: lanes past the row's
+  // units rewrite the last unit with the same bytes.
+  auto step = [&](int i, auto P0, auto QS) {
     constexpr int p0 = decltype(P0)::value;
-    const bool more = i + 1 < th;
-    // queue: rq[d] holds staged row DIR_ROWS + i + d; the newest one was
-    // fetched at the end of the previous step, so the shift below never
-    // waits on a load issued in this step
-    if (more) {
+    constexpr int qs = decltype(QS)::value;
+    const bool live = i < th;
+    {
       const int ns = (DIR_ROWS + i) % DIR_RING;
 #pragma unroll
-      for (int p = 0; p < 2; p++)
-        if (nc[p] < units) put(ring + ns * pitch, nc[p], rq[0][p], rok[0][p]);
+      for (int p = 0; p < 2; p++) put(ring + ns * pitch, ncl[p], rq[qs][p], rok[qs][p]);
     }
+    // The register slot is refilled right after its store (for step i + 2),
+    // so it has a whole step of MFMAs to arrive; the scheduling barrier keeps
+    // the new loads behind the store, whose wait would otherwise cover them.
+    __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
-    for (int d = 0; d < DIR_PF - 1; d++)
-#pragma unroll
-      for (int p = 0; p < 2; p++) {
-        rq[d][p] = rq[d + 1][p];
-        rok[d][p] = rok[d + 1][p];
-      }
+    for (int p = 0; p < 2; p++) rq[qs][p] = fetch(DIR_ROWS + i + 2, ncl[p], rok[qs][p]);
     // this wave's A rows: staged rows 32 wave + m + i
     const uint8_t* arow = ring + ((32 * wave + m + i) % DIR_RING) * pitch + 16 * half;
     const uint8_t* arowN = ring + ((32 * wave + m + i + 1) % DIR_RING) * pitch + 16 * half;
-    const int inext = more ? i + 1 : i;
     if (!PAIRED) {
 #pragma unroll
       for (int j = 0; j < NA; j++) A0[j] = AN[j];
     }
-    if (KSTEP == 1) loadB(Bk[(p0 + 1) & 1], inext, 0);
+    if (KSTEP == 1) loadB(Bk[(p0 + 1) & 1], i + 1, 0);
 #pragma unroll
     for (int rc = 0; rc < KSTEP; rc++) {
       dv4i* ca = ((p0 + rc) & 1) ? A1 : A0;
@@ -342,51 +351,48 @@ __device__ __forceinline__ void direct_block(uint8_t* ring, const uint8_t* __res
         if (lc < KSTEP)
           loadB(Bk[lc], i, lc);
         else
-          loadB(Bk[lc - KSTEP], inext, lc - KSTEP);
+          loadB(Bk[lc - KSTEP], i + 1, lc - KSTEP);
       }
       if (rc + 1 < KSTEP) loadA(na, arow, rc + 1);
       if (rc == KSTEP - 1) {
         // the next step's first class reads the row written at this step's
         // start (program order within the wave; the barrier is a compiler
-        // fence here); past the last row it reloads an unused ring row
+        // fence here); past the last row it reads an unused ring row
         if (KSTEP == 1) __syncthreads();
-        if (PAIRED)
-          loadA(na, arowN, 0);
-        else if (more)
-          loadA(AN, arowN, 0);
+        loadA(PAIRED ? na : AN, arowN, 0);
       }
-      mma(ca, Bk[KSTEP == 1 ? (p0 & 1) : rc], cls[rc]);
+      mma(ca, Bk[KSTEP == 1 ? (p0 & 1) : rc], live ? cls[rc] : 0);
       if (rc == 0 && KSTEP > 1) __syncthreads();
     }
-#pragma unroll
-    for (int p = 0; p < 2; p++) rq[DIR_PF - 1][p] = fetch(DIR_ROWS + i + DIR_PF, ncl[p], rok[DIR_PF - 1][p]);
   };
-  if constexpr (PAIRED) {
-    for (int i = 0; i < th; i += 2) {
-      step(i, std::integral_constant<int, 0>());
-      if (i + 1 < th) step(i + 1, std::integral_constant<int, 1>());
-    }
-  } else {
-    for (int i = 0; i < th; i++) step(i, std::integral_constant<int, 0>());
+  using I0 = std::integral_constant<int, 0>;
+  using I1 = std::integral_constant<int, 1>;
+  using IP = std::integral_constant<int, PAIRED ? 1 : 0>;
+  for (int i = 0; i < th; i += 2) {
+    step(i, I0(), I0());
+    step(i + 1, IP(), I1());
   }
 }
 
 // The same output block (32 rows x 32 T columns) computed by W waves that
 // share one ring of staged rows: wave w takes the template rows i = w mod W,
-// so the W waves of a step read the overlapping row windows [W s + w,
-// W s + w + 32) and the ring holds 2 W + 31 rows (those of the step plus the W
-// being written for the next one) -- about (2 W + 31) / W rows per wave
-// instead of 33, which lets more waves share a CU.  Each step: the waves
-// write the next step's W rows (fetched a step earlier), run their template
-// row (residue classes with double-buffered A and B fragments), and meet at
-// one barrier; the slot a step writes held a row only the previous step read.
-// Each wave ends with its partial sum of the block (the caller adds them).
+// so the W waves of step s read the overlapping row windows [W s + w,
+// W s + w + 32).  The ring holds 3 W + 31 rows: those of step s, the W rows
+// step s + 1 adds (written during step s - 1) and the W being written during
+// step s for step s + 2 -- about (3 W + 31) / W rows per wave instead of 33,
+// which lets more waves share a CU.  Each step: the waves write their row for
+// step s + 2 (fetched a step earlier), run their template row (residue
+// classes with double-buffered A and B fragments) and meet at one barrier;
+// the last class already loads the next step's first A fragments (rows that
+// the barrier of step s - 1 published) and B fragments, so no step starts on
+// a load.  The slot a step writes held a row only step s - 1 read.  Each
+// wave ends with its partial sum of the block (the caller adds them).
 template <int W, int KSTEP, bool VEC, int CMAX>
 __device__ __forceinline__ void direct_block_mw(uint8_t* ring, const uint8_t* __restrict__ img, int64_t istride,
                                                 int iw, int ih, const int8_t* __restrict__ bf, int th, int nk,
                                                 int pitch, int bx, int by, dv16i (&acc)[DIR_T]) {
   constexpr int T = DIR_T;
-  constexpr int R = 2 * W + 31;  // ring rows
+  constexpr int R = direct_ring_rows(W);
   constexpr int SPU = KSTEP == 3 ? 12 : 16;
   constexpr int NA = CMAX + T - 1;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
@@ -395,19 +401,20 @@ __device__ __forceinline__ void direct_block_mw(uint8_t* ring, const uint8_t* __
   const int nrows = th + 31;  // staged rows the block reads
   const RowStager<KSTEP, VEC> stg(img, istride, iw, ih, X0, Y0);
 
-  // step 0 reads staged rows [0, W + 31)
-  for (int e = tid; e < (W + 31) * units; e += 64 * W) {
+  // steps 0 and 1 read staged rows [0, 2 W + 31)
+  for (int e = tid; e < (2 * W + 31) * units; e += 64 * W) {
     const int rho = e / units, c = e - rho * units;
     bool ok;
     const uint4 v = stg.fetch(rho, c, ok);
     RowStager<KSTEP, VEC>::put(ring + rho * pitch, c, v, ok);
   }
-  // this wave's row of step s is staged row W s + W + 31 + wave, fetched
-  // during step s - 1 (units past the row reload the last one, not written)
+  // this wave's row written in step s is staged row W s + 2 W + 31 + wave
+  // (read from step s + 2 on), fetched during step s - 1 (units past the
+  // row reload the last one and are not written)
   const int nc0 = lane, nc1 = lane + 64;
   const int ncl0 = min(nc0, units - 1), ncl1 = min(nc1, units - 1);
   bool k0, k1;
-  uint4 q0 = stg.fetch(W + 31 + wave, ncl0, k0), q1 = stg.fetch(W + 31 + wave, ncl1, k1);
+  uint4 q0 = stg.fetch(2 * W + 31 + wave, ncl0, k0), q1 = stg.fetch(2 * W + 31 + wave, ncl1, k1);
 
 #pragma unroll
   for (int t = 0; t < T; t++)
@@ -431,8 +438,9 @@ __device__ __forceinline__ void direct_block_mw(uint8_t* ring, const uint8_t* __
 #pragma unroll
     for (int j = 0; j < NA; j++) buf[j] = *(const dv4i*)(arow + 32 * min(rc + KSTEP * j, lastChunk));
   };
+  // template rows past the last clamp to it (loaded, not used)
   auto loadB = [&](dv4i* buf, int i, int rc) {
-    const gdv4p p = bq + ((int64_t)i * nk + base[rc]) * 64;
+    const gdv4p p = bq + ((int64_t)min(i, th - 1) * nk + base[rc]) * 64;
 #pragma unroll
     for (int j = 0; j < CMAX; j++) buf[j] = p[(int64_t)j * 64];
   };
@@ -445,9 +453,16 @@ __device__ __forceinline__ void direct_block_mw(uint8_t* ring, const uint8_t* __
   };
   __syncthreads();
 
+  // class c of the whole walk (step s, class rc: c = s * KSTEP + rc) uses
+  // buffers c & 1 and loads class c + 1 into the others, so with odd KSTEP a
+  // step's first class alternates between the buffer pairs and the steps are
+  // walked in pairs, every buffer index static
+  loadA(A0, ring + ((m + wave) % R) * pitch + 16 * half, 0);
+  loadB(B0, wave, 0);
   const int steps = (th + W - 1) / W;
-  for (int st = 0; st < steps; st++) {
-    const int wr = W * st + W + 31 + wave;
+  auto step = [&](int st, auto P0) {
+    constexpr int p0 = decltype(P0)::value;
+    const int wr = W * st + 2 * W + 31 + wave;
     if (wr < nrows) {
       uint8_t* dst = ring + (wr % R) * pitch;
       if (nc0 < units) RowStager<KSTEP, VEC>::put(dst, nc0, q0, k0);
@@ -456,24 +471,34 @@ __device__ __forceinline__ void direct_block_mw(uint8_t* ring, const uint8_t* __
     q0 = stg.fetch(wr + W, ncl0, k0);
     q1 = stg.fetch(wr + W, ncl1, k1);
     const int i = W * st + wave;
-    if (i < th) {
-      const uint8_t* arow = ring + ((m + i) % R) * pitch + 16 * half;
-      loadA(A0, arow, 0);
-      loadB(B0, i, 0);
+    const uint8_t* arow = ring + ((m + i) % R) * pitch + 16 * half;
+    const uint8_t* arowN = ring + ((m + i + W) % R) * pitch + 16 * half;
 #pragma unroll
-      for (int rc = 0; rc < KSTEP; rc++) {
-        dv4i* ca = (rc & 1) ? A1 : A0;
-        dv4i* na = (rc & 1) ? A0 : A1;
-        dv4i* cb = (rc & 1) ? B1 : B0;
-        dv4i* nb = (rc & 1) ? B0 : B1;
-        if (rc + 1 < KSTEP) {
-          loadA(na, arow, rc + 1);
-          loadB(nb, i, rc + 1);
-        }
-        mma(ca, cb, cls[rc]);
+    for (int rc = 0; rc < KSTEP; rc++) {
+      dv4i* ca = ((p0 + rc) & 1) ? A1 : A0;
+      dv4i* na = ((p0 + rc) & 1) ? A0 : A1;
+      dv4i* cb = ((p0 + rc) & 1) ? B1 : B0;
+      dv4i* nb = ((p0 + rc) & 1) ? B0 : B1;
+      if (rc + 1 < KSTEP) {
+        loadA(na, arow, rc + 1);
+        loadB(nb, i, rc + 1);
+      } else {
+        // the next step's first class: its rows were published by the
+        // previous step's barrier (past the last step: unused ring rows)
+        loadA(na, arowN, 0);
+        loadB(nb, i + W, 0);
       }
+      if (i < th) mma(ca, cb, cls[rc]);
     }
     __syncthreads();
+  };
+  if constexpr ((KSTEP & 1) != 0) {
+    for (int st = 0; st < steps; st += 2) {
+      step(st, std::integral_constant<int, 0>());
+      if (st + 1 < steps) step(st + 1, std::integral_constant<int, 1>());
+    }
+  } else {
+    for (int st = 0; st < steps; st++) step(st, std::integral_constant<int, 0>());
   }
 }
 
@@ -532,7 +557,7 @@ __device__ __forceinline__ void direct_block_w(uint8_t* ring, const uint8_t* __r
 }
 
 template <int W, int KSTEP, bool VEC, int CMAX>
-__global__ __launch_bounds__(64 * W) void k_direct_i8(const uint8_t* __restrict__ img, int64_t istride, int iw,
+__global__ __launch_bounds__(64 * W) CVM_DIRECT_OCC void k_direct_i8(const uint8_t* __restrict__ img, int64_t istride, int iw,
                                                       int ih, const int8_t* __restrict__ bf, int th, int nk,
                                                       int pitch, int rw, int rh, int32_t* __restrict__ out) {
   extern __shared__ uint4 ring4[];
@@ -543,7 +568,7 @@ __global__ __launch_bounds__(64 * W) void k_direct_i8(const uint8_t* __restrict_
 // Many templates against one frame in one launch (a Fleet's members, all of
 // one (KSTEP, CMAX) form): workgroup b runs block b - job.wg0 of job wg_job[b].
 template <int W, int KSTEP, bool VEC, int CMAX>
-__global__ __launch_bounds__(64 * W) void k_direct_batch(const uint8_t* __restrict__ img, int64_t istride, int iw,
+__global__ __launch_bounds__(64 * W) CVM_DIRECT_OCC void k_direct_batch(const uint8_t* __restrict__ img, int64_t istride, int iw,
                                                          int ih, const DirectJob* __restrict__ jobs,
                                                          const int* __restrict__ wg_job) {
   extern __shared__ uint4 ring4[];
@@ -570,7 +595,7 @@ int direct_cmax(int nk, int kstep) {
 // and for several waves also the partial sums added at the end.
 static size_t direct_lds_of(int pitch, int waves) {
   if (waves == 1) return (size_t)DIR_RING * pitch;
-  return std::max((size_t)(2 * waves + 31) * pitch, (size_t)(waves - 1) * 16 * 64 * sizeof(int32_t));
+  return std::max((size_t)direct_ring_rows(waves) * pitch, (size_t)(waves - 1) * 16 * 64 * sizeof(int32_t));
 }
 
 size_t direct_lds_bytes(int tw, int kstep, int waves) {
@@ -911,7 +936,7 @@ __device__ __forceinline__ void fused_score(const uint8_t* __restrict__ img, int
 // wg_job[b] and writes that block's extrema partial.  The frame's rows are
 // 16-byte aligned (direct_vec) and iw % CPT == 0.
 template <int W, int KSTEP, int CMAX, bool SQ32>
-__global__ __launch_bounds__(64 * W) void k_direct_fused(const uint8_t* __restrict__ img, int64_t istride, int iw,
+__global__ __launch_bounds__(64 * W) CVM_DIRECT_OCC void k_direct_fused(const uint8_t* __restrict__ img, int64_t istride, int iw,
                                                          int ih, const uint32_t* __restrict__ ps,
                                                          const FusedJob* __restrict__ jobs,
                                                          const int* __restrict__ wg_job) {
@@ -1033,7 +1058,7 @@ static bool fused_form_built(int kstep, int cmax, bool sq32) {
 }
 
 size_t fused_lds_bytes(int max_pitch, int kstep, int cmax, bool sq32, int waves) {
-  const size_t ring = (size_t)(waves == 1 ? DIR_RING : 2 * waves + 31) * max_pitch;
+  const size_t ring = (size_t)direct_ring_rows(waves) * max_pitch;
   const size_t score = (size_t)FB_CBYTES + (size_t)waves * fused_pbytes(fused_cpt(cmax), fused_nch(kstep), sq32);
   return std::max(ring, score);
 }

@@ -16,7 +16,8 @@
 //   CVMATCH_CROSSOVER=px      template side from which the mixed Fleet may
 //                             route members to the FFT path (0: never)
 // The cache file holds one line per key: "key choice t0 t1 ...", the times
-// (ms) of the candidates in the order they were tried.
+// (ms) of the candidates in the order they were tried; a key ends with the
+// device and the library build it was measured with.
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
@@ -72,6 +73,9 @@ struct Tuner {
       device = std::string(p.gcnArchName) + "/" + std::to_string(p.multiProcessorCount);
     else
       device = "unknown";
+    // choices measured by another build of the library (other kernels) are
+    // not reused: the key carries this build's stamp
+    device += "/" __DATE__ "/" __TIME__;
     for (char& c : device)
       if (c == ' ' || c == ':') c = '_';
     path = cache_path();

@@ -285,20 +285,25 @@ struct NormPlan {
   int wave;  // 0: int64 form
   int q;     // result columns per lane (the strip is 64*q columns)
   int sq32;  // window square sums fit uint32
+  int cpt;   // image columns per lane: the fewest of 2, 4, 8 that cover the strip
 };
 
-// Each lane of the wave form owns 8 consecutive image columns of the strip
-// (64*q + tw - 1 <= 512 columns), loaded with 16-byte vector loads.
+// Each lane of the wave form owns CPT consecutive image columns of the strip
+// (64*q + tw - 1 <= 64*CPT columns), loaded with vector loads; the widest
+// strip has CPT = 8.  A narrow strip with a narrow template needs fewer
+// columns per lane, and every column costs its sums' slide and prefix on
+// every row.
 constexpr int NORM_CPT = 8;
 // Prefix row length in LDS: prefix index c lives at c + c/32 (lane l writes
-// columns 8l..8l+7, and the pad word spreads the 64 lanes of one store over
-// all banks).
-constexpr int NORM_NWP = 64 * NORM_CPT + 1 + (64 * NORM_CPT + 1) / 32 + 1;
+// columns CPT*l..CPT*l+CPT-1, and the pad word spreads the 64 lanes of one
+// store over all banks).
+__host__ __device__ constexpr int norm_nwp(int cpt) { return 64 * cpt + 1 + (64 * cpt + 1) / 32 + 1; }
+constexpr int NORM_NWP = norm_nwp(NORM_CPT);
 
 // qpref: 4, 2 or 1 result columns per lane when that strip holds the
 // template (64 q + tw - 1 <= 512 columns), else (or 0) the widest that does.
 static NormPlan norm_plan(int tw, int th, int cnMax, int qpref = 0) {
-  NormPlan p{0, 0, 0};
+  NormPlan p{0, 0, 0, 0};
   double area = (double)tw * th;
   if ((double)cnMax * 65025.0 * area * area >= 9007199254740992.0) return p;
   if (th > 257) return p;  // packed 16-bit channel column sums: 255 * th < 2^16
@@ -308,6 +313,12 @@ static NormPlan norm_plan(int tw, int th, int cnMax, int qpref = 0) {
       p.wave = 1;
       p.q = q;
       p.sq32 = (double)cnMax * 65025.0 * area < 4294967296.0;
+      p.cpt = NORM_CPT;
+      for (int c : {2, 4})  // built: (4, 8), (2, 4), (2, 8), (1, 2), (1, 4), (1, 8)
+        if (c > q && 64 * q + tw - 1 <= 64 * c) {
+          p.cpt = c;
+          break;
+        }
       return p;
     }
   }
@@ -620,28 +631,49 @@ __global__ __launch_bounds__(256) void k_colsum_prefix(int nwords, int nch, cons
   }
 }
 
-// Eight consecutive pixels x0..x0+7 of one row as packed channel words
-// (channel k in bits 8k..).  VMODE 1: RGBA rows 16-byte aligned, two 16-byte
-// loads; VMODE 2: gray rows 8-byte aligned, one 8-byte load (both with all
-// eight columns inside the row); VMODE 0: byte loads of the nvalid valid
-// columns, zeros past them.
-template <int VMODE>
-__device__ __forceinline__ void load_px8(const uint8_t* row, int x0, int step, int cn, int nvalid,
-                                         uint32_t out[NORM_CPT]) {
+// CPT (2, 4 or 8) consecutive pixels x0.. of one row as packed channel
+// words (channel k in bits 8k..).  VMODE 1: RGBA rows 16-byte aligned, one
+// 8-byte or 16-byte loads; VMODE 2: gray rows 8-byte aligned, one 2-, 4- or
+// 8-byte load (both with all CPT columns inside the row, x0 a multiple of
+// CPT); VMODE 0: byte loads of the nvalid valid columns, zeros past them.
+template <int VMODE, int CPT>
+__device__ __forceinline__ void load_pxn(const uint8_t* row, int x0, int step, int cn, int nvalid,
+                                         uint32_t out[CPT]) {
   if constexpr (VMODE == 1) {
-    const uint4* p = (const uint4*)(row + (int64_t)x0 * 4);
-    uint4 a = p[0], b = p[1];
-    out[0] = a.x; out[1] = a.y; out[2] = a.z; out[3] = a.w;
-    out[4] = b.x; out[5] = b.y; out[6] = b.z; out[7] = b.w;
-  } else if constexpr (VMODE == 2) {
-    uint2 v = *(const uint2*)(row + x0);
+    if constexpr (CPT == 2) {
+      const uint2 a = *(const uint2*)(row + (int64_t)x0 * 4);
+      out[0] = a.x;
+      out[1] = a.y;
+    } else {
+      const uint4* p = (const uint4*)(row + (int64_t)x0 * 4);
 #pragma unroll
-    for (int j = 0; j < 4; j++) {
-      out[j] = (v.x >> (8 * j)) & 255u;
-      out[4 + j] = (v.y >> (8 * j)) & 255u;
+      for (int h = 0; h < CPT / 4; h++) {
+        const uint4 a = p[h];
+        out[4 * h] = a.x;
+        out[4 * h + 1] = a.y;
+        out[4 * h + 2] = a.z;
+        out[4 * h + 3] = a.w;
+      }
+    }
+  } else if constexpr (VMODE == 2) {
+    if constexpr (CPT == 8) {
+      const uint2 v = *(const uint2*)(row + x0);
+#pragma unroll
+      for (int j = 0; j < 4; j++) {
+        out[j] = (v.x >> (8 * j)) & 255u;
+        out[4 + j] = (v.y >> (8 * j)) & 255u;
+      }
+    } else if constexpr (CPT == 4) {
+      const uint32_t v = *(const uint32_t*)(row + x0);
+#pragma unroll
+      for (int j = 0; j < 4; j++) out[j] = (v >> (8 * j)) & 255u;
+    } else {
+      const uint32_t v = *(const uint16_t*)(row + x0);
+      out[0] = v & 255u;
+      out[1] = v >> 8;
     }
   } else {
-    for (int j = 0; j < NORM_CPT; j++) {
+    for (int j = 0; j < CPT; j++) {
       uint32_t w = 0;
       if (j < nvalid) {
         const uint8_t* p = row + (int64_t)(x0 + j) * step;
@@ -653,24 +685,35 @@ __device__ __forceinline__ void load_px8(const uint8_t* row, int x0, int step, i
   }
 }
 
-// Eight consecutive uint32 partials: two 16-byte loads when VEC (16-byte
-// aligned, all eight inside the row), else the nvalid valid ones.
-template <bool VEC>
-__device__ __forceinline__ void load_u32x8(const uint32_t* p, int nvalid, uint32_t out[NORM_CPT]) {
+// CPT consecutive uint32 partials: vector loads when VEC (aligned to their
+// size, all CPT inside the row), else the nvalid valid ones.
+template <bool VEC, int CPT>
+__device__ __forceinline__ void load_u32xn(const uint32_t* p, int nvalid, uint32_t out[CPT]) {
   if constexpr (VEC) {
-    uint4 a = ((const uint4*)p)[0], b = ((const uint4*)p)[1];
-    out[0] = a.x; out[1] = a.y; out[2] = a.z; out[3] = a.w;
-    out[4] = b.x; out[5] = b.y; out[6] = b.z; out[7] = b.w;
+    if constexpr (CPT == 2) {
+      const uint2 a = *(const uint2*)p;
+      out[0] = a.x;
+      out[1] = a.y;
+    } else {
+#pragma unroll
+      for (int h = 0; h < CPT / 4; h++) {
+        const uint4 a = ((const uint4*)p)[h];
+        out[4 * h] = a.x;
+        out[4 * h + 1] = a.y;
+        out[4 * h + 2] = a.z;
+        out[4 * h + 3] = a.w;
+      }
+    }
   } else {
-    for (int j = 0; j < NORM_CPT; j++) out[j] = j < nvalid ? p[j] : 0u;
+    for (int j = 0; j < CPT; j++) out[j] = j < nvalid ? p[j] : 0u;
   }
 }
 
 // Slide inputs of one result row: the entering and leaving pixel rows and
 // the correlation values of the row.
-template <int Q>
+template <int Q, int CPT>
 struct NormRow {
-  uint32_t a[NORM_CPT], b[NORM_CPT];
+  uint32_t a[CPT], b[CPT];
   float c[Q];
 };
 
@@ -683,13 +726,12 @@ struct NormRow {
 // NCH: channel sums kept (1 gray, 3 RGB, 4 any cn <= 4); channels at or past
 // cn are zero, so dropping them changes no sum.  P / P2: the workgroup's
 // prefix rows in LDS.
-template <int Q, bool SQ32, int VMODE, bool EXACT, int NCH, typename sq_t>
+template <int Q, bool SQ32, int VMODE, bool EXACT, int NCH, typename sq_t, int CPT>
 __device__ __forceinline__ void norm_wave_block(const uint8_t* __restrict__ img, int64_t istride, int iw, int ih,
                                                 int step, int tw, int th, int rw, int rh,
                                                 const CallState* __restrict__ st, const uint32_t* __restrict__ ps,
                                                 const float* corr, float* map, Extrema* __restrict__ parts, int bx,
-                                                int by, int gx, uint32_t (*P)[NORM_NWP], sq_t* P2) {
-  constexpr int CPT = NORM_CPT;
+                                                int by, int gx, uint32_t (*P)[norm_nwp(CPT)], sq_t* P2) {
   constexpr int SX = 64 * Q;
   const int lane = threadIdx.x;
   const int xs = bx * SX, ys = by * NB_ROWS;
@@ -752,12 +794,12 @@ __device__ __forceinline__ void norm_wave_block(const uint8_t* __restrict__ img,
       const uint32_t* p0 = ps + (int64_t)by * 3 * iw + x0;
       const uint32_t* p1 = ps + (int64_t)(by + nfull) * 3 * iw + x0;
       uint32_t a0[CPT], a1[CPT], a2[CPT], b0[CPT], b1[CPT], b2[CPT];
-      load_u32x8<VMODE != 0>(p0, nload, a0);
-      load_u32x8<VMODE != 0>(p0 + iw, nload, a1);
-      load_u32x8<VMODE != 0>(p0 + 2 * iw, nload, a2);
-      load_u32x8<VMODE != 0>(p1, nload, b0);
-      load_u32x8<VMODE != 0>(p1 + iw, nload, b1);
-      load_u32x8<VMODE != 0>(p1 + 2 * iw, nload, b2);
+      load_u32xn<VMODE != 0, CPT>(p0, nload, a0);
+      load_u32xn<VMODE != 0, CPT>(p0 + iw, nload, a1);
+      load_u32xn<VMODE != 0, CPT>(p0 + 2 * iw, nload, a2);
+      load_u32xn<VMODE != 0, CPT>(p1, nload, b0);
+      load_u32xn<VMODE != 0, CPT>(p1 + iw, nload, b1);
+      load_u32xn<VMODE != 0, CPT>(p1 + 2 * iw, nload, b2);
 #pragma unroll
       for (int j = 0; j < CPT; j++) {
         cs02[j] = b0[j] - a0[j];
@@ -766,7 +808,7 @@ __device__ __forceinline__ void norm_wave_block(const uint8_t* __restrict__ img,
       }
       for (int y = ys + nfull * NB_ROWS; y < ys + th; y++) {
         uint32_t w[CPT];
-        load_px8<VMODE>(img + (int64_t)y * istride, x0, step, cn, nload, w);
+        load_pxn<VMODE, CPT>(img + (int64_t)y * istride, x0, step, cn, nload, w);
 #pragma unroll
         for (int j = 0; j < CPT; j++) {
           uint32_t v = w[j] & cm;
@@ -779,18 +821,18 @@ __device__ __forceinline__ void norm_wave_block(const uint8_t* __restrict__ img,
 
     // Row yy's inputs; rows past the band are clamped into the image (loaded
     // and never used) so the loads need no branch.
-    auto fetch = [&](NormRow<Q>& r, int yy) {
+    auto fetch = [&](NormRow<Q, CPT>& r, int yy) {
       const int yl = min(yy, ye - 1);
       const int ya = min(yl + th, ih - 1);
       if (VMODE) {
-        load_px8<VMODE>(img + (int64_t)ya * istride, x0, step, cn, nload, r.a);
-        load_px8<VMODE>(img + (int64_t)yl * istride, x0, step, cn, nload, r.b);
+        load_pxn<VMODE, CPT>(img + (int64_t)ya * istride, x0, step, cn, nload, r.a);
+        load_pxn<VMODE, CPT>(img + (int64_t)yl * istride, x0, step, cn, nload, r.b);
       } else {
 #pragma unroll
         for (int j = 0; j < CPT; j++) r.a[j] = r.b[j] = 0;
         if (nload > 0) {
-          load_px8<0>(img + (int64_t)ya * istride, x0, step, cn, nload, r.a);
-          load_px8<0>(img + (int64_t)yl * istride, x0, step, cn, nload, r.b);
+          load_pxn<0, CPT>(img + (int64_t)ya * istride, x0, step, cn, nload, r.a);
+          load_pxn<0, CPT>(img + (int64_t)yl * istride, x0, step, cn, nload, r.b);
         }
       }
 #pragma unroll
@@ -800,7 +842,7 @@ __device__ __forceinline__ void norm_wave_block(const uint8_t* __restrict__ img,
       }
     };
 
-    auto row = [&](const NormRow<Q>& r, int y) {
+    auto row = [&](const NormRow<Q, CPT>& r, int y) {
       uint32_t run[NCH];
 #pragma unroll
       for (int k = 0; k < NCH; k++) run[k] = 0;
@@ -905,7 +947,7 @@ __device__ __forceinline__ void norm_wave_block(const uint8_t* __restrict__ img,
 
     // two register sets in turn: row y+1's loads are in flight while row y
     // is evaluated, and no loaded register is copied before use
-    NormRow<Q> ra, rb;
+    NormRow<Q, CPT> ra, rb;
     fetch(ra, ys);
     for (int y = ys; y < ye; y += 2) {
       fetch(rb, y + 1);
@@ -939,18 +981,18 @@ __device__ __forceinline__ void norm_wave_block(const uint8_t* __restrict__ img,
   }
 }
 
-template <int Q, bool SQ32, int VMODE, bool EXACT>
+template <int Q, bool SQ32, int VMODE, bool EXACT, int CPT>
 __global__ __launch_bounds__(64) void k_norm_wave(const uint8_t* __restrict__ img, int64_t istride, int iw, int ih,
                                                   int step, int tw, int th, int rw, int rh,
                                                   const CallState* __restrict__ st, const uint32_t* __restrict__ ps,
                                                   const float* corr, float* map, Extrema* __restrict__ parts) {
   typedef typename std::conditional<SQ32, uint32_t, unsigned long long>::type sq_t;
-  __shared__ uint32_t P[4][NORM_NWP];
-  __shared__ sq_t P2[NORM_NWP];
+  __shared__ uint32_t P[4][norm_nwp(CPT)];
+  __shared__ sq_t P2[norm_nwp(CPT)];
   const int cn = st->cn;
-#define CVM_NWB(N)                                                                                              \
-  norm_wave_block<Q, SQ32, VMODE, EXACT, N, sq_t>(img, istride, iw, ih, step, tw, th, rw, rh, st, ps, corr, map, \
-                                                  parts, blockIdx.x, blockIdx.y, gridDim.x, P, P2)
+#define CVM_NWB(N)                                                                                             \
+  norm_wave_block<Q, SQ32, VMODE, EXACT, N, sq_t, CPT>(img, istride, iw, ih, step, tw, th, rw, rh, st, ps, corr, \
+                                                       map, parts, blockIdx.x, blockIdx.y, gridDim.x, P, P2)
   if (VMODE == 2 || (VMODE == 0 && cn == 1))
     CVM_NWB(1);
   else if (VMODE == 1 && cn == 3)
@@ -973,7 +1015,7 @@ __global__ __launch_bounds__(64) void k_norm_batch(const uint8_t* __restrict__ i
   const int local = blockIdx.x - J.wg0;
   const int cn = J.st->cn;
 #define CVM_NBB(N)                                                                                              \
-  norm_wave_block<Q, SQ32, VMODE, EXACT, N, sq_t>(img, istride, iw, ih, step, J.tw, J.th, J.rw, J.rh, J.st, J.ps, \
+  norm_wave_block<Q, SQ32, VMODE, EXACT, N, sq_t, NORM_CPT>(img, istride, iw, ih, step, J.tw, J.th, J.rw, J.rh, J.st, J.ps, \
                                                   J.corr, nullptr, J.parts, local % J.gx, local / J.gx, J.gx, P, P2)
   if (VMODE == 2 || (VMODE == 0 && cn == 1))
     CVM_NBB(1);
@@ -1005,44 +1047,52 @@ void launch_normalize(const uint8_t* img, int64_t istride, int iw, int ih, int s
     const bool a16 = ((((uintptr_t)img) | (uintptr_t)istride) & 15) == 0;
     const bool a8 = ((((uintptr_t)img) | (uintptr_t)istride) & 7) == 0;
     int vmode = 0;
-    if (iw % NORM_CPT == 0 && step == 4 && a16) vmode = 1;
-    if (iw % NORM_CPT == 0 && step == 1 && a8) vmode = 2;
+    if (iw % np.cpt == 0 && step == 4 && a16) vmode = 1;
+    if (iw % np.cpt == 0 && step == 1 && a8) vmode = 2;
     dim3 grid((rw + 64 * np.q - 1) / (64 * np.q), (rh + NB_ROWS - 1) / NB_ROWS);
-#define CVM_NW(Q, S, V, X)                                                                                       \
-  hipLaunchKernelGGL((k_norm_wave<Q, S, V, X>), grid, dim3(64), 0, s, img, istride, iw, ih, step, tw, th, rw, rh, \
+#define CVM_NW(Q, C, S, V, X)                                                                                       \
+  hipLaunchKernelGGL((k_norm_wave<Q, S, V, X, C>), grid, dim3(64), 0, s, img, istride, iw, ih, step, tw, th, rw, rh, \
                      st, ps, corr, map, partials)
-#define CVM_NWX(Q, S, V) \
-  if (exact)             \
-    CVM_NW(Q, S, V, true); \
-  else                   \
-    CVM_NW(Q, S, V, false)
-#define CVM_NWV(Q, S)   \
-  if (vmode == 1) {     \
-    CVM_NWX(Q, S, 1);   \
-  } else if (vmode == 2) { \
-    CVM_NWX(Q, S, 2);   \
-  } else {              \
-    CVM_NWX(Q, S, 0);   \
+#define CVM_NWX(Q, C, S, V)  \
+  if (exact)                 \
+    CVM_NW(Q, C, S, V, true); \
+  else                       \
+    CVM_NW(Q, C, S, V, false)
+#define CVM_NWV(Q, C, S)      \
+  if (vmode == 1) {           \
+    CVM_NWX(Q, C, S, 1);      \
+  } else if (vmode == 2) {    \
+    CVM_NWX(Q, C, S, 2);      \
+  } else {                    \
+    CVM_NWX(Q, C, S, 0);      \
   }
-    if (np.q == 4) {
-      if (np.sq32) {
-        CVM_NWV(4, true);
-      } else {
-        CVM_NWV(4, false);
-      }
-    } else if (np.q == 2) {
-      if (np.sq32) {
-        CVM_NWV(2, true);
-      } else {
-        CVM_NWV(2, false);
-      }
-    } else {
-      if (np.sq32) {
-        CVM_NWV(1, true);
-      } else {
-        CVM_NWV(1, false);
-      }
+#define CVM_NWS(Q, C)         \
+  if (np.sq32) {              \
+    CVM_NWV(Q, C, true);      \
+  } else {                    \
+    CVM_NWV(Q, C, false);     \
+  }
+    switch (np.q * 10 + np.cpt) {
+      case 48:
+        CVM_NWS(4, 8);
+        break;
+      case 24:
+        CVM_NWS(2, 4);
+        break;
+      case 28:
+        CVM_NWS(2, 8);
+        break;
+      case 12:
+        CVM_NWS(1, 2);
+        break;
+      case 14:
+        CVM_NWS(1, 4);
+        break;
+      default:
+        CVM_NWS(1, 8);
+        break;
     }
+#undef CVM_NWS
 #undef CVM_NWV
 #undef CVM_NWX
 #undef CVM_NW

@@ -64,7 +64,7 @@ def same(got, want, label):
 def test_device_peaks_equal_oracle(cvm, orc, h, w, levels):
     rng = np.random.default_rng(h * 7 + w)
     resp = (rng.integers(0, levels, (h, w)).astype(np.float32) / max(levels - 1, 1) * 2 - 1).astype(np.float32)
-    windows = [(0, 0), (1, 2), (5, 3), (16, 12), (40, 70)]
+    windows = [(0, 0), (1, 2), (5, 3), (16, 12), (40, 70), (384, 4), (385, 4)]
     if h * w <= 5000:
         windows.append((3000, 900))  # wider than the map on both axes
     for rx, ry in windows:

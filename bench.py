@@ -221,8 +221,9 @@ def fleet_leg(frames_timed=63, solo_frames=8, fp64_frames=(0, 21, 42, 63)):
             if where == "device":
                 res[key] = got
     rep = cvm.tune_report()
-    out["direct_form"] = {k.split("@")[0]: {"fused": v["choice"], "ms_fused_unfused": v["ms"], "source": v["source"]}
-                          for k, v in rep.items() if k.startswith("fleet.fused/") and f"{fw}x{fh}" in k}
+    out["direct_form"] = {k.split("@")[0]: {"choice": v["choice"], "ms_per_candidate": v["ms"], "source": v["source"]}
+                          for k, v in rep.items()
+                          if (k.startswith("fleet.fused/") or k.startswith("fleet.fused.merge/")) and f"{fw}x{fh}" in k}
     # the direct form's unfused launches (each member's C' map written to HBM
     # and read back by the scoring launch), forced on a Fleet of its own
     os.environ["CVMATCH_FLEET_FUSED"] = "0"
@@ -783,7 +784,8 @@ def main():
                      "fleet_2560x1440": {k.split("@")[0]: {"choice": v["choice"], "ms": v["ms"], "source": v["source"]}
                                          for k, v in rep.items()
                                          if k.startswith("fleet.") and "2560x1440" in k
-                                         and not k.startswith("fleet.fused/")},
+                                         and not k.startswith("fleet.fused/")
+                                         and not k.startswith("fleet.fused.merge/")},
                      "headline_4k": {k.split("@")[0]: v["choice"] for k, v in rep.items() if "3840x2160" in k},
                      "sources": sorted({v["source"] for v in rep.values()})}
     if not args.no_cpu:

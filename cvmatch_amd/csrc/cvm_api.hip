@@ -1869,7 +1869,20 @@ static int fleet_find_direct_dev(Ctx& X, hipStream_t s, cvm_fleet* f, const uint
     bool fusable = true;
     for (size_t k = 0; k < live.size(); k++)
       if (!fused_ok(d_img, istride, pw, ks[k], cms[k], sq32s[k] != 0)) fusable = false;
-    auto build_fused = [&]() -> int {
+    // merge: members of up to five chunks per class share one launch per
+    // (kstep, sq32) at the largest such class count of the Fleet (their
+    // MFMA loops skip the classes they lack); otherwise one launch per
+    // (kstep, cmax, sq32).  Wider members keep their own launches, since
+    // their scoring strips need eight columns per lane.
+    auto build_fused = [&](int merge) -> int {
+      std::vector<int> ecm(cms);
+      if (merge) {
+        int top[5][2] = {{0, 0}, {0, 0}, {0, 0}, {0, 0}, {0, 0}};
+        for (size_t k = 0; k < live.size(); k++)
+          if (cms[k] <= 5) top[ks[k]][sq32s[k]] = std::max(top[ks[k]][sq32s[k]], cms[k]);
+        for (size_t k = 0; k < live.size(); k++)
+          if (cms[k] <= 5) ecm[k] = top[ks[k]][sq32s[k]];
+      }
       std::vector<FusedJob> fj;
       std::vector<NormJob> rj(live.size());
       std::vector<int> fwg;
@@ -1888,7 +1901,7 @@ static int fleet_find_direct_dev(Ctx& X, hipStream_t s, cvm_fleet* f, const uint
           for (int sq : {1, 0}) {
             int first = (int)fwg.size(), max_pitch = 0;
             for (int k : order) {
-              if (ks[k] != kst || cms[k] != cm || sq32s[k] != sq) continue;
+              if (ks[k] != kst || ecm[k] != cm || sq32s[k] != sq) continue;
               cvm_matcher* m = f->ms[live[k]];
               int rw = pw - m->w + 1, rh = ph - m->h + 1;
               const int8_t* bf = nullptr;
@@ -2123,7 +2136,20 @@ static int fleet_find_direct_dev(Ctx& X, hipStream_t s, cvm_fleet* f, const uint
     int use_fused = 0;
     bool unfused_built = false;  // the unfused tables of this frame shape
     if (fusable) {
-      CHK(build_fused());
+      {
+        char what[96];
+        snprintf(what, sizeof(what), "fleet.fused.merge/%dx%d/n%zu", pw, ph, live.size());
+        int rc = 0;
+        const int merge = tune_choice(what, {1, 0}, "CVMATCH_FLEET_MERGE", [&](int mg) -> float {
+          if (build_fused(mg) != 0) {
+            rc = -1;
+            return -1.0f;
+          }
+          return time_launches(s, run_fused, 1);
+        });
+        if (rc) return rc;
+        CHK(build_fused(merge));
+      }
       char what[96];
       snprintf(what, sizeof(what), "fleet.fused/%dx%d/n%zu", pw, ph, live.size());
       int rc = 0;

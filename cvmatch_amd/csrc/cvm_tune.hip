@@ -15,6 +15,9 @@
 //   CVMATCH_NORM_Q=2|4            result columns per lane of the scoring strips
 //   CVMATCH_CROSSOVER=px      template side from which the mixed Fleet may
 //                             route members to the FFT path (0: never)
+//   CVMATCH_FLEET_FUSED=0|1   the direct Fleet's fused (1) or unfused launches
+//   CVMATCH_FLEET_MERGE=0|1   fused member groups merged per (K step, square
+//                             width) (1) or one launch per chunk count (0)
 // The cache file holds one line per key: "key choice t0 t1 ...", the times
 // (ms) of the candidates in the order they were tried; a key ends with the
 // device and the library build it was measured with.

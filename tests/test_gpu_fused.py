@@ -86,13 +86,16 @@ def test_fused_equals_unfused_and_direct_match(cvm, monkeypatch, kind):
         assert a == b, (kind, sizes[i], a, b)
 
 
-@pytest.mark.parametrize("waves", ["1", "2", "4"])
-def test_fused_waves_change_no_bits(cvm, monkeypatch, waves):
+@pytest.mark.parametrize("waves,merge", [("1", "1"), ("2", "1"), ("4", "1"), ("2", "0")])
+def test_fused_waves_change_no_bits(cvm, monkeypatch, waves, merge):
+    """Every waves-per-workgroup choice, and member groups merged into one
+    launch per (K step, square width) or kept one launch per chunk count."""
     frames, tpls = synth.workload_fleet(seed=31, frames=2, members=24, w=640, h=384)
     frame = np.ascontiguousarray(frames[1])
-    say(f"[fused] 640x384 RGBA fleet of {len(tpls)}, waves {waves}")
+    say(f"[fused] 640x384 RGBA fleet of {len(tpls)}, waves {waves}, merge {merge}")
     want = [tuple(cvm.match_direct(frame, t)[1]) if t.std() > 0 else None for t in tpls]
     monkeypatch.setenv("CVMATCH_FLEET_FUSED", "1")
+    monkeypatch.setenv("CVMATCH_FLEET_MERGE", merge)
     monkeypatch.setenv("CVMATCH_DIRECT_WAVES", waves)
     got = fleet_results(cvm, tpls, frame)
     for i, (g, wv) in enumerate(zip(got, want)):

@@ -60,6 +60,10 @@ static_assert(DIR_WAVES == 1, "a ring of DIR_ROWS + 1 rows relies on one wave's 
 // Ring rows of a workgroup of `waves` waves (the multi-wave form below).
 __host__ __device__ constexpr int direct_ring_rows(int waves) { return waves == 1 ? DIR_RING : 3 * waves + 31; }
 constexpr int DIR_CMAX = 9;   // K chunks per residue class: nk <= kstep * DIR_CMAX
+// LDS behind the ring that the unused A fragments of the last ring row may
+// read: at most chunk KSTEP - 1 + KSTEP (CMAX + DIR_T - 2) of a row, i.e.
+// 32 (4 - 1 + 4 * 11) + 32 = 1536 bytes from the row's start.
+constexpr int DIR_SLACK = 2048;
 
 // Staged bytes per pixel: gray 1, RGBA 4, RGBA with cn = 3 staged as RGB.
 int direct_kstep(int step, int cn) { return step == 1 ? 1 : (cn == 3 ? 3 : 4); }
@@ -251,10 +255,13 @@ __device__ __forceinline__ void direct_block(uint8_t* ring, const uint8_t* __res
   }
   constexpr int NBUF = KSTEP == 1 ? 2 : KSTEP;
   dv4i A0[NA], A1[NA], Bk[NBUF][CMAX];
-  const int lastChunk = nk + KSTEP * (T - 1) - 1;
+  // A fragment j of class rc: chunk rc + KSTEP j of the row, a compile-time
+  // offset.  The fragments past the class's chunk count are loaded but feed
+  // no MFMA; they may read past the row (into the next ring row, or past
+  // the ring's last row into the DIR_SLACK bytes allocated behind it).
   auto loadA = [&](dv4i* buf, const uint8_t* arow, int rc) {
 #pragma unroll
-    for (int j = 0; j < NA; j++) buf[j] = *(const dv4i*)(arow + 32 * min(rc + KSTEP * j, lastChunk));
+    for (int j = 0; j < NA; j++) buf[j] = *(const dv4i*)(arow + 32 * (rc + KSTEP * j));
   };
   // template rows past the last clamp to it (loaded, not used)
   auto loadB = [&](dv4i* buf, int i, int rc) {
@@ -300,9 +307,13 @@ __device__ __forceinline__ void direct_block(uint8_t* ring, const uint8_t* __res
   // four single-wave workgroups share a CU).
   constexpr bool PAIRED = (KSTEP & 1) != 0;
   dv4i AN[PAIRED ? 1 : NA];
+  // this lane's A rows start at rbase + roff, roff = ((m + i) % DIR_RING)
+  // pitch for step i, advanced by one row a step (no division per step)
+  const uint8_t* const rbase = ring + 16 * half;
+  const int ringBytes = DIR_RING * pitch;
+  int roff = ((32 * wave + m) % DIR_RING) * pitch;
   {
-    const uint8_t* arow0 = ring + ((32 * wave + m) % DIR_RING) * pitch + 16 * half;
-    loadA(PAIRED ? A0 : AN, arow0, 0);
+    loadA(PAIRED ? A0 : AN, rbase + roff, 0);
     if (KSTEP == 1) {
       loadB(Bk[0], 0, 0);
     } else {
@@ -335,8 +346,11 @@ __device__ __forceinline__ void direct_block(uint8_t* ring, const uint8_t* __res
 #pragma unroll
     for (int p = 0; p < 2; p++) rq[qs][p] = fetch(DIR_ROWS + i + 2, ncl[p], rok[qs][p]);
     // this wave's A rows: staged rows 32 wave + m + i
-    const uint8_t* arow = ring + ((32 * wave + m + i) % DIR_RING) * pitch + 16 * half;
-    const uint8_t* arowN = ring + ((32 * wave + m + i + 1) % DIR_RING) * pitch + 16 * half;
+    int roffN = roff + pitch;
+    if (roffN >= ringBytes) roffN -= ringBytes;
+    const uint8_t* arow = rbase + roff;
+    const uint8_t* arowN = rbase + roffN;
+    roff = roffN;
     if (!PAIRED) {
 #pragma unroll
       for (int j = 0; j < NA; j++) A0[j] = AN[j];
@@ -432,11 +446,14 @@ __device__ __forceinline__ void direct_block_mw(uint8_t* ring, const uint8_t* __
       acc_s += cls[rc];
     }
   }
-  const int lastChunk = nk + KSTEP * (T - 1) - 1;
   dv4i A0[NA], A1[NA], B0[CMAX], B1[CMAX];
+  // A fragment j of class rc: chunk rc + KSTEP j of the row, a compile-time
+  // offset.  The fragments past the class's chunk count are loaded but feed
+  // no MFMA; they may read past the row (into the next ring row, or past
+  // the ring's last row into the DIR_SLACK bytes allocated behind it).
   auto loadA = [&](dv4i* buf, const uint8_t* arow, int rc) {
 #pragma unroll
-    for (int j = 0; j < NA; j++) buf[j] = *(const dv4i*)(arow + 32 * min(rc + KSTEP * j, lastChunk));
+    for (int j = 0; j < NA; j++) buf[j] = *(const dv4i*)(arow + 32 * (rc + KSTEP * j));
   };
   // template rows past the last clamp to it (loaded, not used)
   auto loadB = [&](dv4i* buf, int i, int rc) {
@@ -457,7 +474,12 @@ __device__ __forceinline__ void direct_block_mw(uint8_t* ring, const uint8_t* __
   // buffers c & 1 and loads class c + 1 into the others, so with odd KSTEP a
   // step's first class alternates between the buffer pairs and the steps are
   // walked in pairs, every buffer index static
-  loadA(A0, ring + ((m + wave) % R) * pitch + 16 * half, 0);
+  // this lane's A rows start at rbase + roff, roff = ((m + i) % R) pitch
+  // for its step's template row i, advanced by W rows a step
+  const uint8_t* const rbase = ring + 16 * half;
+  const int ringBytes = R * pitch, stepBytes = W * pitch;
+  int roff = ((m + wave) % R) * pitch;
+  loadA(A0, rbase + roff, 0);
   loadB(B0, wave, 0);
   const int steps = (th + W - 1) / W;
   auto step = [&](int st, auto P0) {
@@ -471,8 +493,11 @@ __device__ __forceinline__ void direct_block_mw(uint8_t* ring, const uint8_t* __
     q0 = stg.fetch(wr + W, ncl0, k0);
     q1 = stg.fetch(wr + W, ncl1, k1);
     const int i = W * st + wave;
-    const uint8_t* arow = ring + ((m + i) % R) * pitch + 16 * half;
-    const uint8_t* arowN = ring + ((m + i + W) % R) * pitch + 16 * half;
+    int roffN = roff + stepBytes;
+    if (roffN >= ringBytes) roffN -= ringBytes;
+    const uint8_t* arow = rbase + roff;
+    const uint8_t* arowN = rbase + roffN;
+    roff = roffN;
 #pragma unroll
     for (int rc = 0; rc < KSTEP; rc++) {
       dv4i* ca = ((p0 + rc) & 1) ? A1 : A0;
@@ -594,8 +619,9 @@ int direct_cmax(int nk, int kstep) {
 // LDS of one workgroup of `waves` waves on rows of `pitch` bytes: the ring,
 // and for several waves also the partial sums added at the end.
 static size_t direct_lds_of(int pitch, int waves) {
-  if (waves == 1) return (size_t)DIR_RING * pitch;
-  return std::max((size_t)direct_ring_rows(waves) * pitch, (size_t)(waves - 1) * 16 * 64 * sizeof(int32_t));
+  if (waves == 1) return (size_t)DIR_RING * pitch + DIR_SLACK;
+  return std::max((size_t)direct_ring_rows(waves) * pitch + DIR_SLACK,
+                  (size_t)(waves - 1) * 16 * 64 * sizeof(int32_t));
 }
 
 size_t direct_lds_bytes(int tw, int kstep, int waves) {
@@ -1058,7 +1084,7 @@ static bool fused_form_built(int kstep, int cmax, bool sq32) {
 }
 
 size_t fused_lds_bytes(int max_pitch, int kstep, int cmax, bool sq32, int waves) {
-  const size_t ring = (size_t)direct_ring_rows(waves) * max_pitch;
+  const size_t ring = (size_t)direct_ring_rows(waves) * max_pitch + DIR_SLACK;
   const size_t score = (size_t)FB_CBYTES + (size_t)waves * fused_pbytes(fused_cpt(cmax), fused_nch(kstep), sq32);
   return std::max(ring, score);
 }

@@ -242,7 +242,6 @@ __device__ __forceinline__ void direct_block(uint8_t* ring, const uint8_t* __res
   // the template and run into the padding), so the compiler's wait counts
   // are exact.
   constexpr int NA = CMAX + T - 1;
-  const gdv4p bq = (gdv4p)(const dv4i*)bf + lane;
   int cls[KSTEP], base[KSTEP];
   {
     int acc_s = 0;
@@ -264,15 +263,20 @@ __device__ __forceinline__ void direct_block(uint8_t* ring, const uint8_t* __res
     for (int j = 0; j < NA; j++) buf[j] = *(const dv4i*)(arow + 32 * (rc + KSTEP * j));
   };
   // template rows past the last clamp to it (loaded, not used)
+  // (the fragment's address: a wave-uniform row base plus this lane's 16
+  // bytes, so the loads take a scalar base and one vector offset)
   auto loadB = [&](dv4i* buf, int i, int rc) {
-    const gdv4p p = bq + ((int64_t)min(i, th - 1) * nk + base[rc]) * 64;
+    const int f0 = __builtin_amdgcn_readfirstlane(min(i, th - 1) * nk + base[rc]);
+    const gdv4p p = (gdv4p)(const dv4i*)(bf + (int64_t)f0 * 1024) + lane;
 #pragma unroll
-    for (int j = 0; j < CMAX; j++) buf[j] = p[(int64_t)j * 64];
+    for (int j = 0; j < CMAX; j++) buf[j] = p[j * 64];
   };
+  // (the class's chunk count is wave-uniform: scalar branches)
   auto mma = [&](const dv4i* a, const dv4i* bb, int c) {
+    const int cu = __builtin_amdgcn_readfirstlane(c);
 #pragma unroll
     for (int j = 0; j < CMAX; j++)
-      if (j < c)
+      if (j < cu)
 #pragma unroll
         for (int t = 0; t < T; t++) acc[t] = __builtin_amdgcn_mfma_i32_32x32x32_i8(a[j + t], bb[j], acc[t], 0, 0, 0);
   };
@@ -435,7 +439,6 @@ __device__ __forceinline__ void direct_block_mw(uint8_t* ring, const uint8_t* __
 #pragma unroll
     for (int j = 0; j < 16; j++) acc[t][j] = 0;
   const int half = lane >> 5, m = lane & 31;
-  const gdv4p bq = (gdv4p)(const dv4i*)bf + lane;
   int cls[KSTEP], base[KSTEP];
   {
     int acc_s = 0;
@@ -456,15 +459,20 @@ __device__ __forceinline__ void direct_block_mw(uint8_t* ring, const uint8_t* __
     for (int j = 0; j < NA; j++) buf[j] = *(const dv4i*)(arow + 32 * (rc + KSTEP * j));
   };
   // template rows past the last clamp to it (loaded, not used)
+  // (the fragment's address: a wave-uniform row base plus this lane's 16
+  // bytes, so the loads take a scalar base and one vector offset)
   auto loadB = [&](dv4i* buf, int i, int rc) {
-    const gdv4p p = bq + ((int64_t)min(i, th - 1) * nk + base[rc]) * 64;
+    const int f0 = __builtin_amdgcn_readfirstlane(min(i, th - 1) * nk + base[rc]);
+    const gdv4p p = (gdv4p)(const dv4i*)(bf + (int64_t)f0 * 1024) + lane;
 #pragma unroll
-    for (int j = 0; j < CMAX; j++) buf[j] = p[(int64_t)j * 64];
+    for (int j = 0; j < CMAX; j++) buf[j] = p[j * 64];
   };
+  // (the class's chunk count is wave-uniform: scalar branches)
   auto mma = [&](const dv4i* a, const dv4i* bb, int c) {
+    const int cu = __builtin_amdgcn_readfirstlane(c);
 #pragma unroll
     for (int j = 0; j < CMAX; j++)
-      if (j < c)
+      if (j < cu)
 #pragma unroll
         for (int t = 0; t < T; t++) acc[t] = __builtin_amdgcn_mfma_i32_32x32x32_i8(a[j + t], bb[j], acc[t], 0, 0, 0);
   };

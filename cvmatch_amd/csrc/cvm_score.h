@@ -20,27 +20,23 @@ __device__ __forceinline__ uint32_t chan_mask(int cn) { return cn >= 4 ? 0xFFFFF
 
 // Inclusive add-scan of a uint32 over the 64 lanes of a wave with DPP moves
 // (row_shr 1/2/4/8 inside each 16-lane row, then row_bcast 15/31 across
-// rows): VALU-latency steps instead of LDS permutes.
-template <int CTRL>
-__device__ __forceinline__ uint32_t dpp_mov(uint32_t v) {
-  return (uint32_t)__builtin_amdgcn_mov_dpp((int)v, CTRL, 0xf, 0xf, false);
+// rows): VALU-latency steps instead of LDS permutes.  Each move writes 0 to
+// the lanes that take nothing in its step (bound_ctrl for a source outside
+// the row, the row mask for the broadcasts), so every step is one add that
+// the compiler folds with its move into a single DPP add.
+template <int CTRL, int ROW_MASK, bool BOUND_CTRL>
+__device__ __forceinline__ uint32_t dpp_or_zero(uint32_t v) {
+  return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, CTRL, ROW_MASK, 0xf, BOUND_CTRL);
 }
 
 __device__ __forceinline__ uint32_t wave_incl_scan_u32(uint32_t x, int lane) {
-  const int rl = lane & 15;
-  uint32_t t;
-  t = dpp_mov<0x111>(x);
-  if (rl >= 1) x += t;
-  t = dpp_mov<0x112>(x);
-  if (rl >= 2) x += t;
-  t = dpp_mov<0x114>(x);
-  if (rl >= 4) x += t;
-  t = dpp_mov<0x118>(x);
-  if (rl >= 8) x += t;
-  t = dpp_mov<0x142>(x);
-  if ((lane & 31) >= 16) x += t;
-  t = dpp_mov<0x143>(x);
-  if (lane >= 32) x += t;
+  (void)lane;
+  x += dpp_or_zero<0x111, 0xf, true>(x);  // row_shr:1
+  x += dpp_or_zero<0x112, 0xf, true>(x);  // row_shr:2
+  x += dpp_or_zero<0x114, 0xf, true>(x);  // row_shr:4
+  x += dpp_or_zero<0x118, 0xf, true>(x);  // row_shr:8
+  x += dpp_or_zero<0x142, 0xa, false>(x);  // row_bcast:15 into rows 1 and 3
+  x += dpp_or_zero<0x143, 0xc, false>(x);  // row_bcast:31 into rows 2 and 3
   return x;
 }
 

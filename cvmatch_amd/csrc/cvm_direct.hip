@@ -913,18 +913,20 @@ __device__ __forceinline__ void fused_score(const uint8_t* __restrict__ img, int
 #pragma unroll
       for (int k = 0; k < NCH; k++) sd[k] = (double)(uint32_t)(P[k][ib] - P[k][ia]);
       const double s2 = (double)(sq_t)(P2[ib] - P2[ia]);
+      // (every product and sum here is an exact integer, so a fused
+      // multiply-add rounds nothing either: the same bits in fewer steps)
       double sq = sd[0] * sd[0], lane0 = sd[0] * tsd[0];
 #pragma unroll
       for (int k = 1; k < NCH; k++) {
-        sq = sq + sd[k] * sd[k];
-        lane0 = lane0 + sd[k] * tsd[k];
+        sq = __builtin_fma(sd[k], sd[k], sq);
+        lane0 = __builtin_fma(sd[k], tsd[k], lane0);
       }
-      const double idiff = aread * s2 - sq;
+      const double idiff = __builtin_fma(aread, s2, -sq);
       double S = sd[0];
 #pragma unroll
       for (int k = 1; k < NCH; k++) S += sd[k];
-      const double corrI = (double)r.c[q] + 128.0 * (S + tsumAll) - cpOff;
-      vq[q] = (float)score_fp64(aread * corrI - lane0, idiff, s2, aread, varSumD);
+      const double corrI = __builtin_fma(128.0, S + tsumAll, (double)r.c[q] - cpOff);
+      vq[q] = (float)score_fp64(__builtin_fma(aread, corrI, -lane0), idiff, s2, aread, varSumD);
     }
     wave_lds_sync();
     // slide to row y+1: the leaving row is inside the current sums, so each
@@ -941,11 +943,14 @@ __device__ __forceinline__ void fused_score(const uint8_t* __restrict__ img, int
       const int x = xs + lane + 64 * q;
       if (x < rw) {
         const int idx = y * rw + x;
-        if (better_min(vq[q], idx, bmin, bmin_i)) {
+        // a lane meets its windows in raster order, so a strict
+        // compare keeps the first occurrence (the index decides only
+        // between lanes and workgroups)
+        if (vq[q] < bmin) {
           bmin = vq[q];
           bmin_i = idx;
         }
-        if (better_max(vq[q], idx, bmax, bmax_i)) {
+        if (vq[q] > bmax) {
           bmax = vq[q];
           bmax_i = idx;
         }

@@ -899,19 +899,21 @@ __device__ __forceinline__ void norm_wave_block(const uint8_t* __restrict__ img,
 #pragma unroll
         for (int k = 0; k < NCH; k++) sd[k] = (double)(uint32_t)(P[k][ib] - P[k][ia]);
         const double s2 = (double)(sq_t)(P2[ib] - P2[ia]);
+        // (every product and sum here is an exact integer, so a fused
+        // multiply-add rounds nothing either: the same bits in fewer steps)
         double sq = sd[0] * sd[0], lane0 = sd[0] * tsd[0];
 #pragma unroll
         for (int k = 1; k < NCH; k++) {
-          sq = sq + sd[k] * sd[k];
-          lane0 = lane0 + sd[k] * tsd[k];
+          sq = __builtin_fma(sd[k], sd[k], sq);
+          lane0 = __builtin_fma(sd[k], tsd[k], lane0);
         }
-        const double idiff = aread * s2 - sq;
+        const double idiff = __builtin_fma(aread, s2, -sq);
         if constexpr (EXACT) {
           double S = sd[0];
 #pragma unroll
           for (int k = 1; k < NCH; k++) S += sd[k];
-          const double corrI = (double)__float_as_int(r.c[q]) + 128.0 * (S + tsumAll) - cpOff;
-          vq[q] = (float)score_fp64(aread * corrI - lane0, idiff, s2, aread, varSumD);
+          const double corrI = __builtin_fma(128.0, S + tsumAll, (double)__float_as_int(r.c[q]) - cpOff);
+          vq[q] = (float)score_fp64(__builtin_fma(aread, corrI, -lane0), idiff, s2, aread, varSumD);
         } else {
           if (one) lane0 = sd[0];
           vq[q] = norm_one(r.c[q], (float)lane0, (float)idiff, (float)s2, numScale, varScale, templNorm);
@@ -933,11 +935,14 @@ __device__ __forceinline__ void norm_wave_block(const uint8_t* __restrict__ img,
         if (x < rw) {
           const int idx = y * rw + x;
           if (map) map[(int64_t)y * rw + x] = vq[q];
-          if (better_min(vq[q], idx, bmin, bmin_i)) {
+          // a lane meets its windows in raster order, so a strict
+          // compare keeps the first occurrence (the index decides only
+          // between lanes and workgroups)
+          if (vq[q] < bmin) {
             bmin = vq[q];
             bmin_i = idx;
           }
-          if (better_max(vq[q], idx, bmax, bmax_i)) {
+          if (vq[q] > bmax) {
             bmax = vq[q];
             bmax_i = idx;
           }

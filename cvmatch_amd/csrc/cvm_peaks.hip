@@ -8,11 +8,13 @@
 // max is formed by strict comparisons (it is then one of the window's
 // elements), so the window test is separable:
 //   k_peak_rowmax  row pass: max over [x-rx, x+rx] of each row, staged in LDS
-//                  (-inf outside the row stands for "no element there"), by
-//                  doubling: maxima over 1, 2, 4 ... P <= 2rx+1 elements, then
-//                  the window is two overlapping length-P runs (log2 steps
-//                  per point instead of 2rx+1); very wide windows fall back to
-//                  a linear scan of the staged row;
+//                  up to 2048 outputs a workgroup (-inf outside the row stands
+//                  for "no element there"), by doubling: maxima over 1, 4,
+//                  16 ... then 2x as needed, P <= 2rx+1 elements (two
+//                  doublings per LDS round trip), then the window is two
+//                  overlapping length-P runs (log2 steps per point instead of
+//                  2rx+1); very wide windows fall back to a linear scan of the
+//                  staged row;
 //   k_peak_mark    column pass over the row maxima, only for points that pass
 //                  the threshold, stopping at the first greater one; four
 //                  passes of 64 consecutive points per wave, four 64-bit
@@ -26,6 +28,7 @@
 // The map is read about twice (once per pass) plus the window re-reads that
 // the caches absorb; the column pass touches only thresholded points.
 #include <math.h>
+#include <stdlib.h>
 
 #include <algorithm>
 
@@ -41,13 +44,21 @@ constexpr int PK_MASKS = (PK_T / 64) * PK_PTS;  // ballots per segment
 
 // both ping-pong buffers fit 64 KiB
 constexpr int PK_DBL_MAX_N = 65536 / (2 * (int)sizeof(float));
+// largest row segment one row-pass workgroup takes
+constexpr int PK_SEG_MAX = 2048;
 
+__device__ __forceinline__ float pk_max(float u, float v) { return v > u ? v : u; }
+
+// One workgroup per `seg` consecutive outputs of a row (a multiple of PK_T, up
+// to the whole row), staged with their 2rx halo: a wide segment reads its
+// halo once per row instead of once per 256 outputs, and spreads each
+// pass's barrier over more points.
 template <bool DOUBLING>
-__global__ __launch_bounds__(PK_T) void k_peak_rowmax(const float* __restrict__ map, int rw, int rh, int rx,
+__global__ __launch_bounds__(PK_T) void k_peak_rowmax(const float* __restrict__ map, int rw, int rh, int rx, int seg,
                                                       float* __restrict__ rmax) {
   extern __shared__ float lds[];  // DOUBLING: 2 x n, else n
-  const int x0 = blockIdx.x * PK_T;
-  const int n = PK_T + 2 * rx;
+  const int x0 = blockIdx.x * seg;
+  const int n = seg + 2 * rx;
   const int k = 2 * rx + 1;
   for (int y = blockIdx.y; y < rh; y += gridDim.y) {
     const float* row = map + (size_t)y * rw;
@@ -57,35 +68,37 @@ __global__ __launch_bounds__(PK_T) void k_peak_rowmax(const float* __restrict__ 
       a[i] = (x >= 0 && x < rw) ? row[x] : -INFINITY;
     }
     __syncthreads();
-    const int x = x0 + threadIdx.x;
-    float m;
+    float* out = rmax + (size_t)y * rw + x0;
+    const int cnt = min(seg, rw - x0);
     if (DOUBLING) {
       // a holds the maxima over runs of p elements starting at each index
-      // (valid while index + p <= n); b receives those over 2p
+      // (valid while index + p <= n); a pass writes those over 4p (or 2p)
+      // into b, two doublings per LDS round trip while 4p <= k
       float* b = lds + n;
       int p = 1;
-      while (2 * p <= k) {
-        for (int i = threadIdx.x; i + 2 * p <= n; i += PK_T) {
-          const float u = a[i], v = a[i + p];
-          b[i] = v > u ? v : u;
-        }
+      for (; 4 * p <= k; p *= 4) {
+        for (int i = threadIdx.x; i + 4 * p <= n; i += PK_T)
+          b[i] = pk_max(pk_max(a[i], a[i + p]), pk_max(a[i + 2 * p], a[i + 3 * p]));
         __syncthreads();
         float* t = a;
         a = b;
         b = t;
+      }
+      if (2 * p <= k) {
+        for (int i = threadIdx.x; i + 2 * p <= n; i += PK_T) b[i] = pk_max(a[i], a[i + p]);
+        __syncthreads();
+        a = b;
         p *= 2;
       }
-      // window [t, t + k) = runs [t, t + p) and [t + k - p, t + k)
-      const float u = a[threadIdx.x], v = a[threadIdx.x + k - p];
-      m = v > u ? v : u;
+      // p <= k < 2p: window [j, j + k) = runs [j, j + p) and [j + k - p, j + k)
+      for (int j = threadIdx.x; j < cnt; j += PK_T) out[j] = pk_max(a[j], a[j + k - p]);
     } else {
-      m = a[threadIdx.x];
-      for (int i = 1; i < k; i++) {
-        const float v = a[threadIdx.x + i];
-        m = v > m ? v : m;
+      for (int j = threadIdx.x; j < cnt; j += PK_T) {
+        float m = a[j];
+        for (int i = 1; i < k; i++) m = pk_max(m, a[j + i]);
+        out[j] = m;
       }
     }
-    if (x < rw) rmax[(size_t)y * rw + x] = m;
     __syncthreads();
   }
 }
@@ -228,6 +241,24 @@ size_t peaks_scratch_bytes(int rw, int rh) {
          sizeof(int) * (2 * nseg + (size_t)peaks_tiles((int)nseg) + 1);
 }
 
+// Outputs per row-pass workgroup: the row split into the fewest segments of
+// at most PK_SEG_MAX outputs whose doubling buffers fit 64 KiB, balanced and
+// rounded up to PK_T; one PK_T segment for the linear scan of very wide
+// windows.  CVMATCH_PEAK_SEG overrides the cap (a multiple of PK_T).
+static int peaks_row_segment(int rw, int rx) {
+  static const int cap = [] {
+    const char* e = getenv("CVMATCH_PEAK_SEG");
+    const int v = e ? atoi(e) : 0;
+    return v >= PK_T && v % PK_T == 0 ? v : PK_SEG_MAX;
+  }();
+  const int fit = (PK_DBL_MAX_N - 2 * rx) / PK_T * PK_T;
+  if (fit < PK_T) return PK_T;
+  const int most = std::min(cap, fit);
+  const int nseg = (rw + most - 1) / most;
+  const int per = (rw + nseg - 1) / nseg;
+  return (per + PK_T - 1) / PK_T * PK_T;
+}
+
 int peaks_max_rx() { return (65536 / (int)sizeof(float) - PK_T) / 2; }
 
 int* peaks_total(void* scratch, int rw, int rh) { return peaks_layout(scratch, rw, rh).total; }
@@ -236,12 +267,14 @@ void launch_peaks(const float* map, int rw, int rh, float thresh, int rx, int ry
                   int32_t* xy, float* vals, hipStream_t s) {
   const int nseg = peaks_segments(rw, rh);
   const PeakScratch p = peaks_layout(scratch, rw, rh);
-  const int n = PK_T + 2 * rx;
-  const dim3 grow((rw + PK_T - 1) / PK_T, (unsigned)std::min(rh, 65535));
+  const int seg = peaks_row_segment(rw, rx);
+  const int n = seg + 2 * rx;
+  const dim3 grow((rw + seg - 1) / seg, (unsigned)std::min(rh, 65535));
   if (n <= PK_DBL_MAX_N)
-    hipLaunchKernelGGL(k_peak_rowmax<true>, grow, dim3(PK_T), 2 * sizeof(float) * n, s, map, rw, rh, rx, p.rmax);
+    hipLaunchKernelGGL(k_peak_rowmax<true>, grow, dim3(PK_T), 2 * sizeof(float) * n, s, map, rw, rh, rx, seg,
+                       p.rmax);
   else
-    hipLaunchKernelGGL(k_peak_rowmax<false>, grow, dim3(PK_T), sizeof(float) * n, s, map, rw, rh, rx, p.rmax);
+    hipLaunchKernelGGL(k_peak_rowmax<false>, grow, dim3(PK_T), sizeof(float) * n, s, map, rw, rh, rx, seg, p.rmax);
   hipLaunchKernelGGL(k_peak_mark, dim3(nseg), dim3(PK_T), 0, s, map, p.rmax, rw, rh, ry, thresh, p.masks, p.counts,
                      p.total);
   hipLaunchKernelGGL(k_peak_tilescan, dim3(peaks_tiles(nseg)), dim3(PK_TILE), 0, s, p.counts, nseg, p.offsets,

@@ -303,11 +303,13 @@ struct GraphSlot {
 // share no scratch and overlap on the device.
 struct Ctx {
   hipStream_t stream = nullptr;
-  // side stream of a MatchMap call, forked from the caller's stream and
-  // joined back into it twice: before the column pass (call state, template
-  // spectrum) and before the normalisation (its partials)
-  hipStream_t side = nullptr;
-  hipEvent_t fork = nullptr, join = nullptr, join2 = nullptr;
+  // side streams of a MatchMap call, forked from the caller's stream: side
+  // builds the call state (joined back before the column pass) and then the
+  // normalisation partials (joined before the normalisation); side2 builds
+  // the template spectrum meanwhile, which needs no call state (joined
+  // before the column pass)
+  hipStream_t side = nullptr, side2 = nullptr;
+  hipEvent_t fork = nullptr, join = nullptr, join2 = nullptr, join3 = nullptr;
   // the captured device-resident call (run_graphed)
   GraphSlot graph;
   DevBuf img, tpl, spec, tspec, tspec2, corr, parts, gray0, gray1, mapbuf, nscratch, dbf, pkscr, pkout;
@@ -331,9 +333,11 @@ struct Ctx {
   int init() {
     HIPCHK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
     HIPCHK(hipStreamCreateWithFlags(&side, hipStreamNonBlocking));
+    HIPCHK(hipStreamCreateWithFlags(&side2, hipStreamNonBlocking));
     HIPCHK(hipEventCreateWithFlags(&fork, hipEventDisableTiming));
     HIPCHK(hipEventCreateWithFlags(&join, hipEventDisableTiming));
     HIPCHK(hipEventCreateWithFlags(&join2, hipEventDisableTiming));
+    HIPCHK(hipEventCreateWithFlags(&join3, hipEventDisableTiming));
     HIPCHK(hipMalloc(&st, sizeof(CallState)));
     HIPCHK(hipMalloc(&res_d, sizeof(Res6) * 2));
     HIPCHK(hipHostMalloc(&res_h, sizeof(Res6) * 2, hipHostMallocDefault));
@@ -360,12 +364,12 @@ struct Ctx {
     if (res_d) (void)hipFree(res_d);
     if (res_h) (void)hipHostFree(res_h);
     if (rmap_h) (void)hipHostFree(rmap_h);
-    for (hipStream_t q : {stream, side})
+    for (hipStream_t q : {stream, side, side2})
       if (q) (void)xfer_release_stream(q);
     for (hipEvent_t e : evpool) (void)hipEventDestroy(e);
-    for (hipEvent_t e : {fork, join, join2})
+    for (hipEvent_t e : {fork, join, join2, join3})
       if (e) (void)hipEventDestroy(e);
-    for (hipStream_t q : {stream, side})
+    for (hipStream_t q : {stream, side, side2})
       if (q) (void)hipStreamDestroy(q);
   }
 
@@ -492,11 +496,20 @@ struct Stage {
 // capture changes the allocation epoch appended to it.  A call that cannot be
 // captured (it allocates or synchronises) runs eagerly instead.  Nothing
 // inside a call fills memory with hipMemsetAsync (fills are kernels).
+// CVMATCH_GRAPHS=0 runs every call eagerly.
+static bool graphs_enabled() {
+  static const bool on = [] {
+    const char* v = getenv("CVMATCH_GRAPHS");
+    return !(v && v[0] == '0');
+  }();
+  return on;
+}
+
 template <class F>
 static int run_graphed(GraphSlot& slot, std::vector<uint64_t> key, hipStream_t s, F&& run) {
   key.push_back(g_alloc_epoch.load());
   key.push_back((uint64_t)(uintptr_t)s);
-  if (engine().prof.load()) {  // stage events need the eager launches
+  if (engine().prof.load() || !graphs_enabled()) {  // stage events need the eager launches
     slot.drop();
     return run();
   }
@@ -617,10 +630,10 @@ static Res6* res_out(Ctx& X) { return X.rmap_d ? X.rmap_d : X.res_d; }
 // them) run first, then s waits for join.  Otherwise all of them are ready
 // on s, except the partials, built here.
 // side (may be null): enqueues the call's side-stream work on X.side and
-// records X.join once the call state and the template spectrum are ready and
-// X.join2 once the normalisation partials are.  It runs after the parent's row pass has
-// been submitted on s, so that pass (which needs none of it) is first in
-// line and overlaps the side work.
+// X.side2 and records X.join once the call state is ready, X.join3 once the
+// template spectrum is and X.join2 once the normalisation partials are.  It
+// runs after the parent's row pass has been submitted on s, so that pass
+// (which needs none of it) is first in line and overlaps the side work.
 // Partials of the widest range of strip widths a call's normalisation may take.
 static int norm_partials_max(int rw, int rh, int tw, int th, int cnMax) {
   int n = norm_partials_count(rw, rh, tw, th, cnMax);
@@ -708,6 +721,7 @@ static int corr_and_normalize(Ctx& X, hipStream_t s, const CallState* st, const 
   if (side) {
     CHK((*side)());
     HIPCHK(hipStreamWaitEvent(s, X.join, 0));
+    HIPCHK(hipStreamWaitEvent(s, X.join3, 0));
   }
   {
     Stage S(X, "cols", s);
@@ -807,16 +821,20 @@ static int run_match(Ctx& X, hipStream_t s, const uint8_t* d_img, int64_t istrid
     CHK(X.tspec2.ensure(sizeof(float2) * tchan2 * cnMax));
     ts.spec2 = X.tspec2.as<float2>();
   }
-  // the call state, the template spectrum and the normalisation partials
-  // are built on the side stream while s transforms the parent rows
+  // while s transforms the parent rows, the call state and then the
+  // normalisation partials are built on one side stream and the template
+  // spectrum (every channel: it needs no call state) on the other
   SideWork side = [&]() -> int {
+    hipStream_t q2 = X.side2;
+    HIPCHK(hipStreamWaitEvent(q2, X.fork, 0));
+    CHK(build_tspec(X, q2, nullptr, d_tpl, tstride, tw, th, step, cnMax, p, dh2, pt, ts.spec, ts.spec2));
+    HIPCHK(hipEventRecord(X.join3, q2));
     hipStream_t q = X.side;
     {
       Stage S(X, "stats", q);
       launch_state_init(X.st, q);
       launch_call_state(d_img, istride, iw, ih, d_tpl, tstride, tw, th, cnMax, step, cn_mode, X.st, q);
     }
-    CHK(build_tspec(X, q, X.st, d_tpl, tstride, tw, th, step, cnMax, p, dh2, pt, ts.spec, ts.spec2));
     HIPCHK(hipEventRecord(X.join, q));
     {
       Stage S(X, "norm_prep", q);

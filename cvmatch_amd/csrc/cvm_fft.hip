@@ -588,7 +588,8 @@ __device__ __forceinline__ void cols_body(float2* a, int L, const float2* __rest
   const int t = t0 + L % ntiles, rest = L / ntiles;
   const int slab = rest % nslabs, k = rest / nslabs;
   const int c0 = slab * CG;
-  if (st->flat || k >= st->cn || c0 >= g.hw) return;
+  // st null: every channel (a template spectrum built before its call state)
+  if ((st && (st->flat || k >= st->cn)) || c0 >= g.hw) return;
   int x0, y0, bw, bh;
   tile_rect(g, t, x0, y0, bw, bh);
   const float2* ts = tspec + (int64_t)k * tchanStride + c0;

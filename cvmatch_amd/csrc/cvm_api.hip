@@ -581,15 +581,8 @@ static int check_limits(const Plan& p, int tw, int rw, int rh) {
   return 0;
 }
 
-// Template spectrum on plan p (buildTSpecSet impl.go:520-561): forward
-// transform of the template at (0,0), scaled by 1/(dftW*dftH), plus the
-// shrunk band (shrinkInto impl.go:591-610).  dst buffers sized by caller.
-static int build_tspec(Ctx& X, hipStream_t s, const CallState* st, const uint8_t* d_tpl, int64_t tstride, int tw,
-                       int th, int step, int cnMax, const Plan& p, int dh2, const PlanTables& pt, float2* spec,
-                       float2* spec2) {
-  Stage S(X, "tspec", s);
-  int hwPad = round_up(p.hw, 16);
-  int64_t tchan = (int64_t)p.dftH * hwPad, tchan2 = (int64_t)dh2 * hwPad;
+// The template as the single tile of plan p (its spectrum's grid).
+static TileGrid tpl_grid(const Plan& p, int tw, int th) {
   TileGrid gt{};
   gt.ntx = gt.nty = 1;
   gt.blockW = tw;
@@ -601,11 +594,47 @@ static int build_tspec(Ctx& X, hipStream_t s, const CallState* st, const uint8_t
   gt.dftW = p.dftW;
   gt.log2W = ilog2(p.dftW);
   gt.hw = p.hw;
-  gt.hwPad = hwPad;
+  gt.hwPad = round_up(p.hw, 16);
   gt.dftH = p.dftH;
   gt.log2H = ilog2(p.dftH);
   gt.dftH2 = 0;
   gt.log2H2 = 0;
+  return gt;
+}
+
+// The parent's tiles on plan p (crossCorrGo impl.go:616-693), shrunk band dh2.
+static TileGrid parent_grid(const Plan& p, int tw, int th, int rw, int rh, int hwPad, int dh2) {
+  TileGrid g{};
+  g.ntx = (rw + p.blockW - 1) / p.blockW;
+  g.nty = (rh + p.blockH - 1) / p.blockH;
+  g.blockW = p.blockW;
+  g.blockH = p.blockH;
+  g.extW = rw;
+  g.extH = rh;
+  g.padW = tw - 1;
+  g.padH = th - 1;
+  g.lastY0 = ((rh - 1) / p.blockH) * p.blockH;
+  g.dftW = p.dftW;
+  g.log2W = ilog2(p.dftW);
+  g.hw = p.hw;
+  g.hwPad = hwPad;
+  g.dftH = p.dftH;
+  g.log2H = ilog2(p.dftH);
+  g.dftH2 = dh2;
+  g.log2H2 = dh2 ? ilog2(dh2) : 0;
+  return g;
+}
+
+// Template spectrum on plan p (buildTSpecSet impl.go:520-561): forward
+// transform of the template at (0,0), scaled by 1/(dftW*dftH), plus the
+// shrunk band (shrinkInto impl.go:591-610).  dst buffers sized by caller.
+static int build_tspec(Ctx& X, hipStream_t s, const CallState* st, const uint8_t* d_tpl, int64_t tstride, int tw,
+                       int th, int step, int cnMax, const Plan& p, int dh2, const PlanTables& pt, float2* spec,
+                       float2* spec2) {
+  Stage S(X, "tspec", s);
+  int hwPad = round_up(p.hw, 16);
+  int64_t tchan = (int64_t)p.dftH * hwPad, tchan2 = (int64_t)dh2 * hwPad;
+  const TileGrid gt = tpl_grid(p, tw, th);
   launch_row_fwd(d_tpl, tstride, step, cnMax, st, gt, pt, spec, 0, tchan, 1, s);
   float scale = 1.0f / ((float)p.dftW * (float)p.dftH);
   launch_cols(COL_TSPEC, spec, spec, 0, 0, tchan, cnMax, st, gt, pt, nullptr, nullptr, 0, 0, scale, 1, s);
@@ -678,25 +707,8 @@ static int corr_and_normalize(Ctx& X, hipStream_t s, const CallState* st, const 
   const Plan& p = ts.p;
   int rw = iw - tw + 1, rh = ih - th + 1;
   int hwPad = ts.hwPad;
-  int ntx = (rw + p.blockW - 1) / p.blockW, nty = (rh + p.blockH - 1) / p.blockH, ntiles = ntx * nty;
-  TileGrid g{};
-  g.ntx = ntx;
-  g.nty = nty;
-  g.blockW = p.blockW;
-  g.blockH = p.blockH;
-  g.extW = rw;
-  g.extH = rh;
-  g.padW = tw - 1;
-  g.padH = th - 1;
-  g.lastY0 = ((rh - 1) / p.blockH) * p.blockH;
-  g.dftW = p.dftW;
-  g.log2W = ilog2(p.dftW);
-  g.hw = p.hw;
-  g.hwPad = hwPad;
-  g.dftH = p.dftH;
-  g.log2H = ilog2(p.dftH);
-  g.dftH2 = ts.dh2;
-  g.log2H2 = ts.dh2 ? ilog2(ts.dh2) : 0;
+  const TileGrid g = parent_grid(p, tw, th, rw, rh, hwPad, ts.dh2);
+  const int ntiles = g.ntx * g.nty;
   int64_t chan = (int64_t)p.dftH * hwPad, tileStride = chan * cnMax;
   int64_t tchan = chan, tchan2 = (int64_t)ts.dh2 * hwPad;
   // every buffer first: no allocation lands between fork and join
@@ -797,6 +809,82 @@ static int alpha_rule_cn(Ctx& X, hipStream_t s, const uint8_t* d_img, int64_t is
   return 0;
 }
 
+// Parents of at most this many pixels take the compact form of a one-shot
+// call (launch_compact_fft); CVMATCH_COMPACT=<pixels> moves the bound, 0
+// turns the form off.
+constexpr long long kCompactPixels = 2500000;
+static long long compact_max_pixels() {
+  static const long long v = [] {
+    const char* e = getenv("CVMATCH_COMPACT");
+    return e ? atoll(e) : kCompactPixels;
+  }();
+  return v;
+}
+
+// A small one-shot call in one stream: the transforms, the call state and
+// the chunk sums in five launches whose independent jobs share a grid
+// (launch_compact_fft), then the normalisation and the reduction.  The
+// same kernels' code as the forked form, so the same bits.
+static int run_compact(Ctx& X, hipStream_t s, const uint8_t* d_img, int64_t istride, int iw, int ih,
+                       const uint8_t* d_tpl, int64_t tstride, int tw, int th, int cn_mode, int step, int cnMax,
+                       const TplSpec& ts, const PlanTables& pt, const TileGrid& g, float* d_map) {
+  const Plan& p = ts.p;
+  const int rw = iw - tw + 1, rh = ih - th + 1, ntiles = g.ntx * g.nty;
+  const int64_t chan = (int64_t)p.dftH * ts.hwPad, tileStride = chan * cnMax;
+  CHK(X.spec.ensure(sizeof(float2) * (size_t)tileStride * ntiles));
+  float* corr = d_map;
+  if (!corr) {
+    CHK(X.corr.ensure(sizeof(float) * (size_t)rw * rh));
+    corr = X.corr.as<float>();
+  }
+  CHK(X.parts.ensure(sizeof(Extrema) * (size_t)norm_partials_max(rw, rh, tw, th, cnMax)));
+  CompactCall c{};
+  c.img = d_img;
+  c.istride = istride;
+  c.iw = iw;
+  c.ih = ih;
+  c.step = step;
+  c.cnMax = cnMax;
+  c.cn_mode = cn_mode;
+  c.tw = tw;
+  c.th = th;
+  c.rw = rw;
+  c.ntiles = ntiles;
+  c.g = g;
+  c.gt = tpl_grid(p, tw, th);
+  c.spec = X.spec.as<float2>();
+  c.tileStride = tileStride;
+  c.chanStride = chan;
+  c.tpl = d_tpl;
+  c.tstride = tstride;
+  c.tspec = ts.spec;
+  c.tchan = chan;
+  c.scale = 1.0f / ((float)p.dftW * (float)p.dftH);
+  c.pt = pt;
+  c.st = X.st;
+  c.csa = call_state_args(d_img, istride, iw, ih, d_tpl, tstride, tw, th, cnMax, step, cn_mode, X.st, true);
+  c.corr = corr;
+  c.nscratch = X.nscratch.p;
+  {
+    Stage S(X, "compact", s);
+    launch_compact_fft(c, s);
+  }
+  const int q = norm_q_for(X, s, d_img, istride, iw, ih, step, tw, th, cnMax, X.st, corr, X.parts.as<Extrema>(),
+                           X.nscratch.p, false);
+  {
+    Stage S(X, "normalize", s);
+    launch_normalize(d_img, istride, iw, ih, step, tw, th, rw, rh, cnMax, X.st, corr, d_map, X.parts.as<Extrema>(),
+                     X.nscratch.p, s, false, q);
+  }
+  {
+    Stage S(X, "reduce", s);
+    launch_reduce(X.parts.as<Extrema>(), norm_partials_count(rw, rh, tw, th, cnMax, q), X.st, rw, res_out(X),
+                  nullptr, rh, s);
+  }
+  HIPCHK(hipGetLastError());
+  return 0;
+}
+
 // matchU8 on device buffers.  cn_mode 0 = Match's RGBA alpha rule.
 static int run_match(Ctx& X, hipStream_t s, const uint8_t* d_img, int64_t istride, int iw, int ih,
                      const uint8_t* d_tpl, int64_t tstride, int tw, int th, int cn_mode, int step, float* d_map) {
@@ -821,6 +909,9 @@ static int run_match(Ctx& X, hipStream_t s, const uint8_t* d_img, int64_t istrid
     CHK(X.tspec2.ensure(sizeof(float2) * tchan2 * cnMax));
     ts.spec2 = X.tspec2.as<float2>();
   }
+  const TileGrid g = parent_grid(p, tw, th, rw, rh, ts.hwPad, dh2);
+  if ((long long)iw * ih <= compact_max_pixels() && compact_ok(g) && norm_wave_ok(tw, th, cnMax))
+    return run_compact(X, s, d_img, istride, iw, ih, d_tpl, tstride, tw, th, cn_mode, step, cnMax, ts, pt, g, d_map);
   // while s transforms the parent rows, the call state and then the
   // normalisation partials are built on one side stream and the template
   // spectrum (every channel: it needs no call state) on the other

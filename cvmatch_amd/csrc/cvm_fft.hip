@@ -20,6 +20,7 @@
 //     (the channels of one row pair, the tiles of one template-spectrum slab)
 //     run on the same XCD and share its L2.
 #include "cvm_internal.h"
+#include "cvm_state.h"
 
 namespace cvm {
 
@@ -338,14 +339,29 @@ static int row_cpg(int n, int cnMax) {
 // two half spectra, stored at spectrum rows r and r+1 (natural order; the
 // column kernel applies the bit reversal when it loads).  One workgroup
 // handles up to NVEC channels of one row pair.
+// One forward row launch's arguments (a buffer of tiles of one image).
+struct RowFwdArgs {
+  const uint8_t* img;
+  int64_t istride;
+  int step;
+  const CallState* st;
+  TileGrid g;
+  float2* spec;
+  int64_t tileStride, chanStride;
+  int cpg, ngroups, npairs, words, cnAll;
+  int nwg;  // workgroups: ngroups * npairs * ntiles
+};
+
+// Workgroup L of a forward row launch; z: the LDS vectors.
 template <int LOG2N, int NVEC>
-__global__ __launch_bounds__(RowNTF<LOG2N>::value) void k_row_fwd(
-    const uint8_t* __restrict__ img, int64_t istride, int step, const CallState* __restrict__ st, TileGrid g,
-    const float2* __restrict__ triW, float2* __restrict__ spec, int64_t tileStride, int64_t chanStride, int cpg,
-    int ngroups, int npairs, int words, int cnAll) {
+__device__ __forceinline__ void row_fwd_body(float2* z, int L, const RowFwdArgs& A, const float2* __restrict__ triW) {
   constexpr int n = 1 << LOG2N, NT = RowNTF<LOG2N>::value, np = n + (n >> 5);
-  extern __shared__ float2 z[];
-  const int L = xcd_remap(blockIdx.x, gridDim.x);
+  const uint8_t* __restrict__ img = A.img;
+  const int64_t istride = A.istride, tileStride = A.tileStride, chanStride = A.chanStride;
+  const int step = A.step, cpg = A.cpg, ngroups = A.ngroups, npairs = A.npairs, words = A.words, cnAll = A.cnAll;
+  const CallState* __restrict__ st = A.st;
+  const TileGrid& g = A.g;
+  float2* __restrict__ spec = A.spec;
   const int cgi = L % ngroups, rest = L / ngroups;
   const int rp = rest % npairs, t = rest / npairs;
   // without a call state every channel of the buffer is transformed
@@ -428,26 +444,47 @@ __global__ __launch_bounds__(RowNTF<LOG2N>::value) void k_row_fwd(
   }
 }
 
+template <int LOG2N, int NVEC>
+__global__ __launch_bounds__(RowNTF<LOG2N>::value) void k_row_fwd(RowFwdArgs A, const float2* __restrict__ triW) {
+  extern __shared__ float2 z[];
+  row_fwd_body<LOG2N, NVEC>(z, xcd_remap(blockIdx.x, gridDim.x), A, triW);
+}
+
+// The forward row launch of one image on plan width 2^LOG2N.
+template <int LOG2N>
+static RowFwdArgs row_fwd_args(const uint8_t* img, int64_t istride, int step, int cnMax, const CallState* st,
+                               const TileGrid& g, float2* spec, int64_t tileStride, int64_t chanStride, int ntiles) {
+  RowFwdArgs a;
+  a.img = img;
+  a.istride = istride;
+  a.step = step;
+  a.st = st;
+  a.g = g;
+  a.spec = spec;
+  a.tileStride = tileStride;
+  a.chanStride = chanStride;
+  a.cpg = row_cpg(1 << LOG2N, cnMax);
+  a.ngroups = (cnMax + a.cpg - 1) / a.cpg;
+  a.npairs = g.dftH / 2;
+  a.words = (step == 4 && ((((uintptr_t)img) | (uintptr_t)istride) & 3) == 0) ? 1 : 0;
+  a.cnAll = cnMax;
+  a.nwg = a.ngroups * a.npairs * ntiles;
+  return a;
+}
+
 template <int LOG2N>
 static void row_fwd_n(const uint8_t* img, int64_t istride, int step, int cnMax, const CallState* st,
                       const TileGrid& g, const PlanTables& pt, float2* spec, int64_t tileStride, int64_t chanStride,
                       int ntiles, hipStream_t s) {
-  constexpr int n = 1 << LOG2N;
-  int cpg = row_cpg(n, cnMax);
-  int ngroups = (cnMax + cpg - 1) / cpg;
-  int npairs = g.dftH / 2;
-  size_t lds = sizeof(float2) * (size_t)padded_len(n) * cpg;
-  dim3 grid(ngroups * npairs * ntiles), block(RowNTF<LOG2N>::value);
-  int words = (step == 4 && ((((uintptr_t)img) | (uintptr_t)istride) & 3) == 0) ? 1 : 0;
-  if (cpg <= 1)
-    hipLaunchKernelGGL((k_row_fwd<LOG2N, 1>), grid, block, lds, s, img, istride, step, st, g, pt.triW, spec,
-                       tileStride, chanStride, cpg, ngroups, npairs, words, cnMax);
-  else if (cpg == 2)
-    hipLaunchKernelGGL((k_row_fwd<LOG2N, 2>), grid, block, lds, s, img, istride, step, st, g, pt.triW, spec,
-                       tileStride, chanStride, cpg, ngroups, npairs, words, cnMax);
+  const RowFwdArgs a = row_fwd_args<LOG2N>(img, istride, step, cnMax, st, g, spec, tileStride, chanStride, ntiles);
+  size_t lds = sizeof(float2) * (size_t)padded_len(1 << LOG2N) * a.cpg;
+  dim3 grid(a.nwg), block(RowNTF<LOG2N>::value);
+  if (a.cpg <= 1)
+    hipLaunchKernelGGL((k_row_fwd<LOG2N, 1>), grid, block, lds, s, a, pt.triW);
+  else if (a.cpg == 2)
+    hipLaunchKernelGGL((k_row_fwd<LOG2N, 2>), grid, block, lds, s, a, pt.triW);
   else
-    hipLaunchKernelGGL((k_row_fwd<LOG2N, 4>), grid, block, lds, s, img, istride, step, st, g, pt.triW, spec,
-                       tileStride, chanStride, cpg, ngroups, npairs, words, cnMax);
+    hipLaunchKernelGGL((k_row_fwd<LOG2N, 4>), grid, block, lds, s, a, pt.triW);
 }
 
 void launch_row_fwd(const uint8_t* img, int64_t istride, int step, int cnMax, const CallState* st,
@@ -990,14 +1027,28 @@ void launch_shrink(const float2* tspec, float2* tspec2, int cnMax, int rows2, in
 // accumulate the real/imaginary parts into result rows y0+r / y0+r+1 with
 // channel 0 stored and channels 1.. added in order (in registers; each
 // thread owns the same columns for every channel).
+// One inverse row launch's arguments.
+struct RowInvArgs {
+  const float2* spec;
+  int64_t tileStride, chanStride;
+  const CallState* st;
+  TileGrid g;
+  float* corr;
+  int rw, cpg, npairs;
+  int nwg;  // workgroups: npairs * ntiles
+};
+
+// Workgroup L of an inverse row launch; z: the LDS vectors.
 template <int LOG2N, int NVEC>
-__global__ __launch_bounds__(RowNT<LOG2N>::value) void k_row_inv(
-    const float2* __restrict__ spec, int64_t tileStride, int64_t chanStride, const CallState* __restrict__ st,
-    TileGrid g, const float2* __restrict__ triW, float* __restrict__ corr, int rw, int cpg, int npairs) {
+__device__ __forceinline__ void row_inv_body(float2* z, int L, const RowInvArgs& A, const float2* __restrict__ triW) {
   constexpr int n = 1 << LOG2N, NT = RowNT<LOG2N>::value, np = n + (n >> 5);
   constexpr int MAXI = (n + NT - 1) / NT;
-  extern __shared__ float2 z[];
-  const int L = blockIdx.x;
+  const float2* __restrict__ spec = A.spec;
+  const int64_t tileStride = A.tileStride, chanStride = A.chanStride;
+  const CallState* __restrict__ st = A.st;
+  const TileGrid& g = A.g;
+  float* __restrict__ corr = A.corr;
+  const int rw = A.rw, cpg = A.cpg, npairs = A.npairs;
   const int rp = L % npairs, t = L / npairs;
   const int r = 2 * rp;
   if (st->flat) return;
@@ -1070,28 +1121,46 @@ __global__ __launch_bounds__(RowNT<LOG2N>::value) void k_row_inv(
   }
 }
 
+template <int LOG2N, int NVEC>
+__global__ __launch_bounds__(RowNT<LOG2N>::value) void k_row_inv(RowInvArgs A, const float2* __restrict__ triW) {
+  extern __shared__ float2 z[];
+  row_inv_body<LOG2N, NVEC>(z, blockIdx.x, A, triW);
+}
+
 template <int LOG2N>
-static void row_inv_n(const float2* spec, int64_t tileStride, int64_t chanStride, const CallState* st, int cnMax,
-                      const TileGrid& g, const PlanTables& pt, float* corr, int rw, int ntiles, hipStream_t s) {
-  constexpr int n = 1 << LOG2N;
-  int cpg = row_cpg(n, cnMax);
-  size_t lds = sizeof(float2) * (size_t)padded_len(n) * cpg;
+static RowInvArgs row_inv_args(const float2* spec, int64_t tileStride, int64_t chanStride, const CallState* st,
+                               int cnMax, const TileGrid& g, float* corr, int rw, int ntiles) {
+  RowInvArgs a;
+  a.spec = spec;
+  a.tileStride = tileStride;
+  a.chanStride = chanStride;
+  a.st = st;
+  a.g = g;
+  a.corr = corr;
+  a.rw = rw;
+  a.cpg = row_cpg(1 << LOG2N, cnMax);
   // last-band tiles may be taller than blockH (fleet members): cover the
   // largest bh any tile can have
   int bhMax = g.blockH;
   int bhl = g.extH - (g.nty - 1) * g.blockH;
   if (bhl > bhMax) bhMax = bhl;
-  int npairs = (bhMax + 1) / 2;
-  dim3 grid(npairs * ntiles), block(RowNT<LOG2N>::value);
-  if (cpg <= 1)
-    hipLaunchKernelGGL((k_row_inv<LOG2N, 1>), grid, block, lds, s, spec, tileStride, chanStride, st, g, pt.triW, corr,
-                       rw, cpg, npairs);
-  else if (cpg == 2)
-    hipLaunchKernelGGL((k_row_inv<LOG2N, 2>), grid, block, lds, s, spec, tileStride, chanStride, st, g, pt.triW, corr,
-                       rw, cpg, npairs);
+  a.npairs = (bhMax + 1) / 2;
+  a.nwg = a.npairs * ntiles;
+  return a;
+}
+
+template <int LOG2N>
+static void row_inv_n(const float2* spec, int64_t tileStride, int64_t chanStride, const CallState* st, int cnMax,
+                      const TileGrid& g, const PlanTables& pt, float* corr, int rw, int ntiles, hipStream_t s) {
+  const RowInvArgs a = row_inv_args<LOG2N>(spec, tileStride, chanStride, st, cnMax, g, corr, rw, ntiles);
+  size_t lds = sizeof(float2) * (size_t)padded_len(1 << LOG2N) * a.cpg;
+  dim3 grid(a.nwg), block(RowNT<LOG2N>::value);
+  if (a.cpg <= 1)
+    hipLaunchKernelGGL((k_row_inv<LOG2N, 1>), grid, block, lds, s, a, pt.triW);
+  else if (a.cpg == 2)
+    hipLaunchKernelGGL((k_row_inv<LOG2N, 2>), grid, block, lds, s, a, pt.triW);
   else
-    hipLaunchKernelGGL((k_row_inv<LOG2N, 4>), grid, block, lds, s, spec, tileStride, chanStride, st, g, pt.triW, corr,
-                       rw, cpg, npairs);
+    hipLaunchKernelGGL((k_row_inv<LOG2N, 4>), grid, block, lds, s, a, pt.triW);
 }
 
 void launch_row_inv(const float2* spec, int64_t tileStride, int64_t chanStride, const CallState* st, int cnMax,
@@ -1106,6 +1175,225 @@ void launch_row_inv(const float2* spec, int64_t tileStride, int64_t chanStride, 
     default: break;
   }
 #undef CVM_RI
+}
+
+
+// ------------------------------------------------- compact one-shot call --
+//
+// A small one-shot call runs in one stream, and its independent parts share
+// launches instead of streams: a graph with a second stream costs the host
+// two to three times as long to launch (most of a small call's time), while
+// workgroups of different jobs in one grid run side by side just the same.
+// Each launch gives its first workgroups to one job and the rest to the
+// other; every job runs exactly the code of its own launch.
+
+// One column launch's arguments (a band of tiles of one height).
+struct ColArgs {
+  const float2* src;
+  float2* dst;
+  int64_t srcTileStride, dstTileStride, chanStride;
+  const CallState* st;
+  TileGrid g;
+  const float2* tri;
+  const float2* tspec;
+  int64_t tchanStride;
+  float scale;
+  int t0, ntiles, nslabs;
+  int nwg;
+};
+
+template <int LOG2N, int MODE>
+__device__ __forceinline__ void cols_job(float2* a, int p, const ColArgs& C) {
+  cols_body<LOG2N, MODE>(a, xcd_remap(p, C.nwg), C.src, C.dst, C.srcTileStride, C.dstTileStride, C.chanStride, C.st,
+                         C.g, C.tri, C.tspec, C.tchanStride, C.scale, C.t0, C.ntiles, C.nslabs);
+}
+
+// The chunk column sums (a nbx x nch grid of 256-thread workgroups).
+struct ChunkArgs {
+  const uint8_t* img;
+  int64_t istride;
+  int iw, ih, step, words;
+  const CallState* st;
+  uint32_t* ps;
+  int nbx, nch;
+};
+
+// The chunk prefix sums (nwords words, one a thread).
+struct PrefixArgs {
+  int nwords, nch;
+  const CallState* st;
+  uint32_t* ps;
+  int nwg;
+};
+
+// The parent's and the template's forward row transforms (every channel)
+// and the call state's reset.
+template <int LOG2N, int NVEC>
+__global__ __launch_bounds__(RowNTF<LOG2N>::value) void k_front(RowFwdArgs P, RowFwdArgs T,
+                                                                const float2* __restrict__ triW, CallState* st) {
+  extern __shared__ float2 z[];
+  const int b = blockIdx.x;
+  if (b < P.nwg)
+    row_fwd_body<LOG2N, NVEC>(z, xcd_remap(b, P.nwg), P, triW);
+  else if (b < P.nwg + T.nwg)
+    row_fwd_body<LOG2N, NVEC>(z, b - P.nwg, T, triW);
+  else
+    state_clear(st);
+}
+
+// The template's column pass (its spectrum) and the call state's
+// independent parts (template statistics, alpha scans).
+template <int LOG2N>
+__global__ __launch_bounds__(256) void k_tspec_state(ColArgs C, CallStateArgs S) {
+  extern __shared__ float2 a[];
+  if ((int)blockIdx.x < C.nwg)
+    cols_job<LOG2N, COL_TSPEC>(a, blockIdx.x, C);
+  else
+    call_state_block(blockIdx.x - C.nwg, S);
+}
+
+// The correlation's column pass and the chunk column sums.
+template <int LOG2N>
+__global__ __launch_bounds__(256) void k_corr_chunks(ColArgs C, ChunkArgs K) {
+  extern __shared__ float2 a[];
+  if ((int)blockIdx.x < C.nwg) {
+    cols_job<LOG2N, COL_CORR>(a, blockIdx.x, C);
+  } else {
+    const int b = blockIdx.x - C.nwg;
+    colsum_chunks_at((b % K.nbx) * 256 + threadIdx.x, b / K.nbx, K.img, K.istride, K.iw, K.ih, K.step, K.words, K.st,
+                     K.ps);
+  }
+}
+
+// The inverse row pass and the chunk prefix sums.
+template <int LOG2N, int NVEC>
+__global__ __launch_bounds__(RowNT<LOG2N>::value) void k_rowinv_prefix(RowInvArgs R, PrefixArgs Q,
+                                                                        const float2* __restrict__ triW) {
+  extern __shared__ float2 z[];
+  if ((int)blockIdx.x < R.nwg)
+    row_inv_body<LOG2N, NVEC>(z, blockIdx.x, R, triW);
+  else
+    colsum_prefix_at((blockIdx.x - R.nwg) * blockDim.x + threadIdx.x, Q.nwords, Q.nch, Q.st, Q.ps);
+}
+
+template <int LOG2W, int NVEC>
+static void compact_rows(const CompactCall& c, const RowFwdArgs& P, const RowFwdArgs& T, const RowInvArgs& R,
+                         const PrefixArgs& Q, bool front, hipStream_t s) {
+  const size_t lds = sizeof(float2) * (size_t)padded_len(1 << LOG2W) * P.cpg;
+  if (front)
+    hipLaunchKernelGGL((k_front<LOG2W, NVEC>), dim3(P.nwg + T.nwg + 1), dim3(RowNTF<LOG2W>::value), lds, s, P, T,
+                       c.pt.triW, c.st);
+  else
+    hipLaunchKernelGGL((k_rowinv_prefix<LOG2W, NVEC>), dim3(R.nwg + Q.nwg), dim3(RowNT<LOG2W>::value), lds, s, R, Q,
+                       c.pt.triW);
+}
+
+template <int LOG2W>
+static void compact_rows_w(const CompactCall& c, bool front, const PrefixArgs& Q, hipStream_t s) {
+  const RowFwdArgs P = row_fwd_args<LOG2W>(c.img, c.istride, c.step, c.cnMax, nullptr, c.g, c.spec, c.tileStride,
+                                           c.chanStride, c.ntiles);
+  const RowFwdArgs T = row_fwd_args<LOG2W>(c.tpl, c.tstride, c.step, c.cnMax, nullptr, c.gt, c.tspec, 0, c.tchan, 1);
+  const RowInvArgs R = row_inv_args<LOG2W>(c.spec, c.tileStride, c.chanStride, c.st, c.cnMax, c.g, c.corr, c.rw,
+                                           c.ntiles);
+  if (P.cpg <= 1)
+    compact_rows<LOG2W, 1>(c, P, T, R, Q, front, s);
+  else if (P.cpg == 2)
+    compact_rows<LOG2W, 2>(c, P, T, R, Q, front, s);
+  else
+    compact_rows<LOG2W, 4>(c, P, T, R, Q, front, s);
+}
+
+static void compact_rows_dispatch(const CompactCall& c, bool front, const PrefixArgs& Q, hipStream_t s) {
+#define CVM_CR(L)                          \
+  case L:                                  \
+    compact_rows_w<L>(c, front, Q, s);     \
+    break
+  switch (c.g.log2W) {
+    CVM_CR(5); CVM_CR(6); CVM_CR(7); CVM_CR(8); CVM_CR(9); CVM_CR(10); CVM_CR(11);
+    default: break;
+  }
+#undef CVM_CR
+}
+
+template <int LOG2H>
+static void compact_cols(const CompactCall& c, const CallStateArgs& S, const ChunkArgs& K, bool tspec_state,
+                         hipStream_t s) {
+  constexpr int CG = ColCG<LOG2H>::value;
+  const size_t lds = sizeof(float2) * (size_t)(1 << LOG2H) * CG;
+  const int nslabs = (c.g.hw + CG - 1) / CG;
+  ColArgs C;
+  C.tri = c.pt.triH;
+  C.t0 = 0;
+  C.nslabs = nslabs;
+  C.chanStride = tspec_state ? c.tchan : c.chanStride;
+  if (tspec_state) {
+    // the template's spectrum, every channel: pad rows zero, forward, scaled
+    C.src = c.tspec;
+    C.dst = c.tspec;
+    C.srcTileStride = C.dstTileStride = 0;
+    C.st = nullptr;
+    C.g = c.gt;
+    C.tspec = nullptr;
+    C.tchanStride = 0;
+    C.scale = c.scale;
+    C.ntiles = 1;
+    C.nwg = nslabs * c.cnMax;
+    hipLaunchKernelGGL(k_tspec_state<LOG2H>, dim3(C.nwg + S.nb()), dim3(256), lds, s, C, S);
+  } else {
+    C.src = c.spec;
+    C.dst = c.spec;
+    C.srcTileStride = C.dstTileStride = c.tileStride;
+    C.st = c.st;
+    C.g = c.g;
+    C.tspec = c.tspec;
+    C.tchanStride = c.tchan;
+    C.scale = 1.0f;
+    C.ntiles = c.ntiles;
+    C.nwg = nslabs * c.ntiles * c.cnMax;
+    hipLaunchKernelGGL(k_corr_chunks<LOG2H>, dim3(C.nwg + K.nbx * K.nch), dim3(256), lds, s, C, K);
+  }
+}
+
+static void compact_cols_dispatch(const CompactCall& c, const CallStateArgs& S, const ChunkArgs& K, bool tspec_state,
+                                  hipStream_t s) {
+#define CVM_CC(L)                                      \
+  case L:                                              \
+    compact_cols<L>(c, S, K, tspec_state, s);          \
+    break
+  switch (c.g.log2H) {
+    CVM_CC(5); CVM_CC(6); CVM_CC(7); CVM_CC(8); CVM_CC(9); CVM_CC(10); CVM_CC(11);
+    default: break;
+  }
+#undef CVM_CC
+}
+
+bool compact_ok(const TileGrid& g) { return g.dftH2 == 0 && g.log2W >= 5 && g.log2W <= 11 && g.log2H >= 5 && g.log2H <= 11; }
+
+void launch_compact_fft(const CompactCall& c, hipStream_t s) {
+  const int nch = (c.ih + NB_ROWS - 1) / NB_ROWS;
+  ChunkArgs K;
+  K.img = c.img;
+  K.istride = c.istride;
+  K.iw = c.iw;
+  K.ih = c.ih;
+  K.step = c.step;
+  K.words = (c.step == 4 && ((((uintptr_t)c.img) | (uintptr_t)c.istride) & 3) == 0) ? 1 : 0;
+  K.st = c.st;
+  K.ps = (uint32_t*)c.nscratch;
+  K.nbx = (c.iw + 255) / 256;
+  K.nch = nch;
+  PrefixArgs Q;
+  Q.nwords = 3 * c.iw;
+  Q.nch = nch;
+  Q.st = c.st;
+  Q.ps = (uint32_t*)c.nscratch;
+  const int nt = c.g.log2W >= 10 ? 256 : 64;  // the inverse row launch's workgroup size
+  Q.nwg = (Q.nwords + nt - 1) / nt;
+  compact_rows_dispatch(c, true, Q, s);                   // parent + template rows, state reset
+  compact_cols_dispatch(c, c.csa, K, true, s);            // template columns + state statistics
+  launch_state_finalize(c.st, c.tw, c.th, c.cn_mode, s);  // channel count, flat test, scales
+  compact_cols_dispatch(c, c.csa, K, false, s);           // correlation columns + chunk sums
+  compact_rows_dispatch(c, false, Q, s);                  // inverse rows + chunk prefix sums
 }
 
 }  // namespace cvm

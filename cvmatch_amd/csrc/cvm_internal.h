@@ -59,6 +59,21 @@ struct CallState {
   int pad_;
 };
 
+// Geometry of one call-state launch (call_state_block): the template
+// statistics grid, the parent's alpha scan over rows [0, ph0), the
+// template's alpha scan; the scans only under the alpha rule.
+struct CallStateArgs {
+  const uint8_t* img;
+  int64_t istride;
+  int iw, ph0, ivec;
+  const uint8_t* tpl;
+  int64_t tstride;
+  int tw, th, tvec, cnMax, step;
+  int sgx, sgy, pgx, pgy, tgx, tgy;
+  CallState* st;
+  int nb() const { return sgx * sgy + pgx * pgy + tgx * tgy; }
+};
+
 // Exact template statistics of up to four channels (templStats impl.go:1063-1079).
 struct TemplStats {
   unsigned long long tsum[4];
@@ -181,6 +196,41 @@ void launch_state_finalize(CallState* st, int tw, int th, int cn_mode, hipStream
 // (cn_mode 0), then the finalisation.
 void launch_call_state(const uint8_t* img, int64_t istride, int iw, int ih, const uint8_t* tpl, int64_t tstride,
                        int tw, int th, int cnMax, int step, int cn_mode, CallState* st, hipStream_t s);
+// The geometry of launch_call_state's first launch; whole_parent: the
+// parent's alpha scan entirely in it (ph0 = ih), for a launch that takes
+// the state's independent parts beside other work.
+CallStateArgs call_state_args(const uint8_t* img, int64_t istride, int iw, int ih, const uint8_t* tpl,
+                              int64_t tstride, int tw, int th, int cnMax, int step, int cn_mode, CallState* st,
+                              bool whole_parent);
+
+// The compact form of a small one-shot call (one stream, independent jobs
+// sharing launches): the parent's and the template's row transforms with
+// the state's reset; the template's column pass with the call state's
+// statistics and alpha scans (csa, the whole parent scan in it); the state's
+// finalisation; the correlation's column pass with the chunk column sums;
+// the inverse rows with the chunk prefix sums.  The normalisation and the
+// reduction follow as usual.  Every job runs the code of its own launch.
+struct CompactCall {
+  const uint8_t* img;
+  int64_t istride;
+  int iw, ih, step, cnMax, cn_mode, tw, th, rw, ntiles;
+  TileGrid g, gt;  // the parent's tiles; the template as one tile
+  float2* spec;    // the parent's tile spectra
+  int64_t tileStride, chanStride;
+  const uint8_t* tpl;
+  int64_t tstride;
+  float2* tspec;  // the template spectrum, every channel
+  int64_t tchan;
+  float scale;  // 1 / (dftW dftH)
+  PlanTables pt;
+  CallState* st;
+  CallStateArgs csa;
+  float* corr;
+  void* nscratch;  // the chunk sums
+};
+// Plans the compact form takes: no shrunk band, transforms of 32 to 2048.
+bool compact_ok(const TileGrid& g);
+void launch_compact_fft(const CompactCall& c, hipStream_t s);
 
 // Exact-statistics normalisation + tail + per-workgroup extrema.
 // map may alias corr (in place) or be null (extrema only).

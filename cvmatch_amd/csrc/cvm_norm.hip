@@ -20,71 +20,15 @@
 
 #include "cvm_internal.h"
 #include "cvm_score.h"
+#include "cvm_state.h"
 
 namespace cvm {
 
 // ------------------------------------------------------------ call state --
 
-__global__ void k_state_init(CallState* st) {
-  if (threadIdx.x == 0) {
-    for (int k = 0; k < 4; k++) {
-      st->tsum[k] = 0;
-      st->tsq[k] = 0;
-    }
-    st->varSum = 0;
-    st->cn = 0;
-    st->flat = 0;
-    st->alpha_bad[0] = st->alpha_bad[1] = 0;
-    st->templNorm = st->numScale = st->varScale = 0.f;
-  }
-}
+__global__ void k_state_init(CallState* st) { state_clear(st); }
 
 void launch_state_init(CallState* st, hipStream_t s) { hipLaunchKernelGGL(k_state_init, dim3(1), dim3(64), 0, s, st); }
-
-__device__ __forceinline__ unsigned long long wave_sum_u64(unsigned long long v) {
-  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
-  return v;
-}
-
-// templStats impl.go:1063-1079 (sums and square sums per channel, exact).
-// grid.y walks template rows, threads walk columns; no divisions.
-// One workgroup (bx, by) of a gx x gy grid over the template.
-__device__ __forceinline__ void tpl_stats_block(const uint8_t* __restrict__ tpl, int64_t tstride, int tw, int th,
-                                                int cnMax, int step, CallState* st, int bx, int by, int gx, int gy) {
-  __shared__ unsigned long long red[4][8];
-  unsigned long long s[4] = {0, 0, 0, 0}, q[4] = {0, 0, 0, 0};
-  for (int y = by; y < th; y += gy) {
-    const uint8_t* row = tpl + (int64_t)y * tstride;
-    for (int x = bx * blockDim.x + threadIdx.x; x < tw; x += gx * blockDim.x) {
-      const uint8_t* p = row + (int64_t)x * step;
-      for (int k = 0; k < 4; k++)
-        if (k < cnMax) {
-          unsigned long long v = p[k];
-          s[k] += v;
-          q[k] += v * v;
-        }
-    }
-  }
-  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-  for (int k = 0; k < 4; k++) {
-    unsigned long long a = wave_sum_u64(s[k]), b = wave_sum_u64(q[k]);
-    if (lane == 0) {
-      red[k][w] = a;
-      red[k][4 + w] = b;
-    }
-  }
-  __syncthreads();
-  if (threadIdx.x < 4 && (int)threadIdx.x < cnMax) {
-    int k = threadIdx.x;
-    unsigned long long a = 0, b = 0;
-    for (int i = 0; i < (int)(blockDim.x >> 6); i++) {
-      a += red[k][i];
-      b += red[k][4 + i];
-    }
-    atomicAdd(&st->tsum[k], a);
-    atomicAdd(&st->tsq[k], b);
-  }
-}
 
 __global__ __launch_bounds__(256) void k_tpl_stats(const uint8_t* __restrict__ tpl, int64_t tstride, int tw, int th,
                                                    int cnMax, int step, CallState* st) {
@@ -97,36 +41,6 @@ void launch_tpl_stats(const uint8_t* tpl, int64_t tstride, int tw, int th, int c
   if (gx > 16) gx = 16;
   int gy = th < 256 ? th : 256;
   hipLaunchKernelGGL(k_tpl_stats, dim3(gx, gy), dim3(256), 0, s, tpl, tstride, tw, th, cnMax, step, st);
-}
-
-// alphaConst cvmatch.go:146-168: every alpha byte equals the first one.
-// Like the reference's scan it stops at the first mismatch: workgroups poll
-// the flag between rows and leave once any workgroup has published one (the
-// answer is then "not constant" whatever the rest of the image holds).
-// grid.y walks rows, grid.x x threads cover a row in 16-byte groups of four
-// pixels when rows are 16-byte aligned, else in single pixels.
-// One workgroup (bx, by) of a gx x gy grid over rows [y_begin, h).
-__device__ __forceinline__ void alpha_scan_block(const uint8_t* __restrict__ pix, int64_t stride, int w, int y_begin,
-                                                 int h, CallState* st, int which, int vec, int bx, int by, int gx,
-                                                 int gy) {
-  const uint32_t a0 = pix[3];
-  volatile unsigned int* flag = &st->alpha_bad[which];
-  int bad = 0;
-  const int w4 = vec ? (w >> 2) : 0;
-  const uint32_t want = a0 * 0x01010101u;
-  for (int y = y_begin + by; y < h; y += gy) {
-    if (*flag) break;
-    const uint8_t* row = pix + (int64_t)y * stride;
-    for (int q = bx * blockDim.x + threadIdx.x; q < w4; q += gx * blockDim.x) {
-      uint4 v = *(const uint4*)(row + (int64_t)q * 16);
-      uint32_t al = (v.x >> 24) | ((v.y >> 24) << 8) | ((v.z >> 24) << 16) | ((v.w >> 24) << 24);
-      if (al != want) bad = 1;
-    }
-    for (int x = 4 * w4 + bx * blockDim.x + threadIdx.x; x < w; x += gx * blockDim.x)
-      if (row[(int64_t)x * 4 + 3] != a0) bad = 1;
-    if (bad) break;
-  }
-  if (__syncthreads_or(bad) && threadIdx.x == 0) atomicOr(&st->alpha_bad[which], 1u);
 }
 
 __global__ __launch_bounds__(256) void k_alpha_scan(const uint8_t* __restrict__ pix, int64_t stride, int w, int y_begin,
@@ -174,49 +88,46 @@ void launch_state_finalize(CallState* st, int tw, int th, int cn_mode, hipStream
   hipLaunchKernelGGL(k_state_finalize, dim3(1), dim3(64), 0, s, st, tw, th, cn_mode);
 }
 
-// One launch for the independent parts of a call state: workgroups [0, nS)
-// take the template statistics (an sgx x sgy grid), the next pgx x pgy the
-// first ph0 rows of the parent's alpha scan (a varying alpha plane shows up
-// there, and the rest of the scan then leaves at its first flag check), the
-// last tgx x tgy the template's alpha scan.
-__global__ __launch_bounds__(256) void k_call_state(const uint8_t* __restrict__ img, int64_t istride, int iw, int ph0,
-                                                    int ivec, const uint8_t* __restrict__ tpl, int64_t tstride, int tw,
-                                                    int th, int tvec, int cnMax, int step, int sgx, int sgy, int pgx,
-                                                    int pgy, int tgx, int tgy, CallState* st) {
-  const int nS = sgx * sgy, nP = pgx * pgy;
-  int b = blockIdx.x;
-  if (b < nS) {
-    tpl_stats_block(tpl, tstride, tw, th, cnMax, step, st, b % sgx, b / sgx, sgx, sgy);
-  } else if (b < nS + nP) {
-    b -= nS;
-    alpha_scan_block(img, istride, iw, 0, ph0, st, 0, ivec, b % pgx, b / pgx, pgx, pgy);
-  } else {
-    b -= nS + nP;
-    alpha_scan_block(tpl, tstride, tw, 0, th, st, 1, tvec, b % tgx, b / tgx, tgx, tgy);
-  }
-}
+// One launch for the independent parts of a call state (call_state_block).
+__global__ __launch_bounds__(256) void k_call_state(CallStateArgs a) { call_state_block(blockIdx.x, a); }
 
 void launch_call_state(const uint8_t* img, int64_t istride, int iw, int ih, const uint8_t* tpl, int64_t tstride,
                        int tw, int th, int cnMax, int step, int cn_mode, CallState* st, hipStream_t s) {
-  const int sgx = std::min(16, (tw + 255) / 256), sgy = std::min(256, th);
-  int pgx = 0, pgy = 0, tgx = 0, tgy = 0, ivec = 0, tvec = 0, ph0 = 0;
-  if (cn_mode == 0) {
-    ivec = ((((uintptr_t)img) | (uintptr_t)istride) & 15) == 0;
-    tvec = ((((uintptr_t)tpl) | (uintptr_t)tstride) & 15) == 0;
-    ph0 = std::min(ih, 8);
-    pgx = std::min(16, ((ivec ? iw / 4 + 1 : iw) + 255) / 256);
-    pgy = ph0;
-    tgx = std::min(16, ((tvec ? tw / 4 + 1 : tw) + 255) / 256);
-    tgy = std::min(512, th);
-  }
-  const int nb = sgx * sgy + pgx * pgy + tgx * tgy;
-  hipLaunchKernelGGL(k_call_state, dim3(nb), dim3(256), 0, s, img, istride, iw, ph0, ivec, tpl, tstride, tw, th, tvec,
-                     cnMax, step, sgx, sgy, pgx, pgy, tgx, tgy, st);
-  if (cn_mode == 0 && ih > ph0) {
-    const int gy = std::min(512, ih - ph0);
-    hipLaunchKernelGGL(k_alpha_scan, dim3(pgx, gy), dim3(256), 0, s, img, istride, iw, ph0, ih, st, 0, ivec);
+  const CallStateArgs a = call_state_args(img, istride, iw, ih, tpl, tstride, tw, th, cnMax, step, cn_mode, st, false);
+  hipLaunchKernelGGL(k_call_state, dim3(a.nb()), dim3(256), 0, s, a);
+  if (cn_mode == 0 && ih > a.ph0) {
+    const int gy = std::min(512, ih - a.ph0);
+    hipLaunchKernelGGL(k_alpha_scan, dim3(a.pgx, gy), dim3(256), 0, s, img, istride, iw, a.ph0, ih, st, 0, a.ivec);
   }
   launch_state_finalize(st, tw, th, cn_mode, s);
+}
+
+CallStateArgs call_state_args(const uint8_t* img, int64_t istride, int iw, int ih, const uint8_t* tpl,
+                              int64_t tstride, int tw, int th, int cnMax, int step, int cn_mode, CallState* st,
+                              bool whole_parent) {
+  CallStateArgs a{};
+  a.img = img;
+  a.istride = istride;
+  a.iw = iw;
+  a.tpl = tpl;
+  a.tstride = tstride;
+  a.tw = tw;
+  a.th = th;
+  a.cnMax = cnMax;
+  a.step = step;
+  a.st = st;
+  a.sgx = std::min(16, (tw + 255) / 256);
+  a.sgy = std::min(256, th);
+  if (cn_mode == 0) {
+    a.ivec = ((((uintptr_t)img) | (uintptr_t)istride) & 15) == 0;
+    a.tvec = ((((uintptr_t)tpl) | (uintptr_t)tstride) & 15) == 0;
+    a.ph0 = whole_parent ? ih : std::min(ih, 8);
+    a.pgx = std::min(16, ((a.ivec ? iw / 4 + 1 : iw) + 255) / 256);
+    a.pgy = std::min(512, a.ph0);
+    a.tgx = std::min(16, ((a.tvec ? tw / 4 + 1 : tw) + 255) / 256);
+    a.tgy = std::min(512, th);
+  }
+  return a;
 }
 
 // rgbToGray cvmatch.go:222-240: (4899 R + 9617 G + 1868 B + 8192) >> 14.
@@ -279,7 +190,6 @@ static int norm_band_rows(int rh) {
 // squared window sums and area * s2 are then evaluated in binary64 with no
 // rounding at all, exactly the integers the int64 form computes.  Larger
 // templates take the 256-thread int64 form (k_normalize).
-constexpr int NB_ROWS = 16;
 
 struct NormPlan {
   int wave;  // 0: int64 form
@@ -569,66 +479,15 @@ __global__ __launch_bounds__(256) void k_normalize(const uint8_t* __restrict__ i
 // Horizontal window sums are differences of a wave-level prefix scan in LDS.
 
 
-// Partial column sums of chunk c (rows [c*NB_ROWS, c*NB_ROWS + NB_ROWS)),
-// stored at row c + 1 of the scratch (row 0 is the empty prefix),
-// three words per image column: ch0 | ch2 << 16, ch1 | ch3 << 16 (each at
-// most NB_ROWS * 255) and the sum of squares over the first cn channels.
 __global__ __launch_bounds__(256) void k_colsum_chunks(const uint8_t* __restrict__ img, int64_t istride, int iw,
                                                        int ih, int step, int words, const CallState* __restrict__ st,
                                                        uint32_t* __restrict__ ps) {
-  const int x = blockIdx.x * blockDim.x + threadIdx.x, c = blockIdx.y;
-  if (x >= iw || st->flat) return;
-  const int cn = st->cn;
-  const uint32_t m = chan_mask(cn);
-  const int y0 = c * NB_ROWS, y1 = min(ih, y0 + NB_ROWS);
-  uint32_t s02 = 0, s13 = 0, q = 0;
-  for (int y = y0; y < y1; y++) {
-    const uint8_t* p = img + (int64_t)y * istride + (int64_t)x * step;
-    uint32_t w;
-    if (words) {
-      w = *(const uint32_t*)p & m;
-    } else {
-      w = 0;
-      for (int k = 0; k < 4; k++)
-        if (k < cn) w |= (uint32_t)p[k] << (8 * k);
-    }
-    s02 += w & 0x00FF00FFu;
-    s13 += (w >> 8) & 0x00FF00FFu;
-    q = __builtin_amdgcn_udot4(w, w, q, false);
-  }
-  uint32_t* o = ps + (int64_t)(c + 1) * 3 * iw + x;
-  o[0] = s02;
-  o[iw] = s13;
-  o[2 * iw] = q;
+  colsum_chunks_at(blockIdx.x * blockDim.x + threadIdx.x, blockIdx.y, img, istride, iw, ih, step, words, st, ps);
 }
 
-// Running sums of the chunk partials down each column, modulo 2^32: row c
-// becomes the sum of chunks [0, c).  A window of whole chunks is then the
-// difference of two rows; the difference is exact because every window sum
-// fits its field (each packed 16-bit channel half <= 255 * 257, the square
-// sum < 2^32), so the wrap of the running sums cancels.
 __global__ __launch_bounds__(256) void k_colsum_prefix(int nwords, int nch, const CallState* __restrict__ st,
                                                        uint32_t* __restrict__ ps) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= nwords || st->flat) return;
-  uint32_t* p = ps + i;
-  p[0] = 0u;
-  uint32_t acc = 0u;
-  int c = 1;
-  for (; c + 8 <= nch + 1; c += 8) {
-    uint32_t v[8];
-#pragma unroll
-    for (int k = 0; k < 8; k++) v[k] = p[(int64_t)(c + k) * nwords];
-#pragma unroll
-    for (int k = 0; k < 8; k++) {
-      acc += v[k];
-      p[(int64_t)(c + k) * nwords] = acc;
-    }
-  }
-  for (; c <= nch; c++) {
-    acc += p[(int64_t)c * nwords];
-    p[(int64_t)c * nwords] = acc;
-  }
+  colsum_prefix_at(blockIdx.x * blockDim.x + threadIdx.x, nwords, nch, st, ps);
 }
 
 // CPT (2, 4 or 8) consecutive pixels x0.. of one row as packed channel

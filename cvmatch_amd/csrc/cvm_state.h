@@ -1,0 +1,189 @@
+// Device bodies of the call-state and column-sum kernels, shared by their own
+// launches (cvm_norm.hip) and by the launches that run them beside other
+// work (cvm_fft.hip): the template statistics, the alpha scans, the state's
+// reset, and the chunk column sums of the wave-form normalisation.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "cvm_internal.h"
+#include "cvm_score.h"
+
+namespace cvm {
+
+// Rows of one chunk of the column sums (and of one normalisation band).
+constexpr int NB_ROWS = 16;
+
+// The state's accumulators and results cleared (thread 0).
+__device__ __forceinline__ void state_clear(CallState* st) {
+  if (threadIdx.x == 0) {
+    for (int k = 0; k < 4; k++) {
+      st->tsum[k] = 0;
+      st->tsq[k] = 0;
+    }
+    st->varSum = 0;
+    st->cn = 0;
+    st->flat = 0;
+    st->alpha_bad[0] = st->alpha_bad[1] = 0;
+    st->templNorm = st->numScale = st->varScale = 0.f;
+  }
+}
+
+__device__ __forceinline__ unsigned long long wave_sum_u64(unsigned long long v) {
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+
+// templStats impl.go:1063-1079 (sums and square sums per channel, exact).
+// grid.y walks template rows, threads walk columns; no divisions.
+// One workgroup (bx, by) of a gx x gy grid over the template.
+__device__ __forceinline__ void tpl_stats_block(const uint8_t* __restrict__ tpl, int64_t tstride, int tw, int th,
+                                                int cnMax, int step, CallState* st, int bx, int by, int gx, int gy) {
+  __shared__ unsigned long long red[4][8];
+  unsigned long long s[4] = {0, 0, 0, 0}, q[4] = {0, 0, 0, 0};
+  for (int y = by; y < th; y += gy) {
+    const uint8_t* row = tpl + (int64_t)y * tstride;
+    for (int x = bx * blockDim.x + threadIdx.x; x < tw; x += gx * blockDim.x) {
+      const uint8_t* p = row + (int64_t)x * step;
+      for (int k = 0; k < 4; k++)
+        if (k < cnMax) {
+          unsigned long long v = p[k];
+          s[k] += v;
+          q[k] += v * v;
+        }
+    }
+  }
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  for (int k = 0; k < 4; k++) {
+    unsigned long long a = wave_sum_u64(s[k]), b = wave_sum_u64(q[k]);
+    if (lane == 0) {
+      red[k][w] = a;
+      red[k][4 + w] = b;
+    }
+  }
+  __syncthreads();
+  if (threadIdx.x < 4 && (int)threadIdx.x < cnMax) {
+    int k = threadIdx.x;
+    unsigned long long a = 0, b = 0;
+    for (int i = 0; i < (int)(blockDim.x >> 6); i++) {
+      a += red[k][i];
+      b += red[k][4 + i];
+    }
+    atomicAdd(&st->tsum[k], a);
+    atomicAdd(&st->tsq[k], b);
+  }
+}
+
+// alphaConst cvmatch.go:146-168: every alpha byte equals the first one.
+// Like the reference's scan it stops at the first mismatch: workgroups poll
+// the flag between rows and leave once any workgroup has published one (the
+// answer is then "not constant" whatever the rest of the image holds).
+// grid.y walks rows, grid.x x threads cover a row in 16-byte groups of four
+// pixels when rows are 16-byte aligned, else in single pixels.
+// One workgroup (bx, by) of a gx x gy grid over rows [y_begin, h).
+__device__ __forceinline__ void alpha_scan_block(const uint8_t* __restrict__ pix, int64_t stride, int w, int y_begin,
+                                                 int h, CallState* st, int which, int vec, int bx, int by, int gx,
+                                                 int gy) {
+  const uint32_t a0 = pix[3];
+  volatile unsigned int* flag = &st->alpha_bad[which];
+  int bad = 0;
+  const int w4 = vec ? (w >> 2) : 0;
+  const uint32_t want = a0 * 0x01010101u;
+  for (int y = y_begin + by; y < h; y += gy) {
+    if (*flag) break;
+    const uint8_t* row = pix + (int64_t)y * stride;
+    for (int q = bx * blockDim.x + threadIdx.x; q < w4; q += gx * blockDim.x) {
+      uint4 v = *(const uint4*)(row + (int64_t)q * 16);
+      uint32_t al = (v.x >> 24) | ((v.y >> 24) << 8) | ((v.z >> 24) << 16) | ((v.w >> 24) << 24);
+      if (al != want) bad = 1;
+    }
+    for (int x = 4 * w4 + bx * blockDim.x + threadIdx.x; x < w; x += gx * blockDim.x)
+      if (row[(int64_t)x * 4 + 3] != a0) bad = 1;
+    if (bad) break;
+  }
+  if (__syncthreads_or(bad) && threadIdx.x == 0) atomicOr(&st->alpha_bad[which], 1u);
+}
+
+// The independent parts of a call state in one launch: workgroups [0, nS)
+// take the template statistics (an sgx x sgy grid), the next pgx x pgy the
+// first ph0 rows of the parent's alpha scan (a varying alpha plane shows up
+// there, and the rest of the scan then leaves at its first flag check; with
+// ph0 = ih the whole scan), the last tgx x tgy the template's alpha scan.
+// b is the workgroup's index among them (256 threads).
+__device__ __forceinline__ void call_state_block(int b, const CallStateArgs& a) {
+  const int nS = a.sgx * a.sgy, nP = a.pgx * a.pgy;
+  if (b < nS) {
+    tpl_stats_block(a.tpl, a.tstride, a.tw, a.th, a.cnMax, a.step, a.st, b % a.sgx, b / a.sgx, a.sgx, a.sgy);
+  } else if (b < nS + nP) {
+    b -= nS;
+    alpha_scan_block(a.img, a.istride, a.iw, 0, a.ph0, a.st, 0, a.ivec, b % a.pgx, b / a.pgx, a.pgx, a.pgy);
+  } else {
+    b -= nS + nP;
+    alpha_scan_block(a.tpl, a.tstride, a.tw, 0, a.th, a.st, 1, a.tvec, b % a.tgx, b / a.tgx, a.tgx, a.tgy);
+  }
+}
+
+// Partial column sums of chunk c (rows [c*NB_ROWS, c*NB_ROWS + NB_ROWS)),
+// stored at row c + 1 of the scratch (row 0 is the empty prefix),
+// three words per image column: ch0 | ch2 << 16, ch1 | ch3 << 16 (each at
+// most NB_ROWS * 255) and the sum of squares over the first cn channels.
+// Column x of chunk c.
+__device__ __forceinline__ void colsum_chunks_at(int x, int c, const uint8_t* __restrict__ img, int64_t istride,
+                                                 int iw, int ih, int step, int words,
+                                                 const CallState* __restrict__ st, uint32_t* __restrict__ ps) {
+  if (x >= iw || st->flat) return;
+  const int cn = st->cn;
+  const uint32_t m = chan_mask(cn);
+  const int y0 = c * NB_ROWS, y1 = min(ih, y0 + NB_ROWS);
+  uint32_t s02 = 0, s13 = 0, q = 0;
+  for (int y = y0; y < y1; y++) {
+    const uint8_t* p = img + (int64_t)y * istride + (int64_t)x * step;
+    uint32_t w;
+    if (words) {
+      w = *(const uint32_t*)p & m;
+    } else {
+      w = 0;
+      for (int k = 0; k < 4; k++)
+        if (k < cn) w |= (uint32_t)p[k] << (8 * k);
+    }
+    s02 += w & 0x00FF00FFu;
+    s13 += (w >> 8) & 0x00FF00FFu;
+    q = __builtin_amdgcn_udot4(w, w, q, false);
+  }
+  uint32_t* o = ps + (int64_t)(c + 1) * 3 * iw + x;
+  o[0] = s02;
+  o[iw] = s13;
+  o[2 * iw] = q;
+}
+
+// Running sums of the chunk partials down each column, modulo 2^32: row c
+// becomes the sum of chunks [0, c).  A window of whole chunks is then the
+// difference of two rows; the difference is exact because every window sum
+// fits its field (each packed 16-bit channel half <= 255 * 257, the square
+// sum < 2^32), so the wrap of the running sums cancels.
+// Word i of the scratch rows.
+__device__ __forceinline__ void colsum_prefix_at(int i, int nwords, int nch, const CallState* __restrict__ st,
+                                                 uint32_t* __restrict__ ps) {
+  if (i >= nwords || st->flat) return;
+  uint32_t* p = ps + i;
+  p[0] = 0u;
+  uint32_t acc = 0u;
+  int c = 1;
+  for (; c + 8 <= nch + 1; c += 8) {
+    uint32_t v[8];
+#pragma unroll
+    for (int k = 0; k < 8; k++) v[k] = p[(int64_t)(c + k) * nwords];
+#pragma unroll
+    for (int k = 0; k < 8; k++) {
+      acc += v[k];
+      p[(int64_t)(c + k) * nwords] = acc;
+    }
+  }
+  for (; c <= nch; c++) {
+    acc += p[(int64_t)c * nwords];
+    p[(int64_t)c * nwords] = acc;
+  }
+}
+
+}  // namespace cvm

@@ -1252,15 +1252,39 @@ __global__ __launch_bounds__(256) void k_tspec_state(ColArgs C, CallStateArgs S)
     call_state_block(blockIdx.x - C.nwg, S);
 }
 
-// The correlation's column pass and the chunk column sums.
+// The state's finalisation as one launch's prologue: each workgroup
+// finalises a copy of the accumulated state in LDS and works from it, and
+// workgroup 0 publishes the result fields for the later launches.
+struct FinalizeArgs {
+  CallState* st;
+  int tw, th, cn_mode;
+};
+
+// The correlation's column pass and the chunk column sums, after the
+// state's finalisation.
 template <int LOG2N>
-__global__ __launch_bounds__(256) void k_corr_chunks(ColArgs C, ChunkArgs K) {
+__global__ __launch_bounds__(256) void k_corr_chunks(ColArgs C, ChunkArgs K, FinalizeArgs F) {
   extern __shared__ float2 a[];
+  __shared__ CallState sst;
+  if (threadIdx.x == 0) {
+    sst = *F.st;
+    state_finalize_fields(&sst, F.tw, F.th, F.cn_mode);
+    if (blockIdx.x == 0) {
+      F.st->cn = sst.cn;
+      F.st->varSum = sst.varSum;
+      F.st->flat = sst.flat;
+      F.st->templNorm = sst.templNorm;
+      F.st->varScale = sst.varScale;
+      F.st->numScale = sst.numScale;
+    }
+  }
+  __syncthreads();
   if ((int)blockIdx.x < C.nwg) {
+    C.st = &sst;
     cols_job<LOG2N, COL_CORR>(a, blockIdx.x, C);
   } else {
     const int b = blockIdx.x - C.nwg;
-    colsum_chunks_at((b % K.nbx) * 256 + threadIdx.x, b / K.nbx, K.img, K.istride, K.iw, K.ih, K.step, K.words, K.st,
+    colsum_chunks_at((b % K.nbx) * 256 + threadIdx.x, b / K.nbx, K.img, K.istride, K.iw, K.ih, K.step, K.words, &sst,
                      K.ps);
   }
 }
@@ -1318,6 +1342,7 @@ static void compact_rows_dispatch(const CompactCall& c, bool front, const Prefix
 template <int LOG2H>
 static void compact_cols(const CompactCall& c, const CallStateArgs& S, const ChunkArgs& K, bool tspec_state,
                          hipStream_t s) {
+  const FinalizeArgs F{c.st, c.tw, c.th, c.cn_mode};
   constexpr int CG = ColCG<LOG2H>::value;
   const size_t lds = sizeof(float2) * (size_t)(1 << LOG2H) * CG;
   const int nslabs = (c.g.hw + CG - 1) / CG;
@@ -1350,7 +1375,7 @@ static void compact_cols(const CompactCall& c, const CallStateArgs& S, const Chu
     C.scale = 1.0f;
     C.ntiles = c.ntiles;
     C.nwg = nslabs * c.ntiles * c.cnMax;
-    hipLaunchKernelGGL(k_corr_chunks<LOG2H>, dim3(C.nwg + K.nbx * K.nch), dim3(256), lds, s, C, K);
+    hipLaunchKernelGGL(k_corr_chunks<LOG2H>, dim3(C.nwg + K.nbx * K.nch), dim3(256), lds, s, C, K, F);
   }
 }
 
@@ -1389,11 +1414,10 @@ void launch_compact_fft(const CompactCall& c, hipStream_t s) {
   Q.ps = (uint32_t*)c.nscratch;
   const int nt = c.g.log2W >= 10 ? 256 : 64;  // the inverse row launch's workgroup size
   Q.nwg = (Q.nwords + nt - 1) / nt;
-  compact_rows_dispatch(c, true, Q, s);                   // parent + template rows, state reset
-  compact_cols_dispatch(c, c.csa, K, true, s);            // template columns + state statistics
-  launch_state_finalize(c.st, c.tw, c.th, c.cn_mode, s);  // channel count, flat test, scales
-  compact_cols_dispatch(c, c.csa, K, false, s);           // correlation columns + chunk sums
-  compact_rows_dispatch(c, false, Q, s);                  // inverse rows + chunk prefix sums
+  compact_rows_dispatch(c, true, Q, s);         // parent + template rows, state reset
+  compact_cols_dispatch(c, c.csa, K, true, s);  // template columns + state statistics
+  compact_cols_dispatch(c, c.csa, K, false, s);  // state finalised; correlation columns + chunk sums
+  compact_rows_dispatch(c, false, Q, s);        // inverse rows + chunk prefix sums
 }
 
 }  // namespace cvm

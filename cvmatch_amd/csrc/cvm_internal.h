@@ -206,9 +206,9 @@ CallStateArgs call_state_args(const uint8_t* img, int64_t istride, int iw, int i
 // The compact form of a small one-shot call (one stream, independent jobs
 // sharing launches): the parent's and the template's row transforms with
 // the state's reset; the template's column pass with the call state's
-// statistics and alpha scans (csa, the whole parent scan in it); the state's
-// finalisation; the correlation's column pass with the chunk column sums;
-// the inverse rows with the chunk prefix sums.  The normalisation and the
+// statistics and alpha scans (csa, the whole parent scan in it); the
+// correlation's column pass with the chunk column sums, each workgroup
+// finalising the state first; the inverse rows with the chunk prefix sums.  The normalisation and the
 // reduction follow as usual.  Every job runs the code of its own launch.
 struct CompactCall {
   const uint8_t* img;

@@ -63,25 +63,8 @@ void launch_alpha_scan(const uint8_t* pix, int64_t stride, int w, int h, CallSta
   }
 }
 
-// cn choice (Match cvmatch.go:82-85), varSum, flat test (impl.go:1137),
-// templNorm (impl.go:1087), varScale / numScale (impl.go:862-870).
 __global__ void k_state_finalize(CallState* st, int tw, int th, int cn_mode) {
-  if (threadIdx.x != 0) return;
-  int cn = cn_mode;
-  if (cn_mode == 0) cn = (st->alpha_bad[0] == 0 && st->alpha_bad[1] == 0) ? 3 : 4;
-  long long area = (long long)tw * th;
-  long long var = 0;
-  for (int k = 0; k < cn; k++) {
-    long long s = (long long)st->tsum[k], q = (long long)st->tsq[k];
-    var += area * q - s * s;
-  }
-  st->cn = cn;
-  st->varSum = var;
-  st->flat = var == 0;
-  double dtw = (double)tw, dth = (double)th;
-  st->templNorm = (float)sqrt((double)var / (dtw * dth));
-  st->varScale = (float)(1.0 / (dtw * dth));
-  st->numScale = cn == 1 ? (float)((double)(long long)st->tsum[0] / (dtw * dth)) : st->varScale;
+  if (threadIdx.x == 0) state_finalize_fields(st, tw, th, cn_mode);
 }
 
 void launch_state_finalize(CallState* st, int tw, int th, int cn_mode, hipStream_t s) {

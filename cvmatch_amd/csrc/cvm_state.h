@@ -30,6 +30,28 @@ __device__ __forceinline__ void state_clear(CallState* st) {
   }
 }
 
+// cn choice (Match cvmatch.go:82-85), varSum, flat test (impl.go:1137),
+// templNorm (impl.go:1087), varScale / numScale (impl.go:862-870), from the
+// state's accumulated statistics and alpha flags into its result fields
+// (one thread).
+__device__ __forceinline__ void state_finalize_fields(CallState* st, int tw, int th, int cn_mode) {
+  int cn = cn_mode;
+  if (cn_mode == 0) cn = (st->alpha_bad[0] == 0 && st->alpha_bad[1] == 0) ? 3 : 4;
+  long long area = (long long)tw * th;
+  long long var = 0;
+  for (int k = 0; k < cn; k++) {
+    long long s = (long long)st->tsum[k], q = (long long)st->tsq[k];
+    var += area * q - s * s;
+  }
+  st->cn = cn;
+  st->varSum = var;
+  st->flat = var == 0;
+  double dtw = (double)tw, dth = (double)th;
+  st->templNorm = (float)sqrt((double)var / (dtw * dth));
+  st->varScale = (float)(1.0 / (dtw * dth));
+  st->numScale = cn == 1 ? (float)((double)(long long)st->tsum[0] / (dtw * dth)) : st->varScale;
+}
+
 __device__ __forceinline__ unsigned long long wave_sum_u64(unsigned long long v) {
   for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
   return v;

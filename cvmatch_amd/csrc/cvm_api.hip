@@ -663,25 +663,27 @@ static Res6* res_out(Ctx& X) { return X.rmap_d ? X.rmap_d : X.res_d; }
 // template spectrum is and X.join2 once the normalisation partials are.  It
 // runs after the parent's row pass has been submitted on s, so that pass
 // (which needs none of it) is first in line and overlaps the side work.
-// Partials of the widest range of strip widths a call's normalisation may take.
+// Partials of the widest range of strip forms a call's normalisation may take.
 static int norm_partials_max(int rw, int rh, int tw, int th, int cnMax) {
   int n = norm_partials_count(rw, rh, tw, th, cnMax);
-  for (int q : {4, 2, 1})
+  for (int q : {4, 2, 1, 4 | NORM_HALF, 2 | NORM_HALF, 1 | NORM_HALF})
     if (norm_q_ok(tw, th, cnMax, q)) n = std::max(n, norm_partials_count(rw, rh, tw, th, cnMax, q));
   return n;
 }
 
-// Strip width (result columns per lane) of a call's wave-form normalisation:
-// each width that holds the template is timed on the call's own correlation
-// -- extrema only, so the map is left alone -- the first time the shape comes
-// up (a small map fills the chip only with narrow strips).  0 (the default
-// width) for the int64 form and for the allocation-free call.
+// Strip form of a call's wave-form normalisation, the strip width (result
+// columns per lane) in full or half bands: each form that holds the template
+// is timed on the call's own correlation -- extrema only, so the map is left
+// alone -- the first time the shape comes up (a small map fills the chip only
+// with narrow strips, and its waves are latency-bound row chains that half
+// bands shorten).  0 (the default form) for the int64 form and for the
+// allocation-free call.
 static int norm_q_for(Ctx& X, hipStream_t s, const uint8_t* d_img, int64_t istride, int iw, int ih, int step, int tw,
                       int th, int cnMax, const CallState* st, const float* corr, Extrema* parts, void* nscratch,
                       bool exact) {
   if (X.noprof || !norm_wave_ok(tw, th, cnMax)) return 0;
   std::vector<int> cands;
-  for (int q : {4, 2, 1})
+  for (int q : {4, 2, 1, 4 | NORM_HALF, 2 | NORM_HALF, 1 | NORM_HALF})
     if (norm_q_ok(tw, th, cnMax, q)) cands.push_back(q);
   if (cands.size() < 2) return cands.empty() ? 0 : cands[0];
   const int rw = iw - tw + 1, rh = ih - th + 1;
@@ -811,14 +813,11 @@ static int alpha_rule_cn(Ctx& X, hipStream_t s, const uint8_t* d_img, int64_t is
 
 // Parents of at most this many pixels take the compact form of a one-shot
 // call (launch_compact_fft); CVMATCH_COMPACT=<pixels> moves the bound, 0
-// turns the form off.
+// turns the form off (read at every call).
 constexpr long long kCompactPixels = 2500000;
 static long long compact_max_pixels() {
-  static const long long v = [] {
-    const char* e = getenv("CVMATCH_COMPACT");
-    return e ? atoll(e) : kCompactPixels;
-  }();
-  return v;
+  const char* e = getenv("CVMATCH_COMPACT");
+  return e ? atoll(e) : kCompactPixels;
 }
 
 // A small one-shot call in one stream: the transforms, the call state and

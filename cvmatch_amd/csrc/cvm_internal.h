@@ -237,6 +237,8 @@ void launch_compact_fft(const CompactCall& c, hipStream_t s);
 // cnMax: the largest channel count the call can take (the device decides).
 // qpref (0, or 4, 2, 1 result columns per lane of the wave form's strips)
 // selects the strip width; a width that cannot hold the template is ignored.
+// qpref | NORM_HALF: the same strips in half bands (NB_ROWS / 2 rows a wave).
+constexpr int NORM_HALF = 8;
 int norm_partials_count(int rw, int rh, int tw, int th, int cnMax, int qpref = 0);
 bool norm_q_ok(int tw, int th, int cnMax, int q);
 // bytes of scratch launch_normalize needs (chunk partial column sums)

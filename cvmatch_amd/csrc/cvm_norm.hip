@@ -179,6 +179,7 @@ struct NormPlan {
   int q;     // result columns per lane (the strip is 64*q columns)
   int sq32;  // window square sums fit uint32
   int cpt;   // image columns per lane: the fewest of 2, 4, 8 that cover the strip
+  int band;  // result rows per wave: NB_ROWS, or NB_ROWS / 2 (a half band)
 };
 
 // Each lane of the wave form owns CPT consecutive image columns of the strip
@@ -194,9 +195,13 @@ __host__ __device__ constexpr int norm_nwp(int cpt) { return 64 * cpt + 1 + (64 
 constexpr int NORM_NWP = norm_nwp(NORM_CPT);
 
 // qpref: 4, 2 or 1 result columns per lane when that strip holds the
-// template (64 q + tw - 1 <= 512 columns), else (or 0) the widest that does.
+// template (64 q + tw - 1 <= 512 columns), else (or 0) the widest that does;
+// plus NORM_HALF for half bands (twice the waves, each walking half the
+// rows: a small map's waves are latency-bound row chains).
 static NormPlan norm_plan(int tw, int th, int cnMax, int qpref = 0) {
-  NormPlan p{0, 0, 0, 0};
+  NormPlan p{0, 0, 0, 0, NB_ROWS};
+  if (qpref & NORM_HALF) p.band = NB_ROWS / 2;
+  qpref &= NORM_HALF - 1;
   double area = (double)tw * th;
   if ((double)cnMax * 65025.0 * area * area >= 9007199254740992.0) return p;
   if (th > 257) return p;  // packed 16-bit channel column sums: 255 * th < 2^16
@@ -222,12 +227,12 @@ bool norm_wave_ok(int tw, int th, int cnMax) { return norm_plan(tw, th, cnMax).w
 
 bool norm_q_ok(int tw, int th, int cnMax, int q) {
   const NormPlan np = norm_plan(tw, th, cnMax, q);
-  return np.wave && np.q == q;
+  return np.wave && np.q == (q & (NORM_HALF - 1));
 }
 
 int norm_partials_count(int rw, int rh, int tw, int th, int cnMax, int qpref) {
   NormPlan np = norm_plan(tw, th, cnMax, qpref);
-  if (np.wave) return ((rw + 64 * np.q - 1) / (64 * np.q)) * ((rh + NB_ROWS - 1) / NB_ROWS);
+  if (np.wave) return ((rw + 64 * np.q - 1) / (64 * np.q)) * ((rh + np.band - 1) / np.band);
   int br = norm_band_rows(rh);
   return ((rw + NORM_NX - 1) / NORM_NX) * ((rh + br - 1) / br);
 }
@@ -573,11 +578,14 @@ __device__ __forceinline__ void norm_wave_block(const uint8_t* __restrict__ img,
                                                 int step, int tw, int th, int rw, int rh,
                                                 const CallState* __restrict__ st, const uint32_t* __restrict__ ps,
                                                 const float* corr, float* map, Extrema* __restrict__ parts, int bx,
-                                                int by, int gx, uint32_t (*P)[norm_nwp(CPT)], sq_t* P2) {
+                                                int by, int gx, uint32_t (*P)[norm_nwp(CPT)], sq_t* P2,
+                                                int band) {
   constexpr int SX = 64 * Q;
   const int lane = threadIdx.x;
-  const int xs = bx * SX, ys = by * NB_ROWS;
-  const int ye = min(rh, ys + NB_ROWS);
+  const int xs = bx * SX, ys = by * band;
+  const int ye = min(rh, ys + band);
+  // the window sums start at the chunk boundary ys0 <= ys and slide to ys
+  const int ys0 = ys - ys % NB_ROWS, by0 = ys0 / NB_ROWS;
   float bmin = FLT_MAX, bmax = -FLT_MAX;
   int bmin_i = INT_MAX, bmax_i = INT_MAX;
 
@@ -630,11 +638,11 @@ __device__ __forceinline__ void norm_wave_block(const uint8_t* __restrict__ img,
       c2[j] = 0;
     }
     if (nload > 0) {
-      // whole chunks [by, by + nfull): prefix row by + nfull minus row by
-      // (ys + nfull * NB_ROWS <= ys + th <= ih, so both rows exist)
+      // whole chunks [by0, by0 + nfull): prefix row by0 + nfull minus row
+      // by0 (ys0 + nfull * NB_ROWS <= ys0 + th <= ih, so both rows exist)
       const int nfull = th / NB_ROWS;
-      const uint32_t* p0 = ps + (int64_t)by * 3 * iw + x0;
-      const uint32_t* p1 = ps + (int64_t)(by + nfull) * 3 * iw + x0;
+      const uint32_t* p0 = ps + (int64_t)by0 * 3 * iw + x0;
+      const uint32_t* p1 = ps + (int64_t)(by0 + nfull) * 3 * iw + x0;
       uint32_t a0[CPT], a1[CPT], a2[CPT], b0[CPT], b1[CPT], b2[CPT];
       load_u32xn<VMODE != 0, CPT>(p0, nload, a0);
       load_u32xn<VMODE != 0, CPT>(p0 + iw, nload, a1);
@@ -648,7 +656,7 @@ __device__ __forceinline__ void norm_wave_block(const uint8_t* __restrict__ img,
         cs13[j] = b1[j] - a1[j];
         c2[j] = (sq_t)(uint32_t)(b2[j] - a2[j]);
       }
-      for (int y = ys + nfull * NB_ROWS; y < ys + th; y++) {
+      for (int y = ys0 + nfull * NB_ROWS; y < ys0 + th; y++) {
         uint32_t w[CPT];
         load_pxn<VMODE, CPT>(img + (int64_t)y * istride, x0, step, cn, nload, w);
 #pragma unroll
@@ -657,6 +665,20 @@ __device__ __forceinline__ void norm_wave_block(const uint8_t* __restrict__ img,
           cs02[j] += v & 0x00FF00FFu;
           cs13[j] += (v >> 8) & 0x00FF00FFu;
           c2[j] += (sq_t)__builtin_amdgcn_udot4(v, v, 0u, false);
+        }
+      }
+      // a half band's start: the window slides from rows [ys0, ys0 + th) to
+      // [ys, ys + th) (y + th < ys + th <= ih: every row exists)
+      for (int y = ys0; y < ys; y++) {
+        uint32_t a[CPT], b[CPT];
+        load_pxn<VMODE, CPT>(img + (int64_t)(y + th) * istride, x0, step, cn, nload, a);
+        load_pxn<VMODE, CPT>(img + (int64_t)y * istride, x0, step, cn, nload, b);
+#pragma unroll
+        for (int j = 0; j < CPT; j++) {
+          const uint32_t va = a[j] & cm, vb = b[j] & cm;
+          cs02[j] = cs02[j] - (vb & 0x00FF00FFu) + (va & 0x00FF00FFu);
+          cs13[j] = cs13[j] - ((vb >> 8) & 0x00FF00FFu) + ((va >> 8) & 0x00FF00FFu);
+          c2[j] += (sq_t)__builtin_amdgcn_udot4(va, va, 0u, false) - (sq_t)__builtin_amdgcn_udot4(vb, vb, 0u, false);
         }
       }
     }
@@ -832,14 +854,15 @@ template <int Q, bool SQ32, int VMODE, bool EXACT, int CPT>
 __global__ __launch_bounds__(64) void k_norm_wave(const uint8_t* __restrict__ img, int64_t istride, int iw, int ih,
                                                   int step, int tw, int th, int rw, int rh,
                                                   const CallState* __restrict__ st, const uint32_t* __restrict__ ps,
-                                                  const float* corr, float* map, Extrema* __restrict__ parts) {
+                                                  const float* corr, float* map, Extrema* __restrict__ parts,
+                                                  int band) {
   typedef typename std::conditional<SQ32, uint32_t, unsigned long long>::type sq_t;
   __shared__ uint32_t P[4][norm_nwp(CPT)];
   __shared__ sq_t P2[norm_nwp(CPT)];
   const int cn = st->cn;
 #define CVM_NWB(N)                                                                                             \
   norm_wave_block<Q, SQ32, VMODE, EXACT, N, sq_t, CPT>(img, istride, iw, ih, step, tw, th, rw, rh, st, ps, corr, \
-                                                       map, parts, blockIdx.x, blockIdx.y, gridDim.x, P, P2)
+                                                       map, parts, blockIdx.x, blockIdx.y, gridDim.x, P, P2, band)
   if (VMODE == 2 || (VMODE == 0 && cn == 1))
     CVM_NWB(1);
   else if (VMODE == 1 && cn == 3)
@@ -863,7 +886,8 @@ __global__ __launch_bounds__(64) void k_norm_batch(const uint8_t* __restrict__ i
   const int cn = J.st->cn;
 #define CVM_NBB(N)                                                                                              \
   norm_wave_block<Q, SQ32, VMODE, EXACT, N, sq_t, NORM_CPT>(img, istride, iw, ih, step, J.tw, J.th, J.rw, J.rh, J.st, J.ps, \
-                                                  J.corr, nullptr, J.parts, local % J.gx, local / J.gx, J.gx, P, P2)
+                                                  J.corr, nullptr, J.parts, local % J.gx, local / J.gx, J.gx, P, P2, \
+                                                  NB_ROWS)
   if (VMODE == 2 || (VMODE == 0 && cn == 1))
     CVM_NBB(1);
   else if (VMODE == 1 && cn == 3)
@@ -896,10 +920,10 @@ void launch_normalize(const uint8_t* img, int64_t istride, int iw, int ih, int s
     int vmode = 0;
     if (iw % np.cpt == 0 && step == 4 && a16) vmode = 1;
     if (iw % np.cpt == 0 && step == 1 && a8) vmode = 2;
-    dim3 grid((rw + 64 * np.q - 1) / (64 * np.q), (rh + NB_ROWS - 1) / NB_ROWS);
+    dim3 grid((rw + 64 * np.q - 1) / (64 * np.q), (rh + np.band - 1) / np.band);
 #define CVM_NW(Q, C, S, V, X)                                                                                       \
   hipLaunchKernelGGL((k_norm_wave<Q, S, V, X, C>), grid, dim3(64), 0, s, img, istride, iw, ih, step, tw, th, rw, rh, \
-                     st, ps, corr, map, partials)
+                     st, ps, corr, map, partials, np.band)
 #define CVM_NWX(Q, C, S, V)  \
   if (exact)                 \
     CVM_NW(Q, C, S, V, true); \

@@ -124,3 +124,33 @@ def test_tuner_report(cvm):
         assert d["source"] in ("measured", "cache", "override", "default"), (k, d)
         if k.startswith("fleet.direct.waves/") and d["source"] == "measured":
             assert d["choice"] in (1, 2, 4) and len(d["ms"]) == 3
+
+
+@pytest.mark.parametrize("q", ["1", "2", "4", "9", "10", "12"])
+def test_forced_strip_forms_change_no_bits(cvm, orc, monkeypatch, q):
+    """Every strip form of a one-shot call's normalisation forced, full bands
+    (q) and half bands (q + 8), in the compact and the forked call forms:
+    the plain map is the oracle's bit for bit, the direct map the unforced
+    direct call's.  Result heights that are not multiples of 8 or 16 and
+    template heights that are not multiples of 16 (window rows read directly
+    at the window's end) are among the shapes."""
+    rng = np.random.default_rng(500 + int(q))
+    cases = [(200, 150, 4, 4, 24, 19), (333, 97, 1, 1, 40, 33), (150, 90, 3, 4, 17, 20)]
+    for iw, ih, cn, step, tw, th in cases:
+        img = rng.integers(0, 256, ih * iw * step, dtype=np.uint8)
+        x, y = int(rng.integers(0, iw - tw + 1)), int(rng.integers(0, ih - th + 1))
+        tpl = np.ascontiguousarray(img.reshape(ih, iw * step)[y:y + th, x * step:(x + tw) * step]).ravel()
+        args = (img, iw * step, iw, ih, tpl, tw * step, tw, th, cn, step)
+        want = orc.match_u8(*args, threads=8)
+        monkeypatch.delenv("CVMATCH_NORM_Q", raising=False)
+        want_direct = cvm.match_u8(*args, want_map=True, direct=True)
+        for compact in ("2500000", "0"):
+            say(f"[tune] strip form {q}, {iw}x{ih} cn {cn} tpl {tw}x{th}, compact bound {compact}")
+            monkeypatch.setenv("CVMATCH_NORM_Q", q)
+            monkeypatch.setenv("CVMATCH_COMPACT", compact)
+            got = cvm.match_u8(*args, want_map=True)
+            assert got[1] == want[1], (got[1], want[1])
+            assert np.array_equal(got[0].view(np.uint32), want[0].view(np.uint32))
+            gd = cvm.match_u8(*args, want_map=True, direct=True)
+            assert gd[1] == want_direct[1]
+            assert np.array_equal(gd[0].view(np.uint32), want_direct[0].view(np.uint32))

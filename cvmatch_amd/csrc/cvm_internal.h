@@ -70,6 +70,7 @@ struct CallStateArgs {
   int64_t tstride;
   int tw, th, tvec, cnMax, step;
   int sgx, sgy, pgx, pgy, tgx, tgy;
+  int rows8;  // the compact layout: sgx flat statistics workgroups, scans of 8 rows
   CallState* st;
   int nb() const { return sgx * sgy + pgx * pgy + tgx * tgy; }
 };
@@ -197,8 +198,8 @@ void launch_state_finalize(CallState* st, int tw, int th, int cn_mode, hipStream
 void launch_call_state(const uint8_t* img, int64_t istride, int iw, int ih, const uint8_t* tpl, int64_t tstride,
                        int tw, int th, int cnMax, int step, int cn_mode, CallState* st, hipStream_t s);
 // The geometry of launch_call_state's first launch; whole_parent: the
-// parent's alpha scan entirely in it (ph0 = ih), for a launch that takes
-// the state's independent parts beside other work.
+// whole parent's alpha scan in it, in the compact layout (rows8), for a
+// launch that takes the state's independent parts beside other work.
 CallStateArgs call_state_args(const uint8_t* img, int64_t istride, int iw, int ih, const uint8_t* tpl,
                               int64_t tstride, int tw, int th, int cnMax, int step, int cn_mode, CallState* st,
                               bool whole_parent);

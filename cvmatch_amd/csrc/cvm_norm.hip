@@ -99,16 +99,22 @@ CallStateArgs call_state_args(const uint8_t* img, int64_t istride, int iw, int i
   a.cnMax = cnMax;
   a.step = step;
   a.st = st;
-  a.sgx = std::min(16, (tw + 255) / 256);
-  a.sgy = std::min(256, th);
+  a.rows8 = whole_parent;
+  if (whole_parent) {
+    a.sgx = (int)std::min<long long>(256, ((long long)tw * th + 2047) / 2048);
+    a.sgy = 1;
+  } else {
+    a.sgx = std::min(16, (tw + 255) / 256);
+    a.sgy = std::min(256, th);
+  }
   if (cn_mode == 0) {
     a.ivec = ((((uintptr_t)img) | (uintptr_t)istride) & 15) == 0;
     a.tvec = ((((uintptr_t)tpl) | (uintptr_t)tstride) & 15) == 0;
     a.ph0 = whole_parent ? ih : std::min(ih, 8);
     a.pgx = std::min(16, ((a.ivec ? iw / 4 + 1 : iw) + 255) / 256);
-    a.pgy = std::min(512, a.ph0);
+    a.pgy = whole_parent ? (ih + 7) / 8 : std::min(512, a.ph0);
     a.tgx = std::min(16, ((a.tvec ? tw / 4 + 1 : tw) + 255) / 256);
-    a.tgy = std::min(512, th);
+    a.tgy = whole_parent ? (th + 7) / 8 : std::min(512, th);
   }
   return a;
 }

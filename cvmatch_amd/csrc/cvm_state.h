@@ -127,14 +127,104 @@ __device__ __forceinline__ void alpha_scan_block(const uint8_t* __restrict__ pix
   if (__syncthreads_or(bad) && threadIdx.x == 0) atomicOr(&st->alpha_bad[which], 1u);
 }
 
+// The same statistics over a flat pixel index: workgroup b of nb takes
+// pixels b * blockDim + tid + k * nb * blockDim, four of them in flight at
+// a time, so a small template needs a workgroup or two and few atomics.
+__device__ __forceinline__ void tpl_stats_flat(const uint8_t* __restrict__ tpl, int64_t tstride, int tw, int th,
+                                               int cnMax, int step, CallState* st, int b, int nb) {
+  __shared__ unsigned long long red[4][8];
+  unsigned long long s[4] = {0, 0, 0, 0}, q[4] = {0, 0, 0, 0};
+  const int n = tw * th, stride = nb * (int)blockDim.x;
+  for (int i0 = b * (int)blockDim.x + (int)threadIdx.x; i0 < n; i0 += 4 * stride) {
+    uint32_t w[4];
+#pragma unroll
+    for (int u = 0; u < 4; u++) {
+      const int i = i0 + u * stride;
+      w[u] = 0;
+      if (i < n) {
+        const int y = i / tw, x = i - y * tw;
+        const uint8_t* p = tpl + (int64_t)y * tstride + (int64_t)x * step;
+        for (int k = 0; k < 4; k++)
+          if (k < cnMax) w[u] |= (uint32_t)p[k] << (8 * k);
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < 4; u++)
+      for (int k = 0; k < 4; k++) {
+        const unsigned long long v = (w[u] >> (8 * k)) & 255u;
+        s[k] += v;
+        q[k] += v * v;
+      }
+  }
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  for (int k = 0; k < 4; k++) {
+    unsigned long long a = wave_sum_u64(s[k]), c = wave_sum_u64(q[k]);
+    if (lane == 0) {
+      red[k][wv] = a;
+      red[k][4 + wv] = c;
+    }
+  }
+  __syncthreads();
+  if (threadIdx.x < 4 && (int)threadIdx.x < cnMax) {
+    int k = threadIdx.x;
+    unsigned long long a = 0, c = 0;
+    for (int i = 0; i < (int)(blockDim.x >> 6); i++) {
+      a += red[k][i];
+      c += red[k][4 + i];
+    }
+    atomicAdd(&st->tsum[k], a);
+    atomicAdd(&st->tsq[k], c);
+  }
+}
+
+// The alpha scan without polling: workgroup (bx, by) checks rows
+// [8 by, 8 by + 8) (rows past h re-read the last one) of its columns, the
+// eight rows' loads in flight together.
+__device__ __forceinline__ void alpha_scan_rows8(const uint8_t* __restrict__ pix, int64_t stride, int w, int h,
+                                                 CallState* st, int which, int vec, int bx, int by, int gx) {
+  const uint32_t a0 = pix[3];
+  const uint32_t want = a0 * 0x01010101u;
+  const int w4 = vec ? (w >> 2) : 0;
+  const int y0 = by * 8;
+  int bad = 0;
+  for (int q = bx * blockDim.x + threadIdx.x; q < w4; q += gx * blockDim.x) {
+    uint4 v[8];
+#pragma unroll
+    for (int r = 0; r < 8; r++) v[r] = *(const uint4*)(pix + (int64_t)min(y0 + r, h - 1) * stride + (int64_t)q * 16);
+#pragma unroll
+    for (int r = 0; r < 8; r++) {
+      const uint32_t al = (v[r].x >> 24) | ((v[r].y >> 24) << 8) | ((v[r].z >> 24) << 16) | ((v[r].w >> 24) << 24);
+      if (al != want) bad = 1;
+    }
+  }
+  for (int x = 4 * w4 + bx * blockDim.x + threadIdx.x; x < w; x += gx * blockDim.x)
+    for (int r = 0; r < 8; r++)
+      if (y0 + r < h && pix[(int64_t)(y0 + r) * stride + (int64_t)x * 4 + 3] != a0) bad = 1;
+  if (__syncthreads_or(bad) && threadIdx.x == 0) atomicOr(&st->alpha_bad[which], 1u);
+}
+
 // The independent parts of a call state in one launch: workgroups [0, nS)
 // take the template statistics (an sgx x sgy grid), the next pgx x pgy the
 // first ph0 rows of the parent's alpha scan (a varying alpha plane shows up
 // there, and the rest of the scan then leaves at its first flag check; with
-// ph0 = ih the whole scan), the last tgx x tgy the template's alpha scan.
-// b is the workgroup's index among them (256 threads).
+// ph0 = ih the whole scan), the last tgx x tgy the template's alpha scan;
+// rows8: the compact layout (tpl_stats_flat, alpha_scan_rows8).  b is the
+// workgroup's index among them (256 threads).
 __device__ __forceinline__ void call_state_block(int b, const CallStateArgs& a) {
   const int nS = a.sgx * a.sgy, nP = a.pgx * a.pgy;
+  if (a.rows8) {
+    // the compact layout: flat statistics (sgx workgroups), scans of 8 rows
+    if (b < nS) {
+      tpl_stats_flat(a.tpl, a.tstride, a.tw, a.th, a.cnMax, a.step, a.st, b, nS);
+    } else if (b < nS + nP) {
+      b -= nS;
+      alpha_scan_rows8(a.img, a.istride, a.iw, a.ph0, a.st, 0, a.ivec, b % a.pgx, b / a.pgx, a.pgx);
+    } else {
+      b -= nS + nP;
+      alpha_scan_rows8(a.tpl, a.tstride, a.tw, a.th, a.st, 1, a.tvec, b % a.tgx, b / a.tgx, a.tgx);
+    }
+    return;
+  }
   if (b < nS) {
     tpl_stats_block(a.tpl, a.tstride, a.tw, a.th, a.cnMax, a.step, a.st, b % a.sgx, b / a.sgx, a.sgx, a.sgy);
   } else if (b < nS + nP) {

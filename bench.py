@@ -398,13 +398,14 @@ def windowrules_leg(reps=20):
     return out
 
 
-def configs_leg(reps=15):
+def configs_leg(reps=15, small_only=False):
     """The other configurations, device resident (inputs in HBM, the map
     written on the device, graphs replayed like the headline's step):
     640x480 RGBA Match 48x48 (cn = 4), the 1920x1080 gray UI MatchGray 96x32,
     the 2560x1440 gray direct-vs-FFT sweep (ms per call with the map for
     square templates 8..256; the direct form is the FP64-exact one) and the
-    4K RGBA 128x128 call in both forms."""
+    4K RGBA 128x128 call in both forms.  small_only: the two small calls'
+    times alone (profiling)."""
     import cvmatch_amd as cvm
     from cvmatch_amd import synth
 
@@ -449,17 +450,22 @@ def configs_leg(reps=15):
     dp, ds = dev(parent), dev(sub)
     dm = dev(np.zeros(rw * rh, np.float32))
     dt = timed(lambda: call(False, dp, 640 * 4, 640, 480, ds, 48 * 4, 48, 48, 0, 4, dm))
-    res["match_640x480_rgba_48"] = {"ms": round(dt * 1e3, 4), "Gwindows_per_s": round(rw * rh / dt / 1e9, 3),
-                                    "vs_fp64": map_delta_vs_fp64(host_map(dm, rw, rh), parent, sub, 4, seed=11,
-                                                                 extra=extrema_pts())}
+    res["match_640x480_rgba_48"] = {"ms": round(dt * 1e3, 4), "Gwindows_per_s": round(rw * rh / dt / 1e9, 3)}
+    if not small_only:
+        res["match_640x480_rgba_48"]["vs_fp64"] = map_delta_vs_fp64(host_map(dm, rw, rh), parent, sub, 4, seed=11,
+                                                                    extra=extrema_pts())
     gray, gsub = synth.workload_gray_ui_1080p(seed=2)[:2]
     rw, rh = 1920 - 96 + 1, 1080 - 32 + 1
     dg, dgs = dev(gray), dev(gsub)
     dm = dev(np.zeros(rw * rh, np.float32))
     dt = timed(lambda: call(False, dg, 1920, 1920, 1080, dgs, 96, 96, 32, 1, 1, dm))
-    res["match_gray_1920x1080_96x32"] = {"ms": round(dt * 1e3, 4), "Gwindows_per_s": round(rw * rh / dt / 1e9, 3),
-                                         "vs_fp64": map_delta_vs_fp64(host_map(dm, rw, rh), gray, gsub, 1, seed=12,
-                                                                      extra=extrema_pts())}
+    res["match_gray_1920x1080_96x32"] = {"ms": round(dt * 1e3, 4), "Gwindows_per_s": round(rw * rh / dt / 1e9, 3)}
+    if small_only:
+        for p in bufs:
+            L.cvm_device_free(p)
+        return res
+    res["match_gray_1920x1080_96x32"]["vs_fp64"] = map_delta_vs_fp64(host_map(dm, rw, rh), gray, gsub, 1, seed=12,
+                                                                     extra=extrema_pts())
     img, _ = synth.workload_direct_1440p(seed=4, sizes=())
     di = dev(img)
     dm = dev(np.zeros(2560 * 1440, np.float32))
@@ -627,6 +633,8 @@ def main():
     ap.add_argument("--no-host", action="store_true", help="skip the host-buffer (PCIe-inclusive) leg")
     ap.add_argument("--fleet-frames", type=int, default=63, help="frames timed per Fleet form")
     ap.add_argument("--fleet-only", action="store_true", help="only run the Fleet leg (profiling)")
+    ap.add_argument("--small-only", action="store_true",
+                    help="only time the 640x480 RGBA Match and the 1080p gray MatchGray (profiling)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run the plain copy kernel over MIB MiB (calibrates FETCH_SIZE/WRITE_SIZE)")
     args = ap.parse_args()
@@ -635,6 +643,9 @@ def main():
         return
     if args.fleet_only:
         print(json.dumps({"fleet": fleet_leg(args.fleet_frames)}), flush=True)
+        return
+    if args.small_only:
+        print(json.dumps({"configs": configs_leg(reps=args.steps, small_only=True)}), flush=True)
         return
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (the path shards per frame or per "

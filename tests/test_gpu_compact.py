@@ -1,0 +1,72 @@
+"""GPU: the compact form of a small one-shot call (one stream, two
+independent jobs per launch) against the forked form (three streams) and the
+oracle.  Both forms run the same device code per job, so maps and six-tuples
+must be identical bit for bit; the cases cover gray and RGBA, the alpha rule
+with a varying and a constant alpha plane (cn 4 and 3), two- and
+three-channel buffers, a flat template, row strides that defeat the vector
+loads, template heights that are not multiples of 16 and maps that take
+several tiles."""
+import sys
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+# iw, ih, cn, step, tw, th, extra stride bytes, what
+CASES = [
+    (640, 480, 0, 4, 48, 48, 0, "alpha rule, varying alpha"),
+    (300, 200, 0, 4, 33, 21, 0, "alpha rule, constant alpha"),
+    (333, 97, 1, 1, 40, 33, 3, "gray, odd stride"),
+    (257, 190, 2, 2, 30, 17, 0, "two channels"),
+    (210, 150, 3, 3, 19, 40, 5, "three-byte pixels, odd stride"),
+    (500, 300, 4, 4, 8, 8, 0, "flat template"),
+    (1100, 700, 4, 4, 64, 50, 16, "several tiles"),
+]
+
+
+def buffers(rng, iw, ih, cn, step, tw, th, pad, what):
+    istride = iw * step + pad
+    img = rng.integers(0, 256, ih * istride, dtype=np.uint8)
+    v = img.reshape(ih, istride)
+    if "constant alpha" in what:
+        v[:, 3:iw * 4:4] = 200
+    x, y = int(rng.integers(0, iw - tw + 1)), int(rng.integers(0, ih - th + 1))
+    tpl = np.ascontiguousarray(v[y:y + th, x * step:(x + tw) * step]).ravel()
+    if what == "flat template":
+        tpl[:] = 91
+    return img, istride, tpl, tw * step
+
+
+@pytest.mark.parametrize("case", range(len(CASES)))
+def test_compact_equals_forked_and_oracle(cvm, orc, monkeypatch, case):
+    iw, ih, cn, step, tw, th, pad, what = CASES[case]
+    rng = np.random.default_rng(900 + case)
+    img, istride, tpl, tstride = buffers(rng, iw, ih, cn, step, tw, th, pad, what)
+    args = (img, istride, iw, ih, tpl, tstride, tw, th, cn, step)
+    if cn == 0:  # Match's alpha rule: the RGBA entry on a strided view
+        parent = img.reshape(ih, istride)[:, :iw * 4].reshape(ih, iw, 4)
+        sub = tpl.reshape(th, tw, 4)
+    got = {}
+    for bound in ("2500000", "0"):
+        monkeypatch.setenv("CVMATCH_COMPACT", bound)
+        say(f"[compact] {what}: {iw}x{ih} cn {cn} step {step} tpl {tw}x{th}, compact bound {bound}")
+        if cn == 0:
+            got[bound] = (cvm.match_map(parent, sub)[0], tuple(cvm.match(parent, sub)))
+        else:
+            got[bound] = cvm.match_u8(*args, want_map=True)
+    a, b = got["2500000"], got["0"]
+    assert a[1] == b[1], (a[1], b[1])
+    assert np.array_equal(a[0].view(np.uint32), b[0].view(np.uint32))
+    if cn == 0:
+        om, ot = orc.match_rgba(parent, sub, threads=8)
+    else:
+        om, ot = orc.match_u8(*args, threads=8)
+    assert tuple(a[1]) == tuple(ot), (a[1], ot)
+    assert np.array_equal(a[0].view(np.uint32), np.asarray(om, np.float32).reshape(a[0].shape).view(np.uint32))

@@ -512,12 +512,14 @@ struct ColCG {
 };
 
 // Pass schedule of a column transform.  The first pass takes the radix-2
-// head (odd sizes) and up to two radix-4 stages (32 or 16 points per group);
-// every later pass two stages, the last one one or two.
-template <int LOG2N>
+// head (odd sizes) and up to two radix-4 stages (32 or 16 points per group;
+// NST1 >= 0 caps them, for a workgroup with more threads than groups of
+// that size); every later pass two stages, the last one one or two.
+template <int LOG2N, int NST1 = -1>
 struct ColFirst {
   static constexpr int HEAD = LOG2N & 1;
-  static constexpr int NST = (LOG2N - HEAD) / 2 >= 2 ? 2 : (LOG2N - HEAD) / 2;
+  static constexpr int NSTD = (LOG2N - HEAD) / 2 >= 2 ? 2 : (LOG2N - HEAD) / 2;
+  static constexpr int NST = (NST1 >= 0 && NST1 < NSTD) ? NST1 : NSTD;
   static constexpr int NB = HEAD + 2 * NST;
 };
 
@@ -529,12 +531,12 @@ __host__ __device__ constexpr int col_last_lo(int log2n, int lo) {
 
 // The two-stage passes from bit LO up to (not including) the last pass, in
 // place in the LDS slab, each followed by a barrier.
-template <int LOG2N, int LO, int LOL, int CG>
+template <int LOG2N, int LO, int LOL, int CG, int NT = 256>
 __device__ __forceinline__ void col_mid(float2* a, const float2* __restrict__ tri, bool inverse) {
   if constexpr (LO < LOL) {
-    fft_pass<LOG2N, LO, 0, 2, CG, 256, CG, false, true>(a, CG, 1, tri, inverse);
+    fft_pass<LOG2N, LO, 0, 2, CG, NT, CG, false, true>(a, CG, 1, tri, inverse);
     __syncthreads();
-    col_mid<LOG2N, LO + 4, LOL, CG>(a, tri, inverse);
+    col_mid<LOG2N, LO + 4, LOL, CG, NT>(a, tri, inverse);
   }
 }
 
@@ -551,11 +553,11 @@ __device__ __forceinline__ void store_f2(float __attribute__((ext_vector_type(2)
   *p = u;
 }
 
-template <int LOG2N, typename DstPtr>
+template <int LOG2N, int NT, int NST1, typename DstPtr>
 __device__ __forceinline__ void col_inverse_store(float2* a, const float2* __restrict__ tri, DstPtr dp, int hwPad,
                                                   int ncol, int need) {
-  constexpr int CG = ColCG<LOG2N>::value, LCG = ceil_log2c(CG), NT = 256;
-  using F = ColFirst<LOG2N>;
+  constexpr int CG = ColCG<LOG2N>::value, LCG = ceil_log2c(CG);
+  using F = ColFirst<LOG2N, NST1>;
   constexpr int NB1 = F::NB;
   constexpr bool SINGLE = NB1 == LOG2N;
   constexpr int LOL = SINGLE ? 0 : col_last_lo(LOG2N, NB1);
@@ -567,7 +569,7 @@ __device__ __forceinline__ void col_inverse_store(float2* a, const float2* __res
   if constexpr (!SINGLE) {
     fft_pass<LOG2N, 0, F::HEAD, F::NST, CG, NT, CG, false, true>(a, CG, 1, tri, true);
     __syncthreads();
-    col_mid<LOG2N, NB1, LOL, CG>(a, tri, true);
+    col_mid<LOG2N, NB1, LOL, CG, NT>(a, tri, true);
   }
 #pragma unroll
   for (int it = 0; it < ITL; it++) {
@@ -606,14 +608,15 @@ __device__ __forceinline__ void col_inverse_store(float2* a, const float2* __res
 // template-spectrum slab run together on one XCD.  A body covers tiles
 // [t0, t0 + ntiles), all of column height 2^LOG2N; L is its logical
 // workgroup index.
-template <int LOG2N, int MODE>
+// NT threads per workgroup; NST1: the first pass's radix-4 stages (ColFirst).
+template <int LOG2N, int MODE, int NT = 256, int NST1 = -1>
 __device__ __forceinline__ void cols_body(float2* a, int L, const float2* __restrict__ src,
                                           float2* __restrict__ dst, int64_t srcTileStride, int64_t dstTileStride,
                                           int64_t chanStride, const CallState* __restrict__ st, const TileGrid& g,
                                           const float2* __restrict__ tri, const float2* __restrict__ tspec,
                                           int64_t tchanStride, float scale, int t0, int ntiles, int nslabs) {
-  constexpr int n = 1 << LOG2N, CG = ColCG<LOG2N>::value, LCG = ceil_log2c(CG), NT = 256;
-  using F = ColFirst<LOG2N>;
+  constexpr int n = 1 << LOG2N, CG = ColCG<LOG2N>::value, LCG = ceil_log2c(CG);
+  using F = ColFirst<LOG2N, NST1>;
   constexpr int NB1 = F::NB, E1 = 1 << NB1;
   constexpr int G1 = CG << (LOG2N - NB1), IT1 = (G1 + NT - 1) / NT;
   constexpr bool SINGLE = NB1 == LOG2N;
@@ -688,7 +691,7 @@ __device__ __forceinline__ void cols_body(float2* a, int L, const float2* __rest
           for (int e = 0; e < E1; e++) a[((gg << NB1) | e) * CG + v] = r1[it][e];
       }
       __syncthreads();
-      col_mid<LOG2N, NB1, LOL, CG>(a, tri, false);
+      col_mid<LOG2N, NB1, LOL, CG, NT>(a, tri, false);
 #pragma unroll
       for (int it = 0; it < ITL; it++) {
         const int gi = tid + it * NT, v = gi & (CG - 1), gg = gi >> LCG;
@@ -760,7 +763,7 @@ __device__ __forceinline__ void cols_body(float2* a, int L, const float2* __rest
     }
   }
   __syncthreads();
-  col_inverse_store<LOG2N>(a, tri, dp, hwPad, ncol, need);
+  col_inverse_store<LOG2N, NT, NST1>(a, tri, dp, hwPad, ncol, need);
 }
 
 // Two members of one Fleet bucket against the same frame spectrum in one
@@ -838,7 +841,7 @@ __global__ __launch_bounds__(256) void k_cols_pair(const float2* __restrict__ sr
     int need = (bh + 1) & ~1;
     if (need > n) need = n;
     const gf2p dp = (gf2p)M.dst + (int64_t)t * tileStride + (int64_t)k * chanStride + c0;
-    col_inverse_store<LOG2N>(a, tri, dp, hwPad, ncol, need);
+    col_inverse_store<LOG2N, 256, -1>(a, tri, dp, hwPad, ncol, need);
   };
   if (onA) member(A);
   if (onA && onB) __syncthreads();  // the slab is rewritten for B
@@ -885,6 +888,25 @@ __global__ __launch_bounds__(256) void k_cols2(const float2* __restrict__ src, f
                             st, g, b2.tri, b2.tspec, b2.tchanStride, scale, b2.t0, b2.ntiles, nslabs);
 }
 
+// The same with 512 threads a workgroup and one radix-4 stage in the first
+// pass (8 or 4 points a group, so every thread has groups): half the
+// registers a thread and twice the waves a workgroup.
+template <int LOG2N, int LOG2N2, int MODE>
+__global__ __launch_bounds__(512) void k_cols2w(const float2* __restrict__ src, float2* __restrict__ dst,
+                                                int64_t srcTileStride, int64_t dstTileStride, int64_t chanStride,
+                                                const CallState* __restrict__ st, TileGrid g, float scale, int nslabs,
+                                                ColBand b1, ColBand b2) {
+  extern __shared__ float2 a[];
+  const int p = blockIdx.x;
+  if (p < b1.nwg)
+    cols_body<LOG2N, MODE, 512, 1>(a, xcd_remap(p, b1.nwg), src, dst, srcTileStride, dstTileStride, chanStride, st,
+                                   g, b1.tri, b1.tspec, b1.tchanStride, scale, b1.t0, b1.ntiles, nslabs);
+  else
+    cols_body<LOG2N2, MODE, 512, 1>(a, xcd_remap(p - b1.nwg, b2.nwg), src, dst, srcTileStride, dstTileStride,
+                                    chanStride, st, g, b2.tri, b2.tspec, b2.tchanStride, scale, b2.t0, b2.ntiles,
+                                    nslabs);
+}
+
 template <int LOG2N>
 static void cols_n(ColMode mode, const float2* src, float2* dst, int64_t srcTileStride, int64_t dstTileStride,
                    int64_t chanStride, int cnMax, const CallState* st, const TileGrid& g, const float2* tri,
@@ -919,6 +941,21 @@ static bool cols2_n(ColMode mode, const float2* src, float2* dst, int64_t srcTil
   b1.nwg = (nslabs * b1.ntiles * cnMax + 7) & ~7;
   b2.nwg = nslabs * b2.ntiles * cnMax;
   dim3 grid(b1.nwg + b2.nwg), block(256);
+  if constexpr (LOG2N == 9) {
+    // 512-row columns: the 512-thread form (4 waves a SIMD instead of 2;
+    // 4K RGBA column pass 111.5 -> 104.8 us); CVMATCH_COLS_WIDE=0 keeps
+    // the 256-thread one
+    const char* e = getenv("CVMATCH_COLS_WIDE");
+    if (!(e && e[0] == '0') && (mode == COL_CORR || mode == COL_MULINV)) {
+      if (mode == COL_CORR)
+        hipLaunchKernelGGL((k_cols2w<LOG2N, LOG2N2, COL_CORR>), grid, dim3(512), lds, s, src, dst, srcTileStride,
+                           dstTileStride, chanStride, st, g, scale, nslabs, b1, b2);
+      else
+        hipLaunchKernelGGL((k_cols2w<LOG2N, LOG2N2, COL_MULINV>), grid, dim3(512), lds, s, src, dst,
+                           srcTileStride, dstTileStride, chanStride, st, g, scale, nslabs, b1, b2);
+      return true;
+    }
+  }
   if (mode == COL_CORR)
     hipLaunchKernelGGL((k_cols2<LOG2N, LOG2N2, COL_CORR>), grid, block, lds, s, src, dst, srcTileStride,
                        dstTileStride, chanStride, st, g, scale, nslabs, b1, b2);

@@ -1239,9 +1239,9 @@ struct ColArgs {
   int nwg;
 };
 
-template <int LOG2N, int MODE>
+template <int LOG2N, int MODE, int NT = 256, int NST1 = -1>
 __device__ __forceinline__ void cols_job(float2* a, int p, const ColArgs& C) {
-  cols_body<LOG2N, MODE>(a, xcd_remap(p, C.nwg), C.src, C.dst, C.srcTileStride, C.dstTileStride, C.chanStride, C.st,
+  cols_body<LOG2N, MODE, NT, NST1>(a, xcd_remap(p, C.nwg), C.src, C.dst, C.srcTileStride, C.dstTileStride, C.chanStride, C.st,
                          C.g, C.tri, C.tspec, C.tchanStride, C.scale, C.t0, C.ntiles, C.nslabs);
 }
 
@@ -1280,11 +1280,11 @@ __global__ __launch_bounds__(RowNTF<LOG2N>::value) void k_front(RowFwdArgs P, Ro
 
 // The template's column pass (its spectrum) and the call state's
 // independent parts (template statistics, alpha scans).
-template <int LOG2N>
-__global__ __launch_bounds__(256) void k_tspec_state(ColArgs C, CallStateArgs S) {
+template <int LOG2N, int NT = 256, int NST1 = -1>
+__global__ __launch_bounds__(NT) void k_tspec_state(ColArgs C, CallStateArgs S) {
   extern __shared__ float2 a[];
   if ((int)blockIdx.x < C.nwg)
-    cols_job<LOG2N, COL_TSPEC>(a, blockIdx.x, C);
+    cols_job<LOG2N, COL_TSPEC, NT, NST1>(a, blockIdx.x, C);
   else
     call_state_block(blockIdx.x - C.nwg, S);
 }
@@ -1299,8 +1299,8 @@ struct FinalizeArgs {
 
 // The correlation's column pass and the chunk column sums, after the
 // state's finalisation.
-template <int LOG2N>
-__global__ __launch_bounds__(256) void k_corr_chunks(ColArgs C, ChunkArgs K, FinalizeArgs F) {
+template <int LOG2N, int NT = 256, int NST1 = -1>
+__global__ __launch_bounds__(NT) void k_corr_chunks(ColArgs C, ChunkArgs K, FinalizeArgs F) {
   extern __shared__ float2 a[];
   __shared__ CallState sst;
   if (threadIdx.x == 0) {
@@ -1318,10 +1318,10 @@ __global__ __launch_bounds__(256) void k_corr_chunks(ColArgs C, ChunkArgs K, Fin
   __syncthreads();
   if ((int)blockIdx.x < C.nwg) {
     C.st = &sst;
-    cols_job<LOG2N, COL_CORR>(a, blockIdx.x, C);
+    cols_job<LOG2N, COL_CORR, NT, NST1>(a, blockIdx.x, C);
   } else {
     const int b = blockIdx.x - C.nwg;
-    colsum_chunks_at((b % K.nbx) * 256 + threadIdx.x, b / K.nbx, K.img, K.istride, K.iw, K.ih, K.step, K.words, &sst,
+    colsum_chunks_at((b % K.nbx) * NT + threadIdx.x, b / K.nbx, K.img, K.istride, K.iw, K.ih, K.step, K.words, &sst,
                      K.ps);
   }
 }
@@ -1376,10 +1376,11 @@ static void compact_rows_dispatch(const CompactCall& c, bool front, const Prefix
 #undef CVM_CR
 }
 
-template <int LOG2H>
-static void compact_cols(const CompactCall& c, const CallStateArgs& S, const ChunkArgs& K, bool tspec_state,
+template <int LOG2H, int NT = 256, int NST1 = -1>
+static void compact_cols(const CompactCall& c, const CallStateArgs& S, ChunkArgs K, bool tspec_state,
                          hipStream_t s) {
   const FinalizeArgs F{c.st, c.tw, c.th, c.cn_mode};
+  K.nbx = (c.iw + NT - 1) / NT;
   constexpr int CG = ColCG<LOG2H>::value;
   const size_t lds = sizeof(float2) * (size_t)(1 << LOG2H) * CG;
   const int nslabs = (c.g.hw + CG - 1) / CG;
@@ -1400,7 +1401,7 @@ static void compact_cols(const CompactCall& c, const CallStateArgs& S, const Chu
     C.scale = c.scale;
     C.ntiles = 1;
     C.nwg = nslabs * c.cnMax;
-    hipLaunchKernelGGL(k_tspec_state<LOG2H>, dim3(C.nwg + S.nb()), dim3(256), lds, s, C, S);
+    hipLaunchKernelGGL((k_tspec_state<LOG2H, NT, NST1>), dim3(C.nwg + S.nb()), dim3(NT), lds, s, C, S);
   } else {
     C.src = c.spec;
     C.dst = c.spec;
@@ -1412,7 +1413,7 @@ static void compact_cols(const CompactCall& c, const CallStateArgs& S, const Chu
     C.scale = 1.0f;
     C.ntiles = c.ntiles;
     C.nwg = nslabs * c.ntiles * c.cnMax;
-    hipLaunchKernelGGL(k_corr_chunks<LOG2H>, dim3(C.nwg + K.nbx * K.nch), dim3(256), lds, s, C, K, F);
+    hipLaunchKernelGGL((k_corr_chunks<LOG2H, NT, NST1>), dim3(C.nwg + K.nbx * K.nch), dim3(NT), lds, s, C, K, F);
   }
 }
 
@@ -1422,8 +1423,20 @@ static void compact_cols_dispatch(const CompactCall& c, const CallStateArgs& S, 
   case L:                                              \
     compact_cols<L>(c, S, K, tspec_state, s);          \
     break
+  // 512-row columns in the 512-thread form (one radix-4 stage in the first
+  // pass: twice the waves a workgroup, each with half the points; 640x480
+  // RGBA Match 0.075 -> 0.068 ms; 256-row columns measured slower so);
+  // CVMATCH_COMPACT_WIDE=0 keeps the 256-thread one
+  const char* e = getenv("CVMATCH_COMPACT_WIDE");
+  const bool wide = !(e && e[0] == '0');
   switch (c.g.log2H) {
-    CVM_CC(5); CVM_CC(6); CVM_CC(7); CVM_CC(8); CVM_CC(9); CVM_CC(10); CVM_CC(11);
+    CVM_CC(5); CVM_CC(6); CVM_CC(7); CVM_CC(8); CVM_CC(10); CVM_CC(11);
+    case 9:
+      if (wide)
+        compact_cols<9, 512, 1>(c, S, K, tspec_state, s);
+      else
+        compact_cols<9>(c, S, K, tspec_state, s);
+      break;
     default: break;
   }
 #undef CVM_CC

@@ -62,7 +62,7 @@ __device__ __forceinline__ unsigned long long wave_sum_u64(unsigned long long v)
 // One workgroup (bx, by) of a gx x gy grid over the template.
 __device__ __forceinline__ void tpl_stats_block(const uint8_t* __restrict__ tpl, int64_t tstride, int tw, int th,
                                                 int cnMax, int step, CallState* st, int bx, int by, int gx, int gy) {
-  __shared__ unsigned long long red[4][8];
+  __shared__ unsigned long long red[4][16];  // [k][w]: sums, [k][8 + w]: squares (<= 8 waves)
   unsigned long long s[4] = {0, 0, 0, 0}, q[4] = {0, 0, 0, 0};
   for (int y = by; y < th; y += gy) {
     const uint8_t* row = tpl + (int64_t)y * tstride;
@@ -81,7 +81,7 @@ __device__ __forceinline__ void tpl_stats_block(const uint8_t* __restrict__ tpl,
     unsigned long long a = wave_sum_u64(s[k]), b = wave_sum_u64(q[k]);
     if (lane == 0) {
       red[k][w] = a;
-      red[k][4 + w] = b;
+      red[k][8 + w] = b;
     }
   }
   __syncthreads();
@@ -90,7 +90,7 @@ __device__ __forceinline__ void tpl_stats_block(const uint8_t* __restrict__ tpl,
     unsigned long long a = 0, b = 0;
     for (int i = 0; i < (int)(blockDim.x >> 6); i++) {
       a += red[k][i];
-      b += red[k][4 + i];
+      b += red[k][8 + i];
     }
     atomicAdd(&st->tsum[k], a);
     atomicAdd(&st->tsq[k], b);
@@ -132,7 +132,7 @@ __device__ __forceinline__ void alpha_scan_block(const uint8_t* __restrict__ pix
 // a time, so a small template needs a workgroup or two and few atomics.
 __device__ __forceinline__ void tpl_stats_flat(const uint8_t* __restrict__ tpl, int64_t tstride, int tw, int th,
                                                int cnMax, int step, CallState* st, int b, int nb) {
-  __shared__ unsigned long long red[4][8];
+  __shared__ unsigned long long red[4][16];  // [k][w]: sums, [k][8 + w]: squares (<= 8 waves)
   unsigned long long s[4] = {0, 0, 0, 0}, q[4] = {0, 0, 0, 0};
   const int n = tw * th, stride = nb * (int)blockDim.x;
   for (int i0 = b * (int)blockDim.x + (int)threadIdx.x; i0 < n; i0 += 4 * stride) {
@@ -161,7 +161,7 @@ __device__ __forceinline__ void tpl_stats_flat(const uint8_t* __restrict__ tpl, 
     unsigned long long a = wave_sum_u64(s[k]), c = wave_sum_u64(q[k]);
     if (lane == 0) {
       red[k][wv] = a;
-      red[k][4 + wv] = c;
+      red[k][8 + wv] = c;
     }
   }
   __syncthreads();
@@ -170,7 +170,7 @@ __device__ __forceinline__ void tpl_stats_flat(const uint8_t* __restrict__ tpl, 
     unsigned long long a = 0, c = 0;
     for (int i = 0; i < (int)(blockDim.x >> 6); i++) {
       a += red[k][i];
-      c += red[k][4 + i];
+      c += red[k][8 + i];
     }
     atomicAdd(&st->tsum[k], a);
     atomicAdd(&st->tsq[k], c);

@@ -310,6 +310,9 @@ struct Ctx {
   // before the column pass)
   hipStream_t side = nullptr, side2 = nullptr;
   hipEvent_t fork = nullptr, join = nullptr, join2 = nullptr, join3 = nullptr;
+  // a host-buffer call's parent upload, overlapped with the call's own
+  // transforms on the stream (run_match_overlap)
+  hipStream_t cstream = nullptr;
   // the captured device-resident call (run_graphed)
   GraphSlot graph;
   DevBuf img, tpl, spec, tspec, tspec2, corr, parts, gray0, gray1, mapbuf, nscratch, dbf, pkscr, pkout;
@@ -334,6 +337,7 @@ struct Ctx {
     HIPCHK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
     HIPCHK(hipStreamCreateWithFlags(&side, hipStreamNonBlocking));
     HIPCHK(hipStreamCreateWithFlags(&side2, hipStreamNonBlocking));
+    HIPCHK(hipStreamCreateWithFlags(&cstream, hipStreamNonBlocking));
     HIPCHK(hipEventCreateWithFlags(&fork, hipEventDisableTiming));
     HIPCHK(hipEventCreateWithFlags(&join, hipEventDisableTiming));
     HIPCHK(hipEventCreateWithFlags(&join2, hipEventDisableTiming));
@@ -364,12 +368,12 @@ struct Ctx {
     if (res_d) (void)hipFree(res_d);
     if (res_h) (void)hipHostFree(res_h);
     if (rmap_h) (void)hipHostFree(rmap_h);
-    for (hipStream_t q : {stream, side, side2})
+    for (hipStream_t q : {stream, side, side2, cstream})
       if (q) (void)xfer_release_stream(q);
     for (hipEvent_t e : evpool) (void)hipEventDestroy(e);
     for (hipEvent_t e : {fork, join, join2, join3})
       if (e) (void)hipEventDestroy(e);
-    for (hipStream_t q : {stream, side, side2})
+    for (hipStream_t q : {stream, side, side2, cstream})
       if (q) (void)hipStreamDestroy(q);
   }
 
@@ -1002,6 +1006,117 @@ static int peaks_total_host(Ctx& X, hipStream_t s, int rw, int rh, int* tot) {
   return 0;
 }
 
+// Pageable host parents of at least this many bytes take the overlapped
+// form of a one-shot call (run_match_overlap); CVMATCH_OVERLAP=0 turns it
+// off.
+constexpr size_t kOverlapBytes = (size_t)8 << 20;
+static bool overlap_enabled() {
+  const char* e = getenv("CVMATCH_OVERLAP");
+  return !(e && e[0] == '0');
+}
+
+// matchU8 on a host parent whose upload overlaps the call's own transforms:
+// the parent moves on the context's copy stream chunk by chunk, and as soon
+// as the rows of a tile row are in, that tile row's forward rows and column
+// pass run on s (every channel, as the forked form's do: the channel count
+// of the alpha rule is not known before the whole parent is).  The template
+// (already on the device) gets its spectrum first; once the whole parent is
+// in, the call state, the inverse rows, the chunk sums, the normalisation
+// and the reduction follow on s.  Every kernel is the one the other forms
+// run on the same data, so the bits are theirs.
+static int run_match_overlap(Ctx& X, hipStream_t s, const uint8_t* img, size_t ib, int64_t istride, int iw, int ih,
+                             const uint8_t* d_tpl, int64_t tstride, int tw, int th, int cn_mode, int step,
+                             float* d_map) {
+  const int rw = iw - tw + 1, rh = ih - th + 1;
+  const Plan p = make_plan(tw, th, rw, rh);
+  CHK(check_limits(p, tw, rw, rh));
+  const int dh2 = shrink_dh(p, rh, th);
+  PlanTables pt;
+  CHK(plan_tables(X, p, dh2, pt));
+  const int cnMax = cn_mode ? cn_mode : 4;
+  TplSpec ts;
+  ts.p = p;
+  ts.dh2 = dh2;
+  ts.hwPad = round_up(p.hw, 16);
+  ts.cnMax = cnMax;
+  const int64_t tchan = (int64_t)p.dftH * ts.hwPad, tchan2 = (int64_t)dh2 * ts.hwPad;
+  CHK(X.tspec.ensure(sizeof(float2) * (size_t)tchan * cnMax));
+  ts.spec = X.tspec.as<float2>();
+  if (dh2) {
+    CHK(X.tspec2.ensure(sizeof(float2) * (size_t)tchan2 * cnMax));
+    ts.spec2 = X.tspec2.as<float2>();
+  }
+  const TileGrid g = parent_grid(p, tw, th, rw, rh, ts.hwPad, dh2);
+  const int ntiles = g.ntx * g.nty;
+  const int64_t chan = tchan, tileStride = chan * cnMax;
+  CHK(X.spec.ensure(sizeof(float2) * (size_t)tileStride * ntiles));
+  float2* spec = X.spec.as<float2>();
+  float* corr = d_map;
+  if (!corr) {
+    CHK(X.corr.ensure(sizeof(float) * (size_t)rw * rh));
+    corr = X.corr.as<float>();
+  }
+  CHK(X.parts.ensure(sizeof(Extrema) * (size_t)norm_partials_max(rw, rh, tw, th, cnMax)));
+  CHK(X.nscratch.ensure(norm_scratch_bytes(iw, ih, tw, th, cnMax)));
+  const uint8_t* d_img = X.img.as<uint8_t>();
+  CHK(build_tspec(X, s, nullptr, d_tpl, tstride, tw, th, step, cnMax, p, dh2, pt, ts.spec, ts.spec2));
+  int next = 0;  // the first tile row not yet started
+  auto on_chunk = [&](size_t done) -> int {
+    const int64_t rows = done >= ib ? (int64_t)ih : (int64_t)(done / (size_t)istride);
+    int ready = next;
+    while (ready < g.nty) {
+      const int y0 = ready * g.blockH;
+      const int bh = ready == g.nty - 1 ? rh - y0 : g.blockH;
+      if (std::min<int64_t>(ih, (int64_t)y0 + bh + th - 1) > rows) break;
+      ready++;
+    }
+    if (ready == next) return 0;
+    hipEvent_t e = X.ev();
+    if (!e) return 1;
+    if (hipEventRecord(e, X.cstream) != hipSuccess || hipStreamWaitEvent(s, e, 0) != hipSuccess) return 1;
+    X.evpool.push_back(e);  // reusable once recorded: the wait above already captured it
+    for (int r = next; r < ready; r++) {
+      launch_row_fwd_range(d_img, istride, step, cnMax, nullptr, g, pt, spec, tileStride, chan, r * g.ntx, g.ntx, s);
+      launch_cols_range(COL_CORR, spec, spec, tileStride, chan, cnMax, nullptr, g, pt, ts.spec, ts.spec2, tchan,
+                        tchan2, r * g.ntx, g.ntx, s);
+    }
+    next = ready;
+    return 0;
+  };
+  {
+    Stage S(X, "h2d+fft", s);
+    HIPCHK(xfer_h2d_chunks(X.img.p, img, ib, X.cstream, on_chunk));
+  }
+  if (next != g.nty) return fail(CVM_E_HIP, "cvmatch: overlapped upload left tile rows unstarted");
+  {
+    Stage S(X, "stats", s);
+    launch_state_init(X.st, s);
+    launch_call_state(d_img, istride, iw, ih, d_tpl, tstride, tw, th, cnMax, step, cn_mode, X.st, s);
+  }
+  {
+    Stage S(X, "row_inv", s);
+    launch_row_inv(spec, tileStride, chan, X.st, cnMax, g, pt, corr, rw, ntiles, s);
+  }
+  {
+    Stage S(X, "norm_prep", s);
+    launch_norm_prep(d_img, istride, iw, ih, step, tw, th, cnMax, X.st, X.nscratch.p, s);
+  }
+  const int q = norm_q_for(X, s, d_img, istride, iw, ih, step, tw, th, cnMax, X.st, corr, X.parts.as<Extrema>(),
+                           X.nscratch.p, false);
+  {
+    Stage S(X, "normalize", s);
+    launch_normalize(d_img, istride, iw, ih, step, tw, th, rw, rh, cnMax, X.st, corr, d_map, X.parts.as<Extrema>(),
+                     X.nscratch.p, s, false, q);
+  }
+  {
+    Stage S(X, "reduce", s);
+    launch_reduce(X.parts.as<Extrema>(), norm_partials_count(rw, rh, tw, th, cnMax, q), X.st, rw, res_out(X),
+                  nullptr, rh, s);
+  }
+  HIPCHK(hipGetLastError());
+  return 0;
+}
+
 static int match_host(const uint8_t* img, int64_t istride, int iw, int ih, const uint8_t* tpl, int64_t tstride,
                       int tw, int th, int cn_mode, int step, float* map, cvm_result* out, bool direct = false,
                       const PeakReq* pk = nullptr) {
@@ -1011,6 +1126,24 @@ static int match_host(const uint8_t* img, int64_t istride, int iw, int ih, const
   size_t ib = (size_t)img_bytes(istride, iw, ih, step), tb = (size_t)img_bytes(tstride, tw, th, step);
   CHK(X.img.ensure(ib));
   CHK(X.tpl.ensure(tb));
+  // (a pinned parent moves in one copy, so nothing would overlap it)
+  if (!direct && !pk && ib >= kOverlapBytes && overlap_enabled() && !xfer_h2d_source_busy(img, ib) &&
+      (cn_mode != 0 || (int64_t)tw * th <= stats_cap(4))) {
+    // the overlapped form: the template first, the parent chunk by chunk
+    HIPCHK(xfer_h2d(X.tpl.p, tpl, tb, s));
+    float* d_map = nullptr;
+    if (map) {
+      CHK(X.mapbuf.ensure(sizeof(float) * (size_t)rw * rh));
+      d_map = X.mapbuf.as<float>();
+    }
+    CHK(run_match_overlap(X, s, img, ib, istride, iw, ih, X.tpl.as<uint8_t>(), tstride, tw, th, cn_mode, step,
+                          d_map));
+    if (map) {
+      Stage S(X, "d2h", s);
+      HIPCHK(xfer_d2h(map, d_map, sizeof(float) * (size_t)rw * rh, s));
+    }
+    return finish_result(X, s, out);
+  }
   {
     Stage S(X, "h2d", s);
     HIPCHK(xfer_h2d(X.img.p, img, ib, s));

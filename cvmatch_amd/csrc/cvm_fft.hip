@@ -349,6 +349,7 @@ struct RowFwdArgs {
   float2* spec;
   int64_t tileStride, chanStride;
   int cpg, ngroups, npairs, words, cnAll;
+  int t0;   // first tile
   int nwg;  // workgroups: ngroups * npairs * ntiles
 };
 
@@ -363,7 +364,7 @@ __device__ __forceinline__ void row_fwd_body(float2* z, int L, const RowFwdArgs&
   const TileGrid& g = A.g;
   float2* __restrict__ spec = A.spec;
   const int cgi = L % ngroups, rest = L / ngroups;
-  const int rp = rest % npairs, t = rest / npairs;
+  const int rp = rest % npairs, t = A.t0 + rest / npairs;
   // without a call state every channel of the buffer is transformed
   if (st && st->flat) return;
   const int k0 = cgi * cpg;
@@ -468,6 +469,7 @@ static RowFwdArgs row_fwd_args(const uint8_t* img, int64_t istride, int step, in
   a.npairs = g.dftH / 2;
   a.words = (step == 4 && ((((uintptr_t)img) | (uintptr_t)istride) & 3) == 0) ? 1 : 0;
   a.cnAll = cnMax;
+  a.t0 = 0;
   a.nwg = a.ngroups * a.npairs * ntiles;
   return a;
 }
@@ -475,8 +477,9 @@ static RowFwdArgs row_fwd_args(const uint8_t* img, int64_t istride, int step, in
 template <int LOG2N>
 static void row_fwd_n(const uint8_t* img, int64_t istride, int step, int cnMax, const CallState* st,
                       const TileGrid& g, const PlanTables& pt, float2* spec, int64_t tileStride, int64_t chanStride,
-                      int ntiles, hipStream_t s) {
-  const RowFwdArgs a = row_fwd_args<LOG2N>(img, istride, step, cnMax, st, g, spec, tileStride, chanStride, ntiles);
+                      int ntiles, hipStream_t s, int t0 = 0) {
+  RowFwdArgs a = row_fwd_args<LOG2N>(img, istride, step, cnMax, st, g, spec, tileStride, chanStride, ntiles);
+  a.t0 = t0;
   size_t lds = sizeof(float2) * (size_t)padded_len(1 << LOG2N) * a.cpg;
   dim3 grid(a.nwg), block(RowNTF<LOG2N>::value);
   if (a.cpg <= 1)
@@ -493,6 +496,22 @@ void launch_row_fwd(const uint8_t* img, int64_t istride, int step, int cnMax, co
 #define CVM_RF(L) \
   case L:         \
     row_fwd_n<L>(img, istride, step, cnMax, st, g, pt, spec, tileStride, chanStride, ntiles, s); \
+    break
+  switch (g.log2W) {
+    CVM_RF(1); CVM_RF(2); CVM_RF(3); CVM_RF(4); CVM_RF(5); CVM_RF(6);
+    CVM_RF(7); CVM_RF(8); CVM_RF(9); CVM_RF(10); CVM_RF(11); CVM_RF(12);
+    default: break;
+  }
+#undef CVM_RF
+}
+
+// Tiles [t0, t0 + ntiles) only.
+void launch_row_fwd_range(const uint8_t* img, int64_t istride, int step, int cnMax, const CallState* st,
+                          const TileGrid& g, const PlanTables& pt, float2* spec, int64_t tileStride,
+                          int64_t chanStride, int t0, int ntiles, hipStream_t s) {
+#define CVM_RF(L) \
+  case L:         \
+    row_fwd_n<L>(img, istride, step, cnMax, st, g, pt, spec, tileStride, chanStride, ntiles, s, t0); \
     break
   switch (g.log2W) {
     CVM_RF(1); CVM_RF(2); CVM_RF(3); CVM_RF(4); CVM_RF(5); CVM_RF(6);
@@ -1009,6 +1028,21 @@ void launch_cols(ColMode mode, const float2* src, float2* dst, int64_t srcTileSt
   if (nshr > 0)
     cols_dispatch(g.log2H2, mode, src, dst, srcTileStride, dstTileStride, chanStride, cnMax, st, g, pt.triH2, tspec2,
                   tchanStride2, scale, nfull, nshr, s);
+}
+
+// Tiles [t0, t0 + ntiles) of one tile row only: the full-height transform,
+// or the shrunk band's for the last row when there is one.
+void launch_cols_range(ColMode mode, const float2* src, float2* dst, int64_t tileStride, int64_t chanStride,
+                       int cnMax, const CallState* st, const TileGrid& g, const PlanTables& pt, const float2* tspec,
+                       const float2* tspec2, int64_t tchanStride, int64_t tchanStride2, int t0, int ntiles,
+                       hipStream_t s) {
+  const bool shrunk = g.dftH2 != 0 && (t0 / g.ntx) == g.nty - 1;
+  if (shrunk)
+    cols_dispatch(g.log2H2, mode, src, dst, tileStride, tileStride, chanStride, cnMax, st, g, pt.triH2, tspec2,
+                  tchanStride2, 1.0f, t0, ntiles, s);
+  else
+    cols_dispatch(g.log2H, mode, src, dst, tileStride, tileStride, chanStride, cnMax, st, g, pt.triH, tspec,
+                  tchanStride, 1.0f, t0, ntiles, s);
 }
 
 template <int LOG2N>

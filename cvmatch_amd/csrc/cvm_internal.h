@@ -176,6 +176,16 @@ void launch_cols(ColMode mode, const float2* src, float2* dst, int64_t srcTileSt
                  int ntiles, hipStream_t s);
 
 // Shrunk-band template spectrum: rows r<<shift of the full one times 2^shift.
+// One tile row of a call (tiles [t0, t0 + ntiles)): its forward rows, and
+// its column pass (the shrunk band's transform for the last row of a plan
+// that has one).
+void launch_row_fwd_range(const uint8_t* img, int64_t istride, int step, int cnMax, const CallState* st,
+                          const TileGrid& g, const PlanTables& pt, float2* spec, int64_t tileStride,
+                          int64_t chanStride, int t0, int ntiles, hipStream_t s);
+void launch_cols_range(ColMode mode, const float2* src, float2* dst, int64_t tileStride, int64_t chanStride,
+                       int cnMax, const CallState* st, const TileGrid& g, const PlanTables& pt, const float2* tspec,
+                       const float2* tspec2, int64_t tchanStride, int64_t tchanStride2, int t0, int ntiles,
+                       hipStream_t s);
 void launch_shrink(const float2* tspec, float2* tspec2, int cnMax, int rows2, int shift, int hwPad,
                    int64_t chanStride, int64_t chanStride2, hipStream_t s);
 
@@ -317,6 +327,9 @@ int host_threads();
 hipError_t host_alloc(size_t bytes, void** out);
 hipError_t host_free(void* p);
 hipError_t xfer_h2d(void* dst, const void* src, size_t n, hipStream_t s);
+// xfer_h2d with done(bytes) called as each chunk's copy is enqueued on s.
+hipError_t xfer_h2d_chunks(void* dst, const void* src, size_t n, hipStream_t s,
+                           const std::function<int(size_t)>& done);
 bool xfer_h2d_source_busy(const void* src, size_t n);
 hipError_t xfer_release_stream(hipStream_t s);
 hipError_t xfer_d2h(void* dst, const void* src, size_t n, hipStream_t s);

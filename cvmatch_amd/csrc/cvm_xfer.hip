@@ -303,6 +303,36 @@ hipError_t xfer_h2d(void* dst, const void* src, size_t n, hipStream_t s) {
   return hipSuccess;
 }
 
+// xfer_h2d with the caller told of each chunk: done(bytes) runs once the
+// copy of the first `bytes` bytes is enqueued on s (after every staged chunk,
+// or once for the direct routes), so the caller can record an event on s
+// and start work on those bytes on another stream.  A nonzero return from
+// done ends the transfer with an error.
+hipError_t xfer_h2d_chunks(void* dst, const void* src, size_t n, hipStream_t s,
+                           const std::function<int(size_t)>& done) {
+  Staging& S = staging();
+  std::lock_guard<std::mutex> lk(S.mu);
+  if (n < Staging::kDirect || S.covers(src, n) || S.init() != hipSuccess) {
+    XCHK(hipMemcpyAsync(dst, src, n, hipMemcpyHostToDevice, s));
+    return done(n) ? hipErrorUnknown : hipSuccess;
+  }
+  const int threads = host_threads();
+  const size_t chunk = Staging::kChunk;  // smaller chunks measured slower (per-copy cost)
+  size_t c = 0;
+  for (size_t off = 0; off < n; off += chunk, c++) {
+    int k = (int)(c % Staging::kSlots);
+    size_t len = std::min(chunk, n - off);
+    XCHK(S.slot_free(k));
+    S.pool.copy(S.pin[k], (const uint8_t*)src + off, len, threads);
+    XCHK(hipMemcpyAsync((uint8_t*)dst + off, S.pin[k], len, hipMemcpyHostToDevice, s));
+    XCHK(hipEventRecord(S.ev[k], s));
+    S.live[k] = true;
+    S.on[k] = s;
+    if (done(off + len)) return hipErrorUnknown;
+  }
+  return hipSuccess;
+}
+
 // Returns with the destination written, except for a pinned destination,
 // whose copy is left in flight on stream s like any asynchronous copy.
 hipError_t xfer_d2h(void* dst, const void* src, size_t n, hipStream_t s) {

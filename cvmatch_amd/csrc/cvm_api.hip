@@ -335,9 +335,6 @@ struct Ctx {
 
   int init() {
     HIPCHK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
-    HIPCHK(hipStreamCreateWithFlags(&side, hipStreamNonBlocking));
-    HIPCHK(hipStreamCreateWithFlags(&side2, hipStreamNonBlocking));
-    HIPCHK(hipStreamCreateWithFlags(&cstream, hipStreamNonBlocking));
     HIPCHK(hipEventCreateWithFlags(&fork, hipEventDisableTiming));
     HIPCHK(hipEventCreateWithFlags(&join, hipEventDisableTiming));
     HIPCHK(hipEventCreateWithFlags(&join2, hipEventDisableTiming));
@@ -375,6 +372,20 @@ struct Ctx {
       if (e) (void)hipEventDestroy(e);
     for (hipStream_t q : {stream, side, side2, cstream})
       if (q) (void)hipStreamDestroy(q);
+  }
+
+  // The side streams and the copy stream are created on first use (an eager
+  // call, before any capture): every stream takes one of the device's few
+  // hardware queues in turn, and a context that never forks should not
+  // push another context's streams onto a shared queue.
+  int ensure_side() {
+    if (!side) HIPCHK(hipStreamCreateWithFlags(&side, hipStreamNonBlocking));
+    if (!side2) HIPCHK(hipStreamCreateWithFlags(&side2, hipStreamNonBlocking));
+    return 0;
+  }
+  int ensure_copy() {
+    if (!cstream) HIPCHK(hipStreamCreateWithFlags(&cstream, hipStreamNonBlocking));
+    return 0;
   }
 
   int tab(int n, const FftTables** out) { return engine().tab(n, out); }
@@ -915,6 +926,7 @@ static int run_match(Ctx& X, hipStream_t s, const uint8_t* d_img, int64_t istrid
   const TileGrid g = parent_grid(p, tw, th, rw, rh, ts.hwPad, dh2);
   if ((long long)iw * ih <= compact_max_pixels() && compact_ok(g) && norm_wave_ok(tw, th, cnMax))
     return run_compact(X, s, d_img, istride, iw, ih, d_tpl, tstride, tw, th, cn_mode, step, cnMax, ts, pt, g, d_map);
+  CHK(X.ensure_side());
   // while s transforms the parent rows, the call state and then the
   // normalisation partials are built on one side stream and the template
   // spectrum (every channel: it needs no call state) on the other
@@ -1059,6 +1071,7 @@ static int run_match_overlap(Ctx& X, hipStream_t s, const uint8_t* img, size_t i
   CHK(X.parts.ensure(sizeof(Extrema) * (size_t)norm_partials_max(rw, rh, tw, th, cnMax)));
   CHK(X.nscratch.ensure(norm_scratch_bytes(iw, ih, tw, th, cnMax)));
   const uint8_t* d_img = X.img.as<uint8_t>();
+  CHK(X.ensure_copy());
   CHK(build_tspec(X, s, nullptr, d_tpl, tstride, tw, th, step, cnMax, p, dh2, pt, ts.spec, ts.spec2));
   int next = 0;  // the first tile row not yet started
   auto on_chunk = [&](size_t done) -> int {

@@ -324,6 +324,21 @@ template <int LOG2N>
 struct RowNTF {
   static constexpr int value = LOG2N >= 11 ? 512 : RowNT<LOG2N>::value;
 };
+#ifndef CVM_ROWINV_WIDE
+#define CVM_ROWINV_WIDE 1
+#endif
+template <int LOG2N>
+struct RowNTI {
+  static constexpr int value = (CVM_ROWINV_WIDE && LOG2N >= 11) ? 512 : RowNT<LOG2N>::value;
+};
+// Waves a SIMD the inverse row kernel is compiled for: two 512-thread
+// workgroups of 2048 points fit a CU's LDS, so their registers are held to
+// 128 (124 used, no spills); 4096 points fit one workgroup a CU.
+template <int LOG2N>
+struct RowInvWaves {
+  static constexpr int value = (RowNTI<LOG2N>::value == 512 && LOG2N == 11) ? 4 : 1;
+};
+static int row_inv_nt(int log2n) { return (CVM_ROWINV_WIDE && log2n >= 11) ? 512 : log2n >= 10 ? 256 : 64; }
 
 // Channels per workgroup for row transforms of length n: as many as fit a
 // 144 KiB LDS budget (gfx950 gives a workgroup up to 160 KiB), so one
@@ -1112,8 +1127,8 @@ struct RowInvArgs {
 // Workgroup L of an inverse row launch; z: the LDS vectors.
 template <int LOG2N, int NVEC>
 __device__ __forceinline__ void row_inv_body(float2* z, int L, const RowInvArgs& A, const float2* __restrict__ triW) {
-  constexpr int n = 1 << LOG2N, NT = RowNT<LOG2N>::value, np = n + (n >> 5);
-  constexpr int MAXI = (n + NT - 1) / NT;
+  constexpr int n = 1 << LOG2N, NT = RowNTI<LOG2N>::value, np = n + (n >> 5);
+  constexpr int MAXI = (n + NT - 1) / NT, H = n / 2, MAXH = (H + NT - 1) / NT;
   const float2* __restrict__ spec = A.spec;
   const int64_t tileStride = A.tileStride, chanStride = A.chanStride;
   const CallState* __restrict__ st = A.st;
@@ -1126,7 +1141,7 @@ __device__ __forceinline__ void row_inv_body(float2* z, int L, const RowInvArgs&
   int x0, y0, bw, bh;
   tile_rect(g, t, x0, y0, bw, bh);
   if (r >= bh) return;
-  const int cn = st->cn, tid = threadIdx.x, hw = g.hw;
+  const int cn = st->cn, tid = threadIdx.x;
   float2 acc[MAXI];
 #pragma unroll
   for (int i = 0; i < MAXI; i++) acc[i] = make_float2(0.f, 0.f);
@@ -1138,26 +1153,32 @@ __device__ __forceinline__ void row_inv_body(float2* z, int L, const RowInvArgs&
       const float2* sa = spec + (int64_t)t * tileStride + (int64_t)(k0 + v) * chanStride + (int64_t)r * g.hwPad;
       const float2* sb = sa + g.hwPad;
       float2* zv = z + v * np;
-      float2 A[MAXI], B[MAXI];
+      // spectrum column m < n/2 feeds output bins m and n - m (the
+      // conjugate-symmetric half), column n/2 bin n/2 alone: each column
+      // is loaded once
+      float2 A[MAXH], B[MAXH], Ah, Bh;
 #pragma unroll
-      for (int i = 0; i < MAXI; i++) {
-        int q = tid + i * NT;
-        int m = q < hw ? q : n - q;
+      for (int i = 0; i < MAXH; i++) {
+        const int m = tid + i * NT;
         A[i] = B[i] = make_float2(0.f, 0.f);
-        if (q < n) {
+        if (H % NT == 0 || m < H) {
           A[i] = sa[m];
           B[i] = sb[m];
         }
       }
+      if (tid == 0) {
+        Ah = sa[H];
+        Bh = sb[H];
+      }
 #pragma unroll
-      for (int i = 0; i < MAXI; i++) {
-        int q = tid + i * NT;
-        if (q < n) {
-          float2 w = q < hw ? make_float2(A[i].x - B[i].y, A[i].y + B[i].x)
-                            : make_float2(A[i].x + B[i].y, B[i].x - A[i].y);
-          zv[padi(bitrev_idx(q, LOG2N))] = w;
+      for (int i = 0; i < MAXH; i++) {
+        const int m = tid + i * NT;
+        if (H % NT == 0 || m < H) {
+          zv[padi(bitrev_idx(m, LOG2N))] = make_float2(A[i].x - B[i].y, A[i].y + B[i].x);
+          if (m > 0) zv[padi(bitrev_idx(n - m, LOG2N))] = make_float2(A[i].x + B[i].y, B[i].x - A[i].y);
         }
       }
+      if (tid == 0) zv[padi(bitrev_idx(H, LOG2N))] = make_float2(Ah.x - Bh.y, Ah.y + Bh.x);
     }
     __syncthreads();
     fft_lds<LOG2N, NVEC, NT, 1, true, false>(z, nv, np, triW, true);
@@ -1193,7 +1214,7 @@ __device__ __forceinline__ void row_inv_body(float2* z, int L, const RowInvArgs&
 }
 
 template <int LOG2N, int NVEC>
-__global__ __launch_bounds__(RowNT<LOG2N>::value) void k_row_inv(RowInvArgs A, const float2* __restrict__ triW) {
+__global__ __launch_bounds__(RowNTI<LOG2N>::value) __attribute__((amdgpu_waves_per_eu(RowInvWaves<LOG2N>::value))) void k_row_inv(RowInvArgs A, const float2* __restrict__ triW) {
   extern __shared__ float2 z[];
   row_inv_body<LOG2N, NVEC>(z, blockIdx.x, A, triW);
 }
@@ -1225,7 +1246,7 @@ static void row_inv_n(const float2* spec, int64_t tileStride, int64_t chanStride
                       const TileGrid& g, const PlanTables& pt, float* corr, int rw, int ntiles, hipStream_t s) {
   const RowInvArgs a = row_inv_args<LOG2N>(spec, tileStride, chanStride, st, cnMax, g, corr, rw, ntiles);
   size_t lds = sizeof(float2) * (size_t)padded_len(1 << LOG2N) * a.cpg;
-  dim3 grid(a.nwg), block(RowNT<LOG2N>::value);
+  dim3 grid(a.nwg), block(RowNTI<LOG2N>::value);
   if (a.cpg <= 1)
     hipLaunchKernelGGL((k_row_inv<LOG2N, 1>), grid, block, lds, s, a, pt.triW);
   else if (a.cpg == 2)
@@ -1362,7 +1383,7 @@ __global__ __launch_bounds__(NT) void k_corr_chunks(ColArgs C, ChunkArgs K, Fina
 
 // The inverse row pass and the chunk prefix sums.
 template <int LOG2N, int NVEC>
-__global__ __launch_bounds__(RowNT<LOG2N>::value) void k_rowinv_prefix(RowInvArgs R, PrefixArgs Q,
+__global__ __launch_bounds__(RowNTI<LOG2N>::value) __attribute__((amdgpu_waves_per_eu(RowInvWaves<LOG2N>::value))) void k_rowinv_prefix(RowInvArgs R, PrefixArgs Q,
                                                                         const float2* __restrict__ triW) {
   extern __shared__ float2 z[];
   if ((int)blockIdx.x < R.nwg)
@@ -1379,7 +1400,7 @@ static void compact_rows(const CompactCall& c, const RowFwdArgs& P, const RowFwd
     hipLaunchKernelGGL((k_front<LOG2W, NVEC>), dim3(P.nwg + T.nwg + 1), dim3(RowNTF<LOG2W>::value), lds, s, P, T,
                        c.pt.triW, c.st);
   else
-    hipLaunchKernelGGL((k_rowinv_prefix<LOG2W, NVEC>), dim3(R.nwg + Q.nwg), dim3(RowNT<LOG2W>::value), lds, s, R, Q,
+    hipLaunchKernelGGL((k_rowinv_prefix<LOG2W, NVEC>), dim3(R.nwg + Q.nwg), dim3(RowNTI<LOG2W>::value), lds, s, R, Q,
                        c.pt.triW);
 }
 
@@ -1496,7 +1517,7 @@ void launch_compact_fft(const CompactCall& c, hipStream_t s) {
   Q.nch = nch;
   Q.st = c.st;
   Q.ps = (uint32_t*)c.nscratch;
-  const int nt = c.g.log2W >= 10 ? 256 : 64;  // the inverse row launch's workgroup size
+  const int nt = row_inv_nt(c.g.log2W);  // the inverse row launch's workgroup size
   Q.nwg = (Q.nwords + nt - 1) / nt;
   compact_rows_dispatch(c, true, Q, s);         // parent + template rows, state reset
   compact_cols_dispatch(c, c.csa, K, true, s);  // template columns + state statistics

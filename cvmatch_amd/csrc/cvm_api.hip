@@ -318,6 +318,10 @@ struct Ctx {
   DevBuf img, tpl, spec, tspec, tspec2, corr, parts, gray0, gray1, mapbuf, nscratch, dbf, pkscr, pkout;
   DevBuf gate;  // the mixed form's two trial maps
   CallState* st = nullptr;
+  // the compact form's call state: its accumulators are zero between
+  // compact calls (zeroed here, and cleared again by each call's final
+  // reduction once nothing reads them), so a call needs no reset launch
+  CallState* cst = nullptr;
   Res6* res_d = nullptr;
   Res6* res_h = nullptr;
   // a single-result call's six-tuple (Match family, Find) in coherent host
@@ -340,6 +344,8 @@ struct Ctx {
     HIPCHK(hipEventCreateWithFlags(&join2, hipEventDisableTiming));
     HIPCHK(hipEventCreateWithFlags(&join3, hipEventDisableTiming));
     HIPCHK(hipMalloc(&st, sizeof(CallState)));
+    HIPCHK(hipMalloc(&cst, sizeof(CallState)));
+    HIPCHK(hipMemset(cst, 0, sizeof(CallState)));
     HIPCHK(hipMalloc(&res_d, sizeof(Res6) * 2));
     HIPCHK(hipHostMalloc(&res_h, sizeof(Res6) * 2, hipHostMallocDefault));
     if (hipHostMalloc((void**)&rmap_h, sizeof(Res6), hipHostMallocCoherent | hipHostMallocMapped) == hipSuccess) {
@@ -362,6 +368,7 @@ struct Ctx {
                       &pkscr, &pkout, &gate})
       b->release();
     if (st) (void)hipFree(st);
+    if (cst) (void)hipFree(cst);
     if (res_d) (void)hipFree(res_d);
     if (res_h) (void)hipHostFree(res_h);
     if (rmap_h) (void)hipHostFree(rmap_h);
@@ -875,25 +882,25 @@ static int run_compact(Ctx& X, hipStream_t s, const uint8_t* d_img, int64_t istr
   c.tchan = chan;
   c.scale = 1.0f / ((float)p.dftW * (float)p.dftH);
   c.pt = pt;
-  c.st = X.st;
-  c.csa = call_state_args(d_img, istride, iw, ih, d_tpl, tstride, tw, th, cnMax, step, cn_mode, X.st, true);
+  c.st = X.cst;
+  c.csa = call_state_args(d_img, istride, iw, ih, d_tpl, tstride, tw, th, cnMax, step, cn_mode, X.cst, true);
   c.corr = corr;
   c.nscratch = X.nscratch.p;
   {
     Stage S(X, "compact", s);
     launch_compact_fft(c, s);
   }
-  const int q = norm_q_for(X, s, d_img, istride, iw, ih, step, tw, th, cnMax, X.st, corr, X.parts.as<Extrema>(),
+  const int q = norm_q_for(X, s, d_img, istride, iw, ih, step, tw, th, cnMax, X.cst, corr, X.parts.as<Extrema>(),
                            X.nscratch.p, false);
   {
     Stage S(X, "normalize", s);
-    launch_normalize(d_img, istride, iw, ih, step, tw, th, rw, rh, cnMax, X.st, corr, d_map, X.parts.as<Extrema>(),
+    launch_normalize(d_img, istride, iw, ih, step, tw, th, rw, rh, cnMax, X.cst, corr, d_map, X.parts.as<Extrema>(),
                      X.nscratch.p, s, false, q);
   }
   {
     Stage S(X, "reduce", s);
-    launch_reduce(X.parts.as<Extrema>(), norm_partials_count(rw, rh, tw, th, cnMax, q), X.st, rw, res_out(X),
-                  nullptr, rh, s);
+    launch_reduce_clear(X.parts.as<Extrema>(), norm_partials_count(rw, rh, tw, th, cnMax, q), rw, res_out(X), X.cst,
+                        s);
   }
   HIPCHK(hipGetLastError());
   return 0;

@@ -631,6 +631,9 @@ __device__ __forceinline__ void col_inverse_store(float2* a, const float2* __res
 //   COL_FWD   : rows (pad rows = 0) -> forward, all rows stored
 //   COL_CORR  : rows -> forward -> * conj(tspec) -> inverse, needed rows
 //   COL_MULINV: forward rows -> * conj(tspec) -> inverse, needed rows
+//   COL_CORRT : as COL_CORR, the template spectrum's slab first computed in
+//               the workgroup (COL_TSPEC's values, kept in registers) from
+//               the template's row spectra trows on its grid *gtp
 // The forward input in bit-reversed row order (blockForwardGo
 // impl.go:350-374) is read straight from the spectrum by the first pass:
 // transform index i is source row brev(i).  The product for frequency k
@@ -648,8 +651,10 @@ __device__ __forceinline__ void cols_body(float2* a, int L, const float2* __rest
                                           float2* __restrict__ dst, int64_t srcTileStride, int64_t dstTileStride,
                                           int64_t chanStride, const CallState* __restrict__ st, const TileGrid& g,
                                           const float2* __restrict__ tri, const float2* __restrict__ tspec,
-                                          int64_t tchanStride, float scale, int t0, int ntiles, int nslabs) {
+                                          int64_t tchanStride, float scale, int t0, int ntiles, int nslabs,
+                                          const TileGrid* gtp = nullptr, const float2* __restrict__ trows = nullptr) {
   constexpr int n = 1 << LOG2N, CG = ColCG<LOG2N>::value, LCG = ceil_log2c(CG);
+  constexpr bool CORR = MODE == COL_CORR || MODE == COL_CORRT;
   using F = ColFirst<LOG2N, NST1>;
   constexpr int NB1 = F::NB, E1 = 1 << NB1;
   constexpr int G1 = CG << (LOG2N - NB1), IT1 = (G1 + NT - 1) / NT;
@@ -675,27 +680,10 @@ __device__ __forceinline__ void cols_body(float2* a, int L, const float2* __rest
   int need = (bh + 1) & ~1;
   if (need > n) need = n;
 
-  // template spectrum for the elements this thread owns in the last
-  // forward pass
-  float2 tsv[MODE == COL_CORR ? ITL : 1][MODE == COL_CORR ? EL : 1];
-  if constexpr (MODE == COL_CORR) {
-#pragma unroll
-    for (int it = 0; it < ITL; it++) {
-      const int gi = tid + it * NT, v = gi & (CG - 1), gg = gi >> LCG;
-      const int base = grp_base<LOL, NBL>(gg);
-#pragma unroll
-      for (int e = 0; e < EL; e++) {
-        const int idx = base | (e << LOL);
-        tsv[it][e] = make_float2(0.f, 0.f);
-        if ((GL % NT == 0 || gi < GL) && v < ncol) tsv[it][e] = ts[(int64_t)idx * hwPad + v];
-      }
-    }
-  }
-
-  float2 rl[ITL][EL];
-  if constexpr (MODE != COL_MULINV) {
-    int loaded = (bh + g.padH + 1) & ~1;
-    if (loaded > n) loaded = n;
+  // The forward transform of the slab whose first row is at spp (rows from
+  // `loaded` on read as zero), left in registers: out[it][e] is index
+  // grp_base(gg) | (e << LOL) of this thread's last-pass group.
+  auto forward = [&](const float2* __restrict__ spp, int loaded, float2 (&out)[ITL][EL]) {
     // first forward pass straight from the spectrum rows
     float2 r1[IT1][E1];
 #pragma unroll
@@ -706,7 +694,7 @@ __device__ __forceinline__ void cols_body(float2* a, int L, const float2* __rest
       for (int e = 0; e < E1; e++) {
         const int row = bitrev_idx((gg << NB1) | e, LOG2N);
         r1[it][e] = make_float2(0.f, 0.f);
-        if (ok && row < loaded) r1[it][e] = sp[(int64_t)row * hwPad + v];
+        if (ok && row < loaded) r1[it][e] = spp[(int64_t)row * hwPad + v];
       }
     }
 #pragma unroll
@@ -715,7 +703,7 @@ __device__ __forceinline__ void cols_body(float2* a, int L, const float2* __rest
 #pragma unroll
       for (int it = 0; it < ITL; it++)
 #pragma unroll
-        for (int e = 0; e < EL; e++) rl[it][e] = r1[it][e];
+        for (int e = 0; e < EL; e++) out[it][e] = r1[it][e];
     } else {
 #pragma unroll
       for (int it = 0; it < IT1; it++) {
@@ -732,11 +720,53 @@ __device__ __forceinline__ void cols_body(float2* a, int L, const float2* __rest
         const int base = grp_base<LOL, NBL>(gg);
         if (GL % NT == 0 || gi < GL) {
 #pragma unroll
-          for (int e = 0; e < EL; e++) rl[it][e] = a[(base | (e << LOL)) * CG + v];
-          bfly_regs<LOG2N, LOL, 0, NSTL>(rl[it], gg & ((1 << LOL) - 1), tri, false);
+          for (int e = 0; e < EL; e++) out[it][e] = a[(base | (e << LOL)) * CG + v];
+          bfly_regs<LOG2N, LOL, 0, NSTL>(out[it], gg & ((1 << LOL) - 1), tri, false);
         }
       }
     }
+  };
+
+  // template spectrum for the elements this thread owns in the last
+  // forward pass
+  float2 tsv[CORR ? ITL : 1][CORR ? EL : 1];
+  if constexpr (MODE == COL_CORR) {
+#pragma unroll
+    for (int it = 0; it < ITL; it++) {
+      const int gi = tid + it * NT, v = gi & (CG - 1), gg = gi >> LCG;
+      const int base = grp_base<LOL, NBL>(gg);
+#pragma unroll
+      for (int e = 0; e < EL; e++) {
+        const int idx = base | (e << LOL);
+        tsv[it][e] = make_float2(0.f, 0.f);
+        if ((GL % NT == 0 || gi < GL) && v < ncol) tsv[it][e] = ts[(int64_t)idx * hwPad + v];
+      }
+    }
+  }
+  if constexpr (MODE == COL_CORRT) {
+    // the template's slab of channel k: its spectrum values scaled as
+    // COL_TSPEC stores them; the slab in LDS is then free again
+    int tx0, ty0, tbw, tbh;
+    tile_rect(*gtp, 0, tx0, ty0, tbw, tbh);
+    int tl = (tbh + gtp->padH + 1) & ~1;
+    if (tl > n) tl = n;
+#pragma unroll
+    for (int it = 0; it < ITL; it++)
+#pragma unroll
+      for (int e = 0; e < EL; e++) tsv[it][e] = make_float2(0.f, 0.f);
+    forward(trows + (int64_t)k * tchanStride + c0, tl, tsv);
+#pragma unroll
+    for (int it = 0; it < ITL; it++)
+#pragma unroll
+      for (int e = 0; e < EL; e++) tsv[it][e] = make_float2(tsv[it][e].x * scale, tsv[it][e].y * scale);
+    __syncthreads();
+  }
+
+  float2 rl[ITL][EL];
+  if constexpr (MODE != COL_MULINV) {
+    int loaded = (bh + g.padH + 1) & ~1;
+    if (loaded > n) loaded = n;
+    forward(sp, loaded, rl);
     if constexpr (MODE == COL_TSPEC || MODE == COL_FWD) {
 #pragma unroll
       for (int it = 0; it < ITL; it++) {
@@ -1292,12 +1322,15 @@ struct ColArgs {
   float scale;
   int t0, ntiles, nslabs;
   int nwg;
+  TileGrid gt;           // COL_CORRT: the template's grid
+  const float2* trows;   // COL_CORRT: the template's row spectra
 };
 
 template <int LOG2N, int MODE, int NT = 256, int NST1 = -1>
 __device__ __forceinline__ void cols_job(float2* a, int p, const ColArgs& C) {
-  cols_body<LOG2N, MODE, NT, NST1>(a, xcd_remap(p, C.nwg), C.src, C.dst, C.srcTileStride, C.dstTileStride, C.chanStride, C.st,
-                         C.g, C.tri, C.tspec, C.tchanStride, C.scale, C.t0, C.ntiles, C.nslabs);
+  cols_body<LOG2N, MODE, NT, NST1>(a, xcd_remap(p, C.nwg), C.src, C.dst, C.srcTileStride, C.dstTileStride,
+                                   C.chanStride, C.st, C.g, C.tri, C.tspec, C.tchanStride, C.scale, C.t0, C.ntiles,
+                                   C.nslabs, &C.gt, C.trows);
 }
 
 // The chunk column sums (a nbx x nch grid of 256-thread workgroups).
@@ -1319,10 +1352,12 @@ struct PrefixArgs {
 };
 
 // The parent's and the template's forward row transforms (every channel)
-// and the call state's reset.
+// and the call state's independent parts (template statistics, alpha
+// scans), accumulated into a state whose accumulators the previous compact
+// call's reduction left zero.
 template <int LOG2N, int NVEC>
 __global__ __launch_bounds__(RowNTF<LOG2N>::value) void k_front(RowFwdArgs P, RowFwdArgs T,
-                                                                const float2* __restrict__ triW, CallState* st) {
+                                                                const float2* __restrict__ triW, CallStateArgs S) {
   extern __shared__ float2 z[];
   const int b = blockIdx.x;
   if (b < P.nwg)
@@ -1330,18 +1365,7 @@ __global__ __launch_bounds__(RowNTF<LOG2N>::value) void k_front(RowFwdArgs P, Ro
   else if (b < P.nwg + T.nwg)
     row_fwd_body<LOG2N, NVEC>(z, b - P.nwg, T, triW);
   else
-    state_clear(st);
-}
-
-// The template's column pass (its spectrum) and the call state's
-// independent parts (template statistics, alpha scans).
-template <int LOG2N, int NT = 256, int NST1 = -1>
-__global__ __launch_bounds__(NT) void k_tspec_state(ColArgs C, CallStateArgs S) {
-  extern __shared__ float2 a[];
-  if ((int)blockIdx.x < C.nwg)
-    cols_job<LOG2N, COL_TSPEC, NT, NST1>(a, blockIdx.x, C);
-  else
-    call_state_block(blockIdx.x - C.nwg, S);
+    call_state_block(b - P.nwg - T.nwg, S);
 }
 
 // The state's finalisation as one launch's prologue: each workgroup
@@ -1352,7 +1376,8 @@ struct FinalizeArgs {
   int tw, th, cn_mode;
 };
 
-// The correlation's column pass and the chunk column sums, after the
+// The correlation's column pass (each workgroup first transforms the
+// template's slab it multiplies by) and the chunk column sums, after the
 // state's finalisation.
 template <int LOG2N, int NT = 256, int NST1 = -1>
 __global__ __launch_bounds__(NT) void k_corr_chunks(ColArgs C, ChunkArgs K, FinalizeArgs F) {
@@ -1373,7 +1398,7 @@ __global__ __launch_bounds__(NT) void k_corr_chunks(ColArgs C, ChunkArgs K, Fina
   __syncthreads();
   if ((int)blockIdx.x < C.nwg) {
     C.st = &sst;
-    cols_job<LOG2N, COL_CORR, NT, NST1>(a, blockIdx.x, C);
+    cols_job<LOG2N, COL_CORRT, NT, NST1>(a, blockIdx.x, C);
   } else {
     const int b = blockIdx.x - C.nwg;
     colsum_chunks_at((b % K.nbx) * NT + threadIdx.x, b / K.nbx, K.img, K.istride, K.iw, K.ih, K.step, K.words, &sst,
@@ -1397,8 +1422,8 @@ static void compact_rows(const CompactCall& c, const RowFwdArgs& P, const RowFwd
                          const PrefixArgs& Q, bool front, hipStream_t s) {
   const size_t lds = sizeof(float2) * (size_t)padded_len(1 << LOG2W) * P.cpg;
   if (front)
-    hipLaunchKernelGGL((k_front<LOG2W, NVEC>), dim3(P.nwg + T.nwg + 1), dim3(RowNTF<LOG2W>::value), lds, s, P, T,
-                       c.pt.triW, c.st);
+    hipLaunchKernelGGL((k_front<LOG2W, NVEC>), dim3(P.nwg + T.nwg + c.csa.nb()), dim3(RowNTF<LOG2W>::value), lds, s,
+                       P, T, c.pt.triW, c.csa);
   else
     hipLaunchKernelGGL((k_rowinv_prefix<LOG2W, NVEC>), dim3(R.nwg + Q.nwg), dim3(RowNTI<LOG2W>::value), lds, s, R, Q,
                        c.pt.triW);
@@ -1432,8 +1457,7 @@ static void compact_rows_dispatch(const CompactCall& c, bool front, const Prefix
 }
 
 template <int LOG2H, int NT = 256, int NST1 = -1>
-static void compact_cols(const CompactCall& c, const CallStateArgs& S, ChunkArgs K, bool tspec_state,
-                         hipStream_t s) {
+static void compact_cols(const CompactCall& c, ChunkArgs K, hipStream_t s) {
   const FinalizeArgs F{c.st, c.tw, c.th, c.cn_mode};
   K.nbx = (c.iw + NT - 1) / NT;
   constexpr int CG = ColCG<LOG2H>::value;
@@ -1443,40 +1467,28 @@ static void compact_cols(const CompactCall& c, const CallStateArgs& S, ChunkArgs
   C.tri = c.pt.triH;
   C.t0 = 0;
   C.nslabs = nslabs;
-  C.chanStride = tspec_state ? c.tchan : c.chanStride;
-  if (tspec_state) {
-    // the template's spectrum, every channel: pad rows zero, forward, scaled
-    C.src = c.tspec;
-    C.dst = c.tspec;
-    C.srcTileStride = C.dstTileStride = 0;
-    C.st = nullptr;
-    C.g = c.gt;
-    C.tspec = nullptr;
-    C.tchanStride = 0;
-    C.scale = c.scale;
-    C.ntiles = 1;
-    C.nwg = nslabs * c.cnMax;
-    hipLaunchKernelGGL((k_tspec_state<LOG2H, NT, NST1>), dim3(C.nwg + S.nb()), dim3(NT), lds, s, C, S);
-  } else {
-    C.src = c.spec;
-    C.dst = c.spec;
-    C.srcTileStride = C.dstTileStride = c.tileStride;
-    C.st = c.st;
-    C.g = c.g;
-    C.tspec = c.tspec;
-    C.tchanStride = c.tchan;
-    C.scale = 1.0f;
-    C.ntiles = c.ntiles;
-    C.nwg = nslabs * c.ntiles * c.cnMax;
-    hipLaunchKernelGGL((k_corr_chunks<LOG2H, NT, NST1>), dim3(C.nwg + K.nbx * K.nch), dim3(NT), lds, s, C, K, F);
-  }
+  C.chanStride = c.chanStride;
+  C.src = c.spec;
+  C.dst = c.spec;
+  C.srcTileStride = C.dstTileStride = c.tileStride;
+  C.st = c.st;
+  C.g = c.g;
+  // the template's row spectra; each workgroup transforms its slab, pad rows
+  // zero, scaled
+  C.gt = c.gt;
+  C.trows = c.tspec;
+  C.tspec = c.tspec;
+  C.tchanStride = c.tchan;
+  C.scale = c.scale;
+  C.ntiles = c.ntiles;
+  C.nwg = nslabs * c.ntiles * c.cnMax;
+  hipLaunchKernelGGL((k_corr_chunks<LOG2H, NT, NST1>), dim3(C.nwg + K.nbx * K.nch), dim3(NT), lds, s, C, K, F);
 }
 
-static void compact_cols_dispatch(const CompactCall& c, const CallStateArgs& S, const ChunkArgs& K, bool tspec_state,
-                                  hipStream_t s) {
-#define CVM_CC(L)                                      \
-  case L:                                              \
-    compact_cols<L>(c, S, K, tspec_state, s);          \
+static void compact_cols_dispatch(const CompactCall& c, const ChunkArgs& K, hipStream_t s) {
+#define CVM_CC(L)              \
+  case L:                      \
+    compact_cols<L>(c, K, s);  \
     break
   // 512-row columns in the 512-thread form (one radix-4 stage in the first
   // pass: twice the waves a workgroup, each with half the points; 640x480
@@ -1488,9 +1500,9 @@ static void compact_cols_dispatch(const CompactCall& c, const CallStateArgs& S, 
     CVM_CC(5); CVM_CC(6); CVM_CC(7); CVM_CC(8); CVM_CC(10); CVM_CC(11);
     case 9:
       if (wide)
-        compact_cols<9, 512, 1>(c, S, K, tspec_state, s);
+        compact_cols<9, 512, 1>(c, K, s);
       else
-        compact_cols<9>(c, S, K, tspec_state, s);
+        compact_cols<9>(c, K, s);
       break;
     default: break;
   }
@@ -1519,10 +1531,9 @@ void launch_compact_fft(const CompactCall& c, hipStream_t s) {
   Q.ps = (uint32_t*)c.nscratch;
   const int nt = row_inv_nt(c.g.log2W);  // the inverse row launch's workgroup size
   Q.nwg = (Q.nwords + nt - 1) / nt;
-  compact_rows_dispatch(c, true, Q, s);         // parent + template rows, state reset
-  compact_cols_dispatch(c, c.csa, K, true, s);  // template columns + state statistics
-  compact_cols_dispatch(c, c.csa, K, false, s);  // state finalised; correlation columns + chunk sums
-  compact_rows_dispatch(c, false, Q, s);        // inverse rows + chunk prefix sums
+  compact_rows_dispatch(c, true, Q, s);   // parent + template rows, state statistics
+  compact_cols_dispatch(c, K, s);         // state finalised; template + correlation columns, chunk sums
+  compact_rows_dispatch(c, false, Q, s);  // inverse rows + chunk prefix sums
 }
 
 }  // namespace cvm

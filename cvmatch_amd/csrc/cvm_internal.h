@@ -144,7 +144,7 @@ int tri_len(int n);
 void launch_twiddles(float2* tri, int32_t* brev, int n, hipStream_t s);
 
 // Spectrum column modes.
-enum ColMode { COL_TSPEC = 0, COL_CORR = 1, COL_FWD = 2, COL_MULINV = 3 };
+enum ColMode { COL_TSPEC = 0, COL_CORR = 1, COL_FWD = 2, COL_MULINV = 3, COL_CORRT = 4 };
 
 // One member of a paired column launch (two Fleet members of one bucket,
 // COL_MULINV, full-height tiles): its call state, template spectrum, product
@@ -308,6 +308,8 @@ int launch_norm_batch(const uint8_t* img, int64_t istride, int iw, int ih, int s
                       const NormJob* jobs, const int* wg_job, int nwg, hipStream_t s);
 // one workgroup per job: its extrema partials -> its Res6
 void launch_reduce_batch(const NormJob* jobs, int njobs, hipStream_t s);
+// launch_reduce for the compact form, clearing st's accumulators afterwards.
+void launch_reduce_clear(const Extrema* partials, int nparts, int rw, void* res6, CallState* st, hipStream_t s);
 void launch_reduce(const Extrema* partials, int nparts, const CallState* st, int rw, void* res6, float* map_if_flat,
                    int rh, hipStream_t s);
 

@@ -1072,6 +1072,18 @@ __global__ __launch_bounds__(1024) void k_reduce_batch(const NormJob* __restrict
   reduce_block(J.parts, J.gx * J.gy, J.rw, (Res6*)J.res);
 }
 
+// The compact call's reduction, which also clears its state's accumulators
+// for the next compact call (nothing reads them after the normalisation).
+__global__ __launch_bounds__(1024) void k_reduce_clear(const Extrema* __restrict__ parts, int n, int rw, Res6* out,
+                                                       CallState* st) {
+  reduce_block(parts, n, rw, out);
+  if (threadIdx.x == 0) state_clear(st);
+}
+
+void launch_reduce_clear(const Extrema* partials, int nparts, int rw, void* res6, CallState* st, hipStream_t s) {
+  hipLaunchKernelGGL(k_reduce_clear, dim3(1), dim3(1024), 0, s, partials, nparts, rw, (Res6*)res6, st);
+}
+
 void launch_reduce_batch(const NormJob* jobs, int njobs, hipStream_t s) {
   if (njobs > 0) hipLaunchKernelGGL(k_reduce_batch, dim3(njobs), dim3(1024), 0, s, jobs);
 }

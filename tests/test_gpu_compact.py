@@ -70,3 +70,29 @@ def test_compact_equals_forked_and_oracle(cvm, orc, monkeypatch, case):
         om, ot = orc.match_u8(*args, threads=8)
     assert tuple(a[1]) == tuple(ot), (a[1], ot)
     assert np.array_equal(a[0].view(np.uint32), np.asarray(om, np.float32).reshape(a[0].shape).view(np.uint32))
+
+
+def test_state_left_clear_between_compact_calls(cvm, orc, monkeypatch):
+    """The compact form accumulates each call's template statistics and alpha
+    flags into a state that its final reduction clears for the next call (no
+    reset launch).  Alternating a varying-alpha call (cn 4) with a
+    constant-alpha one (cn 3, which a leftover flag would turn into cn 4) and
+    a second template (whose sums would add to leftover ones), through the
+    eager, captured and replayed forms, must give the oracle's bits every
+    time."""
+    monkeypatch.setenv("CVMATCH_COMPACT", "2500000")
+    rng = np.random.default_rng(77)
+    cases = []
+    for what, (tw, th) in (("alpha rule, varying alpha", (48, 48)), ("alpha rule, constant alpha", (33, 21)),
+                           ("alpha rule, constant alpha", (40, 29))):
+        img, istride, tpl, _ = buffers(rng, 320, 240, 0, 4, tw, th, 0, what)
+        parent = img.reshape(240, istride)[:, :320 * 4].reshape(240, 320, 4)
+        sub = tpl.reshape(th, tw, 4)
+        om, ot = orc.match_rgba(parent, sub, threads=8)
+        cases.append((parent, sub, np.asarray(om, np.float32), tuple(ot)))
+    say("[compact] state cleared between calls: three alpha-rule shapes alternated 4 times")
+    for rep in range(4):
+        for i, (parent, sub, om, ot) in enumerate(cases):
+            m, _, _ = cvm.match_map(parent, sub)
+            assert tuple(cvm.match(parent, sub)) == ot, (rep, i)
+            assert np.array_equal(m.view(np.uint32), om.reshape(m.shape).view(np.uint32)), (rep, i)

@@ -216,11 +216,13 @@ CallStateArgs call_state_args(const uint8_t* img, int64_t istride, int iw, int i
 
 // The compact form of a small one-shot call (one stream, independent jobs
 // sharing launches): the parent's and the template's row transforms with
-// the state's reset; the template's column pass with the call state's
-// statistics and alpha scans (csa, the whole parent scan in it); the
-// correlation's column pass with the chunk column sums, each workgroup
-// finalising the state first; the inverse rows with the chunk prefix sums.  The normalisation and the
-// reduction follow as usual.  Every job runs the code of its own launch.
+// the call state's statistics and alpha scans (csa, the whole parent scan
+// in it; st's accumulators must be zero on entry); the correlation's column
+// pass, each workgroup transforming its template slab first, with the chunk
+// column sums, each workgroup finalising the state first; the inverse rows
+// with the chunk prefix sums.  The normalisation and the reduction (which
+// clears st's accumulators again) follow.  Every job runs the code of its
+// own launch.
 struct CompactCall {
   const uint8_t* img;
   int64_t istride;
@@ -230,7 +232,7 @@ struct CompactCall {
   int64_t tileStride, chanStride;
   const uint8_t* tpl;
   int64_t tstride;
-  float2* tspec;  // the template spectrum, every channel
+  float2* tspec;  // the template's row spectra, every channel
   int64_t tchan;
   float scale;  // 1 / (dftW dftH)
   PlanTables pt;

@@ -674,17 +674,48 @@ __device__ __forceinline__ void norm_wave_block(const uint8_t* __restrict__ img,
         }
       }
       // a half band's start: the window slides from rows [ys0, ys0 + th) to
-      // [ys, ys + th) (y + th < ys + th <= ih: every row exists)
-      for (int y = ys0; y < ys; y++) {
-        uint32_t a[CPT], b[CPT];
-        load_pxn<VMODE, CPT>(img + (int64_t)(y + th) * istride, x0, step, cn, nload, a);
-        load_pxn<VMODE, CPT>(img + (int64_t)y * istride, x0, step, cn, nload, b);
+      // [ys, ys + th) (y + th < ys + th <= ih: every row exists).  For a
+      // half band ys - ys0 is 0 or NB_ROWS / 2: a fixed run of rows whose
+      // loads go out SB rows (2 SB row loads) at a time instead of one row
+      // after another, where the lane's columns are few enough to keep the
+      // register count (two image columns, or four gray ones; wider RGBA
+      // lanes measured 112 -> 203 registers that way and slide row by row)
+      constexpr bool BATCH = CPT <= 2 || (VMODE == 2 && CPT <= 4);
+      if (BATCH && ys - ys0 == NB_ROWS / 2) {
+        constexpr int SB = 4;
 #pragma unroll
-        for (int j = 0; j < CPT; j++) {
-          const uint32_t va = a[j] & cm, vb = b[j] & cm;
-          cs02[j] = cs02[j] - (vb & 0x00FF00FFu) + (va & 0x00FF00FFu);
-          cs13[j] = cs13[j] - ((vb >> 8) & 0x00FF00FFu) + ((va >> 8) & 0x00FF00FFu);
-          c2[j] += (sq_t)__builtin_amdgcn_udot4(va, va, 0u, false) - (sq_t)__builtin_amdgcn_udot4(vb, vb, 0u, false);
+        for (int y0 = 0; y0 < NB_ROWS / 2; y0 += SB) {
+          uint32_t a[SB][CPT], b[SB][CPT];
+#pragma unroll
+          for (int u = 0; u < SB; u++) {
+            const int y = ys0 + y0 + u;
+            load_pxn<VMODE, CPT>(img + (int64_t)(y + th) * istride, x0, step, cn, nload, a[u]);
+            load_pxn<VMODE, CPT>(img + (int64_t)y * istride, x0, step, cn, nload, b[u]);
+          }
+#pragma unroll
+          for (int u = 0; u < SB; u++)
+#pragma unroll
+            for (int j = 0; j < CPT; j++) {
+              const uint32_t va = a[u][j] & cm, vb = b[u][j] & cm;
+              cs02[j] = cs02[j] - (vb & 0x00FF00FFu) + (va & 0x00FF00FFu);
+              cs13[j] = cs13[j] - ((vb >> 8) & 0x00FF00FFu) + ((va >> 8) & 0x00FF00FFu);
+              c2[j] += (sq_t)__builtin_amdgcn_udot4(va, va, 0u, false) -
+                       (sq_t)__builtin_amdgcn_udot4(vb, vb, 0u, false);
+            }
+        }
+      } else {
+        for (int y = ys0; y < ys; y++) {
+          uint32_t a[CPT], b[CPT];
+          load_pxn<VMODE, CPT>(img + (int64_t)(y + th) * istride, x0, step, cn, nload, a);
+          load_pxn<VMODE, CPT>(img + (int64_t)y * istride, x0, step, cn, nload, b);
+#pragma unroll
+          for (int j = 0; j < CPT; j++) {
+            const uint32_t va = a[j] & cm, vb = b[j] & cm;
+            cs02[j] = cs02[j] - (vb & 0x00FF00FFu) + (va & 0x00FF00FFu);
+            cs13[j] = cs13[j] - ((vb >> 8) & 0x00FF00FFu) + ((va >> 8) & 0x00FF00FFu);
+            c2[j] += (sq_t)__builtin_amdgcn_udot4(va, va, 0u, false) -
+                     (sq_t)__builtin_amdgcn_udot4(vb, vb, 0u, false);
+          }
         }
       }
     }

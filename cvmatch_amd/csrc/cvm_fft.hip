@@ -587,10 +587,10 @@ __device__ __forceinline__ void store_f2(float __attribute__((ext_vector_type(2)
   *p = u;
 }
 
-template <int LOG2N, int NT, int NST1, typename DstPtr>
+template <int LOG2N, int NT, int NST1, typename DstPtr, int CGO = 0>
 __device__ __forceinline__ void col_inverse_store(float2* a, const float2* __restrict__ tri, DstPtr dp, int hwPad,
                                                   int ncol, int need) {
-  constexpr int CG = ColCG<LOG2N>::value, LCG = ceil_log2c(CG);
+  constexpr int CG = CGO ? CGO : ColCG<LOG2N>::value, LCG = ceil_log2c(CG);
   using F = ColFirst<LOG2N, NST1>;
   constexpr int NB1 = F::NB;
   constexpr bool SINGLE = NB1 == LOG2N;
@@ -646,14 +646,14 @@ __device__ __forceinline__ void col_inverse_store(float2* a, const float2* __res
 // [t0, t0 + ntiles), all of column height 2^LOG2N; L is its logical
 // workgroup index.
 // NT threads per workgroup; NST1: the first pass's radix-4 stages (ColFirst).
-template <int LOG2N, int MODE, int NT = 256, int NST1 = -1>
+template <int LOG2N, int MODE, int NT = 256, int NST1 = -1, int CGO = 0>
 __device__ __forceinline__ void cols_body(float2* a, int L, const float2* __restrict__ src,
                                           float2* __restrict__ dst, int64_t srcTileStride, int64_t dstTileStride,
                                           int64_t chanStride, const CallState* __restrict__ st, const TileGrid& g,
                                           const float2* __restrict__ tri, const float2* __restrict__ tspec,
                                           int64_t tchanStride, float scale, int t0, int ntiles, int nslabs,
                                           const TileGrid* gtp = nullptr, const float2* __restrict__ trows = nullptr) {
-  constexpr int n = 1 << LOG2N, CG = ColCG<LOG2N>::value, LCG = ceil_log2c(CG);
+  constexpr int n = 1 << LOG2N, CG = CGO ? CGO : ColCG<LOG2N>::value, LCG = ceil_log2c(CG);
   constexpr bool CORR = MODE == COL_CORR || MODE == COL_CORRT;
   using F = ColFirst<LOG2N, NST1>;
   constexpr int NB1 = F::NB, E1 = 1 << NB1;
@@ -827,7 +827,7 @@ __device__ __forceinline__ void cols_body(float2* a, int L, const float2* __rest
     }
   }
   __syncthreads();
-  col_inverse_store<LOG2N, NT, NST1>(a, tri, dp, hwPad, ncol, need);
+  col_inverse_store<LOG2N, NT, NST1, float2*, CGO>(a, tri, dp, hwPad, ncol, need);
 }
 
 // Two members of one Fleet bucket against the same frame spectrum in one
@@ -1326,9 +1326,9 @@ struct ColArgs {
   const float2* trows;   // COL_CORRT: the template's row spectra
 };
 
-template <int LOG2N, int MODE, int NT = 256, int NST1 = -1>
+template <int LOG2N, int MODE, int NT = 256, int NST1 = -1, int CGO = 0>
 __device__ __forceinline__ void cols_job(float2* a, int p, const ColArgs& C) {
-  cols_body<LOG2N, MODE, NT, NST1>(a, xcd_remap(p, C.nwg), C.src, C.dst, C.srcTileStride, C.dstTileStride,
+  cols_body<LOG2N, MODE, NT, NST1, CGO>(a, xcd_remap(p, C.nwg), C.src, C.dst, C.srcTileStride, C.dstTileStride,
                                    C.chanStride, C.st, C.g, C.tri, C.tspec, C.tchanStride, C.scale, C.t0, C.ntiles,
                                    C.nslabs, &C.gt, C.trows);
 }
@@ -1379,7 +1379,7 @@ struct FinalizeArgs {
 // The correlation's column pass (each workgroup first transforms the
 // template's slab it multiplies by) and the chunk column sums, after the
 // state's finalisation.
-template <int LOG2N, int NT = 256, int NST1 = -1>
+template <int LOG2N, int NT = 256, int NST1 = -1, int CGO = 0>
 __global__ __launch_bounds__(NT) void k_corr_chunks(ColArgs C, ChunkArgs K, FinalizeArgs F) {
   extern __shared__ float2 a[];
   __shared__ CallState sst;
@@ -1398,7 +1398,7 @@ __global__ __launch_bounds__(NT) void k_corr_chunks(ColArgs C, ChunkArgs K, Fina
   __syncthreads();
   if ((int)blockIdx.x < C.nwg) {
     C.st = &sst;
-    cols_job<LOG2N, COL_CORRT, NT, NST1>(a, blockIdx.x, C);
+    cols_job<LOG2N, COL_CORRT, NT, NST1, CGO>(a, blockIdx.x, C);
   } else {
     const int b = blockIdx.x - C.nwg;
     colsum_chunks_at((b % K.nbx) * NT + threadIdx.x, b / K.nbx, K.img, K.istride, K.iw, K.ih, K.step, K.words, &sst,
@@ -1456,11 +1456,11 @@ static void compact_rows_dispatch(const CompactCall& c, bool front, const Prefix
 #undef CVM_CR
 }
 
-template <int LOG2H, int NT = 256, int NST1 = -1>
+template <int LOG2H, int NT = 256, int NST1 = -1, int CGO = 0>
 static void compact_cols(const CompactCall& c, ChunkArgs K, hipStream_t s) {
   const FinalizeArgs F{c.st, c.tw, c.th, c.cn_mode};
   K.nbx = (c.iw + NT - 1) / NT;
-  constexpr int CG = ColCG<LOG2H>::value;
+  constexpr int CG = CGO ? CGO : ColCG<LOG2H>::value;
   const size_t lds = sizeof(float2) * (size_t)(1 << LOG2H) * CG;
   const int nslabs = (c.g.hw + CG - 1) / CG;
   ColArgs C;
@@ -1482,7 +1482,7 @@ static void compact_cols(const CompactCall& c, ChunkArgs K, hipStream_t s) {
   C.scale = c.scale;
   C.ntiles = c.ntiles;
   C.nwg = nslabs * c.ntiles * c.cnMax;
-  hipLaunchKernelGGL((k_corr_chunks<LOG2H, NT, NST1>), dim3(C.nwg + K.nbx * K.nch), dim3(NT), lds, s, C, K, F);
+  hipLaunchKernelGGL((k_corr_chunks<LOG2H, NT, NST1, CGO>), dim3(C.nwg + K.nbx * K.nch), dim3(NT), lds, s, C, K, F);
 }
 
 static void compact_cols_dispatch(const CompactCall& c, const ChunkArgs& K, hipStream_t s) {
@@ -1492,14 +1492,23 @@ static void compact_cols_dispatch(const CompactCall& c, const ChunkArgs& K, hipS
     break
   // 512-row columns in the 512-thread form (one radix-4 stage in the first
   // pass: twice the waves a workgroup, each with half the points; 640x480
-  // RGBA Match 0.075 -> 0.068 ms; 256-row columns measured slower so);
-  // CVMATCH_COMPACT_WIDE=0 keeps the 256-thread one
+  // RGBA Match 0.075 -> 0.068 ms; 256-row columns measured slower so),
+  // in slabs of 8 columns rather than 16: twice the workgroups, each with
+  // half the points, and every thread still busy (the first pass has 8 x 64
+  // groups): 640x480 Match 0.0656 -> 0.0615 ms.  (256-row columns in
+  // 8-column slabs leave half of a 256-thread workgroup idle: 1080p
+  // MatchGray 0.0746 -> 0.080 ms.)  CVMATCH_COMPACT_WIDE=0 keeps the
+  // 256-thread form, CVMATCH_COMPACT_CG=16 the 16-column slabs.
   const char* e = getenv("CVMATCH_COMPACT_WIDE");
   const bool wide = !(e && e[0] == '0');
+  const char* cge = getenv("CVMATCH_COMPACT_CG");
+  const bool cg8 = !(cge && atoi(cge) == 16);
   switch (c.g.log2H) {
     CVM_CC(5); CVM_CC(6); CVM_CC(7); CVM_CC(8); CVM_CC(10); CVM_CC(11);
     case 9:
-      if (wide)
+      if (wide && cg8)
+        compact_cols<9, 512, 1, 8>(c, K, s);
+      else if (wide)
         compact_cols<9, 512, 1>(c, K, s);
       else
         compact_cols<9>(c, K, s);

@@ -96,3 +96,24 @@ def test_state_left_clear_between_compact_calls(cvm, orc, monkeypatch):
             m, _, _ = cvm.match_map(parent, sub)
             assert tuple(cvm.match(parent, sub)) == ot, (rep, i)
             assert np.array_equal(m.view(np.uint32), om.reshape(m.shape).view(np.uint32)), (rep, i)
+
+
+@pytest.mark.parametrize("case", [0, 6])
+def test_column_slab_widths_change_no_bits(cvm, monkeypatch, case):
+    """512-row compact column passes run in slabs of 8 columns by default;
+    CVMATCH_COMPACT_CG=16 keeps 16-column slabs, CVMATCH_COMPACT_WIDE=0 the
+    256-thread form.  Every form gives the same bits."""
+    iw, ih, cn, step, tw, th, pad, what = CASES[case]
+    rng = np.random.default_rng(950 + case)
+    img, istride, tpl, tstride = buffers(rng, iw, ih, cn, step, tw, th, pad, what)
+    args = (img, istride, iw, ih, tpl, tstride, tw, th, cn if cn else 4, step)
+    monkeypatch.setenv("CVMATCH_COMPACT", "2500000")
+    got = []
+    for cg, wide in (("8", "1"), ("16", "1"), ("8", "0")):
+        monkeypatch.setenv("CVMATCH_COMPACT_CG", cg)
+        monkeypatch.setenv("CVMATCH_COMPACT_WIDE", wide)
+        say(f"[compact] {what}: column slabs of {cg}, wide {wide}")
+        got.append(cvm.match_u8(*args, want_map=True))
+    for m, r in got[1:]:
+        assert tuple(r) == tuple(got[0][1])
+        assert np.array_equal(m.view(np.uint32), got[0][0].view(np.uint32))

@@ -1414,7 +1414,7 @@ __global__ __launch_bounds__(RowNTI<LOG2N>::value) __attribute__((amdgpu_waves_p
   if ((int)blockIdx.x < R.nwg)
     row_inv_body<LOG2N, NVEC>(z, blockIdx.x, R, triW);
   else
-    colsum_prefix_at((blockIdx.x - R.nwg) * blockDim.x + threadIdx.x, Q.nwords, Q.nch, Q.st, Q.ps);
+    colsum_prefix_block(blockIdx.x - R.nwg, Q.nwords, Q.nch, Q.st, Q.ps);
 }
 
 template <int LOG2W, int NVEC>
@@ -1539,7 +1539,7 @@ void launch_compact_fft(const CompactCall& c, hipStream_t s) {
   Q.st = c.st;
   Q.ps = (uint32_t*)c.nscratch;
   const int nt = row_inv_nt(c.g.log2W);  // the inverse row launch's workgroup size
-  Q.nwg = (Q.nwords + nt - 1) / nt;
+  Q.nwg = (Q.nwords + nt / PSEG - 1) / (nt / PSEG);  // nt / PSEG columns a workgroup
   compact_rows_dispatch(c, true, Q, s);   // parent + template rows, state statistics
   compact_cols_dispatch(c, K, s);         // state finalised; template + correlation columns, chunk sums
   compact_rows_dispatch(c, false, Q, s);  // inverse rows + chunk prefix sums

@@ -298,4 +298,60 @@ __device__ __forceinline__ void colsum_prefix_at(int i, int nwords, int nch, con
   }
 }
 
+// The same running sums with the rows of each column split over PSEG
+// threads: block b takes columns [b * cols, b * cols + cols), cols =
+// blockDim / PSEG, thread t column t % cols and segment t / cols of the
+// rows 1..nch.  Each thread sums its segment (up to eight rows' loads in
+// flight), the segment totals meet in LDS, and each thread writes its rows'
+// running sums from the total of the segments above it: a chain of two
+// segment walks instead of one walk over every row.  Modular uint32 sums,
+// so any grouping gives the same words.  Every thread reaches the barrier.
+constexpr int PSEG = 8;
+__device__ __forceinline__ void colsum_prefix_block(int b, int nwords, int nch, const CallState* __restrict__ st,
+                                                    uint32_t* __restrict__ ps) {
+  __shared__ uint32_t tot[PSEG][64];
+  const int cols = (int)blockDim.x / PSEG;
+  const int t = threadIdx.x, col = t % cols, seg = t / cols;
+  const int i = b * cols + col;
+  const bool on = i < nwords && !st->flat;
+  const int per = (nch + PSEG - 1) / PSEG;
+  const int r0 = 1 + seg * per, r1 = min(nch + 1, r0 + per);
+  uint32_t* p = ps + (on ? i : 0);
+  uint32_t sum = 0;
+  uint32_t v[8];
+  if (on)
+    for (int r = r0; r < r1; r += 8) {
+#pragma unroll
+      for (int k = 0; k < 8; k++) v[k] = r + k < r1 ? p[(int64_t)(r + k) * nwords] : 0u;
+#pragma unroll
+      for (int k = 0; k < 8; k++) sum += v[k];
+    }
+  tot[seg][col] = sum;
+  __syncthreads();
+  if (!on) return;
+  uint32_t acc = 0;
+  for (int s = 0; s < seg; s++) acc += tot[s][col];
+  if (seg == 0) p[0] = 0u;
+  if (per <= 8) {
+    // the segment's rows are still in registers
+#pragma unroll
+    for (int k = 0; k < 8; k++)
+      if (r0 + k < r1) {
+        acc += v[k];
+        p[(int64_t)(r0 + k) * nwords] = acc;
+      }
+    return;
+  }
+  for (int r = r0; r < r1; r += 8) {
+#pragma unroll
+    for (int k = 0; k < 8; k++) v[k] = r + k < r1 ? p[(int64_t)(r + k) * nwords] : 0u;
+#pragma unroll
+    for (int k = 0; k < 8; k++)
+      if (r + k < r1) {
+        acc += v[k];
+        p[(int64_t)(r + k) * nwords] = acc;
+      }
+  }
+}
+
 }  // namespace cvm

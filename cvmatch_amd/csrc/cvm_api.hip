@@ -15,6 +15,7 @@
 #include <algorithm>
 #include <array>
 #include <atomic>
+#include <chrono>
 #include <condition_variable>
 #include <deque>
 #include <functional>
@@ -264,6 +265,13 @@ struct GraphSlot {
     int seen = 0;
     hipGraphExec_t exec = nullptr;
     unsigned long long used = 0;
+    // a synchronous call site's choice between replaying the graph and
+    // launching eagerly (GraphMode), from the wall time of a few calls of
+    // each once the graph exists
+    int mode = 0;
+    int last = -1;  // how the last call ran: 0 graph, 1 eager, -1 neither timed
+    double t[2] = {0.0, 0.0};
+    int n[2] = {0, 0};
   };
   std::vector<Entry> e;
   unsigned long long tick = 0;
@@ -527,16 +535,53 @@ static bool graphs_enabled() {
   return on;
 }
 
+// A synchronous call site (one that waits for its result right after) may
+// pass `timed`: once its graph exists, its next calls alternate between the
+// replay and eager launches, the caller reports each call's wall time
+// (graph_note), and after kGraphTrials of each the faster way is kept for the
+// key.  A replay saves the host work of every launch, but the host submits
+// the graph's nodes one after another: a short first kernel can finish
+// before the second node is in the queue, which eager launches of the same
+// few kernels avoid.  CVMATCH_GRAPH_MODE=1 (graph) or 2 (eager) fixes the
+// choice.
+constexpr int kGraphTrials = 4;
+static int graph_mode_env() {
+  const char* v = getenv("CVMATCH_GRAPH_MODE");
+  const int m = v ? atoi(v) : 0;
+  return m == 1 || m == 2 ? m : 0;
+}
+
+static void graph_note(GraphSlot::Entry* g, double seconds) {
+  if (!g || g->last < 0 || g->mode != 0) return;
+  g->t[g->last] += seconds;
+  if (++g->n[g->last] < kGraphTrials || g->n[1 - g->last] < kGraphTrials) return;
+  g->mode = g->t[1] / g->n[1] < g->t[0] / g->n[0] ? 2 : 1;
+}
+
 template <class F>
-static int run_graphed(GraphSlot& slot, std::vector<uint64_t> key, hipStream_t s, F&& run) {
+static int run_graphed(GraphSlot& slot, std::vector<uint64_t> key, hipStream_t s, F&& run,
+                       GraphSlot::Entry** timed = nullptr) {
   key.push_back(g_alloc_epoch.load());
   key.push_back((uint64_t)(uintptr_t)s);
+  if (timed) *timed = nullptr;
   if (engine().prof.load() || !graphs_enabled()) {  // stage events need the eager launches
     slot.drop();
     return run();
   }
   GraphSlot::Entry& g = slot.find(key);
+  g.last = -1;
   if (g.exec) {
+    int mode = graph_mode_env();
+    if (!mode) mode = timed ? g.mode : 1;
+    if (!mode) {
+      // trials: the way with fewer samples so far (graph first)
+      g.last = g.n[1] < g.n[0] ? 1 : 0;
+      *timed = &g;
+      if (g.last == 1) return run();
+      HIPCHK(hipGraphLaunch(g.exec, s));
+      return 0;
+    }
+    if (mode == 2) return run();
     HIPCHK(hipGraphLaunch(g.exec, s));
     return 0;
   }
@@ -2668,8 +2713,15 @@ static int match_device(const uint8_t* d_img, int64_t istride, int32_t iw, int32
   const std::vector<uint64_t> key = {u64p(d_img), u64p(d_tpl), u64p(d_map), (uint64_t)istride, (uint64_t)tstride,
                                      (uint64_t)iw, (uint64_t)ih, (uint64_t)tw, (uint64_t)th, (uint64_t)cn,
                                      (uint64_t)step, direct ? 1u : 0u};
-  CHK(run_graphed(X.graph, key, s, run));
-  return finish_result(X, s, out);
+  // small parents only: a trial of the eager way costs a large call far more
+  // than it can gain
+  GraphSlot::Entry* timed = nullptr;
+  const bool small = (long long)iw * ih <= compact_max_pixels();
+  const auto t0 = std::chrono::steady_clock::now();
+  CHK(run_graphed(X.graph, key, s, run, small ? &timed : nullptr));
+  CHK(finish_result(X, s, out));
+  graph_note(timed, std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count());
+  return 0;
 }
 
 // ---------------------------------------------------- allocation-free call --

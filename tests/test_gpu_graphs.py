@@ -87,6 +87,30 @@ def test_device_match_graph_alternating_buffers(cvm, dev):
             assert out.tuple() == want[k][1], (rep, k)
 
 
+@pytest.mark.parametrize("mode", ["1", "2", "0"])
+def test_device_match_graph_or_eager(cvm, dev, monkeypatch, mode):
+    """A small synchronous device call replays its graph (1), launches
+    eagerly (2) or times both and keeps the faster (0, the default): every
+    call gives the eager first call's bits, through the capture, the trials
+    and the calls after the choice."""
+    monkeypatch.setenv("CVMATCH_GRAPH_MODE", mode)
+    L = cvm.lib()
+    p, t = synth.workload_match_640(seed=23 + int(mode), alpha="random")[:2]
+    want_m, _, _ = cvm.match_map(p, t)
+    want_r = cvm.match(p, t)
+    rw, rh = 640 - 48 + 1, 480 - 48 + 1
+    dp, ds, dm = dev(p), dev(t), dev(np.zeros(rw * rh, np.float32))
+    out = cvm.cvm_result()
+    say(f"[graphs] device Match, graph mode {mode}: 14 calls")
+    for rep in range(14):
+        rc = L.cvm_match_u8_device(dp, 640 * 4, 640, 480, ds, 48 * 4, 48, 48, 0, 4, dm, C.byref(out), None)
+        assert rc == 0, L.cvm_last_error_detail()
+        got = np.empty(rw * rh, np.float32)
+        assert L.cvm_copy_d2h(C.c_void_p(got.ctypes.data), dm, got.nbytes) == 0
+        assert np.array_equal(got.view(np.uint32), want_m.ravel().view(np.uint32)), rep
+        assert out.tuple() == tuple(want_r), rep
+
+
 def test_frame_stream_graph_replays(cvm):
     frames, tpls = synth.workload_fleet(seed=10, frames=6, members=3, w=800, h=600)
     frames = [np.ascontiguousarray(f) for f in frames]

@@ -28,6 +28,7 @@
 
 #include "../../include/cvmatch_hip.h"
 #include "cvm_internal.h"
+#include "cvm_score.h"
 
 namespace cvm {
 
@@ -3548,8 +3549,8 @@ __global__ void k_selftest_fp(const float* a, const float* b, const long long* i
                               float* ocvt) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
-  osq[i] = (float)__dsqrt_rn((double)a[i]);
-  odiv[i] = (float)__ddiv_rn((double)a[i], (double)b[i]);
+  osq[i] = sqrt_rn_f32(a[i]);
+  odiv[i] = div_rn_f32(a[i], b[i]);
   ocvt[i] = (float)(double)ints[i];
 }
 }  // namespace cvm

@@ -139,9 +139,9 @@ void launch_rgba_to_gray(const uint8_t* pix, int64_t stride, int w, int h, uint8
 
 // --------------------------------------------------------- normalisation --
 
-// normOne impl.go:825-849.  sqrt and the final division are evaluated in
-// binary64 and rounded once to binary32: for sqrt and division that double
-// rounding equals the correctly rounded binary32 result (53 >= 2*24+2).
+// normOne impl.go:825-849.  sqrt and the final division are the correctly
+// rounded binary32 operations (sqrt_rn_f32, div_rn_f32), which is what the
+// reference's float32(math.Sqrt(float64(x))) and float32 division give.
 __device__ __forceinline__ float norm_one(float num, float lane0, float idiff, float s2, float numScale,
                                           float varScale, float templNorm) {
   const float eps = 10.0f * 0x1p-23f;
@@ -152,11 +152,11 @@ __device__ __forceinline__ float norm_one(float num, float lane0, float idiff, f
   if (lim > 0.5f) lim = 0.5f;
   float den = 0.0f;
   if (diff2 > lim) {
-    float sq = (float)__dsqrt_rn((double)diff2);
+    float sq = sqrt_rn_f32(diff2);
     den = sq * templNorm;
   }
   float an = fabsf(num);
-  if (an < den) return (float)__ddiv_rn((double)num, (double)den);
+  if (an < den) return div_rn_f32(num, den);
   float den2 = den * 1.125f;
   if (an < den2) return num > 0.0f ? 1.0f : -1.0f;
   return 0.0f;

@@ -62,6 +62,17 @@ __device__ __forceinline__ double score_fp64(double numI, double idiff, double s
   return 0.0;
 }
 
+// The reference's binary32 sqrt, float32(math.Sqrt(float64(x))), and its
+// binary32 quotient: both are the correctly rounded binary32 results (for
+// sqrt the binary64 root rounded again to binary32 is the correctly rounded
+// one, since 53 >= 2 * 24 + 2).  sqrtf and '/' compile to correctly rounded
+// binary32 sequences here (no fast-math), a third of the instructions of the
+// binary64 root; sqrtf was checked against the binary64 root rounded to
+// binary32 on every finite non-negative float on the device, and
+// cvm_selftest_fp runs these very functions.
+__device__ __forceinline__ float sqrt_rn_f32(float x) { return sqrtf(x); }
+__device__ __forceinline__ float div_rn_f32(float a, float b) { return a / b; }
+
 // Orders the LDS accesses of one wave around a cross-lane exchange: the
 // LDS unit executes one wave's accesses in issue order, so only the compiler
 // has to be kept from moving them across this point.

@@ -81,7 +81,7 @@ static int ilog2(int n) { return bit_len((unsigned)n) - 1; }
 
 // newGoPlan impl.go:246-295: integer cost-model argmin over power-of-two
 // transform sizes (areaCap 2^22), ties to the smaller area.
-static Plan make_plan(int tw, int th, int rw, int rh) {
+static Plan make_plan_search(int tw, int th, int rw, int rh) {
   const long long areaCap = 1LL << 22;
   int minW = std::max(2, next_pow2(tw)), minH = std::max(2, next_pow2(th));
   int maxW = std::max(minW, next_pow2(rw + tw - 1)), maxH = std::max(minH, next_pow2(rh + th - 1));
@@ -118,6 +118,26 @@ static Plan make_plan(int tw, int th, int rw, int rh) {
   p.blockW = std::min(bestW - tw + 1, rw);
   p.blockH = std::min(bestH - th + 1, rh);
   p.cost = bestCost;
+  return p;
+}
+
+// The plan of a shape, searched once per process: the search walks every
+// size pair and, for each, every band of the map (a thousand bands for a
+// 1080p map at the smallest heights), tens of microseconds of host time
+// that a repeated call would otherwise spend before its first launch.
+static Plan make_plan(int tw, int th, int rw, int rh) {
+  static std::mutex mu;
+  static std::map<std::array<int, 4>, Plan> memo;
+  const std::array<int, 4> key = {tw, th, rw, rh};
+  {
+    std::lock_guard<std::mutex> lk(mu);
+    auto it = memo.find(key);
+    if (it != memo.end()) return it->second;
+  }
+  const Plan p = make_plan_search(tw, th, rw, rh);
+  std::lock_guard<std::mutex> lk(mu);
+  if (memo.size() >= 4096) memo.clear();
+  memo.emplace(key, p);
   return p;
 }
 

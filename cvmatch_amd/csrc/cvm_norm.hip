@@ -1157,17 +1157,29 @@ __device__ void reduce_block(const Extrema* __restrict__ parts, int n, int rw, R
     ri[1][wv] = bmax_i;
   }
   __syncthreads();
-  if (threadIdx.x == 0) {
-    for (int w = 1; w < (int)(blockDim.x >> 6); w++) {
-      if (better_min(rv[0][w], ri[0][w], bmin, bmin_i)) {
-        bmin = rv[0][w];
-        bmin_i = ri[0][w];
-      }
-      if (better_max(rv[1][w], ri[1][w], bmax, bmax_i)) {
-        bmax = rv[1][w];
-        bmax_i = ri[1][w];
-      }
+  if (wv != 0) return;
+  // the waves' results across the first wave's lanes, reduced by shuffles
+  // (the order is total, so any grouping gives the same winner)
+  const int nw = (int)(blockDim.x >> 6);
+  bmin = lane < nw ? rv[0][lane] : FLT_MAX;
+  bmin_i = lane < nw ? ri[0][lane] : INT_MAX;
+  bmax = lane < nw ? rv[1][lane] : -FLT_MAX;
+  bmax_i = lane < nw ? ri[1][lane] : INT_MAX;
+  for (int o = 8; o > 0; o >>= 1) {
+    float ov = __shfl_xor(bmin, o, 64);
+    int oi = __shfl_xor(bmin_i, o, 64);
+    if (better_min(ov, oi, bmin, bmin_i)) {
+      bmin = ov;
+      bmin_i = oi;
     }
+    ov = __shfl_xor(bmax, o, 64);
+    oi = __shfl_xor(bmax_i, o, 64);
+    if (better_max(ov, oi, bmax, bmax_i)) {
+      bmax = ov;
+      bmax_i = oi;
+    }
+  }
+  if (lane == 0) {
     Res6 r;
     r.min_v = bmin;
     r.min_x = bmin_i % rw;

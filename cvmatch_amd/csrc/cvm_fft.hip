@@ -1421,12 +1421,18 @@ template <int LOG2W, int NVEC>
 static void compact_rows(const CompactCall& c, const RowFwdArgs& P, const RowFwdArgs& T, const RowInvArgs& R,
                          const PrefixArgs& Q, bool front, hipStream_t s) {
   const size_t lds = sizeof(float2) * (size_t)padded_len(1 << LOG2W) * P.cpg;
-  if (front)
-    hipLaunchKernelGGL((k_front<LOG2W, NVEC>), dim3(P.nwg + T.nwg + c.csa.nb()), dim3(RowNTF<LOG2W>::value), lds, s,
-                       P, T, c.pt.triW, c.csa);
-  else
+  if (front) {
+    // the template statistics in workgroups of this launch's size: about
+    // eight pixels a thread, as with the 256-thread layout of the state
+    CallStateArgs S = c.csa;
+    constexpr int NT = RowNTF<LOG2W>::value;
+    S.sgx = (int)std::min<long long>(256, ((long long)c.tw * c.th + 8 * NT - 1) / (8 * NT));
+    hipLaunchKernelGGL((k_front<LOG2W, NVEC>), dim3(P.nwg + T.nwg + S.nb()), dim3(NT), lds, s, P, T, c.pt.triW,
+                       S);
+  } else {
     hipLaunchKernelGGL((k_rowinv_prefix<LOG2W, NVEC>), dim3(R.nwg + Q.nwg), dim3(RowNTI<LOG2W>::value), lds, s, R, Q,
                        c.pt.triW);
+  }
 }
 
 template <int LOG2W>

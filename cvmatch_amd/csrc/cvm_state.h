@@ -135,6 +135,9 @@ __device__ __forceinline__ void tpl_stats_flat(const uint8_t* __restrict__ tpl, 
   __shared__ unsigned long long red[4][16];  // [k][w]: sums, [k][8 + w]: squares (<= 8 waves)
   unsigned long long s[4] = {0, 0, 0, 0}, q[4] = {0, 0, 0, 0};
   const int n = tw * th, stride = nb * (int)blockDim.x;
+  // four-byte pixels on four-byte boundaries: one dword a pixel
+  const bool words = step == 4 && ((((uintptr_t)tpl) | (uintptr_t)tstride) & 3) == 0;
+  const uint32_t cm = chan_mask(cnMax);
   for (int i0 = b * (int)blockDim.x + (int)threadIdx.x; i0 < n; i0 += 4 * stride) {
     uint32_t w[4];
 #pragma unroll
@@ -144,8 +147,12 @@ __device__ __forceinline__ void tpl_stats_flat(const uint8_t* __restrict__ tpl, 
       if (i < n) {
         const int y = i / tw, x = i - y * tw;
         const uint8_t* p = tpl + (int64_t)y * tstride + (int64_t)x * step;
-        for (int k = 0; k < 4; k++)
-          if (k < cnMax) w[u] |= (uint32_t)p[k] << (8 * k);
+        if (words) {
+          w[u] = *(const uint32_t*)p & cm;
+        } else {
+          for (int k = 0; k < 4; k++)
+            if (k < cnMax) w[u] |= (uint32_t)p[k] << (8 * k);
+        }
       }
     }
 #pragma unroll

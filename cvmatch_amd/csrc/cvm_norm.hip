@@ -1124,15 +1124,30 @@ __device__ void reduce_block(const Extrema* __restrict__ parts, int n, int rw, R
   __shared__ int ri[2][16];
   float bmin = FLT_MAX, bmax = -FLT_MAX;
   int bmin_i = INT_MAX, bmax_i = INT_MAX;
-  for (int i = threadIdx.x; i < n; i += blockDim.x) {
-    Extrema e = parts[i];
-    if (better_min(e.min_v, e.min_i, bmin, bmin_i)) {
-      bmin = e.min_v;
-      bmin_i = e.min_i;
+  // four partials' loads in flight at a time
+  for (int i0 = threadIdx.x; i0 < n; i0 += 4 * blockDim.x) {
+    Extrema e[4];
+#pragma unroll
+    for (int u = 0; u < 4; u++) {
+      const int i = i0 + u * (int)blockDim.x;
+      if (i < n) {
+        e[u] = parts[i];
+      } else {
+        e[u].min_v = FLT_MAX;
+        e[u].max_v = -FLT_MAX;
+        e[u].min_i = e[u].max_i = INT_MAX;
+      }
     }
-    if (better_max(e.max_v, e.max_i, bmax, bmax_i)) {
-      bmax = e.max_v;
-      bmax_i = e.max_i;
+#pragma unroll
+    for (int u = 0; u < 4; u++) {
+      if (better_min(e[u].min_v, e[u].min_i, bmin, bmin_i)) {
+        bmin = e[u].min_v;
+        bmin_i = e[u].min_i;
+      }
+      if (better_max(e[u].max_v, e[u].max_i, bmax, bmax_i)) {
+        bmax = e[u].max_v;
+        bmax_i = e[u].max_i;
+      }
     }
   }
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
